@@ -1,0 +1,217 @@
+/*
+ * mesh_bem_amd -- C ABI of the MI355X (gfx950) mesh-BEM library.
+ *
+ * Conventions (every entry point):
+ *   - Array arguments are DEVICE pointers, scalars are passed by value.
+ *   - prec selects the element type of every floating-point array of the call:
+ *     MBA_F32 -> float / complex float, MBA_F64 -> double / complex double.
+ *     Arrays documented as "double" are double regardless of prec.
+ *   - Complex arrays are interleaved (re, im), i.e. the memory of a torch
+ *     complex64 / complex128 tensor.  The reference's op-level kernels write
+ *     separate real and imaginary planes that its Python wrapper recombines
+ *     (src/mesh_BEM/kernel_functions.py:52-72); here the kernels write the
+ *     combined complex array directly.
+ *   - Matrices are row-major with an explicit leading dimension (elements).
+ *   - Index arrays are int32 (as the reference's faces / face_edge_indices).
+ *   - Work is enqueued on `stream` (a hipStream_t); nothing synchronises.
+ *   - Return value: 0 (MBA_OK) or an MBA_ERR_* code.  Argument and size errors
+ *     are detected on the host before any launch.
+ *   - No hidden global state: calls on different streams are independent,
+ *     except through the buffers the caller passes.
+ */
+#ifndef MESH_BEM_AMD_H
+#define MESH_BEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *mba_stream_t;
+
+enum {
+    MBA_OK = 0,
+    MBA_ERR_ARG = 1,     /* null pointer, bad enum, bad leading dimension */
+    MBA_ERR_SIZE = 2,    /* size out of the supported range */
+    MBA_ERR_MESH = 3,    /* malformed mesh (detected on the host side) */
+    MBA_ERR_ALLOC = 4,   /* workspace too small */
+    MBA_ERR_LAUNCH = 5,  /* kernel launch failed */
+    MBA_ERR_NUMERIC = 6, /* zero pivot / non-finite value reported by a kernel */
+    MBA_ERR_TIMEOUT = 7  /* a cooperative kernel's bounded wait expired */
+};
+
+enum { MBA_F32 = 0, MBA_F64 = 1 };
+
+/* Human-readable text of a status code. */
+const char *mba_status_string(int status);
+/* Library ABI version (major * 100 + minor). */
+int mba_version(void);
+
+/* ===================================================================== MoM
+ * RWG basis on face f, local edge e (edge opposite vertex faces[f][e]):
+ *   f(x) = s * l / (2 A) * (x - v_e),  s = +1 iff faces[f][(e+1)%3] < faces[f][(e+2)%3]
+ * (src/mesh_BEM/cuda/mesh_MOM_3d.cu:154-242).  "Face-edge" index = 3 f + e. */
+
+/* Face-edge impedance blocks, reference rule: replaces
+ * create_MOM_Zmat_3d_forward (src/mesh_BEM/cuda/mesh_MOM_3d.cpp:52-77,
+ * kernel mesh_MOM_3d.cu:332-496).  CFIE (EFIE + MFIE, eta = 1), test rule =
+ * collapsed Gauss (gp_x, gw_x) of order nq_x, source rule (gp_y, gw_y) of
+ * order nq_y, the points of local edge e generated from (v_e, v_e+1, v_e+2).
+ * face_areas [F], face_normals [F,3] as in the reference.
+ * zf: [3F, ldzf] complex, row = test face-edge, col = source face-edge. */
+int mba_mom_zmat_faceedge_ref(int prec, const void *verts, const int32_t *faces,
+                              const void *face_areas, const void *face_normals, const void *gp_x,
+                              const void *gw_x, int32_t nq_x, const void *gp_y, const void *gw_y,
+                              int32_t nq_y, double wavenumber, int64_t n_verts, int64_t n_faces,
+                              void *zf, int64_t ldzf, mba_stream_t stream);
+
+/* Face-edge RHS, reference incidence: replaces create_MOM_rhs_3d_forward
+ * (mesh_MOM_3d.cpp:15-36, kernel mesh_MOM_3d.cu:244-330).  vf: [F, 3] complex. */
+int mba_mom_rhs_faceedge_ref(int prec, const void *verts, const int32_t *faces,
+                             const void *face_areas, const void *face_normals, const void *gp,
+                             const void *gw, int32_t nq, double wavenumber, int64_t n_verts,
+                             int64_t n_faces, void *vf, mba_stream_t stream);
+
+/* RWG coefficients to face-vertex currents: replaces
+ * interpolate_Icoeff_to_Js_3d_forward (mesh_MOM_3d.cpp:90-109,
+ * kernel mesh_MOM_3d.cu:498-565).  icoef [B, E] complex, out [B, 3F, 3]
+ * complex (overwritten, not accumulated). */
+int mba_mom_interp_faceedge(int prec, const void *verts, const int32_t *faces,
+                            const int32_t *face_edge_indices, const void *face_areas,
+                            const void *icoef, int64_t batch, int64_t n_edges, int64_t n_verts,
+                            int64_t n_faces, void *out, mba_stream_t stream);
+
+/* Deterministic vertex accumulation (replaces the scatter_add_ of
+ * mesh_BEM_solver_3d.py:186-192): vert[b, v, :] = sum over the (face, slot)
+ * pairs of vertex v, in the order of vinc_idx.  vinc_ptr [V+1] int64,
+ * vinc_idx [3F] int32 (= 3 f + t).  fvj [B, 3F, 3], vert [B, V, 3] complex. */
+int mba_mom_vertex_sum(int prec, const void *fvj, const int64_t *vinc_ptr,
+                       const int32_t *vinc_idx, int64_t batch, int64_t n_verts, int64_t n_faces,
+                       void *vert, mba_stream_t stream);
+
+/* Bytes of device workspace the ns-rule calls need for n_faces faces. */
+size_t mba_mom_workspace_bytes(int prec, int64_t n_faces);
+
+/* Face-edge EFIE blocks, north-star rule (no reference counterpart; the RWG
+ * space result replaces the Zmn of mesh_BEM_solver_3d.py:128-147):
+ *   Z_ij = c_i c_j [ jk <(x-v_i).(y-v_j) G> - (4j/k) <G> ],  G = e^{-jkR}/(4 pi R)
+ * rule: [nq, 4] double (l0, l1, l2, w), physical weight w * 2A.
+ * near_ptr [F+1] int64 / near_idx int32: CSR over source faces b of the test
+ * faces a whose pair takes the singular split; for those the 1/(4 pi R) part
+ * of the inner integral is computed analytically in FP64.
+ * zf: [3F, ldzf] complex rows = test face-edge; cols [col0, col0 + 3*nb)
+ * receive source faces [b0, b0 + nb).  workspace >= mba_mom_workspace_bytes. */
+int mba_mom_zmat_faceedge_ns(int prec, const void *verts, const int32_t *faces, int64_t n_verts,
+                             int64_t n_faces, const double *rule, int32_t nq, double wavenumber,
+                             const int64_t *near_ptr, const int32_t *near_idx, int64_t b0,
+                             int64_t nb, void *zf, int64_t ldzf, void *workspace,
+                             size_t workspace_bytes, mba_stream_t stream);
+
+/* Face-edge plane-wave RHS, north-star rule:
+ *   vf[3f+e, d] = sum_i w_i c_e (x_i - v_e).p_d e^{-jk khat_d.x_i}
+ * dirs, pols: [ndir, 3] double (unit propagation and real polarisation).
+ * vf: [3F, ndir] complex. */
+int mba_mom_rhs_faceedge_ns(int prec, const void *verts, const int32_t *faces, int64_t n_verts,
+                            int64_t n_faces, const double *rule, int32_t nq, double wavenumber,
+                            const double *dirs, const double *pols, int64_t ndir, void *vf,
+                            mba_stream_t stream);
+
+/* RWG accumulation of face-edge blocks into RWG space (replaces the
+ * scatter_add_ of mesh_BEM_solver_3d.py:141-147 with a fixed-order gather):
+ *   z[m, n] (=|+=) sum_{s,t} zf[fe_s(m), fe_t(n) - col0],  rwg_fe [E, 2] ascending.
+ * Only source face-edges in [col0, col0 + ncols) are read: a column n whose
+ * two face-edges both lie in the window is stored; one whose fe_0 lies in it
+ * is stored; one whose fe_1 alone lies in it is added (its fe_0 window must
+ * have been gathered before).  z: [E, ldz] complex. */
+int mba_mom_gather_rwg(int prec, const void *zf, int64_t ldzf, int64_t col0, int64_t ncols,
+                       const int32_t *rwg_fe, int64_t n_edges, void *z, int64_t ldz,
+                       mba_stream_t stream);
+
+/* v[m, d] = vf[fe_0(m), d] + vf[fe_1(m), d]; vf [3F, ndir], v [E, ndir] complex. */
+int mba_mom_gather_rhs(int prec, const void *vf, const int32_t *rwg_fe, int64_t n_edges,
+                       int64_t ndir, void *v, mba_stream_t stream);
+
+/* =================================================================== dense
+ * Row-major complex LU with partial (row) pivoting, in place, replacing the
+ * getrf inside torch.linalg.solve (mesh_BEM_solver_3d.py:174, 346).
+ * ipiv [n] int32 (0-based, LAPACK order: rows j and ipiv[j] swapped at step j).
+ * info: device int32, set to 0 or 1 + (first zero pivot column).
+ * workspace >= mba_lu_workspace_bytes(prec, n). */
+size_t mba_lu_workspace_bytes(int prec, int64_t n);
+int mba_lu_factor(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
+                  void *workspace, size_t workspace_bytes, mba_stream_t stream);
+
+/* Row permutation of a factorisation: row i of P A is row perm[i] of A
+ * (the net effect of the sequential swaps in ipiv). */
+int mba_lu_pivots_to_perm(const int32_t *ipiv, int64_t n, int32_t *perm, mba_stream_t stream);
+
+/* Solve A x = b in place for nrhs right-hand sides b [n, ldb] complex, with
+ * (lu, perm) from mba_lu_factor / mba_lu_pivots_to_perm (the getrs inside
+ * torch.linalg.solve).  workspace >= mba_lu_solve_workspace_bytes. */
+size_t mba_lu_solve_workspace_bytes(int prec, int64_t n, int64_t nrhs);
+int mba_lu_solve(int prec, const void *lu, int64_t n, int64_t lda, const int32_t *perm, void *b,
+                 int64_t nrhs, int64_t ldb, void *workspace, size_t workspace_bytes,
+                 mba_stream_t stream);
+
+/* Complex GEMM C = alpha A B + beta C, row-major, A [m, k], B [k, n], C [m, n];
+ * alpha, beta real.  Used by the LU; exported for tests. */
+int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,
+             const void *b, int64_t ldb, double beta, void *c, int64_t ldc, mba_stream_t stream);
+
+/* FP64 residual r = v - Z x for complex128 Z [n, ldz], x/v/r [n, nrhs]. */
+int mba_residual_c128(const void *z, int64_t n, int64_t ldz, const void *x, const void *v,
+                      void *r, int64_t nrhs, mba_stream_t stream);
+
+/* y = A x for complex A [n, lda] (prec), x, y [n] complex (prec). */
+int mba_gemv(int prec, const void *a, int64_t n, int64_t lda, const void *x, void *y,
+             mba_stream_t stream);
+
+/* Element conversion complex128 <-> complex64 of an [m, n] matrix with leading dims. */
+int mba_convert_c128_c64(const void *src, int64_t m, int64_t n, int64_t lds, void *dst,
+                         int64_t ldd, mba_stream_t stream);
+
+/* ================================================================= Laplace */
+
+/* Unpacked Laplace single (layer = 1) or double (layer = 2) layer with the
+ * reference's relation-typed rule: replaces double_integration_on_panels_3d
+ * (src/mesh_BEM/utils/integrations_3d.py:252-452) as called by
+ * SingleLayer3d.compute_Vmat / DoubleLayer3d.compute_Kmat
+ * (single_layer_3d.py:43-68, double_layer_3d.py:43-68).
+ * table: rows (r1x, r2x, r1y, r2y, weight*jacobian) of the four relation types
+ * concatenated, table_offsets [5] int64 (row ranges of relations 0..3).
+ * out: [F, F, nphi, nphi] (row face fy, column face fx), nphi = 1 (PLANAR) or 3 (LINEAR). */
+int mba_laplace_layer_ref(int prec, const void *verts, const int32_t *faces, const void *face_areas,
+                          const void *face_normals, int64_t n_verts, int64_t n_faces, const void *table,
+                          const int64_t *table_offsets, int32_t layer, int32_t nphi, void *out,
+                          mba_stream_t stream);
+
+/* [F, F, nphi, nphi] -> [V, V] (replaces pack_matrix_tight's scatter_add_,
+ * src/utils/BEM_utils.py:314-357) with the vertex incidence CSR of
+ * mba_mom_vertex_sum. */
+int mba_laplace_pack(int prec, const void *unpacked, int64_t n_faces, int32_t nphi,
+                     const int64_t *vinc_ptr, const int32_t *vinc_idx, int64_t n_verts, void *out,
+                     mba_stream_t stream);
+
+/* Dirichlet RHS rhs = 1/2 g .* vert_areas + K_tight^T g
+ * (src/mesh_BEM/rhs_layer/rhs_layer_3d.py:122-146). */
+int mba_laplace_dirichlet_rhs(int prec, const void *k_tight, int64_t n_verts, const void *g,
+                              const void *vert_areas, void *rhs, mba_stream_t stream);
+
+/* P0 Galerkin Laplace operators, north-star rule (no reference counterpart):
+ *   V[i, j]  = int_Ti int_Tj 1/(4 pi R)
+ *   Kt[i, j] = int_Ti int_Tj n_i . grad_x 1/(4 pi R)   (adjoint double layer)
+ * 7-point (or 3-point) rule on far pairs, analytic FP64 inner integrals on
+ * the near pairs given by the CSR (near_ptr [F+1], near_idx) over columns j.
+ * v_out / kt_out: [F, ld] (either may be null).  rule: [nq, 4] double. */
+size_t mba_laplace_p0_workspace_bytes(int prec, int64_t n_faces);
+int mba_laplace_p0_assemble(int prec, const void *verts, const int32_t *faces, int64_t n_verts,
+                            int64_t n_faces, const double *rule, int32_t nq, const int64_t *near_ptr,
+                            const int32_t *near_idx, void *v_out, void *kt_out, int64_t ld,
+                            void *workspace, size_t workspace_bytes, mba_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MESH_BEM_AMD_H */

@@ -1,0 +1,1163 @@
+// Dense complex linear algebra for gfx950: MFMA complex GEMM, blocked LU with
+// partial pivoting (cooperative LDS-resident panel), triangular solves,
+// FP64 residual, GEMV and precision conversion.
+//
+// All matrices are row-major complex (interleaved re, im).  The LU is the
+// right-looking blocked algorithm:
+//   for each panel of nb columns:
+//     factor rows [k, n) x cols [k, k+nb) with row pivoting   (k_panel)
+//     apply the panel's row swaps to the other columns         (k_laswp)
+//     U12 = L11^{-1} A12                                       (k_trsm_l)
+//     A22 -= L21 U12                                           (MFMA GEMM)
+#include <math.h>
+#include <stdint.h>
+
+#include "mba_common.hpp"
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+inline hipStream_t S(mba_stream_t s) { return (hipStream_t)s; }
+
+// ============================================================== GEMM
+// Complex product as a real MFMA: for one complex k, the real operands are
+//   A'(i, 0..1) = (Re a_ik, Im a_ik)
+//   B'(0, 2j..2j+1) = (Re b_kj, Im b_kj),  B'(1, 2j..2j+1) = (-Im b_kj, Re b_kj)
+// so that C'(i, 2j) = Re c_ij and C'(i, 2j+1) = Im c_ij: the real output tile is
+// exactly the row-major complex memory layout of C.
+//
+// FP32: v_mfma_f32_32x32x2f32, one instruction = 32 rows x 16 complex columns
+// x 1 complex k.  Block tile 128 x 64 complex, 4 waves in 2 x 2, each wave
+// 64 x 32 (2 x 2 instructions), k-chunks of 16 staged in LDS (double buffer).
+// A is staged k-major as separate re / im planes so a wave's operand read is
+// 32 consecutive floats; B is staged row-major real.
+constexpr int G32_BM = 128, G32_BN = 64, G32_BK = 16;
+
+__global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
+                                                   const cpx<float>* __restrict__ A, int64_t lda,
+                                                   const cpx<float>* __restrict__ B, int64_t ldb,
+                                                   float beta, cpx<float>* __restrict__ C,
+                                                   int64_t ldc)
+{
+    __shared__ alignas(16) float Are[2][G32_BK][G32_BM];
+    __shared__ alignas(16) float Aim[2][G32_BK][G32_BM];
+    __shared__ alignas(16) float Bs[2][G32_BK][2 * G32_BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * G32_BM, n0 = (int64_t)blockIdx.x * G32_BN;
+
+    // global -> register staging
+    // A: thread -> row ar = tid/2, complex k = (tid&1)*8 .. +8
+    const int ar = tid >> 1, ak = (tid & 1) * 8;
+    // B: thread -> row bk = tid/16, complex cols (tid&15)*4 .. +4
+    const int bkr = tid >> 4, bc = (tid & 15) * 4;
+    float4 ra[4], rb[2];
+
+    auto load_chunk = [&](int64_t k0) {
+        const int64_t gi = m0 + ar;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t gk = k0 + ak + 2 * q;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gi < M) {
+                const float2* p = (const float2*)(A + gi * lda + gk);
+                if (gk < K) {
+                    float2 u = p[0];
+                    v.x = u.x;
+                    v.y = u.y;
+                }
+                if (gk + 1 < K) {
+                    float2 u = p[1];
+                    v.z = u.x;
+                    v.w = u.y;
+                }
+            }
+            ra[q] = v;
+        }
+        const int64_t gk = k0 + bkr;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int64_t gj = n0 + bc + 2 * q;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gk < K) {
+                const float2* p = (const float2*)(B + gk * ldb + gj);
+                if (gj < N) {
+                    float2 u = p[0];
+                    v.x = u.x;
+                    v.y = u.y;
+                }
+                if (gj + 1 < N) {
+                    float2 u = p[1];
+                    v.z = u.x;
+                    v.w = u.y;
+                }
+            }
+            rb[q] = v;
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Are[buf][ak + 2 * q][ar] = ra[q].x;
+            Aim[buf][ak + 2 * q][ar] = ra[q].y;
+            Are[buf][ak + 2 * q + 1][ar] = ra[q].z;
+            Aim[buf][ak + 2 * q + 1][ar] = ra[q].w;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) *(float4*)&Bs[buf][bkr][2 * (bc + 2 * q)] = rb[q];
+    };
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    const int hi = lane >> 5, jj = lane & 31;
+    const int bcol = hi ? (jj ^ 1) : jj;
+    const float bsign = (hi && !(jj & 1)) ? -1.f : 1.f;
+    const int nchunks = (int)((K + G32_BK - 1) / G32_BK);
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunks) load_chunk((int64_t)(c + 1) * G32_BK);
+#pragma unroll
+        for (int kk = 0; kk < G32_BK; ++kk) {
+            float a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wm * 64 + i * 32 + jj;
+                a[i] = hi ? Aim[buf][kk][row] : Are[buf][kk][row];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = bsign * Bs[buf][kk][wn * 64 + j * 32 + bcol];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (c + 1 < nchunks) {
+            __syncthreads();
+            store_chunk(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    // epilogue: register r of lane -> row 8*(r/4) + 4*hi + r%4, real column jj
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t gjr = 2 * (n0 + wn * 32 + j * 16) + jj;  // real column
+            if (gjr >= 2 * N) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                if (gi < M) {
+                    float* p = (float*)(C + gi * ldc) + gjr;
+                    float v = alpha * acc[i][j][r];
+                    if (beta != 0.f) v += beta * *p;
+                    *p = v;
+                }
+            }
+        }
+}
+
+// FP64: v_mfma_f64_16x16x4f64, one instruction = 16 rows x 8 complex columns
+// x 2 complex k.  Block tile 64 x 32 complex, 4 waves in 2 x 2, each wave
+// 32 x 16 (2 x 2 instructions), k-chunks of 8 complex.
+constexpr int G64_BM = 64, G64_BN = 32, G64_BK = 8;
+
+__global__ __launch_bounds__(256) void k_cgemm_f64(int64_t M, int64_t N, int64_t K, double alpha,
+                                                   const cpx<double>* __restrict__ A, int64_t lda,
+                                                   const cpx<double>* __restrict__ B, int64_t ldb,
+                                                   double beta, cpx<double>* __restrict__ C,
+                                                   int64_t ldc)
+{
+    __shared__ alignas(16) double Are[2][G64_BK][G64_BM];
+    __shared__ alignas(16) double Aim[2][G64_BK][G64_BM];
+    __shared__ alignas(16) double Bs[2][G64_BK][2 * G64_BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * G64_BM, n0 = (int64_t)blockIdx.x * G64_BN;
+    // A: 64 rows x 8 complex: thread -> row tid/4, complex k (tid&3)*2 .. +2
+    const int ar = tid >> 2, ak = (tid & 3) * 2;
+    // B: 8 rows x 32 complex: thread -> row tid/32, complex col tid&31
+    const int bkr = tid >> 5, bc = tid & 31;
+    double2 ra[2], rb;
+
+    auto load_chunk = [&](int64_t k0) {
+        const int64_t gi = m0 + ar;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int64_t gk = k0 + ak + q;
+            double2 v = make_double2(0.0, 0.0);
+            if (gi < M && gk < K) v = *(const double2*)(A + gi * lda + gk);
+            ra[q] = v;
+        }
+        const int64_t gk = k0 + bkr, gj = n0 + bc;
+        rb = make_double2(0.0, 0.0);
+        if (gk < K && gj < N) rb = *(const double2*)(B + gk * ldb + gj);
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            Are[buf][ak + q][ar] = ra[q].x;
+            Aim[buf][ak + q][ar] = ra[q].y;
+        }
+        *(double2*)&Bs[buf][bkr][2 * bc] = rb;
+    };
+
+    doublex4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[i][j][r] = 0.0;
+
+    // lane -> real k index kr = lane/16 (complex k = kr/2, part q = kr%2), column jj = lane%16
+    const int kr = lane >> 4, jj = lane & 15;
+    const int q = kr & 1, kc = kr >> 1;
+    const int bcol = q ? (jj ^ 1) : jj;
+    const double bsign = (q && !(jj & 1)) ? -1.0 : 1.0;
+    const int nchunks = (int)((K + G64_BK - 1) / G64_BK);
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunks) load_chunk((int64_t)(c + 1) * G64_BK);
+#pragma unroll
+        for (int ks = 0; ks < G64_BK; ks += 2) {
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wm * 32 + i * 16 + jj;
+                a[i] = q ? Aim[buf][ks + kc][row] : Are[buf][ks + kc][row];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = bsign * Bs[buf][ks + kc][wn * 32 + j * 16 + bcol];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (c + 1 < nchunks) {
+            __syncthreads();
+            store_chunk(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    // epilogue: register r of lane -> row 4*(lane/16) + r, real column lane%16
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t gjr = 2 * (n0 + wn * 16 + j * 8) + jj;
+            if (gjr >= 2 * N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gi = m0 + wm * 32 + i * 16 + 4 * kr + r;
+                if (gi < M) {
+                    double* p = (double*)(C + gi * ldc) + gjr;
+                    double v = alpha * acc[i][j][r];
+                    if (beta != 0.0) v += beta * *p;
+                    *p = v;
+                }
+            }
+        }
+}
+
+template <class T>
+int gemm(int64_t M, int64_t N, int64_t K, double alpha, const void* A, int64_t lda, const void* B,
+         int64_t ldb, double beta, void* C, int64_t ldc, hipStream_t st);
+
+template <>
+int gemm<float>(int64_t M, int64_t N, int64_t K, double alpha, const void* A, int64_t lda,
+                const void* B, int64_t ldb, double beta, void* C, int64_t ldc, hipStream_t st)
+{
+    if (M <= 0 || N <= 0) return MBA_OK;
+    dim3 grid((unsigned)((N + G32_BN - 1) / G32_BN), (unsigned)((M + G32_BM - 1) / G32_BM));
+    hipLaunchKernelGGL(k_cgemm_f32, grid, dim3(256), 0, st, M, N, K, (float)alpha,
+                       (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, (float)beta,
+                       (cpx<float>*)C, ldc);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+template <>
+int gemm<double>(int64_t M, int64_t N, int64_t K, double alpha, const void* A, int64_t lda,
+                 const void* B, int64_t ldb, double beta, void* C, int64_t ldc, hipStream_t st)
+{
+    if (M <= 0 || N <= 0) return MBA_OK;
+    dim3 grid((unsigned)((N + G64_BN - 1) / G64_BN), (unsigned)((M + G64_BM - 1) / G64_BM));
+    hipLaunchKernelGGL(k_cgemm_f64, grid, dim3(256), 0, st, M, N, K, alpha, (const cpx<double>*)A,
+                       lda, (const cpx<double>*)B, ldb, beta, (cpx<double>*)C, ldc);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+// ============================================================== LU panel
+// Cooperative kernel: G workgroups (one per CU, all resident) each hold R
+// consecutive rows of the m x nb panel in LDS.  Per column j:
+//   local arg-max of |Re| + |Im| (ties: lowest row) -> candidate slot + row copy,
+//   the owner of row j publishes row j, one grid barrier, every workgroup picks
+//   the winner, swaps locally, scales and applies the rank-1 update.
+// Slots are double-buffered on j parity, so one barrier per column suffices.
+// The grid barrier is a monotone agent-scope counter with a bounded wait; on
+// expiry the kernel sets *err and runs to completion (its result is garbage).
+constexpr int PANEL_THREADS = 256;
+constexpr long SPIN_LIMIT = 1L << 24;
+
+template <class T>
+struct PanelArgs {
+    cpx<T>* A;
+    int64_t lda;
+    int64_t k;       // first row / column of the panel
+    int64_t m;       // rows in the panel (n - k)
+    int nb;          // columns in the panel
+    int R;           // rows per workgroup
+    int32_t* ipiv;   // global [n]
+    int32_t* info;
+    unsigned long long* counter;
+    unsigned long long epoch;  // counter value at kernel start
+    int* err;
+    T* cand_val;      // [2][G]
+    int32_t* cand_row;  // [2][G]
+    cpx<T>* cand_data;  // [2][G][nb]
+    cpx<T>* diag_data;  // [2][nb]
+};
+
+__device__ __forceinline__ void grid_barrier(unsigned long long* counter, unsigned long long target,
+                                             int* err)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(counter, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        long spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > SPIN_LIMIT) {
+                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <class T>
+__device__ __forceinline__ T cabs1(cpx<T> v) { return fabs(v.re) + fabs(v.im); }
+
+template <class T>
+__global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* rows = (cpx<T>*)smem_raw;  // [R][nb]
+    __shared__ T red_v[PANEL_THREADS];
+    __shared__ int32_t red_r[PANEL_THREADS];
+    __shared__ int32_t s_piv;
+    __shared__ cpx<T> s_prow[128];
+    __shared__ cpx<T> s_drow[128];
+    __shared__ cpx<T> s_inv;
+
+    const int tid = threadIdx.x;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int nb = P.nb, R = P.R;
+    const int64_t lr0 = (int64_t)g * R;  // first local row of this workgroup
+    const int nr = (int)((P.m - lr0) < R ? (P.m - lr0 > 0 ? P.m - lr0 : 0) : R);
+
+    for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
+        int r = idx / nb, c = idx % nb;
+        rows[idx] = P.A[(P.k + lr0 + r) * P.lda + P.k + c];
+    }
+    __syncthreads();
+
+    for (int j = 0; j < nb; ++j) {
+        const int par = j & 1;
+        // --- local arg-max over local rows with panel row index >= j
+        T bv = (T)-1;
+        int32_t br = 0x7fffffff;
+        for (int r = tid; r < nr; r += PANEL_THREADS) {
+            int64_t pr = lr0 + r;
+            if (pr >= j) {
+                T v = cabs1(rows[r * nb + j]);
+                if (v > bv) {
+                    bv = v;
+                    br = (int32_t)pr;
+                }
+            }
+        }
+        red_v[tid] = bv;
+        red_r[tid] = br;
+        __syncthreads();
+        for (int s = PANEL_THREADS / 2; s > 0; s >>= 1) {
+            if (tid < s) {
+                T v2 = red_v[tid + s];
+                int32_t r2 = red_r[tid + s];
+                if (v2 > red_v[tid] || (v2 == red_v[tid] && r2 < red_r[tid])) {
+                    red_v[tid] = v2;
+                    red_r[tid] = r2;
+                }
+            }
+            __syncthreads();
+        }
+        const int32_t lbest = red_r[0];
+        if (tid == 0) {
+            P.cand_val[par * G + g] = red_v[0];
+            P.cand_row[par * G + g] = lbest;
+        }
+        cpx<T>* slot = P.cand_data + ((int64_t)par * G + g) * nb;
+        if (lbest != 0x7fffffff)
+            for (int c = tid; c < nb; c += PANEL_THREADS) slot[c] = rows[(lbest - lr0) * nb + c];
+        if (j >= lr0 && j < lr0 + nr)
+            for (int c = tid; c < nb; c += PANEL_THREADS)
+                P.diag_data[par * nb + c] = rows[(j - lr0) * nb + c];
+        grid_barrier(P.counter, P.epoch + (unsigned long long)(j + 1) * G, P.err);
+
+        // --- winner (identical in every workgroup)
+        if (tid == 0) {
+            T wv = (T)-1;
+            int32_t wr = 0x7fffffff, wg = 0;
+            for (int q = 0; q < G; ++q) {
+                T v = P.cand_val[par * G + q];
+                int32_t r = P.cand_row[par * G + q];
+                if (v > wv || (v == wv && r < wr)) {
+                    wv = v;
+                    wr = r;
+                    wg = q;
+                }
+            }
+            s_piv = wg;
+            red_r[0] = wr;
+        }
+        __syncthreads();
+        const int wg = s_piv;
+        const int32_t p = red_r[0];
+        for (int c = tid; c < nb; c += PANEL_THREADS) {
+            s_prow[c] = P.cand_data[((int64_t)par * G + wg) * nb + c];
+            s_drow[c] = P.diag_data[par * nb + c];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            cpx<T> pv = s_prow[j];
+            T d = pv.re * pv.re + pv.im * pv.im;
+            cpx<T> inv;
+            if (d == (T)0) {
+                inv.re = 0;
+                inv.im = 0;
+                if (g == 0) atomicCAS(P.info, 0, (int32_t)(P.k + j + 1));
+            } else {
+                inv.re = pv.re / d;
+                inv.im = -pv.im / d;
+            }
+            s_inv = inv;
+            if (g == 0) P.ipiv[P.k + j] = (int32_t)(P.k + p);
+        }
+        // swap rows j and p (panel-local indices)
+        if (p != j) {
+            if (p >= lr0 && p < lr0 + nr)
+                for (int c = tid; c < nb; c += PANEL_THREADS) rows[(p - lr0) * nb + c] = s_drow[c];
+            if (j >= lr0 && j < lr0 + nr)
+                for (int c = tid; c < nb; c += PANEL_THREADS) rows[(j - lr0) * nb + c] = s_prow[c];
+        }
+        __syncthreads();
+        // scale column j below the diagonal, then rank-1 update of columns > j
+        const cpx<T> inv = s_inv;
+        for (int r = tid; r < nr; r += PANEL_THREADS) {
+            if (lr0 + r > j) {
+                cpx<T> a = rows[r * nb + j];
+                cpx<T> l;
+                l.re = a.re * inv.re - a.im * inv.im;
+                l.im = a.re * inv.im + a.im * inv.re;
+                rows[r * nb + j] = l;
+            }
+        }
+        __syncthreads();
+        const int w = nb - j - 1;
+        if (w > 0) {
+            for (int idx = tid; idx < nr * w; idx += PANEL_THREADS) {
+                int r = idx / w, c = j + 1 + idx % w;
+                if (lr0 + r > j) {
+                    cpx<T> l = rows[r * nb + j];
+                    cpx<T> u = s_prow[c];
+                    cpx<T> a = rows[r * nb + c];
+                    a.re -= l.re * u.re - l.im * u.im;
+                    a.im -= l.re * u.im + l.im * u.re;
+                    rows[r * nb + c] = a;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
+        int r = idx / nb, c = idx % nb;
+        P.A[(P.k + lr0 + r) * P.lda + P.k + c] = rows[idx];
+    }
+}
+
+// ============================================================== row swaps
+// Plan (one thread): the panel's sequential swaps (k+j <-> ipiv[k+j]) act on
+// the row set S = {k..k+nb-1} U {ipiv}; slots 0..nb-1 are rows k..k+nb-1, the
+// other pivot rows get slots nb.. in order of first appearance.  plan[j] =
+// (slot_a, slot_b) per swap; slot_row[s] = row of slot s; returns |S| in
+// plan_n.
+__global__ void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
+                            int32_t* __restrict__ slot_row, int32_t* __restrict__ plan,
+                            int32_t* __restrict__ plan_n)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int ns = nb;
+    for (int s = 0; s < nb; ++s) slot_row[s] = (int32_t)(k + s);
+    for (int j = 0; j < nb; ++j) {
+        int32_t p = ipiv[k + j];
+        int sp;
+        if (p < k + nb) {
+            sp = (int)(p - k);
+        } else {
+            sp = -1;
+            for (int s = nb; s < ns; ++s)
+                if (slot_row[s] == p) {
+                    sp = s;
+                    break;
+                }
+            if (sp < 0) {
+                sp = ns;
+                slot_row[ns++] = p;
+            }
+        }
+        plan[2 * j] = j;
+        plan[2 * j + 1] = sp;
+    }
+    *plan_n = ns;
+}
+
+// Each block: LASWP_COLS complex columns of the column range; the |S| rows are
+// staged in LDS, swapped in place, written back.
+constexpr int LASWP_COLS = 16;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t lda, int64_t c_lo,
+                                               int64_t c_hi, int64_t skip_lo, int64_t skip_hi,
+                                               int nb, const int32_t* __restrict__ slot_row,
+                                               const int32_t* __restrict__ plan,
+                                               const int32_t* __restrict__ plan_n)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* buf = (cpx<T>*)smem_raw;  // [2 nb][LASWP_COLS]
+    const int ns = *plan_n;
+    // map block -> columns, skipping [skip_lo, skip_hi)
+    int64_t c0 = c_lo + (int64_t)blockIdx.x * LASWP_COLS;
+    if (c0 >= skip_lo) c0 += (skip_hi - skip_lo);
+    if (c0 >= c_hi) return;
+    int64_t cend = c0 + LASWP_COLS;
+    if (c0 < skip_lo && cend > skip_lo) cend = skip_lo;
+    if (cend > c_hi) cend = c_hi;
+    const int w = (int)(cend - c0);
+    for (int idx = threadIdx.x; idx < ns * LASWP_COLS; idx += blockDim.x) {
+        int s = idx / LASWP_COLS, c = idx % LASWP_COLS;
+        if (c < w) buf[idx] = A[(int64_t)slot_row[s] * lda + c0 + c];
+    }
+    __syncthreads();
+    if (threadIdx.x < LASWP_COLS && (int)threadIdx.x < w) {
+        const int c = threadIdx.x;
+        for (int j = 0; j < nb; ++j) {
+            int sa = plan[2 * j], sb = plan[2 * j + 1];
+            if (sa != sb) {
+                cpx<T> t = buf[sa * LASWP_COLS + c];
+                buf[sa * LASWP_COLS + c] = buf[sb * LASWP_COLS + c];
+                buf[sb * LASWP_COLS + c] = t;
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < ns * LASWP_COLS; idx += blockDim.x) {
+        int s = idx / LASWP_COLS, c = idx % LASWP_COLS;
+        if (c < w) A[(int64_t)slot_row[s] * lda + c0 + c] = buf[idx];
+    }
+}
+
+// ============================================================== TRSM (L unit lower)
+// X = L11^{-1} X for X = rows [k, k+nb) x columns [c0, c0 + 64) of A, L11 the
+// unit lower panel block.  X is staged in LDS; forward substitution row by
+// row with 4 partial dot products per column.
+constexpr int TRSM_COLS = 64;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t lda, int64_t k,
+                                                int nb, int64_t col_begin, int64_t col_end)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* X = (cpx<T>*)smem_raw;  // [nb][TRSM_COLS]
+    __shared__ cpx<T> part[4][TRSM_COLS];
+    const int tid = threadIdx.x;
+    const int64_t c0 = col_begin + (int64_t)blockIdx.x * TRSM_COLS;
+    const int w = (int)((col_end - c0) < TRSM_COLS ? (col_end - c0) : TRSM_COLS);
+    for (int idx = tid; idx < nb * TRSM_COLS; idx += 256) {
+        int r = idx / TRSM_COLS, c = idx % TRSM_COLS;
+        cpx<T> v;
+        v.re = 0;
+        v.im = 0;
+        if (c < w) v = A[(k + r) * lda + c0 + c];
+        X[idx] = v;
+    }
+    __syncthreads();
+    const int c = tid % TRSM_COLS, qd = tid / TRSM_COLS;
+    for (int r = 1; r < nb; ++r) {
+        const cpx<T>* Lr = A + (k + r) * lda + k;
+        T sr = 0, si = 0;
+        for (int t = qd; t < r; t += 4) {
+            cpx<T> l = Lr[t];
+            cpx<T> x = X[t * TRSM_COLS + c];
+            sr += l.re * x.re - l.im * x.im;
+            si += l.re * x.im + l.im * x.re;
+        }
+        part[qd][c].re = sr;
+        part[qd][c].im = si;
+        __syncthreads();
+        if (qd == 0) {
+            cpx<T> x = X[r * TRSM_COLS + c];
+            x.re -= (part[0][c].re + part[1][c].re) + (part[2][c].re + part[3][c].re);
+            x.im -= (part[0][c].im + part[1][c].im) + (part[2][c].im + part[3][c].im);
+            X[r * TRSM_COLS + c] = x;
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nb * TRSM_COLS; idx += 256) {
+        int r = idx / TRSM_COLS, cc = idx % TRSM_COLS;
+        if (cc < w) A[(k + r) * lda + c0 + cc] = X[idx];
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ULL;
+}
+
+__global__ void k_zero_i32(int32_t* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ============================================================== workspace
+struct LuWs {
+    unsigned long long* counter;
+    int* err;
+    void* cand_val;
+    int32_t* cand_row;
+    void* cand_data;
+    void* diag_data;
+    int32_t* slot_row;
+    int32_t* plan;
+    int32_t* plan_n;
+    size_t total;
+};
+
+constexpr int MAX_G = 1024;
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+LuWs lu_ws_layout(int prec, int64_t n, char* base)
+{
+    const size_t esz = prec == MBA_F32 ? 4 : 8;
+    const int nbmax = 128;
+    LuWs w;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base ? base + off : nullptr;
+        off += al256(bytes);
+        return p;
+    };
+    w.counter = (unsigned long long*)take(256);
+    w.err = (int*)take(256);
+    w.cand_val = take(2 * MAX_G * esz);
+    w.cand_row = (int32_t*)take(2 * MAX_G * 4);
+    w.cand_data = take(2 * MAX_G * (size_t)nbmax * 2 * esz);
+    w.diag_data = take(2 * (size_t)nbmax * 2 * esz);
+    w.slot_row = (int32_t*)take(2 * nbmax * 4);
+    w.plan = (int32_t*)take(2 * nbmax * 4);
+    w.plan_n = (int32_t*)take(256);
+    w.total = off;
+    (void)n;
+    return w;
+}
+
+int device_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return cus;
+}
+
+template <class T>
+int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, void* ws_ptr,
+              size_t ws_bytes, hipStream_t st)
+{
+    const int prec = sizeof(T) == 4 ? MBA_F32 : MBA_F64;
+    LuWs ws = lu_ws_layout(prec, n, (char*)ws_ptr);
+    if (ws_bytes < ws.total) return MBA_ERR_ALLOC;
+    cpx<T>* A = (cpx<T>*)a;
+    const int cus = device_cus();
+    if (cus <= 0) return MBA_ERR_LAUNCH;
+    const size_t esz = sizeof(cpx<T>);
+    const size_t lds_budget = 150 * 1024;
+    // panel width: largest of 128/64/32 whose per-CU row share fits the LDS budget
+    int NB = 128;
+    while (NB > 16 && (size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) NB /= 2;
+    if ((size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) return MBA_ERR_SIZE;
+    static bool attr_set[2] = {false, false};
+    if (!attr_set[prec]) {
+        (void)hipFuncSetAttribute((const void*)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_budget);
+        (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(128 * TRSM_COLS * esz));
+        attr_set[prec] = true;
+    }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ws.counter, 64);
+    MBA_LAUNCH_CHECK();
+    // info := 0 and err := 0 (both 32-bit; err lives in its own 256-byte slot)
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ws.err, 32);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, 1);
+    MBA_LAUNCH_CHECK();
+    unsigned long long epoch = 0;
+    for (int64_t k = 0; k < n; k += NB) {
+        const int nb = (int)((n - k) < NB ? (n - k) : NB);
+        const int64_t m = n - k;
+        int G = (int)((m + 7) / 8);
+        if (G > cus) G = cus;
+        if (G > MAX_G) G = MAX_G;
+        int R = (int)((m + G - 1) / G);
+        G = (int)((m + R - 1) / R);
+        PanelArgs<T> P;
+        P.A = A;
+        P.lda = lda;
+        P.k = k;
+        P.m = m;
+        P.nb = nb;
+        P.R = R;
+        P.ipiv = ipiv;
+        P.info = info;
+        P.counter = ws.counter;
+        P.epoch = epoch;
+        P.err = ws.err;
+        P.cand_val = (T*)ws.cand_val;
+        P.cand_row = ws.cand_row;
+        P.cand_data = (cpx<T>*)ws.cand_data;
+        P.diag_data = (cpx<T>*)ws.diag_data;
+        void* args[] = {&P};
+        size_t shm = (size_t)R * nb * esz;
+        if (hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(G), dim3(PANEL_THREADS), args,
+                                       (unsigned)shm, st) != hipSuccess)
+            return MBA_ERR_LAUNCH;
+        epoch += (unsigned long long)nb * G;
+        const int64_t n2 = n - k - nb;
+        if (k > 0 || n2 > 0) {
+            hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(64), 0, st, ipiv, k, nb, ws.slot_row,
+                               ws.plan, ws.plan_n);
+            MBA_LAUNCH_CHECK();
+            const int64_t ncols = n - nb;
+            const int64_t blocks = (k + LASWP_COLS - 1) / LASWP_COLS + (n2 + LASWP_COLS - 1) / LASWP_COLS;
+            // blocks enumerate [0, k) then [k+nb, n); the kernel skips the panel range
+            hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)blocks), dim3(256),
+                               (size_t)2 * nb * LASWP_COLS * esz, st, A, lda, (int64_t)0, n,
+                               k, k + nb, nb, ws.slot_row, ws.plan, ws.plan_n);
+            MBA_LAUNCH_CHECK();
+            (void)ncols;
+        }
+        if (n2 > 0) {
+            hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n2 + TRSM_COLS - 1) / TRSM_COLS)),
+                               dim3(256), (size_t)nb * TRSM_COLS * esz, st, A, lda, k, nb, k + nb, n);
+            MBA_LAUNCH_CHECK();
+            int rc = gemm<T>(n2, n2, nb, -1.0, A + (k + nb) * lda + k, lda, A + k * lda + k + nb, lda,
+                             1.0, A + (k + nb) * lda + k + nb, lda, st);
+            if (rc != MBA_OK) return rc;
+        }
+    }
+    int herr = 0;
+    if (hipMemcpyAsync(&herr, ws.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return MBA_ERR_LAUNCH;
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    return herr ? MBA_ERR_TIMEOUT : MBA_OK;
+}
+
+// ============================================================== triangular solves
+// ipiv -> permutation gather list: after the sequential swaps, position i holds
+// original row perm[i].  One wave; the permutation lives in LDS (n <= PERM_LDS)
+// and pivots are read 64 at a time and broadcast lane by lane.
+constexpr int PERM_LDS = 36 * 1024;
+
+__global__ __launch_bounds__(64) void k_ipiv_perm(const int32_t* __restrict__ ipiv, int64_t n,
+                                                  int32_t* __restrict__ perm)
+{
+    __shared__ int32_t pl[PERM_LDS];
+    const int lane = threadIdx.x;
+    const bool in_lds = n <= PERM_LDS;
+    int32_t* P = in_lds ? pl : perm;
+    for (int64_t i = lane; i < n; i += 64) P[i] = (int32_t)i;
+    __syncthreads();
+    for (int64_t c = 0; c < n; c += 64) {
+        const int32_t v = (c + lane < n) ? ipiv[c + lane] : 0;
+        const int cnt = (int)((n - c) < 64 ? (n - c) : 64);
+        for (int q = 0; q < cnt; ++q) {
+            const int32_t p = __shfl(v, q);
+            if (lane == 0) {
+                const int64_t j = c + q;
+                int32_t t = P[j];
+                P[j] = P[p];
+                P[p] = t;
+            }
+        }
+    }
+    __syncthreads();
+    if (in_lds)
+        for (int64_t i = lane; i < n; i += 64) perm[i] = pl[i];
+}
+
+template <class T>
+__global__ void k_permute_rows(const cpx<T>* __restrict__ src, int64_t ld_src, const int32_t* perm,
+                               int64_t n, int64_t nrhs, cpx<T>* __restrict__ dst, int64_t ld_dst)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n * nrhs) return;
+    int64_t i = id / nrhs, c = id % nrhs;
+    const int64_t si = perm ? (int64_t)perm[i] : i;
+    dst[i * ld_dst + c] = src[si * ld_src + c];
+}
+
+// Diagonal block solve, one workgroup per chunk of 4 right-hand sides: the
+// bs x bs block is staged transposed in LDS (column t contiguous); column
+// sweep: x_t is final, then every (row, rhs) thread updates its entry.
+// lower: unit diagonal, forward; upper: divides by the diagonal, backward.
+constexpr int DS_RHS = 4;
+
+template <class T, bool LOWER>
+__global__ __launch_bounds__(256) void k_diag_solve(const cpx<T>* __restrict__ A, int64_t lda,
+                                                    int64_t r0, int bs, cpx<T>* __restrict__ B,
+                                                    int64_t ldb, int64_t nrhs)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* Lt = (cpx<T>*)smem_raw;  // [bs][bs]: Lt[t * bs + i] = A[r0 + i][r0 + t]
+    __shared__ cpx<T> X[128][DS_RHS];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * DS_RHS;
+    const int nc = (int)((nrhs - c0) < DS_RHS ? (nrhs - c0) : DS_RHS);
+    for (int idx = tid; idx < bs * bs; idx += 256) {
+        int i = idx / bs, t = idx % bs;
+        Lt[t * bs + i] = A[(r0 + i) * lda + r0 + t];
+    }
+    for (int idx = tid; idx < bs * DS_RHS; idx += 256) {
+        int i = idx / DS_RHS, c = idx % DS_RHS;
+        cpx<T> v;
+        v.re = 0;
+        v.im = 0;
+        if (c < nc) v = B[(r0 + i) * ldb + c0 + c];
+        X[i][c] = v;
+    }
+    __syncthreads();
+    const int i = tid / DS_RHS, c = tid % DS_RHS;  // 64 rows per pass
+    for (int s = 0; s < bs; ++s) {
+        const int t = LOWER ? s : bs - 1 - s;
+        if (!LOWER) {
+            if (tid < DS_RHS) {
+                cpx<T> d = Lt[t * bs + t];
+                cpx<T> x = X[t][tid];
+                T dd = d.re * d.re + d.im * d.im;
+                cpx<T> q;
+                q.re = (x.re * d.re + x.im * d.im) / dd;
+                q.im = (x.im * d.re - x.re * d.im) / dd;
+                X[t][tid] = q;
+            }
+            __syncthreads();
+        }
+        const cpx<T> xt = X[t][c];
+        for (int ii = i; ii < bs; ii += 256 / DS_RHS) {
+            if (LOWER ? (ii > t) : (ii < t)) {
+                cpx<T> l = Lt[t * bs + ii];
+                cpx<T> x = X[ii][c];
+                x.re -= l.re * xt.re - l.im * xt.im;
+                x.im -= l.re * xt.im + l.im * xt.re;
+                X[ii][c] = x;
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < bs * DS_RHS; idx += 256) {
+        int ii = idx / DS_RHS, cc = idx % DS_RHS;
+        if (cc < nc) B[(r0 + ii) * ldb + c0 + cc] = X[ii][cc];
+    }
+}
+
+// Block update B[rows] -= A[rows, cols r0..r0+bs) * B[r0..r0+bs), one wave per
+// (row, rhs); lanes stride the bs columns.
+template <class T>
+__global__ __launch_bounds__(256) void k_block_update(const cpx<T>* __restrict__ A, int64_t lda,
+                                                      int64_t row_lo, int64_t row_hi, int64_t r0,
+                                                      int bs, cpx<T>* __restrict__ B, int64_t ldb,
+                                                      int64_t nrhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nrow = row_hi - row_lo;
+    if (wid >= nrow * nrhs) return;
+    const int64_t i = row_lo + wid / nrhs, c = wid % nrhs;
+    const cpx<T>* Ar = A + i * lda + r0;
+    T sr = 0, si = 0;
+    for (int t = lane; t < bs; t += 64) {
+        cpx<T> a = Ar[t], y = B[(r0 + t) * ldb + c];
+        sr += a.re * y.re - a.im * y.im;
+        si += a.re * y.im + a.im * y.re;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off);
+        si += __shfl_xor(si, off);
+    }
+    if (lane == 0) {
+        cpx<T> b = B[i * ldb + c];
+        b.re -= sr;
+        b.im -= si;
+        B[i * ldb + c] = b;
+    }
+}
+
+template <class T>
+int lu_solve(const void* lu, int64_t n, int64_t lda, const int32_t* perm, void* b, int64_t nrhs,
+             int64_t ldb, void* ws_ptr, size_t ws_bytes, hipStream_t st)
+{
+    const size_t esz = sizeof(cpx<T>);
+    if (ws_bytes < (size_t)n * nrhs * esz) return MBA_ERR_ALLOC;
+    cpx<T>* tmp = (cpx<T>*)ws_ptr;
+    const cpx<T>* A = (const cpx<T>*)lu;
+    cpx<T>* B = (cpx<T>*)b;
+    const int64_t tot = n * nrhs;
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, B, ldb,
+                       perm, n, nrhs, tmp, nrhs);
+    MBA_LAUNCH_CHECK();
+    const int BS = 64;
+    const size_t dshm = (size_t)BS * BS * esz;
+    const unsigned dgrid = (unsigned)((nrhs + DS_RHS - 1) / DS_RHS);
+    // forward: L y = P b (tmp holds P b, ld = nrhs)
+    for (int64_t r0 = 0; r0 < n; r0 += BS) {
+        const int bs = (int)((n - r0) < BS ? (n - r0) : BS);
+        hipLaunchKernelGGL((k_diag_solve<T, true>), dim3(dgrid), dim3(256), dshm, st, A, lda, r0, bs,
+                           tmp, nrhs, nrhs);
+        MBA_LAUNCH_CHECK();
+        const int64_t lo = r0 + bs;
+        if (lo < n) {
+            if (nrhs >= 32) {
+                int rc = gemm<T>(n - lo, nrhs, bs, -1.0, A + lo * lda + r0, lda, tmp + r0 * nrhs, nrhs,
+                                 1.0, tmp + lo * nrhs, nrhs, st);
+                if (rc != MBA_OK) return rc;
+            } else {
+                const int64_t waves = (n - lo) * nrhs;
+                hipLaunchKernelGGL(k_block_update<T>, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256),
+                                   0, st, A, lda, lo, n, r0, bs, tmp, nrhs, nrhs);
+                MBA_LAUNCH_CHECK();
+            }
+        }
+    }
+    // backward: U x = y
+    const int64_t nblk = (n + BS - 1) / BS;
+    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+        const int64_t r0 = bi * BS;
+        const int bs = (int)((n - r0) < BS ? (n - r0) : BS);
+        hipLaunchKernelGGL((k_diag_solve<T, false>), dim3(dgrid), dim3(256), dshm, st, A, lda, r0, bs,
+                           tmp, nrhs, nrhs);
+        MBA_LAUNCH_CHECK();
+        if (r0 > 0) {
+            if (nrhs >= 32) {
+                int rc = gemm<T>(r0, nrhs, bs, -1.0, A + r0, lda, tmp + r0 * nrhs, nrhs, 1.0, tmp, nrhs, st);
+                if (rc != MBA_OK) return rc;
+            } else {
+                const int64_t waves = r0 * nrhs;
+                hipLaunchKernelGGL(k_block_update<T>, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256),
+                                   0, st, A, lda, (int64_t)0, r0, r0, bs, tmp, nrhs, nrhs);
+                MBA_LAUNCH_CHECK();
+            }
+        }
+    }
+    // copy back into b (ld = ldb)
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, tmp, nrhs,
+                       nullptr, n, nrhs, B, ldb);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+// ============================================================== residual / gemv
+// r = v - Z x (complex128), one wave per (row, rhs).
+__global__ __launch_bounds__(256) void k_residual_c128(const cpx<double>* __restrict__ Z, int64_t n,
+                                                       int64_t ldz, const cpx<double>* __restrict__ x,
+                                                       const cpx<double>* __restrict__ v,
+                                                       cpx<double>* __restrict__ r, int64_t nrhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wid >= n * nrhs) return;
+    const int64_t i = wid / nrhs, c = wid % nrhs;
+    const cpx<double>* Zr = Z + i * ldz;
+    double sr = 0, si = 0;
+    for (int64_t t = lane; t < n; t += 64) {
+        cpx<double> a = Zr[t], y = x[t * nrhs + c];
+        sr += a.re * y.re - a.im * y.im;
+        si += a.re * y.im + a.im * y.re;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off);
+        si += __shfl_xor(si, off);
+    }
+    if (lane == 0) {
+        cpx<double> b = v[i * nrhs + c];
+        store_cpx(r + i * nrhs + c, b.re - sr, b.im - si);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gemv(const cpx<T>* __restrict__ A, int64_t n, int64_t lda,
+                                              const cpx<T>* __restrict__ x, cpx<T>* __restrict__ y)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (i >= n) return;
+    const cpx<T>* Ar = A + i * lda;
+    T sr = 0, si = 0;
+    for (int64_t t = lane; t < n; t += 64) {
+        cpx<T> a = Ar[t], b = x[t];
+        sr += a.re * b.re - a.im * b.im;
+        si += a.re * b.im + a.im * b.re;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off);
+        si += __shfl_xor(si, off);
+    }
+    if (lane == 0) store_cpx(y + i, sr, si);
+}
+
+__global__ void k_c128_to_c64(const cpx<double>* __restrict__ s, int64_t m, int64_t n, int64_t lds,
+                              cpx<float>* __restrict__ d, int64_t ldd)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= m * n) return;
+    int64_t i = id / n, j = id % n;
+    cpx<double> v = s[i * lds + j];
+    store_cpx(d + i * ldd + j, (float)v.re, (float)v.im);
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+size_t mba_lu_workspace_bytes(int prec, int64_t n)
+{
+    if ((prec != MBA_F32 && prec != MBA_F64) || n <= 0) return 0;
+    return lu_ws_layout(prec, n, nullptr).total;
+}
+
+int mba_lu_factor(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,
+                  void* workspace, size_t workspace_bytes, mba_stream_t stream)
+{
+    if (!a || !ipiv || !info || !workspace) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+    if (lda < n) return MBA_ERR_ARG;
+    if (prec == MBA_F32) return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+    return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+}
+
+int mba_lu_pivots_to_perm(const int32_t* ipiv, int64_t n, int32_t* perm, mba_stream_t stream)
+{
+    if (!ipiv || !perm) return MBA_ERR_ARG;
+    if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+    hipLaunchKernelGGL(k_ipiv_perm, dim3(1), dim3(64), 0, S(stream), ipiv, n, perm);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+size_t mba_lu_solve_workspace_bytes(int prec, int64_t n, int64_t nrhs)
+{
+    if ((prec != MBA_F32 && prec != MBA_F64) || n <= 0 || nrhs <= 0) return 0;
+    return (size_t)n * nrhs * (prec == MBA_F32 ? 8 : 16);
+}
+
+int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const int32_t* perm, void* b,
+                 int64_t nrhs, int64_t ldb, void* workspace, size_t workspace_bytes,
+                 mba_stream_t stream)
+{
+    if (!lu || !perm || !b || !workspace) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n <= 0 || nrhs <= 0) return MBA_ERR_SIZE;
+    if (lda < n || ldb < nrhs) return MBA_ERR_ARG;
+    if (prec == MBA_F32)
+        return lu_solve<float>(lu, n, lda, perm, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
+    return lu_solve<double>(lu, n, lda, perm, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
+}
+
+int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void* a, int64_t lda,
+             const void* b, int64_t ldb, double beta, void* c, int64_t ldc, mba_stream_t stream)
+{
+    if (!a || !b || !c) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (m < 0 || n < 0 || k < 0) return MBA_ERR_SIZE;
+    if (lda < k || ldb < n || ldc < n) return MBA_ERR_ARG;
+    if (prec == MBA_F32) return gemm<float>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, S(stream));
+    return gemm<double>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, S(stream));
+}
+
+int mba_residual_c128(const void* z, int64_t n, int64_t ldz, const void* x, const void* v, void* r,
+                      int64_t nrhs, mba_stream_t stream)
+{
+    if (!z || !x || !v || !r) return MBA_ERR_ARG;
+    if (n <= 0 || nrhs <= 0) return MBA_ERR_SIZE;
+    if (ldz < n) return MBA_ERR_ARG;
+    const int64_t waves = n * nrhs;
+    hipLaunchKernelGGL(k_residual_c128, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0,
+                       S(stream), (const cpx<double>*)z, n, ldz, (const cpx<double>*)x,
+                       (const cpx<double>*)v, (cpx<double>*)r, nrhs);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+int mba_gemv(int prec, const void* a, int64_t n, int64_t lda, const void* x, void* y,
+             mba_stream_t stream)
+{
+    if (!a || !x || !y) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n <= 0) return MBA_ERR_SIZE;
+    if (lda < n) return MBA_ERR_ARG;
+    dim3 grid((unsigned)((n * 64 + 255) / 256));
+    if (prec == MBA_F32)
+        hipLaunchKernelGGL(k_gemv<float>, grid, dim3(256), 0, S(stream), (const cpx<float>*)a, n, lda,
+                           (const cpx<float>*)x, (cpx<float>*)y);
+    else
+        hipLaunchKernelGGL(k_gemv<double>, grid, dim3(256), 0, S(stream), (const cpx<double>*)a, n, lda,
+                           (const cpx<double>*)x, (cpx<double>*)y);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+int mba_convert_c128_c64(const void* src, int64_t m, int64_t n, int64_t lds, void* dst, int64_t ldd,
+                         mba_stream_t stream)
+{
+    if (!src || !dst) return MBA_ERR_ARG;
+    if (m <= 0 || n <= 0) return MBA_ERR_SIZE;
+    if (lds < n || ldd < n) return MBA_ERR_ARG;
+    const int64_t tot = m * n;
+    hipLaunchKernelGGL(k_c128_to_c64, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, S(stream),
+                       (const cpx<double>*)src, m, n, lds, (cpx<float>*)dst, ldd);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+}  // extern "C"

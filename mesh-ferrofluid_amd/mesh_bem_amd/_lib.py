@@ -1,0 +1,115 @@
+"""ctypes binding of the C ABI declared in include/mesh_bem_amd.h.
+
+The library is the in-tree ``libmesh_bem_amd.so`` built by ``_build.build()``.
+There is no fallback: if the library is missing or a GPU is absent, calls
+raise.  Tensors are passed as raw device pointers together with torch's
+current HIP stream.
+"""
+
+import ctypes
+import os
+
+import torch
+
+from ._build import LIB_PATH
+
+MBA_F32, MBA_F64 = 0, 1
+
+_c = ctypes
+_vp, _i32p, _i64p, _dp = _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p
+_i32, _i64, _dbl, _sz = _c.c_int32, _c.c_int64, _c.c_double, _c.c_size_t
+
+# name -> (restype, argtypes) for every symbol of include/mesh_bem_amd.h
+SIGNATURES = {
+    "mba_status_string": (_c.c_char_p, [_c.c_int]),
+    "mba_version": (_c.c_int, []),
+    "mba_mom_zmat_faceedge_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
+                                             _i32, _dbl, _i64, _i64, _vp, _i64, _vp]),
+    "mba_mom_rhs_faceedge_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _vp, _vp, _i32, _dbl, _i64,
+                                            _i64, _vp, _vp]),
+    "mba_mom_interp_faceedge": (_c.c_int, [_c.c_int, _vp, _i32p, _i32p, _vp, _vp, _i64, _i64, _i64,
+                                           _i64, _vp, _vp]),
+    "mba_mom_vertex_sum": (_c.c_int, [_c.c_int, _vp, _i64p, _i32p, _i64, _i64, _i64, _vp, _vp]),
+    "mba_mom_workspace_bytes": (_sz, [_c.c_int, _i64]),
+    "mba_mom_zmat_faceedge_ns": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _dbl, _i64p,
+                                            _i32p, _i64, _i64, _vp, _i64, _vp, _sz, _vp]),
+    "mba_mom_rhs_faceedge_ns": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _dbl, _dp, _dp,
+                                           _i64, _vp, _vp]),
+    "mba_mom_gather_rwg": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i64, _i32p, _i64, _vp, _i64, _vp]),
+    "mba_mom_gather_rhs": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _vp, _vp]),
+    "mba_lu_workspace_bytes": (_sz, [_c.c_int, _i64]),
+    "mba_lu_factor": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp]),
+    "mba_lu_pivots_to_perm": (_c.c_int, [_i32p, _i64, _i32p, _vp]),
+    "mba_lu_solve_workspace_bytes": (_sz, [_c.c_int, _i64, _i64]),
+    "mba_lu_solve": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _vp, _i64, _i64, _vp, _sz, _vp]),
+    "mba_gemm": (_c.c_int, [_c.c_int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64,
+                            _vp]),
+    "mba_residual_c128": (_c.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
+    "mba_gemv": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _vp]),
+    "mba_convert_c128_c64": (_c.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp]),
+    "mba_laplace_layer_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _i64, _i64, _vp, _i64p, _i32,
+                                         _i32, _vp, _vp]),
+    "mba_laplace_pack": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
+    "mba_laplace_dirichlet_rhs": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "mba_laplace_p0_workspace_bytes": (_sz, [_c.c_int, _i64]),
+    "mba_laplace_p0_assemble": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _i64p, _i32p,
+                                           _vp, _vp, _i64, _vp, _sz, _vp]),
+}
+
+_LIB = None
+
+
+class MBAError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise MBAError(
+                f"{LIB_PATH} not found: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`"
+            )
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name, None)
+            if fn is None:
+                continue  # exported-symbol completeness is checked by the tests
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(status: int, what: str = ""):
+    if status != 0:
+        msg = load().mba_status_string(status).decode()
+        raise MBAError(f"{what}: {msg} (status {status})")
+
+
+def ptr(t: torch.Tensor):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def stream():
+    if not torch.cuda.is_available():
+        raise MBAError("no GPU: the mesh-BEM kernels run on a HIP device only")
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def prec_of(dtype: torch.dtype) -> int:
+    if dtype in (torch.float32, torch.complex64):
+        return MBA_F32
+    if dtype in (torch.float64, torch.complex128):
+        return MBA_F64
+    raise MBAError(f"unsupported dtype {dtype}")
+
+
+def require_device(*tensors):
+    for t in tensors:
+        if t is not None and not t.is_cuda:
+            raise MBAError("mesh-BEM kernels take device tensors")
+        if t is not None and not t.is_contiguous():
+            raise MBAError("mesh-BEM kernels take contiguous tensors")

@@ -1,0 +1,141 @@
+"""Dense complex solves on the HIP library.
+
+* ``LUFactor``: blocked row-pivoted LU (MFMA trailing update) of a complex64 or
+  complex128 matrix, then triangular solves -- the getrf/getrs pair behind the
+  reference's ``torch.linalg.solve`` (src/mesh_BEM/mesh_BEM_solver_3d.py:174).
+* ``refine_solve``: mixed precision -- factor in FP32, then iterative
+  refinement whose residual r = V - Z J is formed in FP64 against the
+  complex128 matrix, until the correction falls below ``tol`` relative to J.
+"""
+
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _lib
+
+
+def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha: float = 1.0,
+         beta: float = 0.0) -> torch.Tensor:
+    """C = alpha A B + beta C for row-major complex tensors (MFMA kernel)."""
+    lib = _lib.load()
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2 and A.dtype == B.dtype
+    if C is None:
+        C = torch.zeros((M, N), dtype=A.dtype, device=A.device)
+        beta = 0.0
+    for t in (A, B, C):
+        assert t.stride(1) == 1, "row-major with unit column stride"
+    st = lib.mba_gemm(_lib.prec_of(A.dtype), M, N, K, float(alpha), _lib.ptr(A), A.stride(0),
+                      _lib.ptr(B), B.stride(0), float(beta), _lib.ptr(C), C.stride(0), _lib.stream())
+    _lib.check(st, "mba_gemm")
+    return C
+
+
+class LUFactor:
+    """In-place LU with partial pivoting of a square complex matrix."""
+
+    def __init__(self, A: torch.Tensor, overwrite: bool = False):
+        lib = _lib.load()
+        if A.dim() != 2 or A.shape[0] != A.shape[1]:
+            raise ValueError("LUFactor needs a square matrix")
+        if A.dtype not in (torch.complex64, torch.complex128):
+            raise TypeError("LUFactor takes complex64 or complex128")
+        self.lu = A if overwrite else A.clone()
+        if self.lu.stride(1) != 1:
+            self.lu = self.lu.contiguous()
+        n = A.shape[0]
+        self.n = n
+        self.prec = _lib.prec_of(A.dtype)
+        dev = A.device
+        self.ipiv = torch.empty(n, dtype=torch.int32, device=dev)
+        self.perm = torch.empty(n, dtype=torch.int32, device=dev)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        wsb = lib.mba_lu_workspace_bytes(self.prec, n)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        st = lib.mba_lu_factor(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0), _lib.ptr(self.ipiv),
+                               _lib.ptr(info), _lib.ptr(ws), wsb, _lib.stream())
+        _lib.check(st, "mba_lu_factor")
+        st = lib.mba_lu_pivots_to_perm(_lib.ptr(self.ipiv), n, _lib.ptr(self.perm), _lib.stream())
+        _lib.check(st, "mba_lu_pivots_to_perm")
+        self.info = int(info.item())
+        if self.info != 0:
+            raise _lib.MBAError(f"mba_lu_factor: exactly zero pivot in column {self.info - 1}")
+
+    def solve(self, B: torch.Tensor) -> torch.Tensor:
+        """X = A^{-1} B for B [n] or [n, nrhs] (returns a new tensor of the factor dtype)."""
+        lib = _lib.load()
+        vec = B.dim() == 1
+        X = B.reshape(self.n, -1).to(self.lu.dtype).clone().contiguous()
+        nrhs = X.shape[1]
+        wsb = lib.mba_lu_solve_workspace_bytes(self.prec, self.n, nrhs)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=X.device)
+        st = lib.mba_lu_solve(self.prec, _lib.ptr(self.lu), self.n, self.lu.stride(0),
+                              _lib.ptr(self.perm), _lib.ptr(X), nrhs, X.stride(0), _lib.ptr(ws), wsb,
+                              _lib.stream())
+        _lib.check(st, "mba_lu_solve")
+        return X.reshape(-1) if vec else X
+
+
+def residual_c128(Z: torch.Tensor, X: torch.Tensor, V: torch.Tensor) -> torch.Tensor:
+    """r = V - Z X in FP64 (complex128 Z, X, V)."""
+    lib = _lib.load()
+    n = Z.shape[0]
+    X2 = X.reshape(n, -1).contiguous()
+    V2 = V.reshape(n, -1).to(torch.complex128).contiguous()
+    R = torch.empty_like(V2)
+    st = lib.mba_residual_c128(_lib.ptr(Z), n, Z.stride(0), _lib.ptr(X2), _lib.ptr(V2), _lib.ptr(R),
+                               X2.shape[1], _lib.stream())
+    _lib.check(st, "mba_residual_c128")
+    return R.reshape(V.shape)
+
+
+def to_c64(Z: torch.Tensor) -> torch.Tensor:
+    lib = _lib.load()
+    m, n = Z.shape
+    out = torch.empty((m, n), dtype=torch.complex64, device=Z.device)
+    st = lib.mba_convert_c128_c64(_lib.ptr(Z), m, n, Z.stride(0), _lib.ptr(out), n, _lib.stream())
+    _lib.check(st, "mba_convert_c128_c64")
+    return out
+
+
+@dataclass
+class RefineInfo:
+    iterations: int
+    converged: bool
+    corrections: list
+    fallback: str = ""
+
+
+def refine_solve(Z64: torch.Tensor, V: torch.Tensor, lu32: Optional[LUFactor] = None,
+                 tol: float = 1e-8, maxit: int = 20, fallback_fp64: bool = True):
+    """Mixed-precision solve of Z64 J = V to FP64 accuracy.
+
+    J0 = LU32 \\ V; repeat r = V - Z64 J (FP64), d = LU32 \\ r, J += d until
+    ||d|| <= tol ||J||.  If the correction stops shrinking (|d_k| > 0.5 |d_{k-1}|
+    twice) or maxit is reached, falls back to an FP64 LU of Z64 when
+    ``fallback_fp64`` is set.  Returns (J complex128, RefineInfo)."""
+    if lu32 is None:
+        lu32 = LUFactor(to_c64(Z64), overwrite=True)
+    V128 = V.to(torch.complex128)
+    J = lu32.solve(V.to(torch.complex64)).to(torch.complex128)
+    corr = []
+    stalls = 0
+    for it in range(1, maxit + 1):
+        r = residual_c128(Z64, J, V128)
+        d = lu32.solve(r.to(torch.complex64)).to(torch.complex128)
+        J = J + d
+        rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(J))
+        corr.append(rel)
+        if rel <= tol:
+            return J, RefineInfo(it, True, corr)
+        if len(corr) >= 2 and corr[-1] > 0.5 * corr[-2]:
+            stalls += 1
+            if stalls >= 2:
+                break
+    if not fallback_fp64:
+        return J, RefineInfo(len(corr), False, corr)
+    J = LUFactor(Z64).solve(V128)
+    return J, RefineInfo(len(corr), False, corr, fallback="fp64-lu")

@@ -1,0 +1,461 @@
+"""CPU FP64 oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this module, and only as the checker.  The product path
+(mesh-ferrofluid_amd/) never imports it.
+
+The numeric kernels live in ``mom_oracle.c`` (built into ``liborc.so`` by the
+Makefile next to this file); this module wraps them with numpy and restates
+the host-side pieces of the reference in numpy:
+
+* face areas / normals / vertex areas   -- src/utils/mesh_utils.py:158-310
+* Laplace relation point tables        -- src/mesh_BEM/utils/integrations_3d.py:18-249,
+  fed by the rank-3 rule of src/mesh_BEM/single_layer/abstract_single_layer.py:49-63
+  and the tensorisation of integrations_3d.py:354-364
+* panel relations (brute force)        -- src/utils/mesh_utils.py:700-725
+* pack_matrix_tight                    -- src/utils/BEM_utils.py:314-357
+* Dirichlet RHS / solve                -- src/mesh_BEM/rhs_layer/rhs_layer_3d.py:74-203,
+  src/mesh_BEM/mesh_BEM_solver_3d.py:197-373
+* solve_MOM pipeline                   -- src/mesh_BEM/mesh_BEM_solver_3d.py:96-195
+* dense solves: LAPACK zgesv/dgesv through scipy (what torch.linalg.solve
+  calls on the CPU; mesh_BEM_solver_3d.py:174, 346).
+
+Parity status per function is listed in oracle/README.md.
+"""
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        D = ctypes.POINTER(ctypes.c_double)
+        I32 = ctypes.POINTER(ctypes.c_int32)
+        I64 = ctypes.POINTER(ctypes.c_int64)
+        U8 = ctypes.POINTER(ctypes.c_uint8)
+        i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        L.orc_ref_zmat_faceedge.argtypes = [D, I32, D, D, D, D, i32, D, D, i32, dbl, i64, D]
+        L.orc_ref_rhs_faceedge.argtypes = [D, I32, D, D, D, D, i32, dbl, i64, D]
+        L.orc_interp_faceedge.argtypes = [D, I32, I32, D, D, i64, i64, i64, D]
+        L.orc_tri_potentials.argtypes = [D, D, D, D, D, D, D]
+        L.orc_ns_zmat_faceedge.argtypes = [D, I32, D, D, D, i32, dbl, i64, I64, I32, D]
+        L.orc_ns_rhs_faceedge.argtypes = [D, I32, D, D, D, i32, dbl, i64, D, D, i64, D]
+        L.orc_gather_rwg.argtypes = [D, i64, I32, i64, D]
+        L.orc_gather_rhs.argtypes = [D, I32, i64, i64, D]
+        L.orc_laplace_layer.argtypes = [D, I32, D, D, i64, U8, ctypes.POINTER(D), I64, i32, i32, D]
+        L.orc_tri_S0_grad.argtypes = [D, D, D, D, D, D]
+        L.orc_laplace_p0.argtypes = [D, I32, i64, D, D, i32, I64, I32, D, D]
+        for f in (L.orc_ref_zmat_faceedge, L.orc_ref_rhs_faceedge, L.orc_interp_faceedge,
+                  L.orc_tri_potentials, L.orc_ns_zmat_faceedge, L.orc_ns_rhs_faceedge,
+                  L.orc_gather_rwg, L.orc_gather_rhs, L.orc_laplace_layer, L.orc_tri_S0_grad,
+                  L.orc_laplace_p0):
+            f.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _i64(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _fi32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ------------------------------------------------------------------ geometry
+def face_areas_normals(verts, faces):
+    """Areas [F] and unit normals [F,3]: normalise((v1-v0) x (v2-v1))."""
+    v = _f64(verts)
+    f = np.asarray(faces, dtype=np.int64)
+    v0, v1, v2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    c = np.cross(v1 - v0, v2 - v1)
+    nrm = np.linalg.norm(c, axis=1)
+    return 0.5 * nrm, c / nrm[:, None]
+
+
+def vert_areas(verts, faces):
+    a, _ = face_areas_normals(verts, faces)
+    va = np.zeros(len(verts))
+    np.add.at(va, np.asarray(faces, dtype=np.int64).reshape(-1), np.repeat(a / 3.0, 3))
+    return va
+
+
+def edges_bruteforce(faces, n_verts):
+    """Sorted-key edge table and face-edge map by a dictionary walk."""
+    keys = {}
+    f = np.asarray(faces, dtype=np.int64)
+    fe = np.zeros(3 * len(f), dtype=np.int64)
+    allk = sorted({max(f[i, (e + 1) % 3], f[i, (e + 2) % 3]) * n_verts
+                   + min(f[i, (e + 1) % 3], f[i, (e + 2) % 3])
+                   for i in range(len(f)) for e in range(3)})
+    for idx, k in enumerate(allk):
+        keys[k] = idx
+    for i in range(len(f)):
+        for e in range(3):
+            a, b = f[i, (e + 1) % 3], f[i, (e + 2) % 3]
+            fe[3 * i + e] = keys[max(a, b) * n_verts + min(a, b)]
+    edges = np.array([(k // n_verts, k % n_verts) for k in allk], dtype=np.int64)
+    return edges, fe
+
+
+def rwg_fe(fe_idx, E):
+    order = np.argsort(np.asarray(fe_idx, dtype=np.int64), kind="stable")
+    return order.reshape(E, 2).astype(np.int32)
+
+
+def gl_ref(N):
+    """Reference GL table on [0,1] in FP64 from float32-rounded nodes."""
+    x, w = np.polynomial.legendre.leggauss(N)
+    x = x.astype(np.float32).astype(np.float64)
+    w = w.astype(np.float32).astype(np.float64)
+    return (x + 1.0) / 2.0, w / 2.0
+
+
+def radon7():
+    s15 = math.sqrt(15.0)
+    pts = [(1 / 3, 1 / 3, 1 / 3)]
+    ws = [9 / 40]
+    for a, wa in (((6 - s15) / 21, (155 - s15) / 1200), ((6 + s15) / 21, (155 + s15) / 1200)):
+        b = 1 - 2 * a
+        pts += [(a, a, b), (a, b, a), (b, a, a)]
+        ws += [wa] * 3
+    return np.array(pts), 0.5 * np.array(ws)
+
+
+def near_pairs_bruteforce(verts, faces, eta=2.0):
+    """CSR over source faces b of test faces a with |o_a - o_b| < eta (rho_a + rho_b)."""
+    v = _f64(verts)
+    f = np.asarray(faces, dtype=np.int64)
+    P = v[f]  # [F,3,3]
+    o = P.mean(axis=1)
+    rho = np.linalg.norm(P - o[:, None, :], axis=2).max(axis=1)
+    D = np.linalg.norm(o[:, None, :] - o[None, :, :], axis=2)
+    near = D < eta * (rho[:, None] + rho[None, :])
+    ptr = np.zeros(len(f) + 1, dtype=np.int64)
+    idx = []
+    for b in range(len(f)):
+        a = np.nonzero(near[:, b])[0]
+        idx.append(a)
+        ptr[b + 1] = ptr[b] + a.size
+    return ptr, np.concatenate(idx).astype(np.int32)
+
+
+# ------------------------------------------------------------------ MoM, ref mode
+def ref_zmat_faceedge(verts, faces, gpx, gwx, gpy, gwy, k):
+    v, f = _f64(verts), _fi32(faces)
+    A, n = face_areas_normals(v, f)
+    gpx, gwx, gpy, gwy = map(_f64, (gpx, gwx, gpy, gwy))
+    F = len(f)
+    out = np.zeros((3 * F, 3 * F), dtype=np.complex128)
+    lib().orc_ref_zmat_faceedge(_d(v), _i32(f), _d(A), _d(_f64(n)), _d(gpx), _d(gwx), gpx.size,
+                                _d(gpy), _d(gwy), gpy.size, float(k), F, _d(out.view(np.float64)))
+    return out
+
+
+def ref_rhs_faceedge(verts, faces, gp, gw, k):
+    v, f = _f64(verts), _fi32(faces)
+    A, n = face_areas_normals(v, f)
+    gp, gw = _f64(gp), _f64(gw)
+    out = np.zeros((len(f), 3), dtype=np.complex128)
+    lib().orc_ref_rhs_faceedge(_d(v), _i32(f), _d(A), _d(_f64(n)), _d(gp), _d(gw), gp.size,
+                               float(k), len(f), _d(out.view(np.float64)))
+    return out
+
+
+def interp_faceedge(verts, faces, fe_idx, I):
+    """[B, 3F, 3] complex face-vertex currents from RWG coefficients I [B, E]."""
+    v, f, fe = _f64(verts), _fi32(faces), _fi32(fe_idx)
+    A, _ = face_areas_normals(v, f)
+    I = np.ascontiguousarray(np.atleast_2d(I), dtype=np.complex128)
+    B, E = I.shape
+    out = np.zeros((B, 3 * len(f), 3), dtype=np.complex128)
+    lib().orc_interp_faceedge(_d(v), _i32(f), _i32(fe), _d(A), _d(I.view(np.float64)), B, E,
+                              len(f), _d(out.view(np.float64)))
+    return out
+
+
+def vertex_sum(face_vert_J, faces, n_verts):
+    """vert_Js[b, v] = sum over (f, t) with faces[f][t] == v (mesh_BEM_solver_3d.py:186-192)."""
+    B = face_vert_J.shape[0]
+    out = np.zeros((B, n_verts, 3), dtype=face_vert_J.dtype)
+    idx = np.asarray(faces, dtype=np.int64).reshape(-1)
+    for b in range(B):
+        np.add.at(out[b], idx, face_vert_J[b])
+    return out
+
+
+def gather_rwg(ZF, rfe):
+    F3 = ZF.shape[0]
+    E = rfe.shape[0]
+    ZF = np.ascontiguousarray(ZF, dtype=np.complex128)
+    Z = np.zeros((E, E), dtype=np.complex128)
+    lib().orc_gather_rwg(_d(ZF.view(np.float64)), F3 // 3, _i32(_fi32(rfe)), E, _d(Z.view(np.float64)))
+    return Z
+
+
+def gather_rhs(VF, rfe):
+    VF = np.ascontiguousarray(VF.reshape(VF.shape[0], -1), dtype=np.complex128)
+    E = rfe.shape[0]
+    nd = VF.shape[1]
+    V = np.zeros((E, nd), dtype=np.complex128)
+    lib().orc_gather_rhs(_d(VF.view(np.float64)), _i32(_fi32(rfe)), E, nd, _d(V.view(np.float64)))
+    return V
+
+
+def ref_solve_MOM(verts, faces, gaussQR, k):
+    """FP64 restatement of MeshBEMSolver3d.solve_MOM (mesh_BEM_solver_3d.py:96-195).
+
+    Returns (vert_Js [1, V, 3], I [E], Z [E, E], V [E], edges, fe_idx)."""
+    v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
+    edges, fe = edges_bruteforce(f, len(v))
+    E = len(edges)
+    gpx, gwx = gl_ref(gaussQR)
+    gpy, gwy = gl_ref(gaussQR - 1)
+    ZF = ref_zmat_faceedge(v, f, gpx, gwx, gpy, gwy, k)
+    rfe = rwg_fe(fe, E)
+    Z = gather_rwg(ZF, rfe)
+    VF = ref_rhs_faceedge(v, f, gpx, gwx, k)
+    V = gather_rhs(VF.reshape(-1, 1), rfe)[:, 0]
+    import scipy.linalg
+
+    Icoef = scipy.linalg.solve(Z, V)
+    fvj = interp_faceedge(v, f, fe, Icoef[None, :])
+    vj = vertex_sum(fvj, f, len(v))
+    return vj, Icoef, Z, V, edges, fe
+
+
+# ------------------------------------------------------------------ MoM, ns mode
+def tri_potentials(P0, P1, P2, r):
+    S0 = np.zeros(1)
+    S1 = np.zeros(3)
+    rho = np.zeros(3)
+    lib().orc_tri_potentials(_d(_f64(P0)), _d(_f64(P1)), _d(_f64(P2)), _d(_f64(r)), _d(S0), _d(S1), _d(rho))
+    return S0[0], S1, rho
+
+
+def ns_zmat_faceedge(verts, faces, k, near_ptr, near_idx, rule=None):
+    v, f = _f64(verts), _fi32(faces)
+    A, _ = face_areas_normals(v, f)
+    lam, w = radon7() if rule is None else rule
+    lam, w = _f64(lam), _f64(w)
+    F = len(f)
+    out = np.zeros((3 * F, 3 * F), dtype=np.complex128)
+    lib().orc_ns_zmat_faceedge(_d(v), _i32(f), _d(A), _d(lam), _d(w), w.size, float(k), F,
+                               _i64(np.ascontiguousarray(near_ptr, dtype=np.int64)),
+                               _i32(_fi32(near_idx)), _d(out.view(np.float64)))
+    return out
+
+
+def ns_rhs_faceedge(verts, faces, k, dirs, pols, rule=None):
+    v, f = _f64(verts), _fi32(faces)
+    A, _ = face_areas_normals(v, f)
+    lam, w = radon7() if rule is None else rule
+    dirs = _f64(np.atleast_2d(dirs))
+    pols = _f64(np.atleast_2d(pols))
+    out = np.zeros((3 * len(f), dirs.shape[0]), dtype=np.complex128)
+    lib().orc_ns_rhs_faceedge(_d(v), _i32(f), _d(A), _d(_f64(lam)), _d(_f64(w)), w.size, float(k),
+                              len(f), _d(dirs), _d(pols), dirs.shape[0], _d(out.view(np.float64)))
+    return out
+
+
+def ns_system(verts, faces, k, dirs, pols, eta=2.0):
+    """FP64 EFIE system (Z [E,E], V [E, ndir]) in the sorted-key edge numbering."""
+    v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
+    edges, fe = edges_bruteforce(f, len(v))
+    E = len(edges)
+    ptr, idx = near_pairs_bruteforce(v, f, eta)
+    ZF = ns_zmat_faceedge(v, f, k, ptr, idx)
+    rfe = rwg_fe(fe, E)
+    Z = gather_rwg(ZF, rfe)
+    V = gather_rhs(ns_rhs_faceedge(v, f, k, dirs, pols), rfe)
+    return Z, V, edges, fe
+
+
+# ------------------------------------------------------------------ Laplace, ref mode
+def rank3_rule(gp, gw):
+    """(r1, r2, w) of abstract_single_layer.py:49-63: index a*Q + b -> (gp[b], gp[a] gp[b]), gw[a] gw[b]."""
+    Q = gp.size
+    r1 = np.tile(gp[None, :], (Q, 1)).reshape(-1)
+    r2 = (gp[:, None] * gp[None, :]).reshape(-1)
+    w = (gw[None, :] * gw[:, None]).reshape(-1)
+    return r1, r2, w
+
+
+def laplace_tables(gp, gw):
+    """Point tables per relation type (0..3): rows (r1x, r2x, r1y, r2y, weight*jacobian),
+    fed with the rank-3 rule of abstract_single_layer.py:49-63 (the reference's input)."""
+    r1, r2, w = rank3_rule(_f64(gp), _f64(gw))
+    return laplace_tables_from(r1, r2, w)
+
+
+def laplace_tables_from(r1, r2, w):
+    """Tables from a 2-D rule (r1, r2, w): integrations_3d.py:354-364 tensorises
+    it into (xsi, eta1) x (eta2, eta3); get_rands_and_jacobian_3d (:18-249)
+    maps the 4-D points per relation type."""
+    r1, r2, w = _f64(r1), _f64(r2), _f64(w)
+    Q2 = r1.size
+    xsi = np.tile(r1[None, :], (Q2, 1)).reshape(-1)
+    eta1 = np.tile(r2[None, :], (Q2, 1)).reshape(-1)
+    eta2 = np.tile(r1[:, None], (1, Q2)).reshape(-1)
+    eta3 = np.tile(r2[:, None], (1, Q2)).reshape(-1)
+    wyx = (np.tile(w[:, None], (1, Q2)) * np.tile(w[None, :], (Q2, 1))).reshape(-1)
+
+    maps = {}
+    maps[0] = ([(xsi, eta1 * xsi, eta2, eta2 * eta3)], [xsi * eta2])
+    j1 = xsi ** 3 * eta2
+    maps[1] = ([(xsi, xsi * eta1, xsi * eta2, xsi * eta2 * eta3),
+                (xsi * eta2, xsi * eta2 * eta3, xsi, xsi * eta1)], [j1, j1])
+    ws = [
+        (xsi, -xsi * eta1 * eta2, xsi * eta1 * (1 - eta2), xsi * eta1 * eta3),
+        (xsi, -xsi * eta1 * eta2 * eta3, xsi * eta1 * eta2 * (1 - eta3), xsi * eta1),
+        (xsi * (1 - eta1 * eta2), xsi * eta1 * eta2, xsi * eta1 * eta2 * eta3, xsi * eta1 * (1 - eta2)),
+        (xsi * (1 - eta1 * eta2 * eta3), xsi * eta1 * eta2 * eta3, xsi * eta1, xsi * eta1 * eta2 * (1 - eta3)),
+        (xsi * (1 - eta1 * eta2 * eta3), xsi * eta1 * eta2 * eta3, xsi * eta1 * eta2, xsi * eta1 * (1 - eta2 * eta3)),
+    ]
+    je = [xsi ** 3 * eta1 ** 2] + [xsi ** 3 * eta1 ** 2 * eta2] * 4
+    maps[2] = ([(a, d, a + b, c) for (a, b, c, d) in ws], je)
+    M = np.array([
+        [[1, 0, 0, 0], [1, -1, 1, 0], [0, 0, 0, 1], [0, 0, 1, 0]],
+        [[1, 0, 0, 0], [0, 1, -1, 1], [0, 0, 1, 0], [0, 0, 0, 1]],
+        [[1, 0, 0, -1], [0, 1, 0, -1], [0, 0, 0, -1], [0, 0, 1, -1]],
+    ], dtype=np.float64)
+    wv = np.stack([xsi, xsi * eta1, xsi * eta1 * eta2, xsi * eta1 * eta2 * eta3], axis=0)
+    sf = []
+    for d in range(3):
+        xz = M[d] @ wv
+        sf.append((xz[0], xz[1], xz[0] - xz[2], xz[1] - xz[3]))
+    js = xsi ** 3 * eta1 ** 2 * eta2
+    maps[3] = (sf, [js] * 3)
+    tabs = []
+    for r in range(4):
+        pts, jacs = maps[r]
+        rows = []
+        for (a, b, c, d), jj in zip(pts, jacs):
+            rows.append(np.stack([a, b, c, d, wyx * jj], axis=1))
+        tabs.append(np.ascontiguousarray(np.concatenate(rows, axis=0)))
+    return tabs
+
+
+def panel_relation_bruteforce(faces):
+    f = np.asarray(faces, dtype=np.int64)
+    return (f[:, None, :, None] == f[None, :, None, :]).sum(axis=(2, 3)).astype(np.uint8)
+
+
+def laplace_layer(verts, faces, gp, gw, layer, order_type, rel=None, tabs=None):
+    """Unpacked [F, F, nphi, nphi] single (layer=1) or double (layer=2) layer."""
+    v, f = _f64(verts), _fi32(faces)
+    A, n = face_areas_normals(v, f)
+    F = len(f)
+    if rel is None:
+        rel = panel_relation_bruteforce(f)
+    rel = np.ascontiguousarray(rel, dtype=np.uint8)
+    if tabs is None:
+        tabs = laplace_tables(gp, gw)
+    nphi = 3 if order_type == 1 else 1
+    P = ctypes.POINTER(ctypes.c_double)
+    tp = (P * 4)(*[_d(t) for t in tabs])
+    nt = np.array([t.shape[0] for t in tabs], dtype=np.int64)
+    out = np.zeros((F, F, nphi, nphi))
+    lib().orc_laplace_layer(_d(v), _i32(f), _d(A), _d(_f64(n)), F,
+                            rel.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), tp, _i64(nt),
+                            int(layer), nphi, _d(out))
+    return out
+
+
+def pack_matrix_tight(order_type, U, faces, n_verts):
+    """[Fy, Fx, nphi, nphi] -> [V, V] with A[vert(fy,iy), vert(fx,ix)] += U (BEM_utils.py:314-357)."""
+    f = np.asarray(faces, dtype=np.int64)
+    F = len(f)
+    if order_type == 0:
+        U = np.repeat(np.repeat(U.reshape(F, F, 1, 1), 3, axis=2), 3, axis=3) / 9.0
+    src = U.transpose(0, 2, 1, 3).reshape(-1)
+    rows = np.repeat(f.reshape(-1), 3 * F)
+    cols = np.tile(f.reshape(-1), 3 * F)
+    out = np.zeros(n_verts * n_verts, dtype=U.dtype)
+    np.add.at(out, rows * n_verts + cols, src)
+    return out.reshape(n_verts, n_verts)
+
+
+def ref_laplace_dirichlet(verts, faces, g, gaussQR, order_type=1):
+    """FP64 restatement of MeshBEMSolver3d.solve for an all-Dirichlet batch of one.
+
+    g: [V] Dirichlet data.  Returns Neumann data t [V] solving V t = 1/2 M g + K^T g."""
+    v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
+    nV = len(v)
+    gp, gw = gl_ref(gaussQR)
+    rel = panel_relation_bruteforce(f)
+    tabs = laplace_tables(gp, gw)
+    Vu = laplace_layer(v, f, gp, gw, 1, order_type, rel, tabs)
+    Ku = laplace_layer(v, f, gp, gw, 2, order_type, rel, tabs)
+    Vt = pack_matrix_tight(order_type, Vu, f, nV)
+    if order_type == 0:
+        Ku = np.repeat(np.repeat(Ku.reshape(len(f), len(f), 1, 1), 3, axis=2), 3, axis=3) / 9.0
+    gf = np.asarray(g)[f]  # [F, 3] = g at (fy, iy)
+    src = np.einsum("yxij,yi->xj", Ku, gf)  # sum over fy, iy
+    gk2 = np.zeros(nV)
+    np.add.at(gk2, f.reshape(-1), src.reshape(-1))
+    rhs = 0.5 * np.asarray(g) * vert_areas(v, f) + gk2
+    import scipy.linalg
+
+    return scipy.linalg.solve(Vt, rhs), Vt, rhs
+
+
+# ------------------------------------------------------------------ Laplace, ns mode (P0)
+def tri_S0_grad(P0, P1, P2, r):
+    S0 = np.zeros(1)
+    G = np.zeros(3)
+    lib().orc_tri_S0_grad(_d(_f64(P0)), _d(_f64(P1)), _d(_f64(P2)), _d(_f64(r)), _d(S0), _d(G))
+    return S0[0], G
+
+
+def laplace_p0(verts, faces, near_ptr, near_idx, rule=None):
+    """(V, Kt) [F, F] P0 Galerkin single layer and adjoint double layer."""
+    v, f = _f64(verts), _fi32(faces)
+    lam, w = radon7() if rule is None else rule
+    F = len(f)
+    Vm = np.zeros((F, F))
+    Km = np.zeros((F, F))
+    lib().orc_laplace_p0(_d(v), _i32(f), F, _d(_f64(lam)), _d(_f64(w)), len(w),
+                         _i64(np.ascontiguousarray(near_ptr, dtype=np.int64)), _i32(_fi32(near_idx)),
+                         _d(Vm), _d(Km))
+    return Vm, Km
+
+
+def demag_factors(a, b, c):
+    """Demagnetising factors (N_a, N_b, N_c) of an ellipsoid with semi-axes a, b, c
+    (Osborn 1945): N_i = (abc/2) int_0^inf ds / ((s + i^2) sqrt((s+a^2)(s+b^2)(s+c^2)))."""
+    from scipy.integrate import quad
+
+    def N(i2):
+        g = lambda s: 1.0 / ((s + i2) * math.sqrt((s + a * a) * (s + b * b) * (s + c * c)))
+        return 0.5 * a * b * c * quad(g, 0, np.inf, limit=200)[0]
+
+    return N(a * a), N(b * b), N(c * c)

@@ -1,0 +1,132 @@
+"""GPU tests of the dense kernels: MFMA complex GEMM, LU + solve, FP64 residual,
+mixed-precision refinement.
+
+References: GEMM against an FP64 CPU product of the same inputs; LU against
+LAPACK (scipy zgesv) and the backward error ||A x - b|| / (||A|| ||x||).
+"""
+
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+from mesh_bem_amd import _lib
+from mesh_bem_amd.dense import LUFactor, gemm, refine_solve, residual_c128, to_c64
+
+pytestmark = pytest.mark.gpu
+
+
+def crand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.complex64, 2e-6), (torch.complex128, 1e-14)])
+@pytest.mark.parametrize("M,N,K", [(128, 64, 16), (200, 77, 33), (1, 5, 3), (513, 130, 128)])
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+def test_gemm(dev, dtype, tol, M, N, K, beta):
+    rng = np.random.default_rng(M * 7 + N + K)
+    a, b, c = crand(rng, M, K), crand(rng, K, N), crand(rng, M, N)
+    A = torch.tensor(a, dtype=dtype, device=dev)
+    B = torch.tensor(b, dtype=dtype, device=dev)
+    C = torch.tensor(c, dtype=dtype, device=dev)
+    out = gemm(A, B, C, alpha=-1.0, beta=beta).cpu().numpy()
+    ref = -(A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128)) + \
+        beta * C.cpu().numpy().astype(np.complex128) if beta else \
+        -(A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128))
+    scale = np.abs(a).max() * np.abs(b).max() * K + beta * np.abs(c).max()
+    assert np.abs(out - ref).max() / scale < tol
+
+
+def test_gemm_strided_views(dev):
+    """Sub-matrix views with a leading dimension larger than the width."""
+    rng = np.random.default_rng(5)
+    big = torch.tensor(crand(rng, 300, 300), dtype=torch.complex64, device=dev)
+    A = big[10:140, 3:70]
+    B = big[100:167, 150:251]
+    C = torch.zeros((130, 101), dtype=torch.complex64, device=dev)
+    gemm(A, B, C, 1.0, 0.0)
+    ref = A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128)
+    assert np.abs(C.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (7, 2), (129, 1), (300, 3), (1000, 40)])
+def test_lu_solve_matches_lapack(dev, dtype, n, nrhs):
+    print(f"lu n={n} nrhs={nrhs} dtype={dtype}", flush=True)
+    rng = np.random.default_rng(n)
+    a = crand(rng, n, n)
+    b = crand(rng, n, nrhs)
+    A = torch.tensor(a, dtype=dtype, device=dev)
+    f = LUFactor(A)
+    x = f.solve(torch.tensor(b, dtype=dtype, device=dev)).cpu().numpy()
+    a64 = A.cpu().numpy().astype(np.complex128)
+    ref = scipy.linalg.solve(a64, b)
+    eps = 6e-8 if dtype == torch.complex64 else 1.2e-16
+    bwd = np.linalg.norm(a64 @ x - b) / (np.linalg.norm(a64) * np.linalg.norm(x))
+    assert bwd < 50 * eps * np.sqrt(n)
+    cond = np.linalg.cond(a64)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 50 * eps * cond * np.sqrt(n)
+
+
+def test_lu_pivoting_needed(dev):
+    """A matrix with a zero leading entry only factors with row interchanges."""
+    a = np.array([[0, 1, 2], [3, 4, 5], [6, 7, 9]], dtype=np.complex128)
+    f = LUFactor(torch.tensor(a, device=dev))
+    x = f.solve(torch.ones(3, dtype=torch.complex128, device=dev)).cpu().numpy()
+    assert np.allclose(a @ x, np.ones(3), atol=1e-13)
+    assert f.ipiv[0].item() == 2
+
+
+def test_lu_multi_panel_large(dev):
+    """n spanning many panels (and a ragged last panel)."""
+    n = 2500
+    rng = np.random.default_rng(1)
+    a = crand(rng, n, n)
+    A = torch.tensor(a, dtype=torch.complex64, device=dev)
+    f = LUFactor(A)
+    b = crand(rng, n, 1)
+    x = f.solve(torch.tensor(b, dtype=torch.complex64, device=dev)).cpu().numpy()
+    a64 = A.cpu().numpy().astype(np.complex128)
+    bwd = np.linalg.norm(a64 @ x - b) / (np.linalg.norm(a64) * np.linalg.norm(x))
+    assert bwd < 1e-5
+
+
+def test_residual_c128(dev):
+    rng = np.random.default_rng(2)
+    z, x, v = crand(rng, 333, 333), crand(rng, 333, 2), crand(rng, 333, 2)
+    Z = torch.tensor(z, device=dev)
+    r = residual_c128(Z, torch.tensor(x, device=dev), torch.tensor(v, device=dev)).cpu().numpy()
+    assert np.abs(r - (v - z @ x)).max() < 1e-12 * np.abs(z @ x).max()
+
+
+def test_refinement_reaches_fp64(dev):
+    """FP32 factor + FP64 residual converges to the FP64 LAPACK solution."""
+    n = 800
+    rng = np.random.default_rng(3)
+    a = crand(rng, n, n) + 3 * np.sqrt(n) * np.eye(n)
+    b = crand(rng, n)
+    Z = torch.tensor(a, device=dev)
+    J, info = refine_solve(Z, torch.tensor(b, device=dev), tol=1e-12)
+    ref = scipy.linalg.solve(a, b)
+    assert info.converged and not info.fallback
+    assert np.linalg.norm(J.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-11
+
+
+def test_refinement_fallback_on_ill_conditioning(dev):
+    """An FP32-singular system cannot refine; the solve falls back to FP64 LU."""
+    n = 200
+    rng = np.random.default_rng(4)
+    U, _ = np.linalg.qr(crand(rng, n, n))
+    s = np.logspace(0, -10, n)
+    a = (U * s) @ U.conj().T
+    b = crand(rng, n)
+    J, info = refine_solve(torch.tensor(a, device=dev), torch.tensor(b, device=dev), tol=1e-12)
+    assert info.fallback == "fp64-lu"
+    ref = scipy.linalg.solve(a, b)
+    assert np.linalg.norm(a @ J.cpu().numpy() - b) / np.linalg.norm(b) < 1e-5
+    assert np.isfinite(ref).all()
+
+
+def test_to_c64(dev):
+    z = torch.randn(5, 7, dtype=torch.complex128, device=dev)
+    assert torch.equal(to_c64(z), z.to(torch.complex64))

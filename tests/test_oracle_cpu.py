@@ -1,0 +1,343 @@
+"""CPU tests that pin the FP64 oracle and the host-side logic.
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign,
+swapped operand) fails: quadrature exactness, analytic triangle integrals
+against adaptive quadrature, finite differences for the gradient, the
+reference's own float32 rounding of Gauss nodes, mesh invariants, and
+physics (Mie backscatter of a PEC sphere, the demagnetising field of a
+uniformly magnetised sphere / ellipsoid).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+from scipy import integrate
+
+import oracle as O
+from mesh_bem_amd import quadrature as Qd
+from mesh_bem_amd.geometry import (compute_face_areas, compute_face_normals,
+                                   compute_panel_relation, compute_vert_areas, geodesic_sphere,
+                                   get_edge_unique, icosphere, morton_order, noisy_ellipsoid,
+                                   rwg_face_edges, star_mesh)
+from mesh_bem_amd.mom import near_pairs, vertex_incidence
+
+
+# ------------------------------------------------------------------ quadrature
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8, 12])
+def test_gauss_table_reference_semantics(N):
+    p, w = Qd.get_gaussion_integration_points_and_weights(N, dtype=torch.float64)
+    assert abs(w.sum().item() - 1.0) < 1e-6
+    # exact for polynomials of degree <= 2N-1 (to float32 node rounding)
+    for d in range(2 * N):
+        assert abs((w * p ** d).sum().item() - 1.0 / (d + 1)) < 5e-7
+    # nodes are float32-rounded, then mapped in the target dtype
+    x, _ = np.polynomial.legendre.leggauss(N)
+    assert np.array_equal(p.numpy(), (x.astype(np.float32).astype(np.float64) + 1.0) / 2.0)
+    p32, _ = Qd.get_gaussion_integration_points_and_weights(N, dtype=torch.float32)
+    assert p32.dtype == torch.float32
+
+
+@pytest.mark.parametrize("N", [0, 9, 10, 11, 13])
+def test_gauss_table_unsupported_orders_raise(N):
+    with pytest.raises(NotImplementedError):
+        Qd.get_gaussion_integration_points_and_weights(N)
+
+
+def test_radon7_exact_to_degree_5():
+    lam, w = Qd.radon7()
+    assert abs(w.sum() - 0.5) < 1e-15
+    x, y = lam[:, 1], lam[:, 2]  # unit triangle (0,0), (1,0), (0,1)
+    for a in range(6):
+        for b in range(6 - a):
+            assert abs((w * x ** a * y ** b).sum() - Qd.triangle_monomial_integral(a, b)) < 1e-15
+    # degree 6 is not exact (so the test would notice a weaker rule)
+    assert abs((w * x ** 6).sum() - Qd.triangle_monomial_integral(6, 0)) > 1e-6
+
+
+def test_collapsed_rule_area_and_centroid():
+    gp, gw = O.gl_ref(4)
+    lam, w = Qd.collapsed_gl_rule(gp, gw)
+    assert abs(w.sum() - 0.5) < 1e-7
+    assert np.allclose((w[:, None] * lam).sum(0) / w.sum(), 1 / 3, atol=1e-7)
+
+
+# ------------------------------------------------------------------ geometry
+@pytest.mark.parametrize("sub", [0, 1, 2, 3])
+def test_icosphere_counts_and_topology(sub):
+    V, F = icosphere(sub)
+    assert V.shape[0] == 10 * 4 ** sub + 2 and F.shape[0] == 20 * 4 ** sub
+    e, fe, adj = get_edge_unique(torch.from_numpy(V), torch.from_numpy(F).int())
+    assert e.shape[0] == 30 * 4 ** sub
+    assert V.shape[0] - e.shape[0] + F.shape[0] == 2  # Euler characteristic
+    rfe = rwg_face_edges(fe, e.shape[0])
+    # the two faces of an edge traverse it in opposite directions
+    f = F
+    for m in range(0, e.shape[0], max(1, e.shape[0] // 50)):
+        (s0, s1) = rfe[m]
+        a0 = (f[s0 // 3, (s0 % 3 + 1) % 3], f[s0 // 3, (s0 % 3 + 2) % 3])
+        a1 = (f[s1 // 3, (s1 % 3 + 1) % 3], f[s1 // 3, (s1 % 3 + 2) % 3])
+        assert a0 == a1[::-1]
+    # outward normals, total area -> 4 pi
+    n = compute_face_normals(torch.from_numpy(V), torch.from_numpy(F))
+    c = torch.from_numpy(V[F].mean(axis=1))
+    assert ((n * c).sum(1) > 0).all()
+    area = compute_face_areas(torch.from_numpy(V), torch.from_numpy(F)).sum().item()
+    assert area < 4 * math.pi and (sub < 2 or area > 0.98 * 4 * math.pi)
+
+
+def test_edges_match_bruteforce_and_adjacency():
+    V, F = icosphere(2)
+    e, fe, adj = get_edge_unique(torch.from_numpy(V), torch.from_numpy(F).int())
+    eb, feb = O.edges_bruteforce(F, V.shape[0])
+    assert np.array_equal(e.numpy(), eb) and np.array_equal(fe.numpy(), feb)
+    for m in range(e.shape[0]):
+        f0, f1 = adj[m].tolist()
+        assert f0 < f1 and {3 * f0, 3 * f0 + 1, 3 * f0 + 2} & set(np.nonzero(feb == m)[0])
+
+
+def test_geodesic_and_generated_meshes():
+    for n in (2, 5, 7):
+        V, F = geodesic_sphere(n)
+        assert F.shape[0] == 20 * n * n and V.shape[0] == 10 * n * n + 2
+    V, F, A = star_mesh(0, 5)
+    assert 0 <= A <= 0.3
+    V2, F2, A2 = star_mesh(0, 5)
+    assert np.array_equal(V, V2) and A == A2  # seeded
+    V, F, c = noisy_ellipsoid(1, 4)
+    assert 1 <= c <= 2
+
+
+def test_non_manifold_mesh_rejected():
+    F = np.array([[0, 1, 2], [0, 2, 3], [0, 3, 1]])  # open fan: boundary edges
+    e, fe, _ = get_edge_unique(torch.zeros(4, 3), torch.from_numpy(F).int())
+    with pytest.raises(ValueError):
+        rwg_face_edges(fe, e.shape[0])
+
+
+def test_panel_relation_matches_bruteforce():
+    V, F = icosphere(1)
+    rel = compute_panel_relation(torch.from_numpy(F)).numpy()
+    assert np.array_equal(rel, O.panel_relation_bruteforce(F))
+    assert (np.diag(rel) == 3).all()
+
+
+def test_near_pairs_match_bruteforce():
+    V, F, _ = star_mesh(2, 5)
+    p1, i1 = near_pairs(V, F)
+    p2, i2 = O.near_pairs_bruteforce(V, F)
+    assert np.array_equal(p1, p2) and np.array_equal(i1, i2)
+    # every vertex-sharing pair is near
+    rel = O.panel_relation_bruteforce(F)
+    for b in range(len(F)):
+        assert set(np.nonzero(rel[:, b])[0]) <= set(i1[p1[b]:p1[b + 1]])
+
+
+def test_vertex_incidence_and_areas():
+    V, F = icosphere(1)
+    ptr, idx = vertex_incidence(F, V.shape[0])
+    assert ptr[-1] == F.size
+    for v in range(V.shape[0]):
+        assert all(F[s // 3, s % 3] == v for s in idx[ptr[v]:ptr[v + 1]])
+    va = compute_vert_areas(torch.from_numpy(V), torch.from_numpy(F), keepdim=False).numpy()
+    assert np.allclose(va, O.vert_areas(V, F))
+
+
+def test_morton_order_is_permutation():
+    V, F = icosphere(2)
+    p = morton_order(V)
+    assert np.array_equal(np.sort(p), np.arange(V.shape[0]))
+
+
+# ------------------------------------------------------------------ analytic integrals
+TRI = (np.array([0.0, 0.0, 0.0]), np.array([1.0, 0.0, 0.0]), np.array([0.3, 0.8, 0.0]))
+
+
+def _brute(f, r):
+    """Adaptive quadrature split at the projection of r into three triangles with
+    apex there (signed by orientation), each mapped y = c + s (P_i - c) + s t (P_i+1 - P_i):
+    the Jacobian s cancels the 1/R singularity of an in-plane point."""
+    P = TRI
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    n /= np.linalg.norm(n)
+    c = r - n * (n @ (r - P[0]))
+    tot = 0.0
+    for i in range(3):
+        a, b = P[i], P[(i + 1) % 3]
+        J = np.cross(a - c, b - a) @ n  # signed
+        if abs(J) < 1e-15:
+            continue
+        g = lambda t, s: J * s * f(c + s * (a - c) + s * t * (b - a))
+        tot += integrate.dblquad(g, 0, 1, 0, 1, epsabs=1e-13, epsrel=1e-12)[0]
+    return tot
+
+
+@pytest.mark.parametrize("r", [(0.4, 0.3, 0.0), (0.4, 0.3, 0.2), (1.5, 1.0, -0.3), (0.5, 0.0, 1e-4),
+                               (-0.2, 0.1, 0.0)])
+def test_tri_potentials_against_adaptive_quadrature(r):
+    r = np.array(r)
+    S0, S1, rho = O.tri_potentials(*TRI, r)
+    assert abs(S0 - _brute(lambda y: 1 / np.linalg.norm(r - y), r)) < 1e-9
+    for c in range(2):
+        ref = _brute(lambda y: (y - rho)[c] / np.linalg.norm(r - y), r)
+        assert abs(S1[c] - ref) < 1e-9
+
+
+@pytest.mark.parametrize("r", [(0.4, 0.3, 0.2), (1.5, 1.0, -0.3), (0.4, 0.3, -0.05), (2.0, -1.0, 0.0)])
+def test_tri_gradient_by_finite_differences(r):
+    r = np.array(r)
+    S0, G = O.tri_S0_grad(*TRI, r)
+    assert abs(S0 - O.tri_potentials(*TRI, r)[0]) < 1e-14
+    h = 1e-6
+    fd = [(O.tri_S0_grad(*TRI, r + h * e)[0] - O.tri_S0_grad(*TRI, r - h * e)[0]) / (2 * h)
+          for e in np.eye(3)]
+    assert np.abs(G - fd).max() < 1e-8
+
+
+# ------------------------------------------------------------------ MoM oracle
+def test_ns_far_pairs_symmetric_near_pairs_close():
+    """EFIE Galerkin with one rule on both sides: far pairs exactly symmetric."""
+    V, F = icosphere(1)
+    ptr, idx = O.near_pairs_bruteforce(V, F)
+    Z = O.ns_zmat_faceedge(V, F, 2.0, ptr, idx)
+    nearm = np.zeros((len(F), len(F)), bool)
+    for b in range(len(F)):
+        nearm[idx[ptr[b]:ptr[b + 1]], b] = True
+    far = ~np.kron(nearm | nearm.T, np.ones((3, 3), bool))
+    assert np.abs((Z - Z.T)[far]).max() < 1e-14 * np.abs(Z).max()
+    assert np.abs(Z - Z.T).max() < 2e-3 * np.abs(Z).max()
+
+
+def test_ns_near_split_is_continuous():
+    """Moving one far pair into the near set (singular split) changes Z only by
+    the quadrature error of the 1/R part: the split itself adds no term."""
+    V, F = icosphere(2)
+    ptr, idx = O.near_pairs_bruteforce(V, F, eta=2.0)
+    ptr3, idx3 = O.near_pairs_bruteforce(V, F, eta=3.0)
+    Z2 = O.ns_zmat_faceedge(V, F, 1.5, ptr, idx)
+    Z3 = O.ns_zmat_faceedge(V, F, 1.5, ptr3, idx3)
+    assert np.abs(Z2 - Z3).max() < 1e-3 * np.abs(Z2).max()
+
+
+def mie_backscatter(ka, nmax=None):
+    """Monostatic RCS / (pi a^2) of a PEC sphere (Mie series)."""
+    from scipy.special import spherical_jn, spherical_yn
+
+    nmax = nmax or int(ka + 4 * ka ** (1 / 3) + 10)
+    s = 0
+    for n in range(1, nmax + 1):
+        jn, jp = spherical_jn(n, ka), spherical_jn(n, ka, derivative=True)
+        hn = jn + 1j * spherical_yn(n, ka)
+        hp = jp + 1j * spherical_yn(n, ka, derivative=True)
+        an = jn / hn
+        bn = (jn + ka * jp) / (hn + ka * hp)
+        s += (-1) ** n * (2 * n + 1) * (bn - an)
+    return abs(s) ** 2 / ka ** 2
+
+
+def test_mie_backscatter_limits():
+    assert abs(mie_backscatter(60.0) - 1.0) < 0.1  # optical limit
+    ka = 0.05
+    assert abs(mie_backscatter(ka) / (9 * ka ** 4) - 1) < 0.01  # Rayleigh
+
+
+def far_field_rcs(V, F, fe, I, k, rhat, pol):
+    """sigma / (pi a^2) for unit sphere from RWG coefficients (eta = 1, |E0| = 1)."""
+    lam, w = O.radon7()
+    A, _ = O.face_areas_normals(V, F)
+    N = np.zeros(3, dtype=complex)
+    for f in range(len(F)):
+        P = V[F[f]]
+        for e in range(3):
+            i1, i2 = F[f][(e + 1) % 3], F[f][(e + 2) % 3]
+            c = np.linalg.norm(V[i2] - V[i1]) / (2 * A[f]) * (1 if i1 < i2 else -1)
+            for q in range(len(w)):
+                x = lam[q] @ P
+                N += I[fe[3 * f + e]] * w[q] * 2 * A[f] * c * (x - P[e]) * np.exp(1j * k * rhat @ x)
+    Et = N - rhat * (rhat @ N)
+    # sigma = k^2 |N_t|^2 / (4 pi) for E_s = -jk e^{-jkr}/(4 pi r) N_t; divided by pi a^2
+    return k ** 2 * np.linalg.norm(Et) ** 2 / (4 * np.pi ** 2)
+
+
+@pytest.mark.parametrize("ka", [1.0, 2.0])
+def test_ns_efie_sphere_backscatter_matches_mie(ka):
+    V, F = icosphere(3)
+    Z, Vr, edges, fe = O.ns_system(V, F, ka, [[0, 0, 1.0]], [[1.0, 0, 0]])
+    I = np.linalg.solve(Z, Vr[:, 0])
+    s = far_field_rcs(V, F, fe, I, ka, np.array([0, 0, -1.0]), None)
+    assert abs(s / mie_backscatter(ka) - 1) < 0.05
+
+
+def test_ref_cfie_sphere_backscatter_matches_mie():
+    """The reference's CFIE (Q = 4 / 3, incidence along (cos30, sin30, 0), E = z)."""
+    V, F = icosphere(2)
+    k = 1.0
+    vj, I, Z, Vr, edges, fe = O.ref_solve_MOM(V, F, 4, k)
+    kh = np.array([math.cos(math.pi / 6), math.sin(math.pi / 6), 0.0])
+    s = far_field_rcs(V, F, fe, I, k, -kh, None)
+    assert abs(s / mie_backscatter(k) - 1) < 0.15
+
+
+# ------------------------------------------------------------------ Laplace oracle
+def test_laplace_maps_integrate_unity_with_proper_rule():
+    """With a genuine 4-D tensor Gauss rule the separate, vertex and edge maps
+    cover T x T: sum of weight * jacobian = |T_ref|^2 = 1/4.  The reference's
+    same-face maps are three of Sauter-Schwab's six sub-domains without the
+    symmetry factor 2, so they cover half (1/8).  The reference's own
+    double-collapsed input changes the edge and face sums again."""
+    gp, gw = O.gl_ref(5)
+    # feed proper independent GL nodes: xsi, eta1 from a 2-D tensor rule
+    Q = gp.size
+    r1 = np.repeat(gp, Q)
+    r2 = np.tile(gp, Q)
+    w2 = np.repeat(gw, Q) * np.tile(gw, Q)
+    tabs = O.laplace_tables_from(r1, r2, w2)
+    for r in range(3):
+        assert abs(tabs[r][:, 4].sum() - 0.25) < 1e-6
+    assert abs(tabs[3][:, 4].sum() - 0.125) < 1e-6
+    tabq = O.laplace_tables(gp, gw)
+    assert abs(tabq[0][:, 4].sum() - 0.25) < 1e-6
+    assert abs(tabq[1][:, 4].sum() - 0.25) < 1e-6
+    assert abs(tabq[2][:, 4].sum() - 1 / 6) < 1e-6  # int xsi^3 eta1^2 ... with eta1 = g_a g_b
+    assert abs(tabq[3][:, 4].sum() - 1 / 12) < 1e-6
+
+
+def test_laplace_ref_layers_constant_density():
+    """On the unit sphere the single layer of density 1 is 1 and the double layer
+    (x - y).n_y / (4 pi R^3) of density 1 is -1/2 on the surface, so the packed
+    Galerkin rows sum to ~va and ~-va/2.  The reference rule is coarse near
+    the diagonal (its same-face maps cover half of T x T), hence the loose
+    bounds; a sign or factor-2 slip in the restatement fails them."""
+    V, F = icosphere(2)
+    gp, gw = O.gl_ref(3)
+    va = O.vert_areas(V, F)
+    A = O.pack_matrix_tight(1, O.laplace_layer(V, F, gp, gw, 1, 1), F, V.shape[0])
+    r = A.sum(1) / va
+    assert 0.9 < r.mean() < 1.0 and r.min() > 0.7
+    K = O.pack_matrix_tight(1, O.laplace_layer(V, F, gp, gw, 2, 1), F, V.shape[0])
+    assert -0.5 < (K.sum(1) / va).mean() < -0.45
+    # PLANAR packs the same total
+    Ap = O.pack_matrix_tight(0, O.laplace_layer(V, F, gp, gw, 1, 0), F, V.shape[0])
+    assert abs(Ap.sum() / A.sum() - 1) < 0.05
+
+
+def test_p0_laplace_sphere_eigen_and_demag():
+    """P0 operators on a sphere: V 1 ~ 1 (times area) and the magnetic
+    solution sigma = 3 chi/(3 + chi) H0 n_z."""
+    V, F = icosphere(3)
+    ptr, idx = O.near_pairs_bruteforce(V, F)
+    Vm, Kt = O.laplace_p0(V, F, ptr, idx)
+    A, n = O.face_areas_normals(V, F)
+    assert np.abs(Vm.sum(1) / A - 1).max() < 0.01
+    chi = 3.0
+    lam = (2 + chi) / (2 * chi)
+    sig = np.linalg.solve(lam * np.diag(A) + Kt, A * n[:, 2])
+    ref = 3 * chi / (3 + chi) * n[:, 2]
+    assert np.linalg.norm(sig - ref) / np.linalg.norm(ref) < 0.01
+    assert np.abs(np.diag(Kt)).max() < 1e-15  # principal value of a flat self panel
+
+
+def test_demag_factors_sum_to_one():
+    Na, Nb, Nc = O.demag_factors(1.0, 1.0, 1.7)
+    assert abs(Na + Nb + Nc - 1) < 1e-8 and Nc < 1 / 3 < Na
