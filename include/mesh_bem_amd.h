@@ -120,7 +120,8 @@ int mba_mom_rhs_faceedge_ns(int prec, const void *verts, const int32_t *faces, i
 
 /* RWG accumulation of face-edge blocks into RWG space (replaces the
  * scatter_add_ of mesh_BEM_solver_3d.py:141-147 with a fixed-order gather):
- *   z[m, n] (=|+=) sum_{s,t} zf[fe_s(m), fe_t(n) - col0],  rwg_fe [E, 2] ascending.
+ *   z[m, n] (=|+=) sum_{s,t} zf[fe_s(m), fe_t(n) - col0],  rwg_fe [E, 2] ascending,
+ * summed as (zf[fe0,fe0] + zf[fe1,fe0]) + (zf[fe0,fe1] + zf[fe1,fe1]).
  * Only source face-edges in [col0, col0 + ncols) are read: a column n whose
  * two face-edges both lie in the window is stored; one whose fe_0 lies in it
  * is stored; one whose fe_1 alone lies in it is added (its fe_0 window must
@@ -142,6 +143,15 @@ int mba_mom_gather_rhs(int prec, const void *vf, const int32_t *rwg_fe, int64_t 
 size_t mba_lu_workspace_bytes(int prec, int64_t n);
 int mba_lu_factor(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
                   void *workspace, size_t workspace_bytes, mba_stream_t stream);
+
+/* mba_lu_factor with per-phase device timing (HIP events on `stream`):
+ * phase_ms[4] / phase_launches[4] = total milliseconds / launches of
+ * 0 panel factorisation, 1 row interchanges, 2 triangular solve of the U row
+ * block, 3 trailing MFMA GEMM; gemm_flops = real flops of the trailing GEMMs
+ * (8 m n k per launch).  No counterpart in the reference (measurement aid). */
+int mba_lu_factor_profiled(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
+                           void *workspace, size_t workspace_bytes, double *phase_ms,
+                           int64_t *phase_launches, double *gemm_flops, mba_stream_t stream);
 
 /* Row permutation of a factorisation: row i of P A is row perm[i] of A
  * (the net effect of the sequential swaps in ipiv). */

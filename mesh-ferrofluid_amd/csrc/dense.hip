@@ -12,6 +12,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "mba_common.hpp"
 
 namespace {
@@ -256,7 +258,8 @@ __global__ __launch_bounds__(256) void k_cgemm_f64(int64_t M, int64_t N, int64_t
             __syncthreads();
         }
     }
-    // epilogue: register r of lane -> row 4*(lane/16) + r, real column lane%16
+    // epilogue: register r of lane -> row lane/16 + 4*r, real column lane%16
+    // (v_mfma_f64_16x16x4f64 interleaves its output rows across registers)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(256) void k_cgemm_f64(int64_t M, int64_t N, int64_t
             if (gjr >= 2 * N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gi = m0 + wm * 32 + i * 16 + 4 * kr + r;
+                const int64_t gi = m0 + wm * 32 + i * 16 + kr + 4 * r;
                 if (gi < M) {
                     double* p = (double*)(C + gi * ldc) + gjr;
                     double v = alpha * acc[i][j][r];
@@ -345,6 +348,8 @@ __device__ __forceinline__ void grid_barrier(unsigned long long* counter, unsign
         __hip_atomic_fetch_add(counter, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         long spins = 0;
         while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            // once any workgroup has given up, nobody waits any more
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
             if (++spins > SPIN_LIMIT) {
                 __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
@@ -702,10 +707,47 @@ int device_cus()
     return cus;
 }
 
+// Optional per-phase timing: HIP events recorded on the factorisation stream
+// around each phase of every panel, summed after the final synchronisation.
+struct PhaseTimer {
+    bool on = false;
+    hipStream_t st = nullptr;
+    std::vector<hipEvent_t> ev[4];
+    void mark(int phase)
+    {
+        if (!on) return;
+        hipEvent_t e;
+        if (hipEventCreate(&e) == hipSuccess) {
+            (void)hipEventRecord(e, st);
+            ev[phase].push_back(e);
+        }
+    }
+    // phases: 0 panel, 1 row swaps, 2 triangular solve, 3 trailing GEMM
+    void collect(double* ms, int64_t* launches)
+    {
+        for (int p = 0; p < 4; ++p) {
+            double tot = 0.0;
+            for (size_t i = 0; i + 1 < ev[p].size(); i += 2) {
+                float t = 0.f;
+                if (hipEventElapsedTime(&t, ev[p][i], ev[p][i + 1]) == hipSuccess) tot += t;
+            }
+            if (ms) ms[p] = tot;
+            if (launches) launches[p] = (int64_t)(ev[p].size() / 2);
+            for (auto e : ev[p]) (void)hipEventDestroy(e);
+            ev[p].clear();
+        }
+    }
+};
+
 template <class T>
 int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, void* ws_ptr,
-              size_t ws_bytes, hipStream_t st)
+              size_t ws_bytes, hipStream_t st, double* phase_ms = nullptr,
+              int64_t* phase_launches = nullptr, double* gemm_flops = nullptr)
 {
+    PhaseTimer tm;
+    tm.on = phase_ms != nullptr;
+    tm.st = st;
+    double gflops = 0.0;
     const int prec = sizeof(T) == 4 ? MBA_F32 : MBA_F64;
     LuWs ws = lu_ws_layout(prec, n, (char*)ws_ptr);
     if (ws_bytes < ws.total) return MBA_ERR_ALLOC;
@@ -713,7 +755,9 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     const int cus = device_cus();
     if (cus <= 0) return MBA_ERR_LAUNCH;
     const size_t esz = sizeof(cpx<T>);
-    const size_t lds_budget = 150 * 1024;
+    // dynamic LDS of the panel; the static arrays add up to 11 KiB (FP64), the
+    // total stays under the 160 KiB of a CU
+    const size_t lds_budget = 140 * 1024;
     // panel width: largest of 128/64/32 whose per-CU row share fits the LDS budget
     int NB = 128;
     while (NB > 16 && (size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) NB /= 2;
@@ -760,12 +804,15 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         P.diag_data = (cpx<T>*)ws.diag_data;
         void* args[] = {&P};
         size_t shm = (size_t)R * nb * esz;
+        tm.mark(0);
         if (hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(G), dim3(PANEL_THREADS), args,
                                        (unsigned)shm, st) != hipSuccess)
             return MBA_ERR_LAUNCH;
+        tm.mark(0);
         epoch += (unsigned long long)nb * G;
         const int64_t n2 = n - k - nb;
         if (k > 0 || n2 > 0) {
+            tm.mark(1);
             hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(64), 0, st, ipiv, k, nb, ws.slot_row,
                                ws.plan, ws.plan_n);
             MBA_LAUNCH_CHECK();
@@ -776,20 +823,28 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                                (size_t)2 * nb * LASWP_COLS * esz, st, A, lda, (int64_t)0, n,
                                k, k + nb, nb, ws.slot_row, ws.plan, ws.plan_n);
             MBA_LAUNCH_CHECK();
+            tm.mark(1);
             (void)ncols;
         }
         if (n2 > 0) {
+            tm.mark(2);
             hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n2 + TRSM_COLS - 1) / TRSM_COLS)),
                                dim3(256), (size_t)nb * TRSM_COLS * esz, st, A, lda, k, nb, k + nb, n);
             MBA_LAUNCH_CHECK();
+            tm.mark(2);
+            tm.mark(3);
             int rc = gemm<T>(n2, n2, nb, -1.0, A + (k + nb) * lda + k, lda, A + k * lda + k + nb, lda,
                              1.0, A + (k + nb) * lda + k + nb, lda, st);
             if (rc != MBA_OK) return rc;
+            tm.mark(3);
+            gflops += 8.0 * (double)n2 * (double)n2 * (double)nb;
         }
     }
     int herr = 0;
     if (hipMemcpyAsync(&herr, ws.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return MBA_ERR_LAUNCH;
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    tm.collect(phase_ms, phase_launches);
+    if (gemm_flops) *gemm_flops = gflops;
     return herr ? MBA_ERR_TIMEOUT : MBA_OK;
 }
 
@@ -947,6 +1002,15 @@ int lu_solve(const void* lu, int64_t n, int64_t lda, const int32_t* perm, void* 
     MBA_LAUNCH_CHECK();
     const int BS = 64;
     const size_t dshm = (size_t)BS * BS * esz;
+    static bool attr_set[2] = {false, false};
+    const int pi = sizeof(T) == 4 ? 0 : 1;
+    if (!attr_set[pi]) {
+        (void)hipFuncSetAttribute((const void*)k_diag_solve<T, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dshm);
+        (void)hipFuncSetAttribute((const void*)k_diag_solve<T, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)dshm);
+        attr_set[pi] = true;
+    }
     const unsigned dgrid = (unsigned)((nrhs + DS_RHS - 1) / DS_RHS);
     // forward: L y = P b (tmp holds P b, ld = nrhs)
     for (int64_t r0 = 0; r0 < n; r0 += BS) {
@@ -1074,6 +1138,22 @@ int mba_lu_factor(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int3
     if (lda < n) return MBA_ERR_ARG;
     if (prec == MBA_F32) return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
     return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+}
+
+int mba_lu_factor_profiled(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,
+                           void* workspace, size_t workspace_bytes, double* phase_ms,
+                           int64_t* phase_launches, double* gemm_flops, mba_stream_t stream)
+{
+    if (!a || !ipiv || !info || !workspace || !phase_ms || !phase_launches || !gemm_flops)
+        return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+    if (lda < n) return MBA_ERR_ARG;
+    if (prec == MBA_F32)
+        return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), phase_ms,
+                                phase_launches, gemm_flops);
+    return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), phase_ms,
+                             phase_launches, gemm_flops);
 }
 
 int mba_lu_pivots_to_perm(const int32_t* ipiv, int64_t n, int32_t* perm, mba_stream_t stream)

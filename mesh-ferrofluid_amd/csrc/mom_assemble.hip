@@ -612,24 +612,24 @@ __global__ void k_gather_rwg(const cpx<T>* __restrict__ zf, int64_t ldzf, int64_
     if (!in0 && !in1) return;
     const cpx<T>* r0 = zf + (int64_t)rwg_fe[2 * m] * ldzf;
     const cpx<T>* r1 = zf + (int64_t)rwg_fe[2 * m + 1] * ldzf;
-    T re = 0, im = 0;
+    // z = (zf[fe0(m), fe0(n)] + zf[fe1(m), fe0(n)]) + (zf[fe0(m), fe1(n)] + zf[fe1(m), fe1(n)]):
+    // the two column pairs are summed separately, so a column split over two
+    // windows rounds exactly as one gathered in a single window.
+    cpx<T>* out = z + m * ldz + n;
+    T re, im;
     if (in0) {
         cpx<T> a = r0[c0], b = r1[c0];
         re = a.re + b.re;
         im = a.im + b.im;
+    } else {
+        cpx<T> o = *out;
+        re = o.re;
+        im = o.im;
     }
     if (in1) {
         cpx<T> a = r0[c1], b = r1[c1];
-        re = re + a.re;
-        im = im + a.im;
-        re = re + b.re;
-        im = im + b.im;
-    }
-    cpx<T>* out = z + m * ldz + n;
-    if (!in0) {
-        cpx<T> o = *out;
-        re = o.re + re;
-        im = o.im + im;
+        re = re + (a.re + b.re);
+        im = im + (a.im + b.im);
     }
     store_cpx(out, re, im);
 }

@@ -37,7 +37,7 @@ def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alp
 class LUFactor:
     """In-place LU with partial pivoting of a square complex matrix."""
 
-    def __init__(self, A: torch.Tensor, overwrite: bool = False):
+    def __init__(self, A: torch.Tensor, overwrite: bool = False, profile: bool = False):
         lib = _lib.load()
         if A.dim() != 2 or A.shape[0] != A.shape[1]:
             raise ValueError("LUFactor needs a square matrix")
@@ -55,9 +55,27 @@ class LUFactor:
         info = torch.zeros(1, dtype=torch.int32, device=dev)
         wsb = lib.mba_lu_workspace_bytes(self.prec, n)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
-        st = lib.mba_lu_factor(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0), _lib.ptr(self.ipiv),
-                               _lib.ptr(info), _lib.ptr(ws), wsb, _lib.stream())
-        _lib.check(st, "mba_lu_factor")
+        self.profile = None
+        if profile:
+            import ctypes
+
+            ms = (ctypes.c_double * 4)()
+            nl = (ctypes.c_int64 * 4)()
+            fl = ctypes.c_double()
+            st = lib.mba_lu_factor_profiled(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0),
+                                            _lib.ptr(self.ipiv), _lib.ptr(info), _lib.ptr(ws), wsb,
+                                            ms, nl, ctypes.byref(fl), _lib.stream())
+            _lib.check(st, "mba_lu_factor_profiled")
+            self.profile = {
+                "phase_ms": dict(zip(("panel", "laswp", "trsm", "gemm"), list(ms))),
+                "phase_launches": dict(zip(("panel", "laswp", "trsm", "gemm"), list(nl))),
+                "gemm_flops": fl.value,
+            }
+        else:
+            st = lib.mba_lu_factor(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0),
+                                   _lib.ptr(self.ipiv), _lib.ptr(info), _lib.ptr(ws), wsb,
+                                   _lib.stream())
+            _lib.check(st, "mba_lu_factor")
         st = lib.mba_lu_pivots_to_perm(_lib.ptr(self.ipiv), n, _lib.ptr(self.perm), _lib.stream())
         _lib.check(st, "mba_lu_pivots_to_perm")
         self.info = int(info.item())

@@ -275,13 +275,14 @@ void orc_tri_potentials(const double *P0, const double *P1, const double *P2, co
  * inner integral over b is analytic (orc_tri_potentials).
  * near_ptr/near_idx: CSR over source faces b listing the near test faces a.
  * out: [3F, 3F] complex. */
-void orc_ns_zmat_faceedge(const double *verts, const int32_t *faces, const double *areas,
-                          const double *lam, const double *w, int nq, double k, int64_t F,
-                          const int64_t *near_ptr, const int32_t *near_idx, double *out)
+void orc_ns_zmat_faceedge_window(const double *verts, const int32_t *faces, const double *areas,
+                                 const double *lam, const double *w, int nq, double k, int64_t F,
+                                 const int64_t *near_ptr, const int32_t *near_idx, int64_t b_lo,
+                                 int64_t b_hi, double *out)
 {
-    const int64_t N = 3 * F;
+    const int64_t N = 3 * (b_hi - b_lo);  /* leading dimension of out */
 #pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t b = 0; b < F; ++b) {
+    for (int64_t b = b_lo; b < b_hi; ++b) {
         const double *Q[3] = {verts + 3 * faces[3 * b], verts + 3 * faces[3 * b + 1],
                               verts + 3 * faces[3 * b + 2]};
         double ob[3], yp[64][3], wy[64];
@@ -352,13 +353,20 @@ void orc_ns_zmat_faceedge(const double *verts, const int32_t *faces, const doubl
                                 (ai[0] * Iy[0] + ai[1] * Iy[1] + ai[2] * Iy[2]) +
                                 dot3(ai, bj) * I0;
                     cplx z = ca * cb[j] * (I * k * dotv - (4.0 * I / k) * I0);
-                    int64_t o = 2 * ((3 * a + i) * N + 3 * b + j);
+                    int64_t o = 2 * ((3 * a + i) * N + 3 * (b - b_lo) + j);
                     out[o] = creal(z);
                     out[o + 1] = cimag(z);
                 }
             }
         }
     }
+}
+
+void orc_ns_zmat_faceedge(const double *verts, const int32_t *faces, const double *areas,
+                          const double *lam, const double *w, int nq, double k, int64_t F,
+                          const int64_t *near_ptr, const int32_t *near_idx, double *out)
+{
+    orc_ns_zmat_faceedge_window(verts, faces, areas, lam, w, nq, k, F, near_ptr, near_idx, 0, F, out);
 }
 
 /* ------------------------------------------------------------ ns mode RHS
@@ -395,7 +403,7 @@ void orc_ns_rhs_faceedge(const double *verts, const int32_t *faces, const double
 
 /* ------------------------------------------------------------ RWG accumulation
  * Z[m, n] = sum_{s,t} ZF[fe_s(m), fe_t(n)], rwg_fe [E, 2] ascending; summation
- * order (s,t) = (0,0), (0,1), (1,0), (1,1).  Row-major Z [E, E]. */
+ * order (ZF[0,0] + ZF[1,0]) + (ZF[0,1] + ZF[1,1]).  Row-major Z [E, E]. */
 void orc_gather_rwg(const double *ZF, int64_t F, const int32_t *rwg_fe, int64_t E, double *Z)
 {
     const int64_t N = 3 * F;
@@ -403,12 +411,16 @@ void orc_gather_rwg(const double *ZF, int64_t F, const int32_t *rwg_fe, int64_t 
     for (int64_t m = 0; m < E; ++m)
         for (int64_t n = 0; n < E; ++n) {
             double re = 0.0, im = 0.0;
-            for (int s = 0; s < 2; ++s)
-                for (int t = 0; t < 2; ++t) {
+            for (int t = 0; t < 2; ++t) {
+                double pr = 0.0, pi = 0.0;
+                for (int s = 0; s < 2; ++s) {
                     int64_t o = 2 * ((int64_t)rwg_fe[2 * m + s] * N + rwg_fe[2 * n + t]);
-                    re += ZF[o];
-                    im += ZF[o + 1];
+                    pr += ZF[o];
+                    pi += ZF[o + 1];
                 }
+                re += pr;
+                im += pi;
+            }
             Z[2 * (m * E + n)] = re;
             Z[2 * (m * E + n) + 1] = im;
         }

@@ -55,6 +55,8 @@ def lib():
         L.orc_interp_faceedge.argtypes = [D, I32, I32, D, D, i64, i64, i64, D]
         L.orc_tri_potentials.argtypes = [D, D, D, D, D, D, D]
         L.orc_ns_zmat_faceedge.argtypes = [D, I32, D, D, D, i32, dbl, i64, I64, I32, D]
+        L.orc_ns_zmat_faceedge_window.argtypes = [D, I32, D, D, D, i32, dbl, i64, I64, I32, i64, i64, D]
+        L.orc_ns_zmat_faceedge_window.restype = None
         L.orc_ns_rhs_faceedge.argtypes = [D, I32, D, D, D, i32, dbl, i64, D, D, i64, D]
         L.orc_gather_rwg.argtypes = [D, i64, I32, i64, D]
         L.orc_gather_rhs.argtypes = [D, I32, i64, i64, D]
@@ -271,6 +273,20 @@ def ns_zmat_faceedge(verts, faces, k, near_ptr, near_idx, rule=None):
     lib().orc_ns_zmat_faceedge(_d(v), _i32(f), _d(A), _d(lam), _d(w), w.size, float(k), F,
                                _i64(np.ascontiguousarray(near_ptr, dtype=np.int64)),
                                _i32(_fi32(near_idx)), _d(out.view(np.float64)))
+    return out
+
+
+def ns_zmat_faceedge_window(verts, faces, k, near_ptr, near_idx, b_lo, b_hi, rule=None):
+    """Columns of source faces [b_lo, b_hi): [3F, 3 (b_hi - b_lo)] complex."""
+    v, f = _f64(verts), _fi32(faces)
+    A, _ = face_areas_normals(v, f)
+    lam, w = radon7() if rule is None else rule
+    F = len(f)
+    out = np.zeros((3 * F, 3 * (b_hi - b_lo)), dtype=np.complex128)
+    lib().orc_ns_zmat_faceedge_window(_d(v), _i32(f), _d(A), _d(_f64(lam)), _d(_f64(w)), len(w),
+                                      float(k), F, _i64(np.ascontiguousarray(near_ptr, dtype=np.int64)),
+                                      _i32(_fi32(near_idx)), int(b_lo), int(b_hi),
+                                      _d(out.view(np.float64)))
     return out
 
 

@@ -29,10 +29,9 @@ def test_gemm(dev, dtype, tol, M, N, K, beta):
     A = torch.tensor(a, dtype=dtype, device=dev)
     B = torch.tensor(b, dtype=dtype, device=dev)
     C = torch.tensor(c, dtype=dtype, device=dev)
+    c_in = C.cpu().numpy().astype(np.complex128)  # gemm updates C in place
     out = gemm(A, B, C, alpha=-1.0, beta=beta).cpu().numpy()
-    ref = -(A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128)) + \
-        beta * C.cpu().numpy().astype(np.complex128) if beta else \
-        -(A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128))
+    ref = -(A.cpu().numpy().astype(np.complex128) @ B.cpu().numpy().astype(np.complex128)) + beta * c_in
     scale = np.abs(a).max() * np.abs(b).max() * K + beta * np.abs(c).max()
     assert np.abs(out - ref).max() / scale < tol
 
