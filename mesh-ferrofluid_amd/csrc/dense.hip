@@ -311,14 +311,21 @@ int gemm<double>(int64_t M, int64_t N, int64_t K, double alpha, const void* A, i
 // ============================================================== LU panel
 // Cooperative kernel: G workgroups (one per CU, all resident) each hold R
 // consecutive rows of the m x nb panel in LDS.  Per column j:
-//   local arg-max of |Re| + |Im| (ties: lowest row) -> candidate slot + row copy,
-//   the owner of row j publishes row j, one grid barrier, every workgroup picks
-//   the winner, swaps locally, scales and applies the rank-1 update.
-// Slots are double-buffered on j parity, so one barrier per column suffices.
-// The grid barrier is a monotone agent-scope counter with a bounded wait; on
-// expiry the kernel sets *err and runs to completion (its result is garbage).
+//   local arg-max of |Re| + |Im| (ties: lowest row) and the candidate row are
+//   published; the owner of row j publishes row j; every workgroup gathers the
+//   G candidates, picks the same winner, swaps locally, scales, and applies
+//   the rank-1 update.
+// Messages are self-validating 64-bit words (epoch << 32 | payload) moved with
+// relaxed agent-scope atomics: a reader waits until every word it needs carries
+// the current step's epoch, so no cache write-back / invalidate fences are
+// needed and no shared counter is contended.  Epochs k + j + 1 are unique per
+// step.  Buffers are double-buffered on j parity: a writer reuses a buffer two
+// steps later, after every workgroup has published the step in between, which
+// each does only after finishing its reads of the first.  Waits are bounded;
+// on expiry the kernel sets *err, stops waiting and runs to completion (its
+// result is garbage and the host reports a timeout).
 constexpr int PANEL_THREADS = 256;
-constexpr long SPIN_LIMIT = 1L << 24;
+constexpr long SPIN_LIMIT = 1L << 22;
 
 template <class T>
 struct PanelArgs {
@@ -330,48 +337,97 @@ struct PanelArgs {
     int R;           // rows per workgroup
     int32_t* ipiv;   // global [n]
     int32_t* info;
-    unsigned long long* counter;
-    unsigned long long epoch;  // counter value at kernel start
     int* err;
-    T* cand_val;      // [2][G]
-    int32_t* cand_row;  // [2][G]
-    cpx<T>* cand_data;  // [2][G][nb]
-    cpx<T>* diag_data;  // [2][nb]
+    uint64_t* meta;  // [2][G] (epoch, candidate row)
+    uint64_t* cval;  // [2][G][VW] (epoch, candidate |.|_1 bits)
+    uint64_t* cand;  // [2][G][nb][2 VW] candidate rows
+    uint64_t* diag;  // [2][nb][2 VW] row j
 };
-
-__device__ __forceinline__ void grid_barrier(unsigned long long* counter, unsigned long long target,
-                                             int* err)
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(counter, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        long spins = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            // once any workgroup has given up, nobody waits any more
-            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-            if (++spins > SPIN_LIMIT) {
-                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-}
 
 template <class T>
 __device__ __forceinline__ T cabs1(cpx<T> v) { return fabs(v.re) + fabs(v.im); }
 
+__device__ __forceinline__ void tag_put(uint64_t* p, uint32_t epoch, uint32_t payload)
+{
+    __hip_atomic_store(p, ((uint64_t)epoch << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t tag_get(uint64_t* p, uint32_t epoch, int* err)
+{
+    long spins = 0;
+    while (true) {
+        uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)(w >> 32) == epoch) return (uint32_t)w;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return (uint32_t)w;
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return (uint32_t)w;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// T <-> 32-bit words (VW words per value)
+__device__ __forceinline__ void put_val(uint64_t* p, uint32_t e, float v) { tag_put(p, e, __float_as_uint(v)); }
+__device__ __forceinline__ void put_val(uint64_t* p, uint32_t e, double v)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    tag_put(p, e, (uint32_t)(b >> 32));
+    tag_put(p + 1, e, (uint32_t)b);
+}
+__device__ __forceinline__ void get_val(uint64_t* p, uint32_t e, int* err, float& v)
+{
+    v = __uint_as_float(tag_get(p, e, err));
+}
+__device__ __forceinline__ void get_val(uint64_t* p, uint32_t e, int* err, double& v)
+{
+    unsigned long long hi = tag_get(p, e, err), lo = tag_get(p + 1, e, err);
+    v = __longlong_as_double((long long)((hi << 32) | lo));
+}
+
+// Block-wide arg-max of (v desc, r asc) carrying q; result in red_*[0].
+template <class T>
+__device__ __forceinline__ void block_argmax(T v, int32_t r, int32_t q, T* red_v, int32_t* red_r,
+                                             int32_t* red_q)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T v2 = __shfl_xor(v, off);
+        int32_t r2 = __shfl_xor(r, off);
+        int32_t q2 = __shfl_xor(q, off);
+        if (v2 > v || (v2 == v && r2 < r)) {
+            v = v2;
+            r = r2;
+            q = q2;
+        }
+    }
+    if (lane == 0) {
+        red_v[wave] = v;
+        red_r[wave] = r;
+        red_q[wave] = q;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < PANEL_THREADS / 64; ++w)
+            if (red_v[w] > red_v[0] || (red_v[w] == red_v[0] && red_r[w] < red_r[0])) {
+                red_v[0] = red_v[w];
+                red_r[0] = red_r[w];
+                red_q[0] = red_q[w];
+            }
+    }
+    __syncthreads();
+}
+
 template <class T>
 __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
 {
+    constexpr int VW = sizeof(T) / 4;  // 32-bit words per real
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* rows = (cpx<T>*)smem_raw;  // [R][nb]
-    __shared__ T red_v[PANEL_THREADS];
-    __shared__ int32_t red_r[PANEL_THREADS];
-    __shared__ int32_t s_piv;
+    __shared__ T red_v[PANEL_THREADS / 64];
+    __shared__ int32_t red_r[PANEL_THREADS / 64];
+    __shared__ int32_t red_q[PANEL_THREADS / 64];
     __shared__ cpx<T> s_prow[128];
     __shared__ cpx<T> s_drow[128];
     __shared__ cpx<T> s_inv;
@@ -381,6 +437,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
     const int nb = P.nb, R = P.R;
     const int64_t lr0 = (int64_t)g * R;  // first local row of this workgroup
     const int nr = (int)((P.m - lr0) < R ? (P.m - lr0 > 0 ? P.m - lr0 : 0) : R);
+    const int64_t row_words = (int64_t)nb * 2 * VW;
 
     for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
         int r = idx / nb, c = idx % nb;
@@ -390,6 +447,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
 
     for (int j = 0; j < nb; ++j) {
         const int par = j & 1;
+        const uint32_t epoch = (uint32_t)(P.k + j + 1);
         // --- local arg-max over local rows with panel row index >= j
         T bv = (T)-1;
         int32_t br = 0x7fffffff;
@@ -403,55 +461,61 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
                 }
             }
         }
-        red_v[tid] = bv;
-        red_r[tid] = br;
-        __syncthreads();
-        for (int s = PANEL_THREADS / 2; s > 0; s >>= 1) {
-            if (tid < s) {
-                T v2 = red_v[tid + s];
-                int32_t r2 = red_r[tid + s];
-                if (v2 > red_v[tid] || (v2 == red_v[tid] && r2 < red_r[tid])) {
-                    red_v[tid] = v2;
-                    red_r[tid] = r2;
-                }
-            }
-            __syncthreads();
-        }
+        block_argmax(bv, br, 0, red_v, red_r, red_q);
         const int32_t lbest = red_r[0];
+        // --- publish: candidate meta / value / row, and row j if owned
         if (tid == 0) {
-            P.cand_val[par * G + g] = red_v[0];
-            P.cand_row[par * G + g] = lbest;
+            tag_put(P.meta + par * G + g, epoch, (uint32_t)lbest);
+            put_val(P.cval + ((int64_t)par * G + g) * VW, epoch, red_v[0]);
         }
-        cpx<T>* slot = P.cand_data + ((int64_t)par * G + g) * nb;
+        uint64_t* cw = P.cand + ((int64_t)par * G + g) * row_words;
         if (lbest != 0x7fffffff)
-            for (int c = tid; c < nb; c += PANEL_THREADS) slot[c] = rows[(lbest - lr0) * nb + c];
-        if (j >= lr0 && j < lr0 + nr)
-            for (int c = tid; c < nb; c += PANEL_THREADS)
-                P.diag_data[par * nb + c] = rows[(j - lr0) * nb + c];
-        grid_barrier(P.counter, P.epoch + (unsigned long long)(j + 1) * G, P.err);
-
-        // --- winner (identical in every workgroup)
-        if (tid == 0) {
-            T wv = (T)-1;
-            int32_t wr = 0x7fffffff, wg = 0;
-            for (int q = 0; q < G; ++q) {
-                T v = P.cand_val[par * G + q];
-                int32_t r = P.cand_row[par * G + q];
-                if (v > wv || (v == wv && r < wr)) {
-                    wv = v;
-                    wr = r;
-                    wg = q;
-                }
+            for (int c = tid; c < nb; c += PANEL_THREADS) {
+                cpx<T> x = rows[(lbest - lr0) * nb + c];
+                put_val(cw + (2 * c) * VW, epoch, x.re);
+                put_val(cw + (2 * c + 1) * VW, epoch, x.im);
             }
-            s_piv = wg;
-            red_r[0] = wr;
+        else
+            for (int c = tid; c < nb; c += PANEL_THREADS) {
+                put_val(cw + (2 * c) * VW, epoch, (T)0);
+                put_val(cw + (2 * c + 1) * VW, epoch, (T)0);
+            }
+        if (j >= lr0 && j < lr0 + nr) {
+            uint64_t* dw = P.diag + (int64_t)par * row_words;
+            for (int c = tid; c < nb; c += PANEL_THREADS) {
+                cpx<T> x = rows[(j - lr0) * nb + c];
+                put_val(dw + (2 * c) * VW, epoch, x.re);
+                put_val(dw + (2 * c + 1) * VW, epoch, x.im);
+            }
         }
-        __syncthreads();
-        const int wg = s_piv;
+        // --- gather the G candidates (one slot per thread)
+        T v = (T)-1;
+        int32_t r = 0x7fffffff, q = 0;
+        for (int c = tid; c < G; c += PANEL_THREADS) {
+            int32_t cr = (int32_t)tag_get(P.meta + par * G + c, epoch, P.err);
+            T cv;
+            get_val(P.cval + ((int64_t)par * G + c) * VW, epoch, P.err, cv);
+            if (cv > v || (cv == v && cr < r)) {
+                v = cv;
+                r = cr;
+                q = c;
+            }
+        }
+        block_argmax(v, r, q, red_v, red_r, red_q);
+        const int wg = red_q[0];
         const int32_t p = red_r[0];
-        for (int c = tid; c < nb; c += PANEL_THREADS) {
-            s_prow[c] = P.cand_data[((int64_t)par * G + wg) * nb + c];
-            s_drow[c] = P.diag_data[par * nb + c];
+        {
+            uint64_t* pw = P.cand + ((int64_t)par * G + wg) * row_words;
+            uint64_t* dw = P.diag + (int64_t)par * row_words;
+            for (int c = tid; c < nb; c += PANEL_THREADS) {
+                cpx<T> x, y;
+                get_val(pw + (2 * c) * VW, epoch, P.err, x.re);
+                get_val(pw + (2 * c + 1) * VW, epoch, P.err, x.im);
+                get_val(dw + (2 * c) * VW, epoch, P.err, y.re);
+                get_val(dw + (2 * c + 1) * VW, epoch, P.err, y.im);
+                s_prow[c] = x;
+                s_drow[c] = y;
+            }
         }
         __syncthreads();
         if (tid == 0) {
@@ -479,27 +543,29 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
         __syncthreads();
         // scale column j below the diagonal, then rank-1 update of columns > j
         const cpx<T> inv = s_inv;
-        for (int r = tid; r < nr; r += PANEL_THREADS) {
-            if (lr0 + r > j) {
-                cpx<T> a = rows[r * nb + j];
+        for (int rr = tid; rr < nr; rr += PANEL_THREADS) {
+            if (lr0 + rr > j) {
+                cpx<T> a = rows[rr * nb + j];
                 cpx<T> l;
                 l.re = a.re * inv.re - a.im * inv.im;
                 l.im = a.re * inv.im + a.im * inv.re;
-                rows[r * nb + j] = l;
+                rows[rr * nb + j] = l;
             }
         }
         __syncthreads();
-        const int w = nb - j - 1;
-        if (w > 0) {
-            for (int idx = tid; idx < nr * w; idx += PANEL_THREADS) {
-                int r = idx / w, c = j + 1 + idx % w;
-                if (lr0 + r > j) {
-                    cpx<T> l = rows[r * nb + j];
-                    cpx<T> u = s_prow[c];
-                    cpx<T> a = rows[r * nb + c];
-                    a.re -= l.re * u.re - l.im * u.im;
-                    a.im -= l.re * u.im + l.im * u.re;
-                    rows[r * nb + c] = a;
+        {
+            const int c = tid % nb;
+            const int rstep = PANEL_THREADS / nb;
+            if (c > j && tid < rstep * nb) {
+                const cpx<T> u = s_prow[c];
+                for (int rr = tid / nb; rr < nr; rr += rstep) {
+                    if (lr0 + rr > j) {
+                        cpx<T> l = rows[rr * nb + j];
+                        cpx<T> a = rows[rr * nb + c];
+                        a.re -= l.re * u.re - l.im * u.im;
+                        a.im -= l.re * u.im + l.im * u.re;
+                        rows[rr * nb + c] = a;
+                    }
                 }
             }
         }
@@ -658,16 +724,16 @@ __global__ void k_zero_i32(int32_t* p, int n)
 
 // ============================================================== workspace
 struct LuWs {
-    unsigned long long* counter;
     int* err;
-    void* cand_val;
-    int32_t* cand_row;
-    void* cand_data;
-    void* diag_data;
+    uint64_t* meta;
+    uint64_t* cval;
+    uint64_t* cand;
+    uint64_t* diag;
     int32_t* slot_row;
     int32_t* plan;
     int32_t* plan_n;
     size_t total;
+    size_t msg_words;  // words from meta to the end of diag (zeroed per factorisation)
 };
 
 constexpr int MAX_G = 1024;
@@ -676,7 +742,7 @@ inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 LuWs lu_ws_layout(int prec, int64_t n, char* base)
 {
-    const size_t esz = prec == MBA_F32 ? 4 : 8;
+    const size_t VW = prec == MBA_F32 ? 1 : 2;
     const int nbmax = 128;
     LuWs w;
     size_t off = 0;
@@ -685,12 +751,13 @@ LuWs lu_ws_layout(int prec, int64_t n, char* base)
         off += al256(bytes);
         return p;
     };
-    w.counter = (unsigned long long*)take(256);
     w.err = (int*)take(256);
-    w.cand_val = take(2 * MAX_G * esz);
-    w.cand_row = (int32_t*)take(2 * MAX_G * 4);
-    w.cand_data = take(2 * MAX_G * (size_t)nbmax * 2 * esz);
-    w.diag_data = take(2 * (size_t)nbmax * 2 * esz);
+    const size_t msg0 = off;
+    w.meta = (uint64_t*)take(2 * MAX_G * 8);
+    w.cval = (uint64_t*)take(2 * MAX_G * VW * 8);
+    w.cand = (uint64_t*)take(2 * MAX_G * (size_t)nbmax * 2 * VW * 8);
+    w.diag = (uint64_t*)take(2 * (size_t)nbmax * 2 * VW * 8);
+    w.msg_words = (off - msg0) / 8;
     w.slot_row = (int32_t*)take(2 * nbmax * 4);
     w.plan = (int32_t*)take(2 * nbmax * 4);
     w.plan_n = (int32_t*)take(256);
@@ -770,14 +837,14 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                                   (int)(128 * TRSM_COLS * esz));
         attr_set[prec] = true;
     }
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ws.counter, 64);
-    MBA_LAUNCH_CHECK();
-    // info := 0 and err := 0 (both 32-bit; err lives in its own 256-byte slot)
+    // err := 0, every message word := epoch 0 (epochs of this factorisation are k + j + 1 >= 1)
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ws.err, 32);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((ws.msg_words + 255) / 256)), dim3(256), 0, st,
+                       (unsigned long long*)ws.meta, (int)ws.msg_words);
     MBA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, 1);
     MBA_LAUNCH_CHECK();
-    unsigned long long epoch = 0;
     for (int64_t k = 0; k < n; k += NB) {
         const int nb = (int)((n - k) < NB ? (n - k) : NB);
         const int64_t m = n - k;
@@ -795,13 +862,11 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         P.R = R;
         P.ipiv = ipiv;
         P.info = info;
-        P.counter = ws.counter;
-        P.epoch = epoch;
         P.err = ws.err;
-        P.cand_val = (T*)ws.cand_val;
-        P.cand_row = ws.cand_row;
-        P.cand_data = (cpx<T>*)ws.cand_data;
-        P.diag_data = (cpx<T>*)ws.diag_data;
+        P.meta = ws.meta;
+        P.cval = ws.cval;
+        P.cand = ws.cand;
+        P.diag = ws.diag;
         void* args[] = {&P};
         size_t shm = (size_t)R * nb * esz;
         tm.mark(0);
@@ -809,7 +874,6 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                                        (unsigned)shm, st) != hipSuccess)
             return MBA_ERR_LAUNCH;
         tm.mark(0);
-        epoch += (unsigned long long)nb * G;
         const int64_t n2 = n - k - nb;
         if (k > 0 || n2 > 0) {
             tm.mark(1);
