@@ -170,6 +170,16 @@ int mba_lu_solve(int prec, const void *lu, int64_t n, int64_t lda, const int32_t
 int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,
              const void *b, int64_t ldb, double beta, void *c, int64_t ldc, mba_stream_t stream);
 
+/* Tile variants of the FP32 GEMM (0: 128x64 complex tile, 1: 128x64 ordered
+ * for the 8 XCDs, 2: 128x128, 3: 128x128 XCD-ordered).  mba_gemm_variant runs
+ * one variant; mba_gemm_set_variant selects the one mba_gemm and the LU use
+ * (process-wide, default 0; set once by a tuner, not per call).  No
+ * counterpart in the reference. */
+int mba_gemm_variant(int prec, int variant, int64_t m, int64_t n, int64_t k, double alpha,
+                     const void *a, int64_t lda, const void *b, int64_t ldb, double beta, void *c,
+                     int64_t ldc, mba_stream_t stream);
+int mba_gemm_set_variant(int prec, int variant);
+
 /* FP64 residual r = v - Z x for complex128 Z [n, ldz], x/v/r [n, nrhs]. */
 int mba_residual_c128(const void *z, int64_t n, int64_t ldz, const void *x, const void *v,
                       void *r, int64_t nrhs, mba_stream_t stream);

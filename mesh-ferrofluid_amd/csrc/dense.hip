@@ -31,91 +31,99 @@ inline hipStream_t S(mba_stream_t s) { return (hipStream_t)s; }
 // exactly the row-major complex memory layout of C.
 //
 // FP32: v_mfma_f32_32x32x2f32, one instruction = 32 rows x 16 complex columns
-// x 1 complex k.  Block tile 128 x 64 complex, 4 waves in 2 x 2, each wave
-// 64 x 32 (2 x 2 instructions), k-chunks of 16 staged in LDS (double buffer).
-// A is staged k-major as separate re / im planes so a wave's operand read is
-// 32 consecutive floats; B is staged row-major real.
-constexpr int G32_BM = 128, G32_BN = 64, G32_BK = 16;
+// x 1 complex k.  Block tile 128 x (32 WN) complex, 4 waves in 2 x 2, each wave
+// 64 x (16 WN) (2 x WN instructions), k-chunks of 16 staged in LDS (double
+// buffer).  A is staged k-major as separate re / im planes so a wave's operand
+// read is 32 consecutive floats; B is staged row-major real.
+// XCD = true orders tiles for the 8 XCDs (workgroup L runs on XCD L % 8): each
+// XCD takes a contiguous range of tiles, walked in groups of GROUP_M row tiles
+// so an XCD's L2 keeps the group's A row blocks while its B columns stream.
+constexpr int G32_BM = 128, G32_BK = 16, G32_GROUP_M = 8, NUM_XCD = 8;
 
+__device__ __forceinline__ void tile_of(int64_t L, int64_t nbm, int64_t nbn, int64_t& tm, int64_t& tn)
+{
+    const int64_t T = nbm * nbn;
+    const int64_t per = (T + NUM_XCD - 1) / NUM_XCD;
+    const int64_t xcd = L % NUM_XCD, loc = L / NUM_XCD;
+    int64_t v = xcd * per + loc;  // contiguous range per XCD
+    if (v >= T) {
+        tm = -1;
+        tn = -1;
+        return;
+    }
+    const int64_t group = v / (G32_GROUP_M * nbn);
+    const int64_t first_m = group * G32_GROUP_M;
+    const int64_t gm = (nbm - first_m) < G32_GROUP_M ? (nbm - first_m) : G32_GROUP_M;
+    const int64_t in = v - group * G32_GROUP_M * nbn;
+    tm = first_m + in % gm;
+    tn = in / gm;
+}
+
+template <int WN, bool XCD>
 __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha,
                                                    const cpx<float>* __restrict__ A, int64_t lda,
                                                    const cpx<float>* __restrict__ B, int64_t ldb,
                                                    float beta, cpx<float>* __restrict__ C,
                                                    int64_t ldc)
 {
+    constexpr int BN = 32 * WN;       // complex columns per block
+    constexpr int BPT = BN / 16;      // complex B elements per thread per chunk
     __shared__ alignas(16) float Are[2][G32_BK][G32_BM];
     __shared__ alignas(16) float Aim[2][G32_BK][G32_BM];
-    __shared__ alignas(16) float Bs[2][G32_BK][2 * G32_BN];
+    __shared__ alignas(16) float Bs[2][G32_BK][2 * BN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int64_t m0 = (int64_t)blockIdx.y * G32_BM, n0 = (int64_t)blockIdx.x * G32_BN;
+    const int64_t nbm = (M + G32_BM - 1) / G32_BM, nbn = (N + BN - 1) / BN;
+    int64_t tmi, tni;
+    if (XCD) {
+        tile_of(blockIdx.x, nbm, nbn, tmi, tni);
+        if (tmi < 0) return;
+    } else {
+        tmi = blockIdx.x / nbn;
+        tni = blockIdx.x % nbn;
+    }
+    const int64_t m0 = tmi * G32_BM, n0 = tni * BN;
 
-    // global -> register staging
-    // A: thread -> row ar = tid/2, complex k = (tid&1)*8 .. +8
+    // A: 128 rows x 16 complex: thread -> row tid/2, complex k (tid&1)*8 .. +8
     const int ar = tid >> 1, ak = (tid & 1) * 8;
-    // B: thread -> row bk = tid/16, complex cols (tid&15)*4 .. +4
-    const int bkr = tid >> 4, bc = (tid & 15) * 4;
-    float4 ra[4], rb[2];
+    // B: 16 rows x BN complex: thread -> row tid/16, complex cols (tid&15)*BPT .. +BPT
+    const int bkr = tid >> 4, bc = (tid & 15) * BPT;
+    float2 ra[8], rb[BPT];
 
     auto load_chunk = [&](int64_t k0) {
         const int64_t gi = m0 + ar;
+        const bool rowok = gi < M;
+        const float2* pa = (const float2*)(A + (rowok ? gi : 0) * lda);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int64_t gk = k0 + ak + 2 * q;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gi < M) {
-                const float2* p = (const float2*)(A + gi * lda + gk);
-                if (gk < K) {
-                    float2 u = p[0];
-                    v.x = u.x;
-                    v.y = u.y;
-                }
-                if (gk + 1 < K) {
-                    float2 u = p[1];
-                    v.z = u.x;
-                    v.w = u.y;
-                }
-            }
-            ra[q] = v;
+        for (int q = 0; q < 8; ++q) {
+            const int64_t gk = k0 + ak + q;
+            ra[q] = (rowok && gk < K) ? pa[gk] : make_float2(0.f, 0.f);
         }
         const int64_t gk = k0 + bkr;
+        const bool kok = gk < K;
+        const float2* pb = (const float2*)(B + (kok ? gk : 0) * ldb);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int64_t gj = n0 + bc + 2 * q;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gk < K) {
-                const float2* p = (const float2*)(B + gk * ldb + gj);
-                if (gj < N) {
-                    float2 u = p[0];
-                    v.x = u.x;
-                    v.y = u.y;
-                }
-                if (gj + 1 < N) {
-                    float2 u = p[1];
-                    v.z = u.x;
-                    v.w = u.y;
-                }
-            }
-            rb[q] = v;
+        for (int q = 0; q < BPT; ++q) {
+            const int64_t gj = n0 + bc + q;
+            rb[q] = (kok && gj < N) ? pb[gj] : make_float2(0.f, 0.f);
         }
     };
     auto store_chunk = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            Are[buf][ak + 2 * q][ar] = ra[q].x;
-            Aim[buf][ak + 2 * q][ar] = ra[q].y;
-            Are[buf][ak + 2 * q + 1][ar] = ra[q].z;
-            Aim[buf][ak + 2 * q + 1][ar] = ra[q].w;
+        for (int q = 0; q < 8; ++q) {
+            Are[buf][ak + q][ar] = ra[q].x;
+            Aim[buf][ak + q][ar] = ra[q].y;
         }
 #pragma unroll
-        for (int q = 0; q < 2; ++q) *(float4*)&Bs[buf][bkr][2 * (bc + 2 * q)] = rb[q];
+        for (int q = 0; q < BPT; q += 2)
+            *(float4*)&Bs[buf][bkr][2 * (bc + q)] = make_float4(rb[q].x, rb[q].y, rb[q + 1].x, rb[q + 1].y);
     };
 
-    floatx16 acc[2][2];
+    floatx16 acc[2][WN];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < WN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
@@ -123,6 +131,21 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
     const int bcol = hi ? (jj ^ 1) : jj;
     const float bsign = (hi && !(jj & 1)) ? -1.f : 1.f;
     const int nchunks = (int)((K + G32_BK - 1) / G32_BK);
+    // C tile prefetched into registers so its HBM latency overlaps the MFMA loop
+    float cpre[2][WN][16];
+    if (beta != 0.f) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < WN; ++j) {
+                const int64_t gjr = 2 * (n0 + wn * 16 * WN + j * 16) + jj;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                    cpre[i][j][r] = (gi < M && gjr < 2 * N) ? ((const float*)(C + gi * ldc))[gjr] : 0.f;
+                }
+            }
+    }
     load_chunk(0);
     store_chunk(0);
     __syncthreads();
@@ -131,18 +154,18 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
         if (c + 1 < nchunks) load_chunk((int64_t)(c + 1) * G32_BK);
 #pragma unroll
         for (int kk = 0; kk < G32_BK; ++kk) {
-            float a[2], b[2];
+            float a[2], b[WN];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 const int row = wm * 64 + i * 32 + jj;
                 a[i] = hi ? Aim[buf][kk][row] : Are[buf][kk][row];
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = bsign * Bs[buf][kk][wn * 64 + j * 32 + bcol];
+            for (int j = 0; j < WN; ++j) b[j] = bsign * Bs[buf][kk][wn * 32 * WN + j * 32 + bcol];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < WN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         if (c + 1 < nchunks) {
@@ -155,8 +178,8 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t gjr = 2 * (n0 + wn * 32 + j * 16) + jj;  // real column
+        for (int j = 0; j < WN; ++j) {
+            const int64_t gjr = 2 * (n0 + wn * 16 * WN + j * 16) + jj;  // real column
             if (gjr >= 2 * N) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
@@ -164,11 +187,38 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
                 if (gi < M) {
                     float* p = (float*)(C + gi * ldc) + gjr;
                     float v = alpha * acc[i][j][r];
-                    if (beta != 0.f) v += beta * *p;
+                    if (beta != 0.f) v += beta * cpre[i][j][r];
                     *p = v;
                 }
             }
         }
+}
+
+// FP32 variants: 0 = 128x64 tile, 1 = 128x64 XCD-ordered, 2 = 128x128, 3 = 128x128 XCD-ordered
+template <int WN, bool XCD>
+int launch_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
+                     const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
+{
+    const int64_t BN = 32 * WN;
+    const int64_t tiles = ((N + BN - 1) / BN) * ((M + G32_BM - 1) / G32_BM);
+    const int64_t blocks = XCD ? ((tiles + NUM_XCD - 1) / NUM_XCD) * NUM_XCD : tiles;
+    hipLaunchKernelGGL((k_cgemm_f32<WN, XCD>), dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, alpha,
+                       (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, beta, (cpx<float>*)C, ldc);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+int g_gemm_f32_variant = 0;  // tile choice of the FP32 GEMM (set by mba_gemm_set_variant)
+
+int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
+              const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
+{
+    switch (variant) {
+    case 1: return launch_cgemm_f32<2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    case 2: return launch_cgemm_f32<4, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    case 3: return launch_cgemm_f32<4, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    default: return launch_cgemm_f32<2, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    }
 }
 
 // FP64: v_mfma_f64_16x16x4f64, one instruction = 16 rows x 8 complex columns
@@ -288,12 +338,7 @@ int gemm<float>(int64_t M, int64_t N, int64_t K, double alpha, const void* A, in
                 const void* B, int64_t ldb, double beta, void* C, int64_t ldc, hipStream_t st)
 {
     if (M <= 0 || N <= 0) return MBA_OK;
-    dim3 grid((unsigned)((N + G32_BN - 1) / G32_BN), (unsigned)((M + G32_BM - 1) / G32_BM));
-    hipLaunchKernelGGL(k_cgemm_f32, grid, dim3(256), 0, st, M, N, K, (float)alpha,
-                       (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, (float)beta,
-                       (cpx<float>*)C, ldc);
-    MBA_LAUNCH_CHECK();
-    return MBA_OK;
+    return cgemm_f32(g_gemm_f32_variant, M, N, K, (float)alpha, A, lda, B, ldb, (float)beta, C, ldc, st);
 }
 
 template <>
@@ -845,8 +890,20 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     MBA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, info, 1);
     MBA_LAUNCH_CHECK();
-    for (int64_t k = 0; k < n; k += NB) {
-        const int nb = (int)((n - k) < NB ? (n - k) : NB);
+    // A[r0:r0+M, c0:c0+N] -= A[r0:r0+M, k0:k0+kk] A[k0:k0+kk, c0:c0+N]
+    auto do_gemm_rect = [&](int64_t r0, int64_t M, int64_t c0, int64_t N, int64_t k0, int64_t kk) -> int {
+        if (M <= 0 || N <= 0 || kk <= 0) return MBA_OK;
+        tm.mark(3);
+        int rc = gemm<T>(M, N, kk, -1.0, A + r0 * lda + k0, lda, A + k0 * lda + c0, lda, 1.0,
+                         A + r0 * lda + c0, lda, st);
+        if (rc != MBA_OK) return rc;
+        tm.mark(3);
+        gflops += 8.0 * (double)M * (double)N * (double)kk;
+        return MBA_OK;
+    };
+    // One panel of nb columns at k: factor, then apply its row interchanges to
+    // the column ranges [0, k) and [k + nb, swap_hi).
+    auto do_panel = [&](int64_t k, int nb, int64_t swap_hi) -> int {
         const int64_t m = n - k;
         int G = (int)((m + 7) / 8);
         if (G > cus) G = cus;
@@ -874,35 +931,57 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                                        (unsigned)shm, st) != hipSuccess)
             return MBA_ERR_LAUNCH;
         tm.mark(0);
-        const int64_t n2 = n - k - nb;
-        if (k > 0 || n2 > 0) {
+        const int64_t right = swap_hi - (k + nb);
+        if (k > 0 || right > 0) {
             tm.mark(1);
             hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(64), 0, st, ipiv, k, nb, ws.slot_row,
                                ws.plan, ws.plan_n);
             MBA_LAUNCH_CHECK();
-            const int64_t ncols = n - nb;
-            const int64_t blocks = (k + LASWP_COLS - 1) / LASWP_COLS + (n2 + LASWP_COLS - 1) / LASWP_COLS;
-            // blocks enumerate [0, k) then [k+nb, n); the kernel skips the panel range
+            const int64_t blocks = (k + LASWP_COLS - 1) / LASWP_COLS + (right + LASWP_COLS - 1) / LASWP_COLS;
+            // blocks enumerate [0, k) then [k+nb, swap_hi); the kernel skips the panel range
             hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)blocks), dim3(256),
-                               (size_t)2 * nb * LASWP_COLS * esz, st, A, lda, (int64_t)0, n,
+                               (size_t)2 * nb * LASWP_COLS * esz, st, A, lda, (int64_t)0, swap_hi,
                                k, k + nb, nb, ws.slot_row, ws.plan, ws.plan_n);
             MBA_LAUNCH_CHECK();
             tm.mark(1);
-            (void)ncols;
         }
-        if (n2 > 0) {
-            tm.mark(2);
-            hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n2 + TRSM_COLS - 1) / TRSM_COLS)),
-                               dim3(256), (size_t)nb * TRSM_COLS * esz, st, A, lda, k, nb, k + nb, n);
-            MBA_LAUNCH_CHECK();
-            tm.mark(2);
-            tm.mark(3);
-            int rc = gemm<T>(n2, n2, nb, -1.0, A + (k + nb) * lda + k, lda, A + k * lda + k + nb, lda,
-                             1.0, A + (k + nb) * lda + k + nb, lda, st);
-            if (rc != MBA_OK) return rc;
-            tm.mark(3);
-            gflops += 8.0 * (double)n2 * (double)n2 * (double)nb;
-        }
+        return MBA_OK;
+    };
+    // rows [r0, r0 + nb) x columns [c_lo, n): X = L(r0 block)^{-1} X
+    auto do_trsm = [&](int64_t r0, int nb, int64_t c_lo) -> int {
+        if (c_lo >= n) return MBA_OK;
+        tm.mark(2);
+        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n - c_lo + TRSM_COLS - 1) / TRSM_COLS)),
+                           dim3(256), (size_t)nb * TRSM_COLS * esz, st, A, lda, r0, nb, c_lo, n);
+        MBA_LAUNCH_CHECK();
+        tm.mark(2);
+        return MBA_OK;
+    };
+    // Super panels of 2 NB columns: panel 1, its U rows over the whole trailing
+    // width, an update of panel 2's columns only, panel 2 (its interchanges
+    // applied everywhere, including panel 1's L columns), the delayed update of
+    // panel 2's U rows, their triangular solve, then one trailing GEMM with
+    // k = 2 NB.  Row interchanges commute with the delayed (linear) update
+    // because they are applied to every column before it.
+    for (int64_t k = 0; k < n; k += 2 * NB) {
+        const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+        const int64_t k2 = k + nb1;
+        const int nb2 = (int)((n - k2) < NB ? ((n - k2) > 0 ? (n - k2) : 0) : NB);
+        int rc = do_panel(k, nb1, n);
+        if (rc != MBA_OK) return rc;
+        if (k2 >= n) break;
+        if ((rc = do_trsm(k, nb1, k2)) != MBA_OK) return rc;
+        // panel 2 columns, rows below panel 1: -= L21a U12a
+        if ((rc = do_gemm_rect(k2, n - k2, k2, nb2, k, nb1)) != MBA_OK) return rc;
+        rc = do_panel(k2, nb2, n);
+        if (rc != MBA_OK) return rc;
+        const int64_t c3 = k2 + nb2;
+        if (c3 >= n) break;
+        // panel 2's U rows: delayed update by panel 1, then the triangular solve
+        if ((rc = do_gemm_rect(k2, nb2, c3, n - c3, k, nb1)) != MBA_OK) return rc;
+        if ((rc = do_trsm(k2, nb2, c3)) != MBA_OK) return rc;
+        // trailing update with both panels: k = nb1 + nb2
+        if ((rc = do_gemm_rect(c3, n - c3, c3, n - c3, k, nb1 + nb2)) != MBA_OK) return rc;
     }
     int herr = 0;
     if (hipMemcpyAsync(&herr, ws.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return MBA_ERR_LAUNCH;
@@ -1257,6 +1336,26 @@ int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void
     if (lda < k || ldb < n || ldc < n) return MBA_ERR_ARG;
     if (prec == MBA_F32) return gemm<float>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, S(stream));
     return gemm<double>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, S(stream));
+}
+
+int mba_gemm_set_variant(int prec, int variant)
+{
+    if (prec != MBA_F32) return MBA_ERR_ARG;
+    if (variant < 0 || variant > 3) return MBA_ERR_ARG;
+    g_gemm_f32_variant = variant;
+    return MBA_OK;
+}
+
+int mba_gemm_variant(int prec, int variant, int64_t m, int64_t n, int64_t k, double alpha,
+                     const void* a, int64_t lda, const void* b, int64_t ldb, double beta, void* c,
+                     int64_t ldc, mba_stream_t stream)
+{
+    if (!a || !b || !c) return MBA_ERR_ARG;
+    if (prec != MBA_F32 || variant < 0 || variant > 3) return MBA_ERR_ARG;
+    if (m < 0 || n < 0 || k < 0) return MBA_ERR_SIZE;
+    if (lda < k || ldb < n || ldc < n) return MBA_ERR_ARG;
+    if (m == 0 || n == 0) return MBA_OK;
+    return cgemm_f32(variant, m, n, k, (float)alpha, a, lda, b, ldb, (float)beta, c, ldc, S(stream));
 }
 
 int mba_residual_c128(const void* z, int64_t n, int64_t ldz, const void* x, const void* v, void* r,

@@ -46,6 +46,9 @@ SIGNATURES = {
     "mba_lu_solve": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _vp, _i64, _i64, _vp, _sz, _vp]),
     "mba_gemm": (_c.c_int, [_c.c_int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64,
                             _vp]),
+    "mba_gemm_variant": (_c.c_int, [_c.c_int, _c.c_int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl,
+                                    _vp, _i64, _vp]),
+    "mba_gemm_set_variant": (_c.c_int, [_c.c_int, _c.c_int]),
     "mba_residual_c128": (_c.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mba_gemv": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _vp]),
     "mba_convert_c128_c64": (_c.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp]),
