@@ -157,11 +157,18 @@ int mba_lu_factor_profiled(int prec, void *a, int64_t n, int64_t lda, int32_t *i
  * (the net effect of the sequential swaps in ipiv). */
 int mba_lu_pivots_to_perm(const int32_t *ipiv, int64_t n, int32_t *perm, mba_stream_t stream);
 
-/* Solve A x = b in place for nrhs right-hand sides b [n, ldb] complex, with
- * (lu, perm) from mba_lu_factor / mba_lu_pivots_to_perm (the getrs inside
+/* Solve aid computed once per factorisation: the row permutation and the
+ * inverses of the 64 x 64 diagonal blocks of L and U.  aux >=
+ * mba_lu_solve_aux_bytes(prec, n). */
+size_t mba_lu_solve_aux_bytes(int prec, int64_t n);
+int mba_lu_solve_prepare(int prec, const void *lu, int64_t n, int64_t lda, const int32_t *ipiv,
+                         void *aux, size_t aux_bytes, mba_stream_t stream);
+
+/* Solve A x = b in place for nrhs right-hand sides b [n, ldb] complex, with lu
+ * from mba_lu_factor and aux from mba_lu_solve_prepare (the getrs inside
  * torch.linalg.solve).  workspace >= mba_lu_solve_workspace_bytes. */
 size_t mba_lu_solve_workspace_bytes(int prec, int64_t n, int64_t nrhs);
-int mba_lu_solve(int prec, const void *lu, int64_t n, int64_t lda, const int32_t *perm, void *b,
+int mba_lu_solve(int prec, const void *lu, int64_t n, int64_t lda, const void *aux, void *b,
                  int64_t nrhs, int64_t ldb, void *workspace, size_t workspace_bytes,
                  mba_stream_t stream);
 

@@ -705,9 +705,12 @@ __global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t l
 
 // ============================================================== TRSM (L unit lower)
 // X = L11^{-1} X for X = rows [k, k+nb) x columns [c0, c0 + 64) of A, L11 the
-// unit lower panel block.  X is staged in LDS; forward substitution row by
-// row with 4 partial dot products per column.
+// unit lower panel block.  X is staged in LDS and solved in blocks of 16 rows:
+// each block first subtracts the contribution of the solved blocks above it
+// (thread = (row, 4 columns)), then a 16 x 16 forward substitution runs with
+// one thread per column -- two barriers per block instead of two per row.
 constexpr int TRSM_COLS = 64;
+constexpr int TRSM_RB = 16;
 
 template <class T>
 __global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t lda, int64_t k,
@@ -715,7 +718,6 @@ __global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t 
 {
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* X = (cpx<T>*)smem_raw;  // [nb][TRSM_COLS]
-    __shared__ cpx<T> part[4][TRSM_COLS];
     const int tid = threadIdx.x;
     const int64_t c0 = col_begin + (int64_t)blockIdx.x * TRSM_COLS;
     const int w = (int)((col_end - c0) < TRSM_COLS ? (col_end - c0) : TRSM_COLS);
@@ -728,24 +730,42 @@ __global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t 
         X[idx] = v;
     }
     __syncthreads();
-    const int c = tid % TRSM_COLS, qd = tid / TRSM_COLS;
-    for (int r = 1; r < nb; ++r) {
-        const cpx<T>* Lr = A + (k + r) * lda + k;
-        T sr = 0, si = 0;
-        for (int t = qd; t < r; t += 4) {
-            cpx<T> l = Lr[t];
-            cpx<T> x = X[t * TRSM_COLS + c];
-            sr += l.re * x.re - l.im * x.im;
-            si += l.re * x.im + l.im * x.re;
+    const int rr = tid >> 4, cg = (tid & 15) * 4;  // update role: row rr of the block, columns cg..cg+3
+    for (int b0 = 0; b0 < nb; b0 += TRSM_RB) {
+        const int rb = (nb - b0) < TRSM_RB ? (nb - b0) : TRSM_RB;
+        if (b0 > 0 && rr < rb) {
+            const cpx<T>* Lr = A + (k + b0 + rr) * lda + k;
+            T sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+            for (int t = 0; t < b0; ++t) {
+                const cpx<T> l = Lr[t];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const cpx<T> x = X[t * TRSM_COLS + cg + q];
+                    sr[q] += l.re * x.re - l.im * x.im;
+                    si[q] += l.re * x.im + l.im * x.re;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cpx<T> x = X[(b0 + rr) * TRSM_COLS + cg + q];
+                x.re -= sr[q];
+                x.im -= si[q];
+                X[(b0 + rr) * TRSM_COLS + cg + q] = x;
+            }
         }
-        part[qd][c].re = sr;
-        part[qd][c].im = si;
         __syncthreads();
-        if (qd == 0) {
-            cpx<T> x = X[r * TRSM_COLS + c];
-            x.re -= (part[0][c].re + part[1][c].re) + (part[2][c].re + part[3][c].re);
-            x.im -= (part[0][c].im + part[1][c].im) + (part[2][c].im + part[3][c].im);
-            X[r * TRSM_COLS + c] = x;
+        if (tid < TRSM_COLS) {
+            const int c = tid;
+            for (int i = 1; i < rb; ++i) {
+                const cpx<T>* Lr = A + (k + b0 + i) * lda + k + b0;
+                cpx<T> x = X[(b0 + i) * TRSM_COLS + c];
+                for (int t = 0; t < i; ++t) {
+                    const cpx<T> l = Lr[t], y = X[(b0 + t) * TRSM_COLS + c];
+                    x.re -= l.re * y.re - l.im * y.im;
+                    x.im -= l.re * y.im + l.im * y.re;
+                }
+                X[(b0 + i) * TRSM_COLS + c] = x;
+            }
         }
         __syncthreads();
     }
@@ -1130,6 +1150,254 @@ __global__ __launch_bounds__(256) void k_block_update(const cpx<T>* __restrict__
     }
 }
 
+// ---- fused blocked triangular solves (nrhs <= TS_RHS) ----------------------
+// Diagonal blocks of TS_BS rows are inverted once per factorisation
+// (k_tri_inv); a solve is then one launch per block: workgroup 0 updates the
+// next block's rows with the current block's solution and applies that
+// block's inverse, the other workgroups update the remaining rows (one wave
+// per row).  Lower: unit diagonal, forward; upper: backward.
+constexpr int TS_BS = 64;
+constexpr int TS_RHS = 4;
+
+template <class T, bool LOWER>
+__global__ __launch_bounds__(256) void k_tri_inv(const cpx<T>* __restrict__ A, int64_t lda, int64_t n,
+                                                 cpx<T>* __restrict__ inv)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cpx<T>* Lt = (cpx<T>*)smem_raw;          // [TS_BS][TS_BS + 1], Lt[t][i] = A[r0+i][r0+t]
+    cpx<T>* X = Lt + TS_BS * (TS_BS + 1);    // [TS_BS][TS_BS + 1], X[i][c]
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * TS_BS;
+    const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
+    constexpr int LD = TS_BS + 1;
+    for (int idx = tid; idx < TS_BS * TS_BS; idx += 256) {
+        const int i = idx / TS_BS, t = idx % TS_BS;
+        cpx<T> v;
+        v.re = 0;
+        v.im = 0;
+        if (i < bs && t < bs) v = A[(r0 + i) * lda + r0 + t];
+        Lt[t * LD + i] = v;
+        cpx<T> e;
+        e.re = (i == t && i < bs) ? (T)1 : (T)0;
+        e.im = 0;
+        X[i * LD + t] = e;
+    }
+    __syncthreads();
+    for (int s = 0; s < bs; ++s) {
+        const int t = LOWER ? s : bs - 1 - s;
+        if (!LOWER) {
+            if (tid < bs) {
+                cpx<T> d = Lt[t * LD + t];
+                cpx<T> x = X[t * LD + tid];
+                T dd = d.re * d.re + d.im * d.im;
+                cpx<T> q;
+                q.re = (x.re * d.re + x.im * d.im) / dd;
+                q.im = (x.im * d.re - x.re * d.im) / dd;
+                X[t * LD + tid] = q;
+            }
+            __syncthreads();
+        }
+        for (int idx = tid; idx < bs * bs; idx += 256) {
+            const int i = idx / bs, c = idx % bs;
+            if (LOWER ? (i > t) : (i < t)) {
+                const cpx<T> l = Lt[t * LD + i], xt = X[t * LD + c];
+                cpx<T> x = X[i * LD + c];
+                x.re -= l.re * xt.re - l.im * xt.im;
+                x.im -= l.re * xt.im + l.im * xt.re;
+                X[i * LD + c] = x;
+            }
+        }
+        __syncthreads();
+    }
+    cpx<T>* out = inv + (int64_t)blockIdx.x * TS_BS * TS_BS;
+    for (int idx = tid; idx < TS_BS * TS_BS; idx += 256) out[idx] = X[(idx / TS_BS) * LD + idx % TS_BS];
+}
+
+// Tile of up to 64 rows handled by one workgroup: thread = (row tid/4, part
+// tid%4); each part covers 16 consecutive columns (a 128/256-byte segment of
+// the row) and the four partial sums are combined with two lane shuffles.
+// out[i][c] = base[i][c] - sum_q M[i][q] v[q][c]  (sub = true)
+// out[i][c] =               sum_q M[i][q] v[q][c]  (sub = false)
+template <class T, bool SUB>
+__device__ __forceinline__ void tile_matvec(const cpx<T>* __restrict__ M, int64_t ldm, int64_t row0,
+                                            int nrows, int ncols, const cpx<T>* __restrict__ v,
+                                            cpx<T>* __restrict__ out, int nrhs)
+{
+    const int r = threadIdx.x >> 2, part = threadIdx.x & 3;
+    const bool active = r < nrows;
+    const cpx<T>* Mr = M + (active ? (int64_t)r : 0) * ldm;
+    cpx<T> m[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int col = part * 16 + q;
+        if (active && col < ncols) {
+            m[q] = Mr[col];
+        } else {
+            m[q].re = 0;
+            m[q].im = 0;
+        }
+    }
+    for (int c = 0; c < nrhs; ++c) {
+        T sr = 0, si = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = part * 16 + q;
+            if (col < ncols) {
+                const cpx<T> y = v[col * nrhs + c];
+                sr += m[q].re * y.re - m[q].im * y.im;
+                si += m[q].re * y.im + m[q].im * y.re;
+            }
+        }
+        sr += __shfl_xor(sr, 1);
+        si += __shfl_xor(si, 1);
+        sr += __shfl_xor(sr, 2);
+        si += __shfl_xor(si, 2);
+        if (active && part == 0) {
+            cpx<T>* o = out + ((int64_t)row0 + r) * nrhs + c;
+            if (SUB) {
+                cpx<T> b = *o;
+                b.re -= sr;
+                b.im -= si;
+                *o = b;
+            } else {
+                store_cpx(o, sr, si);
+            }
+        }
+    }
+}
+
+// LOWER: current block [r0, r0 + bs) has its solution in x; next block
+// [r0 + bs, r0 + bs + 64).  UPPER: current block [r0, r0 + bs); next block
+// [r0 - 64, r0).  Workgroup 0 updates the next block and applies its inverse;
+// workgroup b >= 1 updates the b-th 64-row tile of the remaining rows.
+// first != 0: only x of block r0 = inv * t.
+template <class T, bool LOWER>
+__global__ __launch_bounds__(256) void k_tri_step(const cpx<T>* __restrict__ A, int64_t lda, int64_t n,
+                                                  int64_t r0, int bs, const cpx<T>* __restrict__ inv,
+                                                  cpx<T>* __restrict__ t, cpx<T>* __restrict__ x,
+                                                  int nrhs, int first)
+{
+    if (first) {
+        if (blockIdx.x == 0)
+            tile_matvec<T, false>(inv + (r0 / TS_BS) * TS_BS * TS_BS, TS_BS, r0, bs, bs, t + r0 * nrhs, x,
+                                  nrhs);
+        return;
+    }
+    int64_t nx0, nx1, rest0, rest1;  // next block rows, remaining rows
+    if (LOWER) {
+        nx0 = r0 + bs;
+        nx1 = nx0 + TS_BS < n ? nx0 + TS_BS : n;
+        rest0 = nx1;
+        rest1 = n;
+    } else {
+        nx1 = r0;
+        nx0 = r0 - TS_BS > 0 ? r0 - TS_BS : 0;
+        rest0 = 0;
+        rest1 = nx0;
+    }
+    const cpx<T>* xc = x + r0 * nrhs;
+    if (blockIdx.x == 0) {
+        const int nn = (int)(nx1 - nx0);
+        tile_matvec<T, true>(A + nx0 * lda + r0, lda, nx0, nn, bs, xc, t, nrhs);
+        __syncthreads();
+        tile_matvec<T, false>(inv + (nx0 / TS_BS) * TS_BS * TS_BS, TS_BS, nx0, nn, nn, t + nx0 * nrhs, x,
+                              nrhs);
+        return;
+    }
+    const int64_t i0 = rest0 + (int64_t)(blockIdx.x - 1) * 64;
+    if (i0 >= rest1) return;
+    const int nn = (int)((rest1 - i0) < 64 ? (rest1 - i0) : 64);
+    tile_matvec<T, true>(A + i0 * lda + r0, lda, i0, nn, bs, xc, t, nrhs);
+}
+
+size_t solve_aux_bytes(int prec, int64_t n)
+{
+    const size_t esz = prec == MBA_F32 ? 8 : 16;
+    const int64_t nblk = (n + TS_BS - 1) / TS_BS;
+    return al256((size_t)n * 4) + 2 * al256((size_t)nblk * TS_BS * TS_BS * esz);
+}
+
+template <class T>
+int lu_prepare(const void* lu, int64_t n, int64_t lda, const int32_t* ipiv, void* aux, hipStream_t st)
+{
+    const size_t esz = sizeof(cpx<T>);
+    const int64_t nblk = (n + TS_BS - 1) / TS_BS;
+    int32_t* perm = (int32_t*)aux;
+    cpx<T>* linv = (cpx<T>*)((char*)aux + al256((size_t)n * 4));
+    cpx<T>* uinv = (cpx<T>*)((char*)linv + al256((size_t)nblk * TS_BS * TS_BS * esz));
+    hipLaunchKernelGGL(k_ipiv_perm, dim3(1), dim3(64), 0, st, ipiv, n, perm);
+    MBA_LAUNCH_CHECK();
+    const size_t shm = 2 * (size_t)TS_BS * (TS_BS + 1) * esz;
+    static bool attr_set[2] = {false, false};
+    const int pi = sizeof(T) == 4 ? 0 : 1;
+    if (!attr_set[pi]) {
+        (void)hipFuncSetAttribute((const void*)k_tri_inv<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        (void)hipFuncSetAttribute((const void*)k_tri_inv<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        attr_set[pi] = true;
+    }
+    hipLaunchKernelGGL((k_tri_inv<T, true>), dim3((unsigned)nblk), dim3(256), shm, st, (const cpx<T>*)lu, lda, n, linv);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_tri_inv<T, false>), dim3((unsigned)nblk), dim3(256), shm, st, (const cpx<T>*)lu, lda, n, uinv);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+// Solve with the prepared aux (perm + inverted diagonal blocks), nrhs <= TS_RHS.
+template <class T>
+int lu_solve_fused(const void* lu, int64_t n, int64_t lda, const void* aux, void* b, int64_t nrhs,
+                   int64_t ldb, void* ws_ptr, size_t ws_bytes, hipStream_t st)
+{
+    const size_t esz = sizeof(cpx<T>);
+    if (ws_bytes < 2 * al256((size_t)n * nrhs * esz)) return MBA_ERR_ALLOC;
+    const int64_t nblk = (n + TS_BS - 1) / TS_BS;
+    const int32_t* perm = (const int32_t*)aux;
+    const cpx<T>* linv = (const cpx<T>*)((const char*)aux + al256((size_t)n * 4));
+    const cpx<T>* uinv = (const cpx<T>*)((const char*)linv + al256((size_t)nblk * TS_BS * TS_BS * esz));
+    cpx<T>* t = (cpx<T>*)ws_ptr;
+    cpx<T>* x = (cpx<T>*)((char*)ws_ptr + al256((size_t)n * nrhs * esz));
+    const cpx<T>* A = (const cpx<T>*)lu;
+    const int64_t tot = n * nrhs;
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cpx<T>*)b, ldb, perm, n, nrhs, t, nrhs);
+    MBA_LAUNCH_CHECK();
+    const int nr = (int)nrhs;
+    // forward: L y = P b  (y in x)
+    for (int64_t bi = 0; bi < nblk; ++bi) {
+        const int64_t r0 = bi * TS_BS;
+        const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
+        if (bi == 0)
+            hipLaunchKernelGGL((k_tri_step<T, true>), dim3(1), dim3(256), 0, st, A, lda, n, r0, bs,
+                               linv, t, x, nr, 1);
+        const int64_t rest = n - (r0 + bs) - TS_BS;
+        const int64_t blocks = 1 + (rest > 0 ? (rest + 63) / 64 : 0);
+        if (r0 + bs < n)
+            hipLaunchKernelGGL((k_tri_step<T, true>), dim3((unsigned)blocks), dim3(256), 0, st, A, lda, n,
+                               r0, bs, linv, t, x, nr, 0);
+        MBA_LAUNCH_CHECK();
+    }
+    // backward: U z = y  (t <- y, result in x)
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cpx<T>*)x, nrhs, nullptr, n, nrhs, t, nrhs);
+    MBA_LAUNCH_CHECK();
+    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+        const int64_t r0 = bi * TS_BS;
+        const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
+        if (bi == nblk - 1)
+            hipLaunchKernelGGL((k_tri_step<T, false>), dim3(1), dim3(256), 0, st, A, lda, n, r0, bs,
+                               uinv, t, x, nr, 1);
+        const int64_t rest = r0 - TS_BS;
+        const int64_t blocks = 1 + (rest > 0 ? (rest + 63) / 64 : 0);
+        if (r0 > 0)
+            hipLaunchKernelGGL((k_tri_step<T, false>), dim3((unsigned)blocks), dim3(256), 0, st, A, lda, n,
+                               r0, bs, uinv, t, x, nr, 0);
+        MBA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_permute_rows<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cpx<T>*)x, nrhs, nullptr, n, nrhs, (cpx<T>*)b, ldb);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
 template <class T>
 int lu_solve(const void* lu, int64_t n, int64_t lda, const int32_t* perm, void* b, int64_t nrhs,
              int64_t ldb, void* ws_ptr, size_t ws_bytes, hipStream_t st)
@@ -1311,17 +1579,41 @@ int mba_lu_pivots_to_perm(const int32_t* ipiv, int64_t n, int32_t* perm, mba_str
 size_t mba_lu_solve_workspace_bytes(int prec, int64_t n, int64_t nrhs)
 {
     if ((prec != MBA_F32 && prec != MBA_F64) || n <= 0 || nrhs <= 0) return 0;
-    return (size_t)n * nrhs * (prec == MBA_F32 ? 8 : 16);
+    return 2 * al256((size_t)n * nrhs * (prec == MBA_F32 ? 8 : 16));
 }
 
-int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const int32_t* perm, void* b,
+size_t mba_lu_solve_aux_bytes(int prec, int64_t n)
+{
+    if ((prec != MBA_F32 && prec != MBA_F64) || n <= 0) return 0;
+    return solve_aux_bytes(prec, n);
+}
+
+int mba_lu_solve_prepare(int prec, const void* lu, int64_t n, int64_t lda, const int32_t* ipiv, void* aux,
+                         size_t aux_bytes, mba_stream_t stream)
+{
+    if (!lu || !ipiv || !aux) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+    if (lda < n) return MBA_ERR_ARG;
+    if (aux_bytes < solve_aux_bytes(prec, n)) return MBA_ERR_ALLOC;
+    if (prec == MBA_F32) return lu_prepare<float>(lu, n, lda, ipiv, aux, S(stream));
+    return lu_prepare<double>(lu, n, lda, ipiv, aux, S(stream));
+}
+
+int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const void* aux, void* b,
                  int64_t nrhs, int64_t ldb, void* workspace, size_t workspace_bytes,
                  mba_stream_t stream)
 {
-    if (!lu || !perm || !b || !workspace) return MBA_ERR_ARG;
+    if (!lu || !aux || !b || !workspace) return MBA_ERR_ARG;
     if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
     if (n <= 0 || nrhs <= 0) return MBA_ERR_SIZE;
     if (lda < n || ldb < nrhs) return MBA_ERR_ARG;
+    const int32_t* perm = (const int32_t*)aux;
+    if (nrhs <= TS_RHS) {
+        if (prec == MBA_F32)
+            return lu_solve_fused<float>(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
+        return lu_solve_fused<double>(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
+    }
     if (prec == MBA_F32)
         return lu_solve<float>(lu, n, lda, perm, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
     return lu_solve<double>(lu, n, lda, perm, b, nrhs, ldb, workspace, workspace_bytes, S(stream));

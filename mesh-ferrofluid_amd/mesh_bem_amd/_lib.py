@@ -43,7 +43,9 @@ SIGNATURES = {
                                           _vp, _vp]),
     "mba_lu_pivots_to_perm": (_c.c_int, [_i32p, _i64, _i32p, _vp]),
     "mba_lu_solve_workspace_bytes": (_sz, [_c.c_int, _i64, _i64]),
-    "mba_lu_solve": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _vp, _i64, _i64, _vp, _sz, _vp]),
+    "mba_lu_solve_aux_bytes": (_sz, [_c.c_int, _i64]),
+    "mba_lu_solve_prepare": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _vp, _sz, _vp]),
+    "mba_lu_solve": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _i64, _i64, _vp, _sz, _vp]),
     "mba_gemm": (_c.c_int, [_c.c_int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl, _vp, _i64,
                             _vp]),
     "mba_gemm_variant": (_c.c_int, [_c.c_int, _c.c_int, _i64, _i64, _i64, _dbl, _vp, _i64, _vp, _i64, _dbl,

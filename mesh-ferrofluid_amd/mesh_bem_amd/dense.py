@@ -51,7 +51,6 @@ class LUFactor:
         self.prec = _lib.prec_of(A.dtype)
         dev = A.device
         self.ipiv = torch.empty(n, dtype=torch.int32, device=dev)
-        self.perm = torch.empty(n, dtype=torch.int32, device=dev)
         info = torch.zeros(1, dtype=torch.int32, device=dev)
         wsb = lib.mba_lu_workspace_bytes(self.prec, n)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
@@ -76,8 +75,12 @@ class LUFactor:
                                    _lib.ptr(self.ipiv), _lib.ptr(info), _lib.ptr(ws), wsb,
                                    _lib.stream())
             _lib.check(st, "mba_lu_factor")
-        st = lib.mba_lu_pivots_to_perm(_lib.ptr(self.ipiv), n, _lib.ptr(self.perm), _lib.stream())
-        _lib.check(st, "mba_lu_pivots_to_perm")
+        auxb = lib.mba_lu_solve_aux_bytes(self.prec, n)
+        self.aux = torch.empty(auxb, dtype=torch.uint8, device=dev)
+        st = lib.mba_lu_solve_prepare(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0),
+                                      _lib.ptr(self.ipiv), _lib.ptr(self.aux), auxb, _lib.stream())
+        _lib.check(st, "mba_lu_solve_prepare")
+        self.perm = self.aux[: 4 * n].view(torch.int32)
         self.info = int(info.item())
         if self.info != 0:
             raise _lib.MBAError(f"mba_lu_factor: exactly zero pivot in column {self.info - 1}")
@@ -91,7 +94,7 @@ class LUFactor:
         wsb = lib.mba_lu_solve_workspace_bytes(self.prec, self.n, nrhs)
         ws = torch.empty(wsb, dtype=torch.uint8, device=X.device)
         st = lib.mba_lu_solve(self.prec, _lib.ptr(self.lu), self.n, self.lu.stride(0),
-                              _lib.ptr(self.perm), _lib.ptr(X), nrhs, X.stride(0), _lib.ptr(ws), wsb,
+                              _lib.ptr(self.aux), _lib.ptr(X), nrhs, X.stride(0), _lib.ptr(ws), wsb,
                               _lib.stream())
         _lib.check(st, "mba_lu_solve")
         return X.reshape(-1) if vec else X
