@@ -62,9 +62,21 @@ __global__ __launch_bounds__(256) void k_laplace_ref(
         int rel = 0;
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) rel += (yi[t][a] == xi[b]);
-        const T* Y = ys[t];
+        const T* Yabs = ys[t];
+        // coordinates relative to the first vertex of fy: R and (x - y).n_y of
+        // near pairs lose no digits to the absolute position
+        T Y[12], Xr[9];
+        for (int q = 0; q < 3; ++q) {
+            Y[q] = (T)0;
+            Y[3 + q] = Yabs[3 + q] - Yabs[q];
+            Y[6 + q] = Yabs[6 + q] - Yabs[q];
+            Y[9 + q] = Yabs[9 + q];
+            for (int v = 0; v < 3; ++v) Xr[3 * v + q] = X[3 * v + q] - Yabs[q];
+        }
         T acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const int64_t r0 = tab_off[rel], r1 = tab_off[rel + 1];
+        // the double layer of a flat face on itself is exactly zero ((x - y)
+        // lies in the plane); summing rounding noise of (x - y).n / R^3 is skipped
+        const int64_t r0 = tab_off[rel], r1 = (layer == 2 && rel == 3) ? r0 : tab_off[rel + 1];
         for (int64_t g = r0; g < r1; ++g) {
             const T r1x = tab[5 * g], r2x = tab[5 * g + 1], r1y = tab[5 * g + 2], r2y = tab[5 * g + 3];
             const T W = tab[5 * g + 4];
@@ -72,8 +84,7 @@ __global__ __launch_bounds__(256) void k_laplace_ref(
             const T py0 = (T)1 - r1y, py1 = r1y - r2y, py2 = r2y;
             T d[3];
             for (int q = 0; q < 3; ++q)
-                d[q] = (px0 * X[q] + px1 * X[3 + q] + px2 * X[6 + q]) -
-                       (py0 * Y[q] + py1 * Y[3 + q] + py2 * Y[6 + q]);
+                d[q] = (px0 * Xr[q] + px1 * Xr[3 + q] + px2 * Xr[6 + q]) - (py1 * Y[3 + q] + py2 * Y[6 + q]);
             const T R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
             const T rinv = (T)1 / sqrt(R2);
             T fv = rinv * inv4pi;
@@ -87,7 +98,7 @@ __global__ __launch_bounds__(256) void k_laplace_ref(
                     for (int ix = 0; ix < 3; ++ix) acc[3 * iy + ix] += wf * pyv[iy] * pxv[ix];
             }
         }
-        const T s = (T)4 * Ax * Y[12];
+        const T s = (T)4 * Ax * Yabs[12];
         T* o = out + ((fy0 + t) * F + fx) * (nphi * nphi);
         for (int q = 0; q < nphi * nphi; ++q) o[q] = s * acc[q];
     }
