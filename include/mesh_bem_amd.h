@@ -43,6 +43,8 @@ enum {
 };
 
 enum { MBA_F32 = 0, MBA_F64 = 1 };
+/* Scalar field of the FP32 GMRES building blocks. */
+enum { MBA_COMPLEX = 0, MBA_REAL = 1 };
 
 /* Human-readable text of a status code. */
 const char *mba_status_string(int status);
@@ -237,6 +239,31 @@ int mba_laplace_p0_assemble(int prec, const void *verts, const int32_t *faces, i
                             int64_t n_faces, const double *rule, int32_t nq, const int64_t *near_ptr,
                             const int32_t *near_idx, void *v_out, void *kt_out, int64_t ld,
                             void *workspace, size_t workspace_bytes, mba_stream_t stream);
+
+/* ================================================================== GMRES
+ * Building blocks of restarted GMRES (no reference counterpart: the reference
+ * solves only with torch.linalg.solve, mesh_BEM_solver_3d.py:174, 346).
+ * field = MBA_COMPLEX (interleaved complex) or MBA_REAL.  Matrices and bases
+ * are FP32; vprec (MBA_F32 / MBA_F64) is the precision of the vectors a call
+ * reads and writes (x, b, y, w, src).  A basis is row-major [nvec, ldb]:
+ * vector j at basis + j * ldb elements. */
+
+/* y = A x, or y = b - A x when b is not null.  A: FP32 [rows, lda] row-major.
+ * Products and sums in FP64. */
+int mba_mv32(int field, const void *a, int64_t rows, int64_t cols, int64_t lda, int vprec,
+             const void *x, const void *b, void *y, mba_stream_t stream);
+/* Workspace of mba_gmres_dots for vectors of length n and up to max_vec basis vectors. */
+size_t mba_gmres_workspace_bytes(int64_t n, int32_t max_vec);
+/* out[j] = basis_j^H w (j < nvec), out[nvec] = (||w||^2, 0); out: double2 [nvec + 1].
+ * Deterministic (fixed reduction order, FP64 across threads).  nvec <= 512. */
+int mba_gmres_dots(int field, const void *basis, int64_t ldb, int32_t nvec, int vprec, const void *w,
+                   int64_t n, void *out, void *workspace, size_t workspace_bytes, mba_stream_t stream);
+/* w += sign * sum_j basis_j coef[j]; coef: double2 [nvec] (device).  nvec <= 512. */
+int mba_gmres_update(int field, const void *basis, int64_t ldb, int32_t nvec, const void *coef,
+                     double sign, int vprec, void *w, int64_t n, mba_stream_t stream);
+/* dst (FP32) = src / sqrt(nrm2[0].re) (zero vector if the norm is zero); nrm2: double2 (device). */
+int mba_gmres_scale(int field, int vprec, const void *src, int64_t n, const void *nrm2, void *dst,
+                    mba_stream_t stream);
 
 #ifdef __cplusplus
 }

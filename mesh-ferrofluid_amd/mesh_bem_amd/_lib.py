@@ -14,6 +14,7 @@ import torch
 from ._build import LIB_PATH
 
 MBA_F32, MBA_F64 = 0, 1
+MBA_COMPLEX, MBA_REAL = 0, 1
 
 _c = ctypes
 _vp, _i32p, _i64p, _dp = _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p
@@ -54,6 +55,11 @@ SIGNATURES = {
     "mba_residual_c128": (_c.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mba_gemv": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _vp]),
     "mba_convert_c128_c64": (_c.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp]),
+    "mba_mv32": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i64, _c.c_int, _vp, _vp, _vp, _vp]),
+    "mba_gmres_workspace_bytes": (_sz, [_i64, _i32]),
+    "mba_gmres_dots": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _c.c_int, _vp, _i64, _vp, _vp, _sz, _vp]),
+    "mba_gmres_update": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _vp, _dbl, _c.c_int, _vp, _i64, _vp]),
+    "mba_gmres_scale": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _vp, _vp, _vp]),
     "mba_laplace_layer_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _i64, _i64, _vp, _i64p, _i32,
                                          _i32, _vp, _vp]),
     "mba_laplace_pack": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
