@@ -37,6 +37,8 @@ def log(msg):
 
 
 def workload(name):
+    if name == "c5":
+        return dict(freq=45, chi=3.0)
     if name == "c2":
         return dict(subdiv=5, k=10.0)
     if name == "c1":
@@ -88,12 +90,136 @@ def cpu_baseline(V, F, k, E, target_s=12.0):
     }
 
 
+def cpu_baseline_c5(F_full, chi, target_s=10.0):
+    """Oracle P0 assembly (FP64) + LAPACK solve on a smaller droplet, scaled
+    by (F/F_s)^2 (assembly) and (F/F_s)^3 (dense solve, the reference's
+    torch.linalg.solve path)."""
+    import numpy as np
+    import scipy.linalg
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from mesh_bem_amd.geometry import noisy_ellipsoid
+    from mesh_bem_amd.mom import near_pairs
+
+    V, Fc, _ = noisy_ellipsoid(0, 24)
+    Fs = len(Fc)
+    ptr, idx = near_pairs(V, Fc)
+    t0 = time.perf_counter()
+    Vm, Km = O.laplace_p0(V, Fc, ptr, idx)
+    t_asm = time.perf_counter() - t0
+    A, n = O.face_areas_normals(V, Fc)
+    t0 = time.perf_counter()
+    scipy.linalg.solve((1 / chi + 0.5) * np.diag(A) + Km, A * n[:, 2], check_finite=False)
+    t_sol = time.perf_counter() - t0
+    t_cpu = t_asm * (F_full / Fs) ** 2 + t_sol * (F_full / Fs) ** 3
+    return {
+        "value": 1e-6 * F_full * F_full / t_cpu,
+        "unit": "Mpairs/s",
+        "cores": len(os.sched_getaffinity(0)),
+        "kind": "port",
+        "sample": (f"oracle FP64 P0 V+K' assembly at F={Fs} ({t_asm:.2f} s, scaled by (F/{Fs})^2) + scipy dgesv "
+                   f"at F={Fs} ({t_sol:.2f} s, scaled by (F/{Fs})^3); extrapolated step {t_cpu:.0f} s"),
+    }
+
+
+def run_c5(args, dev):
+    """Config 5: magnetic droplet loop.  Each step: GPU near-pair lists, FP32
+    P0 V + K' assembly (FP64 analytic near terms), diagonal shift, GMRES
+    (FP32 matrix, FP64 iterate) warm-started from the previous step, face
+    potentials.  Value: face pairs per second of the whole step."""
+    import numpy as np
+    import torch
+
+    from mesh_bem_amd.gmres import DenseOperator
+    from mesh_bem_amd.magnetic import MagneticSolver, droplet_steps
+
+    freq = 45 if args.workload == "c5" else 15
+    chi = 3.0
+    steps = args.steps
+    ms = MagneticSolver(chi, tol=1e-6, restart=50)
+    gen = droplet_steps(0, freq, steps + args.warmup)
+    prev = None
+    rec = []
+    nF = None
+    log(f"workload {args.workload}: geodesic frequency {freq}, {steps} steps after {args.warmup} warm-up")
+    for s, V, Fc, c in gen:
+        v = torch.tensor(V, device=dev)
+        f = torch.tensor(Fc, dtype=torch.int32, device=dev)
+        nF = len(Fc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = ms.solve(v, f, x0=prev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prev = out.sigma
+        if s >= args.warmup:
+            rec.append((t1 - t0, out.assembly_s, out.solve_s, out.info.iterations, out.info.residual))
+    t = np.array([r[0] for r in rec])
+    ms_step = 1e3 * t.mean()
+    # cold-start GMRES on the last system, for the warm-start comparison
+    A, b, Vm, area = ms.system(v, f)
+    from mesh_bem_amd.gmres import gmres
+
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    _, cold = gmres(A, b, restart=50, tol=1e-6)
+    torch.cuda.synchronize()
+    t_cold = time.perf_counter() - tc
+    # matvec roofline (HBM): 4 F^2 bytes per product, HIP events on the launch stream
+    op = DenseOperator(A)
+    x = torch.randn(nF, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        op.apply(x, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    nrep = 20
+    e0.record()
+    for _ in range(nrep):
+        op.apply(x, y)
+    e1.record()
+    torch.cuda.synchronize()
+    mv_ms = e0.elapsed_time(e1) / nrep
+    gbs = 4.0 * nF * nF / (mv_ms * 1e-3) / 1e9
+    out = {
+        "metric": "C5 magnetic droplet step (P0 V+K' assembly + warm-started GMRES); Mpairs/s",
+        "value": round(1e-6 * nF * nF / (ms_step * 1e-3), 3),
+        "unit": "Mpairs/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 operators + f64 near terms / GMRES iterate",
+        "data": "synthetic",
+        "config": {"workload": f"noisy ellipsoid, geodesic frequency {freq} ({nF} triangles), axes 1:1:{c:.3f}, "
+                               f"radial noise 0.02, step perturbation 0.002, chi={chi}, H0=z",
+                   "faces": nF, "parallelism": "single GPU"},
+        "assembly_ms_mean": round(1e3 * float(np.mean([r[1] for r in rec])), 3),
+        "solve_ms_mean": round(1e3 * float(np.mean([r[2] for r in rec])), 3),
+        "gmres_iterations_warm_mean": round(float(np.mean([r[3] for r in rec])), 2),
+        "gmres_iterations_cold_last": cold.iterations,
+        "gmres_cold_last_s": round(t_cold, 4),
+        "gmres_final_residual_max": float(f"{max(r[4] for r in rec):.3e}"),
+        "roofline": {"kernel": "k_mv32 (GMRES matvec, float32 [F,F])", "bound": "hbm",
+                     "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(mv_ms, 4), "bytes_per_launch": 4 * nF * nF},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle sample)")
+        out["cpu_baseline"] = cpu_baseline_c5(nF, chi)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c5", "c5small"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-extra", action="store_true", help="only the timed steps")
@@ -113,6 +239,8 @@ def main():
     from mesh_bem_amd.mom import MomMesh, assemble_ns, rhs_ns
 
     dev = torch.device("cuda:0")
+    if args.workload in ("c5", "c5small"):
+        return run_c5(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

@@ -265,6 +265,23 @@ int mba_gmres_update(int field, const void *basis, int64_t ldb, int32_t nvec, co
 int mba_gmres_scale(int field, int vprec, const void *src, int64_t n, const void *nrm2, void *dst,
                     mba_stream_t stream);
 
+/* ================================================================== mesh tables
+ * Near-pair CSR over source faces b (ascending test faces a): a is near b when
+ * |o_a - o_b| < eta (rho_a + rho_b), o = centroid, rho = largest vertex
+ * distance from o, evaluated in FP64 (the criterion of the near-singular
+ * terms; the reference has no near treatment).  Two calls: count (writes
+ * near_ptr [F+1]; read near_ptr[F] for the size of near_idx), then fill, with
+ * the same workspace. */
+size_t mba_near_workspace_bytes(int64_t n_faces);
+int mba_near_pairs_count(int prec, const void *verts, const int32_t *faces, int64_t n_verts,
+                         int64_t n_faces, double eta, int64_t *near_ptr, void *workspace,
+                         size_t workspace_bytes, mba_stream_t stream);
+int mba_near_pairs_fill(int64_t n_faces, double eta, const int64_t *near_ptr, int32_t *near_idx,
+                        const void *workspace, size_t workspace_bytes, mba_stream_t stream);
+/* a[i, i] += alpha * d[i] for a real [n, lda] matrix (prec selects float / double). */
+int mba_add_diag(int prec, void *a, int64_t n, int64_t lda, double alpha, const void *d,
+                 mba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

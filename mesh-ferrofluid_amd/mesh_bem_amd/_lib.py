@@ -60,6 +60,10 @@ SIGNATURES = {
     "mba_gmres_dots": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _c.c_int, _vp, _i64, _vp, _vp, _sz, _vp]),
     "mba_gmres_update": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _vp, _dbl, _c.c_int, _vp, _i64, _vp]),
     "mba_gmres_scale": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _vp, _vp, _vp]),
+    "mba_near_workspace_bytes": (_sz, [_i64]),
+    "mba_near_pairs_count": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dbl, _i64p, _vp, _sz, _vp]),
+    "mba_near_pairs_fill": (_c.c_int, [_i64, _dbl, _i64p, _i32p, _vp, _sz, _vp]),
+    "mba_add_diag": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _dbl, _vp, _vp]),
     "mba_laplace_layer_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _i64, _i64, _vp, _i64p, _i32,
                                          _i32, _vp, _vp]),
     "mba_laplace_pack": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),

@@ -29,7 +29,7 @@ import torch
 
 from . import _lib
 from .geometry import compute_face_areas, compute_face_normals, compute_vert_areas
-from .mom import near_pairs, rule_tensor, vertex_incidence
+from .mom import near_pairs_gpu, rule_tensor, vertex_incidence
 from .quadrature import get_gaussion_integration_points_and_weights
 from .types import BEMType, BoundaryType, OrderType
 
@@ -217,8 +217,7 @@ def p0_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, want_v: bo
     vt = verts.to(rdt).contiguous()
     ft = faces.to(torch.int32).contiguous()
     if near is None:
-        p, i = near_pairs(verts.detach().cpu().numpy(), faces.detach().cpu().numpy())
-        near = (torch.from_numpy(p).to(dev), torch.from_numpy(i).to(dev))
+        near = near_pairs_gpu(vt, ft)
     rule = rule_tensor(dev)
     Vm = (out_v if out_v is not None else torch.empty((F, F), dtype=rdt, device=dev)) if want_v else None
     Km = (out_k if out_k is not None else torch.empty((F, F), dtype=rdt, device=dev)) if want_k else None

@@ -51,6 +51,26 @@ def near_pairs(verts: np.ndarray, faces: np.ndarray, eta: float = NEAR_ETA):
     return ptr, np.concatenate(idx).astype(np.int32)
 
 
+def near_pairs_gpu(verts: torch.Tensor, faces: torch.Tensor, eta: float = NEAR_ETA):
+    """Near-pair CSR built on the GPU (same criterion and order as ``near_pairs``)."""
+    lib = _lib.load()
+    vt = verts.contiguous()
+    ft = faces.to(torch.int32).contiguous()
+    F = ft.shape[0]
+    dev = vt.device
+    wsb = lib.mba_near_workspace_bytes(F)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ptr = torch.empty(F + 1, dtype=torch.int64, device=dev)
+    st = lib.mba_near_pairs_count(_lib.prec_of(vt.dtype), _lib.ptr(vt), _lib.ptr(ft), vt.shape[0], F,
+                                  float(eta), _lib.ptr(ptr), _lib.ptr(ws), wsb, _lib.stream())
+    _lib.check(st, "mba_near_pairs_count")
+    nnz = int(ptr[F].item())
+    idx = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    st = lib.mba_near_pairs_fill(F, float(eta), _lib.ptr(ptr), _lib.ptr(idx), _lib.ptr(ws), wsb, _lib.stream())
+    _lib.check(st, "mba_near_pairs_fill")
+    return ptr, idx
+
+
 def vertex_incidence(faces: np.ndarray, n_verts: int):
     """CSR vertex -> face-vertex slots (3f + t), ascending slot order."""
     f = np.asarray(faces, dtype=np.int64).reshape(-1)
@@ -108,19 +128,25 @@ class MomMesh:
         if n_edges is None:
             n_edges = int(fe_np.max()) + 1
         rfe = rwg_face_edges(fe_np, n_edges)
-        if near:
-            nptr, nidx = near_pairs(vnp, fnp, eta)
+        vdev = vt.to(device).contiguous()
+        fdev = ft.to(device).contiguous()
+        if not near:
+            nptr = torch.zeros(len(fnp) + 1, dtype=torch.int64, device=device)
+            nidx = torch.zeros(1, dtype=torch.int32, device=device)
+        elif torch.device(device).type == "cuda":
+            nptr, nidx = near_pairs_gpu(vdev, fdev, eta)
         else:
-            nptr, nidx = np.zeros(len(fnp) + 1, dtype=np.int64), np.zeros(1, dtype=np.int32)
+            p, i = near_pairs(vnp, fnp, eta)
+            nptr, nidx = torch.from_numpy(p).to(device), torch.from_numpy(i).to(device)
         vptr, vidx = vertex_incidence(fnp, vnp.shape[0])
         return MomMesh(
-            verts=vt.to(device).contiguous(),
-            faces=ft.to(device).contiguous(),
+            verts=vdev,
+            faces=fdev,
             face_edge_indices=torch.from_numpy(fe_np.astype(np.int32)).to(device),
             n_edges=int(n_edges),
             rwg_fe=torch.from_numpy(rfe).to(device),
-            near_ptr=torch.from_numpy(nptr).to(device),
-            near_idx=torch.from_numpy(nidx).to(device),
+            near_ptr=nptr,
+            near_idx=nidx,
             vinc_ptr=torch.from_numpy(vptr).to(device),
             vinc_idx=torch.from_numpy(vidx).to(device),
         )
