@@ -36,6 +36,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pmc_traffic(name):
+    """HBM bytes per launch of the dominant kernel, as measured by the
+    FETCH_SIZE / WRITE_SIZE rocprofv3 runs summarised in profiles/<name>."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", open(path).read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def workload(name):
     if name == "c5":
         return dict(freq=45, chi=3.0)
@@ -319,7 +332,8 @@ def main():
             "peak": PEAK_FP32_MFMA_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic("traffic_c2_mixed_solve.md") if args.workload == "c2" else None,
+            "traffic_source": "profiles/traffic_c2_mixed_solve.md (FETCH_SIZE x2.0 + WRITE_SIZE, copy-calibrated)",
             "launches": nl,
             "avg_launch_ms": round(gemm_ms / nl, 4),
             "flops_per_launch_avg": prof["gemm_flops"] / nl,
