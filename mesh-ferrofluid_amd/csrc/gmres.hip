@@ -20,75 +20,117 @@ namespace {
 
 inline hipStream_t S(mba_stream_t s) { return (hipStream_t)s; }
 
-constexpr int MV_UNROLL = 4;
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-template <class VT>
-__device__ __forceinline__ void ld4(const VT* p, double& a, double& b, double& c, double& d)
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// x[4t .. 4t+3] as doubles; xvec: x is 16-byte aligned (vector loads).
+__device__ __forceinline__ void ld4(const float* p, int xvec, double v[4])
 {
-    a = p[0];
-    b = p[1];
-    c = p[2];
-    d = p[3];
+    if (xvec) {
+        const f4v q = *reinterpret_cast<const f4v*>(p);
+        v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+    } else {
+        v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
+    }
+}
+__device__ __forceinline__ void ld4(const double* p, int xvec, double v[4])
+{
+    if (xvec) {
+        const d2v q0 = *reinterpret_cast<const d2v*>(p), q1 = *reinterpret_cast<const d2v*>(p + 2);
+        v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y;
+    } else {
+        v[0] = p[0]; v[1] = p[1]; v[2] = p[2]; v[3] = p[3];
+    }
 }
 
-// y[i] = (b[i] -) sum_t A[i, t] x[t]; one wave per row, 16-byte matrix loads.
+// y[i] = (b[i] -) sum_t A[i, t] x[t].  One wave per MV_ROWS consecutive rows:
+// each x chunk is read once for all of them (x stays in L2; the matrix streams
+// from HBM with 16-byte non-temporal loads).
 // L: row length in floats; ldf: row stride in floats (multiple of 4 when vec).
+constexpr int MV_ROWS = 4;
+
 template <bool CPLX, class VT>
 __global__ __launch_bounds__(256) void k_mv32(const float* __restrict__ A, int64_t rows, int64_t L,
                                               int64_t ldf, const VT* __restrict__ x,
-                                              const VT* __restrict__ b, VT* __restrict__ y, int vec)
+                                              const VT* __restrict__ b, VT* __restrict__ y, int vec,
+                                              int xvec)
 {
     const int lane = threadIdx.x & 63;
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (i >= rows) return;
-    const float* Ar = A + i * ldf;
-    double sr = 0.0, si = 0.0;
+    const int64_t i0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * MV_ROWS;
+    if (i0 >= rows) return;
+    const int nr = (int)((rows - i0) < MV_ROWS ? (rows - i0) : MV_ROWS);
+    const float* Ar[MV_ROWS];
+#pragma unroll
+    for (int r = 0; r < MV_ROWS; ++r) Ar[r] = A + (i0 + (r < nr ? r : 0)) * ldf;
+    double sr[MV_ROWS], si[MV_ROWS];
+#pragma unroll
+    for (int r = 0; r < MV_ROWS; ++r) sr[r] = si[r] = 0.0;
     int64_t done = 0;
-    auto mac4 = [&](const f4v& a, int64_t t) {
-        double v0, v1, v2, v3;
-        ld4(x + 4 * t, v0, v1, v2, v3);
-        if (CPLX) {
-            sr += (double)a.x * v0 - (double)a.y * v1 + (double)a.z * v2 - (double)a.w * v3;
-            si += (double)a.x * v1 + (double)a.y * v0 + (double)a.z * v3 + (double)a.w * v2;
-        } else {
-            sr += (double)a.x * v0 + (double)a.y * v1 + (double)a.z * v2 + (double)a.w * v3;
-        }
-    };
     if (vec) {
         const int64_t nv = L >> 2;  // float4 per row
-        const f4v* A4 = reinterpret_cast<const f4v*>(Ar);
-        int64_t t = lane;
-        for (; t + 64 * (MV_UNROLL - 1) < nv; t += 64 * MV_UNROLL) {
-            f4v a[MV_UNROLL];
+        for (int64_t t = lane; t < nv; t += 64) {
+            f4v a[MV_ROWS];
 #pragma unroll
-            for (int u = 0; u < MV_UNROLL; ++u) a[u] = __builtin_nontemporal_load(A4 + t + 64 * u);
+            for (int r = 0; r < MV_ROWS; ++r)
+                a[r] = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(Ar[r]) + t);
+            double v[4];
+            ld4(x + 4 * t, xvec, v);
 #pragma unroll
-            for (int u = 0; u < MV_UNROLL; ++u) mac4(a[u], t + 64 * u);
+            for (int r = 0; r < MV_ROWS; ++r) {
+                if (CPLX) {
+                    sr[r] += (double)a[r].x * v[0] - (double)a[r].y * v[1] + (double)a[r].z * v[2] -
+                             (double)a[r].w * v[3];
+                    si[r] += (double)a[r].x * v[1] + (double)a[r].y * v[0] + (double)a[r].z * v[3] +
+                             (double)a[r].w * v[2];
+                } else {
+                    sr[r] += (double)a[r].x * v[0] + (double)a[r].y * v[1] + (double)a[r].z * v[2] +
+                             (double)a[r].w * v[3];
+                }
+            }
         }
-        for (; t < nv; t += 64) mac4(__builtin_nontemporal_load(A4 + t), t);
         done = nv << 2;
     }
     // scalar part (unaligned rows, or the tail of fewer than 4 floats)
     if (CPLX) {
         for (int64_t e = (done >> 1) + lane; 2 * e < L; e += 64) {
-            const double ar = Ar[2 * e], ai = Ar[2 * e + 1], xr = x[2 * e], xi = x[2 * e + 1];
-            sr += ar * xr - ai * xi;
-            si += ar * xi + ai * xr;
+            const double xr = x[2 * e], xi = x[2 * e + 1];
+#pragma unroll
+            for (int r = 0; r < MV_ROWS; ++r) {
+                const double ar = Ar[r][2 * e], ai = Ar[r][2 * e + 1];
+                sr[r] += ar * xr - ai * xi;
+                si[r] += ar * xi + ai * xr;
+            }
         }
     } else {
-        for (int64_t e = done + lane; e < L; e += 64) sr += (double)Ar[e] * (double)x[e];
+        for (int64_t e = done + lane; e < L; e += 64) {
+            const double xv = x[e];
+#pragma unroll
+            for (int r = 0; r < MV_ROWS; ++r) sr[r] += (double)Ar[r][e] * xv;
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        sr += __shfl_xor(sr, off);
-        if (CPLX) si += __shfl_xor(si, off);
+#pragma unroll
+    for (int r = 0; r < MV_ROWS; ++r) {
+        for (int off = 32; off > 0; off >>= 1) {
+            sr[r] += __shfl_xor(sr[r], off);
+            if (CPLX) si[r] += __shfl_xor(si[r], off);
+        }
     }
-    if (lane == 0) {
+    if (lane < nr) {
+        // lane r writes row i0 + r (select with a uniform unrolled loop)
+        double vr = 0.0, vi = 0.0;
+#pragma unroll
+        for (int r = 0; r < MV_ROWS; ++r)
+            if (lane == r) {
+                vr = sr[r];
+                vi = si[r];
+            }
+        const int64_t i = i0 + lane;
         if (CPLX) {
-            y[2 * i] = (VT)(b ? (double)b[2 * i] - sr : sr);
-            y[2 * i + 1] = (VT)(b ? (double)b[2 * i + 1] - si : si);
+            y[2 * i] = (VT)(b ? (double)b[2 * i] - vr : vr);
+            y[2 * i + 1] = (VT)(b ? (double)b[2 * i + 1] - vi : vi);
         } else {
-            y[i] = (VT)(b ? (double)b[i] - sr : sr);
+            y[i] = (VT)(b ? (double)b[i] - vr : vr);
         }
     }
 }
@@ -238,10 +280,12 @@ int mba_mv32(int field, const void* a, int64_t rows, int64_t cols, int64_t lda, 
     const int64_t f = field == MBA_COMPLEX ? 2 : 1;
     const int64_t L = f * cols, ldf = f * lda;
     const int vec = (((uintptr_t)a & 15) == 0) && (ldf % 4 == 0);
-    dim3 grid((unsigned)((rows * 64 + 255) / 256));
+    const int xvec = ((uintptr_t)x & 15) == 0;
+    const int64_t waves = (rows + MV_ROWS - 1) / MV_ROWS;
+    dim3 grid((unsigned)((waves * 64 + 255) / 256));
 #define MBA_MV(C, VT)                                                                                   \
     hipLaunchKernelGGL((k_mv32<C, VT>), grid, dim3(256), 0, S(stream), (const float*)a, rows, L, ldf,   \
-                       (const VT*)x, (const VT*)b, (VT*)y, vec)
+                       (const VT*)x, (const VT*)b, (VT*)y, vec, xvec)
     if (field == MBA_COMPLEX) {
         if (vprec == MBA_F32) MBA_MV(true, float); else MBA_MV(true, double);
     } else {
