@@ -282,6 +282,45 @@ int mba_near_pairs_fill(int64_t n_faces, double eta, const int64_t *near_ptr, in
 int mba_add_diag(int prec, void *a, int64_t n, int64_t lda, double alpha, const void *d,
                  mba_stream_t stream);
 
+/* ================================================================== mesh-graph network
+ * Kernels of the learned warm start (no reference counterpart).  GEMM
+ * operands are rounded to BF16 when staged; products accumulate in FP32
+ * (v_mfma_f32_32x32x8bf16); all HBM data is FP32.
+ *
+ * Virtual row-major operand: the concatenation of nseg (<= 3) column segments,
+ * segment q holding w[q] columns of p[q] (row stride ld[q]) at row idx[q][r]
+ * (or r when idx[q] is null), followed by one column of ones when ones != 0. */
+typedef struct {
+    const float *p[3];
+    const int32_t *idx[3];
+    int64_t ld[3];
+    int32_t w[3];
+    int32_t nseg;
+    int32_t ones;
+} mba_mgn_vmat;
+
+/* Y[M, N] = epilogue(Aop Bop): Aop[m, k] = a_trans ? A(k, m) : A(m, k);
+ * Bop[k, n] = b_trans ? B[n ldb + k] : B[k ldb + n].  Epilogue, in order:
+ * v *= silu'(Zp[m, n]) (Zp not null); Z[m, n] = v (Z not null); v = silu(v)
+ * (act = 1); v += R[m, n] (R not null); Y = v or Y += v (accumulate).  Z, R,
+ * Zp share ldy.  split_k > 1 splits K over blocks into partial [split_k, M, N]
+ * (caller-allocated) reduced in a fixed order; epilogue options other than
+ * accumulate are then not allowed. */
+int mba_mgn_gemm(const mba_mgn_vmat *a, int a_trans, const float *b, int64_t ldb, int b_trans, int64_t m,
+                 int64_t n, int64_t k, float *y, int64_t ldy, float *z, const float *r, const float *zp,
+                 int act, int accumulate, int32_t split_k, float *partial, mba_stream_t stream);
+/* out[i, c] (=/+=) sum over q in [ptr[i], ptr[i+1]) of in[idx[q] ldin + coff + c], c < width. */
+int mba_mgn_segsum(const int64_t *ptr, const int32_t *idx, int64_t rows, const float *in, int64_t ldin,
+                   int32_t coff, int32_t width, float *out, int64_t ldout, int accumulate,
+                   mba_stream_t stream);
+/* Relative L2 loss per graph (rows [graph_off[g], graph_off[g+1]) of [*, 2] arrays)
+ * and, when grad is not null, the gradient of the mean over graphs. */
+int mba_mgn_loss(const float *pred, const float *target, const int64_t *graph_off, int32_t n_graphs,
+                 float *loss, float *grad, mba_stream_t stream);
+/* One Adam step (bias-corrected, step >= 1). */
+int mba_mgn_adam(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
+                 float beta2, float eps, int64_t step, mba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

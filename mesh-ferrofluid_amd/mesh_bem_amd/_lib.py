@@ -16,6 +16,13 @@ from ._build import LIB_PATH
 MBA_F32, MBA_F64 = 0, 1
 MBA_COMPLEX, MBA_REAL = 0, 1
 
+
+class MgnVMat(ctypes.Structure):
+    """mba_mgn_vmat of include/mesh_bem_amd.h."""
+
+    _fields_ = [("p", ctypes.c_void_p * 3), ("idx", ctypes.c_void_p * 3), ("ld", ctypes.c_int64 * 3),
+                ("w", ctypes.c_int32 * 3), ("nseg", ctypes.c_int32), ("ones", ctypes.c_int32)]
+
 _c = ctypes
 _vp, _i32p, _i64p, _dp = _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p
 _i32, _i64, _dbl, _sz = _c.c_int32, _c.c_int64, _c.c_double, _c.c_size_t
@@ -64,6 +71,12 @@ SIGNATURES = {
     "mba_near_pairs_count": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dbl, _i64p, _vp, _sz, _vp]),
     "mba_near_pairs_fill": (_c.c_int, [_i64, _dbl, _i64p, _i32p, _vp, _sz, _vp]),
     "mba_add_diag": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _dbl, _vp, _vp]),
+    "mba_mgn_gemm": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _c.c_int, _i64, _i64, _i64, _vp, _i64, _vp, _vp,
+                                _vp, _c.c_int, _c.c_int, _i32, _vp, _vp]),
+    "mba_mgn_segsum": (_c.c_int, [_i64p, _i32p, _i64, _vp, _i64, _i32, _i32, _vp, _i64, _c.c_int, _vp]),
+    "mba_mgn_loss": (_c.c_int, [_vp, _vp, _i64p, _i32, _vp, _vp, _vp]),
+    "mba_mgn_adam": (_c.c_int, [_vp, _vp, _vp, _vp, _i64, _c.c_float, _c.c_float, _c.c_float, _c.c_float, _i64,
+                                _vp]),
     "mba_laplace_layer_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _i64, _i64, _vp, _i64p, _i32,
                                          _i32, _vp, _vp]),
     "mba_laplace_pack": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
