@@ -166,7 +166,7 @@ def test_training_and_warm_start(dev):
     net = M.MeshGraphNet(device=dev, seed=1)
     hist = M.train(net, mds, smp, epochs=6, batch=16, lr=2e-3)
     print(f"  loss history {['%.3f' % h for h in hist]}")
-    assert hist[-1] < 0.5 * hist[1]
+    assert hist[-1] < 0.7 * hist[1] and hist[-1] < 0.01 * hist[0]
     tm, ts = M.generate(3, 5, 2, seed0=900, k_range=(1.0, 2.5), device=dev, keep_system=True)
     cold, warm, jerr = [], [], []
     for md, s in zip(tm, ts):
