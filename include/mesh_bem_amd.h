@@ -136,6 +136,30 @@ int mba_mom_gather_rwg(int prec, const void *zf, int64_t ldzf, int64_t col0, int
 int mba_mom_gather_rhs(int prec, const void *vf, const int32_t *rwg_fe, int64_t n_edges,
                        int64_t ndir, void *v, mba_stream_t stream);
 
+/* Several meshes in one assembly launch (north-star EFIE rule).  verts / faces
+ * are the concatenation of the meshes (faces hold indices into the
+ * concatenated verts; mesh m owns faces [face_off, face_off + n_faces), face_off
+ * equal to the running sum); every mesh writes its own face-edge block
+ * zf [3 n_faces, ldzf] for all its source faces.  n_near = near_ptr[n_faces].
+ * The descriptor array is host memory; the call copies it to the workspace and
+ * waits for that copy.  No reference counterpart (the reference solves one
+ * mesh per call). */
+typedef struct {
+    int64_t n_faces;
+    int64_t face_off;
+    double wavenumber;
+    const int64_t *near_ptr;
+    const int32_t *near_idx;
+    void *zf;
+    int64_t ldzf;
+    int64_t n_near;
+} mba_ns_mesh;
+size_t mba_mom_batched_workspace_bytes(int prec, int64_t total_faces, int32_t n_meshes);
+int mba_mom_zmat_faceedge_ns_batched(int prec, const void *verts, const int32_t *faces, int64_t total_faces,
+                                     const double *rule, int32_t nq, const mba_ns_mesh *meshes,
+                                     int32_t n_meshes, void *workspace, size_t workspace_bytes,
+                                     mba_stream_t stream);
+
 /* =================================================================== dense
  * Row-major complex LU with partial (row) pivoting, in place, replacing the
  * getrf inside torch.linalg.solve (mesh_BEM_solver_3d.py:174, 346).
@@ -180,7 +204,8 @@ int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void
              const void *b, int64_t ldb, double beta, void *c, int64_t ldc, mba_stream_t stream);
 
 /* Tile variants of the FP32 GEMM (0: 128x64 complex tile, 1: 128x64 ordered
- * for the 8 XCDs, 2: 128x128, 3: 128x128 XCD-ordered).  mba_gemm_variant runs
+ * for the 8 XCDs, 2: 128x128, 3: 128x128 XCD-ordered, 4: 128x64 with three
+ * real products per complex product (3M), 5: 3M XCD-ordered).  mba_gemm_variant runs
  * one variant; mba_gemm_set_variant selects the one mba_gemm and the LU use
  * (process-wide, default 0; set once by a tuner, not per call).  No
  * counterpart in the reference. */

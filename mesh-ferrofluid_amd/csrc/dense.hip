@@ -194,7 +194,152 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
         }
 }
 
-// FP32 variants: 0 = 128x64 tile, 1 = 128x64 XCD-ordered, 2 = 128x128, 3 = 128x128 XCD-ordered
+// 3M complex product on real MFMAs: with A = Ar + j Ai, B = Br + j Bi,
+//   T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi),
+//   Re C = T1 - T2, Im C = T3 - T1 - T2:
+// three real products instead of the four of the real embedding (25 % fewer
+// MFMAs; the imaginary part's rounding error grows by a small constant, which
+// the refinement absorbs).  v_mfma_f32_32x32x2f32 takes 2 k per instruction:
+// lane l supplies A[row l % 32][k l / 32] and B[k l / 32][col l % 32]; each
+// lane reads (Ar, Ai) and (Br, Bi) once per k-pair and forms the sums in
+// registers.  Block tile 128 x 64 complex, 4 waves in 2 x 2 (64 x 32 each:
+// 2 x 1 instructions per product), k-chunks of 16 in LDS (double buffer), re
+// and im planes stored k-major (A) and row-major (B).
+template <bool XCD>
+__global__ __launch_bounds__(256) void k_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha,
+                                                     const cpx<float>* __restrict__ A, int64_t lda,
+                                                     const cpx<float>* __restrict__ B, int64_t ldb,
+                                                     float beta, cpx<float>* __restrict__ C, int64_t ldc)
+{
+    constexpr int BN = 64;
+    __shared__ alignas(16) float Ar[2][G32_BK][G32_BM];
+    __shared__ alignas(16) float Ai[2][G32_BK][G32_BM];
+    __shared__ alignas(16) float Br[2][G32_BK][BN];
+    __shared__ alignas(16) float Bi[2][G32_BK][BN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t nbm = (M + G32_BM - 1) / G32_BM, nbn = (N + BN - 1) / BN;
+    int64_t tmi, tni;
+    if (XCD) {
+        tile_of(blockIdx.x, nbm, nbn, tmi, tni);
+        if (tmi < 0) return;
+    } else {
+        tmi = blockIdx.x / nbn;
+        tni = blockIdx.x % nbn;
+    }
+    const int64_t m0 = tmi * G32_BM, n0 = tni * BN;
+    // A: 128 rows x 16 complex k: thread -> row tid/2, k (tid&1)*8 .. +8
+    const int ar = tid >> 1, ak = (tid & 1) * 8;
+    // B: 16 k x 64 complex: thread -> k tid/16, cols (tid&15)*4 .. +4
+    const int bkr = tid >> 4, bc = (tid & 15) * 4;
+    float2 ra[8], rb[4];
+    auto load_chunk = [&](int64_t k0) {
+        const int64_t gi = m0 + ar;
+        const bool rowok = gi < M;
+        const float2* pa = (const float2*)(A + (rowok ? gi : 0) * lda);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int64_t gk = k0 + ak + q;
+            ra[q] = (rowok && gk < K) ? pa[gk] : make_float2(0.f, 0.f);
+        }
+        const int64_t gk = k0 + bkr;
+        const bool kok = gk < K;
+        const float2* pb = (const float2*)(B + (kok ? gk : 0) * ldb);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t gj = n0 + bc + q;
+            rb[q] = (kok && gj < N) ? pb[gj] : make_float2(0.f, 0.f);
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            Ar[buf][ak + q][ar] = ra[q].x;
+            Ai[buf][ak + q][ar] = ra[q].y;
+        }
+        *(float4*)&Br[buf][bkr][bc] = make_float4(rb[0].x, rb[1].x, rb[2].x, rb[3].x);
+        *(float4*)&Bi[buf][bkr][bc] = make_float4(rb[0].y, rb[1].y, rb[2].y, rb[3].y);
+    };
+    floatx16 t1[2], t2[2], t3[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) t1[i][r] = t2[i][r] = t3[i][r] = 0.f;
+    const int hi = lane >> 5, jj = lane & 31;
+    const int nchunks = (int)((K + G32_BK - 1) / G32_BK);
+    float2 cpre[2][16];
+    if (beta != 0.f) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t gj = n0 + wn * 32 + jj;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                cpre[i][r] = (gi < M && gj < N) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
+            }
+        }
+    }
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunks) load_chunk((int64_t)(c + 1) * G32_BK);
+#pragma unroll
+        for (int kk = 0; kk < G32_BK; kk += 2) {
+            const int k = kk + hi;
+            const float br = Br[buf][k][wn * 32 + jj], bi = Bi[buf][k][wn * 32 + jj];
+            const float bs = br + bi;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int row = wm * 64 + i * 32 + jj;
+                const float a_r = Ar[buf][k][row], a_i = Ai[buf][k][row];
+                t1[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, t1[i], 0, 0, 0);
+                t2[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, bi, t2[i], 0, 0, 0);
+                t3[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r + a_i, bs, t3[i], 0, 0, 0);
+            }
+        }
+        if (c + 1 < nchunks) {
+            __syncthreads();
+            store_chunk(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    // epilogue: register r of lane -> row 8*(r/4) + 4*hi + r%4, complex column jj
+    const int64_t gj = n0 + wn * 32 + jj;
+    if (gj < N) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                if (gi < M) {
+                    float re = alpha * (t1[i][r] - t2[i][r]);
+                    float im = alpha * (t3[i][r] - t1[i][r] - t2[i][r]);
+                    if (beta != 0.f) {
+                        re += beta * cpre[i][r].x;
+                        im += beta * cpre[i][r].y;
+                    }
+                    *(float2*)(C + gi * ldc + gj) = make_float2(re, im);
+                }
+            }
+    }
+}
+
+template <bool XCD>
+int launch_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B,
+                       int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
+{
+    const int64_t tiles = ((N + 63) / 64) * ((M + G32_BM - 1) / G32_BM);
+    const int64_t blocks = XCD ? ((tiles + NUM_XCD - 1) / NUM_XCD) * NUM_XCD : tiles;
+    hipLaunchKernelGGL((k_cgemm3m_f32<XCD>), dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, alpha,
+                       (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, beta, (cpx<float>*)C, ldc);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+// FP32 variants: 0 = 128x64 tile, 1 = 128x64 XCD-ordered, 2 = 128x128, 3 = 128x128 XCD-ordered,
+// 4 = 128x64 3M, 5 = 128x64 3M XCD-ordered
 template <int WN, bool XCD>
 int launch_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
                      const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
@@ -214,6 +359,8 @@ int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const v
               const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
 {
     switch (variant) {
+    case 4: return launch_cgemm3m_f32<false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
+    case 5: return launch_cgemm3m_f32<true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
     case 1: return launch_cgemm_f32<2, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
     case 2: return launch_cgemm_f32<4, false>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
     case 3: return launch_cgemm_f32<4, true>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, st);
@@ -1633,7 +1780,7 @@ int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void
 int mba_gemm_set_variant(int prec, int variant)
 {
     if (prec != MBA_F32) return MBA_ERR_ARG;
-    if (variant < 0 || variant > 3) return MBA_ERR_ARG;
+    if (variant < 0 || variant > 5) return MBA_ERR_ARG;
     g_gemm_f32_variant = variant;
     return MBA_OK;
 }
@@ -1643,7 +1790,7 @@ int mba_gemm_variant(int prec, int variant, int64_t m, int64_t n, int64_t k, dou
                      int64_t ldc, mba_stream_t stream)
 {
     if (!a || !b || !c) return MBA_ERR_ARG;
-    if (prec != MBA_F32 || variant < 0 || variant > 3) return MBA_ERR_ARG;
+    if (prec != MBA_F32 || variant < 0 || variant > 5) return MBA_ERR_ARG;
     if (m < 0 || n < 0 || k < 0) return MBA_ERR_SIZE;
     if (lda < k || ldb < n || ldc < n) return MBA_ERR_ARG;
     if (m == 0 || n == 0) return MBA_OK;

@@ -9,6 +9,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "mba_common.hpp"
 
 namespace {
@@ -109,20 +111,20 @@ struct NsTest {
     T c[3];
 };
 
+// One tile: test faces [by NS_TA, +NS_TA) x source faces b0 + [bx NS_BLOCK, +NS_BLOCK)
+// of one mesh.  tst / lam are the block's LDS arrays (lam already loaded and
+// synchronised); every thread of the block calls this with the same tile.
 template <class T, int NQ>
-__global__ __launch_bounds__(NS_BLOCK) void k_ns_zf(
-    const FaceRec<T>* __restrict__ rec, int64_t F, const double* __restrict__ rule, T kk,
-    T k2pi, const int64_t* __restrict__ near_ptr, const int32_t* __restrict__ near_idx,
-    int64_t b0, int64_t nb, cpx<T>* __restrict__ zf, int64_t ldzf)
+__device__ __forceinline__ void ns_zf_tile(NsTest<T, NQ>* tst, const T (*lam)[4],
+                                           const FaceRec<T>* __restrict__ rec, int64_t F, T kk, T k2pi,
+                                           const int64_t* __restrict__ near_ptr,
+                                           const int32_t* __restrict__ near_idx, int64_t b0, int64_t nb,
+                                           cpx<T>* __restrict__ zf, int64_t ldzf, int64_t bx, int64_t by)
 {
-    __shared__ NsTest<T, NQ> tst[NS_TA];
-    __shared__ T lam[NQ][4];
     const int tid = threadIdx.x;
-    const int64_t a0 = (int64_t)blockIdx.y * NS_TA;
-    const int64_t bl = (int64_t)blockIdx.x * NS_BLOCK + tid;  // local source index
+    const int64_t a0 = by * NS_TA;
+    const int64_t bl = bx * NS_BLOCK + tid;  // local source index
     const int64_t b = b0 + bl;
-    if (tid < NQ * 4) lam[tid / 4][tid % 4] = (T)rule[tid];
-    __syncthreads();
     for (int t = tid; t < NS_TA; t += NS_BLOCK) {
         int64_t a = a0 + t;
         if (a < F) {
@@ -168,7 +170,9 @@ __global__ __launch_bounds__(NS_BLOCK) void k_ns_zf(
         for (int q = 0; q < 3; ++q) D[q] = (ta.ohi[q] - rb.ohi[q]) + (ta.olo[q] - rb.olo[q]);
         T I0r = 0, I0i = 0, Ixr[3] = {0, 0, 0}, Ixi[3] = {0, 0, 0}, Iyr[3] = {0, 0, 0},
           Iyi[3] = {0, 0, 0}, Ixyr = 0, Ixyi = 0;
-#pragma unroll
+        // FP64: keep the test-point loop rolled (register pressure of the
+        // unrolled 7 x 7 body limits FP64 occupancy to one wave per SIMD)
+#pragma unroll(sizeof(T) == 4 ? NQ : 1)
         for (int i = 0; i < NQ; ++i) {
             const T xl0 = ta.x[i][0], xl1 = ta.x[i][1], xl2 = ta.x[i][2];
             const T x0 = xl0 + D[0], x1 = xl1 + D[1], x2 = xl2 + D[2];
@@ -233,6 +237,63 @@ __global__ __launch_bounds__(NS_BLOCK) void k_ns_zf(
             }
         }
     }
+}
+
+template <class T, int NQ>
+__global__ __launch_bounds__(NS_BLOCK) void k_ns_zf(
+    const FaceRec<T>* __restrict__ rec, int64_t F, const double* __restrict__ rule, T kk,
+    T k2pi, const int64_t* __restrict__ near_ptr, const int32_t* __restrict__ near_idx,
+    int64_t b0, int64_t nb, cpx<T>* __restrict__ zf, int64_t ldzf)
+{
+    __shared__ NsTest<T, NQ> tst[NS_TA];
+    __shared__ T lam[NQ][4];
+    if (threadIdx.x < NQ * 4) lam[threadIdx.x / 4][threadIdx.x % 4] = (T)rule[threadIdx.x];
+    __syncthreads();
+    ns_zf_tile<T, NQ>(tst, lam, rec, F, kk, k2pi, near_ptr, near_idx, b0, nb, zf, ldzf, blockIdx.x,
+                      blockIdx.y);
+}
+
+// Several meshes in one launch: a 1-D grid over the tiles of all meshes
+// (tile_off: prefix of the per-mesh tile counts); each block finds its mesh by
+// bisection and runs the single-mesh tile on that mesh's records and output.
+struct NsMeshDev {
+    int64_t F;
+    int64_t face_off;
+    double k;
+    const int64_t* near_ptr;
+    const int32_t* near_idx;
+    void* zf;
+    int64_t ldzf;
+    int64_t n_near;
+};
+
+__device__ __forceinline__ int find_segment(const int64_t* __restrict__ off, int n, int64_t t)
+{
+    int lo = 0, hi = n;  // off[lo] <= t < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <class T, int NQ>
+__global__ __launch_bounds__(NS_BLOCK) void k_ns_zf_batched(const FaceRec<T>* __restrict__ rec_all,
+                                                            const NsMeshDev* __restrict__ meshes,
+                                                            const int64_t* __restrict__ tile_off, int nmesh,
+                                                            const double* __restrict__ rule)
+{
+    __shared__ NsTest<T, NQ> tst[NS_TA];
+    __shared__ T lam[NQ][4];
+    if (threadIdx.x < NQ * 4) lam[threadIdx.x / 4][threadIdx.x % 4] = (T)rule[threadIdx.x];
+    __syncthreads();
+    const int64_t t = blockIdx.x;
+    const int m = find_segment(tile_off, nmesh, t);
+    const NsMeshDev md = meshes[m];
+    const int64_t lt = t - tile_off[m];
+    const int64_t nbx = (md.F + NS_BLOCK - 1) / NS_BLOCK;
+    ns_zf_tile<T, NQ>(tst, lam, rec_all + md.face_off, md.F, (T)md.k, (T)(md.k / (2.0 * MBA_PI)), md.near_ptr,
+                      md.near_idx, 0, md.F, (cpx<T>*)md.zf, md.ldzf, lt % nbx, lt / nbx);
 }
 
 // ----------------------------------------------------------------- near (FP64)
@@ -304,6 +365,23 @@ __global__ void k_ns_near(const FaceRec<T>* __restrict__ rec, int64_t F,
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t q = q0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < q1; q += stride)
         near_pair<T>(rec, rule, nq, kk, near_ptr, near_idx, b0, nb, q, zf, ldzf);
+}
+
+// Near pairs of several meshes: a grid-stride loop over all CSR entries
+// (near_off: prefix of the per-mesh counts).
+template <class T>
+__global__ void k_ns_near_batched(const FaceRec<T>* __restrict__ rec_all, const NsMeshDev* __restrict__ meshes,
+                                  const int64_t* __restrict__ near_off, int nmesh,
+                                  const double* __restrict__ rule, int nq)
+{
+    const int64_t total = near_off[nmesh];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+        const int m = find_segment(near_off, nmesh, q);
+        const NsMeshDev& md = meshes[m];
+        near_pair<T>(rec_all + md.face_off, rule, nq, md.k, md.near_ptr, md.near_idx, 0, md.F, q - near_off[m],
+                     (cpx<T>*)md.zf, md.ldzf);
+    }
 }
 
 template <class T>
@@ -779,6 +857,64 @@ int zmat_ns(const void* verts, const int32_t* faces, int64_t F, const double* ru
     return MBA_OK;
 }
 
+static_assert(sizeof(NsMeshDev) == sizeof(mba_ns_mesh), "descriptor layouts must match");
+
+size_t batched_ws_layout(int prec, int64_t total_faces, int n, size_t* off_desc, size_t* off_tile,
+                         size_t* off_near)
+{
+    size_t o = records_bytes(prec, total_faces);
+    *off_desc = o;
+    o += ((size_t)n * sizeof(NsMeshDev) + 255) & ~(size_t)255;
+    *off_tile = o;
+    o += ((size_t)(n + 1) * sizeof(int64_t) + 255) & ~(size_t)255;
+    *off_near = o;
+    o += ((size_t)(n + 1) * sizeof(int64_t) + 255) & ~(size_t)255;
+    return o;
+}
+
+template <class T>
+int zmat_ns_batched(const void* verts, const int32_t* faces, int64_t total_faces, const double* rule, int nq,
+                    const mba_ns_mesh* meshes, int n, void* ws, hipStream_t st)
+{
+    size_t od, ot, on;
+    batched_ws_layout(sizeof(T) == 4 ? MBA_F32 : MBA_F64, total_faces, n, &od, &ot, &on);
+    FaceRec<T>* rec = (FaceRec<T>*)ws;
+    int rc = face_records<T>(verts, faces, nullptr, nullptr, total_faces, rec, st);
+    if (rc != MBA_OK) return rc;
+    std::vector<int64_t> tile_off(n + 1, 0), near_off(n + 1, 0);
+    for (int m = 0; m < n; ++m) {
+        const int64_t F = meshes[m].n_faces;
+        tile_off[m + 1] = tile_off[m] + ((F + NS_BLOCK - 1) / NS_BLOCK) * ((F + NS_TA - 1) / NS_TA);
+        near_off[m + 1] = near_off[m] + meshes[m].n_near;
+    }
+    char* base = (char*)ws;
+    if (hipMemcpyAsync(base + od, meshes, (size_t)n * sizeof(NsMeshDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(base + ot, tile_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(base + on, near_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+        return MBA_ERR_LAUNCH;
+    // the host tables above are released on return: wait for the copies
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    const NsMeshDev* dm = (const NsMeshDev*)(base + od);
+    const int64_t* dt = (const int64_t*)(base + ot);
+    const int64_t* dn = (const int64_t*)(base + on);
+    const int64_t tiles = tile_off[n];
+    if (tiles > 0x7fffffffLL) return MBA_ERR_SIZE;
+    if (nq == 7)
+        hipLaunchKernelGGL((k_ns_zf_batched<T, 7>), dim3((unsigned)tiles), dim3(NS_BLOCK), 0, st, rec, dm, dt, n,
+                           rule);
+    else if (nq == 3)
+        hipLaunchKernelGGL((k_ns_zf_batched<T, 3>), dim3((unsigned)tiles), dim3(NS_BLOCK), 0, st, rec, dm, dt, n,
+                           rule);
+    else
+        return MBA_ERR_ARG;
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_ns_near_batched<T>, dim3(2048), dim3(64), 0, st, rec, dm, dn, n, rule, nq);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -1002,6 +1138,35 @@ int mba_mom_gather_rhs(int prec, const void* vf, const int32_t* rwg_fe, int64_t 
                            (const cpx<double>*)vf, rwg_fe, n_edges, ndir, (cpx<double>*)v);
     MBA_LAUNCH_CHECK();
     return MBA_OK;
+}
+
+size_t mba_mom_batched_workspace_bytes(int prec, int64_t total_faces, int32_t n_meshes)
+{
+    if (total_faces <= 0 || n_meshes <= 0 || (prec != MBA_F32 && prec != MBA_F64)) return 0;
+    size_t a, b, c;
+    return batched_ws_layout(prec, total_faces, n_meshes, &a, &b, &c);
+}
+
+int mba_mom_zmat_faceedge_ns_batched(int prec, const void* verts, const int32_t* faces, int64_t total_faces,
+                                     const double* rule, int32_t nq, const mba_ns_mesh* meshes, int32_t n_meshes,
+                                     void* workspace, size_t workspace_bytes, mba_stream_t stream)
+{
+    if (!verts || !faces || !rule || !meshes || !workspace) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n_meshes <= 0 || total_faces <= 0 || total_faces > kMaxFaces) return MBA_ERR_SIZE;
+    if (nq != 3 && nq != 7) return MBA_ERR_ARG;
+    if (workspace_bytes < mba_mom_batched_workspace_bytes(prec, total_faces, n_meshes)) return MBA_ERR_ALLOC;
+    int64_t sum = 0;
+    for (int m = 0; m < n_meshes; ++m) {
+        const mba_ns_mesh& d = meshes[m];
+        if (d.n_faces <= 0 || d.face_off != sum || !d.near_ptr || !d.near_idx || !d.zf) return MBA_ERR_ARG;
+        if (d.ldzf < 3 * d.n_faces || d.n_near < 0 || !(d.wavenumber > 0.0)) return MBA_ERR_ARG;
+        sum += d.n_faces;
+    }
+    if (sum != total_faces) return MBA_ERR_ARG;
+    if (prec == MBA_F32)
+        return zmat_ns_batched<float>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
+    return zmat_ns_batched<double>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
 }
 
 }  // extern "C"

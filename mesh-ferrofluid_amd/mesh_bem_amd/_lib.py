@@ -17,6 +17,14 @@ MBA_F32, MBA_F64 = 0, 1
 MBA_COMPLEX, MBA_REAL = 0, 1
 
 
+class NsMesh(ctypes.Structure):
+    """mba_ns_mesh of include/mesh_bem_amd.h."""
+
+    _fields_ = [("n_faces", ctypes.c_int64), ("face_off", ctypes.c_int64), ("wavenumber", ctypes.c_double),
+                ("near_ptr", ctypes.c_void_p), ("near_idx", ctypes.c_void_p), ("zf", ctypes.c_void_p),
+                ("ldzf", ctypes.c_int64), ("n_near", ctypes.c_int64)]
+
+
 class MgnVMat(ctypes.Structure):
     """mba_mgn_vmat of include/mesh_bem_amd.h."""
 
@@ -45,6 +53,9 @@ SIGNATURES = {
                                            _i64, _vp, _vp]),
     "mba_mom_gather_rwg": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i64, _i32p, _i64, _vp, _i64, _vp]),
     "mba_mom_gather_rhs": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _vp, _vp]),
+    "mba_mom_batched_workspace_bytes": (_sz, [_c.c_int, _i64, _i32]),
+    "mba_mom_zmat_faceedge_ns_batched": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _dp, _i32, _vp, _i32, _vp, _sz,
+                                                    _vp]),
     "mba_lu_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_lu_factor": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp]),
     "mba_lu_factor_profiled": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp, _vp,

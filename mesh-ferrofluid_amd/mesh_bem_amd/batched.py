@@ -1,0 +1,227 @@
+"""Many meshes and many incident waves at once.
+
+* ``BatchedMoM.assemble``: the EFIE face-edge blocks of every mesh in one
+  assembly launch (``mba_mom_zmat_faceedge_ns_batched``: a 1-D grid over the
+  tiles of all meshes) plus one launch for all near pairs, then a fixed-order
+  gather per mesh into its RWG matrix.
+* ``BatchedMoM.solve``: per mesh, the FP32 LU of the rounded matrix (GEMM tile
+  variant picked by the autotuner for the mesh size), the multi-RHS triangular
+  solves, and refinement whose residual R = V - Z J is an FP64 MFMA GEMM over
+  all right-hand sides at once.
+* ``GemmTuner``: times the FP32 GEMM tile variants on the LU's trailing-update
+  shapes per size bucket and caches the fastest (optionally in a JSON file).
+"""
+
+import ctypes
+import json
+import os
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .dense import LUFactor, to_c64
+from .geometry import get_edge_unique
+from .mom import MomMesh, rhs_ns, rule_tensor
+
+BUCKETS = (1024, 2048, 3072, 4096, 8192, 16384, 32768)
+
+
+def _bucket(n: int) -> int:
+    for b in BUCKETS:
+        if n <= b:
+            return b
+    return BUCKETS[-1]
+
+
+class GemmTuner:
+    """Fastest FP32 GEMM tile variant per matrix-size bucket."""
+
+    def __init__(self, path: Optional[str] = None, reps: int = 3):
+        self.path = path
+        self.reps = reps
+        self.best: Dict[int, int] = {}
+        self.timings: Dict[int, List[float]] = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.best = {int(k): int(v) for k, v in json.load(f).items()}
+
+    def tune(self, n: int, device="cuda") -> int:
+        b = _bucket(n)
+        if b in self.best:
+            return self.best[b]
+        lib = _lib.load()
+        m = max(256, b - 256)
+        k = 256
+        A = torch.randn((m, k), dtype=torch.complex64, device=device)
+        B = torch.randn((k, m), dtype=torch.complex64, device=device)
+        C = torch.randn((m, m), dtype=torch.complex64, device=device)
+        times = []
+        for v in range(6):
+            def run():
+                st = lib.mba_gemm_variant(_lib.MBA_F32, v, m, m, k, -1.0, _lib.ptr(A), k, _lib.ptr(B), m, 1.0,
+                                          _lib.ptr(C), m, _lib.stream())
+                _lib.check(st, "mba_gemm_variant")
+            run()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(self.reps):
+                run()
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) / self.reps)
+        self.timings[b] = times
+        self.best[b] = int(np.argmin(times))
+        if self.path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
+            with open(self.path, "w") as f:
+                json.dump({str(k2): v2 for k2, v2 in self.best.items()}, f)
+        return self.best[b]
+
+    def apply(self, n: int):
+        lib = _lib.load()
+        _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, self.tune(n)), "mba_gemm_set_variant")
+
+
+@dataclass
+class BatchResult:
+    J: List[torch.Tensor]  # per mesh [E, D] complex128
+    iterations: List[int]
+    converged: List[bool]
+    times: Dict[str, float] = field(default_factory=dict)
+
+
+class BatchedMoM:
+    def __init__(self, meshes: List[MomMesh], ks: List[float]):
+        self.meshes = meshes
+        self.ks = [float(k) for k in ks]
+        dev = meshes[0].verts.device
+        self.device = dev
+        voff, foff = 0, 0
+        vs, fs = [], []
+        self.face_off = []
+        for m in meshes:
+            vs.append(m.verts.to(torch.float64))
+            fs.append(m.faces.to(torch.int64) + voff)
+            self.face_off.append(foff)
+            voff += m.n_verts
+            foff += m.n_faces
+        self.verts = torch.cat(vs, 0).contiguous()
+        self.faces = torch.cat(fs, 0).to(torch.int32).contiguous()
+        self.total_faces = foff
+        self.n_near = [int(m.near_ptr[-1].item()) for m in meshes]
+
+    @staticmethod
+    def build(geoms, ks, device="cuda"):
+        """geoms: list of (verts [V,3], faces [F,3]) numpy arrays."""
+        meshes = []
+        for V, F in geoms:
+            vt = torch.tensor(V, dtype=torch.float64)
+            ft = torch.tensor(F, dtype=torch.int32)
+            edges, fe, _ = get_edge_unique(vt, ft)
+            meshes.append(MomMesh.build(vt, F, fe, edges.shape[0], device=device))
+        return BatchedMoM(meshes, ks)
+
+    def assemble(self, prec: int = _lib.MBA_F64) -> List[torch.Tensor]:
+        """RWG matrices [E_m, E_m] of every mesh (complex128 for MBA_F64)."""
+        lib = _lib.load()
+        cdt = torch.complex64 if prec == _lib.MBA_F32 else torch.complex128
+        rdt = torch.float32 if prec == _lib.MBA_F32 else torch.float64
+        n = len(self.meshes)
+        zfs = [torch.empty((3 * m.n_faces, 3 * m.n_faces), dtype=cdt, device=self.device) for m in self.meshes]
+        desc = (_lib.NsMesh * n)()
+        for i, (m, zf) in enumerate(zip(self.meshes, zfs)):
+            desc[i].n_faces = m.n_faces
+            desc[i].face_off = self.face_off[i]
+            desc[i].wavenumber = self.ks[i]
+            desc[i].near_ptr = m.near_ptr.data_ptr()
+            desc[i].near_idx = m.near_idx.data_ptr()
+            desc[i].zf = zf.data_ptr()
+            desc[i].ldzf = zf.stride(0)
+            desc[i].n_near = self.n_near[i]
+        rule = rule_tensor(self.device)
+        wsb = lib.mba_mom_batched_workspace_bytes(prec, self.total_faces, n)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=self.device)
+        verts = self.verts.to(rdt).contiguous()
+        st = lib.mba_mom_zmat_faceedge_ns_batched(prec, _lib.ptr(verts), _lib.ptr(self.faces), self.total_faces,
+                                                  _lib.ptr(rule), rule.shape[0], desc, n, _lib.ptr(ws), wsb,
+                                                  _lib.stream())
+        _lib.check(st, "mba_mom_zmat_faceedge_ns_batched")
+        Zs = []
+        for m, zf in zip(self.meshes, zfs):
+            E = m.n_edges
+            Z = torch.empty((E, E), dtype=cdt, device=self.device)
+            st = lib.mba_mom_gather_rwg(prec, _lib.ptr(zf), zf.stride(0), 0, zf.shape[1], _lib.ptr(m.rwg_fe), E,
+                                        _lib.ptr(Z), Z.stride(0), _lib.stream())
+            _lib.check(st, "mba_mom_gather_rwg")
+            Zs.append(Z)
+        del zfs
+        return Zs
+
+    def rhs(self, dirs: List[np.ndarray], pols: List[np.ndarray], prec=_lib.MBA_F64) -> List[torch.Tensor]:
+        return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
+
+    def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
+              tol: float = 1e-8, maxit: int = 10) -> BatchResult:
+        """Mixed precision per mesh: FP32 LU (tuned GEMM tiles), FP64 GEMM residuals."""
+        lib = _lib.load()
+        Js, its, conv = [], [], []
+        t_lu = t_ref = 0.0
+        for Z, V in zip(Zs, Vs):
+            n = Z.shape[0]
+            if tuner is not None:
+                tuner.apply(n)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lu = LUFactor(to_c64(Z), overwrite=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            V128 = V.to(torch.complex128).contiguous()
+            J = lu.solve(V.to(torch.complex64)).to(torch.complex128)
+            R = torch.empty_like(V128)
+            ok = False
+            it = 0
+            for it in range(1, maxit + 1):
+                R.copy_(V128)
+                st = lib.mba_gemm(_lib.MBA_F64, n, V128.shape[1], n, -1.0, _lib.ptr(Z), Z.stride(0), _lib.ptr(J),
+                                  J.stride(0), 1.0, _lib.ptr(R), R.stride(0), _lib.stream())
+                _lib.check(st, "mba_gemm")
+                d = lu.solve(R.to(torch.complex64)).to(torch.complex128)
+                J += d
+                rel = float((torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max())
+                if rel <= tol:
+                    ok = True
+                    break
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            t_lu += t1 - t0
+            t_ref += t2 - t1
+            Js.append(J)
+            its.append(it)
+            conv.append(ok)
+        if tuner is not None:
+            _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 0), "mba_gemm_set_variant")
+        return BatchResult(Js, its, conv, {"lu_s": t_lu, "solve_refine_s": t_ref})
+
+
+def star_batch(n_meshes: int, seed0: int = 0, freqs=(5, 6, 7, 8, 9, 10, 11), n_dirs: int = 128,
+               k_range=(1.0, 8.0)):
+    """Seeded workload of many uneven meshes: star meshes on geodesic spheres of
+    frequency freqs[i % len] (E = 30 n^2), ka ~ U(k_range), n_dirs waves each."""
+    from .geometry import star_mesh
+    from .mgn import random_waves
+
+    geoms, ks, dirs, pols = [], [], [], []
+    for i in range(n_meshes):
+        s = seed0 + i
+        V, F, _ = star_mesh(s, freqs[i % len(freqs)])
+        rng = np.random.Generator(np.random.PCG64(7919 * s + 3))
+        ks.append(float(rng.uniform(*k_range)))
+        d, p = random_waves(rng, n_dirs)
+        geoms.append((V, F))
+        dirs.append(d)
+        pols.append(p)
+    return geoms, ks, dirs, pols

@@ -129,3 +129,25 @@ def test_refinement_fallback_on_ill_conditioning(dev):
 def test_to_c64(dev):
     z = torch.randn(5, 7, dtype=torch.complex128, device=dev)
     assert torch.equal(to_c64(z), z.to(torch.complex64))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [(300, 200, 256), (129, 65, 17), (1000, 1000, 128)])
+def test_gemm_variants(dev, variant, shape):
+    """Every FP32 GEMM tile variant (incl. the 3M ones) against a complex128 product."""
+    from mesh_bem_amd import _lib
+
+    M_, N_, K_ = shape
+    g = torch.Generator().manual_seed(variant)
+    A = torch.randn((M_, K_), dtype=torch.complex64, generator=g)
+    B = torch.randn((K_, N_), dtype=torch.complex64, generator=g)
+    C = torch.randn((M_, N_), dtype=torch.complex64, generator=g)
+    Ad, Bd, Cd = A.to(dev), B.to(dev), C.to(dev)
+    lib = _lib.load()
+    st = lib.mba_gemm_variant(_lib.MBA_F32, variant, M_, N_, K_, -1.0, _lib.ptr(Ad), K_, _lib.ptr(Bd), N_, 0.5,
+                              _lib.ptr(Cd), N_, _lib.stream())
+    _lib.check(st)
+    ref = -(A.to(torch.complex128) @ B.to(torch.complex128)) + 0.5 * C.to(torch.complex128)
+    scale = (A.abs().to(torch.float64) @ B.abs().to(torch.float64)).max()
+    err = (Cd.cpu().to(torch.complex128) - ref).abs().max() / scale
+    assert err < 3e-6, (variant, float(err))
