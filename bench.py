@@ -520,7 +520,7 @@ def main():
         nl = max(1, prof["phase_launches"]["gemm"])
         achieved = prof["gemm_flops"] / (gemm_ms * 1e-3) / 1e12
         out["roofline"] = {
-            "kernel": "k_cgemm_f32 (LU trailing update, complex-as-real v_mfma_f32_32x32x2f32)",
+            "kernel": "k_cgemm3m_f32 (LU trailing update, 3M complex product on v_mfma_f32_32x32x2f32; achieved counts 8mnk flops)",
             "bound": "mfma",
             "achieved": round(achieved, 2),
             "peak": PEAK_FP32_MFMA_TFLOPS,

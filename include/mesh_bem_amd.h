@@ -207,7 +207,7 @@ int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void
  * for the 8 XCDs, 2: 128x128, 3: 128x128 XCD-ordered, 4: 128x64 with three
  * real products per complex product (3M), 5: 3M XCD-ordered).  mba_gemm_variant runs
  * one variant; mba_gemm_set_variant selects the one mba_gemm and the LU use
- * (process-wide, default 0; set once by a tuner, not per call).  No
+ * (process-wide, default 4; set once by a tuner, not per call).  No
  * counterpart in the reference. */
 int mba_gemm_variant(int prec, int variant, int64_t m, int64_t n, int64_t k, double alpha,
                      const void *a, int64_t lda, const void *b, int64_t ldb, double beta, void *c,

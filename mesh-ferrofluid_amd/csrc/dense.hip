@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
 // 2 x 1 instructions per product), k-chunks of 16 in LDS (double buffer), re
 // and im planes stored k-major (A) and row-major (B).
 template <bool XCD>
-__global__ __launch_bounds__(256) void k_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha,
+__global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha,
                                                      const cpx<float>* __restrict__ A, int64_t lda,
                                                      const cpx<float>* __restrict__ B, int64_t ldb,
                                                      float beta, cpx<float>* __restrict__ C, int64_t ldc)
@@ -267,18 +267,6 @@ __global__ __launch_bounds__(256) void k_cgemm3m_f32(int64_t M, int64_t N, int64
         for (int r = 0; r < 16; ++r) t1[i][r] = t2[i][r] = t3[i][r] = 0.f;
     const int hi = lane >> 5, jj = lane & 31;
     const int nchunks = (int)((K + G32_BK - 1) / G32_BK);
-    float2 cpre[2][16];
-    if (beta != 0.f) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int64_t gj = n0 + wn * 32 + jj;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
-                cpre[i][r] = (gi < M && gj < N) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
-            }
-        }
-    }
     load_chunk(0);
     store_chunk(0);
     __syncthreads();
@@ -316,11 +304,13 @@ __global__ __launch_bounds__(256) void k_cgemm3m_f32(int64_t M, int64_t N, int64
                 if (gi < M) {
                     float re = alpha * (t1[i][r] - t2[i][r]);
                     float im = alpha * (t3[i][r] - t1[i][r] - t2[i][r]);
+                    float2* pc = (float2*)(C + gi * ldc + gj);
                     if (beta != 0.f) {
-                        re += beta * cpre[i][r].x;
-                        im += beta * cpre[i][r].y;
+                        const float2 c0 = *pc;
+                        re += beta * c0.x;
+                        im += beta * c0.y;
                     }
-                    *(float2*)(C + gi * ldc + gj) = make_float2(re, im);
+                    *pc = make_float2(re, im);
                 }
             }
     }
@@ -353,7 +343,7 @@ int launch_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A
     return MBA_OK;
 }
 
-int g_gemm_f32_variant = 0;  // tile choice of the FP32 GEMM (set by mba_gemm_set_variant)
+int g_gemm_f32_variant = 4;  // tile choice of the FP32 GEMM (set by mba_gemm_set_variant)
 
 int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
               const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)

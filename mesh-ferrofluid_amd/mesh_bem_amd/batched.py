@@ -203,7 +203,7 @@ class BatchedMoM:
             its.append(it)
             conv.append(ok)
         if tuner is not None:
-            _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 0), "mba_gemm_set_variant")
+            _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
         return BatchResult(Js, its, conv, {"lu_s": t_lu, "solve_refine_s": t_ref})
 
 
