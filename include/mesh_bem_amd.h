@@ -175,6 +175,14 @@ int mba_lu_factor(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int3
  * 0 panel factorisation, 1 row interchanges, 2 triangular solve of the U row
  * block, 3 trailing MFMA GEMM; gemm_flops = real flops of the trailing GEMMs
  * (8 m n k per launch).  No counterpart in the reference (measurement aid). */
+/* As mba_lu_factor without any host synchronisation: the cooperative-wait
+ * flag (nonzero: a panel wait timed out) is copied to err_out (device int32)
+ * on the stream, and each panel uses at most max_wg workgroups (0: no limit),
+ * so that factorisations on several streams can run side by side.  No
+ * reference counterpart (the reference factors one matrix at a time). */
+int mba_lu_factor_async(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
+                        void *workspace, size_t workspace_bytes, int32_t max_wg, int32_t *err_out,
+                        mba_stream_t stream);
 int mba_lu_factor_profiled(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
                            void *workspace, size_t workspace_bytes, double *phase_ms,
                            int64_t *phase_launches, double *gemm_flops, mba_stream_t stream);

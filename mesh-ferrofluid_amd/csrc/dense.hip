@@ -918,6 +918,12 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
     if (i < n) p[i] = 0ULL;
 }
 
+__global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 __global__ void k_zero_i32(int32_t* p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1011,7 +1017,8 @@ struct PhaseTimer {
 template <class T>
 int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, void* ws_ptr,
               size_t ws_bytes, hipStream_t st, double* phase_ms = nullptr,
-              int64_t* phase_launches = nullptr, double* gemm_flops = nullptr)
+              int64_t* phase_launches = nullptr, double* gemm_flops = nullptr, int max_wg = 0,
+              int32_t* err_out = nullptr)
 {
     PhaseTimer tm;
     tm.on = phase_ms != nullptr;
@@ -1064,6 +1071,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         const int64_t m = n - k;
         int G = (int)((m + 7) / 8);
         if (G > cus) G = cus;
+        if (max_wg > 0 && G > max_wg) G = max_wg;
         if (G > MAX_G) G = MAX_G;
         int R = (int)((m + G - 1) / G);
         G = (int)((m + R - 1) / R);
@@ -1139,6 +1147,12 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         if ((rc = do_trsm(k2, nb2, c3)) != MBA_OK) return rc;
         // trailing update with both panels: k = nb1 + nb2
         if ((rc = do_gemm_rect(c3, n - c3, c3, n - c3, k, nb1 + nb2)) != MBA_OK) return rc;
+    }
+    if (err_out) {
+        // asynchronous form: hand the wait flag to the caller, no host synchronisation
+        hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, st, (const int32_t*)ws.err, err_out, 1);
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
     }
     int herr = 0;
     if (hipMemcpyAsync(&herr, ws.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return MBA_ERR_LAUNCH;
@@ -1686,6 +1700,21 @@ int mba_lu_factor(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int3
     if (lda < n) return MBA_ERR_ARG;
     if (prec == MBA_F32) return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
     return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+}
+
+int mba_lu_factor_async(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,
+                        void* workspace, size_t workspace_bytes, int32_t max_wg, int32_t* err_out,
+                        mba_stream_t stream)
+{
+    if (!a || !ipiv || !info || !workspace || !err_out) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+    if (lda < n || max_wg < 0) return MBA_ERR_ARG;
+    if (prec == MBA_F32)
+        return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
+                                nullptr, max_wg, err_out);
+    return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
+                             nullptr, max_wg, err_out);
 }
 
 int mba_lu_factor_profiled(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,

@@ -75,7 +75,16 @@ __device__ __forceinline__ void cis_rev(double t, double& c, double& s)
 }
 
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+// FP64 1/sqrt(x), x > 0: the hardware estimate refined by two Newton steps
+// y <- y (1 + (1 - x y^2) / 2) (each step squares the relative error).
+__device__ __forceinline__ double rsqrt_t(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
 
 template <class T>
 struct vec3 {

@@ -58,6 +58,7 @@ SIGNATURES = {
                                                     _vp]),
     "mba_lu_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_lu_factor": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp]),
+    "mba_lu_factor_async": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _i32, _i32p, _vp]),
     "mba_lu_factor_profiled": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp, _vp,
                                           _vp, _vp]),
     "mba_lu_pivots_to_perm": (_c.c_int, [_i32p, _i64, _i32p, _vp]),

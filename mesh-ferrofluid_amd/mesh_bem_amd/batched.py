@@ -165,46 +165,70 @@ class BatchedMoM:
         return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
-              tol: float = 1e-8, maxit: int = 10) -> BatchResult:
-        """Mixed precision per mesh: FP32 LU (tuned GEMM tiles), FP64 GEMM residuals."""
+              tol: float = 1e-8, maxit: int = 10, streams: int = 4, sweeps: int = 3) -> BatchResult:
+        """Mixed precision per mesh, meshes spread over ``streams`` HIP streams
+        (default priority) so their factorisations and solves overlap: FP32 LU
+        (GEMM variant of the largest size bucket from the tuner; panels capped
+        at CUs / streams workgroups), ``sweeps`` refinement sweeps with FP64
+        GEMM residuals enqueued without host synchronisation, then a host check
+        of the last corrections; meshes not yet at ``tol`` continue alone."""
         lib = _lib.load()
+        n_mesh = len(Zs)
+        S = max(1, min(streams, n_mesh))
+        if tuner is not None:
+            tuner.apply(max(Z.shape[0] for Z in Zs))
+        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
+        main = torch.cuda.current_stream()
+        sts = [torch.cuda.Stream(device=self.device) for _ in range(S)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        state = []
+        for i, (Z, V) in enumerate(zip(Zs, Vs)):
+            s = sts[i % S]
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                lu = LUFactor(to_c64(Z), overwrite=True, asynchronous=True, max_wg=max(1, cus // S))
+                V128 = V.to(torch.complex128).contiguous()
+                J = lu.solve(V.to(torch.complex64)).to(torch.complex128)
+                R = torch.empty_like(V128)
+                rel = None
+                for _ in range(sweeps):
+                    d = self._sweep(lib, lu, Z, V128, J, R)
+                    J += d
+                    rel = (torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()
+                state.append((lu, J, R, V128, rel))
+        for s in sts:
+            main.wait_stream(s)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         Js, its, conv = [], [], []
-        t_lu = t_ref = 0.0
-        for Z, V in zip(Zs, Vs):
-            n = Z.shape[0]
-            if tuner is not None:
-                tuner.apply(n)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            lu = LUFactor(to_c64(Z), overwrite=True)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            V128 = V.to(torch.complex128).contiguous()
-            J = lu.solve(V.to(torch.complex64)).to(torch.complex128)
-            R = torch.empty_like(V128)
-            ok = False
-            it = 0
-            for it in range(1, maxit + 1):
-                R.copy_(V128)
-                st = lib.mba_gemm(_lib.MBA_F64, n, V128.shape[1], n, -1.0, _lib.ptr(Z), Z.stride(0), _lib.ptr(J),
-                                  J.stride(0), 1.0, _lib.ptr(R), R.stride(0), _lib.stream())
-                _lib.check(st, "mba_gemm")
-                d = lu.solve(R.to(torch.complex64)).to(torch.complex128)
+        for (lu, J, R, V128, rel), Z in zip(state, Zs):
+            lu.check()
+            it = sweeps
+            ok = float(rel) <= tol
+            while not ok and it < maxit:
+                d = self._sweep(lib, lu, Z, V128, J, R)
                 J += d
-                rel = float((torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max())
-                if rel <= tol:
-                    ok = True
-                    break
-            torch.cuda.synchronize()
-            t2 = time.perf_counter()
-            t_lu += t1 - t0
-            t_ref += t2 - t1
+                it += 1
+                ok = float((torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()) <= tol
             Js.append(J)
             its.append(it)
             conv.append(ok)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
         if tuner is not None:
             _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
-        return BatchResult(Js, its, conv, {"lu_s": t_lu, "solve_refine_s": t_ref})
+        return BatchResult(Js, its, conv, {"lu_solve_refine_s": t1 - t0, "extra_refine_s": t2 - t1})
+
+    @staticmethod
+    def _sweep(lib, lu, Z, V128, J, R):
+        """Correction d = LU32 \\ (V - Z J), the residual as an FP64 MFMA GEMM."""
+        n = Z.shape[0]
+        R.copy_(V128)
+        st = lib.mba_gemm(_lib.MBA_F64, n, V128.shape[1], n, -1.0, _lib.ptr(Z), Z.stride(0), _lib.ptr(J),
+                          J.stride(0), 1.0, _lib.ptr(R), R.stride(0), _lib.stream())
+        _lib.check(st, "mba_gemm")
+        return lu.solve(R.to(torch.complex64)).to(torch.complex128)
 
 
 def star_batch(n_meshes: int, seed0: int = 0, freqs=(5, 6, 7, 8, 9, 10, 11), n_dirs: int = 128,

@@ -37,7 +37,12 @@ def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alp
 class LUFactor:
     """In-place LU with partial pivoting of a square complex matrix."""
 
-    def __init__(self, A: torch.Tensor, overwrite: bool = False, profile: bool = False):
+    def __init__(self, A: torch.Tensor, overwrite: bool = False, profile: bool = False, asynchronous: bool = False,
+                 max_wg: int = 0):
+        """asynchronous: enqueue on the current stream without any host
+        synchronisation (call ``check()`` after synchronising); max_wg caps the
+        workgroups of each panel so factorisations on several streams fit side
+        by side."""
         lib = _lib.load()
         if A.dim() != 2 or A.shape[0] != A.shape[1]:
             raise ValueError("LUFactor needs a square matrix")
@@ -70,6 +75,13 @@ class LUFactor:
                 "phase_launches": dict(zip(("panel", "laswp", "trsm", "gemm"), list(nl))),
                 "gemm_flops": fl.value,
             }
+        elif asynchronous:
+            self._err = torch.zeros(1, dtype=torch.int32, device=dev)
+            self._ws = ws
+            st = lib.mba_lu_factor_async(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0), _lib.ptr(self.ipiv),
+                                         _lib.ptr(info), _lib.ptr(ws), wsb, int(max_wg), _lib.ptr(self._err),
+                                         _lib.stream())
+            _lib.check(st, "mba_lu_factor_async")
         else:
             st = lib.mba_lu_factor(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0),
                                    _lib.ptr(self.ipiv), _lib.ptr(info), _lib.ptr(ws), wsb,
@@ -81,7 +93,19 @@ class LUFactor:
                                       _lib.ptr(self.ipiv), _lib.ptr(self.aux), auxb, _lib.stream())
         _lib.check(st, "mba_lu_solve_prepare")
         self.perm = self.aux[: 4 * n].view(torch.int32)
+        self._info_t = info
+        if asynchronous:
+            self.info = None
+            return
         self.info = int(info.item())
+        if self.info != 0:
+            raise _lib.MBAError(f"mba_lu_factor: exactly zero pivot in column {self.info - 1}")
+
+    def check(self):
+        """Status of an asynchronous factorisation (after the stream has been synchronised)."""
+        if getattr(self, "_err", None) is not None and int(self._err.item()) != 0:
+            raise _lib.MBAError("mba_lu_factor_async: a cooperative panel wait timed out")
+        self.info = int(self._info_t.item())
         if self.info != 0:
             raise _lib.MBAError(f"mba_lu_factor: exactly zero pivot in column {self.info - 1}")
 
