@@ -231,8 +231,9 @@ def run_c3(args, dev):
     """Config 3: 64 seeded star meshes on geodesic spheres of frequency 5..11
     (750-3630 RWG), 128 plane waves each, ka ~ U(1, 8).  Step: FP64 assembly of
     all meshes (one launch for the far field of every mesh, one for all near
-    pairs, a gather per mesh), the RHS, FP32 LU per mesh with autotuned GEMM
-    tiles, multi-RHS solves and FP64-GEMM refinement to 1e-8."""
+    pairs, a gather per mesh), the RHS, the FP32 LUs of all meshes in lock step
+    (shared cooperative panel launches, autotuned GEMM tiles), multi-RHS
+    solves and FP64-GEMM refinement to 1e-8."""
     import numpy as np
     import torch
 
@@ -293,7 +294,8 @@ def run_c3(args, dev):
                                f"{n_dir} waves each, ka~U(1,8), EFIE 7-point rule", "parallelism": "single GPU"},
         "assembly_rhs_s": round(t_asm, 4),
         "assembly_Mpairs_s": round(1e-6 * pairs / t_asm, 1),
-        "lu_solve_refine_s": round(res.times["lu_solve_refine_s"], 4),
+        "lu_s": round(res.times["lu_s"], 4),
+        "solve_refine_s": round(res.times["solve_refine_s"], 4),
         "extra_refine_s": round(res.times["extra_refine_s"], 4),
         "refine_iterations_max": int(max(res.iterations)),
         "all_converged": bool(all(res.converged)),

@@ -183,6 +183,44 @@ int mba_lu_factor(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int3
 int mba_lu_factor_async(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
                         void *workspace, size_t workspace_bytes, int32_t max_wg, int32_t *err_out,
                         mba_stream_t stream);
+/* Several factorisations in lock step (one matrix per entry, each with the
+ * workspace of mba_lu_workspace_bytes for its order): the panels of all
+ * matrices are factored together in cooperative launches, so the per-column
+ * exchange latency is shared.  scratch: mba_lu_batched_scratch_bytes; the
+ * call copies its panel schedule there and waits for that copy.  err_out:
+ * device int32 [n_mat], the cooperative-wait flag of each factorisation.  No
+ * reference counterpart. */
+typedef struct {
+    void *a;
+    int64_t n;
+    int64_t lda;
+    int32_t *ipiv;
+    int32_t *info;
+    void *workspace;
+    size_t workspace_bytes;
+} mba_lu_matrix;
+/* Solves of several prepared factorisations (mba_lu_solve_prepare) with many
+ * right-hand sides each, all in two launches (forward L, backward U; one
+ * workgroup per job and 16 right-hand sides).  b [n, ldb] is overwritten with
+ * the solution; work: n * nrhs complex elements per job.  The call copies the
+ * job array to scratch (mba_lu_solve_batched_scratch_bytes) and waits for
+ * that copy.  No reference counterpart. */
+typedef struct {
+    const void *lu;
+    int64_t n;
+    int64_t lda;
+    const void *aux;
+    void *b;
+    int64_t ldb;
+    int64_t nrhs;
+    void *work;
+} mba_lu_solve_job;
+size_t mba_lu_solve_batched_scratch_bytes(int32_t n_jobs);
+int mba_lu_solve_batched(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs, void *scratch,
+                         size_t scratch_bytes, mba_stream_t stream);
+size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
+int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch, size_t scratch_bytes,
+                          int32_t *err_out, mba_stream_t stream);
 int mba_lu_factor_profiled(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
                            void *workspace, size_t workspace_bytes, double *phase_ms,
                            int64_t *phase_launches, double *gemm_flops, mba_stream_t stream);

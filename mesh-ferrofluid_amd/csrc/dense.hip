@@ -601,8 +601,9 @@ __device__ __forceinline__ void block_argmax(T v, int32_t r, int32_t q, T* red_v
     __syncthreads();
 }
 
+// Panel body for workgroup g of G (the workgroups owning this panel's rows).
 template <class T>
-__global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
+__device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, const int g)
 {
     constexpr int VW = sizeof(T) / 4;  // 32-bit words per real
     extern __shared__ unsigned char smem_raw[];
@@ -615,7 +616,6 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
     __shared__ cpx<T> s_inv;
 
     const int tid = threadIdx.x;
-    const int G = gridDim.x, g = blockIdx.x;
     const int nb = P.nb, R = P.R;
     const int64_t lr0 = (int64_t)g * R;  // first local row of this workgroup
     const int nr = (int)((P.m - lr0) < R ? (P.m - lr0 > 0 ? P.m - lr0 : 0) : R);
@@ -757,6 +757,28 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
         int r = idx / nb, c = idx % nb;
         P.A[(P.k + lr0 + r) * P.lda + P.k + c] = rows[idx];
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
+{
+    panel_body<T>(P, gridDim.x, blockIdx.x);
+}
+
+// The panels of several matrices in one cooperative launch: matrix i owns the
+// workgroups [wg_off[i], wg_off[i+1]) and its own message slots, so the
+// per-column exchange latency is paid once for all of them.
+template <class T>
+__global__ __launch_bounds__(PANEL_THREADS) void k_panel_batched(const PanelArgs<T>* __restrict__ Ps,
+                                                                 const int* __restrict__ wg_off, int nmat)
+{
+    int lo = 0, hi = nmat;  // wg_off[lo] <= blockIdx.x < wg_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (wg_off[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+    }
+    const PanelArgs<T> P = Ps[lo];
+    panel_body<T>(P, wg_off[lo + 1] - wg_off[lo], (int)blockIdx.x - wg_off[lo]);
 }
 
 // ============================================================== row swaps
@@ -946,7 +968,7 @@ struct LuWs {
 
 constexpr int MAX_G = 1024;
 
-inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+__host__ __device__ inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 LuWs lu_ws_layout(int prec, int64_t n, char* base)
 {
@@ -1493,6 +1515,136 @@ int lu_prepare(const void* lu, int64_t n, int64_t lda, const int32_t* ipiv, void
     return MBA_OK;
 }
 
+// ---- many right-hand sides, many matrices: one launch per triangle --------
+// Workgroup = (job, group of TSC right-hand sides).  It walks the 64-row
+// blocks in order (forward for L, backward for U): t = rhs - A(block, done) x,
+// then x(block) = inv(A_bb) t with the inverse prepared by lu_prepare.  The
+// solution is kept in the job's work array [n, nrhs]; the forward pass reads
+// the right-hand side through the row permutation, the backward pass writes
+// the result back into b.  Thread = (row tid / 4, columns 4 (tid % 4) .. +4).
+constexpr int TSC = 16;
+
+struct TsJobDev {
+    const void* lu;
+    int64_t n, lda;
+    const void* aux;
+    void* b;
+    int64_t ldb, nrhs;
+    void* work;
+};
+static_assert(sizeof(TsJobDev) == sizeof(mba_lu_solve_job), "descriptor layouts must match");
+
+template <class T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trisolve_cols(const TsJobDev* __restrict__ jobs,
+                                                       const int* __restrict__ wg_off, int njob)
+{
+    __shared__ cpx<T> tb[TS_BS][TSC];
+    int lo = 0, hi = njob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (wg_off[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+    }
+    const TsJobDev J = jobs[lo];
+    const int64_t n = J.n, lda = J.lda, nrhs = J.nrhs;
+    const int64_t c0 = (int64_t)(blockIdx.x - wg_off[lo]) * TSC;
+    const cpx<T>* A = (const cpx<T>*)J.lu;
+    const int32_t* perm = (const int32_t*)J.aux;
+    const int64_t nblk = (n + TS_BS - 1) / TS_BS;
+    const size_t esz = sizeof(cpx<T>);
+    const cpx<T>* linv = (const cpx<T>*)((const char*)J.aux + al256((size_t)n * 4));
+    const cpx<T>* uinv = (const cpx<T>*)((const char*)linv + al256((size_t)nblk * TS_BS * TS_BS * esz));
+    const cpx<T>* inv = LOWER ? linv : uinv;
+    cpx<T>* X = (cpx<T>*)J.work;
+    cpx<T>* B = (cpx<T>*)J.b;
+    const int tid = threadIdx.x, i = tid >> 2, cq = (tid & 3) * 4;
+    bool cok[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cok[q] = c0 + cq + q < nrhs;
+    for (int64_t s = 0; s < nblk; ++s) {
+        const int64_t bi = LOWER ? s : nblk - 1 - s;
+        const int64_t r0 = bi * TS_BS;
+        const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
+        T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+        if (i < bs) {
+            const int64_t row = r0 + i;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (cok[q]) {
+                    const cpx<T> v = LOWER ? B[(int64_t)perm[row] * J.ldb + c0 + cq + q]
+                                           : X[row * nrhs + c0 + cq + q];
+                    ar[q] = v.re;
+                    ai[q] = v.im;
+                }
+            const int64_t qa = LOWER ? 0 : r0 + bs, qb = LOWER ? r0 : n;
+            const cpx<T>* Ar = A + row * lda;
+            for (int64_t qq = qa; qq < qb; ++qq) {
+                const cpx<T> a = Ar[qq];
+                const cpx<T>* xr = X + qq * nrhs + c0 + cq;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (cok[q]) {
+                        const cpx<T> x = xr[q];
+                        ar[q] -= a.re * x.re - a.im * x.im;
+                        ai[q] -= a.re * x.im + a.im * x.re;
+                    }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            cpx<T> v;
+            v.re = ar[q];
+            v.im = ai[q];
+            tb[i][cq + q] = v;
+        }
+        __syncthreads();
+        if (i < bs) {
+            const cpx<T>* Mi = inv + (bi * TS_BS + i) * TS_BS;
+            T xr_[4] = {0, 0, 0, 0}, xi_[4] = {0, 0, 0, 0};
+            for (int t = 0; t < bs; ++t) {
+                const cpx<T> m = Mi[t];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const cpx<T> v = tb[t][cq + q];
+                    xr_[q] += m.re * v.re - m.im * v.im;
+                    xi_[q] += m.re * v.im + m.im * v.re;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (cok[q]) store_cpx(X + (r0 + i) * nrhs + c0 + cq + q, xr_[q], xi_[q]);
+        }
+        __syncthreads();
+    }
+    if (!LOWER) {
+        for (int64_t r = i; r < n; r += TS_BS)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (cok[q]) B[r * J.ldb + c0 + cq + q] = X[r * nrhs + c0 + cq + q];
+    }
+}
+
+template <class T>
+int lu_solve_batched(const mba_lu_solve_job* jobs, int njob, void* scratch, size_t scratch_bytes, hipStream_t st)
+{
+    std::vector<int> wg(njob + 1, 0);
+    for (int m = 0; m < njob; ++m) wg[m + 1] = wg[m] + (int)((jobs[m].nrhs + TSC - 1) / TSC);
+    const size_t need = al256((size_t)njob * sizeof(TsJobDev)) + (size_t)(njob + 1) * sizeof(int);
+    if (scratch_bytes < need) return MBA_ERR_ALLOC;
+    char* base = (char*)scratch;
+    TsJobDev* dj = (TsJobDev*)base;
+    int* dw = (int*)(base + al256((size_t)njob * sizeof(TsJobDev)));
+    if (hipMemcpyAsync(dj, jobs, (size_t)njob * sizeof(TsJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dw, wg.data(), (size_t)(njob + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+        return MBA_ERR_LAUNCH;
+    // the host tables are released on return: wait for the copies
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    hipLaunchKernelGGL((k_trisolve_cols<T, true>), dim3((unsigned)wg[njob]), dim3(256), 0, st, dj, dw, njob);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_trisolve_cols<T, false>), dim3((unsigned)wg[njob]), dim3(256), 0, st, dj, dw, njob);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
 // Solve with the prepared aux (perm + inverted diagonal blocks), nrhs <= TS_RHS.
 template <class T>
 int lu_solve_fused(const void* lu, int64_t n, int64_t lda, const void* aux, void* b, int64_t nrhs,
@@ -1680,6 +1832,191 @@ __global__ void k_c128_to_c64(const cpx<double>* __restrict__ s, int64_t m, int6
     store_cpx(d + i * ldd + j, (float)v.re, (float)v.im);
 }
 
+// ============================================================== batched LU
+// Several matrices factored in lock step: the same super-panel schedule as
+// lu_factor, with each panel phase's panels of all matrices in cooperative
+// launches of k_panel_batched (as many matrices per launch as fit one
+// workgroup per CU); row interchanges, triangular solves and GEMMs are
+// per-matrix launches.  The whole panel schedule (arguments and workgroup
+// offsets) is built on the host and copied to the device once.
+template <class T>
+int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t scratch_bytes, int32_t* err_out,
+                      hipStream_t st)
+{
+    const int prec = sizeof(T) == 4 ? MBA_F32 : MBA_F64;
+    const size_t esz = sizeof(cpx<T>);
+    const int cus = device_cus();
+    if (cus <= 0) return MBA_ERR_LAUNCH;
+    const int NB = 128;
+    const size_t lds_budget = 140 * 1024;
+    const int rmax = (int)(lds_budget / (NB * esz));
+    std::vector<LuWs> ws(nmat);
+    int64_t nmax = 0;
+    for (int i = 0; i < nmat; ++i) {
+        ws[i] = lu_ws_layout(prec, mats[i].n, (char*)mats[i].workspace);
+        if (mats[i].workspace_bytes < ws[i].total) return MBA_ERR_ALLOC;
+        nmax = mats[i].n > nmax ? mats[i].n : nmax;
+    }
+    static bool attr_set[2] = {false, false};
+    if (!attr_set[prec]) {
+        (void)hipFuncSetAttribute((const void*)k_panel_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_budget);
+        (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(128 * TRSM_COLS * esz));
+        attr_set[prec] = true;
+    }
+    // ---- host schedule: per panel phase, launches of (first entry, count, shm)
+    struct Launch {
+        size_t first, count, woff;
+        int wg;
+        size_t shm;
+    };
+    std::vector<PanelArgs<T>> args;
+    std::vector<int> woffs;
+    std::vector<std::vector<Launch>> phases;  // phase p = 2 s + half
+    const int64_t steps = (nmax + 2 * NB - 1) / (2 * NB);
+    for (int64_t s = 0; s < steps; ++s)
+        for (int half = 0; half < 2; ++half) {
+            std::vector<Launch> ls;
+            Launch cur{args.size(), 0, woffs.size(), 0, 0};
+            woffs.push_back(0);
+            for (int i = 0; i < nmat; ++i) {
+                const int64_t n = mats[i].n;
+                const int64_t k = 2 * NB * s + (int64_t)half * NB;
+                if (k >= n) continue;
+                const int nb = (int)((n - k) < NB ? (n - k) : NB);
+                const int64_t m = n - k;
+                int G = (int)((m + rmax - 1) / rmax);
+                if (G > MAX_G) return MBA_ERR_SIZE;
+                const int R = (int)((m + G - 1) / G);
+                G = (int)((m + R - 1) / R);
+                if (G > cus) return MBA_ERR_SIZE;
+                if (cur.wg + G > cus) {
+                    ls.push_back(cur);
+                    cur = Launch{args.size(), 0, woffs.size(), 0, 0};
+                    woffs.push_back(0);
+                }
+                PanelArgs<T> P;
+                P.A = (cpx<T>*)mats[i].a;
+                P.lda = mats[i].lda;
+                P.k = k;
+                P.m = m;
+                P.nb = nb;
+                P.R = R;
+                P.ipiv = mats[i].ipiv;
+                P.info = mats[i].info;
+                P.err = ws[i].err;
+                P.meta = ws[i].meta;
+                P.cval = ws[i].cval;
+                P.cand = ws[i].cand;
+                P.diag = ws[i].diag;
+                args.push_back(P);
+                cur.count += 1;
+                cur.wg += G;
+                woffs.push_back(cur.wg);
+                const size_t shm = (size_t)R * nb * esz;
+                cur.shm = shm > cur.shm ? shm : cur.shm;
+            }
+            if (cur.count > 0) ls.push_back(cur);
+            else woffs.pop_back();
+            phases.push_back(ls);
+        }
+    const size_t need = args.size() * sizeof(PanelArgs<T>) + 256 + woffs.size() * sizeof(int);
+    if (need > scratch_bytes) return MBA_ERR_ALLOC;
+    char* base = (char*)scratch;
+    PanelArgs<T>* d_args = (PanelArgs<T>*)base;
+    int* d_woff = (int*)(base + al256(args.size() * sizeof(PanelArgs<T>)));
+    if (!args.empty() &&
+        (hipMemcpyAsync(d_args, args.data(), args.size() * sizeof(PanelArgs<T>), hipMemcpyHostToDevice, st) !=
+             hipSuccess ||
+         hipMemcpyAsync(d_woff, woffs.data(), woffs.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess))
+        return MBA_ERR_LAUNCH;
+    // the host tables are released on return: wait for the copies
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    for (int i = 0; i < nmat; ++i) {
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ws[i].err, 32);
+        hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((ws[i].msg_words + 255) / 256)), dim3(256), 0, st,
+                           (unsigned long long*)ws[i].meta, (int)ws[i].msg_words);
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, mats[i].info, 1);
+    }
+    MBA_LAUNCH_CHECK();
+    auto gemm_rect = [&](int i, int64_t r0, int64_t M, int64_t c0, int64_t N, int64_t k0, int64_t kk) -> int {
+        if (M <= 0 || N <= 0 || kk <= 0) return MBA_OK;
+        cpx<T>* A = (cpx<T>*)mats[i].a;
+        const int64_t lda = mats[i].lda;
+        return gemm<T>(M, N, kk, -1.0, A + r0 * lda + k0, lda, A + k0 * lda + c0, lda, 1.0, A + r0 * lda + c0, lda,
+                       st);
+    };
+    auto swaps = [&](int i, int64_t k, int nb) -> int {
+        const int64_t n = mats[i].n;
+        const int64_t right = n - (k + nb);
+        if (k == 0 && right <= 0) return MBA_OK;
+        hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(64), 0, st, mats[i].ipiv, k, nb, ws[i].slot_row, ws[i].plan,
+                           ws[i].plan_n);
+        const int64_t blocks = (k + LASWP_COLS - 1) / LASWP_COLS + (right + LASWP_COLS - 1) / LASWP_COLS;
+        hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)blocks), dim3(256), (size_t)2 * nb * LASWP_COLS * esz, st,
+                           (cpx<T>*)mats[i].a, mats[i].lda, (int64_t)0, n, k, k + nb, nb, ws[i].slot_row,
+                           ws[i].plan, ws[i].plan_n);
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
+    };
+    auto trsm = [&](int i, int64_t r0, int nb, int64_t c_lo) -> int {
+        const int64_t n = mats[i].n;
+        if (c_lo >= n) return MBA_OK;
+        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n - c_lo + TRSM_COLS - 1) / TRSM_COLS)), dim3(256),
+                           (size_t)nb * TRSM_COLS * esz, st, (cpx<T>*)mats[i].a, mats[i].lda, r0, nb, c_lo, n);
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
+    };
+    auto run_phase = [&](int p) -> int {
+        for (const Launch& L : phases[p]) {
+            const PanelArgs<T>* pa = d_args + L.first;
+            const int* wo = d_woff + L.woff;
+            int nm = (int)L.count;
+            void* kargs[] = {(void*)&pa, (void*)&wo, (void*)&nm};
+            if (hipLaunchCooperativeKernel((const void*)k_panel_batched<T>, dim3(L.wg), dim3(PANEL_THREADS), kargs,
+                                           (unsigned)L.shm, st) != hipSuccess)
+                return MBA_ERR_LAUNCH;
+        }
+        return MBA_OK;
+    };
+    int rc;
+    for (int64_t s = 0; s < steps; ++s) {
+        const int64_t k = 2 * NB * s;
+        if ((rc = run_phase((int)(2 * s))) != MBA_OK) return rc;
+        for (int i = 0; i < nmat; ++i) {
+            const int64_t n = mats[i].n;
+            if (k >= n) continue;
+            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+            const int64_t k2 = k + nb1;
+            const int nb2 = (int)((n - k2) < NB ? ((n - k2) > 0 ? (n - k2) : 0) : NB);
+            if ((rc = swaps(i, k, nb1)) != MBA_OK) return rc;
+            if (k2 >= n) continue;
+            if ((rc = trsm(i, k, nb1, k2)) != MBA_OK) return rc;
+            if ((rc = gemm_rect(i, k2, n - k2, k2, nb2, k, nb1)) != MBA_OK) return rc;
+        }
+        if ((rc = run_phase((int)(2 * s + 1))) != MBA_OK) return rc;
+        for (int i = 0; i < nmat; ++i) {
+            const int64_t n = mats[i].n;
+            if (k >= n) continue;
+            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+            const int64_t k2 = k + nb1;
+            if (k2 >= n) continue;
+            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
+            if ((rc = swaps(i, k2, nb2)) != MBA_OK) return rc;
+            const int64_t c3 = k2 + nb2;
+            if (c3 >= n) continue;
+            if ((rc = gemm_rect(i, k2, nb2, c3, n - c3, k, nb1)) != MBA_OK) return rc;
+            if ((rc = trsm(i, k2, nb2, c3)) != MBA_OK) return rc;
+            if ((rc = gemm_rect(i, c3, n - c3, c3, n - c3, k, nb1 + nb2)) != MBA_OK) return rc;
+        }
+    }
+    for (int i = 0; i < nmat; ++i)
+        hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, st, (const int32_t*)ws[i].err, err_out + i, 1);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -1715,6 +2052,53 @@ int mba_lu_factor_async(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv
                                 nullptr, max_wg, err_out);
     return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
                              nullptr, max_wg, err_out);
+}
+
+size_t mba_lu_solve_batched_scratch_bytes(int32_t n_jobs)
+{
+    if (n_jobs <= 0) return 0;
+    return al256((size_t)n_jobs * sizeof(TsJobDev)) + (size_t)(n_jobs + 1) * sizeof(int);
+}
+
+int mba_lu_solve_batched(int prec, const mba_lu_solve_job* jobs, int32_t n_jobs, void* scratch,
+                         size_t scratch_bytes, mba_stream_t stream)
+{
+    if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    int64_t wgs = 0;
+    for (int m = 0; m < n_jobs; ++m) {
+        const mba_lu_solve_job& j = jobs[m];
+        if (!j.lu || !j.aux || !j.b || !j.work) return MBA_ERR_ARG;
+        if (j.n <= 0 || j.nrhs <= 0) return MBA_ERR_SIZE;
+        if (j.lda < j.n || j.ldb < j.nrhs) return MBA_ERR_ARG;
+        wgs += (j.nrhs + TSC - 1) / TSC;
+    }
+    if (wgs > 0x7fffffff) return MBA_ERR_SIZE;
+    if (prec == MBA_F32) return lu_solve_batched<float>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+    return lu_solve_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+}
+
+size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max)
+{
+    if (n_mat <= 0 || n_max <= 0) return 0;
+    const size_t ent = (size_t)n_mat * 2 * (size_t)((n_max + 255) / 256);
+    const size_t pa = prec == MBA_F32 ? sizeof(PanelArgs<float>) : sizeof(PanelArgs<double>);
+    return al256(ent * pa) + 256 + (2 * ent + 2 * (size_t)((n_max + 255) / 256) * 2 + 8) * sizeof(int);
+}
+
+int mba_lu_factor_batched(int prec, const mba_lu_matrix* mats, int32_t n_mat, void* scratch, size_t scratch_bytes,
+                          int32_t* err_out, mba_stream_t stream)
+{
+    if (!mats || !scratch || !err_out || n_mat <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    for (int i = 0; i < n_mat; ++i) {
+        const mba_lu_matrix& m = mats[i];
+        if (!m.a || !m.ipiv || !m.info || !m.workspace) return MBA_ERR_ARG;
+        if (m.n <= 0 || m.n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+        if (m.lda < m.n) return MBA_ERR_ARG;
+    }
+    if (prec == MBA_F32) return lu_factor_batched<float>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+    return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
 }
 
 int mba_lu_factor_profiled(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,

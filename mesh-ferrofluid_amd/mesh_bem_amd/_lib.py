@@ -25,6 +25,20 @@ class NsMesh(ctypes.Structure):
                 ("ldzf", ctypes.c_int64), ("n_near", ctypes.c_int64)]
 
 
+class LuMatrix(ctypes.Structure):
+    """mba_lu_matrix of include/mesh_bem_amd.h."""
+
+    _fields_ = [("a", ctypes.c_void_p), ("n", ctypes.c_int64), ("lda", ctypes.c_int64), ("ipiv", ctypes.c_void_p),
+                ("info", ctypes.c_void_p), ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t)]
+
+
+class LuSolveJob(ctypes.Structure):
+    """mba_lu_solve_job of include/mesh_bem_amd.h."""
+
+    _fields_ = [("lu", ctypes.c_void_p), ("n", ctypes.c_int64), ("lda", ctypes.c_int64), ("aux", ctypes.c_void_p),
+                ("b", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("nrhs", ctypes.c_int64), ("work", ctypes.c_void_p)]
+
+
 class MgnVMat(ctypes.Structure):
     """mba_mgn_vmat of include/mesh_bem_amd.h."""
 
@@ -59,6 +73,10 @@ SIGNATURES = {
     "mba_lu_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_lu_factor": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp]),
     "mba_lu_factor_async": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _i32, _i32p, _vp]),
+    "mba_lu_solve_batched_scratch_bytes": (_sz, [_i32]),
+    "mba_lu_solve_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
+    "mba_lu_batched_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),
+    "mba_lu_factor_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _i32p, _vp]),
     "mba_lu_factor_profiled": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp, _vp,
                                           _vp, _vp]),
     "mba_lu_pivots_to_perm": (_c.c_int, [_i32p, _i64, _i32p, _vp]),

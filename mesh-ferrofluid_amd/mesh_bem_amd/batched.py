@@ -4,10 +4,10 @@
   assembly launch (``mba_mom_zmat_faceedge_ns_batched``: a 1-D grid over the
   tiles of all meshes) plus one launch for all near pairs, then a fixed-order
   gather per mesh into its RWG matrix.
-* ``BatchedMoM.solve``: per mesh, the FP32 LU of the rounded matrix (GEMM tile
-  variant picked by the autotuner for the mesh size), the multi-RHS triangular
-  solves, and refinement whose residual R = V - Z J is an FP64 MFMA GEMM over
-  all right-hand sides at once.
+* ``BatchedMoM.solve``: the FP32 LUs of all rounded matrices in lock step
+  (GEMM tile variant picked by the autotuner for the size), the triangular
+  solves of every mesh and right-hand side in two launches, and refinement
+  whose residual R = V - Z J is an FP64 MFMA GEMM over all right-hand sides.
 * ``GemmTuner``: times the FP32 GEMM tile variants on the LU's trailing-update
   shapes per size bucket and caches the fastest (optionally in a JSON file).
 """
@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .dense import LUFactor, to_c64
+from .dense import lu_factor_batched, lu_solve_batched, to_c64
 from .geometry import get_edge_unique
 from .mom import MomMesh, rhs_ns, rule_tensor
 
@@ -165,70 +165,60 @@ class BatchedMoM:
         return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
-              tol: float = 1e-8, maxit: int = 10, streams: int = 4, sweeps: int = 3) -> BatchResult:
-        """Mixed precision per mesh, meshes spread over ``streams`` HIP streams
-        (default priority) so their factorisations and solves overlap: FP32 LU
-        (GEMM variant of the largest size bucket from the tuner; panels capped
-        at CUs / streams workgroups), ``sweeps`` refinement sweeps with FP64
-        GEMM residuals enqueued without host synchronisation, then a host check
-        of the last corrections; meshes not yet at ``tol`` continue alone."""
+              tol: float = 1e-8, maxit: int = 10, sweeps: int = 3) -> BatchResult:
+        """Mixed precision for all meshes: the FP32 LUs in lock step
+        (``lu_factor_batched``: the panels of every matrix in shared
+        cooperative launches; GEMM tile variant of the largest size bucket from
+        the tuner), the triangular solves of all meshes and right-hand sides in
+        two launches (``lu_solve_batched``), ``sweeps`` refinement sweeps with
+        FP64 GEMM residuals enqueued without host synchronisation, then a host
+        check of the last corrections; meshes not yet at ``tol`` continue alone."""
         lib = _lib.load()
-        n_mesh = len(Zs)
-        S = max(1, min(streams, n_mesh))
         if tuner is not None:
             tuner.apply(max(Z.shape[0] for Z in Zs))
-        cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-        main = torch.cuda.current_stream()
-        sts = [torch.cuda.Stream(device=self.device) for _ in range(S)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        state = []
-        for i, (Z, V) in enumerate(zip(Zs, Vs)):
-            s = sts[i % S]
-            s.wait_stream(main)
-            with torch.cuda.stream(s):
-                lu = LUFactor(to_c64(Z), overwrite=True, asynchronous=True, max_wg=max(1, cus // S))
-                V128 = V.to(torch.complex128).contiguous()
-                J = lu.solve(V.to(torch.complex64)).to(torch.complex128)
-                R = torch.empty_like(V128)
-                rel = None
-                for _ in range(sweeps):
-                    d = self._sweep(lib, lu, Z, V128, J, R)
-                    J += d
-                    rel = (torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()
-                state.append((lu, J, R, V128, rel))
-        for s in sts:
-            main.wait_stream(s)
+        lus = lu_factor_batched([to_c64(Z) for Z in Zs], overwrite=True)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        Js, its, conv = [], [], []
-        for (lu, J, R, V128, rel), Z in zip(state, Zs):
+        V128s = [V.to(torch.complex128).contiguous() for V in Vs]
+        Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs])]
+        Rs = [torch.empty_like(V) for V in V128s]
+        rel = [None] * len(Zs)
+        for _ in range(sweeps):
+            Ds = lu_solve_batched(lus, [self._residual(lib, Z, V, J, R) for Z, V, J, R in zip(Zs, V128s, Js, Rs)])
+            for i, (J, D) in enumerate(zip(Js, Ds)):
+                J += D
+                rel[i] = (torch.linalg.vector_norm(D, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        its, conv = [], []
+        for i, (lu, Z, J) in enumerate(zip(lus, Zs, Js)):
             lu.check()
             it = sweeps
-            ok = float(rel) <= tol
+            ok = float(rel[i]) <= tol
             while not ok and it < maxit:
-                d = self._sweep(lib, lu, Z, V128, J, R)
+                d = lu.solve(self._residual(lib, Z, V128s[i], J, Rs[i])).to(torch.complex128)
                 J += d
                 it += 1
                 ok = float((torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()) <= tol
-            Js.append(J)
             its.append(it)
             conv.append(ok)
         torch.cuda.synchronize()
-        t2 = time.perf_counter()
+        t3 = time.perf_counter()
         if tuner is not None:
             _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
-        return BatchResult(Js, its, conv, {"lu_solve_refine_s": t1 - t0, "extra_refine_s": t2 - t1})
+        return BatchResult(Js, its, conv, {"lu_s": t1 - t0, "solve_refine_s": t2 - t1, "extra_refine_s": t3 - t2})
 
     @staticmethod
-    def _sweep(lib, lu, Z, V128, J, R):
-        """Correction d = LU32 \\ (V - Z J), the residual as an FP64 MFMA GEMM."""
+    def _residual(lib, Z, V128, J, R):
+        """R = V - Z J as an FP64 MFMA GEMM, returned rounded to complex64."""
         n = Z.shape[0]
         R.copy_(V128)
         st = lib.mba_gemm(_lib.MBA_F64, n, V128.shape[1], n, -1.0, _lib.ptr(Z), Z.stride(0), _lib.ptr(J),
                           J.stride(0), 1.0, _lib.ptr(R), R.stride(0), _lib.stream())
         _lib.check(st, "mba_gemm")
-        return lu.solve(R.to(torch.complex64)).to(torch.complex128)
+        return R.to(torch.complex64)
 
 
 def star_batch(n_meshes: int, seed0: int = 0, freqs=(5, 6, 7, 8, 9, 10, 11), n_dirs: int = 128,

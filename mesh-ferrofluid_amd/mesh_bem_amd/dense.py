@@ -101,6 +101,23 @@ class LUFactor:
         if self.info != 0:
             raise _lib.MBAError(f"mba_lu_factor: exactly zero pivot in column {self.info - 1}")
 
+    @classmethod
+    def _from_factored(cls, lu: torch.Tensor, ipiv: torch.Tensor, info: torch.Tensor, err: torch.Tensor):
+        """Wrap a factorisation done elsewhere (batched): prepare the solve tables."""
+        lib = _lib.load()
+        self = cls.__new__(cls)
+        self.lu, self.ipiv, self.n = lu, ipiv, lu.shape[0]
+        self.prec = _lib.prec_of(lu.dtype)
+        self.profile = None
+        auxb = lib.mba_lu_solve_aux_bytes(self.prec, self.n)
+        self.aux = torch.empty(auxb, dtype=torch.uint8, device=lu.device)
+        st = lib.mba_lu_solve_prepare(self.prec, _lib.ptr(lu), self.n, lu.stride(0), _lib.ptr(ipiv),
+                                      _lib.ptr(self.aux), auxb, _lib.stream())
+        _lib.check(st, "mba_lu_solve_prepare")
+        self.perm = self.aux[: 4 * self.n].view(torch.int32)
+        self._info_t, self._err, self.info = info, err, None
+        return self
+
     def check(self):
         """Status of an asynchronous factorisation (after the stream has been synchronised)."""
         if getattr(self, "_err", None) is not None and int(self._err.item()) != 0:
@@ -122,6 +139,73 @@ class LUFactor:
                               _lib.stream())
         _lib.check(st, "mba_lu_solve")
         return X.reshape(-1) if vec else X
+
+
+def lu_factor_batched(mats, overwrite: bool = True):
+    """LU of several square complex matrices (same dtype) in lock step
+    (``mba_lu_factor_batched``); returns one LUFactor per matrix.  Nothing is
+    synchronised: call ``check()`` on each after synchronising."""
+    lib = _lib.load()
+    if not mats:
+        return []
+    dt = mats[0].dtype
+    prec = _lib.prec_of(dt)
+    dev = mats[0].device
+    n_mat = len(mats)
+    lus, keep = [], []
+    desc = (_lib.LuMatrix * n_mat)()
+    err = torch.zeros(n_mat, dtype=torch.int32, device=dev)
+    for i, A in enumerate(mats):
+        if A.dim() != 2 or A.shape[0] != A.shape[1] or A.dtype != dt:
+            raise ValueError("lu_factor_batched needs square matrices of one dtype")
+        lu = A if overwrite else A.clone()
+        if lu.stride(1) != 1:
+            lu = lu.contiguous()
+        n = lu.shape[0]
+        ipiv = torch.empty(n, dtype=torch.int32, device=dev)
+        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        wsb = lib.mba_lu_workspace_bytes(prec, n)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        desc[i].a, desc[i].n, desc[i].lda = lu.data_ptr(), n, lu.stride(0)
+        desc[i].ipiv, desc[i].info = ipiv.data_ptr(), info.data_ptr()
+        desc[i].workspace, desc[i].workspace_bytes = ws.data_ptr(), wsb
+        lus.append((lu, ipiv, info))
+        keep.append(ws)
+    nmax = max(A.shape[0] for A in mats)
+    sb = lib.mba_lu_batched_scratch_bytes(prec, n_mat, nmax)
+    scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
+    st = lib.mba_lu_factor_batched(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err), _lib.stream())
+    _lib.check(st, "mba_lu_factor_batched")
+    out = [LUFactor._from_factored(lu, ipiv, info, err[i:i + 1]) for i, (lu, ipiv, info) in enumerate(lus)]
+    out[0]._keep = (keep, scratch)
+    return out
+
+
+def lu_solve_batched(lus, Bs):
+    """X_i = A_i^{-1} B_i for prepared factorisations lus[i] (one dtype) and
+    right-hand sides Bs[i] [n_i, nrhs_i]; two launches for all of them.
+    Returns new tensors (the inputs are not modified)."""
+    lib = _lib.load()
+    n_job = len(lus)
+    if n_job == 0:
+        return []
+    if len(Bs) != n_job or any(lu.lu.dtype != lus[0].lu.dtype for lu in lus):
+        raise ValueError("lu_solve_batched: one right-hand side per factor, one dtype")
+    jobs = (_lib.LuSolveJob * n_job)()
+    Xs, keep = [], []
+    for i, (lu, B) in enumerate(zip(lus, Bs)):
+        X = B.reshape(lu.n, -1).to(lu.lu.dtype).clone().contiguous()
+        W = torch.empty_like(X)
+        jobs[i].lu, jobs[i].n, jobs[i].lda = lu.lu.data_ptr(), lu.n, lu.lu.stride(0)
+        jobs[i].aux, jobs[i].b, jobs[i].ldb = lu.aux.data_ptr(), X.data_ptr(), X.stride(0)
+        jobs[i].nrhs, jobs[i].work = X.shape[1], W.data_ptr()
+        Xs.append(X)
+        keep.append(W)
+    sb = lib.mba_lu_solve_batched_scratch_bytes(n_job)
+    scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)
+    st = lib.mba_lu_solve_batched(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.stream())
+    _lib.check(st, "mba_lu_solve_batched")
+    return Xs
 
 
 def residual_c128(Z: torch.Tensor, X: torch.Tensor, V: torch.Tensor) -> torch.Tensor:
