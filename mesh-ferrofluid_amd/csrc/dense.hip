@@ -1521,7 +1521,14 @@ int lu_prepare(const void* lu, int64_t n, int64_t lda, const int32_t* ipiv, void
 // then x(block) = inv(A_bb) t with the inverse prepared by lu_prepare.  The
 // solution is kept in the job's work array [n, nrhs]; the forward pass reads
 // the right-hand side through the row permutation, the backward pass writes
-// the result back into b.  Thread = (row tid / 4, columns 4 (tid % 4) .. +4).
+// the result back into b.
+//
+// Thread = (row i = tid / 4 of the block, k-lane kq = tid % 4).  The update
+// A(block, done) x runs over chunks of CK columns: the chunk of x (CK x TSC) is
+// staged in LDS, each thread holds its row's A entries k = 4 u + kq in
+// registers and accumulates all TSC columns; the next chunk's loads are issued
+// before the current chunk is multiplied.  The four k-lanes of a row are then
+// summed with lane exchanges and lane kq owns columns 4 kq .. 4 kq + 3.
 constexpr int TSC = 16;
 
 struct TsJobDev {
@@ -1534,11 +1541,56 @@ struct TsJobDev {
 };
 static_assert(sizeof(TsJobDev) == sizeof(mba_lu_solve_job), "descriptor layouts must match");
 
+template <class T>
+struct TsGeom {
+    static constexpr int CK = sizeof(T) == 4 ? 64 : 32;            // columns per staged chunk
+    static constexpr int KPT = CK / 4;                               // chunk columns per thread
+    static constexpr int LDC = TSC + 16 / (int)sizeof(cpx<T>);       // LDS row, 16 bytes of padding
+    static constexpr int XPT = CK * TSC / 256;                       // staged elements per thread
+};
+
+template <class T>
+__device__ __forceinline__ void quad_sum(T* v)
+{
+#pragma unroll
+    for (int c = 0; c < TSC; ++c) {
+        v[c] += __shfl_xor(v[c], 1);
+        v[c] += __shfl_xor(v[c], 2);
+    }
+}
+
+// Loads of one chunk: the thread's A entries (row Ar, columns k0 + 4 u + kq)
+// and its staged x elements (rows k0 + (tid + 256 e) / TSC, column col); zero
+// beyond qb.
+template <class T>
+__device__ __forceinline__ void ts_fetch(cpx<T> (&an)[TsGeom<T>::KPT], cpx<T> (&xn)[TsGeom<T>::XPT],
+                                         const cpx<T>* __restrict__ Ar, const cpx<T>* __restrict__ X, int64_t k0,
+                                         int64_t qb, int kq, bool rok, int tid, bool col_ok, int64_t nrhs,
+                                         int64_t col)
+{
+#pragma unroll
+    for (int u = 0; u < TsGeom<T>::KPT; ++u) {
+        const int64_t k = k0 + 4 * u + kq;
+        an[u].re = 0;
+        an[u].im = 0;
+        if (rok && k < qb) an[u] = Ar[k];
+    }
+#pragma unroll
+    for (int e = 0; e < TsGeom<T>::XPT; ++e) {
+        const int64_t kk = k0 + (tid + 256 * e) / TSC;
+        xn[e].re = 0;
+        xn[e].im = 0;
+        if (kk < qb && col_ok) xn[e] = X[kk * nrhs + col];
+    }
+}
+
 template <class T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trisolve_cols(const TsJobDev* __restrict__ jobs,
                                                        const int* __restrict__ wg_off, int njob)
 {
-    __shared__ cpx<T> tb[TS_BS][TSC];
+    using G = TsGeom<T>;
+    __shared__ cpx<T> xs[G::CK * G::LDC];
+    __shared__ cpx<T> tb[TS_BS * G::LDC];
     int lo = 0, hi = njob;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -1556,70 +1608,97 @@ __global__ __launch_bounds__(256) void k_trisolve_cols(const TsJobDev* __restric
     const cpx<T>* inv = LOWER ? linv : uinv;
     cpx<T>* X = (cpx<T>*)J.work;
     cpx<T>* B = (cpx<T>*)J.b;
-    const int tid = threadIdx.x, i = tid >> 2, cq = (tid & 3) * 4;
-    bool cok[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) cok[q] = c0 + cq + q < nrhs;
+    const int tid = threadIdx.x, i = tid >> 2, kq = tid & 3;
+    const int sc = tid % TSC;  // staged column: (tid + 256 e) % TSC == tid % TSC
+    const bool sc_ok = c0 + sc < nrhs;
+    cpx<T> zero;
+    zero.re = 0;
+    zero.im = 0;
     for (int64_t s = 0; s < nblk; ++s) {
         const int64_t bi = LOWER ? s : nblk - 1 - s;
         const int64_t r0 = bi * TS_BS;
         const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
-        T ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
-        if (i < bs) {
-            const int64_t row = r0 + i;
+        const bool rok = i < bs;
+        const int64_t row = r0 + (rok ? i : 0);
+        const int64_t qa = LOWER ? 0 : r0 + bs, qb = LOWER ? r0 : n;
+        T pr[TSC], pi[TSC];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (cok[q]) {
-                    const cpx<T> v = LOWER ? B[(int64_t)perm[row] * J.ldb + c0 + cq + q]
-                                           : X[row * nrhs + c0 + cq + q];
-                    ar[q] = v.re;
-                    ai[q] = v.im;
-                }
-            const int64_t qa = LOWER ? 0 : r0 + bs, qb = LOWER ? r0 : n;
-            const cpx<T>* Ar = A + row * lda;
-            for (int64_t qq = qa; qq < qb; ++qq) {
-                const cpx<T> a = Ar[qq];
-                const cpx<T>* xr = X + qq * nrhs + c0 + cq;
+        for (int c = 0; c < TSC; ++c) pr[c] = pi[c] = 0;
+        const cpx<T>* Ar = A + row * lda;
+        if (qa < qb) {
+            cpx<T> an[G::KPT], xn[G::XPT];
+            ts_fetch<T>(an, xn, Ar, X, qa, qb, kq, rok, tid, sc_ok, nrhs, c0 + sc);
+            for (int64_t k0 = qa; k0 < qb; k0 += G::CK) {
+                __syncthreads();  // the previous chunk's readers are done
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (cok[q]) {
-                        const cpx<T> x = xr[q];
-                        ar[q] -= a.re * x.re - a.im * x.im;
-                        ai[q] -= a.re * x.im + a.im * x.re;
+                for (int e = 0; e < G::XPT; ++e) xs[((tid + 256 * e) / TSC) * G::LDC + sc] = xn[e];
+                cpx<T> a[G::KPT];
+#pragma unroll
+                for (int u = 0; u < G::KPT; ++u) a[u] = an[u];
+                __syncthreads();
+                if (k0 + G::CK < qb) ts_fetch<T>(an, xn, Ar, X, k0 + G::CK, qb, kq, rok, tid, sc_ok, nrhs, c0 + sc);
+#pragma unroll
+                for (int u = 0; u < G::KPT; ++u) {
+                    const cpx<T>* xr = xs + (4 * u + kq) * G::LDC;
+#pragma unroll
+                    for (int c = 0; c < TSC; ++c) {
+                        const cpx<T> x = xr[c];
+                        pr[c] += a[u].re * x.re - a[u].im * x.im;
+                        pi[c] += a[u].re * x.im + a[u].im * x.re;
                     }
+                }
             }
+            quad_sum(pr);
+            quad_sum(pi);
         }
+        // t = rhs - update for the lane's own columns
+        cpx<T> rhs[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            cpx<T> v;
-            v.re = ar[q];
-            v.im = ai[q];
-            tb[i][cq + q] = v;
+            const int64_t col = c0 + 4 * kq + q;
+            rhs[q] = zero;
+            if (rok && col < nrhs) rhs[q] = LOWER ? B[(int64_t)perm[row] * J.ldb + col] : X[row * nrhs + col];
         }
-        __syncthreads();
-        if (i < bs) {
-            const cpx<T>* Mi = inv + (bi * TS_BS + i) * TS_BS;
-            T xr_[4] = {0, 0, 0, 0}, xi_[4] = {0, 0, 0, 0};
-            for (int t = 0; t < bs; ++t) {
-                const cpx<T> m = Mi[t];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const cpx<T> v = tb[t][cq + q];
-                    xr_[q] += m.re * v.re - m.im * v.im;
-                    xi_[q] += m.re * v.im + m.im * v.re;
-                }
+        for (int c = 0; c < TSC; ++c)
+            if ((c >> 2) == kq) {
+                cpx<T> v;
+                v.re = rhs[c & 3].re - pr[c];
+                v.im = rhs[c & 3].im - pi[c];
+                tb[i * G::LDC + c] = v;
             }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (cok[q]) store_cpx(X + (r0 + i) * nrhs + c0 + cq + q, xr_[q], xi_[q]);
-        }
         __syncthreads();
+        // x(block) = inv(A_bb) t; inverse rows and columns beyond bs are zero
+        T xr[TSC], xi[TSC];
+#pragma unroll
+        for (int c = 0; c < TSC; ++c) xr[c] = xi[c] = 0;
+        const cpx<T>* Mi = inv + (bi * TS_BS + i) * TS_BS;
+#pragma unroll 4
+        for (int u = 0; u < TS_BS / 4; ++u) {
+            const int t = 4 * u + kq;
+            const cpx<T> m = Mi[t];
+            const cpx<T>* tr = tb + t * G::LDC;
+#pragma unroll
+            for (int c = 0; c < TSC; ++c) {
+                const cpx<T> v = tr[c];
+                xr[c] += m.re * v.re - m.im * v.im;
+                xi[c] += m.re * v.im + m.im * v.re;
+            }
+        }
+        quad_sum(xr);
+        quad_sum(xi);
+#pragma unroll
+        for (int c = 0; c < TSC; ++c)
+            if ((c >> 2) == kq && rok && c0 + c < nrhs) store_cpx(X + row * nrhs + c0 + c, xr[c], xi[c]);
+        __syncthreads();  // block rows of x visible to the next block's update; tb free
     }
     if (!LOWER) {
         for (int64_t r = i; r < n; r += TS_BS)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (cok[q]) B[r * J.ldb + c0 + cq + q] = X[r * nrhs + c0 + cq + q];
+            for (int q = 0; q < 4; ++q) {
+                const int64_t col = c0 + 4 * kq + q;
+                if (col < nrhs) B[r * J.ldb + col] = X[r * nrhs + col];
+            }
     }
 }
 
