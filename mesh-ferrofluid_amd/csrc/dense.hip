@@ -787,11 +787,9 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel_batched(const PanelArgs
 // other pivot rows get slots nb.. in order of first appearance.  plan[j] =
 // (slot_a, slot_b) per swap; slot_row[s] = row of slot s; returns |S| in
 // plan_n.
-__global__ void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
-                            int32_t* __restrict__ slot_row, int32_t* __restrict__ plan,
-                            int32_t* __restrict__ plan_n)
+__device__ void swap_plan_body(const int32_t* __restrict__ ipiv, int64_t k, int nb, int32_t* __restrict__ slot_row,
+                               int32_t* __restrict__ plan, int32_t* __restrict__ plan_n)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int ns = nb;
     for (int s = 0; s < nb; ++s) slot_row[s] = (int32_t)(k + s);
     for (int j = 0; j < nb; ++j) {
@@ -817,22 +815,27 @@ __global__ void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
     *plan_n = ns;
 }
 
+__global__ void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
+                            int32_t* __restrict__ slot_row, int32_t* __restrict__ plan,
+                            int32_t* __restrict__ plan_n)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    swap_plan_body(ipiv, k, nb, slot_row, plan, plan_n);
+}
+
 // Each block: LASWP_COLS complex columns of the column range; the |S| rows are
 // staged in LDS, swapped in place, written back.
 constexpr int LASWP_COLS = 16;
 
 template <class T>
-__global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t lda, int64_t c_lo,
-                                               int64_t c_hi, int64_t skip_lo, int64_t skip_hi,
-                                               int nb, const int32_t* __restrict__ slot_row,
-                                               const int32_t* __restrict__ plan,
-                                               const int32_t* __restrict__ plan_n)
+__device__ void laswp_body(cpx<T>* __restrict__ A, int64_t lda, int64_t c_lo, int64_t c_hi, int64_t skip_lo,
+                           int64_t skip_hi, int nb, const int32_t* __restrict__ slot_row,
+                           const int32_t* __restrict__ plan, const int32_t* __restrict__ plan_n, int64_t blk,
+                           cpx<T>* buf)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cpx<T>* buf = (cpx<T>*)smem_raw;  // [2 nb][LASWP_COLS]
     const int ns = *plan_n;
     // map block -> columns, skipping [skip_lo, skip_hi)
-    int64_t c0 = c_lo + (int64_t)blockIdx.x * LASWP_COLS;
+    int64_t c0 = c_lo + blk * LASWP_COLS;
     if (c0 >= skip_lo) c0 += (skip_hi - skip_lo);
     if (c0 >= c_hi) return;
     int64_t cend = c0 + LASWP_COLS;
@@ -862,6 +865,18 @@ __global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t l
     }
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t lda, int64_t c_lo,
+                                               int64_t c_hi, int64_t skip_lo, int64_t skip_hi,
+                                               int nb, const int32_t* __restrict__ slot_row,
+                                               const int32_t* __restrict__ plan,
+                                               const int32_t* __restrict__ plan_n)
+{
+    extern __shared__ unsigned char smem_raw[];
+    laswp_body<T>(A, lda, c_lo, c_hi, skip_lo, skip_hi, nb, slot_row, plan, plan_n, (int64_t)blockIdx.x,
+                  (cpx<T>*)smem_raw);  // buffer [2 nb][LASWP_COLS]
+}
+
 // ============================================================== TRSM (L unit lower)
 // X = L11^{-1} X for X = rows [k, k+nb) x columns [c0, c0 + 64) of A, L11 the
 // unit lower panel block.  X is staged in LDS and solved in blocks of 16 rows:
@@ -872,13 +887,11 @@ constexpr int TRSM_COLS = 64;
 constexpr int TRSM_RB = 16;
 
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t lda, int64_t k,
-                                                int nb, int64_t col_begin, int64_t col_end)
+__device__ void trsm_body(cpx<T>* __restrict__ A, int64_t lda, int64_t k, int nb, int64_t col_begin,
+                          int64_t col_end, int64_t blk, cpx<T>* X /* LDS [nb][TRSM_COLS] */)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cpx<T>* X = (cpx<T>*)smem_raw;  // [nb][TRSM_COLS]
     const int tid = threadIdx.x;
-    const int64_t c0 = col_begin + (int64_t)blockIdx.x * TRSM_COLS;
+    const int64_t c0 = col_begin + blk * TRSM_COLS;
     const int w = (int)((col_end - c0) < TRSM_COLS ? (col_end - c0) : TRSM_COLS);
     for (int idx = tid; idx < nb * TRSM_COLS; idx += 256) {
         int r = idx / TRSM_COLS, c = idx % TRSM_COLS;
@@ -932,6 +945,77 @@ __global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t 
         int r = idx / TRSM_COLS, cc = idx % TRSM_COLS;
         if (cc < w) A[(k + r) * lda + c0 + cc] = X[idx];
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t lda, int64_t k,
+                                                int nb, int64_t col_begin, int64_t col_end)
+{
+    extern __shared__ unsigned char smem_raw[];
+    trsm_body<T>(A, lda, k, nb, col_begin, col_end, (int64_t)blockIdx.x, (cpx<T>*)smem_raw);
+}
+
+// ---- the same three steps for many matrices in one launch each: job j owns
+// workgroups [wg_off[j], wg_off[j + 1]).
+template <class T>
+struct SwapJob {
+    cpx<T>* A;
+    int64_t lda;
+    int64_t c_hi;     // columns [0, c_hi) are swapped, except the panel [k, k + nb)
+    int64_t k;
+    int nb;
+    const int32_t* ipiv;
+    int32_t* slot_row;
+    int32_t* plan;
+    int32_t* plan_n;
+};
+
+template <class T>
+struct TrsmJob {
+    cpx<T>* A;
+    int64_t lda;
+    int64_t k;
+    int nb;
+    int64_t c_lo, c_hi;
+};
+
+__device__ __forceinline__ int job_of(const int* __restrict__ wg_off, int njob, int b)
+{
+    int lo = 0, hi = njob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (wg_off[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__global__ void k_swap_plan_batched(const SwapJob<T>* __restrict__ jobs)
+{
+    if (threadIdx.x != 0) return;
+    const SwapJob<T> J = jobs[blockIdx.x];
+    swap_plan_body(J.ipiv, J.k, J.nb, J.slot_row, J.plan, J.plan_n);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_laswp_batched(const SwapJob<T>* __restrict__ jobs,
+                                                       const int* __restrict__ wg_off, int njob)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int j = job_of(wg_off, njob, (int)blockIdx.x);
+    const SwapJob<T> J = jobs[j];
+    laswp_body<T>(J.A, J.lda, 0, J.c_hi, J.k, J.k + J.nb, J.nb, J.slot_row, J.plan, J.plan_n,
+                  (int64_t)blockIdx.x - wg_off[j], (cpx<T>*)smem_raw);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_trsm_batched(const TrsmJob<T>* __restrict__ jobs,
+                                                      const int* __restrict__ wg_off, int njob)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int j = job_of(wg_off, njob, (int)blockIdx.x);
+    const TrsmJob<T> J = jobs[j];
+    trsm_body<T>(J.A, J.lda, J.k, J.nb, J.c_lo, J.c_hi, (int64_t)blockIdx.x - wg_off[j], (cpx<T>*)smem_raw);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n)
@@ -1940,8 +2024,10 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     if (!attr_set[prec]) {
         (void)hipFuncSetAttribute((const void*)k_panel_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_budget);
-        (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(128 * TRSM_COLS * esz));
+        (void)hipFuncSetAttribute((const void*)k_trsm_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(NB * TRSM_COLS * esz));
+        (void)hipFuncSetAttribute((const void*)k_laswp_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(2 * NB * LASWP_COLS * esz));
         attr_set[prec] = true;
     }
     // ---- host schedule: per panel phase, launches of (first entry, count, shm)
@@ -2000,15 +2086,104 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
             else woffs.pop_back();
             phases.push_back(ls);
         }
-    const size_t need = args.size() * sizeof(PanelArgs<T>) + 256 + woffs.size() * sizeof(int);
+    // ---- host schedule of the row swaps and triangular solves: per phase
+    // (2 s + half) one batched launch over the matrices that have work
+    struct JobRange {
+        size_t first = 0, count = 0, woff = 0;
+        int wg = 0;
+        int nb_max = 0;
+    };
+    std::vector<SwapJob<T>> sjobs;
+    std::vector<TrsmJob<T>> tjobs;
+    std::vector<int> swoff, twoff;
+    std::vector<JobRange> srange(2 * steps), trange(2 * steps);
+    auto open_range = [](JobRange& r, size_t first, std::vector<int>& wo) {
+        r.first = first;
+        r.woff = wo.size();
+        wo.push_back(0);
+    };
+    auto add_swap = [&](JobRange& r, int i, int64_t k, int nb) {
+        const int64_t n = mats[i].n;
+        const int64_t right = n - (k + nb);
+        if (k == 0 && right <= 0) return;
+        SwapJob<T> J;
+        J.A = (cpx<T>*)mats[i].a;
+        J.lda = mats[i].lda;
+        J.c_hi = n;
+        J.k = k;
+        J.nb = nb;
+        J.ipiv = mats[i].ipiv;
+        J.slot_row = ws[i].slot_row;
+        J.plan = ws[i].plan;
+        J.plan_n = ws[i].plan_n;
+        sjobs.push_back(J);
+        r.count += 1;
+        r.wg += (int)((k + LASWP_COLS - 1) / LASWP_COLS + (right + LASWP_COLS - 1) / LASWP_COLS);
+        swoff.push_back(r.wg);
+        r.nb_max = nb > r.nb_max ? nb : r.nb_max;
+    };
+    auto add_trsm = [&](JobRange& r, int i, int64_t k, int nb, int64_t c_lo) {
+        const int64_t n = mats[i].n;
+        if (c_lo >= n) return;
+        TrsmJob<T> J;
+        J.A = (cpx<T>*)mats[i].a;
+        J.lda = mats[i].lda;
+        J.k = k;
+        J.nb = nb;
+        J.c_lo = c_lo;
+        J.c_hi = n;
+        tjobs.push_back(J);
+        r.count += 1;
+        r.wg += (int)((n - c_lo + TRSM_COLS - 1) / TRSM_COLS);
+        twoff.push_back(r.wg);
+        r.nb_max = nb > r.nb_max ? nb : r.nb_max;
+    };
+    for (int64_t s = 0; s < steps; ++s) {
+        const int64_t k = 2 * NB * s;
+        for (int half = 0; half < 2; ++half) {
+            JobRange& sr = srange[2 * s + half];
+            JobRange& tr = trange[2 * s + half];
+            open_range(sr, sjobs.size(), swoff);
+            open_range(tr, tjobs.size(), twoff);
+            for (int i = 0; i < nmat; ++i) {
+                const int64_t n = mats[i].n;
+                if (k >= n) continue;
+                const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+                const int64_t k2 = k + nb1;
+                if (half == 0) {
+                    add_swap(sr, i, k, nb1);
+                    if (k2 < n) add_trsm(tr, i, k, nb1, k2);
+                } else if (k2 < n) {
+                    const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
+                    add_swap(sr, i, k2, nb2);
+                    add_trsm(tr, i, k2, nb2, k2 + nb2);
+                }
+            }
+        }
+    }
+    const size_t o_woff = al256(args.size() * sizeof(PanelArgs<T>));
+    const size_t o_sj = o_woff + al256(woffs.size() * sizeof(int));
+    const size_t o_swoff = o_sj + al256(sjobs.size() * sizeof(SwapJob<T>));
+    const size_t o_tj = o_swoff + al256(swoff.size() * sizeof(int));
+    const size_t o_twoff = o_tj + al256(tjobs.size() * sizeof(TrsmJob<T>));
+    const size_t need = o_twoff + twoff.size() * sizeof(int);
     if (need > scratch_bytes) return MBA_ERR_ALLOC;
     char* base = (char*)scratch;
     PanelArgs<T>* d_args = (PanelArgs<T>*)base;
-    int* d_woff = (int*)(base + al256(args.size() * sizeof(PanelArgs<T>)));
-    if (!args.empty() &&
-        (hipMemcpyAsync(d_args, args.data(), args.size() * sizeof(PanelArgs<T>), hipMemcpyHostToDevice, st) !=
-             hipSuccess ||
-         hipMemcpyAsync(d_woff, woffs.data(), woffs.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess))
+    int* d_woff = (int*)(base + o_woff);
+    SwapJob<T>* d_sj = (SwapJob<T>*)(base + o_sj);
+    int* d_swoff = (int*)(base + o_swoff);
+    TrsmJob<T>* d_tj = (TrsmJob<T>*)(base + o_tj);
+    int* d_twoff = (int*)(base + o_twoff);
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    if (!up(d_args, args.data(), args.size() * sizeof(PanelArgs<T>)) ||
+        !up(d_woff, woffs.data(), woffs.size() * sizeof(int)) ||
+        !up(d_sj, sjobs.data(), sjobs.size() * sizeof(SwapJob<T>)) ||
+        !up(d_swoff, swoff.data(), swoff.size() * sizeof(int)) ||
+        !up(d_tj, tjobs.data(), tjobs.size() * sizeof(TrsmJob<T>)) ||
+        !up(d_twoff, twoff.data(), twoff.size() * sizeof(int)))
         return MBA_ERR_LAUNCH;
     // the host tables are released on return: wait for the copies
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
@@ -2026,24 +2201,21 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         return gemm<T>(M, N, kk, -1.0, A + r0 * lda + k0, lda, A + k0 * lda + c0, lda, 1.0, A + r0 * lda + c0, lda,
                        st);
     };
-    auto swaps = [&](int i, int64_t k, int nb) -> int {
-        const int64_t n = mats[i].n;
-        const int64_t right = n - (k + nb);
-        if (k == 0 && right <= 0) return MBA_OK;
-        hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(64), 0, st, mats[i].ipiv, k, nb, ws[i].slot_row, ws[i].plan,
-                           ws[i].plan_n);
-        const int64_t blocks = (k + LASWP_COLS - 1) / LASWP_COLS + (right + LASWP_COLS - 1) / LASWP_COLS;
-        hipLaunchKernelGGL(k_laswp<T>, dim3((unsigned)blocks), dim3(256), (size_t)2 * nb * LASWP_COLS * esz, st,
-                           (cpx<T>*)mats[i].a, mats[i].lda, (int64_t)0, n, k, k + nb, nb, ws[i].slot_row,
-                           ws[i].plan, ws[i].plan_n);
+    auto swaps = [&](int p) -> int {
+        const JobRange& r = srange[p];
+        if (r.count == 0) return MBA_OK;
+        const SwapJob<T>* jj = d_sj + r.first;
+        hipLaunchKernelGGL(k_swap_plan_batched<T>, dim3((unsigned)r.count), dim3(64), 0, st, jj);
+        hipLaunchKernelGGL(k_laswp_batched<T>, dim3((unsigned)r.wg), dim3(256), (size_t)2 * r.nb_max * LASWP_COLS * esz,
+                           st, jj, (const int*)(d_swoff + r.woff), (int)r.count);
         MBA_LAUNCH_CHECK();
         return MBA_OK;
     };
-    auto trsm = [&](int i, int64_t r0, int nb, int64_t c_lo) -> int {
-        const int64_t n = mats[i].n;
-        if (c_lo >= n) return MBA_OK;
-        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n - c_lo + TRSM_COLS - 1) / TRSM_COLS)), dim3(256),
-                           (size_t)nb * TRSM_COLS * esz, st, (cpx<T>*)mats[i].a, mats[i].lda, r0, nb, c_lo, n);
+    auto trsm = [&](int p) -> int {
+        const JobRange& r = trange[p];
+        if (r.count == 0) return MBA_OK;
+        hipLaunchKernelGGL(k_trsm_batched<T>, dim3((unsigned)r.wg), dim3(256), (size_t)r.nb_max * TRSM_COLS * esz, st,
+                           (const TrsmJob<T>*)(d_tj + r.first), (const int*)(d_twoff + r.woff), (int)r.count);
         MBA_LAUNCH_CHECK();
         return MBA_OK;
     };
@@ -2063,18 +2235,8 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     for (int64_t s = 0; s < steps; ++s) {
         const int64_t k = 2 * NB * s;
         if ((rc = run_phase((int)(2 * s))) != MBA_OK) return rc;
-        for (int i = 0; i < nmat; ++i) {
-            const int64_t n = mats[i].n;
-            if (k >= n) continue;
-            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
-            const int64_t k2 = k + nb1;
-            const int nb2 = (int)((n - k2) < NB ? ((n - k2) > 0 ? (n - k2) : 0) : NB);
-            if ((rc = swaps(i, k, nb1)) != MBA_OK) return rc;
-            if (k2 >= n) continue;
-            if ((rc = trsm(i, k, nb1, k2)) != MBA_OK) return rc;
-            if ((rc = gemm_rect(i, k2, n - k2, k2, nb2, k, nb1)) != MBA_OK) return rc;
-        }
-        if ((rc = run_phase((int)(2 * s + 1))) != MBA_OK) return rc;
+        if ((rc = swaps((int)(2 * s))) != MBA_OK) return rc;
+        if ((rc = trsm((int)(2 * s))) != MBA_OK) return rc;
         for (int i = 0; i < nmat; ++i) {
             const int64_t n = mats[i].n;
             if (k >= n) continue;
@@ -2082,11 +2244,31 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
             const int64_t k2 = k + nb1;
             if (k2 >= n) continue;
             const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
-            if ((rc = swaps(i, k2, nb2)) != MBA_OK) return rc;
+            if ((rc = gemm_rect(i, k2, n - k2, k2, nb2, k, nb1)) != MBA_OK) return rc;
+        }
+        if ((rc = run_phase((int)(2 * s + 1))) != MBA_OK) return rc;
+        if ((rc = swaps((int)(2 * s + 1))) != MBA_OK) return rc;
+        for (int i = 0; i < nmat; ++i) {
+            const int64_t n = mats[i].n;
+            if (k >= n) continue;
+            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+            const int64_t k2 = k + nb1;
+            if (k2 >= n) continue;
+            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
             const int64_t c3 = k2 + nb2;
             if (c3 >= n) continue;
             if ((rc = gemm_rect(i, k2, nb2, c3, n - c3, k, nb1)) != MBA_OK) return rc;
-            if ((rc = trsm(i, k2, nb2, c3)) != MBA_OK) return rc;
+        }
+        if ((rc = trsm((int)(2 * s + 1))) != MBA_OK) return rc;
+        for (int i = 0; i < nmat; ++i) {
+            const int64_t n = mats[i].n;
+            if (k >= n) continue;
+            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+            const int64_t k2 = k + nb1;
+            if (k2 >= n) continue;
+            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
+            const int64_t c3 = k2 + nb2;
+            if (c3 >= n) continue;
             if ((rc = gemm_rect(i, c3, n - c3, c3, n - c3, k, nb1 + nb2)) != MBA_OK) return rc;
         }
     }
@@ -2160,9 +2342,16 @@ int mba_lu_solve_batched(int prec, const mba_lu_solve_job* jobs, int32_t n_jobs,
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max)
 {
     if (n_mat <= 0 || n_max <= 0) return 0;
-    const size_t ent = (size_t)n_mat * 2 * (size_t)((n_max + 255) / 256);
+    const size_t phases = 2 * (size_t)((n_max + 255) / 256);
+    const size_t ent = (size_t)n_mat * phases;
     const size_t pa = prec == MBA_F32 ? sizeof(PanelArgs<float>) : sizeof(PanelArgs<double>);
-    return al256(ent * pa) + 256 + (2 * ent + 2 * (size_t)((n_max + 255) / 256) * 2 + 8) * sizeof(int);
+    const size_t sj = prec == MBA_F32 ? sizeof(SwapJob<float>) : sizeof(SwapJob<double>);
+    const size_t tj = prec == MBA_F32 ? sizeof(TrsmJob<float>) : sizeof(TrsmJob<double>);
+    // panel entries and their launch offsets (at most one launch per entry
+    // plus one per phase), then swap and solve jobs with one offset per job
+    // and one per phase
+    return al256(ent * pa) + al256((2 * ent + phases + 8) * sizeof(int)) + al256(ent * sj) +
+           al256((ent + phases + 8) * sizeof(int)) + al256(ent * tj) + (ent + phases + 8) * sizeof(int);
 }
 
 int mba_lu_factor_batched(int prec, const mba_lu_matrix* mats, int32_t n_mat, void* scratch, size_t scratch_bytes,
