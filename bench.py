@@ -296,7 +296,6 @@ def run_c3(args, dev):
         "assembly_Mpairs_s": round(1e-6 * pairs / t_asm, 1),
         "lu_s": round(res.times["lu_s"], 4),
         "solve_refine_s": round(res.times["solve_refine_s"], 4),
-        "extra_refine_s": round(res.times["extra_refine_s"], 4),
         "refine_iterations_max": int(max(res.iterations)),
         "all_converged": bool(all(res.converged)),
         "relerr_vs_fp64_max": float(f"{max(errs):.3e}"),

@@ -199,6 +199,22 @@ typedef struct {
     void *workspace;
     size_t workspace_bytes;
 } mba_lu_matrix;
+/* Many independent products C_j = alpha_j A_j B_j + beta_j C_j (row-major
+ * complex, the layout of mba_gemm) in one launch: the tiles of all jobs form
+ * one grid (FP32: 3M product on MFMA; FP64: MFMA).  The job table is copied to
+ * scratch (mba_gemm_grouped_scratch_bytes) and the call waits for that copy.
+ * Jobs must not write overlapping C.  No reference counterpart. */
+typedef struct {
+    const void *a;
+    const void *b;
+    void *c;
+    int64_t m, n, k, lda, ldb, ldc;
+    double alpha, beta;
+} mba_gemm_job;
+size_t mba_gemm_grouped_scratch_bytes(int32_t n_jobs);
+int mba_gemm_grouped(int prec, const mba_gemm_job *jobs, int32_t n_jobs, void *scratch, size_t scratch_bytes,
+                     mba_stream_t stream);
+
 /* Solves of several prepared factorisations (mba_lu_solve_prepare) with many
  * right-hand sides each, all in two launches (forward L, backward U; one
  * workgroup per job and 16 right-hand sides).  b [n, ldb] is overwritten with

@@ -205,29 +205,20 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
 // registers.  Block tile 128 x 64 complex, 4 waves in 2 x 2 (64 x 32 each:
 // 2 x 1 instructions per product), k-chunks of 16 in LDS (double buffer), re
 // and im planes stored k-major (A) and row-major (B).
-template <bool XCD>
-__global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha,
-                                                     const cpx<float>* __restrict__ A, int64_t lda,
-                                                     const cpx<float>* __restrict__ B, int64_t ldb,
-                                                     float beta, cpx<float>* __restrict__ C, int64_t ldc)
+constexpr int G3M_BN = 64;
+
+__device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, float alpha,
+                                             const cpx<float>* __restrict__ A, int64_t lda,
+                                             const cpx<float>* __restrict__ B, int64_t ldb, float beta,
+                                             cpx<float>* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0)
 {
-    constexpr int BN = 64;
+    constexpr int BN = G3M_BN;
     __shared__ alignas(16) float Ar[2][G32_BK][G32_BM];
     __shared__ alignas(16) float Ai[2][G32_BK][G32_BM];
     __shared__ alignas(16) float Br[2][G32_BK][BN];
     __shared__ alignas(16) float Bi[2][G32_BK][BN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int64_t nbm = (M + G32_BM - 1) / G32_BM, nbn = (N + BN - 1) / BN;
-    int64_t tmi, tni;
-    if (XCD) {
-        tile_of(blockIdx.x, nbm, nbn, tmi, tni);
-        if (tmi < 0) return;
-    } else {
-        tmi = blockIdx.x / nbn;
-        tni = blockIdx.x % nbn;
-    }
-    const int64_t m0 = tmi * G32_BM, n0 = tni * BN;
     // A: 128 rows x 16 complex k: thread -> row tid/2, k (tid&1)*8 .. +8
     const int ar = tid >> 1, ak = (tid & 1) * 8;
     // B: 16 k x 64 complex: thread -> k tid/16, cols (tid&15)*4 .. +4
@@ -317,6 +308,64 @@ __global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, in
 }
 
 template <bool XCD>
+__global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha,
+                                                     const cpx<float>* __restrict__ A, int64_t lda,
+                                                     const cpx<float>* __restrict__ B, int64_t ldb,
+                                                     float beta, cpx<float>* __restrict__ C, int64_t ldc)
+{
+    const int64_t nbm = (M + G32_BM - 1) / G32_BM, nbn = (N + G3M_BN - 1) / G3M_BN;
+    int64_t tmi, tni;
+    if (XCD) {
+        tile_of(blockIdx.x, nbm, nbn, tmi, tni);
+        if (tmi < 0) return;
+    } else {
+        tmi = blockIdx.x / nbn;
+        tni = blockIdx.x % nbn;
+    }
+    cgemm3m_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tmi * G32_BM, tni * G3M_BN);
+}
+
+// ---- grouped products: job j owns tiles [tile_off[j], tile_off[j + 1]),
+// row-major over its (row tile, column tile) grid.  With XCD the launch is
+// padded to a multiple of the XCD count and block b takes tile
+// (b % NUM_XCD) * (blocks / NUM_XCD) + b / NUM_XCD, so each XCD (blocks are
+// dealt to XCDs round-robin) works on a contiguous run of tiles and keeps its
+// jobs' operands in its own L2.
+struct GemmJobDev {
+    const void* a;
+    const void* b;
+    void* c;
+    int64_t m, n, k, lda, ldb, ldc;
+    double alpha, beta;
+};
+static_assert(sizeof(GemmJobDev) == sizeof(mba_gemm_job), "descriptor layouts must match");
+
+__device__ __forceinline__ int grouped_tile(bool xcd, int total, int& t)
+{
+    const int b = (int)blockIdx.x;
+    t = xcd ? (b % NUM_XCD) * ((int)gridDim.x / NUM_XCD) + b / NUM_XCD : b;
+    return t < total;
+}
+
+template <bool XCD>
+__global__ __launch_bounds__(256, 3) void k_cgemm3m_grouped(const GemmJobDev* __restrict__ jobs,
+                                                         const int* __restrict__ tile_off, int njob)
+{
+    int t;
+    if (!grouped_tile(XCD, tile_off[njob], t)) return;
+    int lo = 0, hi = njob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tile_off[mid] <= t) lo = mid; else hi = mid;
+    }
+    const GemmJobDev& J = jobs[lo];
+    const int64_t nbn = (J.n + G3M_BN - 1) / G3M_BN;
+    const int64_t u = t - tile_off[lo];
+    cgemm3m_tile(J.m, J.n, J.k, (float)J.alpha, (const cpx<float>*)J.a, J.lda, (const cpx<float>*)J.b, J.ldb,
+                 (float)J.beta, (cpx<float>*)J.c, J.ldc, (u / nbn) * G32_BM, (u % nbn) * G3M_BN);
+}
+
+template <bool XCD>
 int launch_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B,
                        int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
 {
@@ -363,18 +412,16 @@ int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const v
 // 32 x 16 (2 x 2 instructions), k-chunks of 8 complex.
 constexpr int G64_BM = 64, G64_BN = 32, G64_BK = 8;
 
-__global__ __launch_bounds__(256) void k_cgemm_f64(int64_t M, int64_t N, int64_t K, double alpha,
-                                                   const cpx<double>* __restrict__ A, int64_t lda,
-                                                   const cpx<double>* __restrict__ B, int64_t ldb,
-                                                   double beta, cpx<double>* __restrict__ C,
-                                                   int64_t ldc)
+__device__ __forceinline__ void cgemm_f64_tile(int64_t M, int64_t N, int64_t K, double alpha,
+                                               const cpx<double>* __restrict__ A, int64_t lda,
+                                               const cpx<double>* __restrict__ B, int64_t ldb, double beta,
+                                               cpx<double>* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0)
 {
     __shared__ alignas(16) double Are[2][G64_BK][G64_BM];
     __shared__ alignas(16) double Aim[2][G64_BK][G64_BM];
     __shared__ alignas(16) double Bs[2][G64_BK][2 * G64_BN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int64_t m0 = (int64_t)blockIdx.y * G64_BM, n0 = (int64_t)blockIdx.x * G64_BN;
     // A: 64 rows x 8 complex: thread -> row tid/4, complex k (tid&3)*2 .. +2
     const int ar = tid >> 2, ak = (tid & 3) * 2;
     // B: 8 rows x 32 complex: thread -> row tid/32, complex col tid&31
@@ -464,6 +511,67 @@ __global__ __launch_bounds__(256) void k_cgemm_f64(int64_t M, int64_t N, int64_t
                 }
             }
         }
+}
+
+__global__ __launch_bounds__(256) void k_cgemm_f64(int64_t M, int64_t N, int64_t K, double alpha,
+                                                   const cpx<double>* __restrict__ A, int64_t lda,
+                                                   const cpx<double>* __restrict__ B, int64_t ldb,
+                                                   double beta, cpx<double>* __restrict__ C,
+                                                   int64_t ldc)
+{
+    cgemm_f64_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, (int64_t)blockIdx.y * G64_BM,
+                   (int64_t)blockIdx.x * G64_BN);
+}
+
+template <bool XCD>
+__global__ __launch_bounds__(256) void k_cgemm_f64_grouped(const GemmJobDev* __restrict__ jobs,
+                                                           const int* __restrict__ tile_off, int njob)
+{
+    int t;
+    if (!grouped_tile(XCD, tile_off[njob], t)) return;
+    int lo = 0, hi = njob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tile_off[mid] <= t) lo = mid; else hi = mid;
+    }
+    const GemmJobDev& J = jobs[lo];
+    const int64_t nbn = (J.n + G64_BN - 1) / G64_BN;
+    const int64_t u = t - tile_off[lo];
+    cgemm_f64_tile(J.m, J.n, J.k, J.alpha, (const cpx<double>*)J.a, J.lda, (const cpx<double>*)J.b, J.ldb, J.beta,
+                   (cpx<double>*)J.c, J.ldc, (u / nbn) * G64_BM, (u % nbn) * G64_BN);
+}
+
+// Tiles of a grouped job and the launch of a device-resident job table.
+inline int64_t grouped_tiles(int prec, int64_t m, int64_t n)
+{
+    if (m <= 0 || n <= 0) return 0;
+    return prec == MBA_F32 ? ((m + G32_BM - 1) / G32_BM) * ((n + G3M_BN - 1) / G3M_BN)
+                           : ((m + G64_BM - 1) / G64_BM) * ((n + G64_BN - 1) / G64_BN);
+}
+
+int launch_grouped(int prec, bool xcd, const GemmJobDev* d_jobs, const int* d_tile_off, int njob, int64_t tiles,
+                   hipStream_t st)
+{
+    if (njob <= 0 || tiles <= 0) return MBA_OK;
+    const int64_t blocks = xcd ? ((tiles + NUM_XCD - 1) / NUM_XCD) * NUM_XCD : tiles;
+    if (blocks > 0x7fffffff) return MBA_ERR_SIZE;
+    if (prec == MBA_F32) {
+        if (xcd)
+            hipLaunchKernelGGL(k_cgemm3m_grouped<true>, dim3((unsigned)blocks), dim3(256), 0, st, d_jobs, d_tile_off,
+                               njob);
+        else
+            hipLaunchKernelGGL(k_cgemm3m_grouped<false>, dim3((unsigned)blocks), dim3(256), 0, st, d_jobs,
+                               d_tile_off, njob);
+    } else {
+        if (xcd)
+            hipLaunchKernelGGL(k_cgemm_f64_grouped<true>, dim3((unsigned)blocks), dim3(256), 0, st, d_jobs,
+                               d_tile_off, njob);
+        else
+            hipLaunchKernelGGL(k_cgemm_f64_grouped<false>, dim3((unsigned)blocks), dim3(256), 0, st, d_jobs,
+                               d_tile_off, njob);
+    }
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
 }
 
 template <class T>
@@ -2161,12 +2269,71 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
             }
         }
     }
+    // trailing products per step: 0 = next panel's columns (after the first
+    // panel), 1 = second panel's rows right of it, 2 = the trailing matrix
+    struct GemmRange {
+        size_t first = 0, count = 0, toff = 0;
+        int64_t tiles = 0;
+    };
+    std::vector<GemmJobDev> gjobs;
+    std::vector<int> gtoff;
+    std::vector<GemmRange> grange(3 * steps);
+    const bool grouped = prec == MBA_F64 || g_gemm_f32_variant == 4 || g_gemm_f32_variant == 5;
+    const bool gxcd = prec == MBA_F32 && g_gemm_f32_variant == 5;
+    if (grouped) {
+        for (int64_t s = 0; s < steps; ++s) {
+            const int64_t k = 2 * NB * s;
+            for (int g = 0; g < 3; ++g) {
+                GemmRange& r = grange[3 * s + g];
+                r.first = gjobs.size();
+                r.toff = gtoff.size();
+                gtoff.push_back(0);
+                for (int i = 0; i < nmat; ++i) {
+                    const int64_t n = mats[i].n;
+                    if (k >= n) continue;
+                    const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+                    const int64_t k2 = k + nb1;
+                    if (k2 >= n) continue;
+                    const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
+                    const int64_t c3 = k2 + nb2;
+                    int64_t r0, M, c0, N, k0, kk;
+                    if (g == 0) {
+                        r0 = k2; M = n - k2; c0 = k2; N = nb2; k0 = k; kk = nb1;
+                    } else if (g == 1) {
+                        r0 = k2; M = nb2; c0 = c3; N = n - c3; k0 = k; kk = nb1;
+                    } else {
+                        r0 = c3; M = n - c3; c0 = c3; N = n - c3; k0 = k; kk = nb1 + nb2;
+                    }
+                    if (M <= 0 || N <= 0 || kk <= 0) continue;
+                    cpx<T>* A = (cpx<T>*)mats[i].a;
+                    const int64_t lda = mats[i].lda;
+                    GemmJobDev J;
+                    J.a = A + r0 * lda + k0;
+                    J.b = A + k0 * lda + c0;
+                    J.c = A + r0 * lda + c0;
+                    J.m = M;
+                    J.n = N;
+                    J.k = kk;
+                    J.lda = J.ldb = J.ldc = lda;
+                    J.alpha = -1.0;
+                    J.beta = 1.0;
+                    gjobs.push_back(J);
+                    r.count += 1;
+                    r.tiles += grouped_tiles(prec, M, N);
+                    if (r.tiles > 0x7fffffff) return MBA_ERR_SIZE;
+                    gtoff.push_back((int)r.tiles);
+                }
+            }
+        }
+    }
     const size_t o_woff = al256(args.size() * sizeof(PanelArgs<T>));
     const size_t o_sj = o_woff + al256(woffs.size() * sizeof(int));
     const size_t o_swoff = o_sj + al256(sjobs.size() * sizeof(SwapJob<T>));
     const size_t o_tj = o_swoff + al256(swoff.size() * sizeof(int));
     const size_t o_twoff = o_tj + al256(tjobs.size() * sizeof(TrsmJob<T>));
-    const size_t need = o_twoff + twoff.size() * sizeof(int);
+    const size_t o_gj = o_twoff + al256(twoff.size() * sizeof(int));
+    const size_t o_gtoff = o_gj + al256(gjobs.size() * sizeof(GemmJobDev));
+    const size_t need = o_gtoff + gtoff.size() * sizeof(int);
     if (need > scratch_bytes) return MBA_ERR_ALLOC;
     char* base = (char*)scratch;
     PanelArgs<T>* d_args = (PanelArgs<T>*)base;
@@ -2175,6 +2342,8 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     int* d_swoff = (int*)(base + o_swoff);
     TrsmJob<T>* d_tj = (TrsmJob<T>*)(base + o_tj);
     int* d_twoff = (int*)(base + o_twoff);
+    GemmJobDev* d_gj = (GemmJobDev*)(base + o_gj);
+    int* d_gtoff = (int*)(base + o_gtoff);
     auto up = [&](void* dst, const void* src, size_t bytes) {
         return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
     };
@@ -2183,7 +2352,9 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         !up(d_sj, sjobs.data(), sjobs.size() * sizeof(SwapJob<T>)) ||
         !up(d_swoff, swoff.data(), swoff.size() * sizeof(int)) ||
         !up(d_tj, tjobs.data(), tjobs.size() * sizeof(TrsmJob<T>)) ||
-        !up(d_twoff, twoff.data(), twoff.size() * sizeof(int)))
+        !up(d_twoff, twoff.data(), twoff.size() * sizeof(int)) ||
+        !up(d_gj, gjobs.data(), gjobs.size() * sizeof(GemmJobDev)) ||
+        !up(d_gtoff, gtoff.data(), gtoff.size() * sizeof(int)))
         return MBA_ERR_LAUNCH;
     // the host tables are released on return: wait for the copies
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
@@ -2231,46 +2402,41 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         }
         return MBA_OK;
     };
+    // trailing products of step s, group g: one grouped launch, or one launch
+    // per matrix for the tile variants without a grouped form
+    auto products = [&](int64_t s, int g) -> int {
+        if (grouped) {
+            const GemmRange& r = grange[3 * s + g];
+            return launch_grouped(prec, gxcd, d_gj + r.first, d_gtoff + r.toff, (int)r.count, r.tiles, st);
+        }
+        const int64_t k = 2 * NB * s;
+        for (int i = 0; i < nmat; ++i) {
+            const int64_t n = mats[i].n;
+            if (k >= n) continue;
+            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
+            const int64_t k2 = k + nb1;
+            if (k2 >= n) continue;
+            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
+            const int64_t c3 = k2 + nb2;
+            int rc2 = MBA_OK;
+            if (g == 0) rc2 = gemm_rect(i, k2, n - k2, k2, nb2, k, nb1);
+            else if (g == 1) rc2 = gemm_rect(i, k2, nb2, c3, n - c3, k, nb1);
+            else rc2 = gemm_rect(i, c3, n - c3, c3, n - c3, k, nb1 + nb2);
+            if (rc2 != MBA_OK) return rc2;
+        }
+        return MBA_OK;
+    };
     int rc;
     for (int64_t s = 0; s < steps; ++s) {
-        const int64_t k = 2 * NB * s;
         if ((rc = run_phase((int)(2 * s))) != MBA_OK) return rc;
         if ((rc = swaps((int)(2 * s))) != MBA_OK) return rc;
         if ((rc = trsm((int)(2 * s))) != MBA_OK) return rc;
-        for (int i = 0; i < nmat; ++i) {
-            const int64_t n = mats[i].n;
-            if (k >= n) continue;
-            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
-            const int64_t k2 = k + nb1;
-            if (k2 >= n) continue;
-            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
-            if ((rc = gemm_rect(i, k2, n - k2, k2, nb2, k, nb1)) != MBA_OK) return rc;
-        }
+        if ((rc = products(s, 0)) != MBA_OK) return rc;
         if ((rc = run_phase((int)(2 * s + 1))) != MBA_OK) return rc;
         if ((rc = swaps((int)(2 * s + 1))) != MBA_OK) return rc;
-        for (int i = 0; i < nmat; ++i) {
-            const int64_t n = mats[i].n;
-            if (k >= n) continue;
-            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
-            const int64_t k2 = k + nb1;
-            if (k2 >= n) continue;
-            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
-            const int64_t c3 = k2 + nb2;
-            if (c3 >= n) continue;
-            if ((rc = gemm_rect(i, k2, nb2, c3, n - c3, k, nb1)) != MBA_OK) return rc;
-        }
+        if ((rc = products(s, 1)) != MBA_OK) return rc;
         if ((rc = trsm((int)(2 * s + 1))) != MBA_OK) return rc;
-        for (int i = 0; i < nmat; ++i) {
-            const int64_t n = mats[i].n;
-            if (k >= n) continue;
-            const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
-            const int64_t k2 = k + nb1;
-            if (k2 >= n) continue;
-            const int nb2 = (int)((n - k2) < NB ? (n - k2) : NB);
-            const int64_t c3 = k2 + nb2;
-            if (c3 >= n) continue;
-            if ((rc = gemm_rect(i, c3, n - c3, c3, n - c3, k, nb1 + nb2)) != MBA_OK) return rc;
-        }
+        if ((rc = products(s, 2)) != MBA_OK) return rc;
     }
     for (int i = 0; i < nmat; ++i)
         hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, st, (const int32_t*)ws[i].err, err_out + i, 1);
@@ -2339,6 +2505,43 @@ int mba_lu_solve_batched(int prec, const mba_lu_solve_job* jobs, int32_t n_jobs,
     return lu_solve_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
 }
 
+size_t mba_gemm_grouped_scratch_bytes(int32_t n_jobs)
+{
+    if (n_jobs <= 0) return 0;
+    return al256((size_t)n_jobs * sizeof(GemmJobDev)) + (size_t)(n_jobs + 1) * sizeof(int);
+}
+
+int mba_gemm_grouped(int prec, const mba_gemm_job* jobs, int32_t n_jobs, void* scratch, size_t scratch_bytes,
+                     mba_stream_t stream)
+{
+    if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (scratch_bytes < mba_gemm_grouped_scratch_bytes(n_jobs)) return MBA_ERR_ALLOC;
+    std::vector<int> toff(n_jobs + 1, 0);
+    int64_t tiles = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const mba_gemm_job& J = jobs[j];
+        if (J.m < 0 || J.n < 0 || J.k < 0) return MBA_ERR_SIZE;
+        if (J.m > 0 && J.n > 0) {
+            if (!J.a || !J.b || !J.c) return MBA_ERR_ARG;
+            if (J.lda < J.k || J.ldb < J.n || J.ldc < J.n) return MBA_ERR_ARG;
+        }
+        tiles += grouped_tiles(prec, J.m, J.n);
+        if (tiles > 0x7fffffff - NUM_XCD) return MBA_ERR_SIZE;
+        toff[j + 1] = (int)tiles;
+    }
+    hipStream_t st = S(stream);
+    char* base = (char*)scratch;
+    GemmJobDev* dj = (GemmJobDev*)base;
+    int* dt = (int*)(base + al256((size_t)n_jobs * sizeof(GemmJobDev)));
+    if (hipMemcpyAsync(dj, jobs, (size_t)n_jobs * sizeof(GemmJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dt, toff.data(), (size_t)(n_jobs + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+        return MBA_ERR_LAUNCH;
+    // the host tables are released on return: wait for the copies
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    return launch_grouped(prec, prec == MBA_F32 && g_gemm_f32_variant == 5, dj, dt, n_jobs, tiles, st);
+}
+
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max)
 {
     if (n_mat <= 0 || n_max <= 0) return 0;
@@ -2350,8 +2553,11 @@ size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max)
     // panel entries and their launch offsets (at most one launch per entry
     // plus one per phase), then swap and solve jobs with one offset per job
     // and one per phase
+    // and the grouped products: three groups per step (3/2 per phase)
+    const size_t gent = (size_t)n_mat * 3 * (phases / 2);
     return al256(ent * pa) + al256((2 * ent + phases + 8) * sizeof(int)) + al256(ent * sj) +
-           al256((ent + phases + 8) * sizeof(int)) + al256(ent * tj) + (ent + phases + 8) * sizeof(int);
+           al256((ent + phases + 8) * sizeof(int)) + al256(ent * tj) + al256((ent + phases + 8) * sizeof(int)) +
+           al256(gent * sizeof(GemmJobDev)) + (gent + 3 * (phases / 2) + 8) * sizeof(int);
 }
 
 int mba_lu_factor_batched(int prec, const mba_lu_matrix* mats, int32_t n_mat, void* scratch, size_t scratch_bytes,

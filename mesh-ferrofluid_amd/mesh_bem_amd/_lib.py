@@ -32,6 +32,14 @@ class LuMatrix(ctypes.Structure):
                 ("info", ctypes.c_void_p), ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_size_t)]
 
 
+class GemmJob(ctypes.Structure):
+    """mba_gemm_job of include/mesh_bem_amd.h."""
+
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p), ("m", ctypes.c_int64),
+                ("n", ctypes.c_int64), ("k", ctypes.c_int64), ("lda", ctypes.c_int64), ("ldb", ctypes.c_int64),
+                ("ldc", ctypes.c_int64), ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
+
+
 class LuSolveJob(ctypes.Structure):
     """mba_lu_solve_job of include/mesh_bem_amd.h."""
 
@@ -73,6 +81,8 @@ SIGNATURES = {
     "mba_lu_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_lu_factor": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp]),
     "mba_lu_factor_async": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _i32, _i32p, _vp]),
+    "mba_gemm_grouped_scratch_bytes": (_sz, [_i32]),
+    "mba_gemm_grouped": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
     "mba_lu_solve_batched_scratch_bytes": (_sz, [_i32]),
     "mba_lu_solve_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
     "mba_lu_batched_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),

@@ -23,7 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .dense import lu_factor_batched, lu_solve_batched, to_c64
+from .dense import gemm_grouped, lu_factor_batched, lu_solve_batched, to_c64
 from .geometry import get_edge_unique
 from .mom import MomMesh, rhs_ns, rule_tensor
 
@@ -165,14 +165,17 @@ class BatchedMoM:
         return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
-              tol: float = 1e-8, maxit: int = 10, sweeps: int = 3) -> BatchResult:
+              tol: float = 1e-8, maxit: int = 10) -> BatchResult:
         """Mixed precision for all meshes: the FP32 LUs in lock step
         (``lu_factor_batched``: the panels of every matrix in shared
-        cooperative launches; GEMM tile variant of the largest size bucket from
-        the tuner), the triangular solves of all meshes and right-hand sides in
-        two launches (``lu_solve_batched``), ``sweeps`` refinement sweeps with
-        FP64 GEMM residuals enqueued without host synchronisation, then a host
-        check of the last corrections; meshes not yet at ``tol`` continue alone."""
+        cooperative launches, the trailing products of every matrix in one
+        grouped launch per step and kind; GEMM tile variant of the largest size
+        bucket from the tuner), then refinement sweeps J += LU32 \\ (V - Z J).
+        A sweep is one grouped FP64 MFMA launch for the residuals of every mesh
+        still refining and two launches for their triangular solves with all
+        right-hand sides (``lu_solve_batched``); one host synchronisation then
+        reads the largest relative correction of every mesh, and a mesh stops
+        once it is at most ``tol`` (its error before that sweep)."""
         lib = _lib.load()
         if tuner is not None:
             tuner.apply(max(Z.shape[0] for Z in Zs))
@@ -181,44 +184,33 @@ class BatchedMoM:
         lus = lu_factor_batched([to_c64(Z) for Z in Zs], overwrite=True)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        for lu in lus:
+            lu.check()
+        n_mesh = len(Zs)
         V128s = [V.to(torch.complex128).contiguous() for V in Vs]
         Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs])]
         Rs = [torch.empty_like(V) for V in V128s]
-        rel = [None] * len(Zs)
-        for _ in range(sweeps):
-            Ds = lu_solve_batched(lus, [self._residual(lib, Z, V, J, R) for Z, V, J, R in zip(Zs, V128s, Js, Rs)])
-            for i, (J, D) in enumerate(zip(Js, Ds)):
-                J += D
-                rel[i] = (torch.linalg.vector_norm(D, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()
+        its, conv = [0] * n_mesh, [False] * n_mesh
+        active = list(range(n_mesh))
+        while active:
+            for i in active:
+                Rs[i].copy_(V128s[i])
+            gemm_grouped([(Zs[i], Js[i], Rs[i]) for i in active], alpha=-1.0, beta=1.0)
+            Ds = lu_solve_batched([lus[i] for i in active], [Rs[i].to(torch.complex64) for i in active])
+            rels = []
+            for i, D in zip(active, Ds):
+                Js[i] += D
+                rels.append((torch.linalg.vector_norm(D, dim=0) / torch.linalg.vector_norm(Js[i], dim=0)).max())
+            rels = torch.stack(rels).cpu().tolist()
+            for i, r in zip(active, rels):
+                its[i] += 1
+                conv[i] = r <= tol
+            active = [i for i in active if not conv[i] and its[i] < maxit]
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        its, conv = [], []
-        for i, (lu, Z, J) in enumerate(zip(lus, Zs, Js)):
-            lu.check()
-            it = sweeps
-            ok = float(rel[i]) <= tol
-            while not ok and it < maxit:
-                d = lu.solve(self._residual(lib, Z, V128s[i], J, Rs[i])).to(torch.complex128)
-                J += d
-                it += 1
-                ok = float((torch.linalg.vector_norm(d, dim=0) / torch.linalg.vector_norm(J, dim=0)).max()) <= tol
-            its.append(it)
-            conv.append(ok)
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
         if tuner is not None:
             _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
-        return BatchResult(Js, its, conv, {"lu_s": t1 - t0, "solve_refine_s": t2 - t1, "extra_refine_s": t3 - t2})
-
-    @staticmethod
-    def _residual(lib, Z, V128, J, R):
-        """R = V - Z J as an FP64 MFMA GEMM, returned rounded to complex64."""
-        n = Z.shape[0]
-        R.copy_(V128)
-        st = lib.mba_gemm(_lib.MBA_F64, n, V128.shape[1], n, -1.0, _lib.ptr(Z), Z.stride(0), _lib.ptr(J),
-                          J.stride(0), 1.0, _lib.ptr(R), R.stride(0), _lib.stream())
-        _lib.check(st, "mba_gemm")
-        return R.to(torch.complex64)
+        return BatchResult(Js, its, conv, {"lu_s": t1 - t0, "solve_refine_s": t2 - t1})
 
 
 def star_batch(n_meshes: int, seed0: int = 0, freqs=(5, 6, 7, 8, 9, 10, 11), n_dirs: int = 128,

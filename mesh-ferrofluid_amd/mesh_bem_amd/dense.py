@@ -181,6 +181,41 @@ def lu_factor_batched(mats, overwrite: bool = True):
     return out
 
 
+def gemm_grouped(products, alpha: float = 1.0, beta: float = 0.0):
+    """C_j = alpha A_j B_j + beta C_j for a list of (A_j, B_j, C_j) row-major
+    complex matrices of one dtype (complex64: 3M product on MFMA; complex128:
+    FP64 MFMA), all in one launch.  C_j is updated in place."""
+    lib = _lib.load()
+    if not products:
+        return
+    dt = products[0][0].dtype
+    for A, B, C in products:
+        if not (A.dtype == B.dtype == C.dtype == dt) or A.dim() != 2 or B.dim() != 2 or C.dim() != 2:
+            raise ValueError("gemm_grouped: 2-D operands of one complex dtype")
+        if A.shape[1] != B.shape[0] or C.shape != (A.shape[0], B.shape[1]):
+            raise ValueError("gemm_grouped: shape mismatch")
+        if any(T.numel() > 0 and T.stride(1) != 1 for T in (A, B, C)):
+            raise ValueError("gemm_grouped: rows must be contiguous")
+    products = [p for p in products if p[2].numel() > 0]  # nothing to write
+    for A, B, C in products:
+        if A.shape[1] == 0:  # empty sum: C = beta C
+            C.mul_(beta)
+    products = [p for p in products if p[0].shape[1] > 0]
+    if not products:
+        return
+    n_job = len(products)
+    jobs = (_lib.GemmJob * n_job)()
+    for i, (A, B, C) in enumerate(products):
+        jobs[i].a, jobs[i].b, jobs[i].c = A.data_ptr(), B.data_ptr(), C.data_ptr()
+        jobs[i].m, jobs[i].n, jobs[i].k = A.shape[0], B.shape[1], A.shape[1]
+        jobs[i].lda, jobs[i].ldb, jobs[i].ldc = A.stride(0), B.stride(0), C.stride(0)
+        jobs[i].alpha, jobs[i].beta = alpha, beta
+    sb = lib.mba_gemm_grouped_scratch_bytes(n_job)
+    scratch = torch.empty(sb, dtype=torch.uint8, device=products[0][2].device)
+    st = lib.mba_gemm_grouped(_lib.prec_of(dt), jobs, n_job, _lib.ptr(scratch), sb, _lib.stream())
+    _lib.check(st, "mba_gemm_grouped")
+
+
 def lu_solve_batched(lus, Bs):
     """X_i = A_i^{-1} B_i for prepared factorisations lus[i] (one dtype) and
     right-hand sides Bs[i] [n_i, nrhs_i]; two launches for all of them.

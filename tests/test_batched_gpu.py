@@ -100,3 +100,66 @@ def test_lu_solve_batched_matches_single(dev, prec):
         r = float((A.to(torch.complex128) @ X.to(torch.complex128) - B.to(torch.complex128)).norm() / B.norm())
         print(f"  n={A.shape[0]} nrhs={B.shape[1]}: vs single {e:.2e}, residual {r:.2e}")
         assert e < bar and r < bar
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("variant", [4, 5])
+def test_gemm_grouped_matches_torch(dev, prec, variant):
+    """One grouped launch of ragged products (empty, single-row, tile-edge and
+    strided operands) equals each product computed alone (FP64 on the CPU);
+    FP32 runs both tile orders (variant 5 deals tiles to XCDs in runs)."""
+    from mesh_bem_amd.dense import gemm_grouped
+
+    cdt = torch.complex64 if prec == "f32" else torch.complex128
+    bar = 1e-5 if prec == "f32" else 1e-13
+    rng = np.random.default_rng(9)
+    shapes = [(300, 77, 129), (1, 5, 3), (128, 64, 16), (0, 4, 4), (513, 200, 1), (64, 1, 257), (17, 0, 9)]
+    cplx = lambda *s: torch.tensor(rng.standard_normal(s) + 1j * rng.standard_normal(s), dtype=cdt, device=dev)
+    prods, refs = [], []
+    for m, n, k in shapes:
+        A = cplx(m, k + 3)[:, :k]  # strided rows
+        B, C = cplx(k, n), cplx(m, n)
+        refs.append((-1.0 * A.cpu().to(torch.complex128) @ B.cpu().to(torch.complex128)
+                     + 0.5 * C.cpu().to(torch.complex128)))
+        prods.append((A, B, C))
+    lib = _lib.load()
+    _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, variant), "mba_gemm_set_variant")
+    try:
+        gemm_grouped(prods, alpha=-1.0, beta=0.5)
+        torch.cuda.synchronize()
+    finally:
+        _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
+    for (A, B, C), ref in zip(prods, refs):
+        if ref.numel() == 0:
+            continue
+        e = float((C.cpu().to(torch.complex128) - ref).norm() / ref.norm())
+        assert e < bar, (A.shape, B.shape, e)
+
+
+def test_lu_factor_batched_tile_variants(dev):
+    """The batched LU with grouped products (variants 4, 5) and with the
+    per-matrix fallback (variant 0) gives the same pivots and accurate factors."""
+    from mesh_bem_amd.dense import lu_factor_batched
+
+    rng = np.random.default_rng(12)
+    sizes = [700, 300, 1030]
+    mats = [torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * n ** 0.5 * np.eye(n),
+                         dtype=torch.complex64, device=dev) for n in sizes]
+    lib = _lib.load()
+    piv = []
+    try:
+        for v in (4, 5, 0):
+            _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, v), "mba_gemm_set_variant")
+            lus = lu_factor_batched([A.clone() for A in mats])
+            torch.cuda.synchronize()
+            for A, lu in zip(mats, lus):
+                lu.check()
+                b = torch.tensor(rng.standard_normal(A.shape[0]) + 0j, dtype=torch.complex64, device=dev)
+                x = lu.solve(b)
+                r = (A.to(torch.complex128) @ x.to(torch.complex128) - b.to(torch.complex128)).norm() / b.norm()
+                assert float(r) < 1e-5, (v, A.shape[0], float(r))
+            piv.append([lu.ipiv.cpu() for lu in lus])
+    finally:
+        _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
+    for p in piv[1:]:
+        assert all(torch.equal(a, b) for a, b in zip(piv[0], p))
