@@ -215,6 +215,22 @@ size_t mba_gemm_grouped_scratch_bytes(int32_t n_jobs);
 int mba_gemm_grouped(int prec, const mba_gemm_job *jobs, int32_t n_jobs, void *scratch, size_t scratch_bytes,
                      mba_stream_t stream);
 
+/* mba_lu_solve_prepare for several factorisations in two launches (the
+ * permutations, then every inverted diagonal block).  The job table is copied
+ * to scratch (mba_lu_solve_prepare_batched_scratch_bytes) and the call waits
+ * for that copy.  No reference counterpart. */
+typedef struct {
+    const void *lu;
+    int64_t n;
+    int64_t lda;
+    const int32_t *ipiv;
+    void *aux;
+    size_t aux_bytes;
+} mba_lu_prepare_job;
+size_t mba_lu_solve_prepare_batched_scratch_bytes(int32_t n_jobs);
+int mba_lu_solve_prepare_batched(int prec, const mba_lu_prepare_job *jobs, int32_t n_jobs, void *scratch,
+                                 size_t scratch_bytes, mba_stream_t stream);
+
 /* Solves of several prepared factorisations (mba_lu_solve_prepare) with many
  * right-hand sides each, all in two launches (forward L, backward U; one
  * workgroup per job and 16 right-hand sides).  b [n, ldb] is overwritten with

@@ -1382,8 +1382,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
 // and pivots are read 64 at a time and broadcast lane by lane.
 constexpr int PERM_LDS = 36 * 1024;
 
-__global__ __launch_bounds__(64) void k_ipiv_perm(const int32_t* __restrict__ ipiv, int64_t n,
-                                                  int32_t* __restrict__ perm)
+__device__ void ipiv_perm_body(const int32_t* __restrict__ ipiv, int64_t n, int32_t* __restrict__ perm)
 {
     __shared__ int32_t pl[PERM_LDS];
     const int lane = threadIdx.x;
@@ -1407,6 +1406,12 @@ __global__ __launch_bounds__(64) void k_ipiv_perm(const int32_t* __restrict__ ip
     __syncthreads();
     if (in_lds)
         for (int64_t i = lane; i < n; i += 64) perm[i] = pl[i];
+}
+
+__global__ __launch_bounds__(64) void k_ipiv_perm(const int32_t* __restrict__ ipiv, int64_t n,
+                                                  int32_t* __restrict__ perm)
+{
+    ipiv_perm_body(ipiv, n, perm);
 }
 
 template <class T>
@@ -1525,14 +1530,13 @@ constexpr int TS_BS = 64;
 constexpr int TS_RHS = 4;
 
 template <class T, bool LOWER>
-__global__ __launch_bounds__(256) void k_tri_inv(const cpx<T>* __restrict__ A, int64_t lda, int64_t n,
-                                                 cpx<T>* __restrict__ inv)
+__device__ void tri_inv_body(const cpx<T>* __restrict__ A, int64_t lda, int64_t n, cpx<T>* __restrict__ inv,
+                             int64_t blk, cpx<T>* smem)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cpx<T>* Lt = (cpx<T>*)smem_raw;          // [TS_BS][TS_BS + 1], Lt[t][i] = A[r0+i][r0+t]
+    cpx<T>* Lt = smem;                       // [TS_BS][TS_BS + 1], Lt[t][i] = A[r0+i][r0+t]
     cpx<T>* X = Lt + TS_BS * (TS_BS + 1);    // [TS_BS][TS_BS + 1], X[i][c]
     const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * TS_BS;
+    const int64_t r0 = blk * TS_BS;
     const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
     constexpr int LD = TS_BS + 1;
     for (int idx = tid; idx < TS_BS * TS_BS; idx += 256) {
@@ -1574,8 +1578,55 @@ __global__ __launch_bounds__(256) void k_tri_inv(const cpx<T>* __restrict__ A, i
         }
         __syncthreads();
     }
-    cpx<T>* out = inv + (int64_t)blockIdx.x * TS_BS * TS_BS;
+    cpx<T>* out = inv + blk * TS_BS * TS_BS;
     for (int idx = tid; idx < TS_BS * TS_BS; idx += 256) out[idx] = X[(idx / TS_BS) * LD + idx % TS_BS];
+}
+
+template <class T, bool LOWER>
+__global__ __launch_bounds__(256) void k_tri_inv(const cpx<T>* __restrict__ A, int64_t lda, int64_t n,
+                                                 cpx<T>* __restrict__ inv)
+{
+    extern __shared__ unsigned char smem_raw[];
+    tri_inv_body<T, LOWER>(A, lda, n, inv, (int64_t)blockIdx.x, (cpx<T>*)smem_raw);
+}
+
+// ---- solve tables of many factorisations: one launch for the permutations
+// (one wave per job), one for the inverted diagonal blocks (job j owns
+// workgroups [wg_off[j], wg_off[j + 1]): its L blocks, then its U blocks).
+struct PrepJobDev {
+    const void* lu;
+    int64_t n, lda;
+    const int32_t* ipiv;
+    void* aux;
+    size_t aux_bytes;
+};
+static_assert(sizeof(PrepJobDev) == sizeof(mba_lu_prepare_job), "descriptor layouts must match");
+
+__global__ __launch_bounds__(64) void k_ipiv_perm_batched(const PrepJobDev* __restrict__ jobs)
+{
+    const PrepJobDev& J = jobs[blockIdx.x];
+    ipiv_perm_body(J.ipiv, J.n, (int32_t*)J.aux);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tri_inv_batched(const PrepJobDev* __restrict__ jobs,
+                                                         const int* __restrict__ wg_off, int njob)
+{
+    extern __shared__ unsigned char smem_raw[];
+    int lo = 0, hi = njob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (wg_off[mid] <= (int)blockIdx.x) lo = mid; else hi = mid;
+    }
+    const PrepJobDev& J = jobs[lo];
+    const int64_t nblk = (J.n + TS_BS - 1) / TS_BS;
+    const int64_t u = (int64_t)blockIdx.x - wg_off[lo];
+    cpx<T>* linv = (cpx<T>*)((char*)J.aux + al256((size_t)J.n * 4));
+    cpx<T>* uinv = (cpx<T>*)((char*)linv + al256((size_t)nblk * TS_BS * TS_BS * sizeof(cpx<T>)));
+    if (u < nblk)
+        tri_inv_body<T, true>((const cpx<T>*)J.lu, J.lda, J.n, linv, u, (cpx<T>*)smem_raw);
+    else
+        tri_inv_body<T, false>((const cpx<T>*)J.lu, J.lda, J.n, uinv, u - nblk, (cpx<T>*)smem_raw);
 }
 
 // Tile of up to 64 rows handled by one workgroup: thread = (row tid/4, part
@@ -1680,6 +1731,35 @@ size_t solve_aux_bytes(int prec, int64_t n)
     const size_t esz = prec == MBA_F32 ? 8 : 16;
     const int64_t nblk = (n + TS_BS - 1) / TS_BS;
     return al256((size_t)n * 4) + 2 * al256((size_t)nblk * TS_BS * TS_BS * esz);
+}
+
+template <class T>
+int lu_prepare_batched(const mba_lu_prepare_job* jobs, int njob, void* scratch, size_t scratch_bytes, hipStream_t st)
+{
+    std::vector<int> wg(njob + 1, 0);
+    for (int j = 0; j < njob; ++j) wg[j + 1] = wg[j] + 2 * (int)((jobs[j].n + TS_BS - 1) / TS_BS);
+    const size_t o_w = al256((size_t)njob * sizeof(PrepJobDev));
+    if (scratch_bytes < o_w + (size_t)(njob + 1) * sizeof(int)) return MBA_ERR_ALLOC;
+    PrepJobDev* dj = (PrepJobDev*)scratch;
+    int* dw = (int*)((char*)scratch + o_w);
+    if (hipMemcpyAsync(dj, jobs, (size_t)njob * sizeof(PrepJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dw, wg.data(), (size_t)(njob + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+        return MBA_ERR_LAUNCH;
+    // the host tables are released on return: wait for the copies
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    const size_t shm = 2 * (size_t)TS_BS * (TS_BS + 1) * sizeof(cpx<T>);
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_tri_inv_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)shm);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_ipiv_perm_batched, dim3((unsigned)njob), dim3(64), 0, st, (const PrepJobDev*)dj);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_tri_inv_batched<T>, dim3((unsigned)wg[njob]), dim3(256), shm, st, (const PrepJobDev*)dj,
+                       (const int*)dw, njob);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
 }
 
 template <class T>
@@ -2622,6 +2702,31 @@ int mba_lu_solve_prepare(int prec, const void* lu, int64_t n, int64_t lda, const
     if (aux_bytes < solve_aux_bytes(prec, n)) return MBA_ERR_ALLOC;
     if (prec == MBA_F32) return lu_prepare<float>(lu, n, lda, ipiv, aux, S(stream));
     return lu_prepare<double>(lu, n, lda, ipiv, aux, S(stream));
+}
+
+size_t mba_lu_solve_prepare_batched_scratch_bytes(int32_t n_jobs)
+{
+    if (n_jobs <= 0) return 0;
+    return al256((size_t)n_jobs * sizeof(PrepJobDev)) + (size_t)(n_jobs + 1) * sizeof(int);
+}
+
+int mba_lu_solve_prepare_batched(int prec, const mba_lu_prepare_job* jobs, int32_t n_jobs, void* scratch,
+                                 size_t scratch_bytes, mba_stream_t stream)
+{
+    if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    int64_t wgs = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const mba_lu_prepare_job& J = jobs[j];
+        if (!J.lu || !J.ipiv || !J.aux) return MBA_ERR_ARG;
+        if (J.n <= 0 || J.n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+        if (J.lda < J.n) return MBA_ERR_ARG;
+        if (J.aux_bytes < solve_aux_bytes(prec, J.n)) return MBA_ERR_ALLOC;
+        wgs += 2 * ((J.n + TS_BS - 1) / TS_BS);
+        if (wgs > 0x7fffffff) return MBA_ERR_SIZE;
+    }
+    if (prec == MBA_F32) return lu_prepare_batched<float>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+    return lu_prepare_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
 }
 
 int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const void* aux, void* b,

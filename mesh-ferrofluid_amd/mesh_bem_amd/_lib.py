@@ -40,6 +40,13 @@ class GemmJob(ctypes.Structure):
                 ("ldc", ctypes.c_int64), ("alpha", ctypes.c_double), ("beta", ctypes.c_double)]
 
 
+class LuPrepareJob(ctypes.Structure):
+    """mba_lu_prepare_job of include/mesh_bem_amd.h."""
+
+    _fields_ = [("lu", ctypes.c_void_p), ("n", ctypes.c_int64), ("lda", ctypes.c_int64), ("ipiv", ctypes.c_void_p),
+                ("aux", ctypes.c_void_p), ("aux_bytes", ctypes.c_size_t)]
+
+
 class LuSolveJob(ctypes.Structure):
     """mba_lu_solve_job of include/mesh_bem_amd.h."""
 
@@ -83,6 +90,8 @@ SIGNATURES = {
     "mba_lu_factor_async": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _i32, _i32p, _vp]),
     "mba_gemm_grouped_scratch_bytes": (_sz, [_i32]),
     "mba_gemm_grouped": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
+    "mba_lu_solve_prepare_batched_scratch_bytes": (_sz, [_i32]),
+    "mba_lu_solve_prepare_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
     "mba_lu_solve_batched_scratch_bytes": (_sz, [_i32]),
     "mba_lu_solve_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
     "mba_lu_batched_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),

@@ -103,7 +103,8 @@ class LUFactor:
 
     @classmethod
     def _from_factored(cls, lu: torch.Tensor, ipiv: torch.Tensor, info: torch.Tensor, err: torch.Tensor):
-        """Wrap a factorisation done elsewhere (batched): prepare the solve tables."""
+        """Wrap a factorisation done elsewhere (batched); the solve tables in
+        ``aux`` are filled by the caller (mba_lu_solve_prepare_batched)."""
         lib = _lib.load()
         self = cls.__new__(cls)
         self.lu, self.ipiv, self.n = lu, ipiv, lu.shape[0]
@@ -111,9 +112,6 @@ class LUFactor:
         self.profile = None
         auxb = lib.mba_lu_solve_aux_bytes(self.prec, self.n)
         self.aux = torch.empty(auxb, dtype=torch.uint8, device=lu.device)
-        st = lib.mba_lu_solve_prepare(self.prec, _lib.ptr(lu), self.n, lu.stride(0), _lib.ptr(ipiv),
-                                      _lib.ptr(self.aux), auxb, _lib.stream())
-        _lib.check(st, "mba_lu_solve_prepare")
         self.perm = self.aux[: 4 * self.n].view(torch.int32)
         self._info_t, self._err, self.info = info, err, None
         return self
@@ -177,7 +175,15 @@ def lu_factor_batched(mats, overwrite: bool = True):
     st = lib.mba_lu_factor_batched(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err), _lib.stream())
     _lib.check(st, "mba_lu_factor_batched")
     out = [LUFactor._from_factored(lu, ipiv, info, err[i:i + 1]) for i, (lu, ipiv, info) in enumerate(lus)]
-    out[0]._keep = (keep, scratch)
+    jobs = (_lib.LuPrepareJob * n_mat)()
+    for i, f in enumerate(out):
+        jobs[i].lu, jobs[i].n, jobs[i].lda = f.lu.data_ptr(), f.n, f.lu.stride(0)
+        jobs[i].ipiv, jobs[i].aux, jobs[i].aux_bytes = f.ipiv.data_ptr(), f.aux.data_ptr(), f.aux.numel()
+    pb = lib.mba_lu_solve_prepare_batched_scratch_bytes(n_mat)
+    pscratch = torch.empty(pb, dtype=torch.uint8, device=dev)
+    st = lib.mba_lu_solve_prepare_batched(prec, jobs, n_mat, _lib.ptr(pscratch), pb, _lib.stream())
+    _lib.check(st, "mba_lu_solve_prepare_batched")
+    out[0]._keep = (keep, scratch, pscratch)
     return out
 
 
