@@ -1974,6 +1974,171 @@ __global__ __launch_bounds__(256) void k_trisolve_cols(const TsJobDev* __restric
     }
 }
 
+// FP32 form of the same walk on MFMA.  The update P = A(block, chunk) x(chunk)
+// (64 rows x 16 complex columns) is a real product in the embedding
+//   A' = [Re A, -Im A] (k' = 2 s + {0, 1}),  B' = [[Re x, Im x], [Im x, -Re x]],
+// so P'[row][2 c] = Re P, P'[row][2 c + 1] = Im P: one v_mfma_f32_32x32x2f32 per
+// complex k.  Wave w: rows 32 (w & 1) .. +32, k-half w >> 1 of each 32-column
+// chunk; the two k-halves are summed through LDS.  The lane's A entries come
+// from global memory into registers (the next chunk's loads are issued before
+// the current chunk's products), x chunks are staged in LDS.  The diagonal
+// block step x = inv(A_bb) t uses the same product with A = the inverse.
+// Workgroups are dealt to XCDs so that the workgroups of one job share an L2.
+constexpr int TSM_CK = 32;
+
+__device__ __forceinline__ void tsm_fetch(float2 (&an)[TSM_CK / 2], float2 (&xn)[TSM_CK * TSC / 256],
+                                          const cpx<float>* __restrict__ Ar, bool rok, int64_t kb,
+                                          const cpx<float>* __restrict__ X, int64_t k0, int64_t qb, int tid,
+                                          bool col_ok, int64_t nrhs, int64_t col)
+{
+#pragma unroll
+    for (int u = 0; u < TSM_CK / 2; ++u) {
+        const int64_t k = kb + u;
+        an[u] = make_float2(0.f, 0.f);
+        if (rok && k < qb) an[u] = *(const float2*)(Ar + k);
+    }
+#pragma unroll
+    for (int e = 0; e < TSM_CK * TSC / 256; ++e) {
+        const int64_t kk = k0 + (tid + 256 * e) / TSC;
+        xn[e] = make_float2(0.f, 0.f);
+        if (kk < qb && col_ok) xn[e] = *(const float2*)(X + kk * nrhs + col);
+    }
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(256, 2) void k_trisolve_mfma(const TsJobDev* __restrict__ jobs,
+                                                          const int* __restrict__ wg_off, int njob)
+{
+    constexpr int LDX = TSC + 2;  // LDS row of 16 complex + 16 bytes
+    constexpr int KH = TSM_CK / 2;
+    constexpr int XPT = TSM_CK * TSC / 256;
+    __shared__ cpx<float> xs[TSM_CK * LDX];
+    __shared__ cpx<float> tb[TS_BS * LDX];
+    __shared__ float red[2][16][64];
+    const int nbk = (int)gridDim.x;
+    const int b = (int)(blockIdx.x % NUM_XCD) * (nbk / NUM_XCD) + (int)(blockIdx.x / NUM_XCD);
+    if (b >= wg_off[njob]) return;
+    int lo = 0, hi_ = njob;
+    while (hi_ - lo > 1) {
+        const int mid = (lo + hi_) >> 1;
+        if (wg_off[mid] <= b) lo = mid; else hi_ = mid;
+    }
+    const TsJobDev J = jobs[lo];
+    const int64_t n = J.n, lda = J.lda, nrhs = J.nrhs;
+    const int64_t c0 = (int64_t)(b - wg_off[lo]) * TSC;
+    const cpx<float>* A = (const cpx<float>*)J.lu;
+    const int32_t* perm = (const int32_t*)J.aux;
+    const int64_t nblk = (n + TS_BS - 1) / TS_BS;
+    const cpx<float>* linv = (const cpx<float>*)((const char*)J.aux + al256((size_t)n * 4));
+    const cpx<float>* uinv =
+        (const cpx<float>*)((const char*)linv + al256((size_t)nblk * TS_BS * TS_BS * sizeof(cpx<float>)));
+    const cpx<float>* inv = LOWER ? linv : uinv;
+    cpx<float>* X = (cpx<float>*)J.work;
+    cpx<float>* B = (cpx<float>*)J.b;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rt = wave & 1, kh = wave >> 1;
+    const int hi = lane >> 5, l32 = lane & 31;
+    const int arow = 32 * rt + l32;       // block row of the lane's A operand
+    const int bc = l32 >> 1, bq = l32 & 1;  // complex column and part of the lane's B operand / output
+    const bool bim = (hi ^ bq) != 0;
+    const float bsg = (hi & bq) ? -1.f : 1.f;
+    const int sc = tid % TSC;
+    const bool sc_ok = c0 + sc < nrhs;
+    const bool oc_ok = c0 + bc < nrhs;
+    for (int64_t st = 0; st < nblk; ++st) {
+        const int64_t bi = LOWER ? st : nblk - 1 - st;
+        const int64_t r0 = bi * TS_BS;
+        const int bs = (int)((n - r0) < TS_BS ? (n - r0) : TS_BS);
+        const bool rok = arow < bs;
+        const cpx<float>* Ar = A + (r0 + (rok ? arow : 0)) * lda;
+        const int64_t qa = LOWER ? 0 : r0 + bs, qb = LOWER ? r0 : n;
+        floatx16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        if (qa < qb) {
+            float2 an[KH], xn[XPT];
+            tsm_fetch(an, xn, Ar, rok, qa + kh * KH, X, qa, qb, tid, sc_ok, nrhs, c0 + sc);
+            for (int64_t k0 = qa; k0 < qb; k0 += TSM_CK) {
+                __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+                for (int e = 0; e < XPT; ++e) {
+                    cpx<float> v;
+                    v.re = xn[e].x;
+                    v.im = xn[e].y;
+                    xs[((tid + 256 * e) / TSC) * LDX + sc] = v;
+                }
+                float2 a[KH];
+#pragma unroll
+                for (int u = 0; u < KH; ++u) a[u] = an[u];
+                __syncthreads();
+                if (k0 + TSM_CK < qb)
+                    tsm_fetch(an, xn, Ar, rok, k0 + TSM_CK + kh * KH, X, k0 + TSM_CK, qb, tid, sc_ok, nrhs, c0 + sc);
+#pragma unroll
+                for (int u = 0; u < KH; ++u) {
+                    const cpx<float> v = xs[(kh * KH + u) * LDX + bc];
+                    const float bv = bsg * (bim ? v.im : v.re);
+                    const float av = hi ? -a[u].y : a[u].x;
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+                }
+            }
+        }
+        // t = rhs - P into tb (waves of k-half 0 after adding k-half 1)
+        if (kh == 1)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) red[rt][r][lane] = acc[r];
+        __syncthreads();
+        if (kh == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * rt + 8 * (r >> 2) + 4 * hi + (r & 3);
+                float rhs = 0.f, p = acc[r] + red[rt][r][lane];
+                if (row < bs && oc_ok) {
+                    const float* src = LOWER ? (const float*)(B + (int64_t)perm[r0 + row] * J.ldb + c0 + bc)
+                                             : (const float*)(X + (r0 + row) * nrhs + c0 + bc);
+                    rhs = src[bq];
+                } else {
+                    p = 0.f;
+                }
+                ((float*)(tb + row * LDX + bc))[bq] = rhs - p;
+            }
+        }
+        __syncthreads();
+        // x(block) = inv(A_bb) t; the inverse is zero beyond bs
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+        {
+            constexpr int KI = TS_BS / 2;  // block columns per k-half
+            const cpx<float>* Mi = inv + (bi * TS_BS + arow) * TS_BS + kh * KI;
+#pragma unroll 8
+            for (int u = 0; u < KI; ++u) {
+                const float2 m = *(const float2*)(Mi + u);
+                const cpx<float> v = tb[(kh * KI + u) * LDX + bc];
+                const float bv = bsg * (bim ? v.im : v.re);
+                const float av = hi ? -m.y : m.x;
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+            }
+        }
+        if (kh == 1)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) red[rt][r][lane] = acc[r];
+        __syncthreads();
+        if (kh == 0 && oc_ok) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = 32 * rt + 8 * (r >> 2) + 4 * hi + (r & 3);
+                if (row < bs) ((float*)(X + (r0 + row) * nrhs + c0 + bc))[bq] = acc[r] + red[rt][r][lane];
+            }
+        }
+        __syncthreads();  // block rows of x visible to the next block's update; tb, red free
+    }
+    if (!LOWER) {
+        for (int64_t idx = tid; idx < n * TSC; idx += 256) {
+            const int64_t r = idx / TSC, c = idx % TSC;
+            if (c0 + c < nrhs) B[r * J.ldb + c0 + c] = X[r * nrhs + c0 + c];
+        }
+    }
+}
+
 template <class T>
 int lu_solve_batched(const mba_lu_solve_job* jobs, int njob, void* scratch, size_t scratch_bytes, hipStream_t st)
 {
@@ -1989,6 +2154,14 @@ int lu_solve_batched(const mba_lu_solve_job* jobs, int njob, void* scratch, size
         return MBA_ERR_LAUNCH;
     // the host tables are released on return: wait for the copies
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    if (sizeof(T) == 4) {
+        const unsigned blocks = (unsigned)(((wg[njob] + NUM_XCD - 1) / NUM_XCD) * NUM_XCD);
+        hipLaunchKernelGGL(k_trisolve_mfma<true>, dim3(blocks), dim3(256), 0, st, dj, dw, njob);
+        MBA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_trisolve_mfma<false>, dim3(blocks), dim3(256), 0, st, dj, dw, njob);
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
+    }
     hipLaunchKernelGGL((k_trisolve_cols<T, true>), dim3((unsigned)wg[njob]), dim3(256), 0, st, dj, dw, njob);
     MBA_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_trisolve_cols<T, false>), dim3((unsigned)wg[njob]), dim3(256), 0, st, dj, dw, njob);
