@@ -233,7 +233,7 @@ def run_c3(args, dev):
     all meshes (one launch for the far field of every mesh, one for all near
     pairs, a gather per mesh), the RHS, the FP32 LUs of all meshes in lock step
     (shared cooperative panel launches, autotuned GEMM tiles), multi-RHS
-    solves and FP64-GEMM refinement to 1e-8."""
+    solves and FP64-GEMM refinement to an estimated relative error of 1e-8."""
     import numpy as np
     import torch
 

@@ -174,8 +174,11 @@ class BatchedMoM:
         A sweep is one grouped FP64 MFMA launch for the residuals of every mesh
         still refining and two launches for their triangular solves with all
         right-hand sides (``lu_solve_batched``); one host synchronisation then
-        reads the largest relative correction of every mesh, and a mesh stops
-        once it is at most ``tol`` (its error before that sweep)."""
+        reads the largest relative correction of every mesh.  The correction of
+        sweep s estimates the error before it, and its ratio to the previous
+        correction the contraction per sweep; a mesh stops once the error left
+        after the sweep (correction x contraction; the correction alone after
+        the first sweep) is at most ``tol``."""
         lib = _lib.load()
         if tuner is not None:
             tuner.apply(max(Z.shape[0] for Z in Zs))
@@ -190,7 +193,7 @@ class BatchedMoM:
         V128s = [V.to(torch.complex128).contiguous() for V in Vs]
         Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs])]
         Rs = [torch.empty_like(V) for V in V128s]
-        its, conv = [0] * n_mesh, [False] * n_mesh
+        its, conv, prev = [0] * n_mesh, [False] * n_mesh, [None] * n_mesh
         active = list(range(n_mesh))
         while active:
             for i in active:
@@ -204,7 +207,11 @@ class BatchedMoM:
             rels = torch.stack(rels).cpu().tolist()
             for i, r in zip(active, rels):
                 its[i] += 1
-                conv[i] = r <= tol
+                # error left after this sweep: its correction r times the
+                # observed contraction r / (previous correction)
+                est = r if prev[i] is None else min(r, r * r / max(prev[i], 1e-300))
+                prev[i] = r
+                conv[i] = est <= tol
             active = [i for i in active if not conv[i] and its[i] < maxit]
         torch.cuda.synchronize()
         t2 = time.perf_counter()
