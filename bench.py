@@ -227,6 +227,97 @@ def run_c5(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_field(args, dev):
+    """Field radiation (post-processing of a MoM solution): scattered H of the
+    vertex currents of the 30 720-RWG icosphere (10 242 vertices, ka = 10) on
+    the demo's 256 x 256 sampling plane (65 536 points), complex64 currents
+    (seeded random values: the cost does not depend on them).  One step = one
+    radiate_H call over all 6.7e8 (point, vertex) pairs.  Roofline: FP32
+    VALU, 61 flops per pair (offset 3, R^2 5, R 1, phase 1, kR 1, 1/R^3 2,
+    complex factor 6, cross product 18, complex multiply-add 24) plus three
+    transcendentals (rsqrt, sin, cos)."""
+    import numpy as np
+    import torch
+
+    from mesh_bem_amd.geometry import icosphere
+    from mesh_bem_amd.radiation import radiate_H, xz_grid
+
+    subdiv, k, res = (5, 10.0, 256) if args.workload == "field" else (3, 4.0, 64)
+    Vn, Fn = icosphere(subdiv)
+    rng = np.random.Generator(np.random.PCG64(11))
+    J = rng.normal(size=(len(Vn), 3)) + 1j * rng.normal(size=(len(Vn), 3))
+    v = torch.tensor(Vn, dtype=torch.float64, device=dev)
+    f = torch.tensor(Fn, dtype=torch.int64, device=dev)
+    a = torch.zeros(len(Vn), dtype=torch.float64, device=dev)
+    e1, e2 = v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 1]]
+    a.index_add_(0, f.reshape(-1), (0.5 * torch.linalg.cross(e1, e2).norm(dim=1) / 3.0).repeat_interleave(3))
+    pts = xz_grid(res, device=dev)
+    P32, V32, A32 = pts.float(), v.float(), a.float()
+    J32 = torch.tensor(J, dtype=torch.complex64, device=dev)
+    n_pairs = pts.shape[0] * len(Vn)
+    log(f"workload {args.workload}: {pts.shape[0]} points x {len(Vn)} vertices")
+    for _ in range(args.warmup):
+        radiate_H(k, P32, V32, J32, A32)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        H = radiate_H(k, P32, V32, J32, A32)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    J64 = J32.to(torch.complex128)
+    radiate_H(k, pts, v, J64, a)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter()
+    H64 = radiate_H(k, pts, v, J64, a)
+    torch.cuda.synchronize()
+    ms64 = (time.perf_counter() - t64) * 1e3
+    err32 = float((H.to(torch.complex128) - H64).norm() / H64.norm())
+    flops = 61.0 * n_pairs
+    tflops = flops / (ev_ms * 1e-3) / 1e12
+    out = {
+        "metric": "field radiation (scattered H from vertex currents on a sampling plane); Gpairs/s",
+        "value": round(1e-9 * n_pairs / (ms * 1e-3), 3),
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"icosphere subdiv {subdiv} ({len(Vn)} vertices), ka={k}, {res}x{res} xz plane "
+                               f"in [-2,2]^2, random complex64 vertex currents", "parallelism": "single GPU"},
+        "fp64_ms": round(ms64, 3),
+        "fp32_relerr_vs_fp64": float(f"{err32:.3e}"),
+        "roofline": {"kernel": "k_radiate_h<float> + k_radiate_sum (61 flops + 3 transcendentals per pair)",
+                     "bound": "valu", "achieved": round(tflops, 2), "peak": PEAK_FP32_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_MFMA_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(ev_ms, 4), "flops_per_launch": flops,
+                     "transcendentals_per_s": float(f"{3 * n_pairs / (ev_ms * 1e-3):.3e}")},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle sample)")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        sample = 2048 if args.workload == "field" else 256
+        Pc = pts[:sample].cpu().numpy()
+        tc = time.perf_counter()
+        O.radiate_H(k, Pc, Vn, J, a.cpu().numpy())
+        dt = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": round(1e-9 * sample * len(Vn) / dt, 6), "unit": "Gpairs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle.radiate_H (numpy, FP64) on the first {sample} grid points x all "
+                                         f"{len(Vn)} vertices, {dt:.2f} s"}
+    print(json.dumps(out), flush=True)
+
+
 def run_c3(args, dev):
     """Config 3: 64 seeded star meshes on geodesic spheres of frequency 5..11
     (750-3630 RWG), 128 plane waves each, ka ~ U(1, 8).  Step: FP64 assembly of
@@ -422,7 +513,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c3", "c3small", "c4", "c4small", "c5",
-                                                            "c5small"])
+                                                            "c5small", "field", "fieldsmall"])
     ap.add_argument("--epochs", type=int, default=4, help="training epochs (c4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -449,6 +540,8 @@ def main():
         return run_c4(args, dev)
     if args.workload in ("c3", "c3small"):
         return run_c3(args, dev)
+    if args.workload in ("field", "fieldsmall"):
+        return run_field(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

@@ -424,6 +424,21 @@ int mba_mgn_loss(const float *pred, const float *target, const int64_t *graph_of
 int mba_mgn_adam(float *param, const float *grad, float *m, float *v, int64_t n, float lr, float beta1,
                  float beta2, float eps, int64_t step, mba_stream_t stream);
 
+/* ---- field radiation (post-processing of a MoM solution) ----------------
+ * Scattered magnetic field at points p (row-major [n_points, 3]) of vertex
+ * currents J_v (complex [n_verts, 3]) with vertex areas A_v:
+ *   out[p] = sum_v A_v J_v x grad_y G(p, v),
+ *   grad_y G = e^{-jkR} (1 + jkR) (p - v) / (4 pi R^3),
+ * pairs with R < 1e-6 left out.  Replaces the dense torch expression of
+ * demo/demo_Helmholtz_MOM_3d.py:246-257 (gradG_y, src/utils/functions.py:844-868).
+ * out: complex [n_points, 3].  The workspace (mba_radiate_h_workspace_bytes,
+ * 0 for small vertex counts) holds per-slice partial sums, added in a fixed
+ * order. */
+size_t mba_radiate_h_workspace_bytes(int prec, int64_t n_points, int64_t n_verts);
+int mba_radiate_h(int prec, const void *points, int64_t n_points, const void *verts, const void *vert_js,
+                  const void *vert_areas, int64_t n_verts, double wavenumber, void *out, void *workspace,
+                  size_t workspace_bytes, mba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

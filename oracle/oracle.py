@@ -475,3 +475,47 @@ def demag_factors(a, b, c):
         return 0.5 * a * b * c * quad(g, 0, np.inf, limit=200)[0]
 
     return N(a * a), N(b * b), N(c * c)
+
+
+# ------------------------------------------------------------------ field radiation
+def green_G(k, p1, p2):
+    """G = e^{-jkR} / (4 pi R), 0 where R < 1e-6 (src/utils/functions.py:806-841,
+    called with pos_k False: the wavenumber enters as -k)."""
+    tk = -k
+    d = np.linalg.norm(np.asarray(p1, dtype=np.float64) - np.asarray(p2, dtype=np.float64), axis=-1)
+    inv = np.where(d < 1e-6, 0.0, 1.0 / np.where(d < 1e-6, 1.0, d))
+    kx = tk * d
+    return (np.cos(kx) + 1j * np.sin(kx)) / (4 * math.pi) * inv
+
+
+def green_gradG_y(k, p1, p2):
+    """d G / d p2 as written in src/utils/functions.py:844-868:
+    G * inv_dist * (1 - j kx) * ray_d with kx = -k R and ray_d = (p1 - p2) / R."""
+    p1 = np.asarray(p1, dtype=np.float64)
+    p2 = np.asarray(p2, dtype=np.float64)
+    tk = -k
+    diff = p1 - p2
+    d = np.linalg.norm(diff, axis=-1, keepdims=True)
+    ray = diff / np.maximum(d, 1e-12)
+    inv = np.where(d < 1e-6, 0.0, 1.0 / np.where(d < 1e-6, 1.0, d))
+    kx = tk * d
+    Gv = green_G(k, p1, p2)[..., None]
+    return Gv * inv * (1 - 1j * kx) * ray
+
+
+def radiate_H(k, points, verts, vert_Js, vert_areas, block=256):
+    """Scattered H of vertex currents at points, the demo's expression
+    (demo/demo_Helmholtz_MOM_3d.py:246-257):
+        -(vert_areas * cross(Js, -gradG_y(k, p1=pm, p2=verts))).sum(-2)
+    evaluated over blocks of points.  points [Nm,3], verts [V,3], vert_Js [V,3]
+    complex, vert_areas [V].  Returns [Nm,3] complex128."""
+    P = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    V = np.asarray(verts, dtype=np.float64).reshape(-1, 3)
+    J = np.asarray(vert_Js, dtype=np.complex128).reshape(-1, 3)
+    A = np.asarray(vert_areas, dtype=np.float64).reshape(-1, 1)
+    out = np.empty((len(P), 3), dtype=np.complex128)
+    for b0 in range(0, len(P), block):
+        pm = P[b0:b0 + block, None, :]
+        g = green_gradG_y(k, pm, V[None, :, :])
+        out[b0:b0 + block] = -(A[None] * np.cross(np.broadcast_to(J[None], g.shape), -g)).sum(axis=-2)
+    return out
