@@ -318,6 +318,93 @@ def run_field(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_hd(args, dev):
+    """Ferrofluid step, surface Helmholtz decomposition
+    (MeshHelmholtzDecomposition.solve): noisy ellipsoid of geodesic frequency
+    45 (40 500 faces, 20 252 vertices, the config-5 geometry), gaussQR = 4
+    (16 points per face), one seeded random velocity field, FP32.  One step =
+    one solve(): the dense sum over 20 252 targets x 648 000 source points
+    (1.3e10 pairs) plus the O(V) normal projection.  Roofline: FP32 VALU, 29
+    flops per pair (offset 3, R^2 5, 1/R^3 scaled 3, (n.u) d + (n x u) x d 12,
+    accumulate 6) and one rsqrt."""
+    import numpy as np
+    import torch
+
+    from mesh_bem_amd.ferrofluid import MeshHelmholtzDecomposition
+    from mesh_bem_amd.geometry import noisy_ellipsoid
+
+    freq, Q = (45, 4) if args.workload == "hd" else (15, 4)
+    Vn, Fn, c = noisy_ellipsoid(0, freq)
+    rng = np.random.Generator(np.random.PCG64(17))
+    vel = rng.normal(size=(1, len(Vn), 3))
+    v = torch.tensor(Vn, dtype=torch.float32, device=dev)
+    f = torch.tensor(Fn, dtype=torch.int32, device=dev)
+    u = torch.tensor(vel, dtype=torch.float32, device=dev)
+    hd = MeshHelmholtzDecomposition(gaussQR=Q, device=dev, dtype=torch.float32)
+    n_pairs = len(Vn) * len(Fn) * Q * Q
+    log(f"workload {args.workload}: {len(Vn)} vertices x {len(Fn)} faces x {Q * Q} points")
+    for _ in range(args.warmup):
+        hd.solve(v, f, u)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        out = hd.solve(v, f, u)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the pair sum alone, HIP events on the launch stream
+    from mesh_bem_amd.ferrofluid import decomposition_velocity
+
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        decomposition_velocity(v, f, u, Q)
+    e1.record()
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / args.steps
+    out64 = MeshHelmholtzDecomposition(gaussQR=Q, device=dev, dtype=torch.float64).solve(v.double(), f, u.double())
+    err32 = float((out.double() - out64).norm() / out64.norm())
+    flops = 29.0 * n_pairs
+    tflops = flops / (k_ms * 1e-3) / 1e12
+    res = {
+        "metric": "ferrofluid step: surface Helmholtz decomposition (vertex x face-quadrature pair sum); Gpairs/s",
+        "value": round(1e-9 * n_pairs / (ms * 1e-3), 3),
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"noisy ellipsoid geodesic frequency {freq} ({len(Fn)} faces, {len(Vn)} vertices), "
+                               f"gaussQR={Q}, one random velocity field", "parallelism": "single GPU"},
+        "fp32_relerr_vs_fp64": float(f"{err32:.3e}"),
+        "roofline": {"kernel": "k_hd_sum<float> + k_hd_slices (29 flops + 1 rsqrt per pair)", "bound": "valu",
+                     "achieved": round(tflops, 2), "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / PEAK_FP32_MFMA_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(k_ms, 4), "flops_per_launch": flops},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle sample)")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        sample = 64
+        tc = time.perf_counter()
+        O.helmholtz_decomposition_u(Vn, Fn, vel, Q, targets=Vn[:sample], block=2)
+        dt = time.perf_counter() - tc
+        res["cpu_baseline"] = {"value": round(1e-9 * sample * len(Fn) * Q * Q / dt, 6), "unit": "Gpairs/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle.helmholtz_decomposition_u (numpy, FP64) at the first {sample} "
+                                         f"vertices x all {len(Fn) * Q * Q} source points, {dt:.2f} s"}
+    print(json.dumps(res), flush=True)
+
+
 def run_c3(args, dev):
     """Config 3: 64 seeded star meshes on geodesic spheres of frequency 5..11
     (750-3630 RWG), 128 plane waves each, ka ~ U(1, 8).  Step: FP64 assembly of
@@ -513,7 +600,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c3", "c3small", "c4", "c4small", "c5",
-                                                            "c5small", "field", "fieldsmall"])
+                                                            "c5small", "field", "fieldsmall", "hd", "hdsmall"])
     ap.add_argument("--epochs", type=int, default=4, help="training epochs (c4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -542,6 +629,8 @@ def main():
         return run_c3(args, dev)
     if args.workload in ("field", "fieldsmall"):
         return run_field(args, dev)
+    if args.workload in ("hd", "hdsmall"):
+        return run_hd(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

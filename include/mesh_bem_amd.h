@@ -439,6 +439,23 @@ int mba_radiate_h(int prec, const void *points, int64_t n_points, const void *ve
                   const void *vert_areas, int64_t n_verts, double wavenumber, void *out, void *workspace,
                   size_t workspace_bytes, mba_stream_t stream);
 
+/* ---- ferrofluid step: surface Helmholtz-decomposition sums --------------
+ * For velocity fields b < batch (vert_vel [batch, n_verts, 3]) and targets x
+ * ([n_targets, 3]):
+ *   out[b, x] = sum_f sum_q W_fq [ (n_f . u_fq) g + (n_f x u_fq) x g ],
+ *   g = (y_fq - x) / (4 pi |y_fq - x|^3), W_fq = 2 A_f w_q r1_q,
+ * y_fq, u_fq the linear interpolation at (r1_q, r2_q) on the unit panel of
+ * face f; rule = [nq, 3] rows (r1, r2, w).  Replaces the u1 + u2 tensor sums of
+ * MeshHelmholtzDecomposition.solve (src/mesh_ferrofluid/mesh_Hemholtz_decomposition.py:80-126).
+ * out: [batch, n_targets, 3].  Workspace: mba_helmholtz_decomp_workspace_bytes
+ * (per-slice partial sums, added in a fixed order; 0 when not sliced). */
+size_t mba_helmholtz_decomp_workspace_bytes(int prec, int64_t n_targets, int64_t n_faces, int32_t nq,
+                                            int64_t batch);
+int mba_helmholtz_decomp(int prec, const void *targets, int64_t n_targets, const void *verts,
+                         const int32_t *faces, int64_t n_verts, int64_t n_faces, const void *vert_vel,
+                         int64_t batch, const void *rule, int32_t nq, void *out, void *workspace,
+                         size_t workspace_bytes, mba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

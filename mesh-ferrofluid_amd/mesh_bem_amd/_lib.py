@@ -120,6 +120,9 @@ SIGNATURES = {
     "mba_near_pairs_count": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dbl, _i64p, _vp, _sz, _vp]),
     "mba_near_pairs_fill": (_c.c_int, [_i64, _dbl, _i64p, _i32p, _vp, _sz, _vp]),
     "mba_add_diag": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _dbl, _vp, _vp]),
+    "mba_helmholtz_decomp_workspace_bytes": (_sz, [_c.c_int, _i64, _i64, _i32, _i64]),
+    "mba_helmholtz_decomp": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i32, _vp, _vp,
+                                        _sz, _vp]),
     "mba_radiate_h_workspace_bytes": (_sz, [_c.c_int, _i64, _i64]),
     "mba_radiate_h": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _vp, _vp, _i64, _dbl, _vp, _vp, _sz, _vp]),
     "mba_mgn_gemm": (_c.c_int, [_vp, _c.c_int, _vp, _i64, _c.c_int, _i64, _i64, _i64, _vp, _i64, _vp, _vp,

@@ -519,3 +519,65 @@ def radiate_H(k, points, verts, vert_Js, vert_areas, block=256):
         g = green_gradG_y(k, pm, V[None, :, :])
         out[b0:b0 + block] = -(A[None] * np.cross(np.broadcast_to(J[None], g.shape), -g)).sum(axis=-2)
     return out
+
+
+# ------------------------------------------------------------------ ferrofluid: Helmholtz decomposition
+def vert_normals(verts, faces):
+    """Area-weighted vertex normals, normalised (src/utils/mesh_utils.py:242-274)."""
+    a, n = face_areas_normals(verts, faces)
+    vn = np.zeros((len(verts), 3))
+    f = np.asarray(faces, dtype=np.int64)
+    for c in range(3):
+        np.add.at(vn, f[:, c], a[:, None] * n)
+    return vn / np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-12)
+
+
+def helmholtz_decomposition_u(verts, faces, vel, gaussQR, targets=None, block=64):
+    """u1 + u2 of MeshHelmholtzDecomposition.solve
+    (src/mesh_ferrofluid/mesh_Hemholtz_decomposition.py:52-126) at targets
+    (default: the vertices), literally: tensor rule r1 = p[b], r2 = p[a] p[b],
+    w = w[a] w[b] (:33-48); weights 2 A_f w r1 (BEM_utils.py:254-310, PLANAR);
+    points and velocities (1-r1) x1 + (r1-r2) x2 + r2 x3 (BEM_utils.py:210-251);
+    grad_G_y_3d(x=y, y=target) = (y - target) / (4 pi R^3) (BEM_utils.py:141-172);
+    u1 = sum W (n . u) g, u2 = sum W (n x u) x g.  vel [B, V, 3] -> [B, Nt, 3]."""
+    v = _f64(verts)
+    f = np.asarray(faces, dtype=np.int64)
+    vel = np.asarray(vel, dtype=np.float64)
+    if vel.ndim == 2:
+        vel = vel[None]
+    p, w = gl_ref(gaussQR)
+    r1 = np.tile(p[None, :], (gaussQR, 1)).reshape(-1)
+    r2 = (p[:, None] * p[None, :]).reshape(-1)
+    ww = (w[:, None] * w[None, :]).reshape(-1)
+    area, n = face_areas_normals(v, f)
+    W = 2.0 * area[:, None] * ww[None, :] * r1[None, :]  # [F, Q2]
+    c0, c1, c2 = (1 - r1)[None, :, None], (r1 - r2)[None, :, None], r2[None, :, None]
+    y = c0 * v[f[:, 0]][:, None] + c1 * v[f[:, 1]][:, None] + c2 * v[f[:, 2]][:, None]  # [F, Q2, 3]
+    tg = v if targets is None else np.asarray(targets, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((vel.shape[0], len(tg), 3))
+    for b in range(vel.shape[0]):
+        ub = vel[b]
+        uy = c0 * ub[f[:, 0]][:, None] + c1 * ub[f[:, 1]][:, None] + c2 * ub[f[:, 2]][:, None]  # [F, Q2, 3]
+        ndu = (n[:, None, :] * uy).sum(-1)  # [F, Q2]
+        ncu = np.cross(np.broadcast_to(n[:, None, :], uy.shape), uy)  # [F, Q2, 3]
+        for t0 in range(0, len(tg), block):
+            x = tg[t0:t0 + block]
+            d = y[None] - x[:, None, None, :]  # [T, F, Q2, 3]
+            R = np.linalg.norm(d, axis=-1, keepdims=True)
+            g = (1.0 / 4.0 / math.pi) * (d / R) / (R * R)
+            u1 = (W[None, :, :, None] * ndu[None, :, :, None] * g).sum(axis=(1, 2))
+            u2 = (np.cross(np.broadcast_to(ncu[None], g.shape), g) * W[None, :, :, None]).sum(axis=(1, 2))
+            out[b, t0:t0 + block] = u1 + u2
+    return out
+
+
+def helmholtz_decomposition_solve(verts, faces, vel, gaussQR):
+    """new_vel of MeshHelmholtzDecomposition.solve (mesh_Hemholtz_decomposition.py:128-143):
+    (I - n n^T) vel + (n . (u1 + u2)) n with area-weighted vertex normals."""
+    u = helmholtz_decomposition_u(verts, faces, vel, gaussQR)
+    vn = vert_normals(verts, faces)
+    vel = np.asarray(vel, dtype=np.float64)
+    if vel.ndim == 2:
+        vel = vel[None]
+    un = (vn[None] * u).sum(-1, keepdims=True)
+    return vel - (vel * vn[None]).sum(-1, keepdims=True) * vn[None] + un * vn[None]
