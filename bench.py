@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "mesh-ferrofluid_amd"))
 
 METRIC = "MoM solve time @30k RWG, relerr<=1e-6 vs FP64, mixed vs FP64; Mpairs/s"
 PEAK_FP32_MFMA_TFLOPS = 157.3
+PEAK_FP16_MFMA_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -605,6 +606,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
     ap.add_argument("--no-extra", action="store_true", help="only the timed steps")
+    ap.add_argument("--lu-update", default="split_f16", choices=["fp32", "split_f16"],
+                    help="trailing-update product of the complex64 LU (c2/c1/small)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -616,7 +619,7 @@ def main():
     import torch
 
     from mesh_bem_amd import _lib
-    from mesh_bem_amd.dense import LUFactor, refine_solve, to_c64
+    from mesh_bem_amd.dense import LUFactor, refine_solve, set_lu_update, to_c64
     from mesh_bem_amd.geometry import icosphere
     from mesh_bem_amd.mom import MomMesh, assemble_ns, rhs_ns
 
@@ -640,6 +643,7 @@ def main():
     E = mesh.n_edges
     dirs, pols = [[0.0, 0.0, 1.0]], [[1.0, 0.0, 0.0]]
     F64 = _lib.MBA_F64
+    set_lu_update(args.lu_update)
     Z64 = torch.empty((E, E), dtype=torch.complex128, device=dev)
     stream = torch.cuda.current_stream()
 
@@ -682,6 +686,7 @@ def main():
         "refine_iterations": info.iterations,
         "refine_converged": info.converged,
         "refine_fallback": info.fallback or None,
+        "lu_update": args.lu_update,
     }
     if not args.no_extra:
         # phase breakdown + roofline of the dominant kernel (trailing GEMM)
@@ -699,22 +704,48 @@ def main():
         torch.cuda.synchronize()
         td = time.perf_counter()
         prof = lu.profile
-        gemm_ms = prof["phase_ms"]["gemm"]
-        nl = max(1, prof["phase_launches"]["gemm"])
-        achieved = prof["gemm_flops"] / (gemm_ms * 1e-3) / 1e12
-        out["roofline"] = {
-            "kernel": "k_cgemm3m_f32 (LU trailing update, 3M complex product on v_mfma_f32_32x32x2f32; achieved counts 8mnk flops)",
-            "bound": "mfma",
-            "achieved": round(achieved, 2),
-            "peak": PEAK_FP32_MFMA_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4),
-            "traffic": pmc_traffic("traffic_c2_mixed_solve.md") if args.workload == "c2" else None,
-            "traffic_source": "profiles/traffic_c2_mixed_solve.md (FETCH_SIZE x2.0 + WRITE_SIZE, copy-calibrated)",
-            "launches": nl,
-            "avg_launch_ms": round(gemm_ms / nl, 4),
-            "flops_per_launch_avg": prof["gemm_flops"] / nl,
-        }
+        if prof["phase_launches"]["split"] > 0:
+            # split-FP16 trailing updates: achieved = algorithmic 8mnk flops;
+            # peak = the scheme's FP32-equivalent MFMA peak (FP16 dense peak /
+            # 3 piece products per real product), so frac = FP16 MFMA utilisation
+            sp_ms = prof["phase_ms"]["split"]
+            nl = max(1, prof["phase_launches"]["split"])
+            achieved = prof["split_flops"] / (sp_ms * 1e-3) / 1e12
+            peak = PEAK_FP16_MFMA_TFLOPS / 3.0
+            out["roofline"] = {
+                "kernel": "k_cgemm_h3 (LU trailing update: FP32-accurate complex product on "
+                          "v_mfma_f32_32x32x16_f16, hi/lo split operands, 12 MFMAs per 32x32 block per 16 k; "
+                          "split kernels included; achieved counts 8mnk flops)",
+                "bound": "mfma",
+                "achieved": round(achieved, 2),
+                "peak": round(peak, 1),
+                "peak_basis": "FP16 dense MFMA 2500 TFLOP/s / 3 piece products per real product",
+                "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4),
+                "traffic": None,
+                "launches": nl,
+                "avg_launch_ms": round(sp_ms / nl, 4),
+                "flops_per_launch_avg": prof["split_flops"] / nl,
+                "fp32_products_ms": round(prof["phase_ms"]["gemm"], 2),
+            }
+        else:
+            gemm_ms = prof["phase_ms"]["gemm"]
+            nl = max(1, prof["phase_launches"]["gemm"])
+            achieved = prof["gemm_flops"] / (gemm_ms * 1e-3) / 1e12
+            out["roofline"] = {
+                "kernel": "k_cgemm3m_f32 (LU trailing update, 3M complex product on v_mfma_f32_32x32x2f32; "
+                          "achieved counts 8mnk flops)",
+                "bound": "mfma",
+                "achieved": round(achieved, 2),
+                "peak": PEAK_FP32_MFMA_TFLOPS,
+                "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4),
+                "traffic": pmc_traffic("traffic_c2_mixed_solve.md") if args.workload == "c2" else None,
+                "traffic_source": "profiles/traffic_c2_mixed_solve.md (FETCH_SIZE x2.0 + WRITE_SIZE, copy-calibrated)",
+                "launches": nl,
+                "avg_launch_ms": round(gemm_ms / nl, 4),
+                "flops_per_launch_avg": prof["gemm_flops"] / nl,
+            }
         out["phases_s"] = {
             "assembly_fp64": round(tb - ta, 4),
             "lu_fp32": round(tc - tb, 4),
@@ -736,6 +767,20 @@ def main():
         Zerr = (Z32a.to(torch.complex128) - Z64).abs().max().item() / Z64.abs().max().item()
         out["assembly_fp32_max_abs_err_rel_max"] = float(f"{Zerr:.3e}")
         del Z32a, Z32
+        if args.lu_update != "fp32":
+            # the same factorisation with the FP32 MFMA trailing update, for comparison
+            set_lu_update("fp32")
+            Z32 = to_c64(Z64)
+            torch.cuda.synchronize()
+            tg = time.perf_counter()
+            lu2 = LUFactor(Z32, overwrite=True)
+            torch.cuda.synchronize()
+            th = time.perf_counter()
+            _, info2 = refine_solve(Z64, V, lu32=lu2, tol=1e-8)
+            out["lu_fp32_update_s"] = round(th - tg, 4)
+            out["refine_iterations_fp32_update"] = info2.iterations
+            set_lu_update(args.lu_update)
+            del lu2, Z32
         if not args.no_fp64:
             log("plain FP64 solve (complex128 assembly + torch.linalg.solve)")
             torch.cuda.synchronize()

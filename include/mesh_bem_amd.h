@@ -253,6 +253,11 @@ int mba_lu_solve_batched(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs,
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
 int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch, size_t scratch_bytes,
                           int32_t *err_out, mba_stream_t stream);
+/* mba_lu_factor with per-phase timing (HIP events on the stream; the call
+ * synchronises): phase_ms[5] and phase_launches[5] for the panels, row
+ * interchanges, triangular solves, FP32 / FP64 MFMA products and split-FP16
+ * trailing updates; gemm_flops[2] = algorithmic flops (8 m n k per complex
+ * product) of the last two phases.  No reference counterpart. */
 int mba_lu_factor_profiled(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
                            void *workspace, size_t workspace_bytes, double *phase_ms,
                            int64_t *phase_launches, double *gemm_flops, mba_stream_t stream);
@@ -291,6 +296,27 @@ int mba_gemm_variant(int prec, int variant, int64_t m, int64_t n, int64_t k, dou
                      const void *a, int64_t lda, const void *b, int64_t ldb, double beta, void *c,
                      int64_t ldc, mba_stream_t stream);
 int mba_gemm_set_variant(int prec, int variant);
+
+/* Complex64 GEMM C = alpha A B + beta C (row-major, 1 <= k <= 512) with FP32
+ * accuracy on the FP16 matrix cores: rows of A and columns of B are scaled by
+ * powers of two, every value is split into two FP16 pieces (hi + lo), and the
+ * three piece products hi.hi + hi.lo + lo.hi accumulate in FP32 (error about
+ * 2^-22 |A||B| against 2^-24 for an FP32 product).  workspace >=
+ * mba_gemm_split_f16_workspace_bytes.  It is the trailing-update product of the
+ * complex64 LU when mba_lu_set_update(1) is selected: a dense contraction
+ * inside the getrf behind torch.linalg.solve
+ * (src/mesh_BEM/mesh_BEM_solver_3d.py:174); no direct reference counterpart. */
+size_t mba_gemm_split_f16_workspace_bytes(int64_t m, int64_t n, int64_t k);
+int mba_gemm_split_f16(int64_t m, int64_t n, int64_t k, double alpha, const void *a, int64_t lda,
+                       const void *b, int64_t ldb, double beta, void *c, int64_t ldc, void *workspace,
+                       size_t workspace_bytes, mba_stream_t stream);
+
+/* Trailing-update product of the complex64 LU (mba_lu_factor and its
+ * asynchronous / profiled forms): 0 = FP32 MFMA (the mba_gemm_set_variant
+ * choice, default), 1 = the split-FP16 product of mba_gemm_split_f16 for
+ * trailing blocks of at least 2048 x 2048.  Process-wide; set before
+ * factorising, not concurrently with one.  No counterpart in the reference. */
+int mba_lu_set_update(int kind);
 
 /* FP64 residual r = v - Z x for complex128 Z [n, ldz], x/v/r [n, nrhs]. */
 int mba_residual_c128(const void *z, int64_t n, int64_t ldz, const void *x, const void *v,

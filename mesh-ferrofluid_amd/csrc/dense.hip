@@ -14,6 +14,7 @@
 
 #include <vector>
 
+#include "gemm_h3.hpp"
 #include "mba_common.hpp"
 
 namespace {
@@ -284,26 +285,33 @@ __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, fl
             __syncthreads();
         }
     }
-    // epilogue: register r of lane -> row 8*(r/4) + 4*hi + r%4, complex column jj
+    // epilogue: register r of lane -> row 8*(r/4) + 4*hi + r%4, complex column jj.
+    // All C loads of a 32-row block are issued before its stores (one pointer
+    // for both, so the compiler would otherwise serialise load after store).
     const int64_t gj = n0 + wn * 32 + jj;
     if (gj < N) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 2; ++i) {
+            float2 cv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                cv[r] = (beta != 0.f && gi < M) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
+            }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
                 if (gi < M) {
                     float re = alpha * (t1[i][r] - t2[i][r]);
                     float im = alpha * (t3[i][r] - t1[i][r] - t2[i][r]);
-                    float2* pc = (float2*)(C + gi * ldc + gj);
                     if (beta != 0.f) {
-                        const float2 c0 = *pc;
-                        re += beta * c0.x;
-                        im += beta * c0.y;
+                        re += beta * cv[r].x;
+                        im += beta * cv[r].y;
                     }
-                    *pc = make_float2(re, im);
+                    *(float2*)(C + gi * ldc + gj) = make_float2(re, im);
                 }
             }
+        }
     }
 }
 
@@ -393,6 +401,11 @@ int launch_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A
 }
 
 int g_gemm_f32_variant = 4;  // tile choice of the FP32 GEMM (set by mba_gemm_set_variant)
+// trailing update of the complex64 LU: 0 = FP32 MFMA (the variant above),
+// 1 = FP32-accurate split-FP16 MFMA product (gemm_h3.hip); set by mba_lu_set_update
+int g_lu_update_f32 = 0;
+// the split product pays off once the trailing block has this many rows and columns
+constexpr int64_t SPLIT_MIN_DIM = 2048;
 
 int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
               const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
@@ -493,7 +506,20 @@ __device__ __forceinline__ void cgemm_f64_tile(int64_t M, int64_t N, int64_t K, 
         }
     }
     // epilogue: register r of lane -> row lane/16 + 4*r, real column lane%16
-    // (v_mfma_f64_16x16x4f64 interleaves its output rows across registers)
+    // (v_mfma_f64_16x16x4f64 interleaves its output rows across registers).
+    // C loads are all issued before the stores (one pointer for both).
+    double cv[2][2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t gjr = 2 * (n0 + wn * 16 + j * 8) + jj;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gi = m0 + wm * 32 + i * 16 + kr + 4 * r;
+                cv[i][j][r] = (beta != 0.0 && gjr < 2 * N && gi < M) ? *((const double*)(C + gi * ldc) + gjr) : 0.0;
+            }
+        }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -506,7 +532,7 @@ __device__ __forceinline__ void cgemm_f64_tile(int64_t M, int64_t N, int64_t K, 
                 if (gi < M) {
                     double* p = (double*)(C + gi * ldc) + gjr;
                     double v = alpha * acc[i][j][r];
-                    if (beta != 0.0) v += beta * *p;
+                    if (beta != 0.0) v += beta * cv[i][j][r];
                     *p = v;
                 }
             }
@@ -709,11 +735,77 @@ __device__ __forceinline__ void block_argmax(T v, int32_t r, int32_t q, T* red_v
     __syncthreads();
 }
 
+// Waits until all N tagged words carry `epoch`; the N loads are issued
+// together, so a wait costs one memory round trip, not N.  Only words still
+// stale are re-read.  Bounded like tag_get.
+template <int N>
+__device__ __forceinline__ void tag_get_n(uint64_t* const (&p)[N], uint32_t epoch, int* err, uint32_t (&out)[N])
+{
+    uint64_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long spins = 0;
+    while (true) {
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i) ok = ok && (uint32_t)(w[i] >> 32) == epoch;
+        if (ok) break;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if ((uint32_t)(w[i] >> 32) != epoch)
+                w[i] = __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (uint32_t)w[i];
+}
+
+template <class T>
+__device__ __forceinline__ T val_of(const uint32_t* wds);
+template <>
+__device__ __forceinline__ float val_of<float>(const uint32_t* wds) { return __uint_as_float(wds[0]); }
+template <>
+__device__ __forceinline__ double val_of<double>(const uint32_t* wds)
+{
+    return __longlong_as_double((long long)(((unsigned long long)wds[0] << 32) | wds[1]));
+}
+
+// Publishes local row r (panel-local row index pr) of the LDS panel, all nb
+// columns, into the message row `dst` (thread c writes column c).
+template <class T>
+__device__ __forceinline__ void put_row(uint64_t* dst, uint32_t epoch, const cpx<T>* row, int nb)
+{
+    constexpr int VW = sizeof(T) / 4;
+    for (int c = threadIdx.x; c < nb; c += PANEL_THREADS) {
+        const cpx<T> x = row ? row[c] : cpx<T>{(T)0, (T)0};
+        put_val(dst + (2 * c) * VW, epoch, x.re);
+        put_val(dst + (2 * c + 1) * VW, epoch, x.im);
+    }
+}
+
 // Panel body for workgroup g of G (the workgroups owning this panel's rows).
+//
+// Per column j the critical path is two memory round trips: (1) the G
+// candidates (row, |.|_1) of column j, (2) the winner's row and row j, both
+// published in full.  The candidates of column j + 1 are formed and
+// published right after column j is scaled and column j + 1 updated, and the
+// two rows that may be read next (the local candidate and row j + 1) are
+// completed and published before the bulk rank-1 update of the other rows,
+// so that update overlaps the exchange.  Buffers alternate on j parity: data
+// of column j + 1 is written (during column j) only after this workgroup has
+// read every workgroup's column-j candidate, and each workgroup publishes its
+// column-j candidate only after finishing its column j - 1 reads, so the
+// column j - 1 buffers are no longer read when they are overwritten.
 template <class T>
 __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, const int g)
 {
     constexpr int VW = sizeof(T) / 4;  // 32-bit words per real
+    constexpr int32_t NONE = 0x7fffffff;
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* rows = (cpx<T>*)smem_raw;  // [R][nb]
     __shared__ T red_v[PANEL_THREADS / 64];
@@ -728,6 +820,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     const int64_t lr0 = (int64_t)g * R;  // first local row of this workgroup
     const int nr = (int)((P.m - lr0) < R ? (P.m - lr0 > 0 ? P.m - lr0 : 0) : R);
     const int64_t row_words = (int64_t)nb * 2 * VW;
+    auto owns = [&](int64_t pr) { return pr >= lr0 && pr < lr0 + nr; };
 
     for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
         int r = idx / nb, c = idx % nb;
@@ -735,56 +828,43 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     }
     __syncthreads();
 
-    for (int j = 0; j < nb; ++j) {
-        const int par = j & 1;
-        const uint32_t epoch = (uint32_t)(P.k + j + 1);
-        // --- local arg-max over local rows with panel row index >= j
+    // column 0: local arg-max, then publish candidate, candidate row, row 0
+    {
         T bv = (T)-1;
-        int32_t br = 0x7fffffff;
+        int32_t br = NONE;
         for (int r = tid; r < nr; r += PANEL_THREADS) {
-            int64_t pr = lr0 + r;
-            if (pr >= j) {
-                T v = cabs1(rows[r * nb + j]);
-                if (v > bv) {
-                    bv = v;
-                    br = (int32_t)pr;
-                }
+            const T v = cabs1(rows[r * nb]);
+            if (v > bv) {
+                bv = v;
+                br = (int32_t)(lr0 + r);
             }
         }
         block_argmax(bv, br, 0, red_v, red_r, red_q);
-        const int32_t lbest = red_r[0];
-        // --- publish: candidate meta / value / row, and row j if owned
+        const int32_t lb = red_r[0];
+        const uint32_t e0 = (uint32_t)(P.k + 1);
         if (tid == 0) {
-            tag_put(P.meta + par * G + g, epoch, (uint32_t)lbest);
-            put_val(P.cval + ((int64_t)par * G + g) * VW, epoch, red_v[0]);
+            tag_put(P.meta + g, e0, (uint32_t)lb);
+            put_val(P.cval + (int64_t)g * VW, e0, red_v[0]);
         }
-        uint64_t* cw = P.cand + ((int64_t)par * G + g) * row_words;
-        if (lbest != 0x7fffffff)
-            for (int c = tid; c < nb; c += PANEL_THREADS) {
-                cpx<T> x = rows[(lbest - lr0) * nb + c];
-                put_val(cw + (2 * c) * VW, epoch, x.re);
-                put_val(cw + (2 * c + 1) * VW, epoch, x.im);
-            }
-        else
-            for (int c = tid; c < nb; c += PANEL_THREADS) {
-                put_val(cw + (2 * c) * VW, epoch, (T)0);
-                put_val(cw + (2 * c + 1) * VW, epoch, (T)0);
-            }
-        if (j >= lr0 && j < lr0 + nr) {
-            uint64_t* dw = P.diag + (int64_t)par * row_words;
-            for (int c = tid; c < nb; c += PANEL_THREADS) {
-                cpx<T> x = rows[(j - lr0) * nb + c];
-                put_val(dw + (2 * c) * VW, epoch, x.re);
-                put_val(dw + (2 * c + 1) * VW, epoch, x.im);
-            }
-        }
-        // --- gather the G candidates (one slot per thread)
+        put_row<T>(P.cand + (int64_t)g * row_words, e0, lb != NONE ? rows + (lb - lr0) * nb : nullptr, nb);
+        if (owns(0)) put_row<T>(P.diag, e0, rows + (0 - lr0) * nb, nb);
+    }
+
+    for (int j = 0; j < nb; ++j) {
+        const int par = j & 1;
+        const uint32_t epoch = (uint32_t)(P.k + j + 1);
+        // --- (1) gather the G candidates of column j (one slot per thread)
         T v = (T)-1;
-        int32_t r = 0x7fffffff, q = 0;
+        int32_t r = NONE, q = 0;
         for (int c = tid; c < G; c += PANEL_THREADS) {
-            int32_t cr = (int32_t)tag_get(P.meta + par * G + c, epoch, P.err);
-            T cv;
-            get_val(P.cval + ((int64_t)par * G + c) * VW, epoch, P.err, cv);
+            uint64_t* pp[1 + VW];
+            pp[0] = P.meta + par * G + c;
+#pragma unroll
+            for (int w = 0; w < VW; ++w) pp[1 + w] = P.cval + ((int64_t)par * G + c) * VW + w;
+            uint32_t wd[1 + VW];
+            tag_get_n<1 + VW>(pp, epoch, P.err, wd);
+            const int32_t cr = (int32_t)wd[0];
+            const T cv = val_of<T>(wd + 1);
             if (cv > v || (cv == v && cr < r)) {
                 v = cv;
                 r = cr;
@@ -794,17 +874,21 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         block_argmax(v, r, q, red_v, red_r, red_q);
         const int wg = red_q[0];
         const int32_t p = red_r[0];
+        // --- (2) the winner's row and row j
         {
             uint64_t* pw = P.cand + ((int64_t)par * G + wg) * row_words;
             uint64_t* dw = P.diag + (int64_t)par * row_words;
             for (int c = tid; c < nb; c += PANEL_THREADS) {
-                cpx<T> x, y;
-                get_val(pw + (2 * c) * VW, epoch, P.err, x.re);
-                get_val(pw + (2 * c + 1) * VW, epoch, P.err, x.im);
-                get_val(dw + (2 * c) * VW, epoch, P.err, y.re);
-                get_val(dw + (2 * c + 1) * VW, epoch, P.err, y.im);
-                s_prow[c] = x;
-                s_drow[c] = y;
+                uint64_t* pp[4 * VW];
+#pragma unroll
+                for (int w = 0; w < 2 * VW; ++w) {
+                    pp[w] = pw + 2 * c * VW + w;
+                    pp[2 * VW + w] = dw + 2 * c * VW + w;
+                }
+                uint32_t wd[4 * VW];
+                tag_get_n<4 * VW>(pp, epoch, P.err, wd);
+                s_prow[c] = cpx<T>{val_of<T>(wd), val_of<T>(wd + VW)};
+                s_drow[c] = cpx<T>{val_of<T>(wd + 2 * VW), val_of<T>(wd + 3 * VW)};
             }
         }
         __syncthreads();
@@ -825,32 +909,87 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         }
         // swap rows j and p (panel-local indices)
         if (p != j) {
-            if (p >= lr0 && p < lr0 + nr)
+            if (owns(p))
                 for (int c = tid; c < nb; c += PANEL_THREADS) rows[(p - lr0) * nb + c] = s_drow[c];
-            if (j >= lr0 && j < lr0 + nr)
+            if (owns(j))
                 for (int c = tid; c < nb; c += PANEL_THREADS) rows[(j - lr0) * nb + c] = s_prow[c];
         }
         __syncthreads();
-        // scale column j below the diagonal, then rank-1 update of columns > j
         const cpx<T> inv = s_inv;
+        const int j1 = j + 1;
+        // --- (3) scale column j below the diagonal; update column j + 1 and
+        // take its local arg-max
+        T bv = (T)-1;
+        int32_t br = NONE;
+        const cpx<T> u1 = j1 < nb ? s_prow[j1] : cpx<T>{(T)0, (T)0};
         for (int rr = tid; rr < nr; rr += PANEL_THREADS) {
-            if (lr0 + rr > j) {
-                cpx<T> a = rows[rr * nb + j];
+            const int64_t pr = lr0 + rr;
+            if (pr > j) {
+                const cpx<T> a = rows[rr * nb + j];
                 cpx<T> l;
                 l.re = a.re * inv.re - a.im * inv.im;
                 l.im = a.re * inv.im + a.im * inv.re;
                 rows[rr * nb + j] = l;
+                if (j1 < nb) {
+                    cpx<T> b = rows[rr * nb + j1];
+                    b.re -= l.re * u1.re - l.im * u1.im;
+                    b.im -= l.re * u1.im + l.im * u1.re;
+                    rows[rr * nb + j1] = b;
+                    const T av = cabs1(b);
+                    if (av > bv) {
+                        bv = av;
+                        br = (int32_t)pr;
+                    }
+                }
             }
         }
-        __syncthreads();
+        if (j1 >= nb) break;
+        block_argmax(bv, br, 0, red_v, red_r, red_q);  // also orders (3) before (4)
+        const int32_t lb = red_r[0];
+        const uint32_t e1 = epoch + 1;
+        const int par1 = j1 & 1;
+        if (tid == 0) {
+            tag_put(P.meta + par1 * G + g, e1, (uint32_t)lb);
+            put_val(P.cval + ((int64_t)par1 * G + g) * VW, e1, red_v[0]);
+        }
+        // --- (4) complete and publish the two rows column j + 1 may read:
+        // the local candidate lb and row j + 1 (thread c owns column c)
+        const bool own1 = owns(j1);
+        for (int c = tid; c < nb; c += PANEL_THREADS) {
+            const cpx<T> u = s_prow[c];
+            auto fin = [&](int64_t pr) {
+                cpx<T>* row = rows + (pr - lr0) * nb;
+                if (c > j1) {
+                    const cpx<T> l = row[j];
+                    cpx<T> a = row[c];
+                    a.re -= l.re * u.re - l.im * u.im;
+                    a.im -= l.re * u.im + l.im * u.re;
+                    row[c] = a;
+                }
+                return row[c];
+            };
+            cpx<T> xc = cpx<T>{(T)0, (T)0};
+            if (lb != NONE) xc = fin(lb);
+            uint64_t* cw = P.cand + ((int64_t)par1 * G + g) * row_words;
+            put_val(cw + (2 * c) * VW, e1, xc.re);
+            put_val(cw + (2 * c + 1) * VW, e1, xc.im);
+            if (own1) {
+                const cpx<T> xd = (lb == j1) ? xc : fin(j1);
+                uint64_t* dw = P.diag + (int64_t)par1 * row_words;
+                put_val(dw + (2 * c) * VW, e1, xd.re);
+                put_val(dw + (2 * c + 1) * VW, e1, xd.im);
+            }
+        }
+        // --- (5) rank-1 update of columns > j + 1 for the other rows
         {
             const int c = tid % nb;
             const int rstep = PANEL_THREADS / nb;
-            if (c > j && tid < rstep * nb) {
+            if (c > j1 && tid < rstep * nb) {
                 const cpx<T> u = s_prow[c];
                 for (int rr = tid / nb; rr < nr; rr += rstep) {
-                    if (lr0 + rr > j) {
-                        cpx<T> l = rows[rr * nb + j];
+                    const int64_t pr = lr0 + rr;
+                    if (pr > j && pr != lb && pr != j1) {
+                        const cpx<T> l = rows[rr * nb + j];
                         cpx<T> a = rows[rr * nb + c];
                         a.re -= l.re * u.re - l.im * u.im;
                         a.im -= l.re * u.im + l.im * u.re;
@@ -861,12 +1000,12 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         }
         __syncthreads();
     }
+    __syncthreads();
     for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
         int r = idx / nb, c = idx % nb;
         P.A[(P.k + lr0 + r) * P.lda + P.k + c] = rows[idx];
     }
 }
-
 template <class T>
 __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
 {
@@ -1154,6 +1293,8 @@ struct LuWs {
     int32_t* slot_row;
     int32_t* plan;
     int32_t* plan_n;
+    void* split;        // operands of the split-FP16 trailing update (complex64 only)
+    size_t split_bytes;
     size_t total;
     size_t msg_words;  // words from meta to the end of diag (zeroed per factorisation)
 };
@@ -1183,6 +1324,8 @@ LuWs lu_ws_layout(int prec, int64_t n, char* base)
     w.slot_row = (int32_t*)take(2 * nbmax * 4);
     w.plan = (int32_t*)take(2 * nbmax * 4);
     w.plan_n = (int32_t*)take(256);
+    w.split_bytes = prec == MBA_F32 ? mba_h3::workspace_bytes(n, n, mba_h3::KMAX) : 0;
+    w.split = w.split_bytes ? (void*)take(w.split_bytes) : nullptr;
     w.total = off;
     (void)n;
     return w;
@@ -1201,7 +1344,7 @@ int device_cus()
 struct PhaseTimer {
     bool on = false;
     hipStream_t st = nullptr;
-    std::vector<hipEvent_t> ev[4];
+    std::vector<hipEvent_t> ev[5];
     void mark(int phase)
     {
         if (!on) return;
@@ -1211,10 +1354,11 @@ struct PhaseTimer {
             ev[phase].push_back(e);
         }
     }
-    // phases: 0 panel, 1 row swaps, 2 triangular solve, 3 trailing GEMM
+    // phases: 0 panel, 1 row swaps, 2 triangular solve, 3 FP32 / FP64 MFMA
+    // products, 4 split-FP16 trailing updates (split kernels included)
     void collect(double* ms, int64_t* launches)
     {
-        for (int p = 0; p < 4; ++p) {
+        for (int p = 0; p < 5; ++p) {
             double tot = 0.0;
             for (size_t i = 0; i + 1 < ev[p].size(); i += 2) {
                 float t = 0.f;
@@ -1237,7 +1381,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     PhaseTimer tm;
     tm.on = phase_ms != nullptr;
     tm.st = st;
-    double gflops = 0.0;
+    double gflops = 0.0, gflops_split = 0.0;
     const int prec = sizeof(T) == 4 ? MBA_F32 : MBA_F64;
     LuWs ws = lu_ws_layout(prec, n, (char*)ws_ptr);
     if (ws_bytes < ws.total) return MBA_ERR_ALLOC;
@@ -1336,31 +1480,45 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         tm.mark(2);
         return MBA_OK;
     };
-    // Super panels of 2 NB columns: panel 1, its U rows over the whole trailing
-    // width, an update of panel 2's columns only, panel 2 (its interchanges
-    // applied everywhere, including panel 1's L columns), the delayed update of
-    // panel 2's U rows, their triangular solve, then one trailing GEMM with
-    // k = 2 NB.  Row interchanges commute with the delayed (linear) update
-    // because they are applied to every column before it.
-    for (int64_t k = 0; k < n; k += 2 * NB) {
-        const int nb1 = (int)((n - k) < NB ? (n - k) : NB);
-        const int64_t k2 = k + nb1;
-        const int nb2 = (int)((n - k2) < NB ? ((n - k2) > 0 ? (n - k2) : 0) : NB);
-        int rc = do_panel(k, nb1, n);
-        if (rc != MBA_OK) return rc;
-        if (k2 >= n) break;
-        if ((rc = do_trsm(k, nb1, k2)) != MBA_OK) return rc;
-        // panel 2 columns, rows below panel 1: -= L21a U12a
-        if ((rc = do_gemm_rect(k2, n - k2, k2, nb2, k, nb1)) != MBA_OK) return rc;
-        rc = do_panel(k2, nb2, n);
-        if (rc != MBA_OK) return rc;
-        const int64_t c3 = k2 + nb2;
+    // Super panels of P panels of NB columns (left-looking inside, right-looking
+    // across super panels).  Panel p: its columns are first updated by the
+    // super panel's earlier panels (rows [kp, n)), then factored (its row
+    // interchanges applied to every column outside it, including the earlier
+    // panels' L columns), then its U rows over all columns to its right get
+    // the delayed update by the earlier panels and the triangular solve.
+    // One trailing update with k = P NB follows.  Row interchanges commute with
+    // the delayed (linear) updates because they are applied to every column
+    // before them.  P = 4 with the split-FP16 trailing product (k = 512 halves
+    // its C traffic per flop), 2 otherwise.
+    const int P = (sizeof(T) == 4 && g_lu_update_f32 == 1) ? 4 : 2;
+    for (int64_t k = 0; k < n; k += (int64_t)P * NB) {
+        int64_t kp = k;
+        int rc;
+        for (int p = 0; p < P && kp < n; ++p) {
+            const int nbp = (int)((n - kp) < NB ? (n - kp) : NB);
+            const int done = (int)(kp - k);  // columns of the earlier panels
+            if (p > 0 && (rc = do_gemm_rect(kp, n - kp, kp, nbp, k, done)) != MBA_OK) return rc;
+            if ((rc = do_panel(kp, nbp, n)) != MBA_OK) return rc;
+            const int64_t right = kp + nbp;
+            kp = right;
+            if (right >= n) break;
+            if (p > 0 && (rc = do_gemm_rect(right - nbp, nbp, right, n - right, k, done)) != MBA_OK) return rc;
+            if ((rc = do_trsm(right - nbp, nbp, right)) != MBA_OK) return rc;
+        }
+        const int64_t c3 = kp;  // first column after the super panel
         if (c3 >= n) break;
-        // panel 2's U rows: delayed update by panel 1, then the triangular solve
-        if ((rc = do_gemm_rect(k2, nb2, c3, n - c3, k, nb1)) != MBA_OK) return rc;
-        if ((rc = do_trsm(k2, nb2, c3)) != MBA_OK) return rc;
-        // trailing update with both panels: k = nb1 + nb2
-        if ((rc = do_gemm_rect(c3, n - c3, c3, n - c3, k, nb1 + nb2)) != MBA_OK) return rc;
+        const int64_t k3 = c3 - k;
+        if (sizeof(T) == 4 && g_lu_update_f32 == 1 && n - c3 >= SPLIT_MIN_DIM) {
+            const int64_t m3 = n - c3;
+            tm.mark(4);
+            rc = mba_h3::gemm(m3, m3, k3, -1.f, A + c3 * lda + k, lda, A + k * lda + c3, lda, 1.f,
+                              A + c3 * lda + c3, lda, ws.split, ws.split_bytes, st);
+            if (rc != MBA_OK) return rc;
+            tm.mark(4);
+            gflops_split += 8.0 * (double)m3 * (double)m3 * (double)k3;
+        } else if ((rc = do_gemm_rect(c3, n - c3, c3, n - c3, k, (int)k3)) != MBA_OK) {
+            return rc;
+        }
     }
     if (err_out) {
         // asynchronous form: hand the wait flag to the caller, no host synchronisation
@@ -1372,7 +1530,10 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     if (hipMemcpyAsync(&herr, ws.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) return MBA_ERR_LAUNCH;
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
     tm.collect(phase_ms, phase_launches);
-    if (gemm_flops) *gemm_flops = gflops;
+    if (gemm_flops) {
+        gemm_flops[0] = gflops;
+        gemm_flops[1] = gflops_split;
+    }
     return herr ? MBA_ERR_TIMEOUT : MBA_OK;
 }
 
@@ -2938,6 +3099,30 @@ int mba_gemm_set_variant(int prec, int variant)
     if (variant < 0 || variant > 5) return MBA_ERR_ARG;
     g_gemm_f32_variant = variant;
     return MBA_OK;
+}
+
+int mba_lu_set_update(int kind)
+{
+    if (kind != 0 && kind != 1) return MBA_ERR_ARG;
+    g_lu_update_f32 = kind;
+    return MBA_OK;
+}
+
+size_t mba_gemm_split_f16_workspace_bytes(int64_t m, int64_t n, int64_t k)
+{
+    if (m <= 0 || n <= 0 || k <= 0 || k > mba_h3::KMAX) return 0;
+    return mba_h3::workspace_bytes(m, n, k);
+}
+
+int mba_gemm_split_f16(int64_t m, int64_t n, int64_t k, double alpha, const void* a, int64_t lda, const void* b,
+                       int64_t ldb, double beta, void* c, int64_t ldc, void* workspace, size_t workspace_bytes,
+                       mba_stream_t stream)
+{
+    if (!a || !b || !c || !workspace) return MBA_ERR_ARG;
+    if (m < 0 || n < 0 || k < 1 || k > mba_h3::KMAX) return MBA_ERR_SIZE;
+    if (lda < k || ldb < n || ldc < n) return MBA_ERR_ARG;
+    return mba_h3::gemm(m, n, k, (float)alpha, a, lda, b, ldb, (float)beta, c, ldc, workspace, workspace_bytes,
+                        S(stream));
 }
 
 int mba_gemm_variant(int prec, int variant, int64_t m, int64_t n, int64_t k, double alpha,

@@ -1,0 +1,339 @@
+// Complex64 GEMM with FP32 accuracy on the FP16 matrix cores of gfx950.
+//
+// On CDNA4 the dense FP16 MFMA rate is 16x the FP32 MFMA rate, so an FP32
+// product can be carried by FP16 pieces:
+//   every value x (already scaled by a power of two) is split into
+//   hi = fp16(x) and lo = fp16(x - hi)   (x - hi is exact in FP32),
+//   x*y ~ hi_x hi_y + hi_x lo_y + lo_x hi_y      (lo_x lo_y ~ 2^-22 x y dropped)
+// with FP32 accumulation in the MFMA.  hi + lo holds x to 2^-22 relative.
+// Rows of A and columns of B are scaled by powers of two so their largest
+// component lies in [2^14, 2^15): FP16 keeps every value down to 2^-29 of the
+// row (column) maximum in its normal range, products stay below 2^30, and the
+// scale 2^(ea_i + eb_j) is undone exactly in the epilogue.
+//
+// The complex product uses the real embedding
+//   Re C = Ar Br - Ai Bi,  Im C = Ar Bi + Ai Br
+// (4 real products x 3 piece products = 12 MFMAs per 32 x 32 block per 16 k).
+//
+// Operand format (written by the split kernels): per row of A (per column of
+// B) and per chunk of 16 k, one 128-byte record of four planes of 16 halves:
+// hi_re, hi_im, lo_re, lo_im.  A: [Mpad][Kpad / 16] records; B (transposed):
+// [Npad][Kpad / 16] records; pad rows / columns / k are zero.
+#include <math.h>
+#include <stdint.h>
+
+#include "gemm_h3.hpp"
+#include "mba_common.hpp"
+
+namespace mba_h3 {
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int KC = 16;                  // k per chunk (one v_mfma_f32_32x32x16_f16)
+constexpr int REC_H = 4 * KC;           // halves per record
+constexpr int BM = 256, BN = 128;       // workgroup tile (complex)
+constexpr int THREADS = 512;            // 8 waves: 4 (rows) x 2 (columns), 64 x 64 each
+constexpr int PITCH_H = REC_H + 8;      // LDS row pitch in halves (144 B: conflict-free 16-byte reads)
+constexpr int NUM_XCD = 8, GROUP_M = 8;
+constexpr int SCALE_TOP = 15;           // scaled maxima in [2^14, 2^15)
+
+// exponent e with max = m 2^e, m in [0.5, 1); 0 for max == 0
+__device__ __forceinline__ int scale_exp(float mx)
+{
+    if (!(mx > 0.f)) return 0;
+    int e;
+    (void)frexpf(mx, &e);
+    return e - SCALE_TOP;  // value = scaled * 2^ea, scaled = value * 2^-ea
+}
+
+__device__ __forceinline__ void split2(float x, _Float16& h, _Float16& l)
+{
+    h = (_Float16)x;
+    l = (_Float16)(x - (float)h);
+}
+
+// ---- A: one wave per row (K <= KMAX = 512: lane holds 4 consecutive k of
+// each 256-wide block)
+__global__ __launch_bounds__(256) void k_split_rows(const cpx<float>* __restrict__ A, int64_t lda, int64_t M,
+                                                    int K, int Kpad, int64_t Mpad, _Float16* __restrict__ Ap,
+                                                    int32_t* __restrict__ ea)
+{
+    constexpr int NBLK = KMAX / 256;
+    typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= Mpad) return;
+    const int nrec = Kpad / KC;
+    float mx = 0.f;
+    cpx<float> v[NBLK][4];
+#pragma unroll
+    for (int b = 0; b < NBLK; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = 256 * b + 4 * lane + q;
+            v[b][q] = (row < M && k < K) ? A[row * lda + k] : cpx<float>{0.f, 0.f};
+            mx = fmaxf(mx, fmaxf(fabsf(v[b][q].re), fabsf(v[b][q].im)));
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    const int e = scale_exp(mx);
+    if (lane == 0) ea[row] = e;
+#pragma unroll
+    for (int b = 0; b < NBLK; ++b) {
+        const int kc = 16 * b + lane / 4;
+        if (kc >= nrec) break;
+        _Float16 h[2][4], l[2][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            split2(ldexpf(v[b][q].re, -e), h[0][q], l[0][q]);
+            split2(ldexpf(v[b][q].im, -e), h[1][q], l[1][q]);
+        }
+        _Float16* rec = Ap + (row * nrec + kc) * REC_H + (lane & 3) * 4;
+        *(half4*)(rec + 0 * KC) = half4{h[0][0], h[0][1], h[0][2], h[0][3]};
+        *(half4*)(rec + 1 * KC) = half4{h[1][0], h[1][1], h[1][2], h[1][3]};
+        *(half4*)(rec + 2 * KC) = half4{l[0][0], l[0][1], l[0][2], l[0][3]};
+        *(half4*)(rec + 3 * KC) = half4{l[1][0], l[1][1], l[1][2], l[1][3]};
+    }
+}
+
+// ---- B (K x N, row-major) -> transposed records.  Workgroup: 64 columns x 4
+// groups; group g takes chunks g, g + 4, ...; thread = (column, group).
+__global__ __launch_bounds__(256) void k_split_cols(const cpx<float>* __restrict__ B, int64_t ldb, int K, int Kpad,
+                                                    int64_t N, int64_t Npad, _Float16* __restrict__ Bp,
+                                                    int32_t* __restrict__ eb)
+{
+    __shared__ float red[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t col = (int64_t)blockIdx.x * 64 + c;
+    const int nrec = Kpad / KC;
+    const bool cok = col < N;
+    float mx = 0.f;
+    for (int kc = g; kc < nrec; kc += 4)
+        for (int q = 0; q < KC; ++q) {
+            const int k = kc * KC + q;
+            if (cok && k < K) {
+                const cpx<float> x = B[(int64_t)k * ldb + col];
+                mx = fmaxf(mx, fmaxf(fabsf(x.re), fabsf(x.im)));
+            }
+        }
+    red[g][c] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0][c], red[1][c]), fmaxf(red[2][c], red[3][c]));
+    const int e = scale_exp(mx);
+    if (col >= Npad) return;
+    if (g == 0) eb[col] = e;
+    for (int kc = g; kc < nrec; kc += 4) {
+        _Float16 h[2][KC], l[2][KC];
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+            const int k = kc * KC + q;
+            const cpx<float> x = (cok && k < K) ? B[(int64_t)k * ldb + col] : cpx<float>{0.f, 0.f};
+            split2(ldexpf(x.re, -e), h[0][q], l[0][q]);
+            split2(ldexpf(x.im, -e), h[1][q], l[1][q]);
+        }
+        half8* rec = (half8*)(Bp + (col * nrec + kc) * REC_H);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            rec[2 * p + 0] = half8{h[p][0], h[p][1], h[p][2], h[p][3], h[p][4], h[p][5], h[p][6], h[p][7]};
+            rec[2 * p + 1] = half8{h[p][8], h[p][9], h[p][10], h[p][11], h[p][12], h[p][13], h[p][14], h[p][15]};
+            rec[4 + 2 * p + 0] = half8{l[p][0], l[p][1], l[p][2], l[p][3], l[p][4], l[p][5], l[p][6], l[p][7]};
+            rec[4 + 2 * p + 1] = half8{l[p][8], l[p][9], l[p][10], l[p][11], l[p][12], l[p][13], l[p][14], l[p][15]};
+        }
+    }
+}
+
+// tile L -> (row tile, column tile): each XCD (workgroup L runs on XCD L % 8)
+// takes a contiguous range of tiles, walked in groups of GROUP_M row tiles so
+// the group's A records stay in the XCD's L2 while B records stream.
+__device__ __forceinline__ bool tile_of(int64_t L, int64_t nbm, int64_t nbn, int64_t& tm, int64_t& tn)
+{
+    const int64_t T = nbm * nbn;
+    const int64_t per = (T + NUM_XCD - 1) / NUM_XCD;
+    const int64_t v = (L % NUM_XCD) * per + L / NUM_XCD;
+    if (v >= T) return false;
+    const int64_t group = v / (GROUP_M * nbn);
+    const int64_t first_m = group * GROUP_M;
+    const int64_t gm = (nbm - first_m) < GROUP_M ? (nbm - first_m) : GROUP_M;
+    const int64_t in = v - group * GROUP_M * nbn;
+    tm = first_m + in % gm;
+    tn = in / gm;
+    return true;
+}
+
+// C[M x N] = alpha * 2^(ea_i + eb_j) * (A' B')_ij + beta * C   (complex64, row-major)
+__global__ __launch_bounds__(THREADS) void k_cgemm_h3(int64_t M, int64_t N, int nrec,
+                                                      const _Float16* __restrict__ Ap,
+                                                      const int32_t* __restrict__ ea,
+                                                      const _Float16* __restrict__ Bp,
+                                                      const int32_t* __restrict__ eb, float alpha, float beta,
+                                                      cpx<float>* __restrict__ C, int64_t ldc)
+{
+    __shared__ alignas(16) _Float16 As[2][BM * PITCH_H];
+    __shared__ alignas(16) _Float16 Bs[2][BN * PITCH_H];
+    const int64_t nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+    int64_t tm, tn;
+    if (!tile_of(blockIdx.x, nbm, nbn, tm, tn)) return;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int l32 = lane & 31, oct = lane >> 5;
+
+    // global -> register staging: A 256 records x 8 parts of 16 B (4 per
+    // thread), B 128 records x 8 parts (2 per thread); part = tid % 8
+    const int part = tid & 7, rsub = tid >> 3;  // rsub in [0, 64)
+    half8 ga[4], gb[2];
+    auto gload = [&](int rc) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t row = m0 + rsub + 64 * q;
+            ga[q] = *(const half8*)(Ap + (row * nrec + rc) * REC_H + part * 8);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int64_t col = n0 + rsub + 64 * q;
+            gb[q] = *(const half8*)(Bp + (col * nrec + rc) * REC_H + part * 8);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) *(half8*)&As[buf][(rsub + 64 * q) * PITCH_H + part * 8] = ga[q];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) *(half8*)&Bs[buf][(rsub + 64 * q) * PITCH_H + part * 8] = gb[q];
+    };
+
+    floatx16 acc[2][2][2];  // [i][j][re/im]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][0][r] = acc[i][j][1][r] = 0.f;
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int rc = 0; rc < nrec; ++rc) {
+        const int buf = rc & 1;
+        if (rc + 1 < nrec) gload(rc + 1);
+        // A operands of this wave: rows wm*64 + i*32 + l32, planes p, k-octet oct
+        half8 a[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                a[i][p] = *(const half8*)&As[buf][(wm * 64 + i * 32 + l32) * PITCH_H + p * KC + oct * 8];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            half8 b[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                b[p] = *(const half8*)&Bs[buf][(wn * 64 + j * 32 + l32) * PITCH_H + p * KC + oct * 8];
+            // -Bi (hi, lo): sign flip of the packed halves
+            half8 nbh = -b[1], nbl = -b[3];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                floatx16 re = acc[i][j][0], im = acc[i][j][1];
+                // hi x hi
+                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[0], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[1], im, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], nbh, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], b[0], im, 0, 0, 0);
+                // hi x lo
+                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[2], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[3], im, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], nbl, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], b[2], im, 0, 0, 0);
+                // lo x hi
+                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][2], b[0], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][2], b[1], im, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][3], nbh, re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][3], b[0], im, 0, 0, 0);
+                acc[i][j][0] = re;
+                acc[i][j][1] = im;
+            }
+        }
+        if (rc + 1 < nrec) sstore(buf ^ 1);
+        __syncthreads();
+    }
+    // epilogue: register r of lane -> row 8 (r / 4) + 4 oct + r % 4, column l32.
+    // Per 32 x 32 block all C loads are issued before any store (C is read
+    // and written through one pointer, so the compiler would otherwise wait
+    // for each load in turn).
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int64_t gj = n0 + wn * 64 + j * 32 + l32;
+        const bool jok = gj < N;
+        const int ej = jok ? eb[gj] : 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t gi0 = m0 + wm * 64 + i * 32 + 4 * oct;
+            float2 cv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = gi0 + 8 * (r >> 2) + (r & 3);
+                cv[r] = (beta != 0.f && jok && gi < M) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = gi0 + 8 * (r >> 2) + (r & 3);
+                if (jok && gi < M) {
+                    // scale the product, not alpha: 2^(ea + eb) alone may leave FP32's range
+                    const int e = ea[gi] + ej;
+                    float re = ldexpf(alpha * acc[i][j][0][r], e), im = ldexpf(alpha * acc[i][j][1][r], e);
+                    if (beta != 0.f) {
+                        re += beta * cv[r].x;
+                        im += beta * cv[r].y;
+                    }
+                    *(float2*)(C + gi * ldc + gj) = make_float2(re, im);
+                }
+            }
+        }
+    }
+}
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t workspace_bytes(int64_t M, int64_t N, int64_t K)
+{
+    const int64_t Kpad = (K + KC - 1) / KC * KC;
+    const int64_t Mpad = (M + BM - 1) / BM * BM, Npad = (N + BN - 1) / BN * BN;
+    return al256((size_t)Mpad * Kpad * 4 * 2) + al256((size_t)Npad * Kpad * 4 * 2) + al256((size_t)Mpad * 4) +
+           al256((size_t)Npad * 4);
+}
+
+int gemm(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+         float beta, void* C, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    if (M <= 0 || N <= 0) return MBA_OK;
+    if (K <= 0 || K > KMAX) return MBA_ERR_SIZE;
+    if (ws_bytes < workspace_bytes(M, N, K)) return MBA_ERR_ALLOC;
+    const int Kpad = (int)((K + KC - 1) / KC * KC);
+    const int nrec = Kpad / KC;
+    const int64_t Mpad = (M + BM - 1) / BM * BM, Npad = (N + BN - 1) / BN * BN;
+    char* p = (char*)ws;
+    _Float16* Ap = (_Float16*)p;
+    p += al256((size_t)Mpad * Kpad * 4 * 2);
+    _Float16* Bp = (_Float16*)p;
+    p += al256((size_t)Npad * Kpad * 4 * 2);
+    int32_t* ea = (int32_t*)p;
+    p += al256((size_t)Mpad * 4);
+    int32_t* eb = (int32_t*)p;
+    hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((Mpad + 3) / 4)), dim3(256), 0, st, (const cpx<float>*)A, lda,
+                       M, (int)K, Kpad, Mpad, Ap, ea);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(Npad / 64)), dim3(256), 0, st, (const cpx<float>*)B, ldb,
+                       (int)K, Kpad, N, Npad, Bp, eb);
+    MBA_LAUNCH_CHECK();
+    const int64_t tiles = (Mpad / BM) * (Npad / BN);
+    const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    hipLaunchKernelGGL(k_cgemm_h3, dim3((unsigned)blocks), dim3(THREADS), 0, st, M, N, nrec, Ap, ea, Bp, eb, alpha,
+                       beta, (cpx<float>*)C, ldc);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+}  // namespace mba_h3

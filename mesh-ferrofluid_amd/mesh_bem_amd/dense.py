@@ -34,6 +34,41 @@ def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alp
     return C
 
 
+def gemm_split_f16(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha: float = 1.0,
+                   beta: float = 0.0) -> torch.Tensor:
+    """C = alpha A B + beta C for row-major complex64 tensors (K <= 512) on the
+    FP16 matrix cores with FP32 accuracy (power-of-two scaled hi/lo FP16
+    pieces, three piece products, FP32 accumulation)."""
+    lib = _lib.load()
+    M, K = A.shape
+    K2, N = B.shape
+    if K != K2 or A.dtype != torch.complex64 or B.dtype != torch.complex64:
+        raise ValueError("gemm_split_f16: complex64 A [M, K], B [K, N]")
+    if C is None:
+        C = torch.zeros((M, N), dtype=torch.complex64, device=A.device)
+        beta = 0.0
+    for t in (A, B, C):
+        if t.stride(1) != 1:
+            raise ValueError("gemm_split_f16: row-major operands with unit column stride")
+    wsb = lib.mba_gemm_split_f16_workspace_bytes(M, N, K)
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=A.device)
+    st = lib.mba_gemm_split_f16(M, N, K, float(alpha), _lib.ptr(A), A.stride(0), _lib.ptr(B), B.stride(0),
+                                float(beta), _lib.ptr(C), C.stride(0), _lib.ptr(ws), wsb, _lib.stream())
+    _lib.check(st, "mba_gemm_split_f16")
+    return C
+
+
+LU_UPDATES = {"fp32": 0, "split_f16": 1}
+
+
+def set_lu_update(kind: str) -> None:
+    """Trailing-update product of complex64 factorisations (process-wide):
+    "fp32" (FP32 MFMA) or "split_f16" (FP32-accurate split-FP16 MFMA)."""
+    if kind not in LU_UPDATES:
+        raise ValueError(f"set_lu_update: one of {sorted(LU_UPDATES)}")
+    _lib.check(_lib.load().mba_lu_set_update(LU_UPDATES[kind]), "mba_lu_set_update")
+
+
 class LUFactor:
     """In-place LU with partial pivoting of a square complex matrix."""
 
@@ -63,17 +98,19 @@ class LUFactor:
         if profile:
             import ctypes
 
-            ms = (ctypes.c_double * 4)()
-            nl = (ctypes.c_int64 * 4)()
-            fl = ctypes.c_double()
+            ms = (ctypes.c_double * 5)()
+            nl = (ctypes.c_int64 * 5)()
+            fl = (ctypes.c_double * 2)()
             st = lib.mba_lu_factor_profiled(self.prec, _lib.ptr(self.lu), n, self.lu.stride(0),
                                             _lib.ptr(self.ipiv), _lib.ptr(info), _lib.ptr(ws), wsb,
-                                            ms, nl, ctypes.byref(fl), _lib.stream())
+                                            ms, nl, fl, _lib.stream())
             _lib.check(st, "mba_lu_factor_profiled")
+            phases = ("panel", "laswp", "trsm", "gemm", "split")
             self.profile = {
-                "phase_ms": dict(zip(("panel", "laswp", "trsm", "gemm"), list(ms))),
-                "phase_launches": dict(zip(("panel", "laswp", "trsm", "gemm"), list(nl))),
-                "gemm_flops": fl.value,
+                "phase_ms": dict(zip(phases, list(ms))),
+                "phase_launches": dict(zip(phases, list(nl))),
+                "gemm_flops": fl[0],
+                "split_flops": fl[1],
             }
         elif asynchronous:
             self._err = torch.zeros(1, dtype=torch.int32, device=dev)
