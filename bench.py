@@ -715,18 +715,19 @@ def main():
             out["roofline"] = {
                 "kernel": "k_cgemm_h3 (LU trailing update: FP32-accurate complex product on "
                           "v_mfma_f32_32x32x16_f16, hi/lo split operands, 12 MFMAs per 32x32 block per 16 k; "
-                          "split kernels included; achieved counts 8mnk flops)",
+                          "achieved counts 8mnk flops)",
                 "bound": "mfma",
                 "achieved": round(achieved, 2),
                 "peak": round(peak, 1),
                 "peak_basis": "FP16 dense MFMA 2500 TFLOP/s / 3 piece products per real product",
                 "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4),
-                "traffic": None,
+                "traffic": pmc_traffic("traffic_c2_mixed_solve.md") if args.workload == "c2" else None,
+                "traffic_source": "profiles/traffic_c2_mixed_solve.md (FETCH_SIZE x2.0 + WRITE_SIZE, copy-calibrated)",
                 "launches": nl,
                 "avg_launch_ms": round(sp_ms / nl, 4),
                 "flops_per_launch_avg": prof["split_flops"] / nl,
-                "fp32_products_ms": round(prof["phase_ms"]["gemm"], 2),
+                "fp32_products_and_splits_ms": round(prof["phase_ms"]["gemm"], 2),
             }
         else:
             gemm_ms = prof["phase_ms"]["gemm"]
@@ -740,8 +741,7 @@ def main():
                 "peak": PEAK_FP32_MFMA_TFLOPS,
                 "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4),
-                "traffic": pmc_traffic("traffic_c2_mixed_solve.md") if args.workload == "c2" else None,
-                "traffic_source": "profiles/traffic_c2_mixed_solve.md (FETCH_SIZE x2.0 + WRITE_SIZE, copy-calibrated)",
+                "traffic": None,
                 "launches": nl,
                 "avg_launch_ms": round(gemm_ms / nl, 4),
                 "flops_per_launch_avg": prof["gemm_flops"] / nl,

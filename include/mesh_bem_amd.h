@@ -255,9 +255,11 @@ int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, vo
                           int32_t *err_out, mba_stream_t stream);
 /* mba_lu_factor with per-phase timing (HIP events on the stream; the call
  * synchronises): phase_ms[5] and phase_launches[5] for the panels, row
- * interchanges, triangular solves, FP32 / FP64 MFMA products and split-FP16
- * trailing updates; gemm_flops[2] = algorithmic flops (8 m n k per complex
- * product) of the last two phases.  No reference counterpart. */
+ * interchanges, triangular solves, FP32 / FP64 MFMA products (with the operand
+ * splits of the split-FP16 products) and the split-FP16 trailing-update
+ * products; gemm_flops[2] = algorithmic flops (8 m n k per complex product)
+ * of the FP32 / FP64 products and of the split-FP16 products.  No reference
+ * counterpart. */
 int mba_lu_factor_profiled(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv, int32_t *info,
                            void *workspace, size_t workspace_bytes, double *phase_ms,
                            int64_t *phase_launches, double *gemm_flops, mba_stream_t stream);

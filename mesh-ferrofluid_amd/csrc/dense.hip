@@ -674,7 +674,9 @@ __device__ __forceinline__ uint32_t tag_get(uint64_t* p, uint32_t epoch, int* er
     while (true) {
         uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if ((uint32_t)(w >> 32) == epoch) return (uint32_t)w;
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return (uint32_t)w;
+        // the error flag costs a memory round trip: look at it every 64 polls
+        if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return (uint32_t)w;
         if (++spins > SPIN_LIMIT) {
             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return (uint32_t)w;
@@ -750,7 +752,7 @@ __device__ __forceinline__ void tag_get_n(uint64_t* const (&p)[N], uint32_t epoc
 #pragma unroll
         for (int i = 0; i < N; ++i) ok = ok && (uint32_t)(w[i] >> 32) == epoch;
         if (ok) break;
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (++spins > SPIN_LIMIT) {
             __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
@@ -1130,68 +1132,95 @@ __global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t l
 // each block first subtracts the contribution of the solved blocks above it
 // (thread = (row, 4 columns)), then a 16 x 16 forward substitution runs with
 // one thread per column -- two barriers per block instead of two per row.
-constexpr int TRSM_COLS = 64;
+// Columns per workgroup of the panel triangular solve (FP32 / FP64 complex).
+template <class T>
+struct TrsmCols {
+    static constexpr int value = sizeof(T) == 4 ? 64 : 32;
+};
 constexpr int TRSM_RB = 16;
 
+// X = L11^{-1} X for the nb x (columns of this workgroup) block X of rows
+// [k, k + nb): X in LDS, L11 read in row blocks of TRSM_RB rows staged in LDS
+// (Ls [TRSM_RB][nb]).  Per block: X(block) -= L(block, done) X(done) by all
+// threads, then the forward substitution inside the block, one thread per
+// column.
 template <class T>
 __device__ void trsm_body(cpx<T>* __restrict__ A, int64_t lda, int64_t k, int nb, int64_t col_begin,
-                          int64_t col_end, int64_t blk, cpx<T>* X /* LDS [nb][TRSM_COLS] */)
+                          int64_t col_end, int64_t blk, cpx<T>* X /* LDS [nb][TC] */,
+                          cpx<T>* Ls /* LDS [TRSM_RB][nb] */)
 {
+    constexpr int TC = TrsmCols<T>::value;
+    constexpr int CPT = TC / 16;  // columns per thread in the update role
     const int tid = threadIdx.x;
-    const int64_t c0 = col_begin + blk * TRSM_COLS;
-    const int w = (int)((col_end - c0) < TRSM_COLS ? (col_end - c0) : TRSM_COLS);
-    for (int idx = tid; idx < nb * TRSM_COLS; idx += 256) {
-        int r = idx / TRSM_COLS, c = idx % TRSM_COLS;
+    const int64_t c0 = col_begin + blk * TC;
+    const int w = (int)((col_end - c0) < TC ? (col_end - c0) : TC);
+    for (int idx = tid; idx < nb * TC; idx += 256) {
+        int r = idx / TC, c = idx % TC;
         cpx<T> v;
         v.re = 0;
         v.im = 0;
         if (c < w) v = A[(k + r) * lda + c0 + c];
         X[idx] = v;
     }
-    __syncthreads();
-    const int rr = tid >> 4, cg = (tid & 15) * 4;  // update role: row rr of the block, columns cg..cg+3
+    const int rr = tid >> 4, cg = (tid & 15) * CPT;  // update role: row rr of the block, columns cg..
     for (int b0 = 0; b0 < nb; b0 += TRSM_RB) {
         const int rb = (nb - b0) < TRSM_RB ? (nb - b0) : TRSM_RB;
+        const int width = b0 + rb;  // L columns this block needs
+        __syncthreads();            // previous block done with Ls (and X loaded)
+        for (int idx = tid; idx < rb * width; idx += 256) {
+            const int i = idx / width, t = idx % width;
+            Ls[i * nb + t] = A[(k + b0 + i) * lda + k + t];
+        }
+        __syncthreads();
         if (b0 > 0 && rr < rb) {
-            const cpx<T>* Lr = A + (k + b0 + rr) * lda + k;
-            T sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+            const cpx<T>* Lr = Ls + rr * nb;
+            T sr[CPT], si[CPT];
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) sr[q] = si[q] = 0;
             for (int t = 0; t < b0; ++t) {
                 const cpx<T> l = Lr[t];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const cpx<T> x = X[t * TRSM_COLS + cg + q];
+                for (int q = 0; q < CPT; ++q) {
+                    const cpx<T> x = X[t * TC + cg + q];
                     sr[q] += l.re * x.re - l.im * x.im;
                     si[q] += l.re * x.im + l.im * x.re;
                 }
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                cpx<T> x = X[(b0 + rr) * TRSM_COLS + cg + q];
+            for (int q = 0; q < CPT; ++q) {
+                cpx<T> x = X[(b0 + rr) * TC + cg + q];
                 x.re -= sr[q];
                 x.im -= si[q];
-                X[(b0 + rr) * TRSM_COLS + cg + q] = x;
+                X[(b0 + rr) * TC + cg + q] = x;
             }
         }
         __syncthreads();
-        if (tid < TRSM_COLS) {
+        if (tid < TC) {
             const int c = tid;
             for (int i = 1; i < rb; ++i) {
-                const cpx<T>* Lr = A + (k + b0 + i) * lda + k + b0;
-                cpx<T> x = X[(b0 + i) * TRSM_COLS + c];
+                const cpx<T>* Lr = Ls + i * nb + b0;
+                cpx<T> x = X[(b0 + i) * TC + c];
                 for (int t = 0; t < i; ++t) {
-                    const cpx<T> l = Lr[t], y = X[(b0 + t) * TRSM_COLS + c];
+                    const cpx<T> l = Lr[t], y = X[(b0 + t) * TC + c];
                     x.re -= l.re * y.re - l.im * y.im;
                     x.im -= l.re * y.im + l.im * y.re;
                 }
-                X[(b0 + i) * TRSM_COLS + c] = x;
+                X[(b0 + i) * TC + c] = x;
             }
         }
-        __syncthreads();
     }
-    for (int idx = tid; idx < nb * TRSM_COLS; idx += 256) {
-        int r = idx / TRSM_COLS, cc = idx % TRSM_COLS;
+    __syncthreads();
+    for (int idx = tid; idx < nb * TC; idx += 256) {
+        int r = idx / TC, cc = idx % TC;
         if (cc < w) A[(k + r) * lda + c0 + cc] = X[idx];
     }
+}
+
+// dynamic LDS of the panel triangular solve: X [nb][TC] + Ls [TRSM_RB][nb]
+template <class T>
+constexpr size_t trsm_lds_bytes(int nb)
+{
+    return ((size_t)nb * TrsmCols<T>::value + (size_t)TRSM_RB * nb) * sizeof(cpx<T>);
 }
 
 template <class T>
@@ -1199,7 +1228,8 @@ __global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t 
                                                 int nb, int64_t col_begin, int64_t col_end)
 {
     extern __shared__ unsigned char smem_raw[];
-    trsm_body<T>(A, lda, k, nb, col_begin, col_end, (int64_t)blockIdx.x, (cpx<T>*)smem_raw);
+    cpx<T>* X = (cpx<T>*)smem_raw;
+    trsm_body<T>(A, lda, k, nb, col_begin, col_end, (int64_t)blockIdx.x, X, X + (size_t)nb * TrsmCols<T>::value);
 }
 
 // ---- the same three steps for many matrices in one launch each: job j owns
@@ -1262,7 +1292,9 @@ __global__ __launch_bounds__(256) void k_trsm_batched(const TrsmJob<T>* __restri
     extern __shared__ unsigned char smem_raw[];
     const int j = job_of(wg_off, njob, (int)blockIdx.x);
     const TrsmJob<T> J = jobs[j];
-    trsm_body<T>(J.A, J.lda, J.k, J.nb, J.c_lo, J.c_hi, (int64_t)blockIdx.x - wg_off[j], (cpx<T>*)smem_raw);
+    cpx<T>* X = (cpx<T>*)smem_raw;
+    trsm_body<T>(J.A, J.lda, J.k, J.nb, J.c_lo, J.c_hi, (int64_t)blockIdx.x - wg_off[j], X,
+                 X + (size_t)J.nb * TrsmCols<T>::value);
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n)
@@ -1355,7 +1387,8 @@ struct PhaseTimer {
         }
     }
     // phases: 0 panel, 1 row swaps, 2 triangular solve, 3 FP32 / FP64 MFMA
-    // products, 4 split-FP16 trailing updates (split kernels included)
+    // products and the operand splits of the split-FP16 products, 4 split-FP16
+    // trailing-update products (k_cgemm_h3 alone)
     void collect(double* ms, int64_t* launches)
     {
         for (int p = 0; p < 5; ++p) {
@@ -1401,7 +1434,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         (void)hipFuncSetAttribute((const void*)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_budget);
         (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(128 * TRSM_COLS * esz));
+                                  (int)trsm_lds_bytes<T>(128));
         attr_set[prec] = true;
     }
     // err := 0, every message word := epoch 0 (epochs of this factorisation are k + j + 1 >= 1)
@@ -1474,8 +1507,9 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     auto do_trsm = [&](int64_t r0, int nb, int64_t c_lo) -> int {
         if (c_lo >= n) return MBA_OK;
         tm.mark(2);
-        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n - c_lo + TRSM_COLS - 1) / TRSM_COLS)),
-                           dim3(256), (size_t)nb * TRSM_COLS * esz, st, A, lda, r0, nb, c_lo, n);
+        constexpr int TC = TrsmCols<T>::value;
+        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n - c_lo + TC - 1) / TC)), dim3(256),
+                           trsm_lds_bytes<T>(nb), st, A, lda, r0, nb, c_lo, n);
         MBA_LAUNCH_CHECK();
         tm.mark(2);
         return MBA_OK;
@@ -1510,9 +1544,12 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         const int64_t k3 = c3 - k;
         if (sizeof(T) == 4 && g_lu_update_f32 == 1 && n - c3 >= SPLIT_MIN_DIM) {
             const int64_t m3 = n - c3;
+            tm.mark(3);
+            rc = mba_h3::split(m3, m3, k3, A + c3 * lda + k, lda, A + k * lda + c3, lda, ws.split, ws.split_bytes, st);
+            if (rc != MBA_OK) return rc;
+            tm.mark(3);
             tm.mark(4);
-            rc = mba_h3::gemm(m3, m3, k3, -1.f, A + c3 * lda + k, lda, A + k * lda + c3, lda, 1.f,
-                              A + c3 * lda + c3, lda, ws.split, ws.split_bytes, st);
+            rc = mba_h3::multiply(m3, m3, k3, -1.f, 1.f, A + c3 * lda + c3, lda, ws.split, ws.split_bytes, st);
             if (rc != MBA_OK) return rc;
             tm.mark(4);
             gflops_split += 8.0 * (double)m3 * (double)m3 * (double)k3;
@@ -2547,7 +2584,7 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         (void)hipFuncSetAttribute((const void*)k_panel_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_budget);
         (void)hipFuncSetAttribute((const void*)k_trsm_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(NB * TRSM_COLS * esz));
+                                  (int)trsm_lds_bytes<T>(NB));
         (void)hipFuncSetAttribute((const void*)k_laswp_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(2 * NB * LASWP_COLS * esz));
         attr_set[prec] = true;
@@ -2656,7 +2693,7 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         J.c_hi = n;
         tjobs.push_back(J);
         r.count += 1;
-        r.wg += (int)((n - c_lo + TRSM_COLS - 1) / TRSM_COLS);
+        r.wg += (int)((n - c_lo + TrsmCols<T>::value - 1) / TrsmCols<T>::value);
         twoff.push_back(r.wg);
         r.nb_max = nb > r.nb_max ? nb : r.nb_max;
     };
@@ -2799,7 +2836,7 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     auto trsm = [&](int p) -> int {
         const JobRange& r = trange[p];
         if (r.count == 0) return MBA_OK;
-        hipLaunchKernelGGL(k_trsm_batched<T>, dim3((unsigned)r.wg), dim3(256), (size_t)r.nb_max * TRSM_COLS * esz, st,
+        hipLaunchKernelGGL(k_trsm_batched<T>, dim3((unsigned)r.wg), dim3(256), trsm_lds_bytes<T>(r.nb_max), st,
                            (const TrsmJob<T>*)(d_tj + r.first), (const int*)(d_twoff + r.woff), (int)r.count);
         MBA_LAUNCH_CHECK();
         return MBA_OK;

@@ -305,35 +305,69 @@ size_t workspace_bytes(int64_t M, int64_t N, int64_t K)
            al256((size_t)Npad * 4);
 }
 
-int gemm(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
-         float beta, void* C, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t st)
+namespace {
+struct Parts {
+    _Float16 *Ap, *Bp;
+    int32_t *ea, *eb;
+    int Kpad, nrec;
+    int64_t Mpad, Npad;
+};
+Parts parts(int64_t M, int64_t N, int64_t K, void* ws)
+{
+    Parts P;
+    P.Kpad = (int)((K + KC - 1) / KC * KC);
+    P.nrec = P.Kpad / KC;
+    P.Mpad = (M + BM - 1) / BM * BM;
+    P.Npad = (N + BN - 1) / BN * BN;
+    char* p = (char*)ws;
+    P.Ap = (_Float16*)p;
+    p += al256((size_t)P.Mpad * P.Kpad * 4 * 2);
+    P.Bp = (_Float16*)p;
+    p += al256((size_t)P.Npad * P.Kpad * 4 * 2);
+    P.ea = (int32_t*)p;
+    p += al256((size_t)P.Mpad * 4);
+    P.eb = (int32_t*)p;
+    return P;
+}
+}  // namespace
+
+int split(int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B, int64_t ldb, void* ws,
+          size_t ws_bytes, hipStream_t st)
 {
     if (M <= 0 || N <= 0) return MBA_OK;
     if (K <= 0 || K > KMAX) return MBA_ERR_SIZE;
     if (ws_bytes < workspace_bytes(M, N, K)) return MBA_ERR_ALLOC;
-    const int Kpad = (int)((K + KC - 1) / KC * KC);
-    const int nrec = Kpad / KC;
-    const int64_t Mpad = (M + BM - 1) / BM * BM, Npad = (N + BN - 1) / BN * BN;
-    char* p = (char*)ws;
-    _Float16* Ap = (_Float16*)p;
-    p += al256((size_t)Mpad * Kpad * 4 * 2);
-    _Float16* Bp = (_Float16*)p;
-    p += al256((size_t)Npad * Kpad * 4 * 2);
-    int32_t* ea = (int32_t*)p;
-    p += al256((size_t)Mpad * 4);
-    int32_t* eb = (int32_t*)p;
-    hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((Mpad + 3) / 4)), dim3(256), 0, st, (const cpx<float>*)A, lda,
-                       M, (int)K, Kpad, Mpad, Ap, ea);
+    const Parts P = parts(M, N, K, ws);
+    hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((P.Mpad + 3) / 4)), dim3(256), 0, st, (const cpx<float>*)A, lda,
+                       M, (int)K, P.Kpad, P.Mpad, P.Ap, P.ea);
     MBA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(Npad / 64)), dim3(256), 0, st, (const cpx<float>*)B, ldb,
-                       (int)K, Kpad, N, Npad, Bp, eb);
-    MBA_LAUNCH_CHECK();
-    const int64_t tiles = (Mpad / BM) * (Npad / BN);
-    const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL(k_cgemm_h3, dim3((unsigned)blocks), dim3(THREADS), 0, st, M, N, nrec, Ap, ea, Bp, eb, alpha,
-                       beta, (cpx<float>*)C, ldc);
+    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.Npad / 64)), dim3(256), 0, st, (const cpx<float>*)B, ldb,
+                       (int)K, P.Kpad, N, P.Npad, P.Bp, P.eb);
     MBA_LAUNCH_CHECK();
     return MBA_OK;
+}
+
+int multiply(int64_t M, int64_t N, int64_t K, float alpha, float beta, void* C, int64_t ldc, const void* ws,
+             size_t ws_bytes, hipStream_t st)
+{
+    if (M <= 0 || N <= 0) return MBA_OK;
+    if (K <= 0 || K > KMAX) return MBA_ERR_SIZE;
+    if (ws_bytes < workspace_bytes(M, N, K)) return MBA_ERR_ALLOC;
+    const Parts P = parts(M, N, K, (void*)ws);
+    const int64_t tiles = (P.Mpad / BM) * (P.Npad / BN);
+    const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+    hipLaunchKernelGGL(k_cgemm_h3, dim3((unsigned)blocks), dim3(THREADS), 0, st, M, N, P.nrec, P.Ap, P.ea, P.Bp, P.eb,
+                       alpha, beta, (cpx<float>*)C, ldc);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+int gemm(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
+         float beta, void* C, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t st)
+{
+    int rc = split(M, N, K, A, lda, B, ldb, ws, ws_bytes, st);
+    if (rc != MBA_OK) return rc;
+    return multiply(M, N, K, alpha, beta, C, ldc, ws, ws_bytes, st);
 }
 
 }  // namespace mba_h3

@@ -18,4 +18,11 @@ size_t workspace_bytes(int64_t M, int64_t N, int64_t K);
 int gemm(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
          float beta, void* C, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t st);
 
+// The two stages of gemm(): split A [M x K] and B [K x N] into the workspace,
+// then C = alpha A B + beta C from the split operands.
+int split(int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B, int64_t ldb, void* ws,
+          size_t ws_bytes, hipStream_t st);
+int multiply(int64_t M, int64_t N, int64_t K, float alpha, float beta, void* C, int64_t ldc, const void* ws,
+             size_t ws_bytes, hipStream_t st);
+
 }  // namespace mba_h3

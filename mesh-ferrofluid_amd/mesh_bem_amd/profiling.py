@@ -17,8 +17,37 @@ import sqlite3
 import sys
 
 
+def _demangle(name: str) -> str:
+    """Itanium-demangled name (c++filt) for names the profiler left mangled."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["c++filt", name], capture_output=True, text=True, timeout=10).stdout.strip()
+    except (OSError, subprocess.SubprocessError):
+        out = name
+    if out and out != name:
+        return out
+    # c++filt does not know every builtin type (e.g. _Float16, "DF16_"): take
+    # the nested name's components (length-prefixed identifiers after _ZN)
+    if not name.startswith("_ZN"):
+        return name
+    comps, i = [], 3
+    while i < len(name) and name[i].isdigit():
+        j = i
+        while j < len(name) and name[j].isdigit():
+            j += 1
+        n = int(name[i:j])
+        comps.append(name[j:j + n])
+        i = j + n
+    comps = [c for c in comps if not c.startswith("_GLOBAL__N")]
+    return "::".join(comps) if comps else name
+
+
 def _short(name: str) -> str:
-    name = name.strip().replace("(anonymous namespace)::", "")
+    name = name.strip()
+    if name.startswith("_Z"):
+        name = _demangle(name)
+    name = name.replace("(anonymous namespace)::", "")
     name = re.sub(r"^void ", "", name)
     cut = name.find("(")
     if cut > 0:
