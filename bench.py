@@ -406,6 +406,107 @@ def run_hd(args, dev):
     print(json.dumps(res), flush=True)
 
 
+def run_bem2d(args, dev):
+    """2-D BEM Dirichlet solve (MeshBEMSolver2d.solve, the reference's
+    src/mesh_BEM/mesh_BEM_solver_2d.py): Laplace, LINEAR shape functions,
+    gaussQR = 8 (the 2-D demo's settings), FP32, on a closed ellipse of
+    semi-axes 0.25 and 0.15 (logarithmic capacity 0.2: the 2-D log single layer
+    is singular on curves of capacity 1) with 16 384 segments.  One step = single layer + double layer assembly
+    ([F, F, 2, 2] each, F^2 Q^2 = 1.7e10 point pairs per layer), packing to
+    [V, V], the right-hand side 1/2 M g + K^T g and the dense LU solve of the
+    [V, V] system.  Value: segment pairs per second of the whole step
+    (F^2 / t).  Roofline: the single-layer kernel, FP32 VALU, 29 flops + 1 log
+    per point pair (offset 8, d^2 3, log scale 1, weight 3, LINEAR shape
+    products 6, four accumulations 8)."""
+    import numpy as np
+    import torch
+
+    from mesh_bem_amd.bem2d import MeshBEMSolver2d, ellipse_curve, layer2d
+    from mesh_bem_amd.types import BEMType
+
+    n = 16384 if args.workload == "bem2d" else 2048
+    Q = 8
+    Vn, Fn = ellipse_curve(n, 0.25, 0.15)
+    g = Vn[:, 0] ** 2 - Vn[:, 1] ** 2
+    v = torch.tensor(Vn, dtype=torch.float32, device=dev)
+    f = torch.tensor(Fn, dtype=torch.int32, device=dev)
+    gt = torch.tensor(g, dtype=torch.float32, device=dev).reshape(1, n, 1)
+    flags = torch.zeros((1, n, 1), dtype=torch.uint8, device=dev)
+    solver = MeshBEMSolver2d(Q, 1, int(BEMType.LAPLACE), 1.0, None, dtype=torch.float32, device=dev)
+    log(f"workload {args.workload}: {n} segments, gaussQR={Q}")
+    for _ in range(args.warmup):
+        solver.solve(v, f, gt, torch.zeros_like(gt), flags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        neu, _ = solver.solve(v, f, gt, torch.zeros_like(gt), flags)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        layer2d(v, f, Q, 1, 1)
+    e1.record()
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / args.steps
+    solver64 = MeshBEMSolver2d(Q, 1, int(BEMType.LAPLACE), 1.0, None, dtype=torch.float64, device=dev)
+    neu64, _ = solver64.solve(v.double(), f, gt.double(), torch.zeros_like(gt).double(), flags)
+    err32 = float((neu.double() - neu64).norm() / neu64.norm())
+    pts = float(n) * n * Q * Q
+    flops = 29.0 * pts
+    tflops = flops / (k_ms * 1e-3) / 1e12
+    res = {
+        "metric": "2-D BEM Dirichlet solve (Laplace single + double layer, LINEAR, gaussQR 8); Mpairs/s",
+        "value": round(1e-6 * n * n / (ms * 1e-3), 3),
+        "unit": "Mpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"ellipse of semi-axes 0.25, 0.15 with {n} segments, Dirichlet data x^2 - y^2, "
+                               f"gaussQR={Q}, LINEAR",
+                   "segments": n, "parallelism": "single GPU"},
+        "fp32_relerr_vs_fp64": float(f"{err32:.3e}"),
+        "roofline": {"kernel": "k_bem2d_layer<float, 0> (single layer, 29 flops + 1 log per point pair)",
+                     "bound": "valu", "achieved": round(tflops, 2), "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / PEAK_FP32_MFMA_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(k_ms, 4), "flops_per_launch": flops,
+                     "point_pairs_per_s": round(pts / (k_ms * 1e-3), 1)},
+    }
+    if not args.no_cpu:
+        import scipy.linalg
+
+        log("CPU baseline (oracle sample)")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        ns = 512
+        Vs, Fs = ellipse_curve(ns, 0.25, 0.15)
+        gp, gw = O.gl_ref(Q)
+        tc = time.perf_counter()
+        O.bem2d_layer(Vs, Fs, gp, gw, 1, 1)
+        O.bem2d_layer(Vs, Fs, gp, gw, 2, 1)
+        t_asm = time.perf_counter() - tc
+        nl = 2048
+        rng = np.random.default_rng(0)
+        A = rng.standard_normal((nl, nl)) + nl * np.eye(nl)
+        tc = time.perf_counter()
+        scipy.linalg.solve(A, rng.standard_normal(nl), check_finite=False)
+        t_lu = time.perf_counter() - tc
+        t_full = t_asm * (n / ns) ** 2 + t_lu * (n / nl) ** 3
+        res["cpu_baseline"] = {"value": round(1e-6 * n * n / t_full, 4), "unit": "Mpairs/s",
+                               "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                               "sample": f"oracle.bem2d_layer (numpy, FP64) single + double layer at {ns} segments "
+                                         f"({t_asm:.2f} s, scaled by (F/{ns})^2) + scipy dgesv at {nl} ({t_lu:.2f} s, "
+                                         f"scaled by (F/{nl})^3); extrapolated step {t_full:.1f} s"}
+    print(json.dumps(res), flush=True)
+
+
 def run_c3(args, dev):
     """Config 3: 64 seeded star meshes on geodesic spheres of frequency 5..11
     (750-3630 RWG), 128 plane waves each, ka ~ U(1, 8).  Step: FP64 assembly of
@@ -601,7 +702,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c3", "c3small", "c4", "c4small", "c5",
-                                                            "c5small", "field", "fieldsmall", "hd", "hdsmall"])
+                                                            "c5small", "field", "fieldsmall", "hd", "hdsmall", "bem2d", "bem2dsmall"])
     ap.add_argument("--epochs", type=int, default=4, help="training epochs (c4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -634,6 +735,8 @@ def main():
         return run_field(args, dev)
     if args.workload in ("hd", "hdsmall"):
         return run_hd(args, dev)
+    if args.workload in ("bem2d", "bem2dsmall"):
+        return run_bem2d(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

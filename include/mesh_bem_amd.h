@@ -484,6 +484,34 @@ int mba_helmholtz_decomp(int prec, const void *targets, int64_t n_targets, const
                          int64_t batch, const void *rule, int32_t nq, void *out, void *workspace,
                          size_t workspace_bytes, mba_stream_t stream);
 
+/* ---- 2-D BEM on closed curves of line segments --------------------------
+ * verts [V, 2], faces [F, 2] int32 (segments), prec as everywhere.
+ * mba_bem2d_layer: the unpacked Galerkin layer of the reference's 2-D solver,
+ * double_integration_on_panels_2d (src/mesh_BEM/utils/integrations_2d.py:56-232;
+ * called by SingleLayer2d.compute_Vmat / DoubleLayer2d.compute_Kmat,
+ * single_layer_2d.py:43-68, double_layer_2d.py:43-68):
+ *   out[fy, fx, iy, ix] = A_y A_x sum_{a,b} w_a w_b J phi_iy phi_ix f(x, y),
+ * x on source segment fx, y on test segment fy, Gauss points gp / gw [nq]
+ * (the reference's float32-rounded table), the SAME_EDGE map of
+ * get_rands_and_jacobian_2d (integrations_2d.py:18-53); layer 1: f = G,
+ * layer 2: f = grad_y G . n_fy; bem_type 0 LAPLACE (real output), 1 / 2
+ * HELMHOLTZ_BEM / HELMHOLTZ_MOM (complex output, G_2d / grad_G_y_2d with the
+ * reference's large-argument Hankel form, BEM_utils.py:9-138).  order_type 0
+ * PLANAR (nphi = 1) or 1 LINEAR (nphi = 2).  out [F, F, nphi, nphi].
+ * mba_bem2d_pack: pack_matrix_tight with dim = 2 (BEM_utils.py:314-357), a
+ * fixed-order gather over the vertex -> slot (2 f + t) CSR; out [V, V] real or
+ * complex (is_complex).
+ * mba_bem2d_dirichlet_rhs: rhs[v] = c_mass g[v] va[v] + c_k sum_vy Kt[vy, v] g[vy]
+ * (RhsLayer2d.compute_rhs, rhs_layer_2d.py:66-130: c_mass = 1/2, c_k = 1, or
+ * c_mass = 1, c_k = 0 for HELMHOLTZ_MOM); g, Kt, rhs real or complex, va real. */
+int mba_bem2d_layer(int prec, int bem_type, int layer, int order_type, const void *verts, const int32_t *faces,
+                    int64_t n_verts, int64_t n_faces, const void *gp, const void *gw, int32_t nq, double wavenumber,
+                    void *out, mba_stream_t stream);
+int mba_bem2d_pack(int prec, int is_complex, const void *unpacked, int64_t n_faces, int32_t nphi,
+                   const int64_t *vinc_ptr, const int32_t *vinc_idx, int64_t n_verts, void *out, mba_stream_t stream);
+int mba_bem2d_dirichlet_rhs(int prec, int is_complex, const void *k_tight, int64_t n_verts, const void *g,
+                            const void *vert_areas, double c_mass, double c_k, void *rhs, mba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

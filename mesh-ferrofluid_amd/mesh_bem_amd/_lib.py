@@ -139,6 +139,10 @@ SIGNATURES = {
                                          _i32, _vp, _vp]),
     "mba_laplace_pack": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
     "mba_laplace_dirichlet_rhs": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "mba_bem2d_layer": (_c.c_int, [_c.c_int, _c.c_int, _c.c_int, _c.c_int, _vp, _i32p, _i64, _i64, _vp, _vp, _i32,
+                                   _dbl, _vp, _vp]),
+    "mba_bem2d_pack": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
+    "mba_bem2d_dirichlet_rhs": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp]),
     "mba_laplace_p0_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_laplace_p0_assemble": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _i64p, _i32p,
                                            _vp, _vp, _i64, _vp, _sz, _vp]),

@@ -33,8 +33,18 @@ def _face_vectors(vertices: torch.Tensor, faces: torch.Tensor):
     return v0, v1, v2
 
 
+def _segment_vector(vertices: torch.Tensor, faces: torch.Tensor) -> torch.Tensor:
+    f = faces.to(torch.int64)
+    return vertices[..., f[:, 1], :] - vertices[..., f[:, 0], :]
+
+
 def compute_face_normals(vertices: torch.Tensor, faces: torch.Tensor) -> torch.Tensor:
-    """[F,3] unit normals normalise((v1-v0) x (v2-v1))."""
+    """[F,3] unit normals normalise((v1-v0) x (v2-v1)) of triangles; for 2-D
+    segments ([F,2] faces, [V,2] vertices) normalise(e_y, -e_x), e = v1 - v0
+    (src/utils/mesh_utils.py:158-197)."""
+    if faces.shape[-1] == 2:
+        e = _segment_vector(vertices, faces)
+        return torch.nn.functional.normalize(torch.stack((e[..., 1], -e[..., 0]), dim=-1), dim=-1)
     v0, v1, v2 = _face_vectors(vertices, faces)
     n = torch.cross(v1 - v0, v2 - v1, dim=-1)
     return torch.nn.functional.normalize(n, dim=-1)
@@ -43,7 +53,10 @@ def compute_face_normals(vertices: torch.Tensor, faces: torch.Tensor) -> torch.T
 def compute_face_areas(
     vertices: torch.Tensor, faces: torch.Tensor, keepdim: bool = True
 ) -> torch.Tensor:
-    """[F,1] (keepdim) or [F] triangle areas."""
+    """[F,1] (keepdim) or [F] triangle areas; segment lengths for 2-D
+    segments (src/utils/mesh_utils.py:200-239)."""
+    if faces.shape[-1] == 2:
+        return _segment_vector(vertices, faces).norm(dim=-1, keepdim=keepdim)
     v0, v1, v2 = _face_vectors(vertices, faces)
     a = 0.5 * torch.cross(v1 - v0, v2 - v1, dim=-1).norm(dim=-1, keepdim=keepdim)
     return a
@@ -52,11 +65,13 @@ def compute_face_areas(
 def compute_vert_areas(
     vertices: torch.Tensor, faces: torch.Tensor, keepdim: bool = True
 ) -> torch.Tensor:
-    """[V,1] lumped vertex areas: sum over adjacent faces of A_f / 3."""
+    """[V,1] lumped vertex areas: sum over adjacent faces of A_f / nv (nv = 3
+    vertices per triangle, 2 per segment; src/utils/mesh_utils.py:277-310)."""
+    nv = faces.shape[-1]
     fa = compute_face_areas(vertices, faces, keepdim=False)
     va = torch.zeros(vertices.shape[-2], dtype=vertices.dtype, device=vertices.device)
     idx = faces.to(torch.int64).reshape(-1)
-    va.index_add_(0, idx, (fa[:, None] / 3.0).expand(-1, 3).reshape(-1))
+    va.index_add_(0, idx, (fa[:, None] / float(nv)).expand(-1, nv).reshape(-1))
     return va[:, None] if keepdim else va
 
 
@@ -66,7 +81,8 @@ def compute_vert_normals(vertices: torch.Tensor, faces: torch.Tensor) -> torch.T
     fa = compute_face_areas(vertices, faces, keepdim=True)
     vn = torch.zeros_like(vertices)
     idx = faces.to(torch.int64).reshape(-1)
-    vn.index_add_(0, idx, (fa * fn)[:, None, :].expand(-1, 3, -1).reshape(-1, 3))
+    nv, dim = faces.shape[-1], vertices.shape[-1]
+    vn.index_add_(0, idx, (fa * fn)[:, None, :].expand(-1, nv, -1).reshape(-1, dim))
     return torch.nn.functional.normalize(vn, dim=-1)
 
 
@@ -124,13 +140,14 @@ def rwg_face_edges(face_edge_indices, n_edges: int) -> np.ndarray:
 
 
 def compute_panel_relation(faces: torch.Tensor) -> torch.Tensor:
-    """[F,F] uint8 number of shared vertex indices (0..3), in O(F) extra memory."""
+    """[F,F] uint8 number of shared vertex indices (0..3 for triangles, 0..2
+    for segments), in O(F) extra memory."""
     f = faces.detach().cpu().to(torch.int64).numpy()
     F = f.shape[0]
     nv = int(f.max()) + 1 if F else 0
     rel = np.zeros((F, F), dtype=np.uint8)
     # vertex -> faces incidence
-    vf_face = np.repeat(np.arange(F), 3)
+    vf_face = np.repeat(np.arange(F), f.shape[1])
     vf_vert = f.reshape(-1)
     order = np.argsort(vf_vert, kind="stable")
     starts = np.searchsorted(vf_vert[order], np.arange(nv + 1))

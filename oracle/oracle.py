@@ -581,3 +581,144 @@ def helmholtz_decomposition_solve(verts, faces, vel, gaussQR):
         vel = vel[None]
     un = (vn[None] * u).sum(-1, keepdims=True)
     return vel - (vel * vn[None]).sum(-1, keepdims=True) * vn[None] + un * vn[None]
+
+
+# ---------------------------------------------------------------- 2-D BEM (line segments)
+# "Faces" are segments [F, 2] of a closed curve, vertices [V, 2].
+def seg_areas_normals(verts, faces):
+    """Segment lengths |v1 - v0| and normals (e_y, -e_x) / |e|, e = v1 - v0
+    (src/utils/mesh_utils.py:174-183, 216-223)."""
+    V = np.asarray(verts, dtype=np.float64)
+    F = np.asarray(faces, dtype=np.int64)
+    e = V[F[:, 1]] - V[F[:, 0]]
+    L = np.linalg.norm(e, axis=1)
+    n = np.stack([e[:, 1], -e[:, 0]], axis=1) / L[:, None]
+    return L, n
+
+
+def seg_vert_areas(verts, faces):
+    """Vertex 'areas': sum of half the lengths of the two incident segments
+    (mesh_utils.py:277-310 with dim = 2)."""
+    L, _ = seg_areas_normals(verts, faces)
+    va = np.zeros(len(verts))
+    F = np.asarray(faces, dtype=np.int64)
+    for t in range(2):
+        np.add.at(va, F[:, t], L / 2.0)
+    return va
+
+
+def hankel2_ref(order, z):
+    """The reference's Hankel function of the second kind (BEM_utils.py:9-31):
+    the large-argument form sqrt(2 / (pi z)) exp(-j (z - pi/4 - order pi/2))."""
+    th = math.pi / 4.0 + order * (math.pi / 2.0)
+    return np.sqrt(2.0 / math.pi / z) * (np.cos(z - th) - 1j * np.sin(z - th))
+
+
+def green2d(bem_type, k, x, y):
+    """G_2d (BEM_utils.py:34-69): LAPLACE (bem_type 0) (1 / 2 pi) log(1 / |x - y|);
+    Helmholtz 0.25 (Im H - j Re H) with H = hankel2_ref(0, k |x - y|)."""
+    d = np.linalg.norm(x - y, axis=-1)
+    if bem_type == 0:
+        return (1.0 / (2.0 * math.pi)) * np.log(1.0 / d)
+    h = hankel2_ref(0, k * d)
+    return 0.25 * (h.imag - 1j * h.real)
+
+
+def grad_green2d_y(bem_type, k, x, y):
+    """grad_G_y_2d (BEM_utils.py:107-138): LAPLACE (1 / 2 pi) (x - y) / |x - y|^2;
+    Helmholtz 0.25 (Im H1 - j Re H1) k (x - y) / |x - y|."""
+    d = np.linalg.norm(x - y, axis=-1, keepdims=True)
+    direc = (x - y) / d
+    if bem_type == 0:
+        return (1.0 / (2.0 * math.pi)) * direc / d
+    h = hankel2_ref(1, k * d)
+    return 0.25 * (h.imag - 1j * h.real) * (k * direc)
+
+
+def panel_relation_2d(faces):
+    """Shared vertex indices of two segments: 0 SEPARATE, 1 SAME_VERTEX,
+    2 SAME_EDGE (mesh_utils.py:700-725 with dim = 2)."""
+    F = np.asarray(faces, dtype=np.int64)
+    return (F[:, None, :, None] == F[None, :, None, :]).sum(axis=(2, 3)).astype(np.uint8)
+
+
+def bem2d_layer(verts, faces, gp, gw, layer, order_type, bem_type=0, k=1.0):
+    """Unpacked [F, F, nphi, nphi] 2-D layer, row = test segment fy, column =
+    source segment fx (integrations_2d.py:18-232):
+      out[fy, fx, iy, ix] = A_y A_x sum_{a,b} w_a w_b J phi_iy(r1y) phi_ix(r1x) f(x, y)
+    x = (1 - r1x) X0 + r1x X1 on fx, y = (1 - r1y) Y0 + r1y Y1 on fy, r1x = gp[a],
+    r1y = gp[b] and J = 1 except SAME_EDGE: r1y = gp[a] gp[b], J = gp[a] (the
+    half square y < x, as the reference maps it).  f = G (layer 1) or
+    grad_y G . n_fy (layer 2)."""
+    V = np.asarray(verts, dtype=np.float64)
+    Fc = np.asarray(faces, dtype=np.int64)
+    nF = len(Fc)
+    L, nrm = seg_areas_normals(V, Fc)
+    rel = panel_relation_2d(Fc)
+    gp = np.asarray(gp, dtype=np.float64)
+    gw = np.asarray(gw, dtype=np.float64)
+    Q = gp.size
+    ra = np.tile(gp, Q)          # x point index a (fast)
+    rb = np.repeat(gp, Q)        # y point index b (slow)
+    w = np.tile(gw, Q) * np.repeat(gw, Q)
+    nphi = 2 if order_type == 1 else 1
+    cplx = bem_type != 0
+    out = np.zeros((nF, nF, nphi, nphi), dtype=np.complex128 if cplx else np.float64)
+    for r in (0, 1, 2):
+        fy, fx = np.nonzero(rel == r)
+        if fy.size == 0:
+            continue
+        r1x = ra
+        r1y = ra * rb if r == 2 else rb
+        jac = ra if r == 2 else np.ones_like(ra)
+        X0, X1 = V[Fc[fx, 0]], V[Fc[fx, 1]]
+        Y0, Y1 = V[Fc[fy, 0]], V[Fc[fy, 1]]
+        x = (1 - r1x)[None, :, None] * X0[:, None, :] + r1x[None, :, None] * X1[:, None, :]
+        y = (1 - r1y)[None, :, None] * Y0[:, None, :] + r1y[None, :, None] * Y1[:, None, :]
+        if layer == 1:
+            f = green2d(bem_type, k, x, y)
+        else:
+            f = (grad_green2d_y(bem_type, k, x, y) * nrm[fy][:, None, :]).sum(-1)
+        phix = [1 - r1x, r1x] if nphi == 2 else [np.ones_like(r1x)]
+        phiy = [1 - r1y, r1y] if nphi == 2 else [np.ones_like(r1y)]
+        s = (L[fy] * L[fx])[:, None]
+        for iy in range(nphi):
+            for ix in range(nphi):
+                out[fy, fx, iy, ix] = (s * f * (w * jac * phiy[iy] * phix[ix])[None, :]).sum(1)
+    return out
+
+
+def bem2d_pack(order_type, U, faces, n_verts):
+    """pack_matrix_tight with dim = 2 (BEM_utils.py:314-357): PLANAR entries
+    spread as U / 4 over the 2 x 2 vertex pairs."""
+    Fc = np.asarray(faces, dtype=np.int64)
+    nF = len(Fc)
+    if order_type == 0:
+        U = np.repeat(np.repeat(U.reshape(nF, nF, 1, 1), 2, 2), 2, 3) / 4.0
+    A = np.zeros((n_verts, n_verts), dtype=U.dtype)
+    for iy in range(2):
+        for ix in range(2):
+            np.add.at(A, (Fc[:, iy][:, None], Fc[:, ix][None, :]), U[:, :, iy, ix])
+    return A
+
+
+def bem2d_dirichlet(verts, faces, g, gaussQR, order_type=1, bem_type=0, k=1.0):
+    """MeshBEMSolver2d.solve with Dirichlet data only (mesh_BEM_solver_2d.py:77-250,
+    rhs_layer_2d.py:66-130): V t = 1/2 M g + K^T g (LAPLACE, HELMHOLTZ_BEM) or
+    V t = M g (HELMHOLTZ_MOM: gk = 2 gk1).  Returns (t, V, rhs)."""
+    import scipy.linalg
+
+    gp, gw = gl_ref(gaussQR)
+    Vu = bem2d_layer(verts, faces, gp, gw, 1, order_type, bem_type, k)
+    Ku = bem2d_layer(verts, faces, gp, gw, 2, order_type, bem_type, k)
+    nV = len(verts)
+    Vt = bem2d_pack(order_type, Vu, faces, nV)
+    Kt = bem2d_pack(order_type, Ku, faces, nV)
+    va = seg_vert_areas(verts, faces)
+    g = np.asarray(g).reshape(-1)
+    if bem_type == 2:
+        rhs = g * va
+    else:
+        rhs = 0.5 * g * va + Kt.T @ g
+    t = scipy.linalg.solve(Vt, rhs)
+    return t, Vt, rhs
