@@ -507,6 +507,56 @@ def run_bem2d(args, dev):
     print(json.dumps(res), flush=True)
 
 
+def run_edges(args, dev):
+    """Mesh topology: the unique-edge table of get_edge_unique
+    (src/utils/mesh_utils.py:921-977) on the GPU (mba_edge_unique: keys,
+    hipCUB radix sort of (key, slot), head flags, scan, fill) for an icosphere of
+    subdivision 8 (1 310 720 faces).  Value: faces per second.  Roofline: HBM;
+    algorithmic bytes 48 per face (faces read 12 B; edges, face-edge indices
+    and adjacency written, 12 B each per face)."""
+    import torch
+
+    from mesh_bem_amd.geometry import get_edge_unique, icosphere
+
+    sub = 8 if args.workload == "edges" else 5
+    Vn, Fn = icosphere(sub)
+    v = torch.tensor(Vn, device=dev)
+    f = torch.tensor(Fn, dtype=torch.int32, device=dev)
+    log(f"workload {args.workload}: icosphere subdiv {sub}, {len(Fn)} faces")
+    for _ in range(args.warmup):
+        get_edge_unique(v, f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e, fe, adj = get_edge_unique(v, f)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    e_h, fe_h, adj_h = get_edge_unique(v.cpu(), f.cpu())
+    exact = bool(torch.equal(e.cpu(), e_h) and torch.equal(fe.cpu(), fe_h) and torch.equal(adj.cpu(), adj_h))
+    nbytes = 48.0 * len(Fn)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    res = {
+        "metric": "unique-edge table (get_edge_unique) on the GPU; Mfaces/s",
+        "value": round(1e-6 * len(Fn) / (ms * 1e-3), 2),
+        "unit": "Mfaces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32/int64", "data": "synthetic",
+        "config": {"workload": f"icosphere subdivision {sub} ({len(Fn)} faces, {len(Vn)} vertices)",
+                   "parallelism": "single GPU"},
+        "bit_exact_vs_host": exact,
+        "roofline": {"kernel": "mba_edge_unique pipeline (keys, radix sort, heads, scan, fill; host-timed step)",
+                     "bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 5), "traffic": None, "bytes_per_step": nbytes},
+    }
+    if not args.no_cpu:
+        tc = time.perf_counter()
+        get_edge_unique(v.cpu(), f.cpu())
+        dt = time.perf_counter() - tc
+        res["cpu_baseline"] = {"value": round(1e-6 * len(Fn) / dt, 3), "unit": "Mfaces/s", "cores": 1, "kind": "port",
+                               "sample": f"host numpy path of get_edge_unique (np.unique + stable argsort) on the "
+                                         f"same mesh, {dt:.2f} s"}
+    print(json.dumps(res), flush=True)
+
+
 def run_c3(args, dev):
     """Config 3: 64 seeded star meshes on geodesic spheres of frequency 5..11
     (750-3630 RWG), 128 plane waves each, ka ~ U(1, 8).  Step: FP64 assembly of
@@ -702,7 +752,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c3", "c3small", "c4", "c4small", "c5",
-                                                            "c5small", "field", "fieldsmall", "hd", "hdsmall", "bem2d", "bem2dsmall"])
+                                                            "c5small", "field", "fieldsmall", "hd", "hdsmall", "bem2d", "bem2dsmall", "edges",
+                                                            "edgessmall"])
     ap.add_argument("--epochs", type=int, default=4, help="training epochs (c4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -737,6 +788,8 @@ def main():
         return run_hd(args, dev)
     if args.workload in ("bem2d", "bem2dsmall"):
         return run_bem2d(args, dev)
+    if args.workload in ("edges", "edgessmall"):
+        return run_edges(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

@@ -39,7 +39,8 @@ enum {
     MBA_ERR_ALLOC = 4,   /* workspace too small */
     MBA_ERR_LAUNCH = 5,  /* kernel launch failed */
     MBA_ERR_NUMERIC = 6, /* zero pivot / non-finite value reported by a kernel */
-    MBA_ERR_TIMEOUT = 7  /* a cooperative kernel's bounded wait expired */
+    MBA_ERR_TIMEOUT = 7, /* a cooperative kernel's bounded wait expired */
+    MBA_ERR_IO = 8       /* file missing or unreadable / unwritable (host mesh I/O) */
 };
 
 enum { MBA_F32 = 0, MBA_F64 = 1 };
@@ -511,6 +512,44 @@ int mba_bem2d_pack(int prec, int is_complex, const void *unpacked, int64_t n_fac
                    const int64_t *vinc_ptr, const int32_t *vinc_idx, int64_t n_verts, void *out, mba_stream_t stream);
 int mba_bem2d_dirichlet_rhs(int prec, int is_complex, const void *k_tight, int64_t n_verts, const void *g,
                             const void *vert_areas, double c_mass, double c_k, void *rhs, mba_stream_t stream);
+
+/* ---- mesh topology on the device -----------------------------------------
+ * Unique edges of a triangle mesh (replaces get_edge_unique,
+ * src/utils/mesh_utils.py:921-977; local edge e of face f joins
+ * faces[f][(e+1)%3] and faces[f][(e+2)%3], key max * V + min).  The
+ * reference's torch.unique(sorted=False) order is backend-dependent; here
+ * edges ascend by key.  Outputs (device, capacity 3 n_faces edges):
+ * edges [E, 2] (max, min), face_edge_indices [3 n_faces] (edge of slot 3f+e),
+ * edge_adjacent_faces [E, 2] (first, second face by ascending slot; -1 if the
+ * edge has one face), *n_edges (device int64) = E.  workspace >=
+ * mba_edge_unique_workspace_bytes (0 when the sizes are out of range). */
+size_t mba_edge_unique_workspace_bytes(int64_t n_faces, int64_t n_verts);
+int mba_edge_unique(const int32_t *faces, int64_t n_faces, int64_t n_verts, int32_t *edges,
+                    int32_t *face_edge_indices, int32_t *edge_adjacent_faces, int64_t *n_edges, void *workspace,
+                    size_t workspace_bytes, mba_stream_t stream);
+
+/* ---- mesh files (host only, no device pointers) --------------------------
+ * mba_mesh_read replaces the readers behind LoadSingleMesh
+ * (src/utils/io.py:60-139: pywavefront for .obj, open3d for .ply, json, trimesh
+ * for .dae) and reads the BEAST .in layout of demo/generate_in_file.py:12-40.
+ * The format follows the extension (.obj .ply .json .dae .in); polygons are
+ * split into fans (p0, p_i, p_i+1); a .dae file gives one part per geometry
+ * (trimesh Scene), every other format one part.  The parts array and its
+ * buffers are allocated by the library: release them with mba_mesh_free.
+ * verts [n_verts, 3] double, faces [n_faces, 3] int32 (0-based).
+ * mba_mesh_write_beast_in replaces save_beast_in (generate_in_file.py:12-40):
+ * "1", "NV NF", vertex lines (17 significant digits), 1-based face lines (the
+ * indices are shifted when the smallest one is 0). */
+typedef struct {
+    int64_t n_verts;
+    int64_t n_faces;
+    double *verts;
+    int32_t *faces;
+} mba_mesh_part;
+int mba_mesh_read(const char *path, mba_mesh_part **parts, int32_t *n_parts);
+void mba_mesh_free(mba_mesh_part *parts, int32_t n_parts);
+int mba_mesh_write_beast_in(const char *path, const double *verts, int64_t n_verts, const int32_t *faces,
+                            int64_t n_faces);
 
 #ifdef __cplusplus
 }

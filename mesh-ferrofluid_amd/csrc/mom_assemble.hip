@@ -931,6 +931,7 @@ const char* mba_status_string(int status)
     case MBA_ERR_LAUNCH: return "kernel launch failed";
     case MBA_ERR_NUMERIC: return "numerical failure (zero pivot or non-finite value)";
     case MBA_ERR_TIMEOUT: return "cooperative wait timed out";
+    case MBA_ERR_IO: return "file missing, unreadable or unwritable";
     default: return "unknown status";
     }
 }

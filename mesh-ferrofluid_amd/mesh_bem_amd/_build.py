@@ -13,7 +13,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(ROOT, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libmesh_bem_amd.so")
-SOURCES = ["mom_assemble.hip", "dense.hip", "laplace.hip", "gmres.hip", "mesh.hip", "mgn.hip", "radiate.hip", "helmholtz.hip", "gemm_h3.hip", "bem2d.hip"]
+SOURCES = ["mom_assemble.hip", "dense.hip", "laplace.hip", "gmres.hip", "mesh.hip", "mgn.hip", "radiate.hip", "helmholtz.hip", "gemm_h3.hip", "bem2d.hip", "topology.hip", "mesh_io.cpp"]
 ARCH = os.environ.get("MBA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -37,7 +37,7 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        obj = os.path.join(build_dir, src.replace(".hip", ".o"))
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [hipcc, *HIPCC_FLAGS, "-c", path, "-o", obj]
         if verbose:

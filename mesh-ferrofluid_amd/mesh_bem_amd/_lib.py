@@ -60,6 +60,13 @@ class MgnVMat(ctypes.Structure):
     _fields_ = [("p", ctypes.c_void_p * 3), ("idx", ctypes.c_void_p * 3), ("ld", ctypes.c_int64 * 3),
                 ("w", ctypes.c_int32 * 3), ("nseg", ctypes.c_int32), ("ones", ctypes.c_int32)]
 
+class MeshPart(ctypes.Structure):
+    """mba_mesh_part of include/mesh_bem_amd.h."""
+
+    _fields_ = [("n_verts", ctypes.c_int64), ("n_faces", ctypes.c_int64),
+                ("verts", ctypes.POINTER(ctypes.c_double)), ("faces", ctypes.POINTER(ctypes.c_int32))]
+
+
 _c = ctypes
 _vp, _i32p, _i64p, _dp = _c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p
 _i32, _i64, _dbl, _sz = _c.c_int32, _c.c_int64, _c.c_double, _c.c_size_t
@@ -143,6 +150,11 @@ SIGNATURES = {
                                    _dbl, _vp, _vp]),
     "mba_bem2d_pack": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
     "mba_bem2d_dirichlet_rhs": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp]),
+    "mba_edge_unique_workspace_bytes": (_sz, [_i64, _i64]),
+    "mba_edge_unique": (_c.c_int, [_i32p, _i64, _i64, _i32p, _i32p, _i32p, _i64p, _vp, _sz, _vp]),
+    "mba_mesh_read": (_c.c_int, [_c.c_char_p, _vp, _vp]),
+    "mba_mesh_free": (None, [_vp, _i32]),
+    "mba_mesh_write_beast_in": (_c.c_int, [_c.c_char_p, _vp, _i64, _vp, _i64]),
     "mba_laplace_p0_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_laplace_p0_assemble": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _i64p, _i32p,
                                            _vp, _vp, _i64, _vp, _sz, _vp]),

@@ -95,6 +95,8 @@ def get_edge_unique(verts: torch.Tensor, faces: torch.Tensor) -> List[torch.Tens
     and ``edge_adjacent_faces [E,2]`` (lower face index first).
     """
     N_verts = verts.shape[0]
+    if faces.is_cuda:
+        return _edge_unique_device(faces, N_verts)
     f = faces.detach().to("cpu").to(torch.int64).numpy()
     F = f.shape[0]
     a = np.stack([f[:, 1], f[:, 2], f[:, 0]], axis=1).reshape(-1)  # (e+1)%3
@@ -118,6 +120,30 @@ def get_edge_unique(verts: torch.Tensor, faces: torch.Tensor) -> List[torch.Tens
         torch.from_numpy(inverse.astype(np.int64)).to(faces.dtype).to(dev),
         torch.from_numpy(adj).to(faces.dtype).to(dev),
     )
+
+
+def _edge_unique_device(faces: torch.Tensor, n_verts: int) -> List[torch.Tensor]:
+    """get_edge_unique of a device mesh on the GPU (mba_edge_unique): the same
+    edges, order and adjacency as the host path above, bit for bit."""
+    from . import _lib
+
+    lib = _lib.load()
+    dev = faces.device
+    F = faces.shape[0]
+    ft = faces.to(torch.int32).contiguous()
+    wsb = lib.mba_edge_unique_workspace_bytes(F, n_verts)
+    if wsb == 0:
+        raise ValueError("get_edge_unique: mesh too large for the device path")
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    edges = torch.empty((3 * F, 2), dtype=torch.int32, device=dev)
+    fe = torch.empty(3 * F, dtype=torch.int32, device=dev)
+    adj = torch.empty((3 * F, 2), dtype=torch.int32, device=dev)
+    ne = torch.zeros(1, dtype=torch.int64, device=dev)
+    st = lib.mba_edge_unique(_lib.ptr(ft), F, n_verts, _lib.ptr(edges), _lib.ptr(fe), _lib.ptr(adj), _lib.ptr(ne),
+                             _lib.ptr(ws), wsb, _lib.stream())
+    _lib.check(st, "mba_edge_unique")
+    E = int(ne.item())
+    return edges[:E].to(faces.dtype), fe.to(faces.dtype), adj[:E].to(faces.dtype)
 
 
 def rwg_face_edges(face_edge_indices, n_edges: int) -> np.ndarray:
