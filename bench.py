@@ -19,6 +19,7 @@ LAPACK zgesv at a smaller order, both scaled).
 """
 
 import argparse
+import math
 import json
 import os
 import sys
@@ -557,6 +558,90 @@ def run_edges(args, dev):
     print(json.dumps(res), flush=True)
 
 
+def run_pressure(args, dev):
+    """Ferrofluid pressure step (MeshPressure.solve,
+    src/mesh_ferrofluid/mesh_pressure.py:54-163) with the droplet demo's
+    parameters (demo_3d_mesh_droplet.py:30-48: rho = 1300, g = 9.81, sigma =
+    0.02, dt = 5e-4, gaussQR = 8, LINEAR, FP32, every vertex DIRICHLET) on a
+    sphere of the demo's radius L / 2 (L = 5 * 2 pi / sqrt(rho g / sigma)),
+    icosphere subdivision 4 (5 120 faces, 2 562 vertices), random velocity.
+    One step = boundary data kernel, single + double layer of the reference's
+    relation-typed rule, packing, RHS, dense solve, gradient / velocity update.
+    Value: face pairs per second (F^2 / t).  Roofline: the single-layer
+    kernel, FP32 VALU, 73 flops + sqrt + reciprocal per table row (LINEAR),
+    Q^4 = 4096 rows for separate face pairs (counted for all F^2 pairs)."""
+    import numpy as np
+    import torch
+
+    from mesh_bem_amd.ferrofluid import MeshPressure
+    from mesh_bem_amd.geometry import icosphere
+    from mesh_bem_amd.laplace import layer_ref
+
+    sub = 4 if args.workload == "pressure" else 2
+    Q = 8
+    rho, g, sigma, dt = 1300.0, 9.81, 0.02, 5e-4
+    L = 5 * (2 * math.pi / math.sqrt(rho * g / sigma))
+    Vn, Fn = icosphere(sub)
+    Vn = Vn * (L / 2)
+    rng = np.random.default_rng(5)
+    vel = torch.tensor(rng.standard_normal((1, len(Vn), 3)) * 1e-3, dtype=torch.float32, device=dev)
+    v = torch.tensor(Vn, dtype=torch.float32, device=dev)
+    f = torch.tensor(Fn, dtype=torch.int32, device=dev)
+    flags = torch.zeros((1, len(Vn), 1), dtype=torch.uint8, device=dev)
+    mp = MeshPressure(rho, g, sigma, 0.5 * math.pi, Q, 1, None, device=dev, dtype=torch.float32)
+    log(f"workload {args.workload}: icosphere {sub} ({len(Fn)} faces), radius {L / 2:.4g}, gaussQR {Q}")
+    for _ in range(args.warmup):
+        mp.solve(v, f, dt, vel, flags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = mp.solve(v, f, dt, vel, flags)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    layer_ref(v, f, Q, 1, 1)
+    e1.record()
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1)
+    nF = len(Fn)
+    flops = 73.0 * Q ** 4 * nF * nF
+    tflops = flops / (k_ms * 1e-3) / 1e12
+    mp64 = MeshPressure(rho, g, sigma, 0.5 * math.pi, Q, 1, None, device=dev, dtype=torch.float64)
+    out64 = mp64.solve(v.double(), f, dt, vel.double(), flags)
+    err32 = float((out.double() - out64).norm() / out64.norm())
+    res = {
+        "metric": "ferrofluid pressure step (MeshPressure.solve: curvature + Laplace BEM + gradient); Mpairs/s",
+        "value": round(1e-6 * nF * nF / (ms * 1e-3), 3),
+        "unit": "Mpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"droplet demo parameters, icosphere subdivision {sub} ({nF} faces) of radius "
+                               f"{L / 2:.4g}, gaussQR={Q}, LINEAR, all vertices Dirichlet", "parallelism": "single GPU"},
+        "fp32_relerr_vs_fp64": float(f"{err32:.3e}"),
+        "roofline": {"kernel": "k_laplace_ref<float> (single layer, 73 flops + sqrt + rcp per table row)",
+                     "bound": "valu", "achieved": round(tflops, 2), "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / PEAK_FP32_MFMA_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(k_ms, 4), "flops_per_launch": flops},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (oracle sample)")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+
+        Vs, Fs = icosphere(2)
+        Vs = Vs * (L / 2)
+        tc = time.perf_counter()
+        O.pressure_step(Vs, Fs, np.zeros_like(Vs), dt, rho, g, sigma, Q)
+        t_s = time.perf_counter() - tc
+        t_full = t_s * (nF / len(Fs)) ** 2
+        res["cpu_baseline"] = {"value": round(1e-6 * nF * nF / t_full, 4), "unit": "Mpairs/s",
+                               "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                               "sample": f"oracle.pressure_step (C/OpenMP layers + numpy + scipy, FP64) at "
+                                         f"{len(Fs)} faces ({t_s:.2f} s), scaled by (F/{len(Fs)})^2 (the layers "
+                                         f"dominate); extrapolated step {t_full:.1f} s"}
+    print(json.dumps(res), flush=True)
+
+
 def run_c3(args, dev):
     """Config 3: 64 seeded star meshes on geodesic spheres of frequency 5..11
     (750-3630 RWG), 128 plane waves each, ka ~ U(1, 8).  Step: FP64 assembly of
@@ -753,7 +838,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c3", "c3small", "c4", "c4small", "c5",
                                                             "c5small", "field", "fieldsmall", "hd", "hdsmall", "bem2d", "bem2dsmall", "edges",
-                                                            "edgessmall"])
+                                                            "edgessmall", "pressure", "pressuresmall"])
     ap.add_argument("--epochs", type=int, default=4, help="training epochs (c4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -790,6 +875,8 @@ def main():
         return run_bem2d(args, dev)
     if args.workload in ("edges", "edgessmall"):
         return run_edges(args, dev)
+    if args.workload in ("pressure", "pressuresmall"):
+        return run_pressure(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

@@ -513,6 +513,29 @@ int mba_bem2d_pack(int prec, int is_complex, const void *unpacked, int64_t n_fac
 int mba_bem2d_dirichlet_rhs(int prec, int is_complex, const void *k_tight, int64_t n_verts, const void *g,
                             const void *vert_areas, double c_mass, double c_k, void *rhs, mba_stream_t stream);
 
+/* ---- ferrofluid pressure step around the Laplace solve -------------------
+ * MeshPressure.solve (src/mesh_ferrofluid/mesh_pressure.py:54-163) without
+ * its BEM solve (mba_laplace_* / MeshBEMSolver3d.solve).  Vertex -> face-slot
+ * CSR vinc_ptr [V+1] int64 / vinc_idx [3F] int32 (slot 3 f + t, ascending).
+ * mba_pressure_bc: vertex normals normalise(sum_f A_f n_f) [V, 3]; signed
+ * mean curvature kappa [V] of pytorch3d's cot_laplacian (mesh_pressure.py:111-124:
+ * lv = 0.5 sum_i L_ij (v_i - v_j) / sum_f A_f, kappa = 0.5 |lv| sign(lv . n));
+ * p_dirichlet [B, V] = rho g v_y - P_mag + sigma kappa (0 at NEUMANN
+ * vertices; p_mag [B, V] or null); dpdn_neumann [B, V] = (rho / dt) vel . n
+ * (0 at DIRICHLET vertices); flags [B, V] uint8 (0 DIRICHLET, 1 NEUMANN),
+ * vel [B, V, 3].  mba_pressure_update: new_vel [B, V, 3] =
+ * vel - (dt / rho) (dpdn n + tangential gradient of p)
+ * (compute_mesh_vert_tangent_gradient, mesh_utils.py:501-615). */
+int mba_pressure_bc(int prec, const void *verts, const int32_t *faces, int64_t n_verts, int64_t n_faces,
+                    const int64_t *vinc_ptr, const int32_t *vinc_idx, const void *vel, const uint8_t *flags,
+                    const void *p_mag, int64_t batch, double density, double gravity_strength,
+                    double surface_tension, double dt, void *vert_normals, void *kappa, void *p_dirichlet,
+                    void *dpdn_neumann, mba_stream_t stream);
+int mba_pressure_update(int prec, const void *verts, const int32_t *faces, int64_t n_verts, int64_t n_faces,
+                        const int64_t *vinc_ptr, const int32_t *vinc_idx, const void *vert_normals, const void *p,
+                        const void *dpdn, const void *vel, int64_t batch, double density, double dt, void *new_vel,
+                        mba_stream_t stream);
+
 /* ---- mesh topology on the device -----------------------------------------
  * Unique edges of a triangle mesh (replaces get_edge_unique,
  * src/utils/mesh_utils.py:921-977; local edge e of face f joins

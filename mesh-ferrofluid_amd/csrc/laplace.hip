@@ -74,29 +74,38 @@ __global__ __launch_bounds__(256) void k_laplace_ref(
             for (int v = 0; v < 3; ++v) Xr[3 * v + q] = X[3 * v + q] - Yabs[q];
         }
         T acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        // the double layer of a flat face on itself is exactly zero ((x - y)
-        // lies in the plane); summing rounding noise of (x - y).n / R^3 is skipped
-        const int64_t r0 = tab_off[rel], r1 = (layer == 2 && rel == 3) ? r0 : tab_off[rel + 1];
-        for (int64_t g = r0; g < r1; ++g) {
-            const T r1x = tab[5 * g], r2x = tab[5 * g + 1], r1y = tab[5 * g + 2], r2y = tab[5 * g + 3];
-            const T W = tab[5 * g + 4];
-            const T px0 = (T)1 - r1x, px1 = r1x - r2x, px2 = r2x;
-            const T py0 = (T)1 - r1y, py1 = r1y - r2y, py2 = r2y;
-            T d[3];
-            for (int q = 0; q < 3; ++q)
-                d[q] = (px0 * Xr[q] + px1 * Xr[3 + q] + px2 * Xr[6 + q]) - (py1 * Y[3 + q] + py2 * Y[6 + q]);
-            const T R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            const T rinv = (T)1 / sqrt(R2);
-            T fv = rinv * inv4pi;
-            if (layer == 2) fv *= (d[0] * Y[9] + d[1] * Y[10] + d[2] * Y[11]) * rinv * rinv;
-            const T wf = W * fv;
-            if (nphi == 1) {
-                acc[0] += wf;
-            } else {
-                const T pyv[3] = {py0, py1, py2}, pxv[3] = {px0, px1, px2};
-                for (int iy = 0; iy < 3; ++iy)
-                    for (int ix = 0; ix < 3; ++ix) acc[3 * iy + ix] += wf * pyv[iy] * pxv[ix];
+        auto accumulate = [&](int64_t g0, int64_t g1) {
+            for (int64_t g = g0; g < g1; ++g) {
+                const T r1x = tab[5 * g], r2x = tab[5 * g + 1], r1y = tab[5 * g + 2], r2y = tab[5 * g + 3];
+                const T W = tab[5 * g + 4];
+                const T px0 = (T)1 - r1x, px1 = r1x - r2x, px2 = r2x;
+                const T py0 = (T)1 - r1y, py1 = r1y - r2y, py2 = r2y;
+                T d[3];
+                for (int q = 0; q < 3; ++q)
+                    d[q] = (px0 * Xr[q] + px1 * Xr[3 + q] + px2 * Xr[6 + q]) - (py1 * Y[3 + q] + py2 * Y[6 + q]);
+                const T R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const T rinv = rsqrt_t(R2);
+                T fv = rinv * inv4pi;
+                if (layer == 2) fv *= (d[0] * Y[9] + d[1] * Y[10] + d[2] * Y[11]) * rinv * rinv;
+                const T wf = W * fv;
+                if (nphi == 1) {
+                    acc[0] += wf;
+                } else {
+                    const T pyv[3] = {py0, py1, py2}, pxv[3] = {px0, px1, px2};
+                    for (int iy = 0; iy < 3; ++iy)
+                        for (int ix = 0; ix < 3; ++ix) acc[3 * iy + ix] += wf * pyv[iy] * pxv[ix];
+                }
             }
+        };
+        if (__all(rel == 0)) {
+            // every lane of the wave has a separate pair: one table range for
+            // the wave, so the row loads are uniform (scalar cache)
+            accumulate(tab_off[0], tab_off[1]);
+        } else {
+            // the double layer of a flat face on itself is exactly zero ((x - y)
+            // lies in the plane); summing rounding noise of (x - y).n / R^3 is skipped
+            const int64_t r0 = tab_off[rel], r1 = (layer == 2 && rel == 3) ? r0 : tab_off[rel + 1];
+            accumulate(r0, r1);
         }
         const T s = (T)4 * Ax * Yabs[12];
         T* o = out + ((fy0 + t) * F + fx) * (nphi * nphi);

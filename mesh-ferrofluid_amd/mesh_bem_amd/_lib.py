@@ -150,6 +150,10 @@ SIGNATURES = {
                                    _dbl, _vp, _vp]),
     "mba_bem2d_pack": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _i32, _i64p, _i32p, _i64, _vp, _vp]),
     "mba_bem2d_dirichlet_rhs": (_c.c_int, [_c.c_int, _c.c_int, _vp, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp]),
+    "mba_pressure_bc": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _i64p, _i32p, _vp, _vp, _vp, _i64, _dbl, _dbl,
+                                   _dbl, _dbl, _vp, _vp, _vp, _vp, _vp]),
+    "mba_pressure_update": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _i64p, _i32p, _vp, _vp, _vp, _vp, _i64, _dbl,
+                                       _dbl, _vp, _vp]),
     "mba_edge_unique_workspace_bytes": (_sz, [_i64, _i64]),
     "mba_edge_unique": (_c.c_int, [_i32p, _i64, _i64, _i32p, _i32p, _i32p, _i64p, _vp, _sz, _vp]),
     "mba_mesh_read": (_c.c_int, [_c.c_char_p, _vp, _vp]),

@@ -85,3 +85,95 @@ class MeshHelmholtzDecomposition(object):
         un = (vn * u).sum(dim=-1, keepdim=True)
         vel = vert_vel.to(verts.dtype)
         return vel - (vel * vn).sum(dim=-1, keepdim=True) * vn + un * vn
+
+
+# ---------------------------------------------------------------- pressure step
+TRIPLE_JUNCTION = 6  # VertexType.TRIPLE_JUNCTION of src/utils/types.py:66-74
+
+
+class MeshPressure:
+    """Drop-in for ``src/mesh_ferrofluid/mesh_pressure.py:20-163``: same
+    constructor and ``solve(verts, faces, dt, vel, vert_bc_flags, P_mag=None)
+    -> new_vel``.
+
+    Per step: vertex normals, pytorch3d's cot-Laplacian mean curvature and the
+    Dirichlet / Neumann data (one kernel, ``mba_pressure_bc``), the Laplace
+    BEM solve of ``MeshBEMSolver3d.solve`` (the reference's rule, Dirichlet
+    vertices), then grad p = dp/dn n + the tangential gradient of p and
+    new_vel = vel - (dt / rho) grad p (one kernel, ``mba_pressure_update``)."""
+
+    def __init__(self, density_fluid: float, gravity_strength: float, surface_tension, contact_angle,
+                 gaussQR: int, order_type: int, logger=None, device=torch.device("cpu"),
+                 dtype: torch.dtype = torch.float32) -> None:
+        from .logger import Logger
+        from .solver import MeshBEMSolver3d
+        from .types import BEMType
+
+        self.gaussQR = gaussQR
+        self.order_type = order_type
+        self.logger = logger if logger is not None else Logger(quiet=True)
+        self.device = device
+        self.dtype = dtype
+        self.density_fluid = float(density_fluid)
+        self.gravity_strength = float(gravity_strength)
+        self.gravity = torch.tensor([0.0, -gravity_strength, 0.0], dtype=dtype, device=device)
+        self.surface_tension = float(surface_tension)
+        self.contact_angle = contact_angle
+        self.BEM_solver = MeshBEMSolver3d(gaussQR=gaussQR, order_type=order_type, BEM_type=int(BEMType.LAPLACE),
+                                          wavenumber=1.0, logger=self.logger, device=device, dtype=dtype)
+        self.last = {}
+
+    def boundary_data(self, verts: torch.Tensor, faces: torch.Tensor, dt: float, vel: torch.Tensor,
+                      vert_bc_flags: torch.Tensor, P_mag: torch.Tensor = None):
+        """(vertex normals [V, 3], kappa [V], Dirichlet p [B, V], Neumann dp/dn [B, V])."""
+        from .mom import vertex_incidence
+
+        lib = _lib.load()
+        rdt = verts.dtype
+        dev = verts.device
+        V, F = verts.shape[0], faces.shape[0]
+        B = vel.shape[0]
+        ft = faces.to(torch.int32).contiguous()
+        vptr, vidx = vertex_incidence(ft.cpu().numpy(), V)
+        self._vinc = (torch.from_numpy(vptr).to(dev), torch.from_numpy(vidx).to(dev))
+        u = vel.to(rdt).reshape(B, V, 3).contiguous()
+        fl = vert_bc_flags.reshape(B, V).to(torch.uint8).contiguous()
+        pm = None
+        if P_mag is not None:
+            pm = P_mag.to(rdt).reshape(-1, V).expand(B, V).contiguous()
+        vn = torch.empty((V, 3), dtype=rdt, device=dev)
+        kap = torch.empty(V, dtype=rdt, device=dev)
+        pd = torch.empty((B, V), dtype=rdt, device=dev)
+        dn = torch.empty((B, V), dtype=rdt, device=dev)
+        st = lib.mba_pressure_bc(_lib.prec_of(rdt), _lib.ptr(verts.contiguous()), _lib.ptr(ft), V, F,
+                                 _lib.ptr(self._vinc[0]), _lib.ptr(self._vinc[1]), _lib.ptr(u), _lib.ptr(fl),
+                                 _lib.ptr(pm) if pm is not None else None, B, self.density_fluid,
+                                 self.gravity_strength, self.surface_tension, float(dt), _lib.ptr(vn), _lib.ptr(kap),
+                                 _lib.ptr(pd), _lib.ptr(dn), _lib.stream())
+        _lib.check(st, "mba_pressure_bc")
+        return vn, kap, pd, dn
+
+    def solve(self, verts: torch.Tensor, faces: torch.Tensor, dt: float, vel: torch.Tensor,
+              vert_bc_flags: torch.Tensor, P_mag: torch.Tensor = None) -> torch.Tensor:
+        if (vert_bc_flags == TRIPLE_JUNCTION).sum().item() > 0:
+            self.logger.ErrorLog("Not implemented for triple-junction points")
+        lib = _lib.load()
+        rdt = verts.dtype
+        V, F = verts.shape[0], faces.shape[0]
+        B = vel.shape[0]
+        vn, kap, pd, dn = self.boundary_data(verts, faces, dt, vel, vert_bc_flags, P_mag)
+        vert_dpdn, vert_p = self.BEM_solver.solve(verts=verts, faces=faces, vert_bc_Dirichlet=pd.reshape(B, V, 1),
+                                                   vert_bc_Neumann=dn.reshape(B, V, 1),
+                                                   vert_bc_flags=vert_bc_flags.reshape(B, V, 1), Amat_scaler=1.0)
+        p = vert_p.reshape(B, V).to(rdt).contiguous()
+        q = vert_dpdn.reshape(B, V).to(rdt).contiguous()
+        u = vel.to(rdt).reshape(B, V, 3).contiguous()
+        out = torch.empty_like(u)
+        ft = faces.to(torch.int32).contiguous()
+        st = lib.mba_pressure_update(_lib.prec_of(rdt), _lib.ptr(verts.contiguous()), _lib.ptr(ft), V, F,
+                                     _lib.ptr(self._vinc[0]), _lib.ptr(self._vinc[1]), _lib.ptr(vn), _lib.ptr(p),
+                                     _lib.ptr(q), _lib.ptr(u), B, self.density_fluid, float(dt), _lib.ptr(out),
+                                     _lib.stream())
+        _lib.check(st, "mba_pressure_update")
+        self.last = {"kappa": kap, "p": p, "dpdn": q, "vert_normals": vn}
+        return out.to(vel.dtype)

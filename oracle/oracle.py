@@ -722,3 +722,90 @@ def bem2d_dirichlet(verts, faces, g, gaussQR, order_type=1, bem_type=0, k=1.0):
         rhs = 0.5 * g * va + Kt.T @ g
     t = scipy.linalg.solve(Vt, rhs)
     return t, Vt, rhs
+
+
+# ---------------------------------------------------------------- ferrofluid pressure step
+def cot_laplacian_p3d(verts, faces, eps=1e-12):
+    """pytorch3d.ops.cot_laplacian (the dependency of mesh_pressure.py:111-112;
+    pytorch3d from git, unpinned in readme.md:14), restated from its published
+    algorithm: Heron areas with the squared area clamped at eps, cot of the
+    angle at vertex k = (sum of the other two squared sides - own squared
+    side) / (4 area), L[i, j] = sum over faces of the cot opposite edge (i, j)
+    (symmetric), inv_areas = 1 / (sum of the areas of the faces at a vertex).
+    Returns (L dense [V, V], inv_areas [V])."""
+    V = np.asarray(verts, dtype=np.float64)
+    F = np.asarray(faces, dtype=np.int64)
+    v0, v1, v2 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    A = np.linalg.norm(v1 - v2, axis=1)
+    B = np.linalg.norm(v0 - v2, axis=1)
+    C = np.linalg.norm(v0 - v1, axis=1)
+    s = 0.5 * (A + B + C)
+    area = np.sqrt(np.maximum(s * (s - A) * (s - B) * (s - C), eps))
+    A2, B2, C2 = A * A, B * B, C * C
+    cot = np.stack([(B2 + C2 - A2) / area, (A2 + C2 - B2) / area, (A2 + B2 - C2) / area], 1) / 4.0
+    nV = len(V)
+    L = np.zeros((nV, nV))
+    np.add.at(L, (F[:, 1], F[:, 2]), cot[:, 0])
+    np.add.at(L, (F[:, 2], F[:, 0]), cot[:, 1])
+    np.add.at(L, (F[:, 0], F[:, 1]), cot[:, 2])
+    L = L + L.T
+    asum = np.zeros(nV)
+    for t in range(3):
+        np.add.at(asum, F[:, t], area)
+    inv = np.where(asum > 0, 1.0 / np.where(asum > 0, asum, 1.0), 0.0)
+    return L, inv
+
+
+def vertex_normals_area(verts, faces):
+    """compute_vert_normals (mesh_utils.py:242-274): normalise(sum_f A_f n_f)."""
+    V = np.asarray(verts, dtype=np.float64)
+    F = np.asarray(faces, dtype=np.int64)
+    A, n = face_areas_normals(V, F)
+    vn = np.zeros_like(V)
+    for t in range(3):
+        np.add.at(vn, F[:, t], A[:, None] * n)
+    return vn / np.linalg.norm(vn, axis=1, keepdims=True)
+
+
+def signed_mean_curvature_p3d(verts, faces):
+    """kappa of mesh_pressure.py:113-124: lv_j = 0.5 inv_area_j sum_i L_ij (v_i - v_j),
+    kappa = 0.5 |lv| sign(lv . n_j)."""
+    V = np.asarray(verts, dtype=np.float64)
+    L, inv = cot_laplacian_p3d(V, faces)
+    lv = 0.5 * inv[:, None] * (L.T @ V - L.sum(axis=0)[:, None] * V)
+    vn = vertex_normals_area(V, faces)
+    return 0.5 * np.linalg.norm(lv, axis=1) * np.sign((lv * vn).sum(1))
+
+
+def vert_tangent_gradient(p, verts, faces):
+    """compute_mesh_vert_tangent_gradient (mesh_utils.py:501-615) for p [V]:
+    per face sum_t p_t (n x e_t) / (2 A), e_t = v_{t+2} - v_{t+1}, shared by
+    its three vertices with weight 1/3."""
+    V = np.asarray(verts, dtype=np.float64)
+    F = np.asarray(faces, dtype=np.int64)
+    A, n = face_areas_normals(V, F)
+    g = np.zeros((len(F), 3))
+    for t in range(3):
+        e = V[F[:, (t + 2) % 3]] - V[F[:, (t + 1) % 3]]
+        g += p[F[:, t]][:, None] * np.cross(n, e)
+    g = g / 2.0 / A[:, None]
+    out = np.zeros_like(V)
+    for t in range(3):
+        np.add.at(out, F[:, t], g / 3.0)
+    return out
+
+
+def pressure_step(verts, faces, vel, dt, rho, g, sigma, gaussQR, order_type=1, p_mag=None):
+    """MeshPressure.solve with every vertex DIRICHLET (the droplet demo,
+    demo_3d_mesh_droplet.py:140-150): p = rho g v_y - P_mag + sigma kappa,
+    Laplace Dirichlet solve for dp/dn (ref_laplace_dirichlet), new_vel =
+    vel - (dt / rho)(dpdn n + tangential grad p).  vel [V, 3]."""
+    V = np.asarray(verts, dtype=np.float64)
+    kap = signed_mean_curvature_p3d(V, faces)
+    p = rho * g * V[:, 1] + sigma * kap
+    if p_mag is not None:
+        p = p - np.asarray(p_mag).reshape(-1)
+    dpdn = np.asarray(ref_laplace_dirichlet(V, faces, p, gaussQR, order_type)[0]).reshape(-1)
+    vn = vertex_normals_area(V, faces)
+    grad = dpdn[:, None] * vn + vert_tangent_gradient(p, V, faces)
+    return vel - (dt / rho) * grad, p, dpdn
