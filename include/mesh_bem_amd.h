@@ -314,6 +314,12 @@ int mba_gemm_split_f16(int64_t m, int64_t n, int64_t k, double alpha, const void
                        const void *b, int64_t ldb, double beta, void *c, int64_t ldc, void *workspace,
                        size_t workspace_bytes, mba_stream_t stream);
 
+/* Workgroup tile of mba_gemm_split_f16 (and of the LU's split update): 0 =
+ * 256 x 128 complex with 8 waves (one workgroup per CU, default), 1 = 128 x
+ * 128 with 4 waves (two workgroups per CU).  Process-wide tuning knob, like
+ * mba_gemm_set_variant.  No counterpart in the reference. */
+int mba_gemm_split_f16_set_tile(int tile);
+
 /* Trailing-update product of the complex64 LU (mba_lu_factor and its
  * asynchronous / profiled forms): 0 = FP32 MFMA (the mba_gemm_set_variant
  * choice, default), 1 = the split-FP16 product of mba_gemm_split_f16 for

@@ -3138,6 +3138,8 @@ int mba_gemm_set_variant(int prec, int variant)
     return MBA_OK;
 }
 
+int mba_gemm_split_f16_set_tile(int tile) { return mba_h3::set_tile(tile); }
+
 int mba_lu_set_update(int kind)
 {
     if (kind != 0 && kind != 1) return MBA_ERR_ARG;

@@ -33,8 +33,11 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int KC = 16;                  // k per chunk (one v_mfma_f32_32x32x16_f16)
 constexpr int REC_H = 4 * KC;           // halves per record
-constexpr int BM = 256, BN = 128;       // workgroup tile (complex)
-constexpr int THREADS = 512;            // 8 waves: 4 (rows) x 2 (columns), 64 x 64 each
+constexpr int BM = 256, BN = 128;       // largest workgroup tile (complex); operand padding unit
+// Tile variants: 256 x 128 with 8 waves (4 x 2, 64 x 64 each, one workgroup
+// per CU) or 128 x 128 with 4 waves (2 x 2, two workgroups per CU, so one
+// workgroup's C read / write overlaps the other's MFMA work).
+int g_tile = 0;
 constexpr int PITCH_H = REC_H + 8;      // LDS row pitch in halves (144 B: conflict-free 16-byte reads)
 constexpr int NUM_XCD = 8, GROUP_M = 8;
 constexpr int SCALE_TOP = 15;           // scaled maxima in [2^14, 2^15)
@@ -163,44 +166,46 @@ __device__ __forceinline__ bool tile_of(int64_t L, int64_t nbm, int64_t nbn, int
 }
 
 // C[M x N] = alpha * 2^(ea_i + eb_j) * (A' B')_ij + beta * C   (complex64, row-major)
-__global__ __launch_bounds__(THREADS) void k_cgemm_h3(int64_t M, int64_t N, int nrec,
+template <int BMV, int NT>
+__global__ __launch_bounds__(NT, 2) void k_cgemm_h3(int64_t M, int64_t N, int nrec,
                                                       const _Float16* __restrict__ Ap,
                                                       const int32_t* __restrict__ ea,
                                                       const _Float16* __restrict__ Bp,
                                                       const int32_t* __restrict__ eb, float alpha, float beta,
                                                       cpx<float>* __restrict__ C, int64_t ldc)
 {
-    __shared__ alignas(16) _Float16 As[2][BM * PITCH_H];
+    constexpr int RSTEP = NT / 8, QA = BMV / RSTEP, QB = BN / RSTEP;
+    __shared__ alignas(16) _Float16 As[2][BMV * PITCH_H];
     __shared__ alignas(16) _Float16 Bs[2][BN * PITCH_H];
-    const int64_t nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+    const int64_t nbm = (M + BMV - 1) / BMV, nbn = (N + BN - 1) / BN;
     int64_t tm, tn;
     if (!tile_of(blockIdx.x, nbm, nbn, tm, tn)) return;
-    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int64_t m0 = tm * BMV, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int l32 = lane & 31, oct = lane >> 5;
 
-    // global -> register staging: A 256 records x 8 parts of 16 B (4 per
-    // thread), B 128 records x 8 parts (2 per thread); part = tid % 8
-    const int part = tid & 7, rsub = tid >> 3;  // rsub in [0, 64)
-    half8 ga[4], gb[2];
+    // global -> register staging: A BMV records x 8 parts of 16 B, B BN
+    // records x 8 parts; part = tid % 8, record tid / 8 + RSTEP q
+    const int part = tid & 7, rsub = tid >> 3;  // rsub in [0, RSTEP)
+    half8 ga[QA], gb[QB];
     auto gload = [&](int rc) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int64_t row = m0 + rsub + 64 * q;
+        for (int q = 0; q < QA; ++q) {
+            const int64_t row = m0 + rsub + RSTEP * q;
             ga[q] = *(const half8*)(Ap + (row * nrec + rc) * REC_H + part * 8);
         }
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int64_t col = n0 + rsub + 64 * q;
+        for (int q = 0; q < QB; ++q) {
+            const int64_t col = n0 + rsub + RSTEP * q;
             gb[q] = *(const half8*)(Bp + (col * nrec + rc) * REC_H + part * 8);
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) *(half8*)&As[buf][(rsub + 64 * q) * PITCH_H + part * 8] = ga[q];
+        for (int q = 0; q < QA; ++q) *(half8*)&As[buf][(rsub + RSTEP * q) * PITCH_H + part * 8] = ga[q];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) *(half8*)&Bs[buf][(rsub + 64 * q) * PITCH_H + part * 8] = gb[q];
+        for (int q = 0; q < QB; ++q) *(half8*)&Bs[buf][(rsub + RSTEP * q) * PITCH_H + part * 8] = gb[q];
     };
 
     floatx16 acc[2][2][2];  // [i][j][re/im]
@@ -354,11 +359,25 @@ int multiply(int64_t M, int64_t N, int64_t K, float alpha, float beta, void* C, 
     if (K <= 0 || K > KMAX) return MBA_ERR_SIZE;
     if (ws_bytes < workspace_bytes(M, N, K)) return MBA_ERR_ALLOC;
     const Parts P = parts(M, N, K, (void*)ws);
-    const int64_t tiles = (P.Mpad / BM) * (P.Npad / BN);
-    const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
-    hipLaunchKernelGGL(k_cgemm_h3, dim3((unsigned)blocks), dim3(THREADS), 0, st, M, N, P.nrec, P.Ap, P.ea, P.Bp, P.eb,
-                       alpha, beta, (cpx<float>*)C, ldc);
+    if (g_tile == 0) {
+        const int64_t tiles = ((M + 255) / 256) * ((N + BN - 1) / BN);
+        const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+        hipLaunchKernelGGL((k_cgemm_h3<256, 512>), dim3((unsigned)blocks), dim3(512), 0, st, M, N, P.nrec, P.Ap, P.ea,
+                           P.Bp, P.eb, alpha, beta, (cpx<float>*)C, ldc);
+    } else {
+        const int64_t tiles = ((M + 127) / 128) * ((N + BN - 1) / BN);
+        const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
+        hipLaunchKernelGGL((k_cgemm_h3<128, 256>), dim3((unsigned)blocks), dim3(256), 0, st, M, N, P.nrec, P.Ap, P.ea,
+                           P.Bp, P.eb, alpha, beta, (cpx<float>*)C, ldc);
+    }
     MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+int set_tile(int tile)
+{
+    if (tile != 0 && tile != 1) return MBA_ERR_ARG;
+    g_tile = tile;
     return MBA_OK;
 }
 

@@ -18,6 +18,10 @@ size_t workspace_bytes(int64_t M, int64_t N, int64_t K);
 int gemm(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B, int64_t ldb,
          float beta, void* C, int64_t ldc, void* ws, size_t ws_bytes, hipStream_t st);
 
+// Workgroup tile of the product: 0 = 256 x 128 (8 waves, one workgroup per
+// CU; default), 1 = 128 x 128 (4 waves, two per CU).  Process-wide.
+int set_tile(int tile);
+
 // The two stages of gemm(): split A [M x K] and B [K x N] into the workspace,
 // then C = alpha A B + beta C from the split operands.
 int split(int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B, int64_t ldb, void* ws,

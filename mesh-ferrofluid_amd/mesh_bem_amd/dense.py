@@ -58,6 +58,13 @@ def gemm_split_f16(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] =
     return C
 
 
+def set_split_tile(tile: int) -> None:
+    """Workgroup tile of the split-FP16 product (process-wide): 0 = 256 x 128
+    complex (8 waves, one workgroup per CU; default), 1 = 128 x 128 (4 waves,
+    two workgroups per CU)."""
+    _lib.check(_lib.load().mba_gemm_split_f16_set_tile(int(tile)), "mba_gemm_split_f16_set_tile")
+
+
 LU_UPDATES = {"fp32": 0, "split_f16": 1}
 
 

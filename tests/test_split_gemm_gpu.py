@@ -13,7 +13,7 @@ import pytest
 import scipy.linalg
 import torch
 
-from mesh_bem_amd.dense import LUFactor, gemm_split_f16, refine_solve, set_lu_update
+from mesh_bem_amd.dense import LUFactor, gemm_split_f16, refine_solve, set_lu_update, set_split_tile
 
 pytestmark = pytest.mark.gpu
 
@@ -111,3 +111,21 @@ def test_lu_with_split_update(dev, n):
     ref = scipy.linalg.solve(a, b[:, 0])
     assert info.converged
     assert np.linalg.norm(J.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-11
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+@pytest.mark.parametrize("M,N,K", [(700, 300, 512), (129, 257, 48), (1000, 1000, 256)])
+def test_split_gemm_tiles(dev, tile, M, N, K):
+    """Both workgroup tiles (256 x 128 with 8 waves, 128 x 128 with 4 waves)."""
+    rng = np.random.default_rng(M + N + K + tile)
+    A = torch.tensor(crand(rng, M, K), dtype=torch.complex64, device=dev)
+    B = torch.tensor(crand(rng, K, N), dtype=torch.complex64, device=dev)
+    C = torch.tensor(crand(rng, M, N), dtype=torch.complex64, device=dev)
+    c_in = C.cpu().numpy().astype(np.complex128)
+    set_split_tile(tile)
+    try:
+        out = gemm_split_f16(A, B, C, alpha=-1.0, beta=1.0).cpu().numpy()
+    finally:
+        set_split_tile(0)
+    ref = -ref_product(A, B) + c_in
+    assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-6
