@@ -327,6 +327,15 @@ int mba_gemm_split_f16_set_tile(int tile);
  * factorising, not concurrently with one.  No counterpart in the reference. */
 int mba_lu_set_update(int kind);
 
+/* Triangular solves of mba_lu_solve for complex64 and nrhs <= 4: 1 = one
+ * persistent launch per triangle (default; workgroups take 64-row blocks in
+ * start order and read solved blocks from epoch-tagged words), 0 = one launch
+ * per 64-row block.  Process-wide; set before solving, not concurrently with
+ * one.  Both give the getrs inside torch.linalg.solve
+ * (src/mesh_BEM/mesh_BEM_solver_3d.py:174); no counterpart of the switch in
+ * the reference. */
+int mba_lu_set_solve(int kind);
+
 /* FP64 residual r = v - Z x for complex128 Z [n, ldz], x/v/r [n, nrhs]. */
 int mba_residual_c128(const void *z, int64_t n, int64_t ldz, const void *x, const void *v,
                       void *r, int64_t nrhs, mba_stream_t stream);

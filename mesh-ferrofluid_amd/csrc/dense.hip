@@ -1924,6 +1924,264 @@ __global__ __launch_bounds__(256) void k_tri_step(const cpx<T>* __restrict__ A, 
     tile_matvec<T, true>(A + i0 * lda + r0, lda, i0, nn, bs, xc, t, nrhs);
 }
 
+// ---- persistent triangular solves (complex64, nrhs <= TS_RHS) ---------------
+// One launch per triangle.  Workgroup q (in start order: a ticket drawn from
+// a counter) owns block row i = q (forward, unit lower) or nblk - 1 - q
+// (backward, upper).  It subtracts A(i, j) x_j over the solved blocks j in
+// solve order, each as soon as x_j appears in a tagged buffer (each 32-bit
+// real travels with the call's epoch in one 64-bit word, so a reader needs no
+// fence and polls exactly the words it reads), then applies the inverted
+// diagonal block and publishes x_i.  A workgroup waits only on lower tickets,
+// which started before it, so every wait ends whatever the dispatch order.
+// Waits are bounded all the same: on expiry *err is set, waiters stop at their
+// next look at it and the affected blocks come out as NaN.
+//
+// Bulk blocks (all but the one solved last before i): wave w takes rows
+// 16w..16w+15, lane l column l of the 64-column block, the next block's loads
+// in flight during the wait; the lane partial sums are reduced once by a
+// halving butterfly (16 rows -> 1 row per lane quad in 6 exchanges).  The last
+// block and the inverted diagonal block sit in registers before the wait
+// (lane = row, wave = 16-column quarter), so the critical path per block is
+// one tagged read, two small products and three barriers.
+constexpr int PS_MAX_NR = TS_RHS;
+
+// Waits until the N tagged words carry `epoch` (null pointers read as 0);
+// returns false when it gave up (timeout or *err set elsewhere).
+template <int N>
+__device__ __forceinline__ bool tag_get_opt(uint64_t* const (&p)[N], uint32_t epoch, int* err, uint32_t (&out)[N])
+{
+    uint64_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        w[i] = p[i] ? __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ((uint64_t)epoch << 32);
+    long spins = 0;
+    bool good = true;
+    while (true) {
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i) ok = ok && (uint32_t)(w[i] >> 32) == epoch;
+        if (ok) break;
+        if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            good = false;
+            break;
+        }
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            good = false;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if ((uint32_t)(w[i] >> 32) != epoch)
+                w[i] = __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (uint32_t)w[i];
+    return good;
+}
+
+// t = P b, tagged words := 0, tickets and error word := 0
+__global__ void k_ps_prep(const cpx<float>* __restrict__ b, int64_t ldb, const int32_t* __restrict__ perm,
+                          int64_t n, int nrhs, cpx<float>* __restrict__ t, uint64_t* __restrict__ tag,
+                          int64_t tag_words, int* __restrict__ ctl)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n * nrhs) {
+        const int64_t i = id / nrhs, c = id % nrhs;
+        t[i * nrhs + c] = b[(int64_t)perm[i] * ldb + c];
+    }
+    if (id < tag_words) tag[id] = 0;
+    if (id < 4) ctl[id] = 0;
+}
+
+// rhs [n][ldr], out [n][ldo]; tag [n][NR][2] words; ctl[0] ticket, ctl[2] error
+template <int NR, bool LOWER>
+__global__ __launch_bounds__(256) void k_trisolve_persist(const cpx<float>* __restrict__ A, int64_t lda, int64_t n,
+                                                          const cpx<float>* __restrict__ inv,
+                                                          const cpx<float>* __restrict__ rhs, int64_t ldr,
+                                                          cpx<float>* __restrict__ out, int64_t ldo,
+                                                          uint64_t* __restrict__ tag, uint32_t epoch,
+                                                          int* __restrict__ ctl)
+{
+    __shared__ int s_ticket, s_bad;
+    __shared__ cpx<float> s_bulk[TS_BS][NR];
+    __shared__ cpx<float> s_part[4][TS_BS][NR];
+    __shared__ cpx<float> s_vec[TS_BS][NR];
+    const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    int* err = ctl + 2;
+    const int nblk = (int)((n + TS_BS - 1) / TS_BS);
+    if (tid == 0) {
+        s_ticket = atomicAdd(ctl + (LOWER ? 0 : 1), 1);
+        s_bad = 0;
+    }
+    __syncthreads();
+    const int q = s_ticket;
+    if (q >= nblk) return;  // grid and block count agree; kept as a guard
+    const int i = LOWER ? q : nblk - 1 - q;
+    const int64_t r0 = (int64_t)i * TS_BS;
+    bool bad = false;
+
+    // critical-path operands: last block jl, inverted diagonal block (row l, quarter w)
+    const int jl = LOWER ? i - 1 : i + 1;
+    const bool has_last = jl >= 0 && jl < nblk;
+    cpx<float> lb[16], iv[16];
+    {
+        const bool rv = r0 + l < n;
+        const cpx<float>* ar = A + (rv ? r0 + l : 0) * lda + (int64_t)(has_last ? jl : 0) * TS_BS + 16 * w;
+        const cpx<float>* ir = inv + (int64_t)i * TS_BS * TS_BS + l * TS_BS + 16 * w;
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const bool cv = has_last && rv && (int64_t)jl * TS_BS + 16 * w + cc < n;
+            lb[cc] = cv ? ar[cc] : cpx<float>{0.f, 0.f};
+            iv[cc] = ir[cc];
+        }
+    }
+    cpx<float> rv_[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) rv_[k] = (w == 0 && r0 + l < n) ? rhs[(r0 + l) * ldr + k] : cpx<float>{0.f, 0.f};
+
+    // bulk: blocks s = 0 .. nbulk - 1 in solve order
+    const int nbulk = LOWER ? (i > 1 ? i - 1 : 0) : (nblk - 2 - i > 0 ? nblk - 2 - i : 0);
+    cpx<float> acc[16][NR];
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr)
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[rr][k] = cpx<float>{0.f, 0.f};
+    const int64_t rowb = r0 + 16 * w;
+    auto load_rows = [&](cpx<float>(&a)[16], int s) {
+        const int j = LOWER ? s : nblk - 1 - s;
+        const int64_t c = (int64_t)j * TS_BS + l;
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr)
+            a[rr] = (rowb + rr < n && c < n) ? A[(rowb + rr) * lda + c] : cpx<float>{0.f, 0.f};
+    };
+    cpx<float> a[16], an[16];
+    if (nbulk > 0) load_rows(a, 0);
+    for (int s = 0; s < nbulk; ++s) {
+        if (s + 1 < nbulk) load_rows(an, s + 1);
+        const int j = LOWER ? s : nblk - 1 - s;
+        const int64_t c = (int64_t)j * TS_BS + l;
+        uint64_t* pp[2 * NR];
+#pragma unroll
+        for (int m = 0; m < 2 * NR; ++m) pp[m] = c < n ? tag + c * NR * 2 + m : nullptr;
+        uint32_t wd[2 * NR] = {};
+        if (!bad && !tag_get_opt<2 * NR>(pp, epoch, err, wd)) bad = true;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const float xr = __uint_as_float(wd[2 * k]), xi = __uint_as_float(wd[2 * k + 1]);
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) {
+                acc[rr][k].re += a[rr].re * xr - a[rr].im * xi;
+                acc[rr][k].im += a[rr].re * xi + a[rr].im * xr;
+            }
+        }
+        if (s + 1 < nbulk) {
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) a[rr] = an[rr];
+        }
+    }
+    // halving butterfly: after the step of width h, lanes keep rows [0, h) of
+    // their half; lane quad (l >> 2) ends with row l >> 2 of the wave
+#pragma unroll
+    for (int h = 8, sh = 32; h >= 1; h >>= 1, sh >>= 1) {
+        const bool up = (l & sh) != 0;
+#pragma unroll
+        for (int rr = 0; rr < h; ++rr)
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const cpx<float> lo = acc[rr][k], hi = acc[rr + h][k];
+                const cpx<float> send = up ? lo : hi, keep = up ? hi : lo;
+                acc[rr][k].re = keep.re + __shfl_xor(send.re, sh);
+                acc[rr][k].im = keep.im + __shfl_xor(send.im, sh);
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        acc[0][k].re += __shfl_xor(acc[0][k].re, 2);
+        acc[0][k].im += __shfl_xor(acc[0][k].im, 2);
+        acc[0][k].re += __shfl_xor(acc[0][k].re, 1);
+        acc[0][k].im += __shfl_xor(acc[0][k].im, 1);
+    }
+    if ((l & 3) == 0)
+#pragma unroll
+        for (int k = 0; k < NR; ++k) s_bulk[16 * w + (l >> 2)][k] = acc[0][k];
+
+    // last block: wave w reads the 16 NR complex of its quarter, one or two
+    // tagged words per lane, and broadcasts them by lane reads
+    constexpr int XW = 32 * NR;
+    constexpr int XPL = (XW + 63) / 64;
+    cpx<float> d[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) d[k] = cpx<float>{0.f, 0.f};
+    if (has_last) {
+        const int64_t v0 = (int64_t)jl * TS_BS + 16 * w;
+        uint64_t* pp[XPL];
+#pragma unroll
+        for (int m = 0; m < XPL; ++m) {
+            const int idx = l + 64 * m;
+            pp[m] = (idx < XW && v0 + idx / (2 * NR) < n) ? tag + v0 * NR * 2 + idx : nullptr;
+        }
+        uint32_t xv[XPL] = {};
+        if (!bad && !tag_get_opt<XPL>(pp, epoch, err, xv)) bad = true;
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc)
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const int ir = (cc * NR + k) * 2, ii = ir + 1;
+                const float xr = __uint_as_float(
+                    (uint32_t)__builtin_amdgcn_readlane((int)xv[ir / 64], ir % 64));
+                const float xi = __uint_as_float(
+                    (uint32_t)__builtin_amdgcn_readlane((int)xv[ii / 64], ii % 64));
+                d[k].re += lb[cc].re * xr - lb[cc].im * xi;
+                d[k].im += lb[cc].re * xi + lb[cc].im * xr;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k) s_part[w][l][k] = d[k];
+    if (bad) s_bad = 1;
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            cpx<float> sv = rv_[k];
+            sv.re -= s_bulk[l][k].re + s_part[0][l][k].re + s_part[1][l][k].re + s_part[2][l][k].re +
+                     s_part[3][l][k].re;
+            sv.im -= s_bulk[l][k].im + s_part[0][l][k].im + s_part[1][l][k].im + s_part[2][l][k].im +
+                     s_part[3][l][k].im;
+            s_vec[l][k] = r0 + l < n ? sv : cpx<float>{0.f, 0.f};
+        }
+    }
+    __syncthreads();
+    // x_i = inv(A_ii) s: quarter w of the columns, then the four partials
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        cpx<float> p = {0.f, 0.f};
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const cpx<float> sv = s_vec[16 * w + cc][k];
+            p.re += iv[cc].re * sv.re - iv[cc].im * sv.im;
+            p.im += iv[cc].re * sv.im + iv[cc].im * sv.re;
+        }
+        s_part[w][l][k] = p;
+    }
+    __syncthreads();
+    if (w == 0 && r0 + l < n) {
+        const bool nan = s_bad != 0;
+        const int64_t row = r0 + l;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            cpx<float> x;
+            x.re = s_part[0][l][k].re + s_part[1][l][k].re + s_part[2][l][k].re + s_part[3][l][k].re;
+            x.im = s_part[0][l][k].im + s_part[1][l][k].im + s_part[2][l][k].im + s_part[3][l][k].im;
+            if (nan) x = cpx<float>{__int_as_float(0x7fc00000), __int_as_float(0x7fc00000)};
+            put_val(tag + (row * NR + k) * 2, epoch, x.re);
+            put_val(tag + (row * NR + k) * 2 + 1, epoch, x.im);
+            out[row * ldo + k] = x;
+        }
+    }
+}
+
 size_t solve_aux_bytes(int prec, int64_t n)
 {
     const size_t esz = prec == MBA_F32 ? 8 : 16;
@@ -2419,6 +2677,60 @@ int lu_solve_fused(const void* lu, int64_t n, int64_t lda, const void* aux, void
     }
     hipLaunchKernelGGL(k_permute_rows<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        (const cpx<T>*)x, nrhs, nullptr, n, nrhs, (cpx<T>*)b, ldb);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+// complex64 triangular solves of mba_lu_solve for nrhs <= TS_RHS: 1 = one
+// persistent launch per triangle (default), 0 = one launch per 64-row block;
+// set by mba_lu_set_solve
+int g_lu_solve_f32 = 1;
+
+size_t persist_ws_bytes(int64_t n, int64_t nrhs)
+{
+    return 2 * al256((size_t)n * nrhs * 8) + al256((size_t)n * nrhs * 16) + 256;
+}
+
+template <int NR>
+void launch_persist(const cpx<float>* A, int64_t lda, int64_t n, const cpx<float>* linv, const cpx<float>* uinv,
+                    const cpx<float>* t, cpx<float>* x, cpx<float>* b, int64_t ldb, uint64_t* tag, int* ctl,
+                    hipStream_t st)
+{
+    const unsigned nblk = (unsigned)((n + TS_BS - 1) / TS_BS);
+    hipLaunchKernelGGL((k_trisolve_persist<NR, true>), dim3(nblk), dim3(256), 0, st, A, lda, n, linv, t,
+                       (int64_t)NR, x, (int64_t)NR, tag, 1u, ctl);
+    hipLaunchKernelGGL((k_trisolve_persist<NR, false>), dim3(nblk), dim3(256), 0, st, A, lda, n, uinv,
+                       (const cpx<float>*)x, (int64_t)NR, b, ldb, tag, 2u, ctl);
+}
+
+// P A = L U: t = P b, L y = t (y in x), U z = y (z into b).  Asynchronous; a
+// wait that gave up leaves NaN in the affected rows.
+int lu_solve_persist(const void* lu, int64_t n, int64_t lda, const void* aux, void* b, int64_t nrhs, int64_t ldb,
+                     void* ws_ptr, size_t ws_bytes, hipStream_t st)
+{
+    if (ws_bytes < persist_ws_bytes(n, nrhs)) return MBA_ERR_ALLOC;
+    const size_t esz = sizeof(cpx<float>);
+    const int64_t nblk = (n + TS_BS - 1) / TS_BS;
+    const int32_t* perm = (const int32_t*)aux;
+    const cpx<float>* linv = (const cpx<float>*)((const char*)aux + al256((size_t)n * 4));
+    const cpx<float>* uinv = (const cpx<float>*)((const char*)linv + al256((size_t)nblk * TS_BS * TS_BS * esz));
+    const size_t tb = al256((size_t)n * nrhs * esz);
+    cpx<float>* t = (cpx<float>*)ws_ptr;
+    cpx<float>* x = (cpx<float>*)((char*)ws_ptr + tb);
+    uint64_t* tag = (uint64_t*)((char*)ws_ptr + 2 * tb);
+    int* ctl = (int*)((char*)tag + al256((size_t)n * nrhs * 16));
+    const int64_t tag_words = n * nrhs * 2;
+    hipLaunchKernelGGL(k_ps_prep, dim3((unsigned)((tag_words + 255) / 256)), dim3(256), 0, st,
+                       (const cpx<float>*)b, ldb, perm, n, (int)nrhs, t, tag, tag_words, ctl);
+    MBA_LAUNCH_CHECK();
+    const cpx<float>* A = (const cpx<float>*)lu;
+    cpx<float>* B = (cpx<float>*)b;
+    switch (nrhs) {
+    case 1: launch_persist<1>(A, lda, n, linv, uinv, t, x, B, ldb, tag, ctl, st); break;
+    case 2: launch_persist<2>(A, lda, n, linv, uinv, t, x, B, ldb, tag, ctl, st); break;
+    case 3: launch_persist<3>(A, lda, n, linv, uinv, t, x, B, ldb, tag, ctl, st); break;
+    default: launch_persist<4>(A, lda, n, linv, uinv, t, x, B, ldb, tag, ctl, st); break;
+    }
     MBA_LAUNCH_CHECK();
     return MBA_OK;
 }
@@ -3054,7 +3366,8 @@ int mba_lu_pivots_to_perm(const int32_t* ipiv, int64_t n, int32_t* perm, mba_str
 size_t mba_lu_solve_workspace_bytes(int prec, int64_t n, int64_t nrhs)
 {
     if ((prec != MBA_F32 && prec != MBA_F64) || n <= 0 || nrhs <= 0) return 0;
-    return 2 * al256((size_t)n * nrhs * (prec == MBA_F32 ? 8 : 16));
+    if (prec == MBA_F32) return persist_ws_bytes(n, nrhs);
+    return 2 * al256((size_t)n * nrhs * 16);
 }
 
 size_t mba_lu_solve_aux_bytes(int prec, int64_t n)
@@ -3110,6 +3423,8 @@ int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const void* a
     if (lda < n || ldb < nrhs) return MBA_ERR_ARG;
     const int32_t* perm = (const int32_t*)aux;
     if (nrhs <= TS_RHS) {
+        if (prec == MBA_F32 && g_lu_solve_f32 == 1)
+            return lu_solve_persist(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
         if (prec == MBA_F32)
             return lu_solve_fused<float>(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
         return lu_solve_fused<double>(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
@@ -3139,6 +3454,13 @@ int mba_gemm_set_variant(int prec, int variant)
 }
 
 int mba_gemm_split_f16_set_tile(int tile) { return mba_h3::set_tile(tile); }
+
+int mba_lu_set_solve(int kind)
+{
+    if (kind != 0 && kind != 1) return MBA_ERR_ARG;
+    g_lu_solve_f32 = kind;
+    return MBA_OK;
+}
 
 int mba_lu_set_update(int kind)
 {

@@ -76,6 +76,18 @@ def set_lu_update(kind: str) -> None:
     _lib.check(_lib.load().mba_lu_set_update(LU_UPDATES[kind]), "mba_lu_set_update")
 
 
+LU_SOLVES = {"per_block": 0, "persistent": 1}
+
+
+def set_lu_solve(kind: str) -> None:
+    """Triangular solves of complex64 ``LUFactor.solve`` with <= 4 right-hand
+    sides (process-wide): "persistent" (one launch per triangle, default) or
+    "per_block" (one launch per 64-row block)."""
+    if kind not in LU_SOLVES:
+        raise ValueError(f"set_lu_solve: one of {sorted(LU_SOLVES)}")
+    _lib.check(_lib.load().mba_lu_set_solve(LU_SOLVES[kind]), "mba_lu_set_solve")
+
+
 class LUFactor:
     """In-place LU with partial pivoting of a square complex matrix."""
 

@@ -151,3 +151,49 @@ def test_gemm_variants(dev, variant, shape):
     scale = (A.abs().to(torch.float64) @ B.abs().to(torch.float64)).max()
     err = (Cd.cpu().to(torch.complex128) - ref).abs().max() / scale
     assert err < 3e-6, (variant, float(err))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 130, 700])
+@pytest.mark.parametrize("nrhs", [1, 2, 3, 4])
+def test_lu_solve_persistent_matches_per_block(dev, n, nrhs):
+    """complex64 solves with <= 4 right-hand sides: the persistent triangular
+    solves (one launch per triangle) agree with the per-block launches and
+    with LAPACK, for ragged last blocks, and through a strided b (ldb > nrhs)
+    on repeated calls with the same factor."""
+    from mesh_bem_amd.dense import set_lu_solve
+
+    print(f"persistent solve n={n} nrhs={nrhs}", flush=True)
+    rng = np.random.default_rng(100 * n + nrhs)
+    a = crand(rng, n, n)
+    A = torch.tensor(a, dtype=torch.complex64, device=dev)
+    f = LUFactor(A)
+    lib = _lib.load()
+    a64 = A.cpu().numpy().astype(np.complex128)
+    rhs = [crand(rng, n, nrhs) for _ in range(2)]
+    try:
+        outs = {}
+        for kind in ("per_block", "persistent"):
+            set_lu_solve(kind)
+            for rep in range(2):
+                b = rhs[rep]
+                wide = torch.zeros((n, nrhs + 3), dtype=torch.complex64, device=dev)
+                wide[:, :nrhs] = torch.tensor(b, dtype=torch.complex64, device=dev)
+                wsb = lib.mba_lu_solve_workspace_bytes(f.prec, n, nrhs)
+                ws = torch.full((wsb,), 0x7f, dtype=torch.uint8, device=dev)  # stale workspace contents
+                st = lib.mba_lu_solve(f.prec, _lib.ptr(f.lu), n, f.lu.stride(0), _lib.ptr(f.aux),
+                                      _lib.ptr(wide), nrhs, wide.stride(0), _lib.ptr(ws), wsb, _lib.stream())
+                _lib.check(st, "mba_lu_solve")
+                w = wide.cpu().numpy()
+                assert np.all(w[:, nrhs:] == 0), "columns beyond nrhs must stay untouched"
+                x = w[:, :nrhs].astype(np.complex128)
+                assert np.isfinite(x).all()
+                ref = scipy.linalg.solve(a64, b)
+                cond = np.linalg.cond(a64)
+                assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 50 * 6e-8 * cond * np.sqrt(n)
+                outs[(kind, rep)] = x
+        # same factor, same order of operations up to FP32 summation order
+        for rep in range(2):
+            d = np.linalg.norm(outs[("persistent", rep)] - outs[("per_block", rep)])
+            assert d / np.linalg.norm(outs[("per_block", rep)]) < 1e-4
+    finally:
+        set_lu_solve("persistent")
