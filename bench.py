@@ -845,6 +845,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the timed steps")
     ap.add_argument("--lu-update", default="split_f16", choices=["fp32", "split_f16"],
                     help="trailing-update product of the complex64 LU (c2/c1/small)")
+    ap.add_argument("--lu-panel", default="partial", choices=["partial", "tournament"],
+                    help="panel pivoting of the complex64 LU (c2/c1/small)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -856,7 +858,7 @@ def main():
     import torch
 
     from mesh_bem_amd import _lib
-    from mesh_bem_amd.dense import LUFactor, refine_solve, set_lu_update, to_c64
+    from mesh_bem_amd.dense import LUFactor, refine_solve, set_lu_panel, set_lu_update, to_c64
     from mesh_bem_amd.geometry import icosphere
     from mesh_bem_amd.mom import MomMesh, assemble_ns, rhs_ns
 
@@ -887,6 +889,7 @@ def main():
     dirs, pols = [[0.0, 0.0, 1.0]], [[1.0, 0.0, 0.0]]
     F64 = _lib.MBA_F64
     set_lu_update(args.lu_update)
+    set_lu_panel(args.lu_panel)
     Z64 = torch.empty((E, E), dtype=torch.complex128, device=dev)
     stream = torch.cuda.current_stream()
 
@@ -930,6 +933,7 @@ def main():
         "refine_converged": info.converged,
         "refine_fallback": info.fallback or None,
         "lu_update": args.lu_update,
+        "lu_panel": args.lu_panel,
     }
     if not args.no_extra:
         # phase breakdown + roofline of the dominant kernel (trailing GEMM)

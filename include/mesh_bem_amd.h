@@ -327,6 +327,19 @@ int mba_gemm_split_f16_set_tile(int tile);
  * factorising, not concurrently with one.  No counterpart in the reference. */
 int mba_lu_set_update(int kind);
 
+/* Panels of the complex64 LU (mba_lu_factor and its asynchronous / profiled
+ * forms): 0 = partial pivoting in LAPACK's row order (default), 1 =
+ * tournament pivoting for 64-column panels of 2048 .. 128 x (compute units)
+ * rows: each workgroup nominates rows by partial pivoting on its own rows, a
+ * binary tree of partial-pivoting eliminations over the nominees picks the 64
+ * pivot rows (about ten cross-workgroup exchanges per panel instead of two
+ * per column), ipiv holds the equivalent LAPACK interchanges.  The pivot
+ * order differs from LAPACK's; |L| may exceed 1 (mildly), so use it with
+ * iterative refinement.  Process-wide; set before factorising.  No
+ * counterpart in the reference (its getrf is the one inside
+ * torch.linalg.solve, src/mesh_BEM/mesh_BEM_solver_3d.py:174). */
+int mba_lu_set_panel(int kind);
+
 /* Triangular solves of mba_lu_solve for complex64 and nrhs <= 4: 1 = one
  * persistent launch per triangle (default; workgroups take 64-row blocks in
  * start order and read solved blocks from epoch-tagged words), 0 = one launch

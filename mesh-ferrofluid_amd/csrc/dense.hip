@@ -406,6 +406,9 @@ int g_gemm_f32_variant = 4;  // tile choice of the FP32 GEMM (set by mba_gemm_se
 int g_lu_update_f32 = 0;
 // the split product pays off once the trailing block has this many rows and columns
 constexpr int64_t SPLIT_MIN_DIM = 2048;
+// panels of the complex64 LU: 0 = partial pivoting (LAPACK's row order), 1 =
+// tournament pivoting (64-column panels of >= 2048 rows); set by mba_lu_set_panel
+int g_lu_panel_f32 = 0;
 
 int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
               const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
@@ -640,7 +643,7 @@ int gemm<double>(int64_t M, int64_t N, int64_t K, double alpha, const void* A, i
 // each does only after finishing its reads of the first.  Waits are bounded;
 // on expiry the kernel sets *err, stops waiting and runs to completion (its
 // result is garbage and the host reports a timeout).
-constexpr int PANEL_THREADS = 256;
+constexpr int PANEL_THREADS = 512;
 constexpr long SPIN_LIMIT = 1L << 22;
 
 template <class T>
@@ -703,38 +706,104 @@ __device__ __forceinline__ void get_val(uint64_t* p, uint32_t e, int* err, doubl
     v = __longlong_as_double((long long)((hi << 32) | lo));
 }
 
-// Block-wide arg-max of (v desc, r asc) carrying q; result in red_*[0].
+// ---- arg-max of (v desc, r asc) carrying q
+// Within a wave: DPP lane moves (quad permutes lane ^ 1 and lane ^ 2, then the
+// half-row and row mirrors) leave every lane with the best of its 16-lane
+// row; the four rows' results are combined from lane reads.  Every lane of
+// the wave gets the result.  Call with all 64 lanes active.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float x) { return __int_as_float(dpp_i<CTRL>(__float_as_int(x))); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = dpp_i<CTRL>((int)(b & 0xffffffffLL)), hi = dpp_i<CTRL>((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_mov(int32_t x) { return dpp_i<CTRL>(x); }
+
+__device__ __forceinline__ int32_t lane_read(int32_t x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ float lane_read(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_read(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 template <class T>
-__device__ __forceinline__ void block_argmax(T v, int32_t r, int32_t q, T* red_v, int32_t* red_r,
-                                             int32_t* red_q)
+__device__ __forceinline__ void better(T& v, int32_t& r, int32_t& q, T v2, int32_t r2, int32_t q2)
+{
+    if (v2 > v || (v2 == v && r2 < r)) {
+        v = v2;
+        r = r2;
+        q = q2;
+    }
+}
+
+template <int CTRL, class T>
+__device__ __forceinline__ void dpp_step(T& v, int32_t& r, int32_t& q)
+{
+    better(v, r, q, dpp_mov<CTRL>(v), dpp_mov<CTRL>(r), dpp_mov<CTRL>(q));
+}
+
+template <class T>
+__device__ __forceinline__ void wave_argmax(T& v, int32_t& r, int32_t& q)
+{
+    dpp_step<0xB1>(v, r, q);   // quad_perm [1, 0, 3, 2]
+    dpp_step<0x4E>(v, r, q);   // quad_perm [2, 3, 0, 1]
+    dpp_step<0x141>(v, r, q);  // row_half_mirror
+    dpp_step<0x140>(v, r, q);  // row_mirror
+    T bv = lane_read(v, 0);
+    int32_t br = lane_read(r, 0), bq = lane_read(q, 0);
+#pragma unroll
+    for (int l = 16; l < 64; l += 16) better(bv, br, bq, lane_read(v, l), lane_read(r, l), lane_read(q, l));
+    v = bv;
+    r = br;
+    q = bq;
+}
+
+// Block-wide: the waves' results meet in LDS after one barrier and every
+// thread combines them itself.  red_* hold 2 x NW entries; two calls not
+// separated by another barrier must use different slots.
+template <int NW, class T>
+__device__ __forceinline__ void block_argmax(T& v, int32_t& r, int32_t& q, T* red_v, int32_t* red_r,
+                                             int32_t* red_q, int slot)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        T v2 = __shfl_xor(v, off);
-        int32_t r2 = __shfl_xor(r, off);
-        int32_t q2 = __shfl_xor(q, off);
-        if (v2 > v || (v2 == v && r2 < r)) {
-            v = v2;
-            r = r2;
-            q = q2;
-        }
-    }
+    wave_argmax(v, r, q);
+    T* rv = red_v + slot * NW;
+    int32_t* rr = red_r + slot * NW;
+    int32_t* rq = red_q + slot * NW;
     if (lane == 0) {
-        red_v[wave] = v;
-        red_r[wave] = r;
-        red_q[wave] = q;
+        rv[wave] = v;
+        rr[wave] = r;
+        rq[wave] = q;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < PANEL_THREADS / 64; ++w)
-            if (red_v[w] > red_v[0] || (red_v[w] == red_v[0] && red_r[w] < red_r[0])) {
-                red_v[0] = red_v[w];
-                red_r[0] = red_r[w];
-                red_q[0] = red_q[w];
-            }
+    T pv[NW];
+    int32_t pr[NW], pq[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        pv[w] = rv[w];
+        pr[w] = rr[w];
+        pq[w] = rq[w];
     }
-    __syncthreads();
+    v = pv[0];
+    r = pr[0];
+    q = pq[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) better(v, r, q, pv[w], pr[w], pq[w]);
 }
 
 // Waits until all N tagged words carry `epoch`; the N loads are issued
@@ -765,6 +834,42 @@ __device__ __forceinline__ void tag_get_n(uint64_t* const (&p)[N], uint32_t epoc
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) out[i] = (uint32_t)w[i];
+}
+
+// Waits until the N tagged words carry `epoch` (null pointers read as 0);
+// returns false when it gave up (timeout or *err set elsewhere).
+template <int N>
+__device__ __forceinline__ bool tag_get_opt(uint64_t* const (&p)[N], uint32_t epoch, int* err, uint32_t (&out)[N])
+{
+    uint64_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        w[i] = p[i] ? __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ((uint64_t)epoch << 32);
+    long spins = 0;
+    bool good = true;
+    while (true) {
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i) ok = ok && (uint32_t)(w[i] >> 32) == epoch;
+        if (ok) break;
+        if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            good = false;
+            break;
+        }
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            good = false;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if ((uint32_t)(w[i] >> 32) != epoch)
+                w[i] = __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (uint32_t)w[i];
+    return good;
 }
 
 template <class T>
@@ -810,9 +915,9 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     constexpr int32_t NONE = 0x7fffffff;
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* rows = (cpx<T>*)smem_raw;  // [R][nb]
-    __shared__ T red_v[PANEL_THREADS / 64];
-    __shared__ int32_t red_r[PANEL_THREADS / 64];
-    __shared__ int32_t red_q[PANEL_THREADS / 64];
+    __shared__ T red_v[2 * PANEL_THREADS / 64];
+    __shared__ int32_t red_r[2 * PANEL_THREADS / 64];
+    __shared__ int32_t red_q[2 * PANEL_THREADS / 64];
     __shared__ cpx<T> s_prow[128];
     __shared__ cpx<T> s_drow[128];
     __shared__ cpx<T> s_inv;
@@ -841,12 +946,13 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 br = (int32_t)(lr0 + r);
             }
         }
-        block_argmax(bv, br, 0, red_v, red_r, red_q);
-        const int32_t lb = red_r[0];
+        int32_t bq = 0;
+        block_argmax<PANEL_THREADS / 64>(bv, br, bq, red_v, red_r, red_q, 0);
+        const int32_t lb = br;
         const uint32_t e0 = (uint32_t)(P.k + 1);
         if (tid == 0) {
             tag_put(P.meta + g, e0, (uint32_t)lb);
-            put_val(P.cval + (int64_t)g * VW, e0, red_v[0]);
+            put_val(P.cval + (int64_t)g * VW, e0, bv);
         }
         put_row<T>(P.cand + (int64_t)g * row_words, e0, lb != NONE ? rows + (lb - lr0) * nb : nullptr, nb);
         if (owns(0)) put_row<T>(P.diag, e0, rows + (0 - lr0) * nb, nb);
@@ -865,17 +971,11 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             for (int w = 0; w < VW; ++w) pp[1 + w] = P.cval + ((int64_t)par * G + c) * VW + w;
             uint32_t wd[1 + VW];
             tag_get_n<1 + VW>(pp, epoch, P.err, wd);
-            const int32_t cr = (int32_t)wd[0];
-            const T cv = val_of<T>(wd + 1);
-            if (cv > v || (cv == v && cr < r)) {
-                v = cv;
-                r = cr;
-                q = c;
-            }
+            better(v, r, q, val_of<T>(wd + 1), (int32_t)wd[0], (int32_t)c);
         }
-        block_argmax(v, r, q, red_v, red_r, red_q);
-        const int wg = red_q[0];
-        const int32_t p = red_r[0];
+        block_argmax<PANEL_THREADS / 64>(v, r, q, red_v, red_r, red_q, 1);
+        const int wg = q;
+        const int32_t p = r;
         // --- (2) the winner's row and row j
         {
             uint64_t* pw = P.cand + ((int64_t)par * G + wg) * row_words;
@@ -946,13 +1046,14 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             }
         }
         if (j1 >= nb) break;
-        block_argmax(bv, br, 0, red_v, red_r, red_q);  // also orders (3) before (4)
-        const int32_t lb = red_r[0];
+        int32_t bq = 0;
+        block_argmax<PANEL_THREADS / 64>(bv, br, bq, red_v, red_r, red_q, 0);  // its barrier orders (3) before (4)
+        const int32_t lb = br;
         const uint32_t e1 = epoch + 1;
         const int par1 = j1 & 1;
         if (tid == 0) {
             tag_put(P.meta + par1 * G + g, e1, (uint32_t)lb);
-            put_val(P.cval + ((int64_t)par1 * G + g) * VW, e1, red_v[0]);
+            put_val(P.cval + ((int64_t)par1 * G + g) * VW, e1, bv);
         }
         // --- (4) complete and publish the two rows column j + 1 may read:
         // the local candidate lb and row j + 1 (thread c owns column c)
@@ -982,20 +1083,33 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 put_val(dw + (2 * c + 1) * VW, e1, xd.im);
             }
         }
-        // --- (5) rank-1 update of columns > j + 1 for the other rows
+        // --- (5) rank-1 update of columns > j + 1 for the other rows, UB rows
+        // per step: the loads use clamped row indices (no branches), so all
+        // 2 UB LDS reads of a step issue before the first is waited for
         {
+            constexpr int UB = 8;
             const int c = tid % nb;
             const int rstep = PANEL_THREADS / nb;
             if (c > j1 && tid < rstep * nb) {
                 const cpx<T> u = s_prow[c];
-                for (int rr = tid / nb; rr < nr; rr += rstep) {
-                    const int64_t pr = lr0 + rr;
-                    if (pr > j && pr != lb && pr != j1) {
-                        const cpx<T> l = rows[rr * nb + j];
-                        cpx<T> a = rows[rr * nb + c];
-                        a.re -= l.re * u.re - l.im * u.im;
-                        a.im -= l.re * u.im + l.im * u.re;
-                        rows[rr * nb + c] = a;
+                for (int rb = tid / nb; rb < nr; rb += rstep * UB) {
+                    cpx<T> l[UB], a[UB];
+#pragma unroll
+                    for (int q = 0; q < UB; ++q) {
+                        const int rr = rb + q * rstep;
+                        const int rc = rr < nr ? rr : 0;
+                        l[q] = rows[rc * nb + j];
+                        a[q] = rows[rc * nb + c];
+                    }
+#pragma unroll
+                    for (int q = 0; q < UB; ++q) {
+                        const int rr = rb + q * rstep;
+                        const int64_t pr = lr0 + rr;
+                        if (rr < nr && pr > j && pr != lb && pr != j1) {
+                            a[q].re -= l[q].re * u.re - l[q].im * u.im;
+                            a[q].im -= l[q].re * u.im + l[q].im * u.re;
+                            rows[rr * nb + c] = a[q];
+                        }
                     }
                 }
             }
@@ -1028,6 +1142,372 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel_batched(const PanelArgs
     }
     const PanelArgs<T> P = Ps[lo];
     panel_body<T>(P, wg_off[lo + 1] - wg_off[lo], (int)blockIdx.x - wg_off[lo]);
+}
+
+// ============================================================== tournament-pivoting panel
+// complex64 panels of 64 columns and m >= 2048 rows (workgroup g owns local
+// rows [g R, g R + R), 64 <= R <= 128):
+//   1. each workgroup picks <= 64 candidate rows of its own by partial
+//      pivoting on its rows (all rows when it has <= 64 and G > 1);
+//   2. binary tree: at level l workgroup g (g % 2^(l+1) == 0) stacks its
+//      candidates with those of g + 2^l (read as row ids from a tagged
+//      message, the rows themselves from A) and keeps the 64 rows that
+//      partial pivoting on the stack picks; the other workgroup publishes its
+//      candidates and leaves the tree;
+//   3. the root's last elimination is the LU of the 64 chosen rows in pivot
+//      order (L11 \ U11); it writes ipiv as the equivalent sequence of
+//      interchanges and publishes the pivot rows and L11 \ U11;
+//   4. every workgroup solves L21 U11 = A21 for its non-pivot rows and writes
+//      them to their final places; workgroup 0 also writes L11 \ U11 to rows
+//      k .. k + 63 and moves the displaced rows of that range.
+// Rows are read from A (never written before step 4, which starts only after
+// the whole tree has read), so messages carry row ids only.  A row leaving
+// rows k .. k + 63 goes to the former place of a pivot row, whose owner never
+// uses that place's old contents.  Tagged words and bounded waits as in the
+// partial-pivoting panel; epochs: (k / 64 + 1) * 16 + level, + 15 for the
+// root message.
+constexpr int TP_NB = 64;
+constexpr int TP_RMAX = 128;
+constexpr int TP_PITCH = TP_NB + 2;  // LDS row pitch (complex): lanes (r, h) hit distinct banks
+constexpr int TP_THREADS = 256;
+constexpr int64_t TP_MIN_M = 2048;
+constexpr int TP_CAND_W = TP_NB + 1;                    // words per candidate set: count, ids
+constexpr int TP_ROOT_W = TP_NB + TP_NB * TP_NB * 2 + 1;  // pivot ids, L11 \ U11 (re, im words), ready
+
+struct TpArgs {
+    cpx<float>* A;
+    int64_t lda;
+    int64_t k;      // first row / column of the panel
+    int64_t m;      // rows (n - k)
+    int R;          // rows per workgroup
+    int32_t* ipiv;  // global [n]
+    int32_t* info;
+    int* err;
+    uint64_t* cand;  // [G][TP_CAND_W]
+    uint64_t* root;  // [TP_ROOT_W]
+    uint32_t epoch0;
+};
+
+// One thread of a workgroup waits for one tagged word, polling every ~0.4 us:
+// long waits (the tree, the root message) then cost one load per workgroup
+// per poll instead of one per waiting word and thread, which would flood the
+// memory system while the tree works.  The words of the message are read
+// afterwards with the usual bounded waits (they are then nearly always there).
+__device__ __forceinline__ void tp_wait_word(uint64_t* p, uint32_t epoch, int* err)
+{
+    if (threadIdx.x == 0) {
+        long spins = 0;
+        while ((uint32_t)(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) != epoch) {
+            if ((spins & 15) == 15 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            if (++spins > SPIN_LIMIT) {
+                __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(16);
+        }
+    }
+    __syncthreads();
+}
+
+// Partial pivoting on F[nrows][TP_NB] (row ids ids[], panel-local): for
+// j < min(nrows, 64) the unselected row of largest |.|_1 in column j (ties:
+// smaller id) becomes pivot j (sel[j] = its F row); the other unselected rows
+// get their multiplier in column j and their columns > j updated.  Returns
+// the first column with an exactly zero pivot, or -1.
+__device__ int tp_gepp(cpx<float>* F, const int32_t* ids, int nrows, int32_t* sel, int32_t* flag, float* red_v,
+                       int32_t* red_r, int32_t* red_q)
+{
+    const int tid = threadIdx.x;
+    const int npiv = nrows < TP_NB ? nrows : TP_NB;
+    int zero_col = -1;
+    for (int r = tid; r < nrows; r += TP_THREADS) flag[r] = 0;
+    __syncthreads();
+    for (int j = 0; j < npiv; ++j) {
+        float v = -1.f;
+        int32_t id = 0x7fffffff, q = -1;
+        for (int r = tid; r < nrows; r += TP_THREADS)
+            if (!flag[r]) {
+                const float a = cabs1(F[r * TP_PITCH + j]);
+                if (a > v || (a == v && ids[r] < id)) {
+                    v = a;
+                    id = ids[r];
+                    q = r;
+                }
+            }
+        block_argmax<TP_THREADS / 64>(v, id, q, red_v, red_r, red_q, 0);
+        const int s = q;
+        const cpx<float> piv = F[s * TP_PITCH + j];
+        const float d = piv.re * piv.re + piv.im * piv.im;
+        cpx<float> inv = {0.f, 0.f};
+        if (d == 0.f) {
+            if (zero_col < 0) zero_col = j;
+        } else {
+            inv.re = piv.re / d;
+            inv.im = -piv.im / d;
+        }
+        if (tid == 0) {
+            sel[j] = s;
+            flag[s] = 1;
+        }
+        // thread pair per row, columns of one parity each; the fixed-trip
+        // loop (columns <= j masked) lets every LDS read issue up front
+        const int r = tid >> 1, h = tid & 1;
+        if (r < nrows && r != s && !flag[r]) {
+            cpx<float>* Fr = F + r * TP_PITCH;
+            const cpx<float>* Fs = F + s * TP_PITCH;
+            const cpx<float> a = Fr[j];
+            cpx<float> l;
+            l.re = a.re * inv.re - a.im * inv.im;
+            l.im = a.re * inv.im + a.im * inv.re;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                cpx<float> u[TP_NB / 4], x[TP_NB / 4];
+#pragma unroll
+                for (int i = 0; i < TP_NB / 4; ++i) {
+                    const int c = 2 * (i + half * TP_NB / 4) + h;
+                    if (c > j) {
+                        u[i] = Fs[c];
+                        x[i] = Fr[c];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < TP_NB / 4; ++i) {
+                    const int c = 2 * (i + half * TP_NB / 4) + h;
+                    if (c > j) {
+                        x[i].re -= l.re * u[i].re - l.im * u[i].im;
+                        x[i].im -= l.re * u[i].im + l.im * u[i].re;
+                        Fr[c] = x[i];
+                    }
+                }
+            }
+            if (h == 0) Fr[j] = l;  // both lanes of the row (one wave) have read Fr[j]
+        }
+        __syncthreads();
+    }
+    return zero_col;
+}
+
+// panel rows (original values) -> F rows [0, n): row r is ids[r], or row0 + r
+// when ids is null.  All loads of a thread are issued before its LDS stores,
+// so the copy costs one memory latency.
+__device__ __forceinline__ void tp_load_rows(const TpArgs& P, const int32_t* ids, int64_t row0, int n, cpx<float>* F)
+{
+    constexpr int PER = TP_RMAX * TP_NB / TP_THREADS;
+    cpx<float> v[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int idx = threadIdx.x + TP_THREADS * i;
+        const int r = idx / TP_NB, c = idx % TP_NB;
+        if (r < n) {
+            const int64_t row = ids ? (int64_t)ids[r] : row0 + r;
+            v[i] = P.A[(P.k + row) * P.lda + P.k + c];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int idx = threadIdx.x + TP_THREADS * i;
+        const int r = idx / TP_NB, c = idx % TP_NB;
+        if (r < n) F[r * TP_PITCH + c] = v[i];
+    }
+}
+
+__global__ __launch_bounds__(TP_THREADS) void k_panel_tp(TpArgs P)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cpx<float>* F = (cpx<float>*)smem_raw;       // [TP_RMAX][TP_PITCH]
+    cpx<float>* U = F + TP_RMAX * TP_PITCH;      // [TP_NB][TP_PITCH]: L11 \ U11 rows in pivot order
+    __shared__ int32_t ids[TP_RMAX], sel[TP_NB], flag[TP_RMAX], cids[TP_NB], pid[TP_NB], piv_of[TP_RMAX];
+    __shared__ int32_t top[TP_NB], wt[TP_NB];
+    __shared__ float red_v[2 * TP_THREADS / 64];
+    __shared__ int32_t red_r[2 * TP_THREADS / 64], red_q[2 * TP_THREADS / 64];
+    __shared__ int s_cnt;
+    __shared__ cpx<float> s_invd[TP_NB];
+
+    const int tid = threadIdx.x;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int64_t lr0 = (int64_t)g * P.R;
+    const int nr = (int)((P.m - lr0) < P.R ? (P.m - lr0) : P.R);  // >= 1: G = ceil-free split, see host
+
+    // ---- 1. local candidates
+    if (G == 1 || nr > TP_NB) {
+        for (int r = tid; r < nr; r += TP_THREADS) ids[r] = (int32_t)(lr0 + r);
+        tp_load_rows(P, nullptr, lr0, nr, F);
+        __syncthreads();
+        tp_gepp(F, ids, nr, sel, flag, red_v, red_r, red_q);
+        const int cnt = nr < TP_NB ? nr : TP_NB;
+        if (tid < cnt) cids[tid] = ids[sel[tid]];
+        if (tid == 0) s_cnt = cnt;
+    } else {
+        if (tid < nr) cids[tid] = (int32_t)(lr0 + tid);
+        if (tid == 0) s_cnt = nr;
+    }
+    __syncthreads();
+
+    // ---- 2. tree
+    for (int level = 0; (1 << level) < G; ++level) {
+        const uint32_t e = P.epoch0 + (uint32_t)level;
+        if ((g >> level) & 1) {  // publish and leave
+            uint64_t* w = P.cand + (int64_t)g * TP_CAND_W;
+            const int cnt = s_cnt;
+            if (tid < TP_NB) tag_put(w + 1 + tid, e, tid < cnt ? (uint32_t)cids[tid] : 0u);
+            __syncthreads();
+            if (tid == 0) tag_put(w, e, (uint32_t)cnt);
+            break;
+        }
+        const int partner = g + (1 << level);
+        if (partner >= G) continue;
+        // partner's count and ids: all 65 words in one round trip (bounded
+        // waits; after a timeout the words are clamped into range)
+        uint64_t* w = P.cand + (int64_t)partner * TP_CAND_W;
+        __shared__ int32_t s_pw[TP_CAND_W];
+        tp_wait_word(w, e, P.err);  // the count, written after the ids
+        if (tid < TP_CAND_W) s_pw[tid] = (int32_t)tag_get(w + tid, e, P.err);
+        __syncthreads();
+        const int pc = s_pw[0] < 0 ? 0 : (s_pw[0] < TP_NB ? s_pw[0] : TP_NB);
+        const int own = s_cnt;
+        if (tid < own) ids[tid] = cids[tid];
+        if (tid < pc) {
+            const int32_t v = s_pw[1 + tid];
+            ids[own + tid] = (v >= 0 && v < P.m) ? v : 0;
+        }
+        __syncthreads();
+        tp_load_rows(P, ids, 0, own + pc, F);
+        __syncthreads();
+        tp_gepp(F, ids, own + pc, sel, flag, red_v, red_r, red_q);
+        const int cnt = (own + pc) < TP_NB ? (own + pc) : TP_NB;
+        if (tid < cnt) cids[tid] = ids[sel[tid]];
+        if (tid == 0) s_cnt = cnt;
+        __syncthreads();
+    }
+
+    // ---- 3. root: interchanges, zero pivots, message
+    const uint32_t eR = P.epoch0 + 15u;
+    if (g == 0) {
+        // F rows sel[j] hold L11 \ U11 of the chosen rows (the last elimination
+        // was on the final stack, or the local one when G == 1)
+        for (int idx = tid; idx < TP_NB * TP_NB; idx += TP_THREADS) {
+            const int j = idx / TP_NB, c = idx % TP_NB;
+            U[j * TP_PITCH + c] = F[sel[j] * TP_PITCH + c];
+        }
+        if (tid < TP_NB) pid[tid] = cids[tid];
+        __syncthreads();
+        if (tid == 0) {
+            for (int i = 0; i < TP_NB; ++i) {
+                top[i] = i;  // row now at top place i
+                wt[i] = i;   // place of original top row i
+            }
+            int zc = -1;
+            for (int j = 0; j < TP_NB; ++j) {
+                const int32_t x = pid[j];
+                const int32_t src = x < TP_NB ? wt[x] : x;
+                P.ipiv[P.k + j] = (int32_t)(P.k + src);
+                const int32_t rj = top[j];
+                if (src < TP_NB) top[src] = rj;
+                if (rj < TP_NB) wt[rj] = src;
+                top[j] = x;
+                if (x < TP_NB) wt[x] = j;
+                const cpx<float> d = U[j * TP_PITCH + j];
+                if (zc < 0 && d.re == 0.f && d.im == 0.f) zc = j;
+            }
+            if (zc >= 0) atomicCAS(P.info, 0, (int32_t)(P.k + zc + 1));
+        }
+        // publish pivot ids and L11 \ U11
+        if (tid < TP_NB) tag_put(P.root + tid, eR, (uint32_t)pid[tid]);
+        for (int idx = tid; idx < TP_NB * TP_NB; idx += TP_THREADS) {
+            const cpx<float> x = U[(idx / TP_NB) * TP_PITCH + idx % TP_NB];
+            tag_put(P.root + TP_NB + 2 * idx, eR, __float_as_uint(x.re));
+            tag_put(P.root + TP_NB + 2 * idx + 1, eR, __float_as_uint(x.im));
+        }
+        __syncthreads();
+        if (tid == 0) tag_put(P.root + TP_ROOT_W - 1, eR, 1u);  // ready
+    } else {
+        tp_wait_word(P.root + TP_ROOT_W - 1, eR, P.err);
+        // every word of the root message in one round trip: thread t takes
+        // values t + 256 i (i < 16) and pivot id t (t < 64)
+        constexpr int VPT = TP_NB * TP_NB / TP_THREADS;
+        uint64_t* pp[2 * VPT];
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const int idx = tid + TP_THREADS * i;
+            pp[2 * i] = P.root + TP_NB + 2 * idx;
+            pp[2 * i + 1] = P.root + TP_NB + 2 * idx + 1;
+        }
+        uint32_t wd[2 * VPT];
+        tag_get_n<2 * VPT>(pp, eR, P.err, wd);
+#pragma unroll
+        for (int i = 0; i < VPT; ++i) {
+            const int idx = tid + TP_THREADS * i;
+            U[(idx / TP_NB) * TP_PITCH + idx % TP_NB] =
+                cpx<float>{__uint_as_float(wd[2 * i]), __uint_as_float(wd[2 * i + 1])};
+        }
+        if (tid < TP_NB) {
+            const int32_t v = (int32_t)tag_get(P.root + tid, eR, P.err);
+            pid[tid] = (v >= 0 && v < P.m) ? v : 0;
+        }
+    }
+    // ---- 4. L21 U11 = A21 for the non-pivot rows of this workgroup
+    for (int r = tid; r < nr; r += TP_THREADS) piv_of[r] = -1;
+    __syncthreads();
+    if (tid < TP_NB && pid[tid] >= lr0 && pid[tid] < lr0 + nr) piv_of[pid[tid] - lr0] = tid;
+    if (tid < TP_NB) {
+        const cpx<float> d = U[tid * TP_PITCH + tid];
+        const float dd = d.re * d.re + d.im * d.im;
+        s_invd[tid] = dd == 0.f ? cpx<float>{0.f, 0.f} : cpx<float>{d.re / dd, -d.im / dd};
+    }
+    tp_load_rows(P, nullptr, lr0, nr, F);
+    __syncthreads();
+    // forward substitution along the row, thread pair (row, column parity)
+    {
+        for (int r0 = 0; r0 < nr; r0 += TP_THREADS / 2) {
+            const int r = r0 + (tid >> 1), h = tid & 1;
+            if (r < nr && piv_of[r] < 0) {
+                cpx<float>* x = F + r * TP_PITCH;
+                for (int c = 0; c < TP_NB; ++c) {
+                    const cpx<float> a = x[c], di = s_invd[c];
+                    cpx<float> l;
+                    l.re = a.re * di.re - a.im * di.im;
+                    l.im = a.re * di.im + a.im * di.re;
+                    const cpx<float>* u = U + c * TP_PITCH;
+#pragma unroll
+                    for (int half = 0; half < 2; ++half) {
+                        cpx<float> uu[TP_NB / 4], yy[TP_NB / 4];
+#pragma unroll
+                        for (int i = 0; i < TP_NB / 4; ++i) {
+                            const int c2 = 2 * (i + half * TP_NB / 4) + h;
+                            if (c2 > c) {
+                                uu[i] = u[c2];
+                                yy[i] = x[c2];
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < TP_NB / 4; ++i) {
+                            const int c2 = 2 * (i + half * TP_NB / 4) + h;
+                            if (c2 > c) {
+                                yy[i].re -= l.re * uu[i].re - l.im * uu[i].im;
+                                yy[i].im -= l.re * uu[i].im + l.im * uu[i].re;
+                                x[c2] = yy[i];
+                            }
+                        }
+                    }
+                    if (h == 0) x[c] = l;  // both lanes (one wave) have read x[c]
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // write: non-pivot rows to their final places, L11 \ U11 to rows k .. k + 63
+    for (int idx = tid; idx < nr * TP_NB; idx += TP_THREADS) {
+        const int r = idx / TP_NB, c = idx % TP_NB;
+        if (piv_of[r] >= 0) continue;
+        const int64_t x = lr0 + r;
+        const int64_t dst = x < TP_NB ? (int64_t)wt[x] : x;  // only workgroup 0 has x < 64 (R >= 64)
+        P.A[(P.k + dst) * P.lda + P.k + c] = F[r * TP_PITCH + c];
+    }
+    if (g == 0)
+        for (int idx = tid; idx < TP_NB * TP_NB; idx += TP_THREADS) {
+            const int j = idx / TP_NB, c = idx % TP_NB;
+            P.A[(P.k + j) * P.lda + P.k + c] = U[j * TP_PITCH + c];
+        }
 }
 
 // ============================================================== row swaps
@@ -1322,6 +1802,8 @@ struct LuWs {
     uint64_t* cval;
     uint64_t* cand;
     uint64_t* diag;
+    uint64_t* tp_cand;  // tournament panel: candidate sets, root message
+    uint64_t* tp_root;
     int32_t* slot_row;
     int32_t* plan;
     int32_t* plan_n;
@@ -1352,6 +1834,8 @@ LuWs lu_ws_layout(int prec, int64_t n, char* base)
     w.cval = (uint64_t*)take(2 * MAX_G * VW * 8);
     w.cand = (uint64_t*)take(2 * MAX_G * (size_t)nbmax * 2 * VW * 8);
     w.diag = (uint64_t*)take(2 * (size_t)nbmax * 2 * VW * 8);
+    w.tp_cand = (uint64_t*)take((size_t)MAX_G * TP_CAND_W * 8);
+    w.tp_root = (uint64_t*)take((size_t)TP_ROOT_W * 8);
     w.msg_words = (off - msg0) / 8;
     w.slot_row = (int32_t*)take(2 * nbmax * 4);
     w.plan = (int32_t*)take(2 * nbmax * 4);
@@ -1429,12 +1913,17 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     int NB = 128;
     while (NB > 16 && (size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) NB /= 2;
     if ((size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) return MBA_ERR_SIZE;
+    const bool tournament = sizeof(T) == 4 && g_lu_panel_f32 == 1 && NB >= TP_NB;
+    if (tournament) NB = TP_NB;
     static bool attr_set[2] = {false, false};
     if (!attr_set[prec]) {
         (void)hipFuncSetAttribute((const void*)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_budget);
         (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)trsm_lds_bytes<T>(128));
+        if (sizeof(T) == 4)
+            (void)hipFuncSetAttribute((const void*)k_panel_tp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)((TP_RMAX + TP_NB) * TP_PITCH * sizeof(cpx<float>)));
         attr_set[prec] = true;
     }
     // err := 0, every message word := epoch 0 (epochs of this factorisation are k + j + 1 >= 1)
@@ -1460,33 +1949,60 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     // the column ranges [0, k) and [k + nb, swap_hi).
     auto do_panel = [&](int64_t k, int nb, int64_t swap_hi) -> int {
         const int64_t m = n - k;
-        int G = (int)((m + 7) / 8);
-        if (G > cus) G = cus;
-        if (max_wg > 0 && G > max_wg) G = max_wg;
-        if (G > MAX_G) G = MAX_G;
-        int R = (int)((m + G - 1) / G);
-        G = (int)((m + R - 1) / R);
-        PanelArgs<T> P;
-        P.A = A;
-        P.lda = lda;
-        P.k = k;
-        P.m = m;
-        P.nb = nb;
-        P.R = R;
-        P.ipiv = ipiv;
-        P.info = info;
-        P.err = ws.err;
-        P.meta = ws.meta;
-        P.cval = ws.cval;
-        P.cand = ws.cand;
-        P.diag = ws.diag;
-        void* args[] = {&P};
-        size_t shm = (size_t)R * nb * esz;
-        tm.mark(0);
-        if (hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(G), dim3(PANEL_THREADS), args,
-                                       (unsigned)shm, st) != hipSuccess)
-            return MBA_ERR_LAUNCH;
-        tm.mark(0);
+        const int tp_cap = max_wg > 0 && max_wg < cus ? max_wg : cus;
+        if (tournament && nb == TP_NB && m >= TP_MIN_M && m <= (int64_t)TP_RMAX * tp_cap) {
+            // G workgroups of R rows, 64 <= R <= 128, none empty
+            int64_t Gt = m / TP_NB < tp_cap ? m / TP_NB : tp_cap;
+            const int64_t Rt = (m + Gt - 1) / Gt;
+            Gt = (m + Rt - 1) / Rt;
+            TpArgs TP;
+            TP.A = (cpx<float>*)A;
+            TP.lda = lda;
+            TP.k = k;
+            TP.m = m;
+            TP.R = (int)Rt;
+            TP.ipiv = ipiv;
+            TP.info = info;
+            TP.err = ws.err;
+            TP.cand = ws.tp_cand;
+            TP.root = ws.tp_root;
+            TP.epoch0 = (uint32_t)((k / TP_NB + 1) * 16);
+            void* targs[] = {&TP};
+            tm.mark(0);
+            if (hipLaunchCooperativeKernel((const void*)k_panel_tp, dim3((unsigned)Gt), dim3(TP_THREADS), targs,
+                                           (unsigned)((TP_RMAX + TP_NB) * TP_PITCH * sizeof(cpx<float>)),
+                                           st) != hipSuccess)
+                return MBA_ERR_LAUNCH;
+            tm.mark(0);
+        } else {
+            int G = (int)((m + 7) / 8);
+            if (G > cus) G = cus;
+            if (max_wg > 0 && G > max_wg) G = max_wg;
+            if (G > MAX_G) G = MAX_G;
+            int R = (int)((m + G - 1) / G);
+            G = (int)((m + R - 1) / R);
+            PanelArgs<T> P;
+            P.A = A;
+            P.lda = lda;
+            P.k = k;
+            P.m = m;
+            P.nb = nb;
+            P.R = R;
+            P.ipiv = ipiv;
+            P.info = info;
+            P.err = ws.err;
+            P.meta = ws.meta;
+            P.cval = ws.cval;
+            P.cand = ws.cand;
+            P.diag = ws.diag;
+            void* args[] = {&P};
+            size_t shm = (size_t)R * nb * esz;
+            tm.mark(0);
+            if (hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(G), dim3(PANEL_THREADS), args,
+                                           (unsigned)shm, st) != hipSuccess)
+                return MBA_ERR_LAUNCH;
+            tm.mark(0);
+        }
         const int64_t right = swap_hi - (k + nb);
         if (k > 0 || right > 0) {
             tm.mark(1);
@@ -1522,9 +2038,10 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     // the delayed update by the earlier panels and the triangular solve.
     // One trailing update with k = P NB follows.  Row interchanges commute with
     // the delayed (linear) updates because they are applied to every column
-    // before them.  P = 4 with the split-FP16 trailing product (k = 512 halves
-    // its C traffic per flop), 2 otherwise.
-    const int P = (sizeof(T) == 4 && g_lu_update_f32 == 1) ? 4 : 2;
+    // before them.  P NB = 512 with the split-FP16 trailing product (k = 512
+    // halves its C traffic per flop), 256 otherwise (NB = 64 with tournament
+    // panels, else 128 while the LDS allows).
+    const int P = ((sizeof(T) == 4 && g_lu_update_f32 == 1) ? 4 : 2) * (tournament ? 2 : 1);
     for (int64_t k = 0; k < n; k += (int64_t)P * NB) {
         int64_t kp = k;
         int rc;
@@ -1944,42 +2461,6 @@ __global__ __launch_bounds__(256) void k_tri_step(const cpx<T>* __restrict__ A, 
 // (lane = row, wave = 16-column quarter), so the critical path per block is
 // one tagged read, two small products and three barriers.
 constexpr int PS_MAX_NR = TS_RHS;
-
-// Waits until the N tagged words carry `epoch` (null pointers read as 0);
-// returns false when it gave up (timeout or *err set elsewhere).
-template <int N>
-__device__ __forceinline__ bool tag_get_opt(uint64_t* const (&p)[N], uint32_t epoch, int* err, uint32_t (&out)[N])
-{
-    uint64_t w[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-        w[i] = p[i] ? __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ((uint64_t)epoch << 32);
-    long spins = 0;
-    bool good = true;
-    while (true) {
-        bool ok = true;
-#pragma unroll
-        for (int i = 0; i < N; ++i) ok = ok && (uint32_t)(w[i] >> 32) == epoch;
-        if (ok) break;
-        if ((spins & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            good = false;
-            break;
-        }
-        if (++spins > SPIN_LIMIT) {
-            __hip_atomic_fetch_or(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            good = false;
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-        for (int i = 0; i < N; ++i)
-            if ((uint32_t)(w[i] >> 32) != epoch)
-                w[i] = __hip_atomic_load(p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) out[i] = (uint32_t)w[i];
-    return good;
-}
 
 // t = P b, tagged words := 0, tickets and error word := 0
 __global__ void k_ps_prep(const cpx<float>* __restrict__ b, int64_t ldb, const int32_t* __restrict__ perm,
@@ -3454,6 +3935,13 @@ int mba_gemm_set_variant(int prec, int variant)
 }
 
 int mba_gemm_split_f16_set_tile(int tile) { return mba_h3::set_tile(tile); }
+
+int mba_lu_set_panel(int kind)
+{
+    if (kind != 0 && kind != 1) return MBA_ERR_ARG;
+    g_lu_panel_f32 = kind;
+    return MBA_OK;
+}
 
 int mba_lu_set_solve(int kind)
 {

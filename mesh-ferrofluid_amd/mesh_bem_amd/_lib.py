@@ -120,6 +120,7 @@ SIGNATURES = {
                                       _vp]),
     "mba_lu_set_update": (_c.c_int, [_c.c_int]),
     "mba_lu_set_solve": (_c.c_int, [_c.c_int]),
+    "mba_lu_set_panel": (_c.c_int, [_c.c_int]),
     "mba_gemm_split_f16_set_tile": (_c.c_int, [_c.c_int]),
     "mba_residual_c128": (_c.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mba_gemv": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _vp]),

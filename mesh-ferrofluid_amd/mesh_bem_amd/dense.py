@@ -76,6 +76,19 @@ def set_lu_update(kind: str) -> None:
     _lib.check(_lib.load().mba_lu_set_update(LU_UPDATES[kind]), "mba_lu_set_update")
 
 
+LU_PANELS = {"partial": 0, "tournament": 1}
+
+
+def set_lu_panel(kind: str) -> None:
+    """Panels of complex64 factorisations (process-wide): "partial" (partial
+    pivoting, LAPACK's row order, default) or "tournament" (tournament
+    pivoting of 64-column panels with >= 2048 rows: a tree of partial-pivoting
+    eliminations over rows nominated per workgroup; pair it with refinement)."""
+    if kind not in LU_PANELS:
+        raise ValueError(f"set_lu_panel: one of {sorted(LU_PANELS)}")
+    _lib.check(_lib.load().mba_lu_set_panel(LU_PANELS[kind]), "mba_lu_set_panel")
+
+
 LU_SOLVES = {"per_block": 0, "persistent": 1}
 
 

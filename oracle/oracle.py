@@ -809,3 +809,75 @@ def pressure_step(verts, faces, vel, dt, rho, g, sigma, gaussQR, order_type=1, p
     vn = vertex_normals_area(V, faces)
     grad = dpdn[:, None] * vn + vert_tangent_gradient(p, V, faces)
     return vel - (dt / rho) * grad, p, dpdn
+
+
+# ---------------------------------------------------------------- tournament pivoting
+# No reference counterpart: the reference's dense solve (torch.linalg.solve,
+# src/mesh_BEM/mesh_BEM_solver_3d.py:174) factors with partial pivoting.  The
+# product's optional tournament-pivoting panel follows CALU's tall-skinny
+# selection (Grigori, Demmel, Xiang, SIAM J. Matrix Anal. Appl. 32 (2011)
+# 1317-1350) on a fixed binary tree; this restates it for the checker.
+
+def _gepp_select(rows, ids, npiv):
+    """Partial pivoting on a copy of ``rows`` [r, c] (complex); ties go to the
+    smaller id.  Returns the positions (into rows) of the npiv pivots in order
+    and the eliminated copy."""
+    F = np.array(rows, dtype=np.complex128, copy=True)
+    ids = np.asarray(ids)
+    free = np.ones(F.shape[0], bool)
+    order = []
+    for j in range(npiv):
+        mag = np.where(free, np.abs(F[:, j].real) + np.abs(F[:, j].imag), -1.0)
+        best = mag.max()
+        cand = np.flatnonzero(mag == best)
+        s = int(cand[np.argmin(ids[cand])])
+        order.append(s)
+        free[s] = False
+        piv = F[s, j]
+        rest = np.flatnonzero(free)
+        if rest.size:
+            l = F[rest, j] / piv if piv != 0 else np.zeros(rest.size, complex)
+            F[rest, j + 1:] -= l[:, None] * F[s, j + 1:][None, :]
+            F[rest, j] = l
+    return order, F
+
+
+def tslu_pivots(panel, n_cu=256, nb=64):
+    """Tournament pivot rows of an [m, nb] panel (m >= 2 nb): G = min(n_cu,
+    m // nb) leaves of R = ceil(m / G) consecutive rows (G recomputed so none
+    is empty); a leaf keeps all its rows when it has <= nb of them and G > 1,
+    else the nb rows partial pivoting picks; level l merges leaf g with
+    g + 2^l by partial pivoting on the stacked original rows (own candidates
+    first).  Returns (pivot ids in order, LAPACK-style interchanges ipiv
+    [nb] with row j swapped with ipiv[j] >= j in sequence)."""
+    panel = np.asarray(panel)
+    m = panel.shape[0]
+    G = min(n_cu, m // nb)
+    R = -(-m // G)
+    G = -(-m // R)
+    cands = []
+    for g in range(G):
+        ids = list(range(g * R, min(m, (g + 1) * R)))
+        if G == 1 or len(ids) > nb:
+            order, _ = _gepp_select(panel[ids], ids, min(nb, len(ids)))
+            ids = [ids[i] for i in order]
+        cands.append(ids)
+    step = 1
+    while step < G:
+        for g in range(0, G, 2 * step):
+            if g + step < G:
+                ids = cands[g] + cands[g + step]
+                order, _ = _gepp_select(panel[ids], ids, min(nb, len(ids)))
+                cands[g] = [ids[i] for i in order]
+        step *= 2
+    piv = cands[0]
+    rows = list(range(m))  # rows[p] = original row now at place p
+    where = {r: r for r in range(m)}
+    ipiv = []
+    for j, r in enumerate(piv):
+        src = where[r]
+        ipiv.append(src)
+        a = rows[j]
+        rows[j], rows[src] = r, a
+        where[r], where[a] = j, src
+    return piv, ipiv

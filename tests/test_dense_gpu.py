@@ -197,3 +197,76 @@ def test_lu_solve_persistent_matches_per_block(dev, n, nrhs):
             assert d / np.linalg.norm(outs[("per_block", rep)]) < 1e-4
     finally:
         set_lu_solve("persistent")
+
+
+def _interchange(a, ipiv):
+    p = a.copy()
+    for j, q in enumerate(ipiv):
+        if q != j:
+            p[[j, q]] = p[[q, j]]
+    return p
+
+
+@pytest.mark.parametrize("n,update", [(2200, "fp32"), (3000, "fp32"), (4500, "split_f16")])
+def test_lu_tournament_panels(dev, n, update):
+    """complex64 LU with tournament-pivoting panels (64 columns, >= 2048 rows;
+    the later, shorter panels use partial pivoting): ipiv is a valid
+    interchange sequence, the first panel's pivots equal the oracle's
+    tournament on the same values, P A = L U to the accuracy of the
+    partial-pivoting factorisation of the same matrix, and solves meet the
+    FP32 backward-error bound."""
+    import oracle as O
+    from mesh_bem_amd.dense import set_lu_panel, set_lu_update
+
+    print(f"tournament LU n={n} update={update}", flush=True)
+    rng = np.random.default_rng(n + 11)
+    a = crand(rng, n, n)
+    A = torch.tensor(a, dtype=torch.complex64, device=dev)
+    a64 = A.cpu().numpy().astype(np.complex128)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    set_lu_update(update)
+    try:
+        set_lu_panel("tournament")
+        ft = LUFactor(A)
+        set_lu_panel("partial")
+        fp = LUFactor(A)
+    finally:
+        set_lu_panel("partial")
+        set_lu_update("fp32")
+    assert ft.info == 0
+    ipiv = ft.ipiv.cpu().numpy().astype(np.int64)
+    assert np.all(ipiv >= np.arange(n)) and np.all(ipiv < n)
+    _, ip_o = O.tslu_pivots(a64[:, :64], n_cu=cus)
+    assert np.array_equal(ipiv[:64], np.array(ip_o))
+    errs = {}
+    for name, f in (("tournament", ft), ("partial", fp)):
+        lu = f.lu.cpu().numpy().astype(np.complex128)
+        L = np.tril(lu, -1) + np.eye(n)
+        U = np.triu(lu)
+        pa = _interchange(a64, f.ipiv.cpu().numpy().astype(np.int64))
+        errs[name] = np.linalg.norm(pa - L @ U) / np.linalg.norm(a64)
+    print(errs, flush=True)
+    assert errs["tournament"] < 4 * errs["partial"] + 1e-7
+    b = crand(rng, n, 1)
+    x = ft.solve(torch.tensor(b, dtype=torch.complex64, device=dev)).cpu().numpy().astype(np.complex128)
+    bwd = np.linalg.norm(a64 @ x - b) / (np.linalg.norm(a64) * np.linalg.norm(x))
+    assert bwd < 50 * 6e-8 * np.sqrt(n)
+
+
+def test_refinement_with_tournament_panels(dev):
+    """Mixed-precision solve with tournament panels refines to the FP64
+    LAPACK solution."""
+    from mesh_bem_amd.dense import set_lu_panel
+
+    n = 3000
+    rng = np.random.default_rng(7)
+    a = crand(rng, n, n) + 3 * np.sqrt(n) * np.eye(n)
+    b = crand(rng, n)
+    try:
+        set_lu_panel("tournament")
+        J, info = refine_solve(torch.tensor(a, device=dev), torch.tensor(b, device=dev), tol=1e-12)
+    finally:
+        set_lu_panel("partial")
+    ref = scipy.linalg.solve(a, b)
+    assert info.converged and not info.fallback
+    assert np.linalg.norm(J.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-11

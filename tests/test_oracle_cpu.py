@@ -341,3 +341,45 @@ def test_p0_laplace_sphere_eigen_and_demag():
 def test_demag_factors_sum_to_one():
     Na, Nb, Nc = O.demag_factors(1.0, 1.0, 1.7)
     assert abs(Na + Nb + Nc - 1) < 1e-8 and Nc < 1 / 3 < Na
+
+
+@pytest.mark.parametrize("m", [64, 100, 127])
+def test_tslu_single_leaf_is_partial_pivoting(m):
+    """With one leaf (m < 2 x 64) the tournament is partial pivoting on the
+    panel: the interchanges equal LAPACK's zgetrf pivots (scipy)."""
+    import scipy.linalg as sl
+
+    rng = np.random.default_rng(m)
+    pan = rng.standard_normal((m, 64)) + 1j * rng.standard_normal((m, 64))
+    _, ipiv = O.tslu_pivots(pan)
+    assert np.array_equal(np.array(ipiv), sl.lu_factor(pan)[1][:64])
+
+
+@pytest.mark.parametrize("m", [2200, 5000])
+def test_tslu_pivots_give_a_stable_lu(m):
+    """Many leaves: the chosen rows are distinct, the interchanges are a valid
+    LAPACK sequence (ipiv[j] >= j), and the panel's rows brought to the top by
+    them factor without further pivoting with |L| of modest size; a planted
+    dominant row is always chosen first."""
+    rng = np.random.default_rng(m)
+    pan = rng.standard_normal((m, 64)) + 1j * rng.standard_normal((m, 64))
+    pan[1234, 0] = 100.0
+    piv, ipiv = O.tslu_pivots(pan)
+    assert len(set(piv)) == 64 and piv[0] == 1234
+    assert all(ipiv[j] >= j for j in range(64))
+    P = pan.copy()
+    for j in range(64):
+        P[[j, ipiv[j]]] = P[[ipiv[j], j]]
+    # LU of the interchanged panel without pivoting: L21 = A21 U11^-1
+    import scipy.linalg as sl
+
+    # factor the top block in the tournament's order (no interchanges)
+    F = P[:64].copy()
+    for j in range(64):
+        F[j + 1:, j] /= F[j, j]
+        F[j + 1:, j + 1:] -= np.outer(F[j + 1:, j], F[j, j + 1:])
+    U = np.triu(F)
+    L21 = sl.solve_triangular(U, P[64:].T, trans="T", lower=False).T
+    # the tree's last elimination is partial pivoting by |re| + |im|: |l| <= sqrt(2)
+    assert np.abs(np.tril(F, -1)).max() <= math.sqrt(2.0) + 1e-12
+    assert np.abs(L21).max() < 10.0
