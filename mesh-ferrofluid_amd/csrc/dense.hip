@@ -1114,7 +1114,9 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 }
             }
         }
-        __syncthreads();
+        // no barrier here: the next column's gather only reads global words,
+        // and the barrier of its arg-max comes before anything that touches
+        // rows or s_prow / s_drow, so this update overlaps that round trip
     }
     __syncthreads();
     for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
