@@ -1555,7 +1555,8 @@ __global__ void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
 }
 
 // Each block: LASWP_COLS complex columns of the column range; the |S| rows are
-// staged in LDS, swapped in place, written back.
+// staged in LDS and written to their final rows (the interchanges composed
+// into one permutation of the slots).
 constexpr int LASWP_COLS = 16;
 
 template <class T>
@@ -1564,6 +1565,10 @@ __device__ void laswp_body(cpx<T>* __restrict__ A, int64_t lda, int64_t c_lo, in
                            const int32_t* __restrict__ plan, const int32_t* __restrict__ plan_n, int64_t blk,
                            cpx<T>* buf)
 {
+    // 256 threads: thread = (slot group s0 = tid / 16, column c = tid % 16);
+    // |S| <= 2 nb <= 256 slots, so 16 slots per thread
+    constexpr int QS = 256 / (256 / LASWP_COLS);
+    __shared__ int32_t srow[256], pl[256], cur[256], dst[256];
     const int ns = *plan_n;
     // map block -> columns, skipping [skip_lo, skip_hi)
     int64_t c0 = c_lo + blk * LASWP_COLS;
@@ -1573,26 +1578,44 @@ __device__ void laswp_body(cpx<T>* __restrict__ A, int64_t lda, int64_t c_lo, in
     if (c0 < skip_lo && cend > skip_lo) cend = skip_lo;
     if (cend > c_hi) cend = c_hi;
     const int w = (int)(cend - c0);
-    for (int idx = threadIdx.x; idx < ns * LASWP_COLS; idx += blockDim.x) {
-        int s = idx / LASWP_COLS, c = idx % LASWP_COLS;
-        if (c < w) buf[idx] = A[(int64_t)slot_row[s] * lda + c0 + c];
+    const int tid = threadIdx.x;
+    if (tid < ns) {
+        srow[tid] = slot_row[tid];
+        cur[tid] = tid;
     }
+    if (tid < 2 * nb) pl[tid] = plan[tid];
     __syncthreads();
-    if (threadIdx.x < LASWP_COLS && (int)threadIdx.x < w) {
-        const int c = threadIdx.x;
-        for (int j = 0; j < nb; ++j) {
-            int sa = plan[2 * j], sb = plan[2 * j + 1];
-            if (sa != sb) {
-                cpx<T> t = buf[sa * LASWP_COLS + c];
-                buf[sa * LASWP_COLS + c] = buf[sb * LASWP_COLS + c];
-                buf[sb * LASWP_COLS + c] = t;
-            }
+    // the rows' values, all loads of a thread in flight together
+    const int c = tid % LASWP_COLS, s0 = tid / LASWP_COLS;
+    for (int q0 = 0; q0 < QS; q0 += 8) {
+        cpx<T> v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int sl = s0 + (256 / LASWP_COLS) * (q0 + q);
+            v[q] = (sl < ns && c < w) ? A[(int64_t)srow[sl] * lda + c0 + c] : cpx<T>{(T)0, (T)0};
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int sl = s0 + (256 / LASWP_COLS) * (q0 + q);
+            if (sl < ns && c < w) buf[sl * LASWP_COLS + c] = v[q];
         }
     }
+    // the panel's interchanges, in order, on slot labels (one thread): the
+    // same for every column, so done once instead of once per column
+    if (tid == 0)
+        for (int j = 0; j < nb; ++j) {
+            const int sa = pl[2 * j], sb = pl[2 * j + 1];
+            const int t = cur[sa];
+            cur[sa] = cur[sb];
+            cur[sb] = t;
+        }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < ns * LASWP_COLS; idx += blockDim.x) {
-        int s = idx / LASWP_COLS, c = idx % LASWP_COLS;
-        if (c < w) A[(int64_t)slot_row[s] * lda + c0 + c] = buf[idx];
+    if (tid < ns) dst[cur[tid]] = tid;  // place tid now holds original slot cur[tid]
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QS; ++q) {
+        const int sl = s0 + (256 / LASWP_COLS) * q;
+        if (sl < ns && c < w) A[(int64_t)srow[dst[sl]] * lda + c0 + c] = buf[sl * LASWP_COLS + c];
     }
 }
 
@@ -1615,9 +1638,11 @@ __global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t l
 // (thread = (row, 4 columns)), then a 16 x 16 forward substitution runs with
 // one thread per column -- two barriers per block instead of two per row.
 // Columns per workgroup of the panel triangular solve (FP32 / FP64 complex).
+// 32 columns keep X + Ls at 48 KiB (complex64), so three workgroups share a
+// CU and a panel's column blocks run in about one round.
 template <class T>
 struct TrsmCols {
-    static constexpr int value = sizeof(T) == 4 ? 64 : 32;
+    static constexpr int value = 32;
 };
 constexpr int TRSM_RB = 16;
 
@@ -1625,7 +1650,8 @@ constexpr int TRSM_RB = 16;
 // [k, k + nb): X in LDS, L11 read in row blocks of TRSM_RB rows staged in LDS
 // (Ls [TRSM_RB][nb]).  Per block: X(block) -= L(block, done) X(done) by all
 // threads, then the forward substitution inside the block, one thread per
-// column.
+// column.  Global loads are issued in batches (all of a thread's loads
+// before its LDS stores), so each staging step costs one memory latency.
 template <class T>
 __device__ void trsm_body(cpx<T>* __restrict__ A, int64_t lda, int64_t k, int nb, int64_t col_begin,
                           int64_t col_end, int64_t blk, cpx<T>* X /* LDS [nb][TC] */,
@@ -1633,25 +1659,43 @@ __device__ void trsm_body(cpx<T>* __restrict__ A, int64_t lda, int64_t k, int nb
 {
     constexpr int TC = TrsmCols<T>::value;
     constexpr int CPT = TC / 16;  // columns per thread in the update role
+    constexpr int XB = 16;        // X loads in flight per thread
     const int tid = threadIdx.x;
     const int64_t c0 = col_begin + blk * TC;
     const int w = (int)((col_end - c0) < TC ? (col_end - c0) : TC);
-    for (int idx = tid; idx < nb * TC; idx += 256) {
-        int r = idx / TC, c = idx % TC;
-        cpx<T> v;
-        v.re = 0;
-        v.im = 0;
-        if (c < w) v = A[(k + r) * lda + c0 + c];
-        X[idx] = v;
+    for (int base = 0; base < nb * TC; base += 256 * XB) {
+        cpx<T> v[XB];
+#pragma unroll
+        for (int q = 0; q < XB; ++q) {
+            const int idx = base + tid + 256 * q;
+            const int r = idx / TC, c = idx % TC;
+            v[q] = (idx < nb * TC && c < w) ? A[(k + r) * lda + c0 + c] : cpx<T>{(T)0, (T)0};
+        }
+#pragma unroll
+        for (int q = 0; q < XB; ++q) {
+            const int idx = base + tid + 256 * q;
+            if (idx < nb * TC) X[idx] = v[q];
+        }
     }
     const int rr = tid >> 4, cg = (tid & 15) * CPT;  // update role: row rr of the block, columns cg..
+    constexpr int LPT = TRSM_RB * 128 / 256;          // L values per thread (nb <= 128)
     for (int b0 = 0; b0 < nb; b0 += TRSM_RB) {
         const int rb = (nb - b0) < TRSM_RB ? (nb - b0) : TRSM_RB;
         const int width = b0 + rb;  // L columns this block needs
         __syncthreads();            // previous block done with Ls (and X loaded)
-        for (int idx = tid; idx < rb * width; idx += 256) {
-            const int i = idx / width, t = idx % width;
-            Ls[i * nb + t] = A[(k + b0 + i) * lda + k + t];
+        {
+            cpx<T> lv[LPT];
+#pragma unroll
+            for (int q = 0; q < LPT; ++q) {
+                const int idx = tid + 256 * q;
+                const int i = idx / width, t = idx % width;
+                lv[q] = idx < rb * width ? A[(k + b0 + i) * lda + k + t] : cpx<T>{(T)0, (T)0};
+            }
+#pragma unroll
+            for (int q = 0; q < LPT; ++q) {
+                const int idx = tid + 256 * q;
+                if (idx < rb * width) Ls[(idx / width) * nb + idx % width] = lv[q];
+            }
         }
         __syncthreads();
         if (b0 > 0 && rr < rb) {
@@ -1659,13 +1703,17 @@ __device__ void trsm_body(cpx<T>* __restrict__ A, int64_t lda, int64_t k, int nb
             T sr[CPT], si[CPT];
 #pragma unroll
             for (int q = 0; q < CPT; ++q) sr[q] = si[q] = 0;
-            for (int t = 0; t < b0; ++t) {
-                const cpx<T> l = Lr[t];
+            for (int t0 = 0; t0 < b0; t0 += 4) {  // b0 is a multiple of TRSM_RB (16)
 #pragma unroll
-                for (int q = 0; q < CPT; ++q) {
-                    const cpx<T> x = X[t * TC + cg + q];
-                    sr[q] += l.re * x.re - l.im * x.im;
-                    si[q] += l.re * x.im + l.im * x.re;
+                for (int tt = 0; tt < 4; ++tt) {
+                    const int t = t0 + tt;
+                    const cpx<T> l = Lr[t];
+#pragma unroll
+                    for (int q = 0; q < CPT; ++q) {
+                        const cpx<T> x = X[t * TC + cg + q];
+                        sr[q] += l.re * x.re - l.im * x.im;
+                        si[q] += l.re * x.im + l.im * x.re;
+                    }
                 }
             }
 #pragma unroll
@@ -1677,7 +1725,26 @@ __device__ void trsm_body(cpx<T>* __restrict__ A, int64_t lda, int64_t k, int nb
             }
         }
         __syncthreads();
-        if (tid < TC) {
+        if (tid < TC && rb == TRSM_RB) {
+            // full block: the column segment in registers, L rows read from
+            // LDS (one address per wave)
+            const int c = tid;
+            cpx<T> y[TRSM_RB];
+#pragma unroll
+            for (int i = 0; i < TRSM_RB; ++i) y[i] = X[(b0 + i) * TC + c];
+#pragma unroll
+            for (int i = 1; i < TRSM_RB; ++i) {
+                const cpx<T>* Lr = Ls + i * nb + b0;
+#pragma unroll
+                for (int t = 0; t < i; ++t) {
+                    const cpx<T> l = Lr[t];
+                    y[i].re -= l.re * y[t].re - l.im * y[t].im;
+                    y[i].im -= l.re * y[t].im + l.im * y[t].re;
+                }
+            }
+#pragma unroll
+            for (int i = 1; i < TRSM_RB; ++i) X[(b0 + i) * TC + c] = y[i];
+        } else if (tid < TC) {
             const int c = tid;
             for (int i = 1; i < rb; ++i) {
                 const cpx<T>* Lr = Ls + i * nb + b0;
@@ -2111,7 +2178,7 @@ __device__ void ipiv_perm_body(const int32_t* __restrict__ ipiv, int64_t n, int3
         const int32_t v = (c + lane < n) ? ipiv[c + lane] : 0;
         const int cnt = (int)((n - c) < 64 ? (n - c) : 64);
         for (int q = 0; q < cnt; ++q) {
-            const int32_t p = __shfl(v, q);
+            const int32_t p = __builtin_amdgcn_readlane(v, q);  // q is uniform: a scalar read, no LDS trip
             if (lane == 0) {
                 const int64_t j = c + q;
                 int32_t t = P[j];
