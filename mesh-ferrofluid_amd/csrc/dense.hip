@@ -920,7 +920,6 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     __shared__ int32_t red_q[2 * PANEL_THREADS / 64];
     __shared__ cpx<T> s_prow[128];
     __shared__ cpx<T> s_drow[128];
-    __shared__ cpx<T> s_inv;
 
     const int tid = threadIdx.x;
     const int nb = P.nb, R = P.R;
@@ -994,46 +993,43 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             }
         }
         __syncthreads();
-        if (tid == 0) {
-            cpx<T> pv = s_prow[j];
-            T d = pv.re * pv.re + pv.im * pv.im;
-            cpx<T> inv;
+        // every thread forms the pivot's inverse itself (same value everywhere)
+        cpx<T> inv;
+        {
+            const cpx<T> pv = s_prow[j];
+            const T d = pv.re * pv.re + pv.im * pv.im;
             if (d == (T)0) {
                 inv.re = 0;
                 inv.im = 0;
-                if (g == 0) atomicCAS(P.info, 0, (int32_t)(P.k + j + 1));
             } else {
                 inv.re = pv.re / d;
                 inv.im = -pv.im / d;
             }
-            s_inv = inv;
-            if (g == 0) P.ipiv[P.k + j] = (int32_t)(P.k + p);
+            if (tid == 0 && g == 0) {
+                if (d == (T)0) atomicCAS(P.info, 0, (int32_t)(P.k + j + 1));
+                P.ipiv[P.k + j] = (int32_t)(P.k + p);
+            }
         }
-        // swap rows j and p (panel-local indices)
-        if (p != j) {
-            if (owns(p))
-                for (int c = tid; c < nb; c += PANEL_THREADS) rows[(p - lr0) * nb + c] = s_drow[c];
-            if (owns(j))
-                for (int c = tid; c < nb; c += PANEL_THREADS) rows[(j - lr0) * nb + c] = s_prow[c];
-        }
-        __syncthreads();
-        const cpx<T> inv = s_inv;
         const int j1 = j + 1;
         // --- (3) scale column j below the diagonal; update column j + 1 and
-        // take its local arg-max
+        // take its local arg-max.  The interchange of rows j and p is folded
+        // in: place p (if here, p > j) takes old row j, whose columns j and
+        // j + 1 come from s_drow; the interchange writes the other columns
+        // (disjoint from what (3) writes), so no barrier separates them.
         T bv = (T)-1;
         int32_t br = NONE;
         const cpx<T> u1 = j1 < nb ? s_prow[j1] : cpx<T>{(T)0, (T)0};
         for (int rr = tid; rr < nr; rr += PANEL_THREADS) {
             const int64_t pr = lr0 + rr;
             if (pr > j) {
-                const cpx<T> a = rows[rr * nb + j];
+                const bool moved = pr == p;
+                const cpx<T> a = moved ? s_drow[j] : rows[rr * nb + j];
                 cpx<T> l;
                 l.re = a.re * inv.re - a.im * inv.im;
                 l.im = a.re * inv.im + a.im * inv.re;
                 rows[rr * nb + j] = l;
                 if (j1 < nb) {
-                    cpx<T> b = rows[rr * nb + j1];
+                    cpx<T> b = moved ? s_drow[j1] : rows[rr * nb + j1];
                     b.re -= l.re * u1.re - l.im * u1.im;
                     b.im -= l.re * u1.im + l.im * u1.re;
                     rows[rr * nb + j1] = b;
@@ -1044,6 +1040,13 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                     }
                 }
             }
+        }
+        if (p != j) {
+            if (owns(p))
+                for (int c = tid; c < nb; c += PANEL_THREADS)
+                    if (c != j && c != j1) rows[(p - lr0) * nb + c] = s_drow[c];
+            if (owns(j))
+                for (int c = tid; c < nb; c += PANEL_THREADS) rows[(j - lr0) * nb + c] = s_prow[c];
         }
         if (j1 >= nb) break;
         int32_t bq = 0;
