@@ -229,36 +229,34 @@ __global__ __launch_bounds__(NT, 2) void k_cgemm_h3(int64_t M, int64_t N, int nr
 #pragma unroll
             for (int p = 0; p < 4; ++p)
                 a[i][p] = *(const half8*)&As[buf][(wm * 64 + i * 32 + l32) * PITCH_H + p * KC + oct * 8];
+        // B operands of both 32-column blocks, -Bi (hi, lo) by a sign flip of
+        // the packed halves
+        half8 b[2][4], nbh[2], nbl[2];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            half8 b[4];
 #pragma unroll
             for (int p = 0; p < 4; ++p)
-                b[p] = *(const half8*)&Bs[buf][(wn * 64 + j * 32 + l32) * PITCH_H + p * KC + oct * 8];
-            // -Bi (hi, lo): sign flip of the packed halves
-            half8 nbh = -b[1], nbl = -b[3];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                floatx16 re = acc[i][j][0], im = acc[i][j][1];
-                // hi x hi
-                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[0], re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[1], im, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], nbh, re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], b[0], im, 0, 0, 0);
-                // hi x lo
-                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[2], re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][0], b[3], im, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], nbl, re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][1], b[2], im, 0, 0, 0);
-                // lo x hi
-                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][2], b[0], re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][2], b[1], im, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][3], nbh, re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][3], b[0], im, 0, 0, 0);
-                acc[i][j][0] = re;
-                acc[i][j][1] = im;
-            }
+                b[j][p] = *(const half8*)&Bs[buf][(wn * 64 + j * 32 + l32) * PITCH_H + p * KC + oct * 8];
+            nbh[j] = -b[j][1];
+            nbl[j] = -b[j][3];
         }
+        // six steps (hi x hi, hi x lo, lo x hi, each as a real and an
+        // imaginary pair); every step issues one MFMA into each of the eight
+        // accumulators, so consecutive MFMAs never depend on each other.
+        // Each accumulator receives its 12 products in the same order as a
+        // block-by-block loop would give (bit-identical results).
+#define MBA_H3_STEP(AR, BR, AI, BI)                                                                           \
+        _Pragma("unroll") for (int j = 0; j < 2; ++j) _Pragma("unroll") for (int i = 0; i < 2; ++i) {        \
+            acc[i][j][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][AR], BR, acc[i][j][0], 0, 0, 0);       \
+            acc[i][j][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][AI], BI, acc[i][j][1], 0, 0, 0);       \
+        }
+        MBA_H3_STEP(0, b[j][0], 0, b[j][1])  // hi x hi: Ar Br, Ar Bi
+        MBA_H3_STEP(1, nbh[j], 1, b[j][0])   //          -Ai Bi, Ai Br
+        MBA_H3_STEP(0, b[j][2], 0, b[j][3])  // hi x lo
+        MBA_H3_STEP(1, nbl[j], 1, b[j][2])
+        MBA_H3_STEP(2, b[j][0], 2, b[j][1])  // lo x hi
+        MBA_H3_STEP(3, nbh[j], 3, b[j][0])
+#undef MBA_H3_STEP
         if (rc + 1 < nrec) sstore(buf ^ 1);
         __syncthreads();
     }
