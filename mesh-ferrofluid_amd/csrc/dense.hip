@@ -1549,12 +1549,50 @@ __device__ void swap_plan_body(const int32_t* __restrict__ ipiv, int64_t k, int 
     *plan_n = ns;
 }
 
-__global__ void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
-                            int32_t* __restrict__ slot_row, int32_t* __restrict__ plan,
-                            int32_t* __restrict__ plan_n)
+// The same plan by 128 threads (nb <= 128), thread j for swap j: a far pivot
+// row (>= k + nb) takes a new slot at its first appearance among the swaps
+// (found by scanning the earlier pivots in LDS), numbered nb + (first
+// appearances before it, from wave ballots); later appearances reuse it.
+__global__ __launch_bounds__(128) void k_swap_plan(const int32_t* __restrict__ ipiv, int64_t k, int nb,
+                                                   int32_t* __restrict__ slot_row, int32_t* __restrict__ plan,
+                                                   int32_t* __restrict__ plan_n)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    swap_plan_body(ipiv, k, nb, slot_row, plan, plan_n);
+    __shared__ int32_t pv[128], first_of[128], slot_of[128];
+    __shared__ int s_cnt0;
+    const int j = threadIdx.x, lane = j & 63, wave = j >> 6;
+    const bool act = j < nb;
+    const int32_t p = act ? ipiv[k + j] : -1;
+    pv[j] = p;
+    __syncthreads();
+    const bool far = act && p >= k + nb;
+    int f = j;  // first swap with the same pivot row
+    if (far)
+        for (int i = 0; i < j; ++i)
+            if (pv[i] == p) {
+                f = i;
+                break;
+            }
+    const bool is_first = far && f == j;
+    first_of[j] = f;
+    const unsigned long long bal = __ballot(is_first);
+    const int below = __popcll(bal & ((1ull << lane) - 1ull));
+    if (wave == 0 && lane == 0) s_cnt0 = __popcll(bal);
+    __syncthreads();
+    const int rank = below + (wave == 1 ? s_cnt0 : 0);
+    if (is_first) {
+        slot_of[j] = nb + rank;
+        slot_row[nb + rank] = p;
+    }
+    if (act) slot_row[j] = (int32_t)(k + j);
+    __syncthreads();
+    if (act) {
+        plan[2 * j] = j;
+        plan[2 * j + 1] = far ? slot_of[first_of[j]] : (int32_t)(p - k);
+    }
+    if (wave == 1 && lane == 63) {
+        // total = first appearances in both waves
+        *plan_n = nb + s_cnt0 + __popcll(bal);
+    }
 }
 
 // Each block: LASWP_COLS complex columns of the column range; the |S| rows are
@@ -2078,7 +2116,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         const int64_t right = swap_hi - (k + nb);
         if (k > 0 || right > 0) {
             tm.mark(1);
-            hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(64), 0, st, ipiv, k, nb, ws.slot_row,
+            hipLaunchKernelGGL(k_swap_plan, dim3(1), dim3(128), 0, st, ipiv, k, nb, ws.slot_row,
                                ws.plan, ws.plan_n);
             MBA_LAUNCH_CHECK();
             const int64_t blocks = (k + LASWP_COLS - 1) / LASWP_COLS + (right + LASWP_COLS - 1) / LASWP_COLS;
@@ -2180,6 +2218,25 @@ __device__ void ipiv_perm_body(const int32_t* __restrict__ ipiv, int64_t n, int3
     for (int64_t c = 0; c < n; c += 64) {
         const int32_t v = (c + lane < n) ? ipiv[c + lane] : 0;
         const int cnt = (int)((n - c) < 64 ? (n - c) : 64);
+        // 64 interchanges at once when no two of the real ones (p != j) touch
+        // a common place: then their order does not matter
+        const int64_t jl = c + lane;
+        const int32_t pl_ = (jl < n) ? v : (int32_t)jl;
+        const bool real = pl_ != jl;
+        bool conf = false;
+        for (int q = 0; q < 64; ++q) {
+            const int32_t pq = __builtin_amdgcn_readlane(pl_, q);
+            const int64_t jq = c + q;
+            if (q != lane && real && pq != jq && (pq == jl || pq == pl_ || jq == pl_)) conf = true;
+        }
+        if (!__any(conf)) {
+            if (real) {
+                const int32_t a = P[jl], b = P[pl_];
+                P[jl] = b;
+                P[pl_] = a;
+            }
+            continue;
+        }
         for (int q = 0; q < cnt; ++q) {
             const int32_t p = __builtin_amdgcn_readlane(v, q);  // q is uniform: a scalar read, no LDS trip
             if (lane == 0) {
