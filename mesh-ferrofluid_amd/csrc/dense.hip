@@ -2085,7 +2085,9 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                 return MBA_ERR_LAUNCH;
             tm.mark(0);
         } else {
-            int G = (int)((m + 7) / 8);
+            // >= 32 rows per workgroup: fewer candidates to gather and less
+            // spread between workgroups for short panels (measured best)
+            int G = (int)((m + 31) / 32);
             if (G > cus) G = cus;
             if (max_wg > 0 && G > max_wg) G = max_wg;
             if (G > MAX_G) G = MAX_G;
