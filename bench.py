@@ -320,6 +320,82 @@ def run_field(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def run_remesh(args, dev):
+    """Remeshing (remesh_and_transfer_velocity): the config-5 droplet geometry,
+    geodesic frequency 45 (40 500 faces, 20 252 vertices), one Loop pass on
+    the host (-> 162 000 faces, 81 002 vertices) and the nearest-vertex
+    transfer of one seeded velocity field [1, 20 252, 3] on the GPU, FP32.
+    One step = one transfer (1.64e9 (new, old) vertex pairs); the subdivision
+    time is reported beside it.  Roofline: FP32 VALU, 8 flops per pair
+    (offset 3, squared length 5) plus the compare-select."""
+    import numpy as np
+    import torch
+
+    from mesh_bem_amd.geometry import geodesic_sphere
+    from mesh_bem_amd.remesh import loop_subdivide, transfer_velocity_nearest
+
+    freq = 45 if args.workload == "remesh" else 9
+    Vn, Fn = geodesic_sphere(freq)
+    rng = np.random.Generator(np.random.PCG64(5))
+    vel = torch.tensor(rng.normal(size=(1, len(Vn), 3)), dtype=torch.float32, device=dev)
+    tc = time.perf_counter()
+    nv, nf = loop_subdivide(Vn, Fn)
+    t_sub = time.perf_counter() - tc
+    P = torch.tensor(nv, dtype=torch.float32, device=dev)
+    Q = torch.tensor(np.asarray(Vn), dtype=torch.float32, device=dev)
+    n_pairs = P.shape[0] * Q.shape[0]
+    log(f"workload {args.workload}: {len(Fn)} -> {len(nf)} faces, {P.shape[0]} new x {Q.shape[0]} old vertices")
+    for _ in range(args.warmup):
+        transfer_velocity_nearest(P, Q, vel)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        out, nearest = transfer_velocity_nearest(P, Q, vel)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ev_ms = ev0.elapsed_time(ev1) / args.steps
+    flops = 8.0 * n_pairs
+    tflops = flops / (ev_ms * 1e-3) / 1e12
+    res = {
+        "metric": "remesh velocity transfer (nearest old vertex of every Loop-subdivided vertex); Gpairs/s",
+        "value": round(1e-9 * n_pairs / (ms * 1e-3), 3),
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"geodesic frequency {freq} sphere ({len(Fn)} faces) -> one Loop pass "
+                               f"({len(nf)} faces), one seeded velocity field", "parallelism": "single GPU"},
+        "loop_subdivide_host_ms": round(t_sub * 1e3, 2),
+        "roofline": {"kernel": "k_nearest<float> + k_nearest_finish (8 flops + compare per pair)",
+                     "bound": "valu", "achieved": round(tflops, 2), "peak": PEAK_FP32_MFMA_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_MFMA_TFLOPS, 4), "traffic": None,
+                     "avg_launch_ms": round(ev_ms, 4), "flops_per_launch": flops},
+    }
+    if not args.no_cpu:
+        log("CPU baseline (numpy brute force sample)")
+        sample = 2048 if args.workload == "remesh" else 256
+        pc = nv[:sample]
+        qc = np.asarray(Vn, dtype=np.float64)
+        tcb = time.perf_counter()
+        d = ((pc[:, None, :] - qc[None, :, :]) ** 2).sum(-1)
+        d.argmin(axis=1)
+        dt = time.perf_counter() - tcb
+        res["cpu_baseline"] = {"value": round(1e-9 * sample * len(qc) / dt, 6), "unit": "Gpairs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"numpy brute-force argmin (FP64) for the first {sample} new vertices x "
+                                         f"all {len(qc)} old vertices, {dt:.2f} s"}
+    print(json.dumps(res), flush=True)
+
+
 def run_hd(args, dev):
     """Ferrofluid step, surface Helmholtz decomposition
     (MeshHelmholtzDecomposition.solve): noisy ellipsoid of geodesic frequency
@@ -838,7 +914,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "small", "c3", "c3small", "c4", "c4small", "c5",
                                                             "c5small", "field", "fieldsmall", "hd", "hdsmall", "bem2d", "bem2dsmall", "edges",
-                                                            "edgessmall", "pressure", "pressuresmall"])
+                                                            "edgessmall", "pressure", "pressuresmall", "remesh", "remeshsmall"])
     ap.add_argument("--epochs", type=int, default=4, help="training epochs (c4)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison")
@@ -879,6 +955,8 @@ def main():
         return run_edges(args, dev)
     if args.workload in ("pressure", "pressuresmall"):
         return run_pressure(args, dev)
+    if args.workload in ("remesh", "remeshsmall"):
+        return run_remesh(args, dev)
     wl = workload(args.workload)
     Vn, Fn = icosphere(wl["subdiv"])
     k = wl["k"]

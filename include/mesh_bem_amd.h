@@ -602,6 +602,27 @@ void mba_mesh_free(mba_mesh_part *parts, int32_t n_parts);
 int mba_mesh_write_beast_in(const char *path, const double *verts, int64_t n_verts, const int32_t *faces,
                             int64_t n_faces);
 
+/* Remeshing (remesh_and_transfer_velocity, src/utils/mesh_utils.py:728-815).
+ * mba_loop_subdivide: one pass of Loop subdivision on the host (the
+ * reference calls open3d's subdivide_loop, mesh_utils.py:781): verts
+ * [n_verts, 3] double, faces [n_faces, 3] int32 -> new_verts [n_new_verts, 3]
+ * (the moved old vertices, then one per unique edge in ascending (min, max)
+ * order), new_faces [4 n_faces, 3].  Call with new_verts == NULL to get
+ * n_new_verts only.  MBA_ERR_ARG for an index out of range or a face with a
+ * repeated vertex.
+ * mba_nearest_transfer: the GPU step (mesh_utils.py:800-815): for every
+ * point, the nearest vertex (squared distance from coordinate differences;
+ * ties to the smallest index, as argmin of torch.cdist) into nearest [n_points]
+ * and new_vel[b][i][:] = vel[b][nearest[i]][:] for vel [batch, n_verts, 3]
+ * (prec reals; batch may be 0).  workspace >=
+ * mba_nearest_transfer_workspace_bytes. */
+int mba_loop_subdivide(const double *verts, int64_t n_verts, const int32_t *faces, int64_t n_faces,
+                       double *new_verts, int32_t *new_faces, int64_t *n_new_verts);
+size_t mba_nearest_transfer_workspace_bytes(int prec, int64_t n_points, int64_t n_verts);
+int mba_nearest_transfer(int prec, const void *points, int64_t n_points, const void *verts, int64_t n_verts,
+                         const void *vel, int32_t batch, int32_t *nearest, void *new_vel, void *workspace,
+                         size_t workspace_bytes, mba_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

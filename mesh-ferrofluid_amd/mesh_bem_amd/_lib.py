@@ -162,6 +162,9 @@ SIGNATURES = {
     "mba_mesh_read": (_c.c_int, [_c.c_char_p, _vp, _vp]),
     "mba_mesh_free": (None, [_vp, _i32]),
     "mba_mesh_write_beast_in": (_c.c_int, [_c.c_char_p, _vp, _i64, _vp, _i64]),
+    "mba_loop_subdivide": (_c.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "mba_nearest_transfer_workspace_bytes": (_sz, [_c.c_int, _i64, _i64]),
+    "mba_nearest_transfer": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _i64, _vp, _c.c_int32, _vp, _vp, _vp, _sz, _vp]),
     "mba_laplace_p0_workspace_bytes": (_sz, [_c.c_int, _i64]),
     "mba_laplace_p0_assemble": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _i64p, _i32p,
                                            _vp, _vp, _i64, _vp, _sz, _vp]),
