@@ -325,14 +325,15 @@ def run_remesh(args, dev):
     geodesic frequency 45 (40 500 faces, 20 252 vertices), one Loop pass on
     the host (-> 162 000 faces, 81 002 vertices) and the nearest-vertex
     transfer of one seeded velocity field [1, 20 252, 3] on the GPU, FP32.
-    One step = one transfer (1.64e9 (new, old) vertex pairs); the subdivision
-    time is reported beside it.  Roofline: FP32 VALU, 8 flops per pair
+    One step = one transfer (1.64e9 (new, old) vertex pairs, the transfer
+    runs on the subdivided mesh); the host subdivision and the quadric
+    decimation to 90 % of the faces are timed beside it.  Roofline: FP32 VALU, 8 flops per pair
     (offset 3, squared length 5) plus the compare-select."""
     import numpy as np
     import torch
 
     from mesh_bem_amd.geometry import geodesic_sphere
-    from mesh_bem_amd.remesh import loop_subdivide, transfer_velocity_nearest
+    from mesh_bem_amd.remesh import decimate_quadric, loop_subdivide, transfer_velocity_nearest
 
     freq = 45 if args.workload == "remesh" else 9
     Vn, Fn = geodesic_sphere(freq)
@@ -341,6 +342,9 @@ def run_remesh(args, dev):
     tc = time.perf_counter()
     nv, nf = loop_subdivide(Vn, Fn)
     t_sub = time.perf_counter() - tc
+    tc = time.perf_counter()
+    dv, df = decimate_quadric(nv, nf, int(len(nf) * 0.9))
+    t_dec = time.perf_counter() - tc
     P = torch.tensor(nv, dtype=torch.float32, device=dev)
     Q = torch.tensor(np.asarray(Vn), dtype=torch.float32, device=dev)
     n_pairs = P.shape[0] * Q.shape[0]
@@ -375,6 +379,8 @@ def run_remesh(args, dev):
         "config": {"workload": f"geodesic frequency {freq} sphere ({len(Fn)} faces) -> one Loop pass "
                                f"({len(nf)} faces), one seeded velocity field", "parallelism": "single GPU"},
         "loop_subdivide_host_ms": round(t_sub * 1e3, 2),
+        "decimate_host_ms": round(t_dec * 1e3, 2),
+        "decimated_faces": int(len(df)),
         "roofline": {"kernel": "k_nearest<float> + k_nearest_finish (8 flops + compare per pair)",
                      "bound": "valu", "achieved": round(tflops, 2), "peak": PEAK_FP32_MFMA_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP32_MFMA_TFLOPS, 4), "traffic": None,

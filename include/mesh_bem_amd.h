@@ -604,7 +604,7 @@ int mba_mesh_write_beast_in(const char *path, const double *verts, int64_t n_ver
 
 /* Remeshing (remesh_and_transfer_velocity, src/utils/mesh_utils.py:728-815).
  * mba_loop_subdivide: one pass of Loop subdivision on the host (the
- * reference calls open3d's subdivide_loop, mesh_utils.py:781): verts
+ * reference calls open3d's subdivide_loop, mesh_utils.py:786-791): verts
  * [n_verts, 3] double, faces [n_faces, 3] int32 -> new_verts [n_new_verts, 3]
  * (the moved old vertices, then one per unique edge in ascending (min, max)
  * order), new_faces [4 n_faces, 3].  Call with new_verts == NULL to get
@@ -618,6 +618,16 @@ int mba_mesh_write_beast_in(const char *path, const double *verts, int64_t n_ver
  * mba_nearest_transfer_workspace_bytes. */
 int mba_loop_subdivide(const double *verts, int64_t n_verts, const int32_t *faces, int64_t n_faces,
                        double *new_verts, int32_t *new_faces, int64_t *n_new_verts);
+/* Quadric edge-collapse decimation on the host (the reference's open3d
+ * simplify_quadric_decimation, mesh_utils.py:793-794; Garland-Heckbert quadrics,
+ * link-condition and normal-flip checks, boundary vertices kept) down to at
+ * most target_faces faces (fewer collapses when none is valid).  out_verts
+ * [n_verts, 3] / out_faces [n_faces, 3] are caller buffers of the input
+ * sizes; the counts used go to n_out_verts / n_out_faces (survivors in their
+ * original order, renumbered). */
+int mba_decimate_quadric(const double *verts, int64_t n_verts, const int32_t *faces, int64_t n_faces,
+                         int64_t target_faces, double *out_verts, int32_t *out_faces, int64_t *n_out_verts,
+                         int64_t *n_out_faces);
 size_t mba_nearest_transfer_workspace_bytes(int prec, int64_t n_points, int64_t n_verts);
 int mba_nearest_transfer(int prec, const void *points, int64_t n_points, const void *verts, int64_t n_verts,
                          const void *vel, int32_t batch, int32_t *nearest, void *new_vel, void *workspace,

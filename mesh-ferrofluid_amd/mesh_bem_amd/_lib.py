@@ -163,6 +163,7 @@ SIGNATURES = {
     "mba_mesh_free": (None, [_vp, _i32]),
     "mba_mesh_write_beast_in": (_c.c_int, [_c.c_char_p, _vp, _i64, _vp, _i64]),
     "mba_loop_subdivide": (_c.c_int, [_vp, _i64, _vp, _i64, _vp, _vp, _vp]),
+    "mba_decimate_quadric": (_c.c_int, [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
     "mba_nearest_transfer_workspace_bytes": (_sz, [_c.c_int, _i64, _i64]),
     "mba_nearest_transfer": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _i64, _vp, _c.c_int32, _vp, _vp, _vp, _sz, _vp]),
     "mba_laplace_p0_workspace_bytes": (_sz, [_c.c_int, _i64]),

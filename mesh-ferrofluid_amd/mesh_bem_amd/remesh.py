@@ -3,19 +3,20 @@
 
 The reference refines the surface with open3d (Loop subdivision while the
 mean unique-edge length exceeds the target, at most three passes,
-``mesh_utils.py:770-782``), then decimates it by quadric edge collapse to 90 %
+``mesh_utils.py:782-791``), then decimates it by quadric edge collapse to 90 %
 of the triangles and removes duplicated / degenerate / non-manifold elements
-(``:785-788``), and finally gives every new vertex the velocity of its nearest
+(``:793-797``), and finally gives every new vertex the velocity of its nearest
 old vertex on the GPU (``torch.cdist`` + ``argmin`` + ``gather``, ``:800-815``).
 
 Here the subdivision is a native Loop pass (``mba_loop_subdivide``, host C++,
-Loop's published masks) and the transfer is a GPU kernel
-(``mba_nearest_transfer``).  open3d is not available in this environment, so
-its quadric decimation and clean-up passes are not reproduced: the returned
-mesh keeps every subdivided triangle (no duplicates or degenerate faces arise
-from subdividing a clean mesh).  Bit-parity with open3d's subdivision is not
-established (open3d's source is not here); the Loop masks are pinned by their
-mathematical properties in the tests.
+Loop's published masks), the decimation a native quadric edge collapse
+(``mba_decimate_quadric``, Garland-Heckbert, to 90 % of the triangles as the
+reference asks) and the transfer a GPU kernel (``mba_nearest_transfer``).
+open3d is not available in this environment, so bit-parity with its
+subdivision and decimation is not established; both are pinned by their
+mathematical properties in the tests.  The clean-up passes find nothing to
+remove: neither step creates duplicated vertices, degenerate faces or
+non-manifold edges from a clean 2-manifold.
 """
 
 import ctypes
@@ -45,6 +46,21 @@ def loop_subdivide(verts, faces):
                                 nf.ctypes.data, ctypes.addressof(n_new))
     _lib.check(st, "mba_loop_subdivide")
     return nv, nf
+
+
+def decimate_quadric(verts, faces, target_faces: int):
+    """Quadric edge-collapse decimation (host) to at most ``target_faces``
+    faces: (verts float64 numpy, faces int32 numpy)."""
+    lib = _lib.load()
+    v = np.ascontiguousarray(np.asarray(verts.detach().cpu() if torch.is_tensor(verts) else verts, dtype=np.float64))
+    f = np.ascontiguousarray(np.asarray(faces.detach().cpu() if torch.is_tensor(faces) else faces, dtype=np.int32))
+    ov = np.empty_like(v)
+    of = np.empty_like(f)
+    nv, nf = ctypes.c_int64(0), ctypes.c_int64(0)
+    st = lib.mba_decimate_quadric(v.ctypes.data, v.shape[0], f.ctypes.data, f.shape[0], int(target_faces),
+                                  ov.ctypes.data, of.ctypes.data, ctypes.addressof(nv), ctypes.addressof(nf))
+    _lib.check(st, "mba_decimate_quadric")
+    return ov[: nv.value].copy(), of[: nf.value].copy()
 
 
 def average_edge_length(verts, faces) -> float:
@@ -83,9 +99,9 @@ def remesh_and_transfer_velocity(verts: torch.Tensor, faces: torch.Tensor, vel: 
                                  target_edge_length: float = 0.1) -> List[torch.Tensor]:
     """Drop-in for src/utils/mesh_utils.py:728-815 (same arguments and returns:
     new_verts [N_new, 3], new_faces [F_new, 3], new_vel [B, N_new, 3]).  Loop
-    passes while the mean edge length exceeds the target (at most three), then
-    the nearest-vertex velocity transfer on the GPU; open3d's decimation to
-    90 % of the triangles is not reproduced (see the module docstring)."""
+    passes while the mean edge length exceeds the target (at most three), the
+    quadric decimation to 90 % of the triangles, then the nearest-vertex
+    velocity transfer on the GPU."""
     device, dtype = vel.device, vel.dtype
     B, n_verts, _ = vel.shape
     assert verts.shape[0] == n_verts and verts.shape[1] == 3
@@ -97,6 +113,8 @@ def remesh_and_transfer_velocity(verts: torch.Tensor, faces: torch.Tensor, vel: 
             break
         v, f = loop_subdivide(v, f)
         it += 1
+    # the reference's "simplify slightly to stabilise the triangle count"
+    v, f = decimate_quadric(v, f, int(len(f) * 0.9))
     new_verts = torch.from_numpy(v).to(device).to(dtype)
     new_faces = torch.from_numpy(f).to(device).to(faces.dtype)
     new_vel, _ = transfer_velocity_nearest(new_verts, verts.to(device), vel)
