@@ -129,3 +129,21 @@ def test_split_gemm_tiles(dev, tile, M, N, K):
         set_split_tile(0)
     ref = -ref_product(A, B) + c_in
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-6
+
+
+@pytest.mark.parametrize("M,N,K", [(700, 300, 512), (1000, 1000, 256)])
+def test_split_gemm_tiles_bit_identical(dev, M, N, K):
+    """Each accumulator receives its products in the same order whatever the
+    tile, so both tiles give bit-identical results."""
+    rng = np.random.default_rng(M * N + K)
+    A = torch.tensor(crand(rng, M, K), dtype=torch.complex64, device=dev)
+    B = torch.tensor(crand(rng, K, N), dtype=torch.complex64, device=dev)
+    C0 = torch.tensor(crand(rng, M, N), dtype=torch.complex64, device=dev)
+    outs = []
+    try:
+        for tile in (0, 1):
+            set_split_tile(tile)
+            outs.append(gemm_split_f16(A, B, C0.clone(), alpha=-1.0, beta=1.0))
+    finally:
+        set_split_tile(0)
+    assert torch.equal(outs[0], outs[1])
