@@ -676,40 +676,56 @@ __global__ void k_ns_rhs(const FaceRec<T>* __restrict__ rec, int64_t F,
 }
 
 // ----------------------------------------------------------------- gathers
+// Workgroup: 256 consecutive columns n x `rows` consecutive rows m (8 for
+// large matrices; 1 below 8192 columns, where fewer workgroups would leave
+// CUs idle).  The column's window test is done once for all its rows, and the
+// workgroups of one row group take consecutive logical indices on one XCD
+// (workgroup i runs on XCD i % 8), so the group's face-edge rows, read at
+// scattered columns, are fetched into that XCD's L2 once.
+constexpr int GATHER_XCD = 8;
+inline int gather_rows(int64_t E) { return E >= 8192 ? 8 : 1; }
+
 template <class T>
-__global__ void k_gather_rwg(const cpx<T>* __restrict__ zf, int64_t ldzf, int64_t col0,
-                             int64_t ncols, const int32_t* __restrict__ rwg_fe, int64_t E,
-                             cpx<T>* __restrict__ z, int64_t ldz)
+__global__ __launch_bounds__(256) void k_gather_rwg(const cpx<T>* __restrict__ zf, int64_t ldzf, int64_t col0,
+                                                    int64_t ncols, const int32_t* __restrict__ rwg_fe, int64_t E,
+                                                    cpx<T>* __restrict__ z, int64_t ldz, int rows)
 {
-    const int64_t nbx = (E + blockDim.x - 1) / blockDim.x;
-    const int64_t m = (int64_t)blockIdx.x / nbx;
-    const int64_t n = ((int64_t)blockIdx.x % nbx) * blockDim.x + threadIdx.x;
+    const int64_t nbx = (E + 255) / 256;
+    const int64_t total = nbx * ((E + rows - 1) / rows);
+    const int64_t per = (int64_t)gridDim.x / GATHER_XCD;  // gridDim.x = per * GATHER_XCD >= total
+    const int64_t L = (int64_t)(blockIdx.x % GATHER_XCD) * per + blockIdx.x / GATHER_XCD;
+    if (L >= total) return;
+    const int64_t m0 = (L / nbx) * rows;
+    const int64_t n = (L % nbx) * 256 + threadIdx.x;
     if (n >= E) return;
     const int64_t c0 = (int64_t)rwg_fe[2 * n] - col0, c1 = (int64_t)rwg_fe[2 * n + 1] - col0;
     const bool in0 = (c0 >= 0 && c0 < ncols), in1 = (c1 >= 0 && c1 < ncols);
     if (!in0 && !in1) return;
-    const cpx<T>* r0 = zf + (int64_t)rwg_fe[2 * m] * ldzf;
-    const cpx<T>* r1 = zf + (int64_t)rwg_fe[2 * m + 1] * ldzf;
-    // z = (zf[fe0(m), fe0(n)] + zf[fe1(m), fe0(n)]) + (zf[fe0(m), fe1(n)] + zf[fe1(m), fe1(n)]):
-    // the two column pairs are summed separately, so a column split over two
-    // windows rounds exactly as one gathered in a single window.
-    cpx<T>* out = z + m * ldz + n;
-    T re, im;
-    if (in0) {
-        cpx<T> a = r0[c0], b = r1[c0];
-        re = a.re + b.re;
-        im = a.im + b.im;
-    } else {
-        cpx<T> o = *out;
-        re = o.re;
-        im = o.im;
+    const int64_t m1 = (m0 + rows) < E ? (m0 + rows) : E;
+    for (int64_t m = m0; m < m1; ++m) {
+        const cpx<T>* r0 = zf + (int64_t)rwg_fe[2 * m] * ldzf;
+        const cpx<T>* r1 = zf + (int64_t)rwg_fe[2 * m + 1] * ldzf;
+        // z = (zf[fe0(m), fe0(n)] + zf[fe1(m), fe0(n)]) + (zf[fe0(m), fe1(n)] + zf[fe1(m), fe1(n)]):
+        // the two column pairs are summed separately, so a column split over two
+        // windows rounds exactly as one gathered in a single window.
+        cpx<T>* out = z + m * ldz + n;
+        T re, im;
+        if (in0) {
+            cpx<T> a = r0[c0], b = r1[c0];
+            re = a.re + b.re;
+            im = a.im + b.im;
+        } else {
+            cpx<T> o = *out;
+            re = o.re;
+            im = o.im;
+        }
+        if (in1) {
+            cpx<T> a = r0[c1], b = r1[c1];
+            re = re + (a.re + b.re);
+            im = im + (a.im + b.im);
+        }
+        store_cpx(out, re, im);
     }
-    if (in1) {
-        cpx<T> a = r0[c1], b = r1[c1];
-        re = re + (a.re + b.re);
-        im = im + (a.im + b.im);
-    }
-    store_cpx(out, re, im);
 }
 
 template <class T>
@@ -1109,16 +1125,18 @@ int mba_mom_gather_rwg(int prec, const void* zf, int64_t ldzf, int64_t col0, int
     if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
     if (n_edges <= 0 || n_edges > ((int64_t)1 << 31) - 1 || ncols <= 0 || col0 < 0) return MBA_ERR_SIZE;
     if (ldz < n_edges || ldzf < ncols) return MBA_ERR_ARG;
-    int threads = 256;
-    dim3 grid((unsigned)(((n_edges + threads - 1) / threads) * n_edges));
+    const int rows = gather_rows(n_edges);
+    const int64_t total = ((n_edges + 255) / 256) * ((n_edges + rows - 1) / rows);
+    dim3 grid((unsigned)((total + GATHER_XCD - 1) / GATHER_XCD * GATHER_XCD));
+    const int threads = 256;
     if (prec == MBA_F32)
         hipLaunchKernelGGL(k_gather_rwg<float>, grid, dim3(threads), 0, S(stream),
                            (const cpx<float>*)zf, ldzf, col0, ncols, rwg_fe, n_edges,
-                           (cpx<float>*)z, ldz);
+                           (cpx<float>*)z, ldz, rows);
     else
         hipLaunchKernelGGL(k_gather_rwg<double>, grid, dim3(threads), 0, S(stream),
                            (const cpx<double>*)zf, ldzf, col0, ncols, rwg_fe, n_edges,
-                           (cpx<double>*)z, ldz);
+                           (cpx<double>*)z, ldz, rows);
     MBA_LAUNCH_CHECK();
     return MBA_OK;
 }

@@ -187,14 +187,16 @@ def assemble_ns(mesh: MomMesh, k: float, prec: int, window_faces: Optional[int] 
     """RWG-space EFIE matrix Z [E, E] (row = test RWG), complex64/complex128.
 
     Source faces are processed in windows of ``window_faces`` (default: the
-    whole mesh when the face-edge buffer stays under 8 GiB)."""
+    whole mesh when the face-edge buffer stays under 64 GiB, so every entry of
+    Z is written once and in order; a column whose two face-edges fall in
+    different windows is stored, then added to)."""
     lib = _lib.load()
     F, E = mesh.n_faces, mesh.n_edges
     dev = mesh.verts.device
     cdt = _cdt(prec)
     esz = 8 if prec == _lib.MBA_F32 else 16
     if window_faces is None:
-        window_faces = max(64, min(F, int((8 << 30) // (9 * F * esz))))
+        window_faces = max(64, min(F, int((64 << 30) // (9 * F * esz))))
     window_faces = min(window_faces, F)
     if out is None:
         out = torch.empty((E, E), dtype=cdt, device=dev)

@@ -116,3 +116,25 @@ def test_ns_rwg_matrix_and_windows(dev, dtype, tol):
     assert fro < tol and ent < 20 * tol
     fro, _ = relerr(Vr.cpu().numpy(), Vo)
     assert fro < tol
+
+
+def test_ns_rwg_gather_large_windows(dev):
+    """E >= 8192 (eight rows per gather workgroup): several windows give the
+    single-window matrix bit for bit, and sampled entries equal the four
+    face-edge terms summed in the stated order from the face-edge matrix."""
+    V, F, _ = star_mesh(2, 17)  # 5780 faces, 8670 RWG
+    mesh = MomMesh.build(torch.tensor(V, dtype=torch.float64), F, device=dev)
+    E, nF = mesh.n_edges, mesh.n_faces
+    assert E >= 8192
+    prec = _lib.MBA_F64
+    Z1 = assemble_ns(mesh, 4.0, prec)
+    Zw = assemble_ns(mesh, 4.0, prec, window_faces=1000)
+    assert torch.equal(Z1, Zw)
+    zf = zmat_faceedge_ns(mesh, 4.0, prec, 0, nF)
+    fe = mesh.rwg_fe.long()
+    rng = np.random.default_rng(3)
+    m = torch.tensor(rng.integers(0, E, 4096), device=dev)
+    n = torch.tensor(rng.integers(0, E, 4096), device=dev)
+    a0, a1, b0, b1 = fe[m, 0], fe[m, 1], fe[n, 0], fe[n, 1]
+    want = (zf[a0, b0] + zf[a1, b0]) + (zf[a0, b1] + zf[a1, b1])
+    assert torch.equal(Z1[m, n], want)
