@@ -105,7 +105,10 @@ size_t mba_mom_workspace_bytes(int prec, int64_t n_faces);
  * faces a whose pair takes the singular split; for those the 1/(4 pi R) part
  * of the inner integral is computed analytically in FP64.
  * zf: [3F, ldzf] complex rows = test face-edge; cols [col0, col0 + 3*nb)
- * receive source faces [b0, b0 + nb).  workspace >= mba_mom_workspace_bytes. */
+ * receive source faces [b0, b0 + nb).  The near relation must be symmetric
+ * (a near b <=> b near a; mba_near_pairs_* build it so): in FP64, pairs of two
+ * faces in [b0, b0 + nb) are evaluated once and stored in both orders.
+ * workspace >= mba_mom_workspace_bytes. */
 int mba_mom_zmat_faceedge_ns(int prec, const void *verts, const int32_t *faces, int64_t n_verts,
                              int64_t n_faces, const double *rule, int32_t nq, double wavenumber,
                              const int64_t *near_ptr, const int32_t *near_idx, int64_t b0,
@@ -141,7 +144,8 @@ int mba_mom_gather_rhs(int prec, const void *vf, const int32_t *rwg_fe, int64_t 
  * are the concatenation of the meshes (faces hold indices into the
  * concatenated verts; mesh m owns faces [face_off, face_off + n_faces), face_off
  * equal to the running sum); every mesh writes its own face-edge block
- * zf [3 n_faces, ldzf] for all its source faces.  n_near = near_ptr[n_faces].
+ * zf [3 n_faces, ldzf] for all its source faces.  n_near = near_ptr[n_faces];
+ * each mesh's near relation must be symmetric (see mba_mom_zmat_faceedge_ns).
  * The descriptor array is host memory; the call copies it to the workspace and
  * waits for that copy.  No reference counterpart (the reference solves one
  * mesh per call). */

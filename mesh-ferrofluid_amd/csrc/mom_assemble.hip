@@ -99,6 +99,17 @@ __global__ void k_face_records(const T* __restrict__ verts, const int32_t* __res
 //   <(x - v_i).(y - v_j) G> = Ixy - v_j.Ix - v_i.Iy + (v_i.v_j) I0.
 // Near pairs integrate (e^{-jkR} - 1)/(4 pi R); the analytic 1/(4 pi R) part is
 // added by k_ns_near.
+//
+// The face-edge matrix is complex symmetric: the integrand of entry (a i, b j)
+// is symmetric in the two faces and both sides use the same rule, so
+// z(a i, b j) = z(b j, a i).  In FP64 (bound by its FP64 arithmetic) a pair
+// whose test face a also lies in the window of source faces is evaluated
+// once: for a < b the values go to both (a, b) and the mirrored (b, a) block,
+// pairs with a > b are skipped.  This needs the near relation to be symmetric
+// (a near b <=> b near a), as the centroid-distance criterion of the near-pair
+// builders is.  In FP32 the hardware transcendentals make the kernel cheap
+// enough that the mirrored block's column-wise stores cost more than they
+// save, so every ordered pair is evaluated.
 constexpr int NS_TA = 64;     // test faces per block
 constexpr int NS_BLOCK = 256; // source faces per block
 
@@ -125,6 +136,14 @@ __device__ __forceinline__ void ns_zf_tile(NsTest<T, NQ>* tst, const T (*lam)[4]
     const int64_t a0 = by * NS_TA;
     const int64_t bl = bx * NS_BLOCK + tid;  // local source index
     const int64_t b = b0 + bl;
+    constexpr bool SYM = sizeof(T) == 8;
+    // whole tile below the diagonal of the window's square part: nothing to do
+    // (its pairs come from the mirrored tile).  The test depends on the block
+    // only, so every thread leaves before the barrier below.
+    if (SYM) {
+        const int64_t bend = bx * NS_BLOCK + NS_BLOCK < nb ? bx * NS_BLOCK + NS_BLOCK : nb;
+        if (a0 >= b0 && a0 + NS_TA <= b0 + nb && a0 >= b0 + bend) return;
+    }
     for (int t = tid; t < NS_TA; t += NS_BLOCK) {
         int64_t a = a0 + t;
         if (a < F) {
@@ -165,6 +184,8 @@ __device__ __forceinline__ void ns_zf_tile(NsTest<T, NQ>* tst, const T (*lam)[4]
             nearf = (T)1;
             ++np;
         }
+        const bool square = SYM && a >= b0 && a < b0 + nb;  // a is also a source face of this window
+        if (square && a > b) continue;
         const NsTest<T, NQ>& ta = tst[t];
         T D[3];
         for (int q = 0; q < 3; ++q) D[q] = (ta.ohi[q] - rb.ohi[q]) + (ta.olo[q] - rb.olo[q]);
@@ -216,8 +237,11 @@ __device__ __forceinline__ void ns_zf_tile(NsTest<T, NQ>* tst, const T (*lam)[4]
             Ixyr += w * (xl0 * gyr[0] + xl1 * gyr[1] + xl2 * gyr[2]);
             Ixyi += w * (xl0 * gyi[0] + xl1 * gyi[1] + xl2 * gyi[2]);
         }
-        // 9 local entries: z = c_i c_j [ jk dot - (4j/k) I0 ]
+        // 9 local entries: z = c_i c_j [ jk dot - (4j/k) I0 ]; mirrored to
+        // rows 3b + j, columns 3(a - b0) + i when a < b lies in the window
         cpx<T>* row0 = zf + (3 * a) * ldzf + 3 * bl;
+        const bool mirror = square && a < b;
+        cpx<T>* rowm = mirror ? zf + (3 * b) * ldzf + 3 * (a - b0) : row0;
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const T ai0 = ta.p[3 * i], ai1 = ta.p[3 * i + 1], ai2 = ta.p[3 * i + 2];
@@ -234,6 +258,7 @@ __device__ __forceinline__ void ns_zf_tile(NsTest<T, NQ>* tst, const T (*lam)[4]
                 const T zr = cc * (-kk * di + kinv4 * I0i);
                 const T zi = cc * (kk * dr - kinv4 * I0r);
                 store_cpx(row0 + i * ldzf + j, zr, zi);
+                if (mirror) store_cpx(rowm + j * ldzf + i, zr, zi);
             }
         }
     }

@@ -106,7 +106,11 @@ def test_ns_rwg_matrix_and_windows(dev, dtype, tol):
     prec = _lib.prec_of(dtype)
     Z = assemble_ns(mesh, 3.0, prec, window_faces=37)
     Z1 = assemble_ns(mesh, 3.0, prec)
-    assert torch.equal(Z, Z1) or dtype == torch.float32
+    # FP64 pairs with both faces in one window are evaluated once and
+    # mirrored, so the windowing changes roundings only
+    assert (Z - Z1).abs().max().item() <= 4 * tol * Z1.abs().max().item()
+    if dtype == torch.float32:
+        assert torch.equal(Z, Z1)
     dirs = np.array([[0, 0, 1.0], [0.6, 0, 0.8]])
     pols = np.array([[1.0, 0, 0], [0, 1.0, 0]])
     Vr = rhs_ns(mesh, 3.0, prec, dirs, pols)
@@ -120,7 +124,7 @@ def test_ns_rwg_matrix_and_windows(dev, dtype, tol):
 
 def test_ns_rwg_gather_large_windows(dev):
     """E >= 8192 (eight rows per gather workgroup): several windows give the
-    single-window matrix bit for bit, and sampled entries equal the four
+    single-window matrix to rounding, and sampled entries equal the four
     face-edge terms summed in the stated order from the face-edge matrix."""
     V, F, _ = star_mesh(2, 17)  # 5780 faces, 8670 RWG
     mesh = MomMesh.build(torch.tensor(V, dtype=torch.float64), F, device=dev)
@@ -129,7 +133,7 @@ def test_ns_rwg_gather_large_windows(dev):
     prec = _lib.MBA_F64
     Z1 = assemble_ns(mesh, 4.0, prec)
     Zw = assemble_ns(mesh, 4.0, prec, window_faces=1000)
-    assert torch.equal(Z1, Zw)
+    assert (Z1 - Zw).abs().max().item() <= 1e-13 * Z1.abs().max().item()
     zf = zmat_faceedge_ns(mesh, 4.0, prec, 0, nF)
     fe = mesh.rwg_fe.long()
     rng = np.random.default_rng(3)
@@ -138,3 +142,27 @@ def test_ns_rwg_gather_large_windows(dev):
     a0, a1, b0, b1 = fe[m, 0], fe[m, 1], fe[n, 0], fe[n, 1]
     want = (zf[a0, b0] + zf[a1, b0]) + (zf[a0, b1] + zf[a1, b1])
     assert torch.equal(Z1[m, n], want)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-11)])
+def test_ns_faceedge_symmetric(dev, dtype, tol):
+    """FP64, one window over all faces: each far pair is evaluated once and mirrored,
+    so far blocks are exactly symmetric (zf[3a+i, 3b+j] == zf[3b+j, 3a+i]);
+    near blocks carry their own analytic corrections.  Both orders agree with
+    the oracle, which evaluates every ordered pair directly."""
+    V, F, _ = star_mesh(4, 5)
+    mesh = MomMesh.build(torch.tensor(V, dtype=dtype), F, device=dev)
+    prec = _lib.prec_of(dtype)
+    zf = zmat_faceedge_ns(mesh, 2.5, prec, 0, len(F)).cpu().numpy()
+    nF = len(F)
+    near = np.zeros((nF, nF), dtype=bool)
+    ptr, idx = mesh.near_ptr.cpu().numpy(), mesh.near_idx.cpu().numpy()
+    for b in range(nF):
+        near[idx[ptr[b]:ptr[b + 1]], b] = True
+    assert np.array_equal(near, near.T)
+    far = np.repeat(np.repeat(~near, 3, axis=0), 3, axis=1)
+    assert np.array_equal(zf[far], zf.T[far])
+    Vd = mesh.verts.double().cpu().numpy()
+    Zo = O.ns_zmat_faceedge(Vd, F, 2.5, ptr, idx)
+    fro, ent = relerr(zf, Zo)
+    assert fro < tol and ent < 20 * tol
