@@ -2219,7 +2219,6 @@ __device__ void ipiv_perm_body(const int32_t* __restrict__ ipiv, int64_t n, int3
     __syncthreads();
     for (int64_t c = 0; c < n; c += 64) {
         const int32_t v = (c + lane < n) ? ipiv[c + lane] : 0;
-        const int cnt = (int)((n - c) < 64 ? (n - c) : 64);
         // 64 interchanges at once when no two of the real ones (p != j) touch
         // a common place: then their order does not matter
         const int64_t jl = c + lane;
@@ -2239,8 +2238,13 @@ __device__ void ipiv_perm_body(const int32_t* __restrict__ ipiv, int64_t n, int3
             }
             continue;
         }
-        for (int q = 0; q < cnt; ++q) {
-            const int32_t p = __builtin_amdgcn_readlane(v, q);  // q is uniform: a scalar read, no LDS trip
+        // in order, skipping the interchanges with p == j (often about half of
+        // them): walk the set bits of the wave's mask of real interchanges
+        uint64_t mv = __ballot(real);
+        while (mv) {
+            const int q = __builtin_ctzll(mv);
+            mv &= mv - 1;
+            const int32_t p = __builtin_amdgcn_readlane(pl_, q);  // uniform q: a scalar read, no LDS trip
             if (lane == 0) {
                 const int64_t j = c + q;
                 int32_t t = P[j];

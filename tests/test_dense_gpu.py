@@ -285,3 +285,29 @@ def test_lu_pivots_match_lapack(dev, n):
     ref = scipy.linalg.lu_factor(a)[1]
     got = f.ipiv.cpu().numpy().astype(np.int64)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n", [1, 63, 3000, 40000])
+def test_pivots_to_perm_sequential(dev, n):
+    """mba_lu_pivots_to_perm equals the interchanges applied one by one
+    (position i ends holding original row perm[i]) for LAPACK-style pivots
+    p_j >= j with about half no-ops, near pivots and repeated far rows; n =
+    40000 exceeds the in-LDS size and takes the global-memory path."""
+    print(f"pivots_to_perm n={n}", flush=True)
+    rng = np.random.default_rng(77 + n)
+    j = np.arange(n)
+    kind = rng.integers(0, 4, n)
+    far = j + rng.integers(0, n - j)
+    near = np.minimum(j + rng.integers(0, 3, n), n - 1)
+    rep = np.minimum(j + (n - j) // 2, n - 1)  # many columns share this far row
+    piv = np.where(kind < 2, j, np.where(kind == 2, near, np.where(rng.random(n) < 0.5, far, rep)))
+    piv = np.maximum(piv, j).astype(np.int32)
+    want = np.arange(n, dtype=np.int64)
+    for i in range(n):
+        p = piv[i]
+        want[i], want[p] = want[p], want[i]
+    ip = torch.tensor(piv, device=dev)
+    perm = torch.empty(n, dtype=torch.int32, device=dev)
+    _lib.check(_lib.load().mba_lu_pivots_to_perm(_lib.ptr(ip), n, _lib.ptr(perm), _lib.stream()),
+               "mba_lu_pivots_to_perm")
+    assert np.array_equal(perm.cpu().numpy().astype(np.int64), want)
