@@ -54,6 +54,8 @@ def decimate_quadric(verts, faces, target_faces: int):
     lib = _lib.load()
     v = np.ascontiguousarray(np.asarray(verts.detach().cpu() if torch.is_tensor(verts) else verts, dtype=np.float64))
     f = np.ascontiguousarray(np.asarray(faces.detach().cpu() if torch.is_tensor(faces) else faces, dtype=np.int32))
+    if v.ndim != 2 or v.shape[1] != 3 or f.ndim != 2 or f.shape[1] != 3:
+        raise ValueError("decimate_quadric needs verts [V, 3] and faces [F, 3]")
     ov = np.empty_like(v)
     of = np.empty_like(f)
     nv, nf = ctypes.c_int64(0), ctypes.c_int64(0)
@@ -75,9 +77,12 @@ def average_edge_length(verts, faces) -> float:
 def transfer_velocity_nearest(new_verts: torch.Tensor, verts: torch.Tensor, vel: torch.Tensor):
     """new_vel [B, N_new, 3] = vel[:, nearest old vertex of each new vertex]
     (mesh_utils.py:800-815) and the nearest indices [N_new] (int64), on the GPU."""
-    lib = _lib.load()
+    if (new_verts.ndim != 2 or new_verts.shape[1] != 3 or verts.ndim != 2 or verts.shape[1] != 3
+            or vel.ndim != 3 or vel.shape[1] != verts.shape[0] or vel.shape[2] != 3):
+        raise ValueError("transfer_velocity_nearest needs new_verts [N, 3], verts [V, 3] and vel [B, V, 3]")
     if verts.device.type != "cuda":
         raise ValueError("transfer_velocity_nearest runs on the GPU: verts must be a device tensor")
+    lib = _lib.load()
     dt = vel.dtype if vel.dtype in (torch.float32, torch.float64) else torch.float32
     prec = _lib.prec_of(dt)
     dev = verts.device

@@ -158,3 +158,24 @@ def test_decimation_of_a_plane_stays_planar_and_keeps_the_boundary():
     # the decimated mesh still covers the square: areas add up to n^2
     a = 0.5 * np.abs(np.cross(dv[df[:, 1]] - dv[df[:, 0]], dv[df[:, 2]] - dv[df[:, 0]])[:, 2])
     assert abs(a.sum() - n * n) < 1e-9
+
+
+def test_shapes_are_checked_before_any_native_call():
+    """Wrong operand shapes raise ValueError in the host layer, before the
+    native code would read past an array."""
+    import torch
+
+    from mesh_bem_amd.remesh import decimate_quadric, loop_subdivide, transfer_velocity_nearest
+
+    v = np.zeros((4, 3))
+    f = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int32)
+    with pytest.raises(ValueError):
+        decimate_quadric(v.reshape(-1), f, 1)
+    with pytest.raises(ValueError):
+        decimate_quadric(v, f[:, :2], 1)
+    with pytest.raises(ValueError):
+        loop_subdivide(v[:, :2], f)
+    with pytest.raises(ValueError):
+        transfer_velocity_nearest(torch.zeros(5, 3), torch.zeros(4, 3), torch.zeros(1, 3, 3))
+    with pytest.raises(ValueError):
+        transfer_velocity_nearest(torch.zeros(5, 2), torch.zeros(4, 3), torch.zeros(1, 4, 3))
