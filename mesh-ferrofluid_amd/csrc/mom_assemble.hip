@@ -787,7 +787,8 @@ __global__ void k_interp(const T* __restrict__ verts, const int32_t* __restrict_
                    (v[i2][2] - v[i1][2]) * (v[i2][2] - v[i1][2]));
         T c = l / ((T)2 * A);
         if (!(vi[i1] < vi[i2])) c = -c;
-        cpx<T> I = icoef[bb * E + fe_idx[3 * f + e]];
+        const int32_t ei = fe_idx[3 * f + e];  // an index outside [0, E) contributes nothing
+        const cpx<T> I = (ei >= 0 && ei < E) ? icoef[bb * E + ei] : cpx<T>{(T)0, (T)0};
         for (int s = 1; s <= 2; ++s) {
             int t = (e + s) % 3;
             for (int q = 0; q < 3; ++q) {

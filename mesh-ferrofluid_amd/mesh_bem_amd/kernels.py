@@ -20,6 +20,23 @@ def _cdtype(dtype):
     return torch.complex64 if dtype == torch.float32 else torch.complex128
 
 
+def _check_mesh(verts, faces, face_areas, face_normals=None):
+    """Operand shapes the kernels index by: verts [V, 3], faces [F, 3], one
+    area (and one normal) per face.  Raises ValueError before any launch."""
+    if verts.ndim != 2 or verts.shape[1] != 3 or faces.ndim != 2 or faces.shape[1] != 3:
+        raise ValueError("verts must be [V, 3] and faces [F, 3]")
+    F = faces.shape[0]
+    if face_areas.numel() != F:
+        raise ValueError(f"face_areas must hold one value per face ({F}), got {face_areas.numel()}")
+    if face_normals is not None and face_normals.numel() != 3 * F:
+        raise ValueError(f"face_normals must be [F, 3] (F = {F}), got {tuple(face_normals.shape)}")
+
+
+def _check_rule(points, weights):
+    if points.numel() != weights.numel() or points.numel() == 0:
+        raise ValueError("gaussian points and weights must be non-empty and of equal length")
+
+
 class MeshMOMZmatKernel(torch.autograd.Function):
     @staticmethod
     def forward(
@@ -34,6 +51,9 @@ class MeshMOMZmatKernel(torch.autograd.Function):
         gaussian_weights_1d_y: torch.Tensor,
         wavenumber: float,
     ) -> Any:
+        _check_mesh(verts, faces, face_areas, face_normals)
+        _check_rule(gaussian_points_1d_x, gaussian_weights_1d_x)
+        _check_rule(gaussian_points_1d_y, gaussian_weights_1d_y)
         lib = _lib.load()
         N_faces, dim = faces.shape
         dt = verts.dtype
@@ -63,6 +83,8 @@ class MeshMOMRhsKernel(torch.autograd.Function):
         gaussian_weights_1d: torch.Tensor,
         wavenumber: float,
     ) -> Any:
+        _check_mesh(verts, faces, face_areas, face_normals)
+        _check_rule(gaussian_points_1d, gaussian_weights_1d)
         lib = _lib.load()
         N_faces, dim = faces.shape
         dt = verts.dtype
@@ -88,6 +110,9 @@ class MeshMOMInterpolateJsKernel(torch.autograd.Function):
         face_areas: torch.Tensor,
         edge_I_coeff: torch.Tensor,
     ) -> Any:
+        _check_mesh(verts, faces, face_areas)
+        if face_edge_indices.numel() != 3 * faces.shape[0]:
+            raise ValueError("face_edge_indices must hold 3 entries per face")
         lib = _lib.load()
         N_faces, dim = faces.shape
         dt = verts.dtype

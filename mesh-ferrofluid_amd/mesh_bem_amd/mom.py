@@ -120,6 +120,10 @@ class MomMesh:
         ft = torch.as_tensor(faces).to(torch.int32)
         vnp = vt.detach().cpu().numpy().astype(np.float64)
         fnp = ft.detach().cpu().numpy().astype(np.int64)
+        if vnp.ndim != 2 or vnp.shape[1] != 3 or fnp.ndim != 2 or fnp.shape[1] != 3 or len(fnp) == 0:
+            raise ValueError("MomMesh.build needs verts [V, 3] and faces [F, 3], F > 0")
+        if fnp.min() < 0 or fnp.max() >= len(vnp):
+            raise ValueError("face vertex index out of range")
         if face_edge_indices is None:
             edges, fe, _ = get_edge_unique(vt.cpu(), ft.cpu())
             n_edges = edges.shape[0]

@@ -78,3 +78,41 @@ def test_product_has_no_cpu_fallback(monkeypatch):
     A = torch.zeros((4, 4), dtype=torch.complex64)
     with pytest.raises(Exception):
         gemm(A, A)
+
+
+def test_reference_ops_check_shapes_before_launch():
+    """The reference-API ops reject mis-shaped operands with ValueError before
+    any device check or launch (CPU tensors suffice)."""
+    import torch
+
+    from mesh_bem_amd.kernels import MeshMOMInterpolateJsKernel, MeshMOMRhsKernel, MeshMOMZmatKernel
+
+    v = torch.zeros(4, 3, dtype=torch.float64)
+    f = torch.tensor([[0, 1, 2], [0, 2, 3]], dtype=torch.int32)
+    a = torch.ones(2, 1, dtype=torch.float64)
+    n = torch.zeros(2, 3, dtype=torch.float64)
+    g = torch.tensor([0.5], dtype=torch.float64)
+    w = torch.tensor([1.0], dtype=torch.float64)
+    with pytest.raises(ValueError):
+        MeshMOMZmatKernel.apply(v.reshape(-1), f, a, n, g, w, g, w, 1.0)
+    with pytest.raises(ValueError):
+        MeshMOMZmatKernel.apply(v, f, a[:1], n, g, w, g, w, 1.0)
+    with pytest.raises(ValueError):
+        MeshMOMRhsKernel.apply(v, f, a, n[:, :2], g, w, 1.0)
+    with pytest.raises(ValueError):
+        MeshMOMRhsKernel.apply(v, f, a, n, g, torch.ones(2, dtype=torch.float64), 1.0)
+    with pytest.raises(ValueError):
+        MeshMOMInterpolateJsKernel.apply(v, f, torch.zeros(5, dtype=torch.int32), a,
+                                         torch.zeros(1, 5, dtype=torch.complex128))
+
+
+def test_mom_mesh_build_checks_indices():
+    import numpy as np
+
+    from mesh_bem_amd.mom import MomMesh
+
+    v = np.zeros((4, 3))
+    with pytest.raises(ValueError):
+        MomMesh.build(v, np.array([[0, 1, 4]]), device="cpu")
+    with pytest.raises(ValueError):
+        MomMesh.build(v, np.array([[0, 1]]), device="cpu")
