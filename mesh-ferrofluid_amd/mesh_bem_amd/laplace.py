@@ -96,6 +96,11 @@ def relation_tables(gp: np.ndarray, gw: np.ndarray):
     return np.ascontiguousarray(np.concatenate(rows, 0)), np.array(offsets, dtype=np.int64)
 
 
+def _check_vf(verts, faces):
+    if verts.ndim != 2 or verts.shape[1] != 3 or faces.ndim != 2 or faces.shape[1] != 3:
+        raise ValueError("verts must be [V, 3] and faces [F, 3]")
+
+
 def _rdt(prec):
     return torch.float32 if prec == _lib.MBA_F32 else torch.float64
 
@@ -103,6 +108,7 @@ def _rdt(prec):
 def layer_ref(verts: torch.Tensor, faces: torch.Tensor, gaussQR: int, layer: int, order_type: int,
               tables=None) -> torch.Tensor:
     """Unpacked [F, F, nphi, nphi] Laplace layer (1 single, 2 double), dtype of verts."""
+    _check_vf(verts, faces)
     lib = _lib.load()
     dt = verts.dtype
     prec = _lib.prec_of(dt)
@@ -128,9 +134,11 @@ def layer_ref(verts: torch.Tensor, faces: torch.Tensor, gaussQR: int, layer: int
 
 def pack(unpacked: torch.Tensor, faces: torch.Tensor, n_verts: int) -> torch.Tensor:
     """[F, F, nphi, nphi] -> [V, V] by the vertex incidence (fixed summation order)."""
-    lib = _lib.load()
     F = faces.shape[0]
     nphi = unpacked.shape[-1]
+    if unpacked.shape != (F, F, nphi, nphi) or nphi not in (1, 3):
+        raise ValueError(f"unpacked must be [F, F, nphi, nphi] with F = {F}, nphi 1 or 3")
+    lib = _lib.load()
     vptr, vidx = vertex_incidence(faces.detach().cpu().numpy(), n_verts)
     dev = unpacked.device
     vp = torch.from_numpy(vptr).to(dev)
@@ -143,8 +151,10 @@ def pack(unpacked: torch.Tensor, faces: torch.Tensor, n_verts: int) -> torch.Ten
 
 
 def dirichlet_rhs(K_tight: torch.Tensor, g: torch.Tensor, vert_areas: torch.Tensor) -> torch.Tensor:
-    lib = _lib.load()
     V = K_tight.shape[0]
+    if K_tight.shape != (V, V) or g.numel() != V or vert_areas.numel() != V:
+        raise ValueError(f"K must be [V, V] and g, vert_areas hold V = {V} values")
+    lib = _lib.load()
     out = torch.empty(V, dtype=K_tight.dtype, device=K_tight.device)
     st = lib.mba_laplace_dirichlet_rhs(_lib.prec_of(K_tight.dtype), _lib.ptr(K_tight.contiguous()), V,
                                        _lib.ptr(g.reshape(-1).to(K_tight.dtype).contiguous()),
@@ -210,9 +220,13 @@ def laplace_solve(solver, verts: torch.Tensor, faces: torch.Tensor, vert_bc_Diri
 def p0_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, want_v: bool = True,
                  want_k: bool = True, near=None, out_v=None, out_k=None):
     """P0 Galerkin single layer V [F, F] and adjoint double layer K' [F, F]."""
+    _check_vf(verts, faces)
+    F = faces.shape[0]
+    for o in (out_v, out_k):
+        if o is not None and (o.shape != (F, F) or not o.is_contiguous()):
+            raise ValueError(f"output buffers must be contiguous [F, F] (F = {F})")
     lib = _lib.load()
     dev = verts.device
-    F = faces.shape[0]
     rdt = _rdt(prec)
     vt = verts.to(rdt).contiguous()
     ft = faces.to(torch.int32).contiguous()

@@ -116,3 +116,20 @@ def test_mom_mesh_build_checks_indices():
         MomMesh.build(v, np.array([[0, 1, 4]]), device="cpu")
     with pytest.raises(ValueError):
         MomMesh.build(v, np.array([[0, 1]]), device="cpu")
+
+
+def test_laplace_entry_points_check_shapes():
+    import torch
+
+    from mesh_bem_amd import laplace
+
+    v = torch.zeros(4, 3, dtype=torch.float64)
+    f = torch.tensor([[0, 1, 2], [0, 2, 3]], dtype=torch.int32)
+    with pytest.raises(ValueError):
+        laplace.layer_ref(v[:, :2], f, 2, 1, 1)
+    with pytest.raises(ValueError):
+        laplace.pack(torch.zeros(2, 3, 3, 3, dtype=torch.float64), f, 4)
+    with pytest.raises(ValueError):
+        laplace.dirichlet_rhs(torch.zeros(4, 4, dtype=torch.float64), torch.zeros(3), torch.zeros(4))
+    with pytest.raises(ValueError):
+        laplace.p0_operators(v, f, 1, out_v=torch.zeros(3, 3))
