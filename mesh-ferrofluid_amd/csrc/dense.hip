@@ -1680,7 +1680,8 @@ __global__ __launch_bounds__(256) void k_laswp(cpx<T>* __restrict__ A, int64_t l
 // one thread per column -- two barriers per block instead of two per row.
 // Columns per workgroup of the panel triangular solve (FP32 / FP64 complex).
 // 32 columns keep X + Ls at 48 KiB (complex64), so three workgroups share a
-// CU and a panel's column blocks run in about one round.
+// CU by LDS; the complex64 kernels are also held to three waves per SIMD (168
+// VGPRs; unbounded, the compiler takes 198 and fits only two workgroups).
 template <class T>
 struct TrsmCols {
     static constexpr int value = 32;
@@ -1814,7 +1815,7 @@ constexpr size_t trsm_lds_bytes(int nb)
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_l(cpx<T>* __restrict__ A, int64_t lda, int64_t k,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void k_trsm_l(cpx<T>* __restrict__ A, int64_t lda, int64_t k,
                                                 int nb, int64_t col_begin, int64_t col_end)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -1876,7 +1877,7 @@ __global__ __launch_bounds__(256) void k_laswp_batched(const SwapJob<T>* __restr
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_trsm_batched(const TrsmJob<T>* __restrict__ jobs,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void k_trsm_batched(const TrsmJob<T>* __restrict__ jobs,
                                                       const int* __restrict__ wg_off, int njob)
 {
     extern __shared__ unsigned char smem_raw[];
