@@ -101,38 +101,62 @@ __global__ __launch_bounds__(256) void k_split_rows(const cpx<float>* __restrict
     }
 }
 
-// ---- B (K x N, row-major) -> transposed records.  Workgroup: 64 columns x 4
-// groups; group g takes chunks g, g + 4, ...; thread = (column, group).
+// ---- B (K x N, row-major) -> transposed records.  Workgroup: SC columns;
+// the K x SC block is read once into LDS (all 32 loads of a thread in flight
+// together), then the column maxima and the split records come from LDS.
+constexpr int SC = 16;
+
 __global__ __launch_bounds__(256) void k_split_cols(const cpx<float>* __restrict__ B, int64_t ldb, int K, int Kpad,
                                                     int64_t N, int64_t Npad, _Float16* __restrict__ Bp,
                                                     int32_t* __restrict__ eb)
 {
-    __shared__ float red[4][64];
-    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int64_t col = (int64_t)blockIdx.x * 64 + c;
+    constexpr int LPT = KMAX * SC / 256;  // block elements per thread
+    __shared__ cpx<float> blk[KMAX * SC];  // [k][c]; rows k >= K are zero
+    __shared__ float red[256 / SC][SC];
+    __shared__ int s_e[SC];
+    const int tid = threadIdx.x;
+    const int64_t col0 = (int64_t)blockIdx.x * SC;
     const int nrec = Kpad / KC;
-    const bool cok = col < N;
-    float mx = 0.f;
-    for (int kc = g; kc < nrec; kc += 4)
-        for (int q = 0; q < KC; ++q) {
-            const int k = kc * KC + q;
-            if (cok && k < K) {
-                const cpx<float> x = B[(int64_t)k * ldb + col];
-                mx = fmaxf(mx, fmaxf(fabsf(x.re), fabsf(x.im)));
-            }
+    {
+        cpx<float> v[LPT];
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) {
+            const int e = tid + 256 * i, k = e / SC;
+            const int64_t col = col0 + e % SC;
+            v[i] = (k < K && col < N) ? B[(int64_t)k * ldb + col] : cpx<float>{0.f, 0.f};
         }
-    red[g][c] = mx;
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) blk[tid + 256 * i] = v[i];
+    }
     __syncthreads();
-    mx = fmaxf(fmaxf(red[0][c], red[1][c]), fmaxf(red[2][c], red[3][c]));
-    const int e = scale_exp(mx);
-    if (col >= Npad) return;
-    if (g == 0) eb[col] = e;
-    for (int kc = g; kc < nrec; kc += 4) {
+    {
+        const int c = tid % SC, g = tid / SC;
+        float mx = 0.f;
+        for (int k = g; k < K; k += 256 / SC) {
+            const cpx<float> x = blk[k * SC + c];
+            mx = fmaxf(mx, fmaxf(fabsf(x.re), fabsf(x.im)));
+        }
+        red[g][c] = mx;
+    }
+    __syncthreads();
+    if (tid < SC) {
+        float mx = 0.f;
+#pragma unroll
+        for (int g = 0; g < 256 / SC; ++g) mx = fmaxf(mx, red[g][tid]);
+        const int e = scale_exp(mx);
+        s_e[tid] = e;
+        if (col0 + tid < Npad) eb[col0 + tid] = e;
+    }
+    __syncthreads();
+    for (int pi = tid; pi < SC * nrec; pi += 256) {
+        const int c = pi % SC, kc = pi / SC;
+        const int64_t col = col0 + c;
+        if (col >= Npad) continue;
+        const int e = s_e[c];
         _Float16 h[2][KC], l[2][KC];
 #pragma unroll
         for (int q = 0; q < KC; ++q) {
-            const int k = kc * KC + q;
-            const cpx<float> x = (cok && k < K) ? B[(int64_t)k * ldb + col] : cpx<float>{0.f, 0.f};
+            const cpx<float> x = blk[(kc * KC + q) * SC + c];
             split2(ldexpf(x.re, -e), h[0][q], l[0][q]);
             split2(ldexpf(x.im, -e), h[1][q], l[1][q]);
         }
@@ -344,7 +368,7 @@ int split(int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const voi
     hipLaunchKernelGGL(k_split_rows, dim3((unsigned)((P.Mpad + 3) / 4)), dim3(256), 0, st, (const cpx<float>*)A, lda,
                        M, (int)K, P.Kpad, P.Mpad, P.Ap, P.ea);
     MBA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.Npad / 64)), dim3(256), 0, st, (const cpx<float>*)B, ldb,
+    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.Npad / SC)), dim3(256), 0, st, (const cpx<float>*)B, ldb,
                        (int)K, P.Kpad, N, P.Npad, P.Bp, P.eb);
     MBA_LAUNCH_CHECK();
     return MBA_OK;
