@@ -1557,21 +1557,24 @@ __global__ __launch_bounds__(128) void k_swap_plan(const int32_t* __restrict__ i
                                                    int32_t* __restrict__ slot_row, int32_t* __restrict__ plan,
                                                    int32_t* __restrict__ plan_n)
 {
-    __shared__ int32_t pv[128], first_of[128], slot_of[128];
+    __shared__ int32_t pv[132], first_of[128], slot_of[128];  // pv[128..131] = -1 (never a row)
     __shared__ int s_cnt0;
     const int j = threadIdx.x, lane = j & 63, wave = j >> 6;
     const bool act = j < nb;
     const int32_t p = act ? ipiv[k + j] : -1;
     pv[j] = p;
+    if (j < 4) pv[128 + j] = -1;
     __syncthreads();
     const bool far = act && p >= k + nb;
-    int f = j;  // first swap with the same pivot row
+    int f = j;  // first swap with the same pivot row; four LDS reads per step
     if (far)
-        for (int i = 0; i < j; ++i)
-            if (pv[i] == p) {
-                f = i;
-                break;
-            }
+        for (int i = 0; i < j; i += 4) {
+            const int32_t q0 = pv[i], q1 = pv[i + 1], q2 = pv[i + 2], q3 = pv[i + 3];
+            if (q0 == p) { f = i; break; }
+            if (i + 1 < j && q1 == p) { f = i + 1; break; }
+            if (i + 2 < j && q2 == p) { f = i + 2; break; }
+            if (i + 3 < j && q3 == p) { f = i + 3; break; }
+        }
     const bool is_first = far && f == j;
     first_of[j] = f;
     const unsigned long long bal = __ballot(is_first);
