@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/mesh_bem_amd.h"
+#include "mba_guard.hpp"
 
 namespace {
 
@@ -75,11 +76,11 @@ void cross3(const double* u, const double* v, double* w)
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int mba_decimate_quadric(const double* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces,
-                         int64_t target_faces, double* out_verts, int32_t* out_faces, int64_t* n_out_verts,
-                         int64_t* n_out_faces)
+int decimate_quadric(const double* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces,
+                       int64_t target_faces, double* out_verts, int32_t* out_faces, int64_t* n_out_verts,
+                       int64_t* n_out_faces)
 {
     if (!verts || !faces || !out_verts || !out_faces || !n_out_verts || !n_out_faces) return MBA_ERR_ARG;
     if (n_verts <= 0 || n_faces <= 0 || n_verts > INT32_MAX || n_faces > INT32_MAX) return MBA_ERR_SIZE;
@@ -298,6 +299,17 @@ int mba_decimate_quadric(const double* verts, int64_t n_verts, const int32_t* fa
     *n_out_verts = nv;
     *n_out_faces = nf;
     return MBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mba_decimate_quadric(const double* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces,
+                         int64_t target_faces, double* out_verts, int32_t* out_faces, int64_t* n_out_verts,
+                         int64_t* n_out_faces)
+{
+    return mba_guard([&] { return decimate_quadric(verts, n_verts, faces, n_faces, target_faces, out_verts, out_faces, n_out_verts, n_out_faces); });
 }
 
 }  // extern "C"

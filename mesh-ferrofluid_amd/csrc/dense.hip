@@ -12,10 +12,13 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "gemm_h3.hpp"
 #include "mba_common.hpp"
+#include "mba_guard.hpp"
 
 namespace {
 
@@ -400,15 +403,18 @@ int launch_cgemm_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A
     return MBA_OK;
 }
 
-int g_gemm_f32_variant = 4;  // tile choice of the FP32 GEMM (set by mba_gemm_set_variant)
+// Process-wide defaults behind the mba_*_set_* calls.  Atomic, so setting one
+// while other threads run calls is not a data race; the LU reads its own
+// (update, panel, solve) choices once at the start of each call.
+std::atomic<int> g_gemm_f32_variant{4};  // tile choice of the FP32 GEMM (set by mba_gemm_set_variant)
 // trailing update of the complex64 LU: 0 = FP32 MFMA (the variant above),
 // 1 = FP32-accurate split-FP16 MFMA product (gemm_h3.hip); set by mba_lu_set_update
-int g_lu_update_f32 = 0;
+std::atomic<int> g_lu_update_f32{0};
 // the split product pays off once the trailing block has this many rows and columns
 constexpr int64_t SPLIT_MIN_DIM = 2048;
 // panels of the complex64 LU: 0 = partial pivoting (LAPACK's row order), 1 =
 // tournament pivoting (64-column panels of >= 2048 rows); set by mba_lu_set_panel
-int g_lu_panel_f32 = 0;
+std::atomic<int> g_lu_panel_f32{0};
 
 int cgemm_f32(int variant, int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda,
               const void* B, int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
@@ -612,7 +618,7 @@ int gemm<float>(int64_t M, int64_t N, int64_t K, double alpha, const void* A, in
                 const void* B, int64_t ldb, double beta, void* C, int64_t ldc, hipStream_t st)
 {
     if (M <= 0 || N <= 0) return MBA_OK;
-    return cgemm_f32(g_gemm_f32_variant, M, N, K, (float)alpha, A, lda, B, ldb, (float)beta, C, ldc, st);
+    return cgemm_f32(g_gemm_f32_variant.load(std::memory_order_relaxed), M, N, K, (float)alpha, A, lda, B, ldb, (float)beta, C, ldc, st);
 }
 
 template <>
@@ -2023,14 +2029,22 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     // dynamic LDS of the panel; the static arrays add up to 11 KiB (FP64), the
     // total stays under the 160 KiB of a CU
     const size_t lds_budget = 140 * 1024;
-    // panel width: largest of 128/64/32 whose per-CU row share fits the LDS budget
+    // panel width: largest of 128/64/32 whose per-workgroup row share fits the
+    // LDS budget, with the workgroup count the panel launches will use (one
+    // per CU, at most max_wg when the caller caps it, at most MAX_G)
+    int64_t gcap = cus;
+    if (max_wg > 0 && max_wg < gcap) gcap = max_wg;
+    if (gcap > MAX_G) gcap = MAX_G;
     int NB = 128;
-    while (NB > 16 && (size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) NB /= 2;
-    if ((size_t)((n + cus - 1) / cus) * NB * esz > lds_budget) return MBA_ERR_SIZE;
-    const bool tournament = sizeof(T) == 4 && g_lu_panel_f32 == 1 && NB >= TP_NB;
+    while (NB > 16 && (size_t)((n + gcap - 1) / gcap) * NB * esz > lds_budget) NB /= 2;
+    if ((size_t)((n + gcap - 1) / gcap) * NB * esz > lds_budget) return MBA_ERR_SIZE;
+    const int upd_kind = g_lu_update_f32.load(std::memory_order_relaxed);
+    const bool tournament = sizeof(T) == 4 && g_lu_panel_f32.load(std::memory_order_relaxed) == 1 && NB >= TP_NB;
     if (tournament) NB = TP_NB;
-    static bool attr_set[2] = {false, false};
-    if (!attr_set[prec]) {
+    // kernel attributes: set once per process (per element type); call_once
+    // makes concurrent first calls from several host threads safe
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [&] {
         (void)hipFuncSetAttribute((const void*)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_budget);
         (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -2038,8 +2052,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         if (sizeof(T) == 4)
             (void)hipFuncSetAttribute((const void*)k_panel_tp, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)((TP_RMAX + TP_NB) * TP_PITCH * sizeof(cpx<float>)));
-        attr_set[prec] = true;
-    }
+    });
     // err := 0, every message word := epoch 0 (epochs of this factorisation are k + j + 1 >= 1)
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ws.err, 32);
     MBA_LAUNCH_CHECK();
@@ -2157,7 +2170,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     // before them.  P NB = 512 with the split-FP16 trailing product (k = 512
     // halves its C traffic per flop), 256 otherwise (NB = 64 with tournament
     // panels, else 128 while the LDS allows).
-    const int P = ((sizeof(T) == 4 && g_lu_update_f32 == 1) ? 4 : 2) * (tournament ? 2 : 1);
+    const int P = ((sizeof(T) == 4 && upd_kind == 1) ? 4 : 2) * (tournament ? 2 : 1);
     for (int64_t k = 0; k < n; k += (int64_t)P * NB) {
         int64_t kp = k;
         int rc;
@@ -2175,7 +2188,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         const int64_t c3 = kp;  // first column after the super panel
         if (c3 >= n) break;
         const int64_t k3 = c3 - k;
-        if (sizeof(T) == 4 && g_lu_update_f32 == 1 && n - c3 >= SPLIT_MIN_DIM) {
+        if (sizeof(T) == 4 && upd_kind == 1 && n - c3 >= SPLIT_MIN_DIM) {
             const int64_t m3 = n - c3;
             tm.mark(3);
             rc = mba_h3::split(m3, m3, k3, A + c3 * lda + k, lda, A + k * lda + c3, lda, ws.split, ws.split_bytes, st);
@@ -2824,12 +2837,11 @@ int lu_prepare_batched(const mba_lu_prepare_job* jobs, int njob, void* scratch, 
     // the host tables are released on return: wait for the copies
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
     const size_t shm = 2 * (size_t)TS_BS * (TS_BS + 1) * sizeof(cpx<T>);
-    static bool attr_set = false;
-    if (!attr_set) {
+    static std::once_flag attr_once;  // per element type (template instance)
+    std::call_once(attr_once, [&] {
         (void)hipFuncSetAttribute((const void*)k_tri_inv_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)shm);
-        attr_set = true;
-    }
+    });
     hipLaunchKernelGGL(k_ipiv_perm_batched, dim3((unsigned)njob), dim3(64), 0, st, (const PrepJobDev*)dj);
     MBA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_tri_inv_batched<T>, dim3((unsigned)wg[njob]), dim3(256), shm, st, (const PrepJobDev*)dj,
@@ -2849,13 +2861,11 @@ int lu_prepare(const void* lu, int64_t n, int64_t lda, const int32_t* ipiv, void
     hipLaunchKernelGGL(k_ipiv_perm, dim3(1), dim3(64), 0, st, ipiv, n, perm);
     MBA_LAUNCH_CHECK();
     const size_t shm = 2 * (size_t)TS_BS * (TS_BS + 1) * esz;
-    static bool attr_set[2] = {false, false};
-    const int pi = sizeof(T) == 4 ? 0 : 1;
-    if (!attr_set[pi]) {
+    static std::once_flag attr_once;  // per element type (template instance)
+    std::call_once(attr_once, [&] {
         (void)hipFuncSetAttribute((const void*)k_tri_inv<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         (void)hipFuncSetAttribute((const void*)k_tri_inv<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-        attr_set[pi] = true;
-    }
+    });
     hipLaunchKernelGGL((k_tri_inv<T, true>), dim3((unsigned)nblk), dim3(256), shm, st, (const cpx<T>*)lu, lda, n, linv);
     MBA_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_tri_inv<T, false>), dim3((unsigned)nblk), dim3(256), shm, st, (const cpx<T>*)lu, lda, n, uinv);
@@ -3304,7 +3314,7 @@ int lu_solve_fused(const void* lu, int64_t n, int64_t lda, const void* aux, void
 // complex64 triangular solves of mba_lu_solve for nrhs <= TS_RHS: 1 = one
 // persistent launch per triangle (default), 0 = one launch per 64-row block;
 // set by mba_lu_set_solve
-int g_lu_solve_f32 = 1;
+std::atomic<int> g_lu_solve_f32{1};
 
 size_t persist_ws_bytes(int64_t n, int64_t nrhs)
 {
@@ -3370,15 +3380,13 @@ int lu_solve(const void* lu, int64_t n, int64_t lda, const int32_t* perm, void* 
     MBA_LAUNCH_CHECK();
     const int BS = 64;
     const size_t dshm = (size_t)BS * BS * esz;
-    static bool attr_set[2] = {false, false};
-    const int pi = sizeof(T) == 4 ? 0 : 1;
-    if (!attr_set[pi]) {
+    static std::once_flag attr_once;  // per element type (template instance)
+    std::call_once(attr_once, [&] {
         (void)hipFuncSetAttribute((const void*)k_diag_solve<T, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dshm);
         (void)hipFuncSetAttribute((const void*)k_diag_solve<T, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)dshm);
-        attr_set[pi] = true;
-    }
+    });
     const unsigned dgrid = (unsigned)((nrhs + DS_RHS - 1) / DS_RHS);
     // forward: L y = P b (tmp holds P b, ld = nrhs)
     for (int64_t r0 = 0; r0 < n; r0 += BS) {
@@ -3511,16 +3519,15 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         if (mats[i].workspace_bytes < ws[i].total) return MBA_ERR_ALLOC;
         nmax = mats[i].n > nmax ? mats[i].n : nmax;
     }
-    static bool attr_set[2] = {false, false};
-    if (!attr_set[prec]) {
+    static std::once_flag attr_once;  // per element type (template instance)
+    std::call_once(attr_once, [&] {
         (void)hipFuncSetAttribute((const void*)k_panel_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds_budget);
         (void)hipFuncSetAttribute((const void*)k_trsm_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)trsm_lds_bytes<T>(NB));
         (void)hipFuncSetAttribute((const void*)k_laswp_batched<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)(2 * NB * LASWP_COLS * esz));
-        attr_set[prec] = true;
-    }
+    });
     // ---- host schedule: per panel phase, launches of (first entry, count, shm)
     struct Launch {
         size_t first, count, woff;
@@ -3661,8 +3668,9 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     std::vector<GemmJobDev> gjobs;
     std::vector<int> gtoff;
     std::vector<GemmRange> grange(3 * steps);
-    const bool grouped = prec == MBA_F64 || g_gemm_f32_variant == 4 || g_gemm_f32_variant == 5;
-    const bool gxcd = prec == MBA_F32 && g_gemm_f32_variant == 5;
+    const int gv = g_gemm_f32_variant.load(std::memory_order_relaxed);
+    const bool grouped = prec == MBA_F64 || gv == 4 || gv == 5;
+    const bool gxcd = prec == MBA_F32 && gv == 5;
     if (grouped) {
         for (int64_t s = 0; s < steps; ++s) {
             const int64_t k = 2 * NB * s;
@@ -3845,8 +3853,10 @@ int mba_lu_factor(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int3
     if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
     if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
     if (lda < n) return MBA_ERR_ARG;
-    if (prec == MBA_F32) return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
-    return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+    return mba_guard([&] {
+        if (prec == MBA_F32) return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+        return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream));
+    });
 }
 
 int mba_lu_factor_async(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,
@@ -3857,11 +3867,13 @@ int mba_lu_factor_async(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv
     if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
     if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
     if (lda < n || max_wg < 0) return MBA_ERR_ARG;
-    if (prec == MBA_F32)
-        return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
-                                nullptr, max_wg, err_out);
-    return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
-                             nullptr, max_wg, err_out);
+    return mba_guard([&] {
+        if (prec == MBA_F32)
+            return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
+                                    nullptr, max_wg, err_out);
+        return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), nullptr, nullptr,
+                                 nullptr, max_wg, err_out);
+    });
 }
 
 size_t mba_lu_solve_batched_scratch_bytes(int32_t n_jobs)
@@ -3884,8 +3896,10 @@ int mba_lu_solve_batched(int prec, const mba_lu_solve_job* jobs, int32_t n_jobs,
         wgs += (j.nrhs + TSC - 1) / TSC;
     }
     if (wgs > 0x7fffffff) return MBA_ERR_SIZE;
-    if (prec == MBA_F32) return lu_solve_batched<float>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
-    return lu_solve_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+    return mba_guard([&] {
+        if (prec == MBA_F32) return lu_solve_batched<float>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+        return lu_solve_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+    });
 }
 
 size_t mba_gemm_grouped_scratch_bytes(int32_t n_jobs)
@@ -3898,31 +3912,34 @@ int mba_gemm_grouped(int prec, const mba_gemm_job* jobs, int32_t n_jobs, void* s
                      mba_stream_t stream)
 {
     if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
-    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
-    if (scratch_bytes < mba_gemm_grouped_scratch_bytes(n_jobs)) return MBA_ERR_ALLOC;
-    std::vector<int> toff(n_jobs + 1, 0);
-    int64_t tiles = 0;
-    for (int j = 0; j < n_jobs; ++j) {
-        const mba_gemm_job& J = jobs[j];
-        if (J.m < 0 || J.n < 0 || J.k < 0) return MBA_ERR_SIZE;
-        if (J.m > 0 && J.n > 0) {
-            if (!J.a || !J.b || !J.c) return MBA_ERR_ARG;
-            if (J.lda < J.k || J.ldb < J.n || J.ldc < J.n) return MBA_ERR_ARG;
+    return mba_guard([&]() -> int {
+        if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+        if (scratch_bytes < mba_gemm_grouped_scratch_bytes(n_jobs)) return MBA_ERR_ALLOC;
+        std::vector<int> toff(n_jobs + 1, 0);
+        int64_t tiles = 0;
+        for (int j = 0; j < n_jobs; ++j) {
+            const mba_gemm_job& J = jobs[j];
+            if (J.m < 0 || J.n < 0 || J.k < 0) return MBA_ERR_SIZE;
+            if (J.m > 0 && J.n > 0) {
+                if (!J.a || !J.b || !J.c) return MBA_ERR_ARG;
+                if (J.lda < J.k || J.ldb < J.n || J.ldc < J.n) return MBA_ERR_ARG;
+            }
+            tiles += grouped_tiles(prec, J.m, J.n);
+            if (tiles > 0x7fffffff - NUM_XCD) return MBA_ERR_SIZE;
+            toff[j + 1] = (int)tiles;
         }
-        tiles += grouped_tiles(prec, J.m, J.n);
-        if (tiles > 0x7fffffff - NUM_XCD) return MBA_ERR_SIZE;
-        toff[j + 1] = (int)tiles;
-    }
-    hipStream_t st = S(stream);
-    char* base = (char*)scratch;
-    GemmJobDev* dj = (GemmJobDev*)base;
-    int* dt = (int*)(base + al256((size_t)n_jobs * sizeof(GemmJobDev)));
-    if (hipMemcpyAsync(dj, jobs, (size_t)n_jobs * sizeof(GemmJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dt, toff.data(), (size_t)(n_jobs + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
-        return MBA_ERR_LAUNCH;
-    // the host tables are released on return: wait for the copies
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
-    return launch_grouped(prec, prec == MBA_F32 && g_gemm_f32_variant == 5, dj, dt, n_jobs, tiles, st);
+        hipStream_t st = S(stream);
+        char* base = (char*)scratch;
+        GemmJobDev* dj = (GemmJobDev*)base;
+        int* dt = (int*)(base + al256((size_t)n_jobs * sizeof(GemmJobDev)));
+        if (hipMemcpyAsync(dj, jobs, (size_t)n_jobs * sizeof(GemmJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(dt, toff.data(), (size_t)(n_jobs + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+            return MBA_ERR_LAUNCH;
+        // the host tables are released on return: wait for the copies
+        if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+        return launch_grouped(prec, prec == MBA_F32 && g_gemm_f32_variant.load(std::memory_order_relaxed) == 5, dj,
+                              dt, n_jobs, tiles, st);
+    });
 }
 
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max)
@@ -3954,8 +3971,11 @@ int mba_lu_factor_batched(int prec, const mba_lu_matrix* mats, int32_t n_mat, vo
         if (m.n <= 0 || m.n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
         if (m.lda < m.n) return MBA_ERR_ARG;
     }
-    if (prec == MBA_F32) return lu_factor_batched<float>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
-    return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+    return mba_guard([&] {
+        if (prec == MBA_F32)
+            return lu_factor_batched<float>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+        return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+    });
 }
 
 int mba_lu_factor_profiled(int prec, void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info,
@@ -3967,11 +3987,13 @@ int mba_lu_factor_profiled(int prec, void* a, int64_t n, int64_t lda, int32_t* i
     if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
     if (n <= 0 || n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
     if (lda < n) return MBA_ERR_ARG;
-    if (prec == MBA_F32)
-        return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), phase_ms,
-                                phase_launches, gemm_flops);
-    return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), phase_ms,
-                             phase_launches, gemm_flops);
+    return mba_guard([&] {
+        if (prec == MBA_F32)
+            return lu_factor<float>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), phase_ms,
+                                    phase_launches, gemm_flops);
+        return lu_factor<double>(a, n, lda, ipiv, info, workspace, workspace_bytes, S(stream), phase_ms,
+                                 phase_launches, gemm_flops);
+    });
 }
 
 int mba_lu_pivots_to_perm(const int32_t* ipiv, int64_t n, int32_t* perm, mba_stream_t stream)
@@ -4029,8 +4051,10 @@ int mba_lu_solve_prepare_batched(int prec, const mba_lu_prepare_job* jobs, int32
         wgs += 2 * ((J.n + TS_BS - 1) / TS_BS);
         if (wgs > 0x7fffffff) return MBA_ERR_SIZE;
     }
-    if (prec == MBA_F32) return lu_prepare_batched<float>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
-    return lu_prepare_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+    return mba_guard([&] {
+        if (prec == MBA_F32) return lu_prepare_batched<float>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+        return lu_prepare_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+    });
 }
 
 int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const void* aux, void* b,
@@ -4043,7 +4067,7 @@ int mba_lu_solve(int prec, const void* lu, int64_t n, int64_t lda, const void* a
     if (lda < n || ldb < nrhs) return MBA_ERR_ARG;
     const int32_t* perm = (const int32_t*)aux;
     if (nrhs <= TS_RHS) {
-        if (prec == MBA_F32 && g_lu_solve_f32 == 1)
+        if (prec == MBA_F32 && g_lu_solve_f32.load(std::memory_order_relaxed) == 1)
             return lu_solve_persist(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));
         if (prec == MBA_F32)
             return lu_solve_fused<float>(lu, n, lda, aux, b, nrhs, ldb, workspace, workspace_bytes, S(stream));

@@ -20,6 +20,7 @@
 // hi_re, hi_im, lo_re, lo_im.  A: [Mpad][Kpad / 16] records; B (transposed):
 // [Npad][Kpad / 16] records; pad rows / columns / k are zero.
 #include <math.h>
+#include <atomic>
 #include <stdint.h>
 
 #include "gemm_h3.hpp"
@@ -37,7 +38,7 @@ constexpr int BM = 256, BN = 128;       // largest workgroup tile (complex); ope
 // Tile variants: 256 x 128 with 8 waves (4 x 2, 64 x 64 each, one workgroup
 // per CU) or 128 x 128 with 4 waves (2 x 2, two workgroups per CU, so one
 // workgroup's C read / write overlaps the other's MFMA work).
-int g_tile = 0;
+std::atomic<int> g_tile{0};  // process-wide default (mba_gemm_split_f16_set_tile)
 constexpr int PITCH_H = REC_H + 8;      // LDS row pitch in halves (144 B: conflict-free 16-byte reads)
 constexpr int NUM_XCD = 8, GROUP_M = 8;
 constexpr int SCALE_TOP = 15;           // scaled maxima in [2^14, 2^15)
@@ -381,7 +382,7 @@ int multiply(int64_t M, int64_t N, int64_t K, float alpha, float beta, void* C, 
     if (K <= 0 || K > KMAX) return MBA_ERR_SIZE;
     if (ws_bytes < workspace_bytes(M, N, K)) return MBA_ERR_ALLOC;
     const Parts P = parts(M, N, K, (void*)ws);
-    if (g_tile == 0) {
+    if (g_tile.load(std::memory_order_relaxed) == 0) {
         const int64_t tiles = ((M + 255) / 256) * ((N + BN - 1) / BN);
         const int64_t blocks = (tiles + NUM_XCD - 1) / NUM_XCD * NUM_XCD;
         hipLaunchKernelGGL((k_cgemm_h3<256, 512>), dim3((unsigned)blocks), dim3(512), 0, st, M, N, P.nrec, P.Ap, P.ea,

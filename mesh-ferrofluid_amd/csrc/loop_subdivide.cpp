@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/mesh_bem_amd.h"
+#include "mba_guard.hpp"
 
 namespace {
 
@@ -33,10 +34,10 @@ struct FaceEdge {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int mba_loop_subdivide(const double* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces,
-                       double* new_verts, int32_t* new_faces, int64_t* n_new_verts)
+int loop_subdivide(const double* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces,
+                   double* new_verts, int32_t* new_faces, int64_t* n_new_verts)
 {
     if (!verts || !faces || !n_new_verts) return MBA_ERR_ARG;
     if (n_verts <= 0 || n_faces <= 0 || n_verts > INT32_MAX / 2 || 4 * n_faces > INT32_MAX) return MBA_ERR_SIZE;
@@ -124,6 +125,16 @@ int mba_loop_subdivide(const double* verts, int64_t n_verts, const int32_t* face
             for (int d = 0; d < 3; ++d) new_faces[12 * f + 3 * q + d] = quad[q][d];
     }
     return MBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mba_loop_subdivide(const double* verts, int64_t n_verts, const int32_t* faces, int64_t n_faces,
+                       double* new_verts, int32_t* new_faces, int64_t* n_new_verts)
+{
+    return mba_guard([&] { return loop_subdivide(verts, n_verts, faces, n_faces, new_verts, new_faces, n_new_verts); });
 }
 
 }  // extern "C"

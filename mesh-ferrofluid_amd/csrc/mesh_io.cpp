@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "../../include/mesh_bem_amd.h"
+#include "mba_guard.hpp"
 
 namespace {
 
@@ -71,13 +72,27 @@ std::string lower_ext(const char* path)
     return e;
 }
 
-void fan(const std::vector<int64_t>& poly, std::vector<int32_t>& f)
+// x as an integer index: a finite double in [-2^53, 2^53] truncates to int64
+// without undefined behaviour; anything else maps to an out-of-range value
+// that the index checks reject.
+int64_t as_index(double x)
 {
+    if (!(x >= -9007199254740992.0 && x <= 9007199254740992.0)) return INT64_MIN;
+    return (int64_t)x;
+}
+
+// Triangle fan of a polygon; false when an index does not fit int32 (such
+// an index cannot name a vertex of a mesh this library accepts).
+bool fan(const std::vector<int64_t>& poly, std::vector<int32_t>& f)
+{
+    for (int64_t i : poly)
+        if (i < INT32_MIN || i > INT32_MAX) return false;
     for (size_t i = 1; i + 1 < poly.size(); ++i) {
         f.push_back((int32_t)poly[0]);
         f.push_back((int32_t)poly[i]);
         f.push_back((int32_t)poly[i + 1]);
     }
+    return true;
 }
 
 bool check_indices(const Part& p)
@@ -122,8 +137,7 @@ int parse_obj(const std::string& s, Part& p)
                 q = r;
                 while (q < e && *q != ' ' && *q != '\t' && *q != '\r') ++q;  // skip /vt/vn
             }
-            if (poly.size() < 3) return MBA_ERR_MESH;
-            fan(poly, p.f);
+            if (poly.size() < 3 || !fan(poly, p.f)) return MBA_ERR_MESH;
         }
         pos = eol + 1;
     }
@@ -277,12 +291,12 @@ int parse_ply(const std::string& s, Part& p)
             double xyz[3] = {0, 0, 0};
             for (const PlyProp& pr : e.props) {
                 if (pr.list) {
-                    const int64_t c = (int64_t)r.get(pr.count);
+                    const int64_t c = as_index(r.get(pr.count));
+                    if (c < 0) return MBA_ERR_MESH;
                     poly.clear();
-                    for (int64_t j = 0; j < c && r.ok; ++j) poly.push_back((int64_t)r.get(pr.item));
+                    for (int64_t j = 0; j < c && r.ok; ++j) poly.push_back(as_index(r.get(pr.item)));
                     if (is_f && (pr.name == "vertex_indices" || pr.name == "vertex_index")) {
-                        if (poly.size() < 3) return MBA_ERR_MESH;
-                        fan(poly, p.f);
+                        if (poly.size() < 3 || !fan(poly, p.f)) return MBA_ERR_MESH;
                     }
                 } else {
                     const double x = r.get(pr.item);
@@ -339,7 +353,10 @@ int parse_json(const std::string& s, Part& p)
     std::vector<double> f;
     if (!json_numbers(s, "verts", p.v) || !json_numbers(s, "faces", f)) return MBA_ERR_MESH;
     if (p.v.size() % 3 || f.size() % 3) return MBA_ERR_MESH;
-    for (double x : f) p.f.push_back((int32_t)x);
+    for (double x : f) {
+        if (!(x >= 0.0 && x <= (double)INT32_MAX)) return MBA_ERR_MESH;
+        p.f.push_back((int32_t)x);
+    }
     return MBA_OK;
 }
 
@@ -356,6 +373,12 @@ int parse_in(const std::string& s, Part& p)
     a = r;
     long long nf = strtoll(a, &r, 10);
     if (r == a || nv < 0 || nf < 0) return MBA_ERR_MESH;
+    // every value the counts announce takes at least two bytes of the file
+    // (a digit and a separator): larger counts are malformed, and bounding
+    // them here keeps 3 * count from overflowing and the resizes below from
+    // asking for more memory than the file could describe
+    const long long cap = (long long)(s.size() / 6);
+    if (nv > cap || nf > cap) return MBA_ERR_MESH;
     a = r;
     p.v.resize((size_t)nv * 3);
     for (auto& x : p.v) {
@@ -366,7 +389,7 @@ int parse_in(const std::string& s, Part& p)
     p.f.resize((size_t)nf * 3);
     for (auto& i : p.f) {
         long long x = strtoll(a, &r, 10);
-        if (r == a) return MBA_ERR_MESH;
+        if (r == a || x < 1 || x > nv) return MBA_ERR_MESH;
         i = (int32_t)(x - 1);
         a = r;
     }
@@ -488,6 +511,7 @@ int parse_dae(const std::string& s, std::vector<Part>& parts)
                 while (find_tag(s, "input", qp, prim.end, in)) {
                     qp = in.body;
                     const int off = atoi(in.attr["offset"].c_str());
+                    if (off < 0 || off > 4096) return MBA_ERR_MESH;
                     if (off + 1 > stride) stride = off + 1;
                     if (in.attr["semantic"] == "VERTEX") voff = off;
                 }
@@ -509,22 +533,27 @@ int parse_dae(const std::string& s, std::vector<Part>& parts)
                     const size_t nvtx = idx.size() / (size_t)stride;
                     if (k == 0) {
                         for (size_t t = 0; t + 3 <= nvtx; t += 3)
-                            for (int c = 0; c < 3; ++c)
-                                p.f.push_back((int32_t)idx[(t + c) * stride + voff]);
+                            for (int c = 0; c < 3; ++c) {
+                                const int64_t x = as_index(idx[(t + c) * stride + voff]);
+                                if (x < 0 || x > INT32_MAX) return MBA_ERR_MESH;
+                                p.f.push_back((int32_t)x);
+                            }
                     } else if (k == 1) {
                         size_t at = 0;
                         for (; ci < counts.size(); ++ci) {
-                            const size_t c = (size_t)counts[ci];
+                            const int64_t cc = as_index(counts[ci]);
+                            if (cc < 0) return MBA_ERR_MESH;
+                            const size_t c = (size_t)cc;
                             if (at + c > nvtx) return MBA_ERR_MESH;
                             poly.clear();
-                            for (size_t j = 0; j < c; ++j) poly.push_back((int64_t)idx[(at + j) * stride + voff]);
-                            if (poly.size() >= 3) fan(poly, p.f);
+                            for (size_t j = 0; j < c; ++j) poly.push_back(as_index(idx[(at + j) * stride + voff]));
+                            if (poly.size() >= 3 && !fan(poly, p.f)) return MBA_ERR_MESH;
                             at += c;
                         }
                     } else {
                         poly.clear();
-                        for (size_t j = 0; j < nvtx; ++j) poly.push_back((int64_t)idx[j * stride + voff]);
-                        if (poly.size() >= 3) fan(poly, p.f);
+                        for (size_t j = 0; j < nvtx; ++j) poly.push_back(as_index(idx[j * stride + voff]));
+                        if (poly.size() >= 3 && !fan(poly, p.f)) return MBA_ERR_MESH;
                     }
                 }
             }
@@ -534,15 +563,9 @@ int parse_dae(const std::string& s, std::vector<Part>& parts)
     return parts.empty() ? MBA_ERR_MESH : MBA_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int mba_mesh_read(const char* path, mba_mesh_part** parts, int32_t* n_parts)
+// Reads every part of the mesh file at path into malloc'ed arrays.
+int mesh_read(const char* path, mba_mesh_part** parts, int32_t* n_parts)
 {
-    if (!path || !parts || !n_parts) return MBA_ERR_ARG;
-    *parts = nullptr;
-    *n_parts = 0;
     const std::string ext = lower_ext(path);
     if (ext != ".obj" && ext != ".ply" && ext != ".json" && ext != ".dae" && ext != ".in") return MBA_ERR_ARG;
     std::string s;
@@ -558,6 +581,7 @@ int mba_mesh_read(const char* path, mba_mesh_part** parts, int32_t* n_parts)
         if (rc == MBA_OK) ps.push_back(std::move(p));
     }
     if (rc != MBA_OK) return rc;
+    if (ps.size() > (size_t)INT32_MAX) return MBA_ERR_SIZE;
     for (const Part& p : ps)
         if (!check_indices(p)) return MBA_ERR_MESH;
     mba_mesh_part* out = (mba_mesh_part*)calloc(ps.size(), sizeof(mba_mesh_part));
@@ -571,12 +595,25 @@ int mba_mesh_read(const char* path, mba_mesh_part** parts, int32_t* n_parts)
             mba_mesh_free(out, (int32_t)ps.size());
             return MBA_ERR_ALLOC;
         }
-        memcpy(out[i].verts, ps[i].v.data(), ps[i].v.size() * sizeof(double));
-        memcpy(out[i].faces, ps[i].f.data(), ps[i].f.size() * sizeof(int32_t));
+        // an empty part has no source arrays (data() may be null)
+        if (!ps[i].v.empty()) memcpy(out[i].verts, ps[i].v.data(), ps[i].v.size() * sizeof(double));
+        if (!ps[i].f.empty()) memcpy(out[i].faces, ps[i].f.data(), ps[i].f.size() * sizeof(int32_t));
     }
     *parts = out;
     *n_parts = (int32_t)ps.size();
     return MBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mba_mesh_read(const char* path, mba_mesh_part** parts, int32_t* n_parts)
+{
+    if (!path || !parts || !n_parts) return MBA_ERR_ARG;
+    *parts = nullptr;
+    *n_parts = 0;
+    return mba_guard([&] { return mesh_read(path, parts, n_parts); }, MBA_ERR_MESH);
 }
 
 void mba_mesh_free(mba_mesh_part* parts, int32_t n_parts)

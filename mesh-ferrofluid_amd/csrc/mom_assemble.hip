@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mba_common.hpp"
+#include "mba_guard.hpp"
 
 namespace {
 
@@ -1209,9 +1210,12 @@ int mba_mom_zmat_faceedge_ns_batched(int prec, const void* verts, const int32_t*
         sum += d.n_faces;
     }
     if (sum != total_faces) return MBA_ERR_ARG;
-    if (prec == MBA_F32)
-        return zmat_ns_batched<float>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
-    return zmat_ns_batched<double>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
+    return mba_guard([&] {
+        if (prec == MBA_F32)
+            return zmat_ns_batched<float>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace,
+                                          S(stream));
+        return zmat_ns_batched<double>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
+    });
 }
 
 }  // extern "C"

@@ -27,35 +27,42 @@ HIPCC_FLAGS = [
 
 
 def build(verbose: bool = False, jobs: int = 3) -> str:
-    """Compile every source to an object, link the shared library, return its path."""
+    """Compile every source to an object (at most `jobs` compilers at once),
+    link the shared library, return its path."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
-    procs = []
     build_dir = os.path.join(ROOT, "_build")
     os.makedirs(build_dir, exist_ok=True)
+    cmds = []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
-        objs.append(obj)
-        cmd = [hipcc, *HIPCC_FLAGS, "-c", path, "-o", obj]
+        cmds.append((src, obj, [hipcc, *HIPCC_FLAGS, "-c", path, "-o", obj]))
+
+    def compile_one(item):
+        src, _, cmd = item
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
-        while len([p for _, p in procs if p.poll() is None]) >= jobs:
-            procs[0][1].wait() if procs[0][1].poll() is None else None
-            break
+        # capture_output drains both pipes while the compiler runs, so a
+        # compile that prints a lot cannot block on a full pipe
+        return src, subprocess.run(cmd, capture_output=True)
+
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
+        results = list(pool.map(compile_one, cmds))
     failed = []
-    for src, p in procs:
-        out, _ = p.communicate()
-        if p.returncode != 0:
-            failed.append((src, out.decode(errors="replace")))
+    for src, r in results:
+        out = (r.stdout + r.stderr).decode(errors="replace")
+        if r.returncode != 0:
+            failed.append((src, out))
         elif verbose and out:
-            print(out.decode(errors="replace"))
+            print(out)
     if failed:
         msg = "\n".join(f"--- {s}\n{o}" for s, o in failed)
         raise RuntimeError(f"hipcc failed:\n{msg}")
+    objs = [obj for _, obj, _ in cmds]
     tmp = LIB_PATH + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:

@@ -145,3 +145,32 @@ def test_errors(tmp_path):
     p.write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\n")  # vertex 3 does not exist
     with pytest.raises(MBAError):
         read_mesh_parts(str(p))
+
+
+@pytest.mark.parametrize("name,text", [
+    # counts far beyond what the file holds: rejected before any allocation
+    # (once aborted the process with std::bad_alloc)
+    ("huge.in", "1 1000000000000 1"),
+    ("huge2.in", "1 3 4611686018427387904\n0 0 0\n1 0 0\n0 1 0\n1 2 3\n"),
+    # a 1-based face index past the vertex count / zero
+    ("oob.in", "1\n3 1\n0 0 0\n1 0 0\n0 1 0\n1 2 4\n"),
+    ("zero.in", "1\n3 1\n0 0 0\n1 0 0\n0 1 0\n0 1 2\n"),
+    # indices that do not fit int32 must not wrap onto valid vertices
+    ("wrap.obj", "v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4294967299\n"),
+    ("wrap.json", '{"verts": [[0,0,0],[1,0,0],[0,1,0]], "faces": [[0, 1, 4294967298]]}'),
+    ("nan.json", '{"verts": [[0,0,0],[1,0,0],[0,1,0]], "faces": [[0, 1, 1e300]]}'),
+])
+def test_malformed_counts_and_indices_return_errors(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    with pytest.raises(MBAError):
+        read_mesh_parts(str(p))
+
+
+def test_ply_huge_list_count_is_an_error(tmp_path):
+    p = tmp_path / "m.ply"
+    hdr = ("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+           "element face 1\nproperty list uchar int vertex_indices\nend_header\n")
+    p.write_text(hdr + "0 0 0\n1 0 0\n0 1 0\n3 0 1 4294967298\n")
+    with pytest.raises(MBAError):
+        read_mesh_parts(str(p))
