@@ -62,10 +62,21 @@ static double rwg_coef(const double *verts, const int32_t *faces, const double *
  * Z_loc(f1 e1, f2 e2) of .cu:332-496 with the vertex rotation of .cu:391-403:
  * the test points of local edge e1 are generated from (v_e1, v_e1+1, v_e1+2),
  * the source points likewise from e2.  out: [3F, 3F] complex, row = 3 f1 + e1. */
-void orc_ref_zmat_faceedge(const double *verts, const int32_t *faces, const double *areas,
-                           const double *normals, const double *gpx, const double *gwx, int nqx,
-                           const double *gpy, const double *gwy, int nqy, double k, int64_t F,
-                           double *out)
+/* Terms of the reference's CFIE integrand (.cu:426-492), selectable so each
+ * can be pinned on its own (EFIE part, MFIE part, identity term):
+ *   ZT_ID   1/2 f1.f2 on f1 == f2                   (.cu:426-436)
+ *   ZT_K    f1.(n1 x sum W f2 x grad_x G)           (.cu:472-473, 482-489)
+ *   ZT_VEC  f1.(jk sum W f2 G)                      (.cu:475-476)
+ *   ZT_SCA  div1 (-j/k) sum W div2 G                (.cu:479-480, 491-492)
+ * The reference sums all four (ZT_ALL). */
+enum { ZT_ID = 1, ZT_K = 2, ZT_VEC = 4, ZT_SCA = 8, ZT_ALL = 15 };
+/* RHS terms: RT_E f.E_inc, RT_H f.(n x H_inc)  (.cu:316-326). */
+enum { RT_E = 1, RT_H = 2, RT_ALL = 3 };
+
+void orc_ref_zmat_faceedge_terms(const double *verts, const int32_t *faces, const double *areas,
+                                 const double *normals, const double *gpx, const double *gwx, int nqx,
+                                 const double *gpy, const double *gwy, int nqy, double k, int64_t F,
+                                 int terms, double *out)
 {
     const int64_t N = 3 * F;
 #pragma omp parallel for schedule(dynamic, 4)
@@ -97,7 +108,7 @@ void orc_ref_zmat_faceedge(const double *verts, const int32_t *faces, const doub
                     x[a] = (1.0 - r1) * P0[a] + (r1 - r2) * P1[a] + r2 * P2[a];
                     f1v[a] = c1 * (x[a] - P0[a]);
                 }
-                if (f1 == f2) {
+                if (f1 == f2 && (terms & ZT_ID)) {
                     double f2x[3];
                     for (int a = 0; a < 3; ++a) f2x[a] = c2 * (x[a] - Q0[a]);
                     acc += W1 * 0.5 * dot3(f1v, f2x);
@@ -131,8 +142,9 @@ void orc_ref_zmat_faceedge(const double *verts, const int32_t *faces, const doub
                 nK[0] = n1[1] * K[2] - n1[2] * K[1];
                 nK[1] = n1[2] * K[0] - n1[0] * K[2];
                 nK[2] = n1[0] * K[1] - n1[1] * K[0];
-                for (int a = 0; a < 3; ++a) acc += W1 * f1v[a] * (nK[a] + L1[a]);
-                acc += W1 * div1 * L2;
+                for (int a = 0; a < 3; ++a)
+                    acc += W1 * f1v[a] * (((terms & ZT_K) ? nK[a] : 0.0) + ((terms & ZT_VEC) ? L1[a] : 0.0));
+                if (terms & ZT_SCA) acc += W1 * div1 * L2;
             }
             out[2 * (fe1 * N + fe2)] = creal(acc);
             out[2 * (fe1 * N + fe2) + 1] = cimag(acc);
@@ -140,13 +152,21 @@ void orc_ref_zmat_faceedge(const double *verts, const int32_t *faces, const doub
     }
 }
 
+void orc_ref_zmat_faceedge(const double *verts, const int32_t *faces, const double *areas,
+                           const double *normals, const double *gpx, const double *gwx, int nqx,
+                           const double *gpy, const double *gwy, int nqy, double k, int64_t F,
+                           double *out)
+{
+    orc_ref_zmat_faceedge_terms(verts, faces, areas, normals, gpx, gwx, nqx, gpy, gwy, nqy, k, F, ZT_ALL, out);
+}
+
 /* ------------------------------------------------------------ ref mode RHS
  * E = z e^{-j k khat.x}, H = (cos(-60deg), sin(-60deg), 0) e^{-j k khat.x},
  * khat = (cos 30deg, sin 30deg, 0); V = sum W f.(E + n x H)  (.cu:24-70, 244-330).
  * out: [F, 3] complex. */
-void orc_ref_rhs_faceedge(const double *verts, const int32_t *faces, const double *areas,
-                          const double *normals, const double *gp, const double *gw, int nq,
-                          double k, int64_t F, double *out)
+void orc_ref_rhs_faceedge_terms(const double *verts, const int32_t *faces, const double *areas,
+                                const double *normals, const double *gp, const double *gw, int nq,
+                                double k, int64_t F, int terms, double *out)
 {
     const double az = 30.0 / 180.0 * PI;
     const double kh[3] = {cos(az), sin(az), 0.0};
@@ -174,11 +194,19 @@ void orc_ref_rhs_faceedge(const double *verts, const int32_t *faces, const doubl
             double nxh[3];
             cross3(n, hp, nxh);
             double Ev[3] = {0.0, 0.0, 1.0};
-            for (int a = 0; a < 3; ++a) acc += W * fv[a] * (Ev[a] + nxh[a]) * ph;
+            for (int a = 0; a < 3; ++a)
+                acc += W * fv[a] * (((terms & RT_E) ? Ev[a] : 0.0) + ((terms & RT_H) ? nxh[a] : 0.0)) * ph;
         }
         out[2 * fe] = creal(acc);
         out[2 * fe + 1] = cimag(acc);
     }
+}
+
+void orc_ref_rhs_faceedge(const double *verts, const int32_t *faces, const double *areas,
+                          const double *normals, const double *gp, const double *gw, int nq,
+                          double k, int64_t F, double *out)
+{
+    orc_ref_rhs_faceedge_terms(verts, faces, areas, normals, gp, gw, nq, k, F, RT_ALL, out);
 }
 
 /* ------------------------------------------------------------ interpolation

@@ -52,6 +52,8 @@ def lib():
         i64, i32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_double
         L.orc_ref_zmat_faceedge.argtypes = [D, I32, D, D, D, D, i32, D, D, i32, dbl, i64, D]
         L.orc_ref_rhs_faceedge.argtypes = [D, I32, D, D, D, D, i32, dbl, i64, D]
+        L.orc_ref_zmat_faceedge_terms.argtypes = [D, I32, D, D, D, D, i32, D, D, i32, dbl, i64, i32, D]
+        L.orc_ref_rhs_faceedge_terms.argtypes = [D, I32, D, D, D, D, i32, dbl, i64, i32, D]
         L.orc_interp_faceedge.argtypes = [D, I32, I32, D, D, i64, i64, i64, D]
         L.orc_tri_potentials.argtypes = [D, D, D, D, D, D, D]
         L.orc_ns_zmat_faceedge.argtypes = [D, I32, D, D, D, i32, dbl, i64, I64, I32, D]
@@ -63,7 +65,8 @@ def lib():
         L.orc_laplace_layer.argtypes = [D, I32, D, D, i64, U8, ctypes.POINTER(D), I64, i32, i32, D]
         L.orc_tri_S0_grad.argtypes = [D, D, D, D, D, D]
         L.orc_laplace_p0.argtypes = [D, I32, i64, D, D, i32, I64, I32, D, D]
-        for f in (L.orc_ref_zmat_faceedge, L.orc_ref_rhs_faceedge, L.orc_interp_faceedge,
+        for f in (L.orc_ref_zmat_faceedge, L.orc_ref_rhs_faceedge, L.orc_ref_zmat_faceedge_terms,
+                  L.orc_ref_rhs_faceedge_terms, L.orc_interp_faceedge,
                   L.orc_tri_potentials, L.orc_ns_zmat_faceedge, L.orc_ns_rhs_faceedge,
                   L.orc_gather_rwg, L.orc_gather_rhs, L.orc_laplace_layer, L.orc_tri_S0_grad,
                   L.orc_laplace_p0):
@@ -171,24 +174,34 @@ def near_pairs_bruteforce(verts, faces, eta=2.0):
 
 
 # ------------------------------------------------------------------ MoM, ref mode
-def ref_zmat_faceedge(verts, faces, gpx, gwx, gpy, gwy, k):
+# Terms of the reference's CFIE (mesh_MOM_3d.cu:426-492): identity 1/2 f.f on
+# the self pair, MFIE n x (f x grad G), EFIE vector potential jk f.f G, EFIE
+# scalar potential -(j/k) div div G; RHS E_inc and n x H_inc (.cu:316-326).
+ZT_ID, ZT_K, ZT_VEC, ZT_SCA = 1, 2, 4, 8
+ZT_ALL, ZT_EFIE, ZT_MFIE = 15, ZT_VEC | ZT_SCA, ZT_ID | ZT_K
+RT_E, RT_H = 1, 2
+RT_ALL = 3
+
+
+def ref_zmat_faceedge(verts, faces, gpx, gwx, gpy, gwy, k, terms=ZT_ALL):
     v, f = _f64(verts), _fi32(faces)
     A, n = face_areas_normals(v, f)
     gpx, gwx, gpy, gwy = map(_f64, (gpx, gwx, gpy, gwy))
     F = len(f)
     out = np.zeros((3 * F, 3 * F), dtype=np.complex128)
-    lib().orc_ref_zmat_faceedge(_d(v), _i32(f), _d(A), _d(_f64(n)), _d(gpx), _d(gwx), gpx.size,
-                                _d(gpy), _d(gwy), gpy.size, float(k), F, _d(out.view(np.float64)))
+    lib().orc_ref_zmat_faceedge_terms(_d(v), _i32(f), _d(A), _d(_f64(n)), _d(gpx), _d(gwx), gpx.size,
+                                      _d(gpy), _d(gwy), gpy.size, float(k), F, int(terms),
+                                      _d(out.view(np.float64)))
     return out
 
 
-def ref_rhs_faceedge(verts, faces, gp, gw, k):
+def ref_rhs_faceedge(verts, faces, gp, gw, k, terms=RT_ALL):
     v, f = _f64(verts), _fi32(faces)
     A, n = face_areas_normals(v, f)
     gp, gw = _f64(gp), _f64(gw)
     out = np.zeros((len(f), 3), dtype=np.complex128)
-    lib().orc_ref_rhs_faceedge(_d(v), _i32(f), _d(A), _d(_f64(n)), _d(gp), _d(gw), gp.size,
-                               float(k), len(f), _d(out.view(np.float64)))
+    lib().orc_ref_rhs_faceedge_terms(_d(v), _i32(f), _d(A), _d(_f64(n)), _d(gp), _d(gw), gp.size,
+                                     float(k), len(f), int(terms), _d(out.view(np.float64)))
     return out
 
 
@@ -232,19 +245,21 @@ def gather_rhs(VF, rfe):
     return V
 
 
-def ref_solve_MOM(verts, faces, gaussQR, k):
+def ref_solve_MOM(verts, faces, gaussQR, k, zterms=ZT_ALL, rterms=RT_ALL, gaussQR_y=None):
     """FP64 restatement of MeshBEMSolver3d.solve_MOM (mesh_BEM_solver_3d.py:96-195).
 
-    Returns (vert_Js [1, V, 3], I [E], Z [E, E], V [E], edges, fe_idx)."""
+    zterms / rterms select parts of the CFIE (all by default, as the
+    reference); gaussQR_y overrides the source rule order (the reference uses
+    gaussQR - 1).  Returns (vert_Js [1, V, 3], I [E], Z [E, E], V [E], edges, fe_idx)."""
     v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
     edges, fe = edges_bruteforce(f, len(v))
     E = len(edges)
     gpx, gwx = gl_ref(gaussQR)
-    gpy, gwy = gl_ref(gaussQR - 1)
-    ZF = ref_zmat_faceedge(v, f, gpx, gwx, gpy, gwy, k)
+    gpy, gwy = gl_ref(gaussQR - 1 if gaussQR_y is None else gaussQR_y)
+    ZF = ref_zmat_faceedge(v, f, gpx, gwx, gpy, gwy, k, zterms)
     rfe = rwg_fe(fe, E)
     Z = gather_rwg(ZF, rfe)
-    VF = ref_rhs_faceedge(v, f, gpx, gwx, k)
+    VF = ref_rhs_faceedge(v, f, gpx, gwx, k, rterms)
     V = gather_rhs(VF.reshape(-1, 1), rfe)[:, 0]
     import scipy.linalg
 
