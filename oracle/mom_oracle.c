@@ -474,30 +474,85 @@ void orc_gather_rhs(const double *VF, const int32_t *rwg_fe, int64_t E, int64_t 
  *   phi (LINEAR) = (1 - r1, r1 - r2, r2); PLANAR: 1 (nphi = 1).
  * (integrations_3d.py:345-450, BEM_utils.py:72-104, 141-172, 175-207.)
  * out: [F, F, nphi, nphi] double. */
-void orc_laplace_layer(const double *verts, const int32_t *faces, const double *areas,
-                       const double *normals, int64_t F, const uint8_t *rel,
-                       const double *const *tab, const int64_t *n_tab, int layer, int nphi,
-                       double *out)
+/* Laplace single (layer 1: 1/(4 pi R)) / double (layer 2: (x - y).n_y /
+ * (4 pi R^3), n_y the normal of the row face fy) layer on relation-typed
+ * point tables (integrations_3d.py:252-452).  out[fy, fx, iy, ix] =
+ * 4 A_x A_y sum_g W_g phi_iy(y_g) phi_ix(x_g) kernel, with x_g, y_g from the
+ * table rows (r1x, r2x, r1y, r2y, W) mapped through each face's local vertex
+ * order, as the reference maps them (no reordering).
+ *
+ * fix (bit mask; 0 reproduces the reference):
+ *   LAP_REORDER   for same-vertex / same-edge pairs, map through local
+ *                 vertex orders that put the shared vertex (edge) first in
+ *                 both faces, as the Sauter-Schwab maps assume
+ *   LAP_SYMMETRY  same-face pairs: add the transposed block, i.e. the three
+ *                 sub-domains x <-> y the reference's maps leave out
+ * With both bits and a proper (non-collapsed) 4-D rule in the tables, out
+ * is the exact Galerkin integral to the rule's accuracy. */
+enum { LAP_REORDER = 1, LAP_SYMMETRY = 2 };
+
+/* Local orders of faces X (fx) and Y (fy) putting their shared vertices
+ * first: shared vertex at position 0 (relation 1), shared edge at positions
+ * (0, 1) with the same two vertices in both (relation 2); X keeps its cyclic
+ * order, Y follows X's edge direction. */
+static void shared_first(const int32_t *X, const int32_t *Y, int r, int px[3], int py[3])
+{
+    for (int i = 0; i < 3; ++i) px[i] = py[i] = i;
+    if (r == 1) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                if (X[a] == Y[b])
+                    for (int i = 0; i < 3; ++i) {
+                        px[i] = (a + i) % 3;
+                        py[i] = (b + i) % 3;
+                    }
+    } else if (r == 2) {
+        int a0 = -1;
+        for (int a = 0; a < 3 && a0 < 0; ++a) {
+            int in_y0 = 0, in_y1 = 0;
+            for (int b = 0; b < 3; ++b) {
+                in_y0 |= X[a] == Y[b];
+                in_y1 |= X[(a + 1) % 3] == Y[b];
+            }
+            if (in_y0 && in_y1) a0 = a;
+        }
+        if (a0 < 0) return;
+        for (int i = 0; i < 3; ++i) px[i] = (a0 + i) % 3;
+        for (int b = 0; b < 3; ++b) {
+            if (Y[b] == X[px[0]]) py[0] = b;
+            if (Y[b] == X[px[1]]) py[1] = b;
+        }
+        py[2] = 3 - py[0] - py[1];
+    }
+}
+
+void orc_laplace_layer_ex(const double *verts, const int32_t *faces, const double *areas,
+                          const double *normals, int64_t F, const uint8_t *rel,
+                          const double *const *tab, const int64_t *n_tab, int layer, int nphi,
+                          int fix, double *out)
 {
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t fy = 0; fy < F; ++fy) {
-        const double *Y[3] = {verts + 3 * faces[3 * fy], verts + 3 * faces[3 * fy + 1],
-                              verts + 3 * faces[3 * fy + 2]};
         const double *ny = normals + 3 * fy;
         for (int64_t fx = 0; fx < F; ++fx) {
-            const double *X[3] = {verts + 3 * faces[3 * fx], verts + 3 * faces[3 * fx + 1],
-                                  verts + 3 * faces[3 * fx + 2]};
             int r = rel[fy * F + fx];
+            int px[3] = {0, 1, 2}, py[3] = {0, 1, 2};
+            if ((fix & LAP_REORDER) && (r == 1 || r == 2)) shared_first(faces + 3 * fx, faces + 3 * fy, r, px, py);
+            const double *X[3], *Y[3];
+            for (int i = 0; i < 3; ++i) {
+                X[i] = verts + 3 * faces[3 * fx + px[i]];
+                Y[i] = verts + 3 * faces[3 * fy + py[i]];
+            }
             const double *t = tab[r];
             double acc[9] = {0};
             for (int64_t g = 0; g < n_tab[r]; ++g) {
                 double r1x = t[5 * g], r2x = t[5 * g + 1], r1y = t[5 * g + 2], r2y = t[5 * g + 3];
                 double W = t[5 * g + 4];
-                double px[3] = {1.0 - r1x, r1x - r2x, r2x}, py[3] = {1.0 - r1y, r1y - r2y, r2y};
+                double bx[3] = {1.0 - r1x, r1x - r2x, r2x}, by[3] = {1.0 - r1y, r1y - r2y, r2y};
                 double x[3], y[3], d[3];
                 for (int a = 0; a < 3; ++a) {
-                    x[a] = px[0] * X[0][a] + px[1] * X[1][a] + px[2] * X[2][a];
-                    y[a] = py[0] * Y[0][a] + py[1] * Y[1][a] + py[2] * Y[2][a];
+                    x[a] = bx[0] * X[0][a] + bx[1] * X[1][a] + bx[2] * X[2][a];
+                    y[a] = by[0] * Y[0][a] + by[1] * Y[1][a] + by[2] * Y[2][a];
                 }
                 sub3(x, y, d);
                 double R = norm3(d);
@@ -507,12 +562,26 @@ void orc_laplace_layer(const double *verts, const int32_t *faces, const double *
                     acc[0] += W * fv;
                 else
                     for (int iy = 0; iy < 3; ++iy)
-                        for (int ix = 0; ix < 3; ++ix) acc[iy * 3 + ix] += W * py[iy] * px[ix] * fv;
+                        for (int ix = 0; ix < 3; ++ix) acc[py[iy] * 3 + px[ix]] += W * by[iy] * bx[ix] * fv;
+            }
+            if ((fix & LAP_SYMMETRY) && r == 3) {
+                double sym[9];
+                for (int iy = 0; iy < nphi; ++iy)
+                    for (int ix = 0; ix < nphi; ++ix) sym[iy * nphi + ix] = acc[iy * nphi + ix] + acc[ix * nphi + iy];
+                memcpy(acc, sym, sizeof(double) * nphi * nphi);
             }
             double s = 4.0 * areas[fx] * areas[fy];
             for (int q = 0; q < nphi * nphi; ++q) out[(fy * F + fx) * nphi * nphi + q] = s * acc[q];
         }
     }
+}
+
+void orc_laplace_layer(const double *verts, const int32_t *faces, const double *areas,
+                       const double *normals, int64_t F, const uint8_t *rel,
+                       const double *const *tab, const int64_t *n_tab, int layer, int nphi,
+                       double *out)
+{
+    orc_laplace_layer_ex(verts, faces, areas, normals, F, rel, tab, n_tab, layer, nphi, 0, out);
 }
 
 /* ------------------------------------------------------------ P0 Laplace (ns)

@@ -63,12 +63,14 @@ def lib():
         L.orc_gather_rwg.argtypes = [D, i64, I32, i64, D]
         L.orc_gather_rhs.argtypes = [D, I32, i64, i64, D]
         L.orc_laplace_layer.argtypes = [D, I32, D, D, i64, U8, ctypes.POINTER(D), I64, i32, i32, D]
+        L.orc_laplace_layer_ex.argtypes = [D, I32, D, D, i64, U8, ctypes.POINTER(D), I64, i32, i32, i32, D]
         L.orc_tri_S0_grad.argtypes = [D, D, D, D, D, D]
         L.orc_laplace_p0.argtypes = [D, I32, i64, D, D, i32, I64, I32, D, D]
         for f in (L.orc_ref_zmat_faceedge, L.orc_ref_rhs_faceedge, L.orc_ref_zmat_faceedge_terms,
                   L.orc_ref_rhs_faceedge_terms, L.orc_interp_faceedge,
                   L.orc_tri_potentials, L.orc_ns_zmat_faceedge, L.orc_ns_rhs_faceedge,
-                  L.orc_gather_rwg, L.orc_gather_rhs, L.orc_laplace_layer, L.orc_tri_S0_grad,
+                  L.orc_gather_rwg, L.orc_gather_rhs, L.orc_laplace_layer, L.orc_laplace_layer_ex,
+                  L.orc_tri_S0_grad,
                   L.orc_laplace_p0):
             f.restype = None
         _LIB = L
@@ -395,13 +397,32 @@ def laplace_tables_from(r1, r2, w):
     return tabs
 
 
+def proper_tables(N):
+    """Sauter-Schwab tables from a genuine 4-D tensor Gauss-Legendre rule of
+    order N (FP64 nodes): (xsi, eta1) and (eta2, eta3) independent, no
+    collapse.  The reference instead feeds its collapsed rank-3 rule
+    (``laplace_tables``)."""
+    gp, gw = np.polynomial.legendre.leggauss(N)
+    gp, gw = (gp + 1.0) / 2.0, gw / 2.0
+    return laplace_tables_from(np.repeat(gp, N), np.tile(gp, N), np.repeat(gw, N) * np.tile(gw, N))
+
+
+# Fixes of the reference's Laplace rule (orc_laplace_layer_ex): shared vertex /
+# edge first for the Sauter-Schwab maps; the left-out same-face sub-domains.
+LAP_REORDER, LAP_SYMMETRY = 1, 2
+LAP_EXACT = LAP_REORDER | LAP_SYMMETRY
+
+
 def panel_relation_bruteforce(faces):
     f = np.asarray(faces, dtype=np.int64)
     return (f[:, None, :, None] == f[None, :, None, :]).sum(axis=(2, 3)).astype(np.uint8)
 
 
-def laplace_layer(verts, faces, gp, gw, layer, order_type, rel=None, tabs=None):
-    """Unpacked [F, F, nphi, nphi] single (layer=1) or double (layer=2) layer."""
+def laplace_layer(verts, faces, gp, gw, layer, order_type, rel=None, tabs=None, fix=0):
+    """Unpacked [F, F, nphi, nphi] single (layer=1) or double (layer=2) layer.
+
+    fix = 0 and tabs from (gp, gw) reproduce the reference; fix = LAP_EXACT with
+    tabs = proper_tables(N) gives the exact Galerkin layers."""
     v, f = _f64(verts), _fi32(faces)
     A, n = face_areas_normals(v, f)
     F = len(f)
@@ -415,9 +436,9 @@ def laplace_layer(verts, faces, gp, gw, layer, order_type, rel=None, tabs=None):
     tp = (P * 4)(*[_d(t) for t in tabs])
     nt = np.array([t.shape[0] for t in tabs], dtype=np.int64)
     out = np.zeros((F, F, nphi, nphi))
-    lib().orc_laplace_layer(_d(v), _i32(f), _d(A), _d(_f64(n)), F,
-                            rel.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), tp, _i64(nt),
-                            int(layer), nphi, _d(out))
+    lib().orc_laplace_layer_ex(_d(v), _i32(f), _d(A), _d(_f64(n)), F,
+                               rel.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), tp, _i64(nt),
+                               int(layer), nphi, int(fix), _d(out))
     return out
 
 
@@ -435,17 +456,33 @@ def pack_matrix_tight(order_type, U, faces, n_verts):
     return out.reshape(n_verts, n_verts)
 
 
-def ref_laplace_dirichlet(verts, faces, g, gaussQR, order_type=1):
+def consistent_mass(verts, faces):
+    """P1 mass matrix M_ij = int phi_i phi_j (A/12 (1 + delta_ij) per face pair of corners)."""
+    v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
+    A, _ = face_areas_normals(v, f)
+    M = np.zeros((len(v), len(v)))
+    loc = (np.ones((3, 3)) + np.eye(3)) / 12.0
+    for a in range(3):
+        for b in range(3):
+            np.add.at(M, (f[:, a], f[:, b]), A * loc[a, b])
+    return M
+
+
+def ref_laplace_dirichlet(verts, faces, g, gaussQR, order_type=1, tabs=None, fix=0, mass="lumped"):
     """FP64 restatement of MeshBEMSolver3d.solve for an all-Dirichlet batch of one.
 
-    g: [V] Dirichlet data.  Returns Neumann data t [V] solving V t = 1/2 M g + K^T g."""
+    g: [V] Dirichlet data.  Returns Neumann data t [V] solving V t = 1/2 M g + K^T g.
+    tabs / fix replace the reference's layer rule (``laplace_layer``); mass
+    "lumped" is the reference's 1/2 g vert_area (rhs_layer_3d.py:127),
+    "consistent" the Galerkin P1 mass matrix."""
     v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
     nV = len(v)
     gp, gw = gl_ref(gaussQR)
     rel = panel_relation_bruteforce(f)
-    tabs = laplace_tables(gp, gw)
-    Vu = laplace_layer(v, f, gp, gw, 1, order_type, rel, tabs)
-    Ku = laplace_layer(v, f, gp, gw, 2, order_type, rel, tabs)
+    if tabs is None:
+        tabs = laplace_tables(gp, gw)
+    Vu = laplace_layer(v, f, gp, gw, 1, order_type, rel, tabs, fix)
+    Ku = laplace_layer(v, f, gp, gw, 2, order_type, rel, tabs, fix)
     Vt = pack_matrix_tight(order_type, Vu, f, nV)
     if order_type == 0:
         Ku = np.repeat(np.repeat(Ku.reshape(len(f), len(f), 1, 1), 3, axis=2), 3, axis=3) / 9.0
@@ -453,7 +490,10 @@ def ref_laplace_dirichlet(verts, faces, g, gaussQR, order_type=1):
     src = np.einsum("yxij,yi->xj", Ku, gf)  # sum over fy, iy
     gk2 = np.zeros(nV)
     np.add.at(gk2, f.reshape(-1), src.reshape(-1))
-    rhs = 0.5 * np.asarray(g) * vert_areas(v, f) + gk2
+    if mass == "lumped":
+        rhs = 0.5 * np.asarray(g) * vert_areas(v, f) + gk2
+    else:
+        rhs = 0.5 * consistent_mass(v, f) @ np.asarray(g) + gk2
     import scipy.linalg
 
     return scipy.linalg.solve(Vt, rhs), Vt, rhs
