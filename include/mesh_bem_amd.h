@@ -70,6 +70,27 @@ int mba_mom_zmat_faceedge_ref(int prec, const void *verts, const int32_t *faces,
                               int32_t nq_y, double wavenumber, int64_t n_verts, int64_t n_faces,
                               void *zf, int64_t ldzf, mba_stream_t stream);
 
+/* Device memory of mba_mom_zmat_faceedge_ref_noalloc per GPU (bytes). */
+#define MBA_NOALLOC_SCRATCH_BYTES (4u << 20)
+
+/* mba_mom_zmat_faceedge_ref without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and result as mba_mom_zmat_faceedge_ref; the
+ * per-face records it needs (96 B per face in MBA_F32, 192 B in MBA_F64) live
+ * in a fixed device array of MBA_NOALLOC_SCRATCH_BYTES per GPU, so n_faces is
+ * at most 43 690 (MBA_F32) / 21 845 (MBA_F64); larger meshes return
+ * MBA_ERR_SIZE before any launch.  The array is shared by every call on one
+ * GPU: calls that overlap on one GPU (different streams or host threads
+ * without ordering) race on it.  Every record is written before it is read,
+ * so nothing needs clearing between calls.  The stream must belong to the
+ * current device.  Calls no hipMalloc, hipFree or allocating library routine. */
+int mba_mom_zmat_faceedge_ref_noalloc(int prec, const void *verts, const int32_t *faces,
+                                      const void *face_areas, const void *face_normals, const void *gp_x,
+                                      const void *gw_x, int32_t nq_x, const void *gp_y, const void *gw_y,
+                                      int32_t nq_y, double wavenumber, int64_t n_verts, int64_t n_faces,
+                                      void *zf, int64_t ldzf, mba_stream_t stream);
+
 /* Face-edge RHS, reference incidence: replaces create_MOM_rhs_3d_forward
  * (mesh_MOM_3d.cpp:15-36, kernel mesh_MOM_3d.cu:244-330).  vf: [F, 3] complex. */
 int mba_mom_rhs_faceedge_ref(int prec, const void *verts, const int32_t *faces,

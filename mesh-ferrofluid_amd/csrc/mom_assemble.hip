@@ -9,6 +9,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "mba_common.hpp"
@@ -842,14 +843,14 @@ size_t records_bytes(int prec, int64_t F)
     return ((size_t)F * rs + 255) & ~(size_t)255;
 }
 
+// Reference-rule face-edge blocks with the face records in `rec` (at least
+// records_bytes(prec, F) bytes of device memory; every record is written
+// before the Z kernel reads it).
 template <class T>
-int zmat_ref(const void* verts, const int32_t* faces, const void* areas, const void* normals,
-             const void* gpx, const void* gwx, int nqx, const void* gpy, const void* gwy, int nqy,
-             double k, int64_t F, void* zf, int64_t ldzf, hipStream_t st)
+int zmat_ref_with(const void* verts, const int32_t* faces, const void* areas, const void* normals,
+                  const void* gpx, const void* gwx, int nqx, const void* gpy, const void* gwy, int nqy,
+                  double k, int64_t F, void* zf, int64_t ldzf, FaceRec<T>* rec, hipStream_t st)
 {
-    FaceRec<T>* rec = nullptr;
-    if (hipMallocAsync((void**)&rec, records_bytes(sizeof(T) == 4 ? MBA_F32 : MBA_F64, F), st) != hipSuccess)
-        return MBA_ERR_ALLOC;
     int rc = face_records<T>(verts, faces, areas, normals, F, rec, st);
     if (rc == MBA_OK) {
         const int NX = nqx * nqx, NY = nqy * nqy;
@@ -865,6 +866,19 @@ int zmat_ref(const void* verts, const int32_t* faces, const void* areas, const v
                            (T)(k / (2.0 * MBA_PI)), rpb, (cpx<T>*)zf, ldzf);
         if (hipGetLastError() != hipSuccess) rc = MBA_ERR_LAUNCH;
     }
+    return rc;
+}
+
+template <class T>
+int zmat_ref(const void* verts, const int32_t* faces, const void* areas, const void* normals,
+             const void* gpx, const void* gwx, int nqx, const void* gpy, const void* gwy, int nqy,
+             double k, int64_t F, void* zf, int64_t ldzf, hipStream_t st)
+{
+    FaceRec<T>* rec = nullptr;
+    if (hipMallocAsync((void**)&rec, records_bytes(sizeof(T) == 4 ? MBA_F32 : MBA_F64, F), st) != hipSuccess)
+        return MBA_ERR_ALLOC;
+    const int rc = zmat_ref_with<T>(verts, faces, areas, normals, gpx, gwx, nqx, gpy, gwy, nqy, k, F, zf, ldzf,
+                                    rec, st);
     (void)hipFreeAsync(rec, st);
     return rc;
 }
@@ -960,6 +974,41 @@ int zmat_ns_batched(const void* verts, const int32_t* faces, int64_t total_faces
 
 }  // namespace
 
+// ======================================================= no-alloc scratch
+// Device memory of mba_mom_zmat_faceedge_ref_noalloc: the per-face records of
+// the reference-rule assembly, MBA_NOALLOC_SCRATCH_BYTES per GPU, held by
+// every GPU the library runs on until the process ends.  Host code never
+// passes `scratch` itself to a kernel (in host code the name is a host-side
+// stand-in, not a device address): it uses the address hipGetSymbolAddress
+// returns for the current device.
+static __device__ __align__(256) unsigned char scratch[MBA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct DeviceScratch {
+    std::once_flag once;
+    void* ptr = nullptr;  // device address of `scratch` on this GPU, or null on failure
+};
+DeviceScratch g_scratch[kMaxDevices];
+
+// Address of the current device's copy of `scratch`, looked up once per
+// device with that device current; null if the lookup failed.
+void* scratch_address()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    DeviceScratch& d = g_scratch[dev];
+    std::call_once(d.once, [&] {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) d.ptr = p;
+    });
+    return d.ptr;
+}
+
+}  // namespace
+
 // ============================================================== C ABI
 extern "C" {
 
@@ -1005,6 +1054,29 @@ int mba_mom_zmat_faceedge_ref(int prec, const void* verts, const int32_t* faces,
                                nq_y, wavenumber, n_faces, zf, ldzf, S(stream));
     return zmat_ref<double>(verts, faces, face_areas, face_normals, gp_x, gw_x, nq_x, gp_y, gw_y,
                             nq_y, wavenumber, n_faces, zf, ldzf, S(stream));
+}
+
+int mba_mom_zmat_faceedge_ref_noalloc(int prec, const void* verts, const int32_t* faces,
+                                      const void* face_areas, const void* face_normals, const void* gp_x,
+                                      const void* gw_x, int32_t nq_x, const void* gp_y, const void* gw_y,
+                                      int32_t nq_y, double wavenumber, int64_t n_verts, int64_t n_faces,
+                                      void* zf, int64_t ldzf, mba_stream_t stream)
+{
+    if (!verts || !faces || !face_areas || !face_normals || !gp_x || !gw_x || !gp_y || !gw_y || !zf)
+        return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n_faces <= 0 || n_faces > kMaxFaces || n_verts <= 0) return MBA_ERR_SIZE;
+    if (nq_x < 1 || nq_x > 16 || nq_y < 1 || nq_y > 16) return MBA_ERR_SIZE;
+    if (ldzf < 3 * n_faces) return MBA_ERR_ARG;
+    if (wavenumber == 0.0) return MBA_ERR_ARG;
+    if (records_bytes(prec, n_faces) > MBA_NOALLOC_SCRATCH_BYTES) return MBA_ERR_SIZE;
+    void* rec = scratch_address();
+    if (!rec) return MBA_ERR_LAUNCH;
+    if (prec == MBA_F32)
+        return zmat_ref_with<float>(verts, faces, face_areas, face_normals, gp_x, gw_x, nq_x, gp_y, gw_y, nq_y,
+                                    wavenumber, n_faces, zf, ldzf, (FaceRec<float>*)rec, S(stream));
+    return zmat_ref_with<double>(verts, faces, face_areas, face_normals, gp_x, gw_x, nq_x, gp_y, gw_y, nq_y,
+                                 wavenumber, n_faces, zf, ldzf, (FaceRec<double>*)rec, S(stream));
 }
 
 int mba_mom_rhs_faceedge_ref(int prec, const void* verts, const int32_t* faces,

@@ -77,6 +77,8 @@ SIGNATURES = {
     "mba_version": (_c.c_int, []),
     "mba_mom_zmat_faceedge_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
                                              _i32, _dbl, _i64, _i64, _vp, _i64, _vp]),
+    "mba_mom_zmat_faceedge_ref_noalloc": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
+                                                     _i32, _dbl, _i64, _i64, _vp, _i64, _vp]),
     "mba_mom_rhs_faceedge_ref": (_c.c_int, [_c.c_int, _vp, _i32p, _vp, _vp, _vp, _vp, _i32, _dbl, _i64,
                                             _i64, _vp, _vp]),
     "mba_mom_interp_faceedge": (_c.c_int, [_c.c_int, _vp, _i32p, _i32p, _vp, _vp, _i64, _i64, _i64,
