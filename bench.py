@@ -785,12 +785,9 @@ def run_c3(args, dev):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("c64 factor (trailing update: FP32-accurate split-FP16 MFMA, 3 hi/lo piece products)"
-                  if args.lu_update == "split_f16" else "c64 factor (FP32 MFMA trailing update)")
-                 + " + c128 assembly/residual",
-        "value_definition": "1e-6 F^2 / t_step: face pairs per second of the whole mixed solve "
-                            "(FP64 assembly + FP32 LU + FP64 refinement); assembly-only rates are "
-                            "assembly_fp64_Mpairs_s / assembly_fp32_Mpairs_s",
+        "dtype": "c64 factors (lock-step batched LU, FP32 MFMA 3M trailing updates) + c128 assembly/residual",
+        "value_definition": "right-hand sides per second: meshes x waves / (FP64 assembly + RHS + FP32 LUs "
+                            "+ solves + FP64 refinement) per step",
         "data": "synthetic",
         "config": {"workload": f"{n_mesh} star meshes (geodesic frequency 5-11, RWG {min(Es)}-{max(Es)}), "
                                f"{n_dir} waves each, ka~U(1,8), EFIE 7-point rule", "parallelism": "single GPU"},

@@ -6,6 +6,7 @@ raise.  Tensors are passed as raw device pointers together with torch's
 current HIP stream.
 """
 
+import atexit
 import ctypes
 import os
 
@@ -197,7 +198,29 @@ def load():
             fn.restype = res
             fn.argtypes = args
         _LIB = lib
+        atexit.register(_quiesce)
     return _LIB
+
+
+def _quiesce():
+    """At interpreter exit: wait for all queued device work (no kernel of the
+    library is still running when the HIP runtime and any profiler tear down)
+    and hand the memory cached by the stream-ordered allocator (the library's
+    hipMallocAsync records) back to the device."""
+    try:
+        if not torch.cuda.is_available() or not torch.cuda.is_initialized():
+            return
+        torch.cuda.synchronize()
+        hip = ctypes.CDLL("libamdhip64.so")
+        dev = ctypes.c_int(0)
+        if hip.hipGetDevice(ctypes.byref(dev)) != 0:
+            return
+        pool = ctypes.c_void_p()
+        if hip.hipDeviceGetDefaultMemPool(ctypes.byref(pool), dev) == 0:
+            hip.hipMemPoolTrimTo(pool, ctypes.c_size_t(0))
+        hip.hipDeviceSynchronize()
+    except Exception:
+        pass
 
 
 def check(status: int, what: str = ""):

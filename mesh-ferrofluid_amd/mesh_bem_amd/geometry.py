@@ -165,6 +165,23 @@ def rwg_face_edges(face_edge_indices, n_edges: int) -> np.ndarray:
     return order.reshape(n_edges, 2).astype(np.int32)
 
 
+def check_orientation(faces: np.ndarray, rwg_fe: np.ndarray) -> None:
+    """Raise ValueError unless the two faces of every RWG edge traverse it in
+    opposite directions (a consistently oriented surface).  The RWG sign of
+    local edge e of face f is + iff faces[f][(e+1)%3] < faces[f][(e+2)%3]
+    (src/mesh_BEM/cuda/mesh_MOM_3d.cu:154-165); on a consistently oriented
+    surface exactly one face of every edge is T+, else the basis is invalid."""
+    f = np.asarray(faces, dtype=np.int64)
+    slots = np.asarray(rwg_fe, dtype=np.int64)
+    fi, ei = slots // 3, slots % 3
+    a = f[fi, (ei + 1) % 3]
+    b = f[fi, (ei + 2) % 3]
+    plus = a < b  # [E, 2]
+    bad = int(np.count_nonzero(plus[:, 0] == plus[:, 1]))
+    if bad:
+        raise ValueError(f"inconsistently oriented mesh: {bad} edges traversed in the same direction by both faces")
+
+
 def compute_panel_relation(faces: torch.Tensor) -> torch.Tensor:
     """[F,F] uint8 number of shared vertex indices (0..3 for triangles, 0..2
     for segments), in O(F) extra memory."""
@@ -185,12 +202,14 @@ def compute_panel_relation(faces: torch.Tensor) -> torch.Tensor:
 
 
 def check_mesh(verts: np.ndarray, faces: np.ndarray) -> None:
-    """Raise ValueError for index-out-of-range or zero-area faces."""
+    """Raise ValueError for index-out-of-range or zero-area faces (area at most
+    1e-12 of the squared longest edge: collinear corners up to rounding)."""
     if faces.size and (faces.min() < 0 or faces.max() >= verts.shape[0]):
         raise ValueError("face index out of range")
     v0, v1, v2 = verts[faces[:, 0]], verts[faces[:, 1]], verts[faces[:, 2]]
     a = 0.5 * np.linalg.norm(np.cross(v1 - v0, v2 - v1), axis=1)
-    if np.any(a <= 0.0):
+    l2 = np.maximum.reduce([((v1 - v0) ** 2).sum(1), ((v2 - v1) ** 2).sum(1), ((v0 - v2) ** 2).sum(1)])
+    if np.any(~(a > 1e-12 * l2)):
         raise ValueError("zero-area face")
 
 

@@ -9,8 +9,13 @@ Orthogonalisation is classical Gram-Schmidt applied twice (CGS2) on device.
 
 Convergence is declared on the true relative residual ||b - A x|| / ||b|| <=
 tol, recomputed at every restart; the Arnoldi estimate ends an inner cycle
-early.  Three restarts without a 10% drop of the true residual end the solve
-unconverged.
+early.  Three restarts without a 10% drop of the true residual end the
+iteration unconverged.
+
+Fallbacks (recorded in ``GmresInfo.fallback``): a warm start x0 holding NaN or
+Inf is replaced by zero ("x0-nonfinite"); with ``fallback="lu"`` an
+unconverged solve of a dense matrix finishes with a complex64 LU of the
+matrix and FP64-residual refinement ("lu").
 """
 
 from dataclasses import dataclass, field
@@ -31,6 +36,7 @@ class GmresInfo:
     converged: bool
     residual: float  # final true relative residual
     history: List[float] = field(default_factory=list)  # Arnoldi estimates per iteration
+    fallback: str = ""  # "", "x0-nonfinite", "lu" or "x0-nonfinite+lu"
 
 
 def _vprec(dt: torch.dtype) -> int:
@@ -100,11 +106,38 @@ def _givens(a, c):
     return abs(a) / r, (a / abs(a)) * np.conj(c) / r
 
 
+def _lu_finish(op: DenseOperator, b: torch.Tensor, x: torch.Tensor, tol: float, maxit: int = 10):
+    """x refined with a complex64 LU of the operator's matrix: r = b - A x in
+    FP64 (FP32 matrix, FP64 accumulation), x += LU \\ r, until ||r|| <= tol ||b||."""
+    from .dense import LUFactor
+
+    A = op.A
+    lu = LUFactor(A.to(torch.complex64) if A.dtype == torch.float32 else A.clone(), overwrite=True)
+    r = torch.empty_like(x)
+    bn = float(torch.linalg.vector_norm(b))
+    res = float("inf")
+    for _ in range(maxit + 1):
+        op.apply(x, r, b)
+        res = float(torch.linalg.vector_norm(r)) / bn
+        if res <= tol:
+            break
+        d = lu.solve(r.to(torch.complex64))
+        x += d.real.to(x.dtype) if not x.is_complex() else d.to(x.dtype)
+    return x, res
+
+
 def gmres(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, restart: int = 50, tol: float = 1e-6,
-          maxiter: int = 2000, callback: Optional[Callable] = None):
+          maxiter: int = 2000, callback: Optional[Callable] = None, fallback: str = "none"):
     """Solve A x = b.  A: square FP32 tensor (complex64 / float32) or an operator
-    with ``apply(x, y, b=None)``, ``n`` and ``dtype``.  Returns (x FP64, GmresInfo)."""
+    with ``apply(x, y, b=None)``, ``n`` and ``dtype``.  Returns (x FP64, GmresInfo).
+
+    fallback="lu" (dense A only): an unconverged iteration is finished by a
+    complex64 LU with FP64-residual refinement."""
+    if fallback not in ("none", "lu"):
+        raise ValueError("fallback: 'none' or 'lu'")
     op = A if hasattr(A, "apply") else DenseOperator(A)
+    if fallback == "lu" and not isinstance(op, DenseOperator):
+        raise ValueError("fallback='lu' needs a dense matrix")
     n = op.n
     dev = b.device
     if restart < 1 or restart > 500:
@@ -116,6 +149,10 @@ def gmres(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, restart: int = 
     blk = _Blocks(n, restart, fld, dev)
     b = b.reshape(-1).to(dt64).contiguous()
     x = torch.zeros(n, dtype=dt64, device=dev) if x0 is None else x0.reshape(-1).to(dt64).clone().contiguous()
+    fb = []
+    if x0 is not None and not bool(torch.isfinite(torch.view_as_real(x) if cplx else x).all()):
+        x.zero_()
+        fb.append("x0-nonfinite")
     V = torch.empty((restart + 1, n), dtype=dt32, device=dev)
     w = torch.empty(n, dtype=dt32, device=dev)
     r = torch.empty(n, dtype=dt64, device=dev)
@@ -127,7 +164,7 @@ def gmres(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, restart: int = 
     bnorm = float(np.sqrt(nb[0, 0].item()))
     if bnorm == 0.0:
         x.zero_()
-        return x, GmresInfo(0, 0, True, 0.0)
+        return x, GmresInfo(0, 0, True, 0.0, fallback="+".join(fb))
     it = 0
     restarts = 0
     hist = []
@@ -192,4 +229,7 @@ def gmres(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, restart: int = 
         yb[:k] = torch.from_numpy(np.stack([y.real, y.imag], 1)).to(dev)
         blk.update(V, k, yb, 1.0, x)
         restarts += 1
-    return x, GmresInfo(it, restarts, res <= tol, float(res), hist)
+    if res > tol and fallback == "lu":
+        x, res = _lu_finish(op, b, x, tol)
+        fb.append("lu")
+    return x, GmresInfo(it, restarts, res <= tol, float(res), hist, "+".join(fb))

@@ -90,7 +90,9 @@ class MagneticSolver:
         A, b, Vm, area = self.system(verts, faces)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        sigma, info = gmres(A, b, x0=x0, restart=self.restart, tol=self.tol, maxiter=maxiter)
+        # a non-finite warm start restarts from zero; an unconverged GMRES is
+        # finished by LU (both recorded in info.fallback)
+        sigma, info = gmres(A, b, x0=x0, restart=self.restart, tol=self.tol, maxiter=maxiter, fallback="lu")
         phi = matvec(Vm, sigma) / area if Vm is not None else None
         torch.cuda.synchronize()
         t2 = time.perf_counter()

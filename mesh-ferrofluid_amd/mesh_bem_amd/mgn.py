@@ -20,6 +20,7 @@ uses Adam, all on the GPU.  The prediction seeds GMRES (FP32 matrix).
 """
 
 import ctypes
+import os
 import time
 from dataclasses import dataclass
 from typing import List, Optional
@@ -261,6 +262,45 @@ class MeshGraphNet:
         y, _ = self.forward(g, store=False)
         return y
 
+    # ------------------------------------------------------------- checkpoints
+    def state_dict(self) -> dict:
+        """Parameters, Adam moments and step count (tensors only, so the file
+        loads with torch.load(weights_only=True))."""
+        return {
+            "config": torch.tensor([self.H, self.L, self.nf, self.ef], dtype=torch.int64),
+            "params": self.params.detach().cpu(),
+            "adam_m": self.m.detach().cpu(),
+            "adam_v": self.v.detach().cpu(),
+            "step_count": torch.tensor(self.step_count, dtype=torch.int64),
+        }
+
+    def load_state_dict(self, sd: dict) -> None:
+        cfg = [int(x) for x in sd["config"].tolist()]
+        if cfg != [self.H, self.L, self.nf, self.ef]:
+            raise ValueError(f"checkpoint of a network with (hidden, layers, nf, ef) = {cfg}")
+        for name in ("params", "adam_m", "adam_v"):
+            if sd[name].numel() != self.n_params:
+                raise ValueError(f"checkpoint {name}: {sd[name].numel()} values, network has {self.n_params}")
+        self.params.copy_(sd["params"].to(self.device))
+        self.m.copy_(sd["adam_m"].to(self.device))
+        self.v.copy_(sd["adam_v"].to(self.device))
+        self.step_count = int(sd["step_count"])
+
+    def save(self, path: str, extra: Optional[dict] = None) -> None:
+        """Checkpoint to ``path`` (written to a temporary file, then renamed)."""
+        sd = self.state_dict()
+        for k, v in (extra or {}).items():
+            sd[f"extra_{k}"] = torch.as_tensor(v)
+        tmp = path + ".tmp"
+        torch.save(sd, tmp)
+        os.replace(tmp, path)
+
+    def load(self, path: str) -> dict:
+        """Restore a checkpoint; returns its extra entries."""
+        sd = torch.load(path, map_location="cpu", weights_only=True)
+        self.load_state_dict(sd)
+        return {k[len("extra_"):]: v for k, v in sd.items() if k.startswith("extra_")}
+
 
 # ----------------------------------------------------------------- features
 @dataclass
@@ -460,6 +500,33 @@ def generate(n_meshes: int, freq: int, n_dirs: int, seed0: int, k_range=(1.0, 4.
     return mds, samples
 
 
+def save_samples(path: str, samples: List["Sample"]) -> None:
+    """Generated triples (mesh index, ka, waves, features, FP64-solve currents)
+    as one .npz shard; the systems (Z32 ...) are not stored."""
+    arrays = {}
+    for i, smp in enumerate(samples):
+        arrays[f"s{i}_meta"] = np.array([smp.mesh, smp.k], dtype=np.float64)
+        arrays[f"s{i}_dirs"] = np.asarray(smp.dirs)
+        arrays[f"s{i}_pols"] = np.asarray(smp.pols)
+        arrays[f"s{i}_x"] = smp.x.detach().cpu().numpy()
+        arrays[f"s{i}_J"] = smp.J.detach().cpu().numpy()
+    arrays["count"] = np.array([len(samples)], dtype=np.int64)
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, **arrays)
+    os.replace(tmp, path)
+
+
+def load_samples(path: str, device="cpu") -> List["Sample"]:
+    with np.load(path, allow_pickle=False) as z:
+        n = int(z["count"][0])
+        out = []
+        for i in range(n):
+            mesh, k = z[f"s{i}_meta"]
+            out.append(Sample(int(mesh), float(k), z[f"s{i}_dirs"], z[f"s{i}_pols"],
+                              torch.from_numpy(z[f"s{i}_x"]).to(device), torch.from_numpy(z[f"s{i}_J"]).to(device)))
+    return out
+
+
 def _residual_grad(y, sel, samples, goff, res_weight):
     """Adds res_weight * d/dy of mean_g |b_g - Z_g J_g|^2 / |b_g|^2 to nothing and
     returns (grad [Nn, 2], per-sample residual^2).  J_g = y_re + i y_im of sample g."""
@@ -486,18 +553,30 @@ def _residual_grad(y, sel, samples, goff, res_weight):
 
 
 def train(net: MeshGraphNet, mds, samples, epochs=5, batch=16, lr=1e-3, seed=0, log=None, lr_final=1e-4,
-          res_weight=0.0):
+          res_weight=0.0, checkpoint: Optional[str] = None):
     """Mini-batch training over all (mesh, direction) pairs.  Loss per sample:
     |J_pred - J|^2 / |J|^2 + res_weight |b - Z J_pred|^2 / |b|^2 (the second
     term needs samples generated with keep_system=True).  Returns the history
-    of the mean loss per epoch."""
+    of the mean loss per epoch.
+
+    checkpoint: path of a checkpoint written after every epoch (network,
+    Adam state, epochs done, loss history); when it exists, training resumes
+    after its last epoch.  The batch order of epoch e depends only on (seed,
+    e), so a resumed run repeats the uninterrupted one exactly."""
     pairs = [(i, d) for i, s in enumerate(samples) for d in range(s.x.shape[0])]
-    rng = np.random.Generator(np.random.PCG64(seed))
     hist = []
     n_steps = epochs * ((len(pairs) + batch - 1) // batch)
     step = 0
-    for ep in range(epochs):
-        order = rng.permutation(len(pairs))
+    ep0 = 0
+    if checkpoint and os.path.exists(checkpoint):
+        extra = net.load(checkpoint)
+        ep0 = int(extra["epochs_done"])
+        hist = [float(h) for h in extra["history"].tolist()][:ep0]
+        step = ep0 * ((len(pairs) + batch - 1) // batch)
+        if log:
+            log(f"resumed from {checkpoint} after epoch {ep0}")
+    for ep in range(ep0, epochs):
+        order = np.random.Generator(np.random.PCG64(seed * 1000003 + ep)).permutation(len(pairs))
         tot, nb = 0.0, 0
         for b0 in range(0, len(order), batch):
             sel = [pairs[j] for j in order[b0:b0 + batch]]
@@ -521,6 +600,8 @@ def train(net: MeshGraphNet, mds, samples, epochs=5, batch=16, lr=1e-3, seed=0, 
         hist.append(tot / nb)
         if log:
             log(f"epoch {ep + 1}/{epochs}: mean relative L2^2 {hist[-1]:.4f}")
+        if checkpoint:
+            net.save(checkpoint, {"epochs_done": ep + 1, "history": torch.tensor(hist, dtype=torch.float64)})
     return hist
 
 

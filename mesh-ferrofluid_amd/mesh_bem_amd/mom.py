@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .geometry import get_edge_unique, rwg_face_edges
+from .geometry import check_mesh, check_orientation, get_edge_unique, rwg_face_edges
 from .quadrature import radon7
 
 NEAR_ETA = 2.0
@@ -132,6 +132,8 @@ class MomMesh:
         if n_edges is None:
             n_edges = int(fe_np.max()) + 1
         rfe = rwg_face_edges(fe_np, n_edges)
+        check_mesh(vnp, fnp)
+        check_orientation(fnp, rfe)
         vdev = vt.to(device).contiguous()
         fdev = ft.to(device).contiguous()
         if not near:

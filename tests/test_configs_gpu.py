@@ -1,0 +1,218 @@
+"""Every workload configuration at its real size on the GPU.
+
+C1  icosphere subdivision 3 (1280 faces, 1920 RWG), ka = 1, x-polarised wave
+    along +z: FP64 north-star assembly elementwise against the oracle, the
+    complex128 LU against LAPACK, the mixed-precision solve; the reference
+    rule (Q = 4, the demo default) against the oracle's restatement of
+    solve_MOM (src/mesh_BEM/mesh_BEM_solver_3d.py:96-195).
+C2  icosphere subdivision 5 (20 480 faces, 30 720 RWG), ka = 10: the
+    mixed-precision solve (FP32 LU with the split-FP16 trailing update, FP64
+    refinement) against the complex128 LU of the same matrix; FP32 assembly
+    entries against FP64.
+C3  64 star meshes (750-3630 RWG), 128 waves each, ka ~ U(1, 8): the batched
+    mixed-precision solve against complex128 solves, mesh by mesh.
+C5  the frequency-45 droplet (40 500 faces): the magnetic surface charge of
+    the smooth ellipsoid against the closed form, then three warm-started steps
+    of the noisy, deforming droplet.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from mesh_bem_amd import _lib
+from mesh_bem_amd.dense import LUFactor, refine_solve, set_lu_update, to_c64
+from mesh_bem_amd.geometry import geodesic_sphere, get_edge_unique, icosphere
+from mesh_bem_amd.mom import MomMesh, assemble_ns, rhs_ns
+
+pytestmark = pytest.mark.gpu
+
+DIRS, POLS = [[0.0, 0.0, 1.0]], [[1.0, 0.0, 0.0]]
+
+
+def rel(a, b):
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+# ------------------------------------------------------------------ C1
+@pytest.fixture(scope="module")
+def c1_oracle():
+    V, F = icosphere(3)
+    Z, Vr, edges, fe = O.ns_system(V, F, 1.0, DIRS, POLS)
+    return V, F, Z, Vr, fe
+
+
+def test_c1_fp64_assembly_and_complex128_lu(dev, c1_oracle):
+    V, F, Zo, Vo, fe = c1_oracle
+    print(f"C1: F={len(F)} E={Zo.shape[0]} FP64 assembly + complex128 LU", flush=True)
+    mesh = MomMesh.build(torch.tensor(V, dtype=torch.float64), F, face_edge_indices=fe, n_edges=Zo.shape[0],
+                         device=dev)
+    Z = assemble_ns(mesh, 1.0, _lib.MBA_F64)
+    Vr = rhs_ns(mesh, 1.0, _lib.MBA_F64, DIRS, POLS)
+    Zt, Vt = torch.from_numpy(Zo).to(dev), torch.from_numpy(Vo).to(dev)
+    big = Zt.abs() >= 1e-3 * Zt.abs().max()
+    e_max = float(((Z - Zt).abs()[big] / Zt.abs()[big]).max())
+    e_fro = rel(Z, Zt)
+    print(f"  Z: max entry relerr {e_max:.2e}, Frobenius {e_fro:.2e}; V relerr {rel(Vr, Vt):.2e}")
+    assert e_max < 1e-9 and e_fro < 1e-11 and rel(Vr, Vt) < 1e-12
+    J = LUFactor(Z).solve(Vr)
+    Jo = torch.from_numpy(np.linalg.solve(Zo, Vo)).to(dev)
+    print(f"  complex128 LU vs LAPACK: {rel(J, Jo):.2e}")
+    assert rel(J, Jo) < 1e-9
+
+
+def test_c1_mixed_precision(dev, c1_oracle):
+    V, F, Zo, Vo, fe = c1_oracle
+    mesh = MomMesh.build(torch.tensor(V, dtype=torch.float64), F, face_edge_indices=fe, n_edges=Zo.shape[0],
+                         device=dev)
+    Z = assemble_ns(mesh, 1.0, _lib.MBA_F64)
+    Vr = rhs_ns(mesh, 1.0, _lib.MBA_F64, DIRS, POLS)
+    J, info = refine_solve(Z, Vr, tol=1e-8)
+    Jo = torch.from_numpy(np.linalg.solve(Zo, Vo)).to(dev)
+    print(f"C1 mixed: relerr {rel(J, Jo):.2e}, {info}")
+    assert info.converged and not info.fallback and rel(J, Jo) < 1e-6
+
+
+def test_c1_reference_rule_q4(dev):
+    """solve_MOM in the reference's mode (CFIE, Q = 4 / 3, fixed incidence) at
+    1280 faces: face-edge blocks elementwise and vertex currents vs the oracle."""
+    from mesh_bem_amd.kernels import MeshMOMZmatKernel
+    from mesh_bem_amd.quadrature import get_gaussion_integration_points_and_weights
+    from mesh_bem_amd.solver import MeshBEMSolver3d
+    from mesh_bem_amd.types import BEMType
+
+    V, F = icosphere(3)
+    k = 1.0
+    print(f"C1 reference rule: F={len(F)} Q=4", flush=True)
+    f64 = np.asarray(F, dtype=np.int64)
+    gpx, gwx = O.gl_ref(4)
+    gpy, gwy = O.gl_ref(3)
+    ZF = O.ref_zmat_faceedge(V, f64, gpx, gwx, gpy, gwy, k)
+    v = torch.tensor(V, dtype=torch.float64, device=dev)
+    f = torch.tensor(F, dtype=torch.int32, device=dev)
+    A, n = O.face_areas_normals(V, f64)
+    gx = get_gaussion_integration_points_and_weights(4, dtype=torch.float64)
+    gy = get_gaussion_integration_points_and_weights(3, dtype=torch.float64)
+    zf = MeshMOMZmatKernel.apply(v, f, torch.tensor(A, device=dev), torch.tensor(n, device=dev),
+                                 gx[0].to(dev), gx[1].to(dev), gy[0].to(dev), gy[1].to(dev), k)[0]
+    ZFt = torch.from_numpy(ZF).to(dev)
+    print(f"  face-edge blocks relerr {rel(zf, ZFt):.2e}")
+    assert rel(zf, ZFt) < 1e-11
+    edges, fe, _ = get_edge_unique(v, f)
+    solver = MeshBEMSolver3d(4, 1, int(BEMType.HELMHOLTZ_MOM), k, None, dtype=torch.float64, device=dev)
+    vj = solver.solve_MOM(v, f, edges, fe)
+    fe_np = fe.cpu().numpy().astype(np.int64)
+    E = edges.shape[0]
+    rfe = O.rwg_fe(fe_np, E)
+    Zr = O.gather_rwg(ZF, rfe)
+    Vr = O.gather_rhs(O.ref_rhs_faceedge(V, f64, gpx, gwx, k).reshape(-1, 1), rfe)[:, 0]
+    I = np.linalg.solve(Zr, Vr)
+    vjo = O.vertex_sum(O.interp_faceedge(V, f64, fe_np, I[None, :]), f64, len(V))
+    e = float(np.linalg.norm(vj.cpu().numpy() - vjo) / np.linalg.norm(vjo))
+    print(f"  vertex currents relerr {e:.2e}")
+    assert e < 1e-9
+
+
+# ------------------------------------------------------------------ C2
+def test_c2_mixed_solve_full_size(dev):
+    V, F = icosphere(5)
+    k = 10.0
+    print(f"C2: F={len(F)} ka={k}: FP64 assembly, FP32 LU (split-FP16 update), FP64 refinement", flush=True)
+    mesh = MomMesh.build(torch.tensor(V, dtype=torch.float64), F, device=dev)
+    E = mesh.n_edges
+    assert E == 30720
+    Z64 = assemble_ns(mesh, k, _lib.MBA_F64)
+    Vr = rhs_ns(mesh, k, _lib.MBA_F64, DIRS, POLS)
+    set_lu_update("split_f16")
+    J, info = refine_solve(Z64, Vr, lu32=LUFactor(to_c64(Z64), overwrite=True), tol=1e-8)
+    r = Vr[:, 0] - torch.mv(Z64, J[:, 0])
+    res = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(Vr))
+    print(f"  refinement {info}; ||V - Z64 J|| / ||V|| = {res:.2e}", flush=True)
+    assert info.converged and not info.fallback and res < 1e-12
+    J64 = LUFactor(Z64.clone(), overwrite=True).solve(Vr)
+    print(f"  relerr vs complex128 LU: {rel(J, J64):.2e}", flush=True)
+    assert rel(J, J64) < 1e-6
+    del J64
+    # FP32 assembly against FP64 (row blocks to bound the temporaries):
+    # Frobenius and max |dZ| / max |Z| within 1e-5.  Entry by entry (|Z| >=
+    # 1e-3 max |Z|) the error reaches a few 1e-5: the scalar-potential part of
+    # a far RWG pair is a difference of four face-pair integrals (the charges
+    # of T+ and T- cancel to a dipole-dipole term ~ (h / R)^2 of each), so the
+    # FP32 rounding of each G (eps |G|) grows by (R / h)^2 relative to the
+    # entry; the FP64 assembly (the mixed solve's input) has no such loss.
+    m32 = MomMesh(**{**mesh.__dict__, "verts": mesh.verts.to(torch.float32)})
+    Z32 = assemble_ns(m32, k, _lib.MBA_F32)
+    zmax = float(Z64.abs().max())
+    worst_big, worst_abs, num, den = 0.0, 0.0, 0.0, 0.0
+    for r0 in range(0, E, 2048):
+        a, b = Z32[r0:r0 + 2048].to(torch.complex128), Z64[r0:r0 + 2048]
+        d = (a - b).abs()
+        big = b.abs() >= 1e-3 * zmax
+        worst_big = max(worst_big, float((d[big] / b.abs()[big]).max()))
+        worst_abs = max(worst_abs, float(d.max()) / zmax)
+        num += float((d * d).sum())
+        den += float((b.abs() ** 2).sum())
+    fro = math.sqrt(num / den)
+    print(f"  FP32 assembly: Frobenius {fro:.2e}, max |dZ| / max |Z| {worst_abs:.2e}, "
+          f"entrywise (|Z| >= 1e-3 max) {worst_big:.2e}")
+    assert fro < 1e-5 and worst_abs < 1e-5 and worst_big < 1e-4
+
+
+# ------------------------------------------------------------------ C3
+def test_c3_full_batch(dev):
+    from mesh_bem_amd.batched import BatchedMoM, GemmTuner, star_batch
+
+    geoms, ks, dirs, pols = star_batch(64, seed0=0, n_dirs=128)
+    bm = BatchedMoM.build(geoms, ks, device=dev)
+    Es = [m.n_edges for m in bm.meshes]
+    print(f"C3: 64 meshes, E {min(Es)}-{max(Es)}, 128 waves", flush=True)
+    assert min(Es) >= 500 and max(Es) <= 4000
+    Zs = bm.assemble(_lib.MBA_F64)
+    Vs = bm.rhs(dirs, pols)
+    res = bm.solve(Zs, Vs, tuner=GemmTuner(), tol=1e-8)
+    assert all(res.converged)
+    worst = 0.0
+    for Z, V, J in zip(Zs, Vs, res.J):
+        worst = max(worst, rel(J, torch.linalg.solve(Z, V)))
+    print(f"  worst relerr vs complex128 solve over 64 meshes: {worst:.2e}")
+    assert worst < 1e-9
+
+
+# ------------------------------------------------------------------ C5
+def test_c5_ellipsoid_closed_form_and_warm_steps(dev):
+    from mesh_bem_amd.magnetic import MagneticSolver, droplet_steps
+
+    chi = 3.0
+    # the configuration's axis ratio (seed 0), without the radial noise
+    _, _, _, c = next(droplet_steps(0, 3, 1))
+    V, F = geodesic_sphere(45)
+    V = V * np.array([1.0, 1.0, c])
+    print(f"C5: smooth ellipsoid 1:1:{c:.3f}, F={len(F)}, chi={chi}", flush=True)
+    assert len(F) == 40500
+    ms = MagneticSolver(chi, tol=1e-8, with_potential=False)
+    out = ms.solve(torch.tensor(V, device=dev), torch.tensor(F, dtype=torch.int32, device=dev))
+    _, _, Nc = O.demag_factors(1.0, 1.0, c)
+    _, n = O.face_areas_normals(V, F)
+    ref = chi / (1 + Nc * chi) * n[:, 2]
+    e = float(np.linalg.norm(out.sigma.cpu().numpy() - ref) / np.linalg.norm(ref))
+    print(f"  relerr vs closed form {e:.2e}, GMRES iterations {out.info.iterations}")
+    assert out.info.converged and e < 5e-3
+    # three steps of the deforming noisy droplet, warm-started from the previous charge
+    ms = MagneticSolver(chi, tol=1e-6, restart=50)
+    prev, its = None, []
+    for s, Vs, Fs, _ in droplet_steps(0, 45, 3):
+        v = torch.tensor(Vs, device=dev)
+        f = torch.tensor(Fs, dtype=torch.int32, device=dev)
+        step = ms.solve(v, f, x0=prev)
+        assert step.info.converged and step.info.residual <= 1e-6
+        if prev is not None:
+            cold = ms.solve(v, f)
+            d = float(torch.linalg.vector_norm(step.sigma - cold.sigma) / torch.linalg.vector_norm(cold.sigma))
+            its.append((cold.info.iterations, step.info.iterations))
+            assert d < 1e-4
+        prev = step.sigma
+    print(f"  (cold, warm) GMRES iterations per step: {its}")
+    assert all(w < c_ for c_, w in its)

@@ -142,3 +142,46 @@ def test_gmres_warm_start_exact(dev):
     x, info = gmres(At, bt, tol=1e-10)
     res = np.linalg.norm(A.astype(np.float32).astype(np.float64) @ x.cpu().numpy() - bt.double().cpu().numpy())
     assert info.converged and res / np.linalg.norm(b) < 2e-10
+
+
+def test_nonfinite_warm_start_restarts_from_zero(dev):
+    rng = np.random.default_rng(5)
+    n = 400
+    A = np.eye(n) + 0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2 * n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    At = torch.tensor(A, dtype=torch.complex64, device=dev)
+    bt = torch.tensor(b, dtype=torch.complex64, device=dev)
+    x0 = torch.zeros(n, dtype=torch.complex128, device=dev)
+    x0[7] = complex(float("nan"), 0.0)
+    x, info = gmres(At, bt, x0=x0, restart=30, tol=1e-8)
+    xz, infoz = gmres(At, bt, restart=30, tol=1e-8)
+    assert info.fallback == "x0-nonfinite" and info.converged and not infoz.fallback
+    assert torch.equal(x, xz)
+    x0 = torch.full((n,), float("inf"), dtype=torch.complex128, device=dev)
+    x, info = gmres(At, bt, x0=x0, restart=30, tol=1e-8)
+    assert info.fallback == "x0-nonfinite" and torch.equal(x, xz)
+
+
+@pytest.mark.parametrize("cplx", [True, False])
+def test_unconverged_gmres_falls_back_to_lu(dev, cplx):
+    """An iteration cap far below what the system needs leaves GMRES
+    unconverged; fallback="lu" finishes with a complex64 LU and FP64-residual
+    refinement to the same tolerance."""
+    rng = np.random.default_rng(6)
+    n = 500
+    # spread spectrum: slow for GMRES with a small restart
+    U, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = U @ np.diag(np.linspace(0.01, 1.0, n)) @ U.T + 0.05 * rng.standard_normal((n, n)) / np.sqrt(n)
+    if cplx:
+        A = A + 0.05j * rng.standard_normal((n, n)) / np.sqrt(n)
+    b = rng.standard_normal(n)
+    dt = torch.complex64 if cplx else torch.float32
+    At = torch.tensor(A, dtype=dt, device=dev)
+    bt = torch.tensor(b, dtype=dt, device=dev)
+    x, info = gmres(At, bt, restart=5, tol=1e-9, maxiter=20)
+    assert not info.converged and not info.fallback
+    x, info = gmres(At, bt, restart=5, tol=1e-9, maxiter=20, fallback="lu")
+    Af = At.cpu().numpy().astype(np.complex128)
+    res = np.linalg.norm(bt.cpu().numpy() - Af @ x.cpu().numpy()) / np.linalg.norm(b)
+    print(f"  LU fallback: info {info.fallback} converged {info.converged} residual {res:.2e}")
+    assert info.fallback == "lu" and info.converged and res < 1e-9

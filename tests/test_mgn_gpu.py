@@ -183,3 +183,28 @@ def test_training_and_warm_start(dev):
     print(f"  held-out J relative error {np.mean(jerr):.3f}; GMRES iterations cold {np.mean(cold):.1f} "
           f"warm {np.mean(warm):.1f}")
     assert max(jerr) < 1.0
+
+
+def test_training_resumes_exactly_from_checkpoint(dev, tmp_path):
+    """A run interrupted after its first epoch and resumed from the checkpoint
+    ends with the parameters and Adam state of the uninterrupted run."""
+    mds, smp = M.generate(3, 3, 4, seed0=40, k_range=(1.0, 2.0), device=dev)
+    ref = M.MeshGraphNet(hidden=32, layers=2, device=dev, seed=1)
+    h_ref = M.train(ref, mds, smp, epochs=2, batch=4, lr=2e-3, seed=5)
+
+    class Interrupt(Exception):
+        pass
+
+    def stop_in_epoch_2(msg):
+        if msg.startswith("epoch 2/"):
+            raise Interrupt
+
+    ck = str(tmp_path / "net.pt")
+    net = M.MeshGraphNet(hidden=32, layers=2, device=dev, seed=1)
+    with pytest.raises(Interrupt):
+        M.train(net, mds, smp, epochs=2, batch=4, lr=2e-3, seed=5, checkpoint=ck, log=stop_in_epoch_2)
+    fresh = M.MeshGraphNet(hidden=32, layers=2, device=dev, seed=77)
+    h = M.train(fresh, mds, smp, epochs=2, batch=4, lr=2e-3, seed=5, checkpoint=ck)
+    assert h == h_ref
+    assert torch.equal(fresh.params, ref.params) and torch.equal(fresh.m, ref.m) and torch.equal(fresh.v, ref.v)
+    assert fresh.step_count == ref.step_count
