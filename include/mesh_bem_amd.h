@@ -413,6 +413,30 @@ int mba_laplace_pack(int prec, const void *unpacked, int64_t n_faces, int32_t np
 int mba_laplace_dirichlet_rhs(int prec, const void *k_tight, int64_t n_verts, const void *g,
                               const void *vert_areas, void *rhs, mba_stream_t stream);
 
+/* North-star P1 Galerkin single and double layer (no reference counterpart:
+ * the reference's layers are double_integration_on_panels_3d,
+ * src/mesh_BEM/utils/integrations_3d.py:252-452, whose collapsed rule is
+ * mba_laplace_layer_ref).  Unpacked layout of mba_laplace_layer_ref with
+ * nphi = 3: v_unpacked / k_unpacked [F, F, 3, 3], entry (fy, fx, iy, ix) =
+ * int_Tx int_Ty phi_ix(x) phi_iy(y) k(x, y), k = 1/(4 pi R) (single) or
+ * (x - y).n_y / (4 pi R^3) (double, n_y the normal of face fy).  Far pairs:
+ * rule [nq, 4] (barycentric l0 l1 l2, weight; nq 3 or 7) on both triangles;
+ * pairs of the near CSR (a symmetric relation, mba_near_pairs_*): the rule on
+ * T_x and the inner integral over T_y in closed form, in FP64.  Either output
+ * may be null.  workspace: mba_laplace_p0_workspace_bytes(prec, n_faces). */
+int mba_laplace_p1_assemble(int prec, const void *verts, const int32_t *faces, int64_t n_verts, int64_t n_faces,
+                            const double *rule, int32_t nq, const int64_t *near_ptr, const int32_t *near_idx,
+                            void *v_unpacked, void *k_unpacked, void *workspace, size_t workspace_bytes,
+                            mba_stream_t stream);
+
+/* out = alpha M g + beta out with the P1 mass matrix M_ij = int phi_i phi_j
+ * (the consistent counterpart of the lumped 1/2 g vert_area of
+ * RhsLayer3d.compute_rhs, src/mesh_BEM/rhs_layer/rhs_layer_3d.py:127).
+ * vinc_ptr / vinc_idx: the vertex -> face-corner CSR of mba_laplace_pack. */
+int mba_laplace_mass_apply(int prec, const void *verts, const int32_t *faces, const int64_t *vinc_ptr,
+                           const int32_t *vinc_idx, int64_t n_verts, const void *g, double alpha, double beta,
+                           void *out, mba_stream_t stream);
+
 /* P0 Galerkin Laplace operators, north-star rule (no reference counterpart):
  *   V[i, j]  = int_Ti int_Tj 1/(4 pi R)
  *   Kt[i, j] = int_Ti int_Tj n_i . grad_x 1/(4 pi R)   (adjoint double layer)

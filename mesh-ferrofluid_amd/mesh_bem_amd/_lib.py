@@ -170,6 +170,9 @@ SIGNATURES = {
     "mba_nearest_transfer_workspace_bytes": (_sz, [_c.c_int, _i64, _i64]),
     "mba_nearest_transfer": (_c.c_int, [_c.c_int, _vp, _i64, _vp, _i64, _vp, _c.c_int32, _vp, _vp, _vp, _sz, _vp]),
     "mba_laplace_p0_workspace_bytes": (_sz, [_c.c_int, _i64]),
+    "mba_laplace_p1_assemble": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _i64p, _i32p, _vp, _vp,
+                                           _vp, _sz, _vp]),
+    "mba_laplace_mass_apply": (_c.c_int, [_c.c_int, _vp, _i32p, _i64p, _i32p, _i64, _vp, _dbl, _dbl, _vp, _vp]),
     "mba_laplace_p0_assemble": (_c.c_int, [_c.c_int, _vp, _i32p, _i64, _i64, _dp, _i32, _i64p, _i32p,
                                            _vp, _vp, _i64, _vp, _sz, _vp]),
 }

@@ -104,7 +104,9 @@ class MeshPressure:
 
     def __init__(self, density_fluid: float, gravity_strength: float, surface_tension, contact_angle,
                  gaussQR: int, order_type: int, logger=None, device=torch.device("cpu"),
-                 dtype: torch.dtype = torch.float32) -> None:
+                 dtype: torch.dtype = torch.float32, *, bem_mode: int = None) -> None:
+        """bem_mode (keyword-only, new): AssemblyMode of the Laplace solve;
+        default the reference's rule (REF), NS for the north-star P1 layers."""
         from .logger import Logger
         from .solver import MeshBEMSolver3d
         from .types import BEMType
@@ -119,8 +121,12 @@ class MeshPressure:
         self.gravity = torch.tensor([0.0, -gravity_strength, 0.0], dtype=dtype, device=device)
         self.surface_tension = float(surface_tension)
         self.contact_angle = contact_angle
+        from .types import AssemblyMode
+
+        mode = int(AssemblyMode.REF) if bem_mode is None else int(bem_mode)
         self.BEM_solver = MeshBEMSolver3d(gaussQR=gaussQR, order_type=order_type, BEM_type=int(BEMType.LAPLACE),
-                                          wavenumber=1.0, logger=self.logger, device=device, dtype=dtype)
+                                          wavenumber=1.0, logger=self.logger, device=device, dtype=dtype,
+                                          mode=mode)
         self.last = {}
 
     def boundary_data(self, verts: torch.Tensor, faces: torch.Tensor, dt: float, vel: torch.Tensor,

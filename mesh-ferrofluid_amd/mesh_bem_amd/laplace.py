@@ -15,6 +15,14 @@ src/mesh_BEM/mesh_BEM_solver_3d.py:197-373):
   layer on the GPU; ``pack`` forms the [V, V] matrix, ``dirichlet_rhs`` the
   right-hand side 1/2 M g + K^T g.
 
+North-star P1 rule (``MeshBEMSolver3d(..., mode=AssemblyMode.NS).solve``):
+``p1_operators`` returns the unpacked P1 Galerkin single and double layer
+(7-point rule, FP64 closed-form inner integrals on near pairs) in the layout
+of ``layer_ref``; the Dirichlet system uses the consistent P1 mass matrix
+(``mass_apply``) where the reference lumps it (rhs_layer_3d.py:127): with
+the lumped mass the exact layers miss the demo's known answer by tens of
+per cent, with the consistent one they converge to it.
+
 North-star rule for the magnetic scalar potential: ``p0_operators`` returns
 the P0 Galerkin single layer V and adjoint double layer K' (7-point rule, FP64
 analytic inner integrals on near pairs); ``magnetic_charge`` solves
@@ -185,16 +193,25 @@ def laplace_solve(solver, verts: torch.Tensor, faces: torch.Tensor, vert_bc_Diri
     raise NotImplementedError here."""
     if solver.BEM_type != int(BEMType.LAPLACE):
         raise NotImplementedError("solve supports BEM_type == LAPLACE")
+    from .types import AssemblyMode
+
     dt = verts.dtype
     V = verts.shape[-2]
     B = vert_bc_flags.shape[0]
     faces = faces.to(torch.int32)
-    gp, gw = get_gaussion_integration_points_and_weights(solver.gaussQR, dtype=torch.float64)
-    tables = relation_tables(gp.numpy(), gw.numpy())
-    Vu = layer_ref(verts, faces, solver.gaussQR, 1, solver.order_type, tables)
-    Ku = layer_ref(verts, faces, solver.gaussQR, 2, solver.order_type, tables)
+    ns = getattr(solver, "mode", int(AssemblyMode.REF)) == int(AssemblyMode.NS)
+    if ns:
+        if solver.order_type != int(OrderType.LINEAR):
+            raise NotImplementedError("the north-star Laplace rule is P1 (order_type LINEAR)")
+        Vu, Ku = p1_operators(verts, faces, _lib.prec_of(dt))
+    else:
+        gp, gw = get_gaussion_integration_points_and_weights(solver.gaussQR, dtype=torch.float64)
+        tables = relation_tables(gp.numpy(), gw.numpy())
+        Vu = layer_ref(verts, faces, solver.gaussQR, 1, solver.order_type, tables)
+        Ku = layer_ref(verts, faces, solver.gaussQR, 2, solver.order_type, tables)
     Vt = pack(Vu, faces, V)
     Kt = pack(Ku, faces, V)
+    del Vu, Ku
     if Amat_scaler is not None:
         Vt = Amat_scaler * Vt
     va = compute_vert_areas(verts, faces, keepdim=False)
@@ -209,7 +226,11 @@ def laplace_solve(solver, verts: torch.Tensor, faces: torch.Tensor, vert_bc_Diri
         if nD != V:
             raise ValueError(f"batch {b}: every vertex flag must be DIRICHLET or NEUMANN")
         g = vert_bc_Dirichlet[b].reshape(-1).to(dt)
-        rhs = dirichlet_rhs(Kt, g, va)
+        if ns:
+            rhs = dirichlet_rhs(Kt, g, torch.zeros_like(va))  # K g
+            rhs = mass_apply(verts, faces, g, 0.5, 1.0, rhs)  # + 1/2 M g
+        else:
+            rhs = dirichlet_rhs(Kt, g, va)
         t = _real_solve(Vt, rhs)
         neu.append(t.reshape(V, 1))
         dir_.append(vert_bc_Dirichlet[b].reshape(V, 1))
@@ -244,3 +265,49 @@ def p0_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, want_v: bo
                                      _lib.stream())
     _lib.check(st, "mba_laplace_p0_assemble")
     return Vm, Km
+
+
+# ---------------------------------------------------------------- P1 (north-star rule)
+def p1_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, near=None, want_v: bool = True,
+                 want_k: bool = True):
+    """Unpacked P1 Galerkin single and double layer [F, F, 3, 3] (layout of
+    ``layer_ref``; either may be skipped)."""
+    _check_vf(verts, faces)
+    F = faces.shape[0]
+    lib = _lib.load()
+    dev = verts.device
+    rdt = _rdt(prec)
+    vt = verts.to(rdt).contiguous()
+    ft = faces.to(torch.int32).contiguous()
+    if near is None:
+        near = near_pairs_gpu(vt, ft)
+    rule = rule_tensor(dev)
+    Vu = torch.empty((F, F, 3, 3), dtype=rdt, device=dev) if want_v else None
+    Ku = torch.empty((F, F, 3, 3), dtype=rdt, device=dev) if want_k else None
+    wsb = lib.mba_laplace_p0_workspace_bytes(prec, F)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    st = lib.mba_laplace_p1_assemble(prec, _lib.ptr(vt), _lib.ptr(ft), verts.shape[0], F, _lib.ptr(rule),
+                                     rule.shape[0], _lib.ptr(near[0]), _lib.ptr(near[1]), _lib.ptr(Vu),
+                                     _lib.ptr(Ku), _lib.ptr(ws), wsb, _lib.stream())
+    _lib.check(st, "mba_laplace_p1_assemble")
+    return Vu, Ku
+
+
+def mass_apply(verts: torch.Tensor, faces: torch.Tensor, g: torch.Tensor, alpha: float, beta: float,
+               out: torch.Tensor) -> torch.Tensor:
+    """out = alpha M g + beta out with the P1 mass matrix M (in place; returns out)."""
+    _check_vf(verts, faces)
+    V = verts.shape[0]
+    if g.numel() != V or out.numel() != V or not out.is_contiguous() or out.dtype != verts.dtype:
+        raise ValueError(f"g and out must hold V = {V} values of the vertices' dtype")
+    lib = _lib.load()
+    vptr, vidx = vertex_incidence(faces.detach().cpu().numpy(), V)
+    dev = verts.device
+    st = lib.mba_laplace_mass_apply(_lib.prec_of(verts.dtype), _lib.ptr(verts.contiguous()),
+                                    _lib.ptr(faces.to(torch.int32).contiguous()),
+                                    _lib.ptr(torch.from_numpy(vptr).to(dev)),
+                                    _lib.ptr(torch.from_numpy(vidx).to(dev)), V,
+                                    _lib.ptr(g.reshape(-1).to(verts.dtype).contiguous()), float(alpha),
+                                    float(beta), _lib.ptr(out), _lib.stream())
+    _lib.check(st, "mba_laplace_mass_apply")
+    return out

@@ -112,3 +112,106 @@ def droplet_steps(seed: int, freq: int, steps: int, noise: float = 0.02, step_no
         rng = np.random.Generator(np.random.PCG64(seed * 100003 + s))
         V = V + nrm * rng.normal(0.0, step_noise, size=(V.shape[0], 1))
         yield s, V.copy(), F0, c
+
+
+# ---------------------------------------------------------------- magnetic pressure
+def magnetic_pressure(verts: torch.Tensor, faces: torch.Tensor, sigma: torch.Tensor, phi: torch.Tensor,
+                      chi: float, H0: Sequence[float] = (0.0, 0.0, 1.0), mu0: float = 1.0) -> torch.Tensor:
+    """Magnetic normal traction at the vertices, the ``P_mag`` of the
+    reference's pressure step (src/mesh_ferrofluid/mesh_pressure.py:108-110,
+    subtracted from the Dirichlet pressure there).
+
+    For a linear fluid (M = chi H) the traction on the interface is
+        P_mag = mu0 / 2 [ (M . n)^2 + chi |H_in|^2 ]
+    (the jump of the Maxwell stress plus the magnetic fluid pressure
+    mu0 int_0^H M dH of a linear material), with H_in the interior field:
+    normal part H_in . n = sigma / chi (sigma = M . n the surface charge),
+    tangential part (I - n n^T)(H0 - grad phi) (phi = S sigma, continuous
+    across the surface).  sigma and phi are P0 face values (MagneticSolver);
+    phi is averaged to the vertices (area weights), differentiated on each
+    face, and the face values of P_mag are averaged back to the vertices.
+    For an ellipsoid, H_in is uniform, H0 / (1 + N_z chi), which pins the
+    result: P_mag = mu0 / 2 chi |H_in|^2 (chi n_z^2 + 1)."""
+    v = verts.to(torch.float64)
+    f = faces.to(torch.int64)
+    V, F = v.shape[0], f.shape[0]
+    P = v[f]  # [F, 3, 3]
+    cr = torch.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 1], dim=-1)
+    a2 = cr.norm(dim=-1)
+    n = cr / a2[:, None]
+    area = 0.5 * a2
+    sig = sigma.to(torch.float64).reshape(F)
+    ph = phi.to(torch.float64).reshape(F)
+    # vertex potential: area-weighted mean of the incident faces
+    wsum = torch.zeros(V, dtype=torch.float64, device=v.device).index_add_(0, f.reshape(-1), area.repeat_interleave(3))
+    phv = torch.zeros(V, dtype=torch.float64, device=v.device).index_add_(
+        0, f.reshape(-1), (area * ph).repeat_interleave(3)) / wsum
+    # gradient of the linear interpolant on each face: sum_i phi_i n x (P_i+2 - P_i+1) / (2A)
+    grad = torch.zeros((F, 3), dtype=torch.float64, device=v.device)
+    for i in range(3):
+        e = P[:, (i + 2) % 3] - P[:, (i + 1) % 3]
+        grad += phv[f[:, i]][:, None] * torch.cross(n, e, dim=-1) / a2[:, None]
+    h0 = torch.as_tensor(np.asarray(H0, dtype=np.float64), device=v.device)
+    Ht = h0[None, :] - grad
+    Ht = Ht - (Ht * n).sum(-1, keepdim=True) * n
+    Hn = sig / chi
+    pf = 0.5 * mu0 * (sig * sig + chi * ((Ht * Ht).sum(-1) + Hn * Hn))
+    pv = torch.zeros(V, dtype=torch.float64, device=v.device).index_add_(0, f.reshape(-1),
+                                                                          (area * pf).repeat_interleave(3))
+    return (pv / wsum).to(verts.dtype)
+
+
+class MeshAdvection:
+    """Drop-in for src/mesh_ferrofluid/mesh_advection.py:4-22: verts + vel * dt."""
+
+    def solve(self, verts: torch.Tensor, dt: float, vel: torch.Tensor) -> torch.Tensor:
+        return verts + vel * dt
+
+
+class FerrofluidDroplet:
+    """One time step of the surface-only ferrofluid droplet loop
+    (demo/demo_3d_mesh_droplet.py:134-164) with the magnetic pressure the
+    reference leaves out (it passes P_mag=None): advection, optional remesh,
+    Helmholtz decomposition, the magnetic surface charge (P0 Laplace BEM,
+    GMRES warm-started from the previous step while the mesh keeps its
+    faces), P_mag, and the pressure solve with P_mag."""
+
+    def __init__(self, chi: float, density_fluid: float = 1.0, gravity_strength: float = 0.0,
+                 surface_tension: float = 0.0, gaussQR: int = 3, order_type: int = 1,
+                 H0: Sequence[float] = (0.0, 0.0, 1.0), mu0: float = 1.0, target_edge_length: Optional[float] = None,
+                 device=torch.device("cuda"), dtype: torch.dtype = torch.float64, bem_mode: Optional[int] = None,
+                 tol: float = 1e-6):
+        from .ferrofluid import MeshHelmholtzDecomposition, MeshPressure
+
+        self.chi = float(chi)
+        self.H0 = tuple(float(x) for x in H0)
+        self.mu0 = float(mu0)
+        self.target_edge_length = target_edge_length
+        self.advection = MeshAdvection()
+        self.decomposition = MeshHelmholtzDecomposition(gaussQR, device=device, dtype=dtype)
+        self.pressure = MeshPressure(density_fluid, gravity_strength, surface_tension, 0.5 * np.pi, gaussQR,
+                                     order_type, device=device, dtype=dtype, bem_mode=bem_mode)
+        self.magnetic = MagneticSolver(chi, H0=self.H0, tol=tol, with_potential=True)
+        self._sigma = None
+
+    def step(self, verts: torch.Tensor, faces: torch.Tensor, vel: torch.Tensor, dt: float):
+        """verts [V, 3], faces [F, 3], vel [B, V, 3] -> (verts, faces, vel, record)."""
+        from .remesh import remesh_and_transfer_velocity
+        from .types import BoundaryType
+
+        verts = self.advection.solve(verts, dt, vel[0])
+        if self.target_edge_length is not None:
+            verts, faces, vel = remesh_and_transfer_velocity(verts, faces, vel, self.target_edge_length)
+            self._sigma = None
+        flags = torch.full((vel.shape[0], verts.shape[0], 1), int(BoundaryType.DIRICHLET), dtype=torch.uint8,
+                           device=verts.device)
+        vel = self.decomposition.solve(verts, faces, vel)
+        x0 = self._sigma if self._sigma is not None and self._sigma.numel() == faces.shape[0] else None
+        mag = self.magnetic.solve(verts, faces, x0=x0)
+        self._sigma = mag.sigma
+        p_mag = magnetic_pressure(verts, faces, mag.sigma, mag.phi, self.chi, self.H0, self.mu0)
+        vel = self.pressure.solve(verts, faces, dt, vel, flags, P_mag=p_mag.reshape(1, -1, 1))
+        rec = {"gmres_iterations": mag.info.iterations, "gmres_converged": mag.info.converged,
+               "gmres_fallback": mag.info.fallback, "warm_start": x0 is not None,
+               "p_mag_max": float(p_mag.max()), "p_mag_min": float(p_mag.min())}
+        return verts, faces, vel, rec

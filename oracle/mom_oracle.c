@@ -690,3 +690,96 @@ void orc_laplace_p0(const double *verts, const int32_t *faces, int64_t F, const 
         }
     }
 }
+
+/* ------------------------------------------------------------ P1 Laplace (ns)
+ * North-star P1 Galerkin layers, unpacked like orc_laplace_layer:
+ *   Vu[fy, fx, iy, ix] = int_Tx int_Ty phi_ix(x) phi_iy(y) / (4 pi R)
+ *   Ku[fy, fx, iy, ix] = int_Tx int_Ty phi_ix(x) phi_iy(y) (x - y).n_y / (4 pi R^3)
+ * Far pairs: the (lam, w) rule on both triangles.  Near pairs (fx in the
+ * near list of fy): the rule on T_x, the inner integral over T_y in closed
+ * form: int phi/R = phi(rho) S0 + grad phi . S1, and
+ * int phi (x - y).n_y / R^3 = -phi(rho) (grad S0 . n_y) + h grad phi . (grad S0)_tangential,
+ * h = (x - Y0).n_y, rho the projection of x on the plane of T_y. */
+void orc_laplace_p1_ns(const double *verts, const int32_t *faces, int64_t F, const double *lam,
+                       const double *w, int nq, const int64_t *near_ptr, const int32_t *near_idx,
+                       double *Vu, double *Ku)
+{
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t fy = 0; fy < F; ++fy) {
+        const double *Y[3] = {verts + 3 * faces[3 * fy], verts + 3 * faces[3 * fy + 1],
+                              verts + 3 * faces[3 * fy + 2]};
+        double e01[3], e12[3], ny[3];
+        sub3(Y[1], Y[0], e01);
+        sub3(Y[2], Y[1], e12);
+        cross3(e01, e12, ny);
+        const double Ay = 0.5 * norm3(ny);
+        for (int a = 0; a < 3; ++a) ny[a] /= 2.0 * Ay;
+        double gphi[3][3];
+        for (int i = 0; i < 3; ++i) {
+            double e[3];
+            sub3(Y[(i + 2) % 3], Y[(i + 1) % 3], e);
+            cross3(ny, e, gphi[i]);
+            for (int a = 0; a < 3; ++a) gphi[i][a] /= 2.0 * Ay;
+        }
+        int64_t np = near_ptr[fy];
+        const int64_t npe = near_ptr[fy + 1];
+        for (int64_t fx = 0; fx < F; ++fx) {
+            int near = 0;
+            while (np < npe && near_idx[np] < fx) ++np;
+            if (np < npe && near_idx[np] == fx) near = 1;
+            const double *X[3] = {verts + 3 * faces[3 * fx], verts + 3 * faces[3 * fx + 1],
+                                  verts + 3 * faces[3 * fx + 2]};
+            double f01[3], f12[3], c[3];
+            sub3(X[1], X[0], f01);
+            sub3(X[2], X[1], f12);
+            cross3(f01, f12, c);
+            const double Ax = 0.5 * norm3(c);
+            double v[9] = {0}, k[9] = {0};
+            for (int p = 0; p < nq; ++p) {
+                double x[3];
+                for (int a = 0; a < 3; ++a)
+                    x[a] = lam[3 * p] * X[0][a] + lam[3 * p + 1] * X[1][a] + lam[3 * p + 2] * X[2][a];
+                const double wx = w[p] * 2.0 * Ax / (4.0 * PI);
+                double iv[3] = {0, 0, 0}, ik[3] = {0, 0, 0};
+                if (near) {
+                    double S0, S1[3], rho[3], S0b, G[3];
+                    orc_tri_potentials(Y[0], Y[1], Y[2], x, &S0, S1, rho);
+                    orc_tri_S0_grad(Y[0], Y[1], Y[2], x, &S0b, G);
+                    double xr[3];
+                    sub3(x, Y[0], xr);
+                    const double h = dot3(xr, ny), Gn = dot3(G, ny);
+                    double Gt[3] = {G[0] - Gn * ny[0], G[1] - Gn * ny[1], G[2] - Gn * ny[2]};
+                    for (int i = 0; i < 3; ++i) {
+                        double r1[3];
+                        sub3(rho, Y[(i + 1) % 3], r1);
+                        const double phr = dot3(gphi[i], r1);
+                        iv[i] = phr * S0 + dot3(gphi[i], S1);
+                        ik[i] = -phr * Gn + h * dot3(gphi[i], Gt);
+                    }
+                } else {
+                    for (int q = 0; q < nq; ++q) {
+                        double y[3], d[3];
+                        for (int a = 0; a < 3; ++a)
+                            y[a] = lam[3 * q] * Y[0][a] + lam[3 * q + 1] * Y[1][a] + lam[3 * q + 2] * Y[2][a];
+                        sub3(x, y, d);
+                        const double R = norm3(d), wy = w[q] * 2.0 * Ay;
+                        const double gv = wy / R, gk = wy * dot3(d, ny) / (R * R * R);
+                        for (int i = 0; i < 3; ++i) {
+                            iv[i] += gv * lam[3 * q + i];
+                            ik[i] += gk * lam[3 * q + i];
+                        }
+                    }
+                }
+                for (int iy = 0; iy < 3; ++iy)
+                    for (int ix = 0; ix < 3; ++ix) {
+                        v[iy * 3 + ix] += wx * lam[3 * p + ix] * iv[iy];
+                        k[iy * 3 + ix] += wx * lam[3 * p + ix] * ik[iy];
+                    }
+            }
+            for (int q = 0; q < 9; ++q) {
+                if (Vu) Vu[(fy * F + fx) * 9 + q] = v[q];
+                if (Ku) Ku[(fy * F + fx) * 9 + q] = k[q];
+            }
+        }
+    }
+}

@@ -63,6 +63,7 @@ def lib():
         L.orc_gather_rwg.argtypes = [D, i64, I32, i64, D]
         L.orc_gather_rhs.argtypes = [D, I32, i64, i64, D]
         L.orc_laplace_layer.argtypes = [D, I32, D, D, i64, U8, ctypes.POINTER(D), I64, i32, i32, D]
+        L.orc_laplace_p1_ns.argtypes = [D, I32, i64, D, D, i32, I64, I32, D, D]
         L.orc_laplace_layer_ex.argtypes = [D, I32, D, D, i64, U8, ctypes.POINTER(D), I64, i32, i32, i32, D]
         L.orc_tri_S0_grad.argtypes = [D, D, D, D, D, D]
         L.orc_laplace_p0.argtypes = [D, I32, i64, D, D, i32, I64, I32, D, D]
@@ -70,6 +71,7 @@ def lib():
                   L.orc_ref_rhs_faceedge_terms, L.orc_interp_faceedge,
                   L.orc_tri_potentials, L.orc_ns_zmat_faceedge, L.orc_ns_rhs_faceedge,
                   L.orc_gather_rwg, L.orc_gather_rhs, L.orc_laplace_layer, L.orc_laplace_layer_ex,
+                  L.orc_laplace_p1_ns,
                   L.orc_tri_S0_grad,
                   L.orc_laplace_p0):
             f.restype = None
@@ -518,6 +520,40 @@ def laplace_p0(verts, faces, near_ptr, near_idx, rule=None):
                          _i64(np.ascontiguousarray(near_ptr, dtype=np.int64)), _i32(_fi32(near_idx)),
                          _d(Vm), _d(Km))
     return Vm, Km
+
+
+def laplace_p1_ns(verts, faces, near_ptr, near_idx, rule=None):
+    """North-star P1 layers (Vu, Ku) [F, F, 3, 3] in the layout of laplace_layer:
+    7-point rule on far pairs, outer 7-point rule x analytic inner integral on
+    near pairs."""
+    v, f = _f64(verts), _fi32(faces)
+    lam, w = radon7() if rule is None else rule
+    F = len(f)
+    Vu = np.zeros((F, F, 3, 3))
+    Ku = np.zeros((F, F, 3, 3))
+    lib().orc_laplace_p1_ns(_d(v), _i32(f), F, _d(_f64(lam)), _d(_f64(w)), len(w),
+                            _i64(np.ascontiguousarray(near_ptr, dtype=np.int64)), _i32(_fi32(near_idx)),
+                            _d(Vu), _d(Ku))
+    return Vu, Ku
+
+
+def dirichlet_solve_from_layers(verts, faces, g, Vu, Ku, mass="consistent"):
+    """V t = 1/2 M g + K g from unpacked P1 layers (the Dirichlet branch of
+    mesh_BEM_solver_3d.py:197-373 with the P1 mass matrix, or the reference's
+    lumped 1/2 g vert_area).  Returns (t, V packed, rhs)."""
+    v, f = _f64(verts), np.asarray(faces, dtype=np.int64)
+    nV = len(v)
+    Vt = pack_matrix_tight(1, Vu, f, nV)
+    src = np.einsum("yxij,yi->xj", Ku, np.asarray(g)[f])
+    gk = np.zeros(nV)
+    np.add.at(gk, f.reshape(-1), src.reshape(-1))
+    if mass == "lumped":
+        rhs = 0.5 * np.asarray(g) * vert_areas(v, f) + gk
+    else:
+        rhs = 0.5 * consistent_mass(v, f) @ np.asarray(g) + gk
+    import scipy.linalg
+
+    return scipy.linalg.solve(Vt, rhs), Vt, rhs
 
 
 def demag_factors(a, b, c):

@@ -10,6 +10,7 @@ the surface).  Also: the product's tensor rule equals the oracle's.
 """
 
 import numpy as np
+import torch
 
 import oracle as O
 from mesh_bem_amd.ferrofluid import tensor_rule
@@ -54,3 +55,33 @@ def test_tensor_rule_matches_oracle():
         assert np.array_equal(R[:, 0], np.tile(p[None, :], (Q, 1)).reshape(-1))
         assert np.array_equal(R[:, 1], (p[:, None] * p[None, :]).reshape(-1))
         assert np.allclose(R[:, 2], (w[:, None] * w[None, :]).reshape(-1), rtol=1e-15, atol=0)
+
+
+def test_magnetic_pressure_ellipsoid_closed_form():
+    """P_mag of the surface charge of a magnetised ellipsoid (oracle P0 solve):
+    inside, H is uniform, H0 / (1 + N_z chi), so P_mag = chi |H_in|^2 / 2
+    (chi n_z^2 + 1) at every surface point (mu0 = 1).  Converges at O(h);
+    dropping the tangential field or the chi |H|^2 term fails."""
+    import oracle as O
+    from mesh_bem_amd.geometry import geodesic_sphere
+    from mesh_bem_amd.magnetic import magnetic_pressure
+
+    c, chi = 1.5, 3.0
+    V, F = geodesic_sphere(10)
+    V = V * np.array([1.0, 1.0, c])
+    ptr, idx = O.near_pairs_bruteforce(V, F)
+    Vo, Ko = O.laplace_p0(V, F, ptr, idx)
+    A, n = O.face_areas_normals(V, F)
+    sig = np.linalg.solve((1 / chi + 0.5) * np.diag(A) + Ko, A * n[:, 2])
+    phi = (Vo @ sig) / A
+    pm = magnetic_pressure(torch.tensor(V), torch.tensor(F), torch.tensor(sig), torch.tensor(phi), chi).numpy()
+    _, _, Nc = O.demag_factors(1.0, 1.0, c)
+    hin = 1.0 / (1.0 + Nc * chi)
+    ne = V / np.array([1.0, 1.0, c * c])
+    ne /= np.linalg.norm(ne, axis=1, keepdims=True)
+    ref = 0.5 * chi * hin ** 2 * (chi * ne[:, 2] ** 2 + 1.0)
+    assert np.linalg.norm(pm - ref) / np.linalg.norm(ref) < 0.025
+    # planted: no tangential field (the potential taken as the applied field's own)
+    pm_bad = magnetic_pressure(torch.tensor(V), torch.tensor(F), torch.tensor(sig),
+                               torch.tensor((V[F].mean(1)[:, 2])), chi).numpy()
+    assert np.linalg.norm(pm_bad - ref) / np.linalg.norm(ref) > 0.2

@@ -99,3 +99,47 @@ def test_p0_operators_match_oracle(dev, dtype, tol):
     ev, ek = fro(Vm.double().cpu().numpy(), Vo), fro(Km.double().cpu().numpy(), Ko)
     print(f"  P0 V fro={ev:.3e} K' fro={ek:.3e}")
     assert ev < tol and ek < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-12), (torch.float32, 2e-5)])
+def test_p1_operators_match_oracle(dev, dtype, tol):
+    from mesh_bem_amd.laplace import p1_operators
+
+    V, F = icosphere(2)
+    v = torch.tensor(V, dtype=torch.float64, device=dev)
+    f = torch.tensor(F, dtype=torch.int32, device=dev)
+    Vu, Ku = p1_operators(v, f, _lib.prec_of(dtype))
+    ptr, idx = near_pairs(V, F)
+    Vo, Ko = O.laplace_p1_ns(V, F, ptr, idx)
+    ev, ek = fro(Vu.double().cpu().numpy(), Vo), fro(Ku.double().cpu().numpy(), Ko)
+    print(f"  P1 V fro={ev:.3e} K fro={ek:.3e}")
+    assert ev < tol and ek < tol
+
+
+def test_ns_dirichlet_solve_known_answer(dev):
+    """MeshBEMSolver3d(..., mode=NS).solve: f = x^2 - y^2 on the unit sphere,
+    Neumann trace 2 (x^2 - y^2) (demo/demo_Laplace_3d.py:39-78) within 2 % at
+    320 faces, and the oracle's composition to 1e-9."""
+    from mesh_bem_amd.types import AssemblyMode
+
+    V, F = icosphere(2)
+    v = torch.tensor(V, dtype=torch.float64, device=dev)
+    f = torch.tensor(F, dtype=torch.int32, device=dev)
+    nV = len(V)
+    g = V[:, 0] ** 2 - V[:, 1] ** 2
+    solver = MeshBEMSolver3d(3, int(OrderType.LINEAR), int(BEMType.LAPLACE), 0.0, None, dtype=torch.float64,
+                             device=dev, mode=int(AssemblyMode.NS))
+    gD = torch.tensor(g[None, :, None], device=dev)
+    flags = torch.full((1, nV, 1), int(BoundaryType.DIRICHLET), dtype=torch.int32, device=dev)
+    neu, _ = solver.solve(v, f, gD, torch.zeros_like(gD), flags)
+    t = neu[0, :, 0].cpu().numpy()
+    e_known = np.linalg.norm(t - 2 * g) / np.linalg.norm(2 * g)
+    ptr, idx = near_pairs(V, F)
+    Vo, Ko = O.laplace_p1_ns(V, F, ptr, idx)
+    to, _, _ = O.dirichlet_solve_from_layers(V, F, g, Vo, Ko)
+    e_or = fro(t, to)
+    print(f"  NS Dirichlet: vs known answer {e_known:.3e}, vs oracle {e_or:.3e}")
+    assert e_known < 0.02 and e_or < 1e-9
+    with pytest.raises(NotImplementedError):
+        MeshBEMSolver3d(3, int(OrderType.PLANAR), int(BEMType.LAPLACE), 0.0, None, dtype=torch.float64,
+                        device=dev, mode=int(AssemblyMode.NS)).solve(v, f, gD, torch.zeros_like(gD), flags)

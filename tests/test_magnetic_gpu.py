@@ -75,3 +75,34 @@ def test_warm_start_cuts_iterations(dev):
         prev = cold.sigma
     print(f"  (cold, warm) iterations: {its}")
     assert all(w < c for c, w in its)
+
+
+def test_droplet_step_with_magnetic_pressure(dev):
+    """One step of the ferrofluid loop on a resting sphere in a uniform field
+    (no gravity, no surface tension): the magnetic traction is largest at the
+    poles, so the pressure solve accelerates the poles outwards and the
+    equator inwards (the droplet starts to elongate along H0).  Later steps
+    warm-start the magnetic GMRES and converge."""
+    from mesh_bem_amd.magnetic import FerrofluidDroplet
+    from mesh_bem_amd.types import AssemblyMode
+
+    V, F = geodesic_sphere(8)
+    v = torch.tensor(V, dtype=torch.float64, device=dev)
+    f = torch.tensor(F, dtype=torch.int32, device=dev)
+    vel = torch.zeros((1, len(V), 3), dtype=torch.float64, device=dev)
+    drop = FerrofluidDroplet(chi=3.0, gaussQR=3, device=dev, dtype=torch.float64, bem_mode=int(AssemblyMode.NS))
+    v1, f1, vel1, rec = drop.step(v, f, vel, dt=1e-2)
+    n = torch.tensor(V / np.linalg.norm(V, axis=1, keepdims=True), device=dev)
+    un = (vel1[0] * n).sum(-1)
+    poles, equator = n[:, 2].abs() > 0.9, n[:, 2].abs() < 0.1
+    print(f"  step 1: {rec}; outward velocity poles {float(un[poles].mean()):.3e} "
+          f"equator {float(un[equator].mean()):.3e}")
+    assert torch.isfinite(vel1).all() and rec["gmres_converged"] and not rec["warm_start"]
+    assert float(un[poles].min()) > 0 and float(un[equator].max()) < 0
+    its = [rec["gmres_iterations"]]
+    for _ in range(2):
+        v1, f1, vel1, rec = drop.step(v1, f1, vel1, dt=1e-2)
+        assert rec["warm_start"] and rec["gmres_converged"] and torch.isfinite(vel1).all()
+        its.append(rec["gmres_iterations"])
+    print(f"  magnetic GMRES iterations per step {its}")
+    assert its[2] < its[0]

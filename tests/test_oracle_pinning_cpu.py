@@ -350,3 +350,34 @@ def test_dirichlet_solve_with_exact_layers_reaches_known_answer():
     assert err(t_lumped) > 0.2
     t_ref, _, _ = O.ref_laplace_dirichlet(V, F, g, 3, 1)
     assert err(t_ref) > 1.0
+
+
+# ------------------------------------------------------------------ north-star P1 layers
+def test_ns_p1_layers_against_exact_layers_and_known_answer():
+    """The north-star P1 rule (7-point rule on far pairs; rule on T_x times
+    closed-form inner integral on near pairs) against the exact Galerkin
+    layers: far blocks to the rule's accuracy, near blocks within the outer
+    rule's error; its Dirichlet solve reaches the demo's known answer as the
+    exact layers do."""
+    V, F = icosphere(2)
+    f = np.asarray(F, dtype=np.int64)
+    ptr, idx = O.near_pairs_bruteforce(V, f)
+    Vu, Ku = O.laplace_p1_ns(V, f, ptr, idx)
+    rel = O.panel_relation_bruteforce(f)
+    tabs = O.proper_tables(8)
+    Ve = O.laplace_layer(V, f, None, None, 1, 1, rel, tabs, O.LAP_EXACT)
+    Ke = O.laplace_layer(V, f, None, None, 2, 1, rel, tabs, O.LAP_EXACT)
+    near = np.zeros((len(f), len(f)), dtype=bool)
+    for b in range(len(f)):
+        near[b, idx[ptr[b]:ptr[b + 1]]] = True
+    assert np.array_equal(near, near.T)
+    assert np.abs(Vu - Ve)[~near].max() < 1e-4 * np.abs(Ve[~near]).max()
+    assert np.abs(Ku - Ke)[~near].max() < 5e-4 * np.abs(Ke[~near]).max()
+    assert np.abs(Vu - Ve)[near].max() < 0.03 * np.abs(Ve).max()
+    assert np.abs(Ku - Ke)[near].max() < 0.05 * np.abs(Ke).max()
+    g = V[:, 0] ** 2 - V[:, 1] ** 2
+    t, _, _ = O.dirichlet_solve_from_layers(V, f, g, Vu, Ku)
+    assert np.linalg.norm(t - 2 * g) / np.linalg.norm(2 * g) < 0.02
+    # planted: the double layer with the opposite normal
+    t_bad, _, _ = O.dirichlet_solve_from_layers(V, f, g, Vu, -Ku)
+    assert np.linalg.norm(t_bad - 2 * g) / np.linalg.norm(2 * g) > 0.3
