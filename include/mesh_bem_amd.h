@@ -13,11 +13,30 @@
  *     combined complex array directly.
  *   - Matrices are row-major with an explicit leading dimension (elements).
  *   - Index arrays are int32 (as the reference's faces / face_edge_indices).
- *   - Work is enqueued on `stream` (a hipStream_t); nothing synchronises.
+ *   - Work is enqueued on `stream` (a hipStream_t).  Calls return without
+ *     waiting for the device, except these, which synchronise `stream`:
+ *     mba_lu_factor and mba_lu_factor_profiled (to read the panel wait flag;
+ *     mba_lu_factor_async does not wait), and the calls that copy a host
+ *     job table to the device (mba_lu_factor_batched,
+ *     mba_lu_solve_prepare_batched, mba_lu_solve_batched, mba_gemm_grouped,
+ *     mba_mom_zmat_faceedge_ns_batched: the host table is released on return).
  *   - Return value: 0 (MBA_OK) or an MBA_ERR_* code.  Argument and size errors
- *     are detected on the host before any launch.
- *   - No hidden global state: calls on different streams are independent,
- *     except through the buffers the caller passes.
+ *     are detected on the host before any launch.  No C++ exception leaves an
+ *     entry point (allocation failures return MBA_ERR_ALLOC).
+ *   - Memory: the caller passes every output and workspace, except that
+ *     mba_mom_zmat_faceedge_ref, mba_mom_rhs_faceedge_ref and
+ *     mba_mom_rhs_faceedge_ns take their per-face records from the stream-
+ *     ordered allocator (hipMallocAsync / hipFreeAsync on `stream`), and
+ *     mba_mom_zmat_faceedge_ref_noalloc uses a fixed device array.
+ *   - State: calls on different streams are independent except through the
+ *     buffers the caller passes and these process-wide defaults, each an
+ *     atomic value set by its setter: the FP32 GEMM tile variant
+ *     (mba_gemm_set_variant) and the split-FP16 tile (mba_gemm_split_f16_set_tile),
+ *     which change only how a product is tiled and may take effect inside a
+ *     running call; and the complex64 LU's trailing product, panel and
+ *     solve kinds (mba_lu_set_update / _panel / _solve), read once at the
+ *     start of each factorisation or solve.  mba_mom_zmat_faceedge_ref_noalloc
+ *     shares one device array per GPU between all its calls.
  */
 #ifndef MESH_BEM_AMD_H
 #define MESH_BEM_AMD_H

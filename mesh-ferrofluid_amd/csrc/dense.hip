@@ -2149,12 +2149,13 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         return MBA_OK;
     };
     // rows [r0, r0 + nb) x columns [c_lo, n): X = L(r0 block)^{-1} X
-    auto do_trsm = [&](int64_t r0, int nb, int64_t c_lo) -> int {
-        if (c_lo >= n) return MBA_OK;
+    auto do_trsm = [&](int64_t r0, int nb, int64_t c_lo, int64_t c_hi = -1) -> int {
+        if (c_hi < 0) c_hi = n;
+        if (c_lo >= c_hi) return MBA_OK;
         tm.mark(2);
         constexpr int TC = TrsmCols<T>::value;
-        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((n - c_lo + TC - 1) / TC)), dim3(256),
-                           trsm_lds_bytes<T>(nb), st, A, lda, r0, nb, c_lo, n);
+        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((c_hi - c_lo + TC - 1) / TC)), dim3(256),
+                           trsm_lds_bytes<T>(nb), st, A, lda, r0, nb, c_lo, c_hi);
         MBA_LAUNCH_CHECK();
         tm.mark(2);
         return MBA_OK;
@@ -2163,27 +2164,82 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     // across super panels).  Panel p: its columns are first updated by the
     // super panel's earlier panels (rows [kp, n)), then factored (its row
     // interchanges applied to every column outside it, including the earlier
-    // panels' L columns), then its U rows over all columns to its right get
-    // the delayed update by the earlier panels and the triangular solve.
-    // One trailing update with k = P NB follows.  Row interchanges commute with
-    // the delayed (linear) updates because they are applied to every column
-    // before them.  P NB = 512 with the split-FP16 trailing product (k = 512
-    // halves its C traffic per flop), 256 otherwise (NB = 64 with tournament
-    // panels, else 128 while the LDS allows).
-    const int P = ((sizeof(T) == 4 && upd_kind == 1) ? 4 : 2) * (tournament ? 2 : 1);
+    // panels' L columns), then its U rows to its right get the delayed update
+    // by the earlier panels and the triangular solve.  One trailing update with
+    // k = P NB follows.  Row interchanges commute with the delayed (linear)
+    // updates because they are applied to every column before them.
+    // P NB = 1024 with the split-FP16 trailing product (a wider k spreads the
+    // C read / write over more MFMA work: 316 -> 359 TFLOP/s at 20 480^2, k 512
+    // -> 1024), 256 otherwise (NB = 64 with tournament panels, else 128 while
+    // the LDS allows).  With the split product the super panel is two halves,
+    // each left-looking on its own, so the narrow FP32 products only span one
+    // half's earlier panels: when the second half starts, the first half's
+    // contribution to its columns (rows [kh, n)) is one split product of
+    // k = P NB / 2; the second half's U rows past the super panel are formed
+    // after all its interchanges (the panels never read those columns), the
+    // first half's part again as one product.
+    const bool split_upd = sizeof(T) == 4 && upd_kind == 1;
+    const int P = (split_upd ? 8 : 2) * (tournament ? 2 : 1);
+    const int HALF = split_upd ? P / 2 : P;
+    // C[r0 : r0 + M, c0 : c0 + N] -= A[r0 :, k0 : k0 + kk] A[k0 : k0 + kk, c0 :], on
+    // the split-FP16 product when both sides are large enough, else FP32
+    auto do_gemm_wide = [&](int64_t r0, int64_t M, int64_t c0, int64_t N, int64_t k0, int64_t kk) -> int {
+        if (M <= 0 || N <= 0 || kk <= 0) return MBA_OK;
+        if (M + N < SPLIT_MIN_DIM) return do_gemm_rect(r0, M, c0, N, k0, (int)kk);
+        tm.mark(3);
+        int rc2 = mba_h3::split(M, N, kk, A + r0 * lda + k0, lda, A + k0 * lda + c0, lda, ws.split,
+                                ws.split_bytes, st);
+        if (rc2 != MBA_OK) return rc2;
+        tm.mark(3);
+        tm.mark(4);
+        rc2 = mba_h3::multiply(M, N, kk, -1.f, 1.f, A + r0 * lda + c0, lda, ws.split, ws.split_bytes, st);
+        if (rc2 != MBA_OK) return rc2;
+        tm.mark(4);
+        gflops_split += 8.0 * (double)M * (double)N * (double)kk;
+        return MBA_OK;
+    };
     for (int64_t k = 0; k < n; k += (int64_t)P * NB) {
         int64_t kp = k;
         int rc;
+        const int64_t kh = k + (int64_t)HALF * NB;  // first column of the second half
+        const int64_t ce = (k + (int64_t)P * NB) < n ? (k + (int64_t)P * NB) : n;  // end of the super panel
+        bool two = false;  // the super panel is split into halves
+        int np = 0;
+        int64_t pr0[64];
+        int pnb[64];
         for (int p = 0; p < P && kp < n; ++p) {
             const int nbp = (int)((n - kp) < NB ? (n - kp) : NB);
-            const int done = (int)(kp - k);  // columns of the earlier panels
-            if (p > 0 && (rc = do_gemm_rect(kp, n - kp, kp, nbp, k, done)) != MBA_OK) return rc;
+            if (p == HALF && split_upd) {
+                // first half -> every column of the second half, rows [kp, n)
+                if ((rc = do_gemm_wide(kp, n - kp, kp, ce - kp, k, kp - k)) != MBA_OK) return rc;
+                two = true;
+            }
+            const int64_t kb = two ? kh : k;  // first column of this panel's half
+            if (kp > kb && (rc = do_gemm_rect(kp, n - kp, kp, nbp, kb, (int)(kp - kb))) != MBA_OK) return rc;
             if ((rc = do_panel(kp, nbp, n)) != MBA_OK) return rc;
-            const int64_t right = kp + nbp;
+            const int64_t r0 = kp, right = kp + nbp;
+            pr0[np] = r0;
+            pnb[np++] = nbp;
             kp = right;
             if (right >= n) break;
-            if (p > 0 && (rc = do_gemm_rect(right - nbp, nbp, right, n - right, k, done)) != MBA_OK) return rc;
-            if ((rc = do_trsm(right - nbp, nbp, right)) != MBA_OK) return rc;
+            // this panel's U rows (its half's earlier panels): the first half's
+            // over every column to the right, the second half's inside the
+            // super panel only
+            const int64_t ue = two ? ce : n;
+            if ((rc = do_gemm_rect(r0, nbp, right, ue - right, kb, (int)(r0 - kb))) != MBA_OK) return rc;
+            if ((rc = do_trsm(r0, nbp, right, ue)) != MBA_OK) return rc;
+        }
+        if (two && ce < n) {
+            // the second half's U rows past the super panel, once all its
+            // interchanges are known: the first half's part in one product,
+            // then one panel at a time inside the half
+            if ((rc = do_gemm_wide(kh, ce - kh, ce, n - ce, k, kh - k)) != MBA_OK) return rc;
+            for (int q = 0; q < np; ++q) {
+                const int64_t r0 = pr0[q];
+                if (r0 < kh) continue;
+                if ((rc = do_gemm_rect(r0, pnb[q], ce, n - ce, kh, (int)(r0 - kh))) != MBA_OK) return rc;
+                if ((rc = do_trsm(r0, pnb[q], ce)) != MBA_OK) return rc;
+            }
         }
         const int64_t c3 = kp;  // first column after the super panel
         if (c3 >= n) break;

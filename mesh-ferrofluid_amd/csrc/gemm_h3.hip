@@ -58,7 +58,7 @@ __device__ __forceinline__ void split2(float x, _Float16& h, _Float16& l)
     l = (_Float16)(x - (float)h);
 }
 
-// ---- A: one wave per row (K <= KMAX = 512: lane holds 4 consecutive k of
+// ---- A: one wave per row (K <= KMAX: lane holds 4 consecutive k of
 // each 256-wide block)
 __global__ __launch_bounds__(256) void k_split_rows(const cpx<float>* __restrict__ A, int64_t lda, int64_t M,
                                                     int K, int Kpad, int64_t Mpad, _Float16* __restrict__ Ap,

@@ -7,7 +7,7 @@
 
 namespace mba_h3 {
 
-constexpr int KMAX = 512;  // largest inner dimension
+constexpr int KMAX = 1024;  // largest inner dimension
 
 // Device workspace for gemm() with these sizes (split operands and scales).
 size_t workspace_bytes(int64_t M, int64_t N, int64_t K);

@@ -76,8 +76,8 @@ def test_split_gemm_strided_views(dev):
 
 
 def test_split_gemm_rejects_large_k(dev):
-    A = torch.zeros((4, 600), dtype=torch.complex64, device=dev)
-    B = torch.zeros((600, 4), dtype=torch.complex64, device=dev)
+    A = torch.zeros((4, 1100), dtype=torch.complex64, device=dev)
+    B = torch.zeros((1100, 4), dtype=torch.complex64, device=dev)
     from mesh_bem_amd._lib import MBAError
 
     with pytest.raises(MBAError):
