@@ -754,7 +754,7 @@ def run_c3(args, dev):
         Vs = bm.rhs(dirs, pols)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        res = bm.solve(Zs, Vs, tuner=tuner, tol=1e-8)
+        res = bm.solve(Zs, Vs, tuner=tuner, tol=1e-8, persistent=args.batched_lu == "persistent")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         return res, t1 - t0, t2 - t1, Zs, Vs
@@ -785,7 +785,10 @@ def run_c3(args, dev):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 factors (lock-step batched LU, FP32 MFMA 3M trailing updates) + c128 assembly/residual",
+        "dtype": ("c64 factors (" + ("one persistent batched-LU launch" if args.batched_lu == "persistent"
+                                     else "lock-step batched LU") +
+                  ", FP32 MFMA 3M trailing updates) + c128 assembly/residual"),
+        "batched_lu": args.batched_lu,
         "value_definition": "right-hand sides per second: meshes x waves / (FP64 assembly + RHS + FP32 LUs "
                             "+ solves + FP64 refinement) per step",
         "data": "synthetic",
@@ -959,6 +962,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the timed steps")
     ap.add_argument("--lu-update", default="split_f16", choices=["fp32", "split_f16"],
                     help="trailing-update product of the complex64 LU (c2/c1/small)")
+    ap.add_argument("--batched-lu", default="persistent", choices=["persistent", "lockstep"],
+                    help="batched LU of c3: one persistent launch or the lock-step launches")
     ap.add_argument("--lu-panel", default="partial", choices=["partial", "tournament"],
                     help="panel pivoting of the complex64 LU (c2/c1/small)")
     args = ap.parse_args()

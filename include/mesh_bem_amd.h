@@ -298,6 +298,27 @@ int mba_lu_solve_batched(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs,
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
 int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch, size_t scratch_bytes,
                           int32_t *err_out, mba_stream_t stream);
+/* The same factorisations (complex64) in ONE persistent cooperative launch:
+ * resident workgroups step through a table of phases separated by grid
+ * barriers (32-column panels of every matrix together, left-looking super
+ * panels of 256 columns, one trailing product per super panel).  Same
+ * arguments and results as mba_lu_factor_batched (ipiv with LAPACK's partial
+ * pivoting); scratch: mba_lu_batched_persistent_scratch_bytes.  complex128
+ * runs the lock-step path.  Replaces the per-mesh loop of solves in
+ * demo/demo_Helmholtz_MOM_3d.py:206-216 (each a torch.linalg.solve,
+ * src/mesh_BEM/mesh_BEM_solver_3d.py:174); no single reference call. */
+size_t mba_lu_batched_persistent_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
+int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch,
+                                     size_t scratch_bytes, int32_t *err_out, mba_stream_t stream);
+/* mba_lu_factor_batched_persistent (complex64) with phase timing from the
+ * launch's own wall clock (the call synchronises): per phase kind (0 init,
+ * 1 products, 2 panels, 3 row interchanges, 4 triangular solves, 5 final)
+ * kind_ms[kind] = summed work time (phase start to the last workgroup's end
+ * of work), kind_ms[6 + kind] = summed grid-barrier time, kind_phases[kind]
+ * = phase count.  No reference counterpart. */
+int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix *mats, int32_t n_mat, void *scratch,
+                                              size_t scratch_bytes, int32_t *err_out, double *kind_ms,
+                                              int64_t *kind_phases, mba_stream_t stream);
 /* mba_lu_factor with per-phase timing (HIP events on the stream; the call
  * synchronises): phase_ms[5] and phase_launches[5] for the panels, row
  * interchanges, triangular solves, FP32 / FP64 MFMA products (with the operand

@@ -13,6 +13,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -211,16 +212,20 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
 // and im planes stored k-major (A) and row-major (B).
 constexpr int G3M_BN = 64;
 
+// LDS of one tile: Ar, Ai [2][G32_BK][G32_BM], Br, Bi [2][G32_BK][G3M_BN] floats
+constexpr int G3M_LDS_FLOATS = 4 * G32_BK * G32_BM + 4 * G32_BK * G3M_BN;
+
 __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, float alpha,
                                              const cpx<float>* __restrict__ A, int64_t lda,
                                              const cpx<float>* __restrict__ B, int64_t ldb, float beta,
-                                             cpx<float>* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0)
+                                             cpx<float>* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0,
+                                             float* lds)
 {
     constexpr int BN = G3M_BN;
-    __shared__ alignas(16) float Ar[2][G32_BK][G32_BM];
-    __shared__ alignas(16) float Ai[2][G32_BK][G32_BM];
-    __shared__ alignas(16) float Br[2][G32_BK][BN];
-    __shared__ alignas(16) float Bi[2][G32_BK][BN];
+    float(*Ar)[G32_BK][G32_BM] = (float(*)[G32_BK][G32_BM])lds;
+    float(*Ai)[G32_BK][G32_BM] = (float(*)[G32_BK][G32_BM])(lds + 2 * G32_BK * G32_BM);
+    float(*Br)[G32_BK][BN] = (float(*)[G32_BK][BN])(lds + 4 * G32_BK * G32_BM);
+    float(*Bi)[G32_BK][BN] = (float(*)[G32_BK][BN])(lds + 4 * G32_BK * G32_BM + 2 * G32_BK * BN);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     // A: 128 rows x 16 complex k: thread -> row tid/2, k (tid&1)*8 .. +8
@@ -333,7 +338,8 @@ __global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, in
         tmi = blockIdx.x / nbn;
         tni = blockIdx.x % nbn;
     }
-    cgemm3m_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tmi * G32_BM, tni * G3M_BN);
+    __shared__ alignas(16) float lds[G3M_LDS_FLOATS];
+    cgemm3m_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tmi * G32_BM, tni * G3M_BN, lds);
 }
 
 // ---- grouped products: job j owns tiles [tile_off[j], tile_off[j + 1]),
@@ -372,8 +378,9 @@ __global__ __launch_bounds__(256, 3) void k_cgemm3m_grouped(const GemmJobDev* __
     const GemmJobDev& J = jobs[lo];
     const int64_t nbn = (J.n + G3M_BN - 1) / G3M_BN;
     const int64_t u = t - tile_off[lo];
+    __shared__ alignas(16) float lds[G3M_LDS_FLOATS];
     cgemm3m_tile(J.m, J.n, J.k, (float)J.alpha, (const cpx<float>*)J.a, J.lda, (const cpx<float>*)J.b, J.ldb,
-                 (float)J.beta, (cpx<float>*)J.c, J.ldc, (u / nbn) * G32_BM, (u % nbn) * G3M_BN);
+                 (float)J.beta, (cpx<float>*)J.c, J.ldc, (u / nbn) * G32_BM, (u % nbn) * G3M_BN, lds);
 }
 
 template <bool XCD>
@@ -890,11 +897,11 @@ __device__ __forceinline__ double val_of<double>(const uint32_t* wds)
 
 // Publishes local row r (panel-local row index pr) of the LDS panel, all nb
 // columns, into the message row `dst` (thread c writes column c).
-template <class T>
+template <class T, int NT = PANEL_THREADS>
 __device__ __forceinline__ void put_row(uint64_t* dst, uint32_t epoch, const cpx<T>* row, int nb)
 {
     constexpr int VW = sizeof(T) / 4;
-    for (int c = threadIdx.x; c < nb; c += PANEL_THREADS) {
+    for (int c = threadIdx.x; c < nb; c += NT) {
         const cpx<T> x = row ? row[c] : cpx<T>{(T)0, (T)0};
         put_val(dst + (2 * c) * VW, epoch, x.re);
         put_val(dst + (2 * c + 1) * VW, epoch, x.im);
@@ -914,16 +921,16 @@ __device__ __forceinline__ void put_row(uint64_t* dst, uint32_t epoch, const cpx
 // read every workgroup's column-j candidate, and each workgroup publishes its
 // column-j candidate only after finishing its column j - 1 reads, so the
 // column j - 1 buffers are no longer read when they are overwritten.
-template <class T>
+template <class T, int NT = PANEL_THREADS>
 __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, const int g)
 {
     constexpr int VW = sizeof(T) / 4;  // 32-bit words per real
     constexpr int32_t NONE = 0x7fffffff;
     extern __shared__ unsigned char smem_raw[];
     cpx<T>* rows = (cpx<T>*)smem_raw;  // [R][nb]
-    __shared__ T red_v[2 * PANEL_THREADS / 64];
-    __shared__ int32_t red_r[2 * PANEL_THREADS / 64];
-    __shared__ int32_t red_q[2 * PANEL_THREADS / 64];
+    __shared__ T red_v[2 * NT / 64];
+    __shared__ int32_t red_r[2 * NT / 64];
+    __shared__ int32_t red_q[2 * NT / 64];
     __shared__ cpx<T> s_prow[128];
     __shared__ cpx<T> s_drow[128];
 
@@ -934,7 +941,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     const int64_t row_words = (int64_t)nb * 2 * VW;
     auto owns = [&](int64_t pr) { return pr >= lr0 && pr < lr0 + nr; };
 
-    for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
+    for (int idx = tid; idx < nr * nb; idx += NT) {
         int r = idx / nb, c = idx % nb;
         rows[idx] = P.A[(P.k + lr0 + r) * P.lda + P.k + c];
     }
@@ -944,7 +951,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     {
         T bv = (T)-1;
         int32_t br = NONE;
-        for (int r = tid; r < nr; r += PANEL_THREADS) {
+        for (int r = tid; r < nr; r += NT) {
             const T v = cabs1(rows[r * nb]);
             if (v > bv) {
                 bv = v;
@@ -952,15 +959,15 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             }
         }
         int32_t bq = 0;
-        block_argmax<PANEL_THREADS / 64>(bv, br, bq, red_v, red_r, red_q, 0);
+        block_argmax<NT / 64>(bv, br, bq, red_v, red_r, red_q, 0);
         const int32_t lb = br;
         const uint32_t e0 = (uint32_t)(P.k + 1);
         if (tid == 0) {
             tag_put(P.meta + g, e0, (uint32_t)lb);
             put_val(P.cval + (int64_t)g * VW, e0, bv);
         }
-        put_row<T>(P.cand + (int64_t)g * row_words, e0, lb != NONE ? rows + (lb - lr0) * nb : nullptr, nb);
-        if (owns(0)) put_row<T>(P.diag, e0, rows + (0 - lr0) * nb, nb);
+        put_row<T, NT>(P.cand + (int64_t)g * row_words, e0, lb != NONE ? rows + (lb - lr0) * nb : nullptr, nb);
+        if (owns(0)) put_row<T, NT>(P.diag, e0, rows + (0 - lr0) * nb, nb);
     }
 
     for (int j = 0; j < nb; ++j) {
@@ -969,7 +976,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         // --- (1) gather the G candidates of column j (one slot per thread)
         T v = (T)-1;
         int32_t r = NONE, q = 0;
-        for (int c = tid; c < G; c += PANEL_THREADS) {
+        for (int c = tid; c < G; c += NT) {
             uint64_t* pp[1 + VW];
             pp[0] = P.meta + par * G + c;
 #pragma unroll
@@ -978,14 +985,14 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             tag_get_n<1 + VW>(pp, epoch, P.err, wd);
             better(v, r, q, val_of<T>(wd + 1), (int32_t)wd[0], (int32_t)c);
         }
-        block_argmax<PANEL_THREADS / 64>(v, r, q, red_v, red_r, red_q, 1);
+        block_argmax<NT / 64>(v, r, q, red_v, red_r, red_q, 1);
         const int wg = q;
         const int32_t p = r;
         // --- (2) the winner's row and row j
         {
             uint64_t* pw = P.cand + ((int64_t)par * G + wg) * row_words;
             uint64_t* dw = P.diag + (int64_t)par * row_words;
-            for (int c = tid; c < nb; c += PANEL_THREADS) {
+            for (int c = tid; c < nb; c += NT) {
                 uint64_t* pp[4 * VW];
 #pragma unroll
                 for (int w = 0; w < 2 * VW; ++w) {
@@ -1025,7 +1032,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         T bv = (T)-1;
         int32_t br = NONE;
         const cpx<T> u1 = j1 < nb ? s_prow[j1] : cpx<T>{(T)0, (T)0};
-        for (int rr = tid; rr < nr; rr += PANEL_THREADS) {
+        for (int rr = tid; rr < nr; rr += NT) {
             const int64_t pr = lr0 + rr;
             if (pr > j) {
                 const bool moved = pr == p;
@@ -1049,14 +1056,14 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         }
         if (p != j) {
             if (owns(p))
-                for (int c = tid; c < nb; c += PANEL_THREADS)
+                for (int c = tid; c < nb; c += NT)
                     if (c != j && c != j1) rows[(p - lr0) * nb + c] = s_drow[c];
             if (owns(j))
-                for (int c = tid; c < nb; c += PANEL_THREADS) rows[(j - lr0) * nb + c] = s_prow[c];
+                for (int c = tid; c < nb; c += NT) rows[(j - lr0) * nb + c] = s_prow[c];
         }
         if (j1 >= nb) break;
         int32_t bq = 0;
-        block_argmax<PANEL_THREADS / 64>(bv, br, bq, red_v, red_r, red_q, 0);  // its barrier orders (3) before (4)
+        block_argmax<NT / 64>(bv, br, bq, red_v, red_r, red_q, 0);  // its barrier orders (3) before (4)
         const int32_t lb = br;
         const uint32_t e1 = epoch + 1;
         const int par1 = j1 & 1;
@@ -1067,7 +1074,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         // --- (4) complete and publish the two rows column j + 1 may read:
         // the local candidate lb and row j + 1 (thread c owns column c)
         const bool own1 = owns(j1);
-        for (int c = tid; c < nb; c += PANEL_THREADS) {
+        for (int c = tid; c < nb; c += NT) {
             const cpx<T> u = s_prow[c];
             auto fin = [&](int64_t pr) {
                 cpx<T>* row = rows + (pr - lr0) * nb;
@@ -1098,7 +1105,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         {
             constexpr int UB = 8;
             const int c = tid % nb;
-            const int rstep = PANEL_THREADS / nb;
+            const int rstep = NT / nb;
             if (c > j1 && tid < rstep * nb) {
                 const cpx<T> u = s_prow[c];
                 for (int rb = tid / nb; rb < nr; rb += rstep * UB) {
@@ -1128,7 +1135,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
         // rows or s_prow / s_drow, so this update overlaps that round trip
     }
     __syncthreads();
-    for (int idx = tid; idx < nr * nb; idx += PANEL_THREADS) {
+    for (int idx = tid; idx < nr * nb; idx += NT) {
         int r = idx / nb, c = idx % nb;
         P.A[(P.k + lr0 + r) * P.lda + P.k + c] = rows[idx];
     }
@@ -3891,6 +3898,803 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     return MBA_OK;
 }
 
+// ============================================================== persistent batched LU
+// Every factorisation of a batch (complex64) in ONE cooperative launch: the
+// workgroups stay resident and step through a host-built table of phases,
+// each phase a set of independent workgroup tasks over all matrices, phases
+// separated by a grid barrier (release / acquire at agent scope, so the
+// stores of one phase are visible to every XCD in the next).
+//   * Panels are PB_NB = 32 columns: the panel rows of every matrix then fit
+//     the LDS of the resident workgroups together, so one panel phase serves
+//     the whole batch with short rows per workgroup (the exchange protocol of
+//     k_panel); the panel's first workgroup also writes the row-swap plan.
+//   * Super panels of PB_P = 8 panels: a panel's columns are first updated by
+//     the super panel's earlier panels (left-looking), after the panel the
+//     row interchanges go to every other column, and its U rows over every
+//     column to its right get the same update and the triangular solve; one
+//     trailing product with k = 256 per matrix closes the super panel.
+// Waits (panel exchange, grid barrier) are bounded: on expiry a flag is set,
+// the kernel runs to completion and every matrix reports the timeout.
+constexpr int PB_NB = 32, PB_P = 8, PB_THREADS = 256;
+constexpr int PB_DST = 128;  // offset of the slot destinations in the plan words (2 nb <= 64 pairs before it)
+constexpr int PB_WG_LDS = 80 * 1024;  // per workgroup: two workgroups share a CU
+
+enum PbKind : int { PB_INIT = 0, PB_GEMM = 1, PB_PANEL = 2, PB_LASWP = 3, PB_TRSM = 4, PB_FINAL = 5 };
+
+struct PbPhase {
+    int kind;
+    int njob;      // jobs in this phase
+    int nvb;       // workgroup tasks in this phase
+    int pad;
+    int64_t job0;  // first job in the kind's job table
+    int64_t off0;  // first entry in the kind's offset table (njob + 1 entries)
+};
+
+struct PbPanelJob {
+    PanelArgs<float> P;
+    int32_t* slot_row;
+    int32_t* plan;
+    int32_t* plan_n;
+    int64_t swaps;  // the row interchanges go to other columns (the plan is needed)
+};
+
+struct PbMat {
+    uint64_t* zero[4];  // message words to clear: meta, cval, cand (used part), diag
+    int64_t nzero[4];
+    int* err;
+    int32_t* info;
+    int32_t* err_out;
+};
+
+struct PbTables {
+    const PbPhase* phases;
+    int nphase;
+    int nmat;
+    const PbMat* mats;
+    const PbPanelJob* panels;
+    const int* panel_off;
+    const SwapJob<float>* swaps;
+    const int* swap_off;
+    const TrsmJob<float>* trsms;
+    const int* trsm_off;
+    const GemmJobDev* gemms;
+    const int* gemm_off;
+    unsigned* barrier;  // barrier words (PB_BAR_WORDS, zero at launch)
+    int* gerr;          // barrier timeout flag, zero at launch
+    uint64_t* tstamp;   // optional [3 nphase + 1]: wall clock (100 MHz) of workgroup 0 at each phase
+                        // start and at the end, then per phase the last workgroup's end of work
+                        // (before its barrier) and the last workgroup's release by that barrier
+};
+
+// Row interchanges of one panel (nb <= PB_NB) on 128 columns outside the
+// panel: thread (h, c) holds column c's values of slots [32 h, 32 h + 32) in
+// registers; all loads precede all stores (one barrier), and slot s goes to
+// row slot_row[dst[s]] (dst composed once by the panel's first workgroup).
+constexpr int PB_SWAP_COLS = 128;
+
+__device__ __forceinline__ void pb_laswp(cpx<float>* __restrict__ A, int64_t lda, int64_t n, int64_t kp, int nb,
+                                         const int32_t* __restrict__ slot_row, const int32_t* __restrict__ dst,
+                                         const int32_t* __restrict__ plan_n, int64_t blk)
+{
+    const int tid = threadIdx.x, h = tid / PB_SWAP_COLS, c = tid % PB_SWAP_COLS;
+    const int ns = *plan_n;
+    int64_t col = blk * PB_SWAP_COLS + c;  // columns [0, n - nb) with the panel's nb columns skipped
+    if (col >= kp) col += nb;
+    const bool ok = col < n;
+    cpx<float> v[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        const int sl = 32 * h + q;
+        v[q] = (ok && sl < ns) ? A[(int64_t)slot_row[sl] * lda + col] : cpx<float>{0.f, 0.f};
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // every load of the block has returned before any store
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+        const int sl = 32 * h + q;
+        if (ok && sl < ns) A[(int64_t)slot_row[dst[sl]] * lda + col] = v[q];
+    }
+}
+
+// X = L11^{-1} X for rows [kp, kp + PB_NB) of a full panel and 256 columns
+// from c_lo + 256 blk: one column per thread in registers, L11 in LDS.  No
+// per-row conditions: the loads read a clamped column and the stores are one
+// branch per thread (per-row selects turn into branches around every load and
+// the register allocator then spills the column).  Partial panels take
+// trsm_body.
+__device__ __forceinline__ void pb_trsm_full(cpx<float>* __restrict__ A, int64_t lda, int64_t kp, int64_t c_lo,
+                                             int64_t c_hi, int64_t blk, cpx<float>* Ls /* LDS [PB_NB][PB_NB] */)
+{
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < PB_NB * PB_NB; idx += PB_THREADS) Ls[idx] = A[(kp + idx / PB_NB) * lda + kp + idx % PB_NB];
+    const int64_t col = c_lo + blk * PB_THREADS + tid;
+    const bool ok = col < c_hi;
+    cpx<float>* q = A + kp * lda + (ok ? col : c_lo);
+    cpx<float> x[PB_NB];
+#pragma unroll
+    for (int r = 0; r < PB_NB; ++r) x[r] = q[(int64_t)r * lda];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PB_NB - 1; ++t) {
+        const cpx<float> xt = x[t];
+#pragma unroll
+        for (int i = t + 1; i < PB_NB; ++i) {
+            const cpx<float> l = Ls[i * PB_NB + t];
+            x[i].re -= l.re * xt.re - l.im * xt.im;
+            x[i].im -= l.re * xt.im + l.im * xt.re;
+        }
+    }
+    if (ok) {
+#pragma unroll
+        for (int r = 1; r < PB_NB; ++r) q[(int64_t)r * lda] = x[r];
+    }
+}
+
+// After swap_plan_body: dst[s] = the slot whose row slot s's values go to
+// (the panel's interchanges applied in order to slot labels).  One thread.
+__device__ __forceinline__ void pb_swap_dst(int nb, const int32_t* __restrict__ plan, int ns, int32_t* __restrict__ dst)
+{
+    int32_t cur[2 * PB_NB];
+    for (int q = 0; q < ns; ++q) cur[q] = q;
+    for (int j = 0; j < nb; ++j) {
+        const int a = plan[2 * j], b = plan[2 * j + 1];
+        const int32_t t = cur[a];
+        cur[a] = cur[b];
+        cur[b] = t;
+    }
+    for (int q = 0; q < ns; ++q) dst[cur[q]] = q;  // place q holds original slot cur[q]
+}
+
+// Grid barrier number `epoch` (1, 2, ...): every workgroup adds one to the
+// arrival count and waits until it reaches epoch x (workgroups).  The count
+// only grows, so no reset races with early arrivals at the next barrier.
+// Grid barriers.  The L2 of each XCD is not coherent with the others for
+// ordinary device memory, so a phase's stores reach another XCD only through
+// an L2 write-back on the writer's XCD and an L2 invalidate on the reader's.
+// Both are whole-cache operations: one per XCD per barrier, not one per
+// workgroup (64 of them per XCD and barrier measured ~100 us per phase).
+//   arrive: every wave drains its stores (they are in its XCD's L2); the last
+//     arriver of the XCD writes that L2 back and adds the XCD to the global count;
+//   leave: that workgroup waits for every XCD, invalidates its L2 (and its
+//     CU's L1) and raises the XCD's flag; the others wait for the flag and
+//     invalidate their CU's L1.
+// Counters only grow (barrier e waits for e x members), so no reset races
+// with early arrivals at the next barrier.  Each word sits on its own 128-byte
+// line; agent-scope atomics are performed beyond the L2s.  Waits are bounded:
+// after a timeout (gerr set) barriers stop waiting and the remaining phases
+// are skipped, so a failed launch ends after one bounded wait.
+constexpr int PB_LINE = 32;                           // words per 128-byte line
+constexpr int PB_BAR_GLOBAL = 0;                      // XCDs arrived (all barriers)
+constexpr int PB_BAR_START = 1 * PB_LINE;             // workgroups registered (start barrier)
+constexpr int PB_BAR_MEMBERS = 2 * PB_LINE;           // [8] workgroups per XCD
+constexpr int PB_BAR_ARRIVE = 10 * PB_LINE;           // [8] arrivals per XCD
+constexpr int PB_BAR_DONE = 18 * PB_LINE;             // [8] barrier number whose L2 invalidate is done
+constexpr int PB_BAR_WORDS = 26 * PB_LINE;
+constexpr int PB_MAX_XCD = 8;
+
+__device__ __forceinline__ unsigned pb_ld(unsigned* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool pb_wait_ge(unsigned* p, unsigned target, int* gerr)
+{
+    long spins = 0;
+    while (pb_ld(p) < target) {
+        if ((spins & 63) == 63 && __hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+        if (++spins > SPIN_LIMIT) {
+            __hip_atomic_fetch_or(gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return true;
+}
+
+// XCD of the calling workgroup (HW_REG_XCC_ID, bits [3:0])
+__device__ __forceinline__ int pb_xcc() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15; }
+
+struct PbBarrier {
+    int xcc;           // this workgroup's XCD
+    unsigned members;  // workgroups on it
+    unsigned nxcd;     // XCDs with workgroups
+    unsigned epoch;    // barriers passed
+};
+
+// Start: every workgroup registers on its XCD; one barrier with a full fence
+// per workgroup (once per launch) makes the counts visible.
+__device__ __forceinline__ void pb_barrier_init(PbBarrier& B, unsigned* bar, unsigned nwg, int* gerr)
+{
+    __shared__ unsigned s_members, s_nxcd;
+    __shared__ int s_xcc;
+    if (threadIdx.x == 0) {
+        int x = pb_xcc();
+        if (x >= PB_MAX_XCD) {
+            __hip_atomic_fetch_or(gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            x = 0;
+        }
+        __hip_atomic_fetch_add(bar + PB_BAR_MEMBERS + x * PB_LINE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        __hip_atomic_fetch_add(bar + PB_BAR_START, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        pb_wait_ge(bar + PB_BAR_START, nwg, gerr);
+        __threadfence();
+        unsigned nx = 0;
+        for (int q = 0; q < PB_MAX_XCD; ++q) nx += pb_ld(bar + PB_BAR_MEMBERS + q * PB_LINE) > 0 ? 1u : 0u;
+        s_xcc = x;
+        s_members = pb_ld(bar + PB_BAR_MEMBERS + x * PB_LINE);
+        s_nxcd = nx;
+    }
+    __syncthreads();
+    B.xcc = s_xcc;
+    B.members = s_members;
+    B.nxcd = s_nxcd;
+    B.epoch = 0;
+}
+
+__device__ __forceinline__ void pb_grid_sync(PbBarrier& B, unsigned* bar, int* gerr)
+{
+    const unsigned e = ++B.epoch;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores of the phase are in the L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* arrive = bar + PB_BAR_ARRIVE + B.xcc * PB_LINE;
+        unsigned* done = bar + PB_BAR_DONE + B.xcc * PB_LINE;
+        const unsigned a =
+            __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (a == e * B.members) {
+            // last of this XCD: write its L2 back, then count the XCD
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(bar + PB_BAR_GLOBAL, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            pb_wait_ge(bar + PB_BAR_GLOBAL, e * B.nxcd, gerr);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // this XCD's L2 (and this CU's L1)
+            __hip_atomic_store(done, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            pb_wait_ge(done, e, gerr);
+            asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1
+        }
+    }
+    __syncthreads();
+}
+
+// Phase bodies, each its own function so that every one gets its own
+// register allocation (inlined into one switch the largest body's pressure
+// spread to all of them).
+__device__ __noinline__ void pb_phase_panel(const PbTables& T, const PbPhase& ph, int b)
+{
+    if (b >= ph.nvb) return;
+    const int* off = T.panel_off + ph.off0;
+    const int j = job_of(off, ph.njob, b);
+    const PbPanelJob& J = T.panels[ph.job0 + j];
+    const int g = b - off[j];
+    panel_body<float, PB_THREADS>(J.P, off[j + 1] - off[j], g);
+    if (g == 0 && J.swaps) {
+        // ipiv of this panel was written by this workgroup's thread 0: the
+        // slot plan, then the slots' destinations (at plan + PB_DST, after
+        // the 2 nb pairs)
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            swap_plan_body(J.P.ipiv, J.P.k, J.P.nb, J.slot_row, J.plan, J.plan_n);
+            pb_swap_dst(J.P.nb, J.plan, *J.plan_n, J.plan + PB_DST);
+        }
+    }
+}
+
+__device__ __noinline__ void pb_phase_laswp(const PbTables& T, const PbPhase& ph, int b, int nwg)
+{
+    const int* off = T.swap_off + ph.off0;
+    for (int vb = b; vb < ph.nvb; vb += nwg) {
+        const int j = job_of(off, ph.njob, vb);
+        const SwapJob<float>& J = T.swaps[ph.job0 + j];
+        pb_laswp(J.A, J.lda, J.c_hi, J.k, J.nb, J.slot_row, J.plan + PB_DST, J.plan_n, (int64_t)(vb - off[j]));
+        __syncthreads();
+    }
+}
+
+__device__ __noinline__ void pb_phase_trsm(const PbTables& T, const PbPhase& ph, int b, int nwg, cpx<float>* lds)
+{
+    const int* off = T.trsm_off + ph.off0;
+    for (int vb = b; vb < ph.nvb; vb += nwg) {
+        const int j = job_of(off, ph.njob, vb);
+        const TrsmJob<float>& J = T.trsms[ph.job0 + j];
+        if (J.nb == PB_NB)
+            pb_trsm_full(J.A, J.lda, J.k, J.c_lo, J.c_hi, (int64_t)(vb - off[j]), lds);
+        else
+            trsm_body<float>(J.A, J.lda, J.k, J.nb, J.c_lo, J.c_hi, (int64_t)(vb - off[j]), lds,
+                             lds + (size_t)J.nb * TrsmCols<float>::value);
+        __syncthreads();
+    }
+}
+
+// tasks dealt so that each XCD (workgroup b runs on XCD b % 8; the grid is a
+// multiple of 8) walks a contiguous run of tiles
+__device__ __noinline__ void pb_phase_gemm(const PbTables& T, const PbPhase& ph, int b, int nwg, float* lds)
+{
+    const int* off = T.gemm_off + ph.off0;
+    const int per = (ph.nvb + NUM_XCD - 1) / NUM_XCD;
+    const int span = per * NUM_XCD;
+    for (int vb = b; vb < span; vb += nwg) {
+        const int t = (vb % NUM_XCD) * per + vb / NUM_XCD;
+        if (t < ph.nvb) {
+            const int j = job_of(off, ph.njob, t);
+            const GemmJobDev& J = T.gemms[ph.job0 + j];
+            const int64_t nbn = (J.n + G3M_BN - 1) / G3M_BN;
+            const int64_t u = t - off[j];
+            cgemm3m_tile(J.m, J.n, J.k, (float)J.alpha, (const cpx<float>*)J.a, J.lda, (const cpx<float>*)J.b, J.ldb,
+                         (float)J.beta, (cpx<float>*)J.c, J.ldc, (u / nbn) * G32_BM, (u % nbn) * G3M_BN, lds);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __noinline__ void pb_phase_init(const PbTables& T, int b, int nwg)
+{
+    const int64_t gt = (int64_t)b * PB_THREADS + threadIdx.x, gs = (int64_t)nwg * PB_THREADS;
+    for (int i = 0; i < T.nmat; ++i) {
+        const PbMat& M = T.mats[i];
+        for (int z = 0; z < 4; ++z)
+            for (int64_t w = gt; w < M.nzero[z]; w += gs) M.zero[z][w] = 0ull;
+        if (gt == 0) {
+            *M.err = 0;
+            *M.info = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(PB_THREADS, 2) void k_lu_batched_persistent(PbTables T)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const int b = blockIdx.x, nwg = gridDim.x, tid = threadIdx.x;
+    PbBarrier B;
+    pb_barrier_init(B, T.barrier, (unsigned)nwg, T.gerr);
+    __shared__ int s_failed;
+    for (int p = 0; p < T.nphase; ++p) {
+        const PbPhase ph = T.phases[p];
+        if (tid == 0) {
+            s_failed = __hip_atomic_load(T.gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (T.tstamp && b == 0) T.tstamp[p] = __builtin_amdgcn_s_memrealtime();
+        }
+        __syncthreads();
+        const int kind = (s_failed && ph.kind != PB_FINAL) ? -1 : ph.kind;
+        if (kind == PB_INIT) {
+            pb_phase_init(T, b, nwg);
+        } else if (kind == PB_PANEL) {
+            pb_phase_panel(T, ph, b);
+        } else if (kind == PB_LASWP) {
+            pb_phase_laswp(T, ph, b, nwg);
+        } else if (kind == PB_TRSM) {
+            pb_phase_trsm(T, ph, b, nwg, (cpx<float>*)smem_raw);
+        } else if (kind == PB_GEMM) {
+            pb_phase_gemm(T, ph, b, nwg, (float*)smem_raw);
+        } else if (kind == PB_FINAL && b == 0) {
+            for (int i = tid; i < T.nmat; i += PB_THREADS) {
+                const int e = __hip_atomic_load(T.mats[i].err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+                              __hip_atomic_load(T.gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                T.mats[i].err_out[0] = e;
+            }
+        }
+        if (T.tstamp) {
+            __syncthreads();
+            if (tid == 0)
+                __hip_atomic_fetch_max(T.tstamp + T.nphase + 1 + p, (uint64_t)__builtin_amdgcn_s_memrealtime(),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (p + 1 < T.nphase) {
+            pb_grid_sync(B, T.barrier, T.gerr);
+            if (T.tstamp && tid == 0)
+                __hip_atomic_fetch_max(T.tstamp + 2 * T.nphase + 1 + p, (uint64_t)__builtin_amdgcn_s_memrealtime(),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (T.tstamp && b == 0 && tid == 0) T.tstamp[T.nphase] = __builtin_amdgcn_s_memrealtime();
+}
+
+// Host schedule of the persistent launch (job tables and phases), built for a
+// given number of resident workgroups; `dry` only counts (scratch sizing).
+struct PbSchedule {
+    std::vector<PbPhase> phases;
+    std::vector<PbMat> mats;
+    std::vector<PbPanelJob> panels;
+    std::vector<int> panel_off;
+    std::vector<SwapJob<float>> swaps;
+    std::vector<int> swap_off;
+    std::vector<TrsmJob<float>> trsms;
+    std::vector<int> trsm_off;
+    std::vector<GemmJobDev> gemms;
+    std::vector<int> gemm_off;
+    int rmax = 0;           // largest panel rows per workgroup
+    std::vector<int> gmax;  // per matrix: largest panel workgroup count
+};
+
+struct PbMatIn {
+    cpx<float>* a;
+    int64_t n, lda;
+    int32_t* ipiv;
+    int32_t* info;
+    LuWs ws;
+};
+
+inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSchedule& S)
+{
+    const int nmat = (int)M.size();
+    int64_t nmax = 0;
+    for (const auto& m : M) nmax = m.n > nmax ? m.n : nmax;
+    S.gmax.assign(nmat, 0);
+    S.phases.push_back(PbPhase{PB_INIT, 0, 0, 0, 0, 0});
+    auto open = [&](int kind, size_t job0, std::vector<int>& off) {
+        PbPhase ph{kind, 0, 0, 0, (int64_t)job0, (int64_t)off.size()};
+        off.push_back(0);
+        return ph;
+    };
+    auto close = [&](PbPhase& ph, std::vector<int>& off) {
+        if (ph.njob > 0) S.phases.push_back(ph);
+        else off.pop_back();
+    };
+    auto gemm_job = [&](PbPhase& ph, int i, int64_t r0, int64_t Mr, int64_t c0, int64_t Nc, int64_t k0,
+                        int64_t kk) -> int {
+        if (Mr <= 0 || Nc <= 0 || kk <= 0) return MBA_OK;
+        GemmJobDev J;
+        const int64_t lda = M[i].lda;
+        J.a = M[i].a + r0 * lda + k0;
+        J.b = M[i].a + k0 * lda + c0;
+        J.c = M[i].a + r0 * lda + c0;
+        J.m = Mr;
+        J.n = Nc;
+        J.k = kk;
+        J.lda = J.ldb = J.ldc = lda;
+        J.alpha = -1.0;
+        J.beta = 1.0;
+        S.gemms.push_back(J);
+        const int64_t tiles = (int64_t)ph.nvb + grouped_tiles(MBA_F32, Mr, Nc);
+        if (tiles > 0x7fffffff) return MBA_ERR_SIZE;
+        ph.nvb = (int)tiles;
+        ph.njob += 1;
+        S.gemm_off.push_back(ph.nvb);
+        return MBA_OK;
+    };
+    const int64_t SP = (int64_t)PB_NB * PB_P;
+    for (int64_t k = 0; k < nmax; k += SP) {
+        for (int p = 0; p < PB_P; ++p) {
+            const int64_t kp = k + (int64_t)PB_NB * p;
+            if (kp >= nmax) break;
+            // [A] panel columns <- the super panel's earlier panels
+            if (p > 0) {
+                PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
+                for (int i = 0; i < nmat; ++i) {
+                    const int64_t n = M[i].n;
+                    if (kp >= n) continue;
+                    const int64_t nbp = (n - kp) < PB_NB ? (n - kp) : PB_NB;
+                    const int rc = gemm_job(ph, i, kp, n - kp, kp, nbp, k, kp - k);
+                    if (rc != MBA_OK) return rc;
+                }
+                close(ph, S.gemm_off);
+            }
+            // [B] panels: rows per workgroup R, as few as fit the resident
+            // workgroups, at least 32, at most rmax_lds; matrices that do not
+            // fit go to another panel phase
+            {
+                std::vector<int> todo;
+                for (int i = 0; i < nmat; ++i)
+                    if (kp < M[i].n) todo.push_back(i);
+                size_t t0 = 0;
+                while (t0 < todo.size()) {
+                    // largest prefix of todo that fits with R = rmax_lds
+                    size_t t1 = t0;
+                    int64_t gsum = 0;
+                    while (t1 < todo.size()) {
+                        const int64_t m = M[todo[t1]].n - kp;
+                        const int64_t g = (m + rmax_lds - 1) / rmax_lds;
+                        if (gsum + g > nwg) break;
+                        gsum += g;
+                        ++t1;
+                    }
+                    if (t1 == t0) return MBA_ERR_SIZE;  // one panel exceeds the launch
+                    // smallest R >= 32 whose workgroup total fits
+                    int64_t msum = 0;
+                    for (size_t t = t0; t < t1; ++t) msum += M[todo[t]].n - kp;
+                    int64_t R = (msum + nwg - 1) / nwg;
+                    if (R < 32) R = 32;
+                    for (;; ++R) {
+                        if (R >= rmax_lds) {
+                            R = rmax_lds;
+                            break;
+                        }
+                        int64_t gs = 0;
+                        for (size_t t = t0; t < t1; ++t) gs += (M[todo[t]].n - kp + R - 1) / R;
+                        if (gs <= nwg) break;
+                    }
+                    PbPhase ph = open(PB_PANEL, S.panels.size(), S.panel_off);
+                    for (size_t t = t0; t < t1; ++t) {
+                        const int i = todo[t];
+                        const int64_t n = M[i].n, m = n - kp;
+                        const int nbp = (int)((n - kp) < PB_NB ? (n - kp) : PB_NB);
+                        int64_t G = (m + R - 1) / R;
+                        const int64_t Ri = (m + G - 1) / G;
+                        G = (m + Ri - 1) / Ri;
+                        if (G > MAX_G) return MBA_ERR_SIZE;
+                        S.rmax = (int)Ri > S.rmax ? (int)Ri : S.rmax;
+                        S.gmax[i] = (int)G > S.gmax[i] ? (int)G : S.gmax[i];
+                        PbPanelJob J;
+                        J.P.A = M[i].a;
+                        J.P.lda = M[i].lda;
+                        J.P.k = kp;
+                        J.P.m = m;
+                        J.P.nb = nbp;
+                        J.P.R = (int)Ri;
+                        J.P.ipiv = M[i].ipiv;
+                        J.P.info = M[i].info;
+                        J.P.err = M[i].ws.err;
+                        J.P.meta = M[i].ws.meta;
+                        J.P.cval = M[i].ws.cval;
+                        J.P.cand = M[i].ws.cand;
+                        J.P.diag = M[i].ws.diag;
+                        J.slot_row = M[i].ws.slot_row;
+                        J.plan = M[i].ws.plan;
+                        J.plan_n = M[i].ws.plan_n;
+                        J.swaps = (kp > 0 || kp + nbp < n) ? 1 : 0;
+                        S.panels.push_back(J);
+                        ph.njob += 1;
+                        ph.nvb += (int)G;
+                        S.panel_off.push_back(ph.nvb);
+                    }
+                    close(ph, S.panel_off);
+                    t0 = t1;
+                }
+            }
+            // [C] row interchanges on every other column
+            {
+                PbPhase ph = open(PB_LASWP, S.swaps.size(), S.swap_off);
+                for (int i = 0; i < nmat; ++i) {
+                    const int64_t n = M[i].n;
+                    if (kp >= n) continue;
+                    const int nbp = (int)((n - kp) < PB_NB ? (n - kp) : PB_NB);
+                    const int64_t right = n - (kp + nbp);
+                    if (kp == 0 && right <= 0) continue;
+                    SwapJob<float> J;
+                    J.A = M[i].a;
+                    J.lda = M[i].lda;
+                    J.c_hi = n;
+                    J.k = kp;
+                    J.nb = nbp;
+                    J.ipiv = M[i].ipiv;
+                    J.slot_row = M[i].ws.slot_row;
+                    J.plan = M[i].ws.plan;
+                    J.plan_n = M[i].ws.plan_n;
+                    S.swaps.push_back(J);
+                    ph.njob += 1;
+                    ph.nvb += (int)((n - nbp + PB_SWAP_COLS - 1) / PB_SWAP_COLS);
+                    S.swap_off.push_back(ph.nvb);
+                }
+                close(ph, S.swap_off);
+            }
+            // [D] U rows right of the panel: the earlier panels' part, then L11^{-1}
+            if (p > 0) {
+                PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
+                for (int i = 0; i < nmat; ++i) {
+                    const int64_t n = M[i].n;
+                    if (kp >= n) continue;
+                    const int64_t nbp = (n - kp) < PB_NB ? (n - kp) : PB_NB;
+                    const int rc = gemm_job(ph, i, kp, nbp, kp + nbp, n - kp - nbp, k, kp - k);
+                    if (rc != MBA_OK) return rc;
+                }
+                close(ph, S.gemm_off);
+            }
+            {
+                PbPhase ph = open(PB_TRSM, S.trsms.size(), S.trsm_off);
+                for (int i = 0; i < nmat; ++i) {
+                    const int64_t n = M[i].n;
+                    if (kp >= n) continue;
+                    const int nbp = (int)((n - kp) < PB_NB ? (n - kp) : PB_NB);
+                    if (kp + nbp >= n) continue;
+                    TrsmJob<float> J;
+                    J.A = M[i].a;
+                    J.lda = M[i].lda;
+                    J.k = kp;
+                    J.nb = nbp;
+                    J.c_lo = kp + nbp;
+                    J.c_hi = n;
+                    S.trsms.push_back(J);
+                    ph.njob += 1;
+                    const int tc = nbp == PB_NB ? PB_THREADS : TrsmCols<float>::value;
+                    ph.nvb += (int)((n - kp - nbp + tc - 1) / tc);
+                    S.trsm_off.push_back(ph.nvb);
+                }
+                close(ph, S.trsm_off);
+            }
+        }
+        // [E] trailing product of the super panel, k = its width
+        {
+            PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
+            for (int i = 0; i < nmat; ++i) {
+                const int64_t n = M[i].n;
+                const int64_t c3 = (k + SP) < n ? (k + SP) : n;
+                if (k >= n || c3 >= n) continue;
+                const int rc = gemm_job(ph, i, c3, n - c3, c3, n - c3, k, c3 - k);
+                if (rc != MBA_OK) return rc;
+            }
+            close(ph, S.gemm_off);
+        }
+    }
+    S.phases.push_back(PbPhase{PB_FINAL, 0, 0, 0, 0, 0});
+    return MBA_OK;
+}
+
+int pb_resident(int cus, size_t dyn_lds, int* per_cu)
+{
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_lu_batched_persistent, PB_THREADS,
+                                                     dyn_lds) != hipSuccess)
+        return MBA_ERR_LAUNCH;
+    if (occ < 1) return MBA_ERR_LAUNCH;
+    *per_cu = occ < 2 ? occ : 2;
+    (void)cus;
+    return MBA_OK;
+}
+
+size_t pb_dyn_lds()
+{
+    hipFuncAttributes fa;
+    size_t stat = 8 * 1024;
+    if (hipFuncGetAttributes(&fa, (const void*)k_lu_batched_persistent) == hipSuccess) stat = fa.sharedSizeBytes;
+    const size_t dyn = PB_WG_LDS > stat + 256 ? PB_WG_LDS - stat - 256 : 0;
+    return dyn & ~(size_t)255;
+}
+
+template <class V>
+size_t pb_bytes(const std::vector<V>& v) { return al256(v.size() * sizeof(V)); }
+
+int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scratch, size_t scratch_bytes,
+                                 int32_t* err_out, hipStream_t st, double* kind_ms = nullptr,
+                                 int64_t* kind_phases = nullptr)
+{
+    const int cus = device_cus();
+    if (cus <= 0) return MBA_ERR_LAUNCH;
+    static std::once_flag attr_once;
+    std::call_once(attr_once, [&] {
+        (void)hipFuncSetAttribute((const void*)k_lu_batched_persistent, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)pb_dyn_lds());
+    });
+    const size_t dyn = pb_dyn_lds();
+    if (dyn < (size_t)G3M_LDS_FLOATS * sizeof(float) || dyn < (size_t)PB_NB * PB_NB * sizeof(cpx<float>) ||
+        dyn < trsm_lds_bytes<float>(PB_NB))
+        return MBA_ERR_LAUNCH;
+    int per_cu = 0;
+    int rc = pb_resident(cus, dyn, &per_cu);
+    if (rc != MBA_OK) return rc;
+    // a multiple of the XCD count, so workgroup b stays on XCD b % 8 in every task loop
+    const int nwg = (cus * per_cu) / NUM_XCD * NUM_XCD;
+    if (nwg < NUM_XCD) return MBA_ERR_LAUNCH;
+    const int rmax_lds = (int)(dyn / (PB_NB * sizeof(cpx<float>)));
+    std::vector<PbMatIn> in(nmat);
+    for (int i = 0; i < nmat; ++i) {
+        in[i].a = (cpx<float>*)mats[i].a;
+        in[i].n = mats[i].n;
+        in[i].lda = mats[i].lda;
+        in[i].ipiv = mats[i].ipiv;
+        in[i].info = mats[i].info;
+        in[i].ws = lu_ws_layout(MBA_F32, mats[i].n, (char*)mats[i].workspace);
+        if (mats[i].workspace_bytes < in[i].ws.total) return MBA_ERR_ALLOC;
+    }
+    PbSchedule S;
+    if ((rc = pb_build(in, nwg, rmax_lds, S)) != MBA_OK) return rc;
+    // message words each matrix's panels use: meta and cval (two parities of
+    // MAX_G), candidate rows of its largest panel workgroup count, row j
+    S.mats.resize(nmat);
+    const std::vector<int>& gmax = S.gmax;
+    for (int i = 0; i < nmat; ++i) {
+        PbMat& m = S.mats[i];
+        const LuWs& w = in[i].ws;
+        m.zero[0] = w.meta;
+        m.nzero[0] = 2 * MAX_G;
+        m.zero[1] = w.cval;
+        m.nzero[1] = 2 * MAX_G;
+        m.zero[2] = w.cand;
+        m.nzero[2] = (int64_t)2 * gmax[i] * PB_NB * 2;
+        m.zero[3] = w.diag;
+        m.nzero[3] = (int64_t)2 * PB_NB * 2;
+        m.err = w.err;
+        m.info = mats[i].info;
+        m.err_out = err_out + i;
+    }
+    // one upload: barrier words and the timeout flag, then the tables
+    const size_t o_gerr = al256(PB_BAR_WORDS * sizeof(unsigned));
+    const size_t o_ph = o_gerr + 256;
+    const size_t o_mat = o_ph + pb_bytes(S.phases);
+    const size_t o_pj = o_mat + pb_bytes(S.mats);
+    const size_t o_po = o_pj + pb_bytes(S.panels);
+    const size_t o_sj = o_po + pb_bytes(S.panel_off);
+    const size_t o_so = o_sj + pb_bytes(S.swaps);
+    const size_t o_tj = o_so + pb_bytes(S.swap_off);
+    const size_t o_to = o_tj + pb_bytes(S.trsms);
+    const size_t o_gj = o_to + pb_bytes(S.trsm_off);
+    const size_t o_go = o_gj + pb_bytes(S.gemms);
+    const size_t o_ts = o_go + pb_bytes(S.gemm_off);
+    const size_t need = o_ts + al256((3 * S.phases.size() + 1) * sizeof(uint64_t));
+    if (need > scratch_bytes) return MBA_ERR_ALLOC;
+    std::vector<unsigned char> blob(need, 0);
+    auto put = [&](size_t off, const auto& v) {
+        if (!v.empty()) std::memcpy(blob.data() + off, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(o_ph, S.phases);
+    put(o_mat, S.mats);
+    put(o_pj, S.panels);
+    put(o_po, S.panel_off);
+    put(o_sj, S.swaps);
+    put(o_so, S.swap_off);
+    put(o_tj, S.trsms);
+    put(o_to, S.trsm_off);
+    put(o_gj, S.gemms);
+    put(o_go, S.gemm_off);
+    char* base = (char*)scratch;
+    if (hipMemcpyAsync(base, blob.data(), need, hipMemcpyHostToDevice, st) != hipSuccess) return MBA_ERR_LAUNCH;
+    // the host table is released on return: wait for the copy
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    PbTables T;
+    T.phases = (const PbPhase*)(base + o_ph);
+    T.nphase = (int)S.phases.size();
+    T.nmat = nmat;
+    T.mats = (const PbMat*)(base + o_mat);
+    T.panels = (const PbPanelJob*)(base + o_pj);
+    T.panel_off = (const int*)(base + o_po);
+    T.swaps = (const SwapJob<float>*)(base + o_sj);
+    T.swap_off = (const int*)(base + o_so);
+    T.trsms = (const TrsmJob<float>*)(base + o_tj);
+    T.trsm_off = (const int*)(base + o_to);
+    T.gemms = (const GemmJobDev*)(base + o_gj);
+    T.gemm_off = (const int*)(base + o_go);
+    T.barrier = (unsigned*)base;
+    T.gerr = (int*)(base + o_gerr);
+    T.tstamp = kind_ms ? (uint64_t*)(base + o_ts) : nullptr;
+    void* kargs[] = {&T};
+    if (hipLaunchCooperativeKernel((const void*)k_lu_batched_persistent, dim3((unsigned)nwg), dim3(PB_THREADS), kargs,
+                                   (unsigned)dyn, st) != hipSuccess)
+        return MBA_ERR_LAUNCH;
+    if (kind_ms) {
+        // per phase kind: summed work time (phase start to the last
+        // workgroup's end of work), summed barrier time (from there to the
+        // next phase's start) and phase counts
+        const size_t np = S.phases.size();
+        std::vector<uint64_t> ts(3 * np + 1);
+        if (hipMemcpyAsync(ts.data(), base + o_ts, ts.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return MBA_ERR_LAUNCH;
+        for (int q = 0; q < 12; ++q) kind_ms[q] = 0.0;
+        for (int q = 0; q < 6; ++q)
+            if (kind_phases) kind_phases[q] = 0;
+        for (size_t q = 0; q < np; ++q) {
+            // a phase runs from the last workgroup's release by the previous
+            // barrier (the first phase: workgroup 0's start) to the last
+            // workgroup's end of work; its barrier from there to the last release
+            const int kd = S.phases[q].kind;
+            const uint64_t t0 = q > 0 && ts[2 * np + q] > ts[q] ? ts[2 * np + q] : ts[q];
+            const uint64_t t1 = ts[np + 1 + q] > t0 ? ts[np + 1 + q] : t0;
+            const uint64_t t2 = q + 1 < np ? (ts[2 * np + 1 + q] > t1 ? ts[2 * np + 1 + q] : t1) : (ts[np] > t1 ? ts[np] : t1);
+            kind_ms[kd] += (double)(t1 - t0) * 1e-5;  // 100 MHz ticks -> ms
+            kind_ms[6 + kd] += (double)(t2 - t1) * 1e-5;
+            if (kind_phases) kind_phases[kd] += 1;
+        }
+    }
+    return MBA_OK;
+}
+
+// Upper bound of the persistent schedule's scratch for n_mat matrices of order <= n_max.
+size_t pb_scratch_bound(int32_t n_mat, int64_t n_max)
+{
+    const size_t np = (size_t)((n_max + PB_NB - 1) / PB_NB);  // panels per matrix
+    const size_t sp = (size_t)((n_max + PB_NB * PB_P - 1) / (PB_NB * PB_P));
+    const size_t nm = (size_t)n_mat;
+    const size_t phases = 2 + sp + np * (4 + nm);  // panel phases may split, at most once per matrix
+    const size_t pj = nm * np, sj = nm * np, tj = nm * np, gj = nm * (2 * np + sp);
+    auto a = [](size_t x) { return al256(x); };
+    return a(PB_BAR_WORDS * sizeof(unsigned)) + 256 + a((3 * phases + 1) * sizeof(uint64_t)) +
+           a(phases * sizeof(PbPhase)) + a(nm * sizeof(PbMat)) +
+           a(pj * sizeof(PbPanelJob)) +
+           a((pj + phases) * sizeof(int)) + a(sj * sizeof(SwapJob<float>)) + a((sj + phases) * sizeof(int)) +
+           a(tj * sizeof(TrsmJob<float>)) + a((tj + phases) * sizeof(int)) + a(gj * sizeof(GemmJobDev)) +
+           a((gj + phases) * sizeof(int));
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -4031,6 +4835,47 @@ int mba_lu_factor_batched(int prec, const mba_lu_matrix* mats, int32_t n_mat, vo
         if (prec == MBA_F32)
             return lu_factor_batched<float>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
         return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+    });
+}
+
+size_t mba_lu_batched_persistent_scratch_bytes(int prec, int32_t n_mat, int64_t n_max)
+{
+    if (n_mat <= 0 || n_max <= 0) return 0;
+    const size_t a = pb_scratch_bound(n_mat, n_max), b = mba_lu_batched_scratch_bytes(prec, n_mat, n_max);
+    return a > b ? a : b;
+}
+
+int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix* mats, int32_t n_mat, void* scratch,
+                                     size_t scratch_bytes, int32_t* err_out, mba_stream_t stream)
+{
+    if (!mats || !scratch || !err_out || n_mat <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    for (int i = 0; i < n_mat; ++i) {
+        const mba_lu_matrix& m = mats[i];
+        if (!m.a || !m.ipiv || !m.info || !m.workspace) return MBA_ERR_ARG;
+        if (m.n <= 0 || m.n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+        if (m.lda < m.n) return MBA_ERR_ARG;
+    }
+    return mba_guard([&] {
+        if (prec == MBA_F32) return lu_factor_batched_persistent(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+        return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+    });
+}
+
+int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix* mats, int32_t n_mat, void* scratch,
+                                              size_t scratch_bytes, int32_t* err_out, double* kind_ms,
+                                              int64_t* kind_phases, mba_stream_t stream)
+{
+    if (!mats || !scratch || !err_out || !kind_ms || n_mat <= 0) return MBA_ERR_ARG;
+    for (int i = 0; i < n_mat; ++i) {
+        const mba_lu_matrix& m = mats[i];
+        if (!m.a || !m.ipiv || !m.info || !m.workspace) return MBA_ERR_ARG;
+        if (m.n <= 0 || m.n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+        if (m.lda < m.n) return MBA_ERR_ARG;
+    }
+    return mba_guard([&] {
+        return lu_factor_batched_persistent(mats, n_mat, scratch, scratch_bytes, err_out, S(stream), kind_ms,
+                                            kind_phases);
     });
 }
 

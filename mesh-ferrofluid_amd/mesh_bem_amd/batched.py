@@ -165,7 +165,7 @@ class BatchedMoM:
         return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
-              tol: float = 1e-8, maxit: int = 10) -> BatchResult:
+              tol: float = 1e-8, maxit: int = 10, persistent: bool = True) -> BatchResult:
         """Mixed precision for all meshes: the FP32 LUs in lock step
         (``lu_factor_batched``: the panels of every matrix in shared
         cooperative launches, the trailing products of every matrix in one
@@ -184,7 +184,7 @@ class BatchedMoM:
             tuner.apply(max(Z.shape[0] for Z in Zs))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lus = lu_factor_batched([to_c64(Z) for Z in Zs], overwrite=True)
+        lus = lu_factor_batched([to_c64(Z) for Z in Zs], overwrite=True, persistent=persistent)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for lu in lus:

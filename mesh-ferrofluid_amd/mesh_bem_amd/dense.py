@@ -20,14 +20,20 @@ def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alp
          beta: float = 0.0) -> torch.Tensor:
     """C = alpha A B + beta C for row-major complex tensors (MFMA kernel)."""
     lib = _lib.load()
+    if A.dim() != 2 or B.dim() != 2:
+        raise ValueError("gemm: 2-D operands")
     M, K = A.shape
     K2, N = B.shape
-    assert K == K2 and A.dtype == B.dtype
+    if K != K2 or A.dtype != B.dtype or A.dtype not in (torch.complex64, torch.complex128):
+        raise ValueError("gemm: complex A [M, K], B [K, N] of one dtype")
     if C is None:
         C = torch.zeros((M, N), dtype=A.dtype, device=A.device)
         beta = 0.0
+    if C.dtype != A.dtype or tuple(C.shape) != (M, N):
+        raise ValueError("gemm: C [M, N] of the operands' dtype")
     for t in (A, B, C):
-        assert t.stride(1) == 1, "row-major with unit column stride"
+        if t.stride(1) != 1:
+            raise ValueError("gemm: row-major operands with unit column stride")
     st = lib.mba_gemm(_lib.prec_of(A.dtype), M, N, K, float(alpha), _lib.ptr(A), A.stride(0),
                       _lib.ptr(B), B.stride(0), float(beta), _lib.ptr(C), C.stride(0), _lib.stream())
     _lib.check(st, "mba_gemm")
@@ -208,10 +214,14 @@ class LUFactor:
         return X.reshape(-1) if vec else X
 
 
-def lu_factor_batched(mats, overwrite: bool = True):
-    """LU of several square complex matrices (same dtype) in lock step
-    (``mba_lu_factor_batched``); returns one LUFactor per matrix.  Nothing is
-    synchronised: call ``check()`` on each after synchronising."""
+def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True):
+    """LU of several square complex matrices (same dtype); returns one
+    LUFactor per matrix.  ``persistent`` (default): the whole batch in one
+    persistent cooperative launch (``mba_lu_factor_batched_persistent``:
+    32-column panels of every matrix together, phases separated by grid
+    barriers; complex128 takes the lock-step path); otherwise the lock-step
+    multi-launch form (``mba_lu_factor_batched``).  Nothing is synchronised:
+    call ``check()`` on each after synchronising."""
     lib = _lib.load()
     if not mats:
         return []
@@ -239,10 +249,17 @@ def lu_factor_batched(mats, overwrite: bool = True):
         lus.append((lu, ipiv, info))
         keep.append(ws)
     nmax = max(A.shape[0] for A in mats)
-    sb = lib.mba_lu_batched_scratch_bytes(prec, n_mat, nmax)
-    scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
-    st = lib.mba_lu_factor_batched(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err), _lib.stream())
-    _lib.check(st, "mba_lu_factor_batched")
+    if persistent:
+        sb = lib.mba_lu_batched_persistent_scratch_bytes(prec, n_mat, nmax)
+        scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
+        st = lib.mba_lu_factor_batched_persistent(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err),
+                                                  _lib.stream())
+        _lib.check(st, "mba_lu_factor_batched_persistent")
+    else:
+        sb = lib.mba_lu_batched_scratch_bytes(prec, n_mat, nmax)
+        scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
+        st = lib.mba_lu_factor_batched(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err), _lib.stream())
+        _lib.check(st, "mba_lu_factor_batched")
     out = [LUFactor._from_factored(lu, ipiv, info, err[i:i + 1]) for i, (lu, ipiv, info) in enumerate(lus)]
     jobs = (_lib.LuPrepareJob * n_mat)()
     for i, f in enumerate(out):
@@ -319,11 +336,15 @@ def lu_solve_batched(lus, Bs):
 
 
 def residual_c128(Z: torch.Tensor, X: torch.Tensor, V: torch.Tensor) -> torch.Tensor:
-    """r = V - Z X in FP64 (complex128 Z, X, V)."""
+    """r = V - Z X in FP64 (complex128 Z; X and V are taken in complex128)."""
+    if Z.dim() != 2 or Z.shape[0] != Z.shape[1] or Z.dtype != torch.complex128 or Z.stride(1) != 1:
+        raise ValueError("residual_c128: square complex128 Z with unit column stride")
     lib = _lib.load()
     n = Z.shape[0]
-    X2 = X.reshape(n, -1).contiguous()
+    X2 = X.reshape(n, -1).to(torch.complex128).contiguous()
     V2 = V.reshape(n, -1).to(torch.complex128).contiguous()
+    if X2.shape != V2.shape:
+        raise ValueError("residual_c128: X and V of one shape")
     R = torch.empty_like(V2)
     st = lib.mba_residual_c128(_lib.ptr(Z), n, Z.stride(0), _lib.ptr(X2), _lib.ptr(V2), _lib.ptr(R),
                                X2.shape[1], _lib.stream())
@@ -332,6 +353,18 @@ def residual_c128(Z: torch.Tensor, X: torch.Tensor, V: torch.Tensor) -> torch.Te
 
 
 def to_c64(Z: torch.Tensor) -> torch.Tensor:
+    """complex64 copy of a 2-D matrix (rounded from complex128 on the device;
+    a complex64 input is copied as is).  The conversion kernel reads 16-byte
+    elements, so anything but a complex128 matrix with unit column stride is
+    rejected (or copied) before the launch."""
+    if Z.dim() != 2:
+        raise ValueError("to_c64: a 2-D matrix")
+    if Z.dtype == torch.complex64:
+        return Z.clone(memory_format=torch.contiguous_format)
+    if Z.dtype != torch.complex128:
+        raise ValueError(f"to_c64: complex128 or complex64 input, not {Z.dtype}")
+    if Z.stride(1) != 1:
+        Z = Z.contiguous()
     lib = _lib.load()
     m, n = Z.shape
     out = torch.empty((m, n), dtype=torch.complex64, device=Z.device)

@@ -48,16 +48,19 @@ def test_batched_matches_oracle_and_solves(dev):
     assert set(tuner.best.values()) <= set(range(6))
 
 
-def test_lu_factor_batched_matches_single(dev):
-    """Lock-step batched LU of uneven matrices: every factor solves its system
-    like the single-matrix LU (same pivots in exact ties aside) to FP32 accuracy."""
+@pytest.mark.parametrize("persistent", [True, False])
+def test_lu_factor_batched_matches_single(dev, persistent):
+    """Batched LU of uneven matrices (one persistent launch, or lock step):
+    every factor solves its system like the single-matrix LU (same pivots in
+    exact ties aside) to FP32 accuracy."""
     from mesh_bem_amd.dense import LUFactor, lu_factor_batched
 
+    print(f"batched LU persistent={persistent}", flush=True)
     rng = np.random.default_rng(4)
     sizes = [300, 1000, 77, 640, 1301, 256]
     mats = [torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * n ** 0.5 * np.eye(n),
                          dtype=torch.complex64, device=dev) for n in sizes]
-    lus = lu_factor_batched([A.clone() for A in mats])
+    lus = lu_factor_batched([A.clone() for A in mats], persistent=persistent)
     torch.cuda.synchronize()
     for A, lu in zip(mats, lus):
         lu.check()
@@ -163,3 +166,100 @@ def test_lu_factor_batched_tile_variants(dev):
         _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, 4), "mba_gemm_set_variant")
     for p in piv[1:]:
         assert all(torch.equal(a, b) for a, b in zip(piv[0], p))
+
+
+def _perm_of(ipiv):
+    """Row permutation of LAPACK-style sequential interchanges (0-based)."""
+    p = np.arange(len(ipiv))
+    for j, q in enumerate(ipiv):
+        p[j], p[q] = p[q], p[j]
+    return p
+
+
+def _check_factor(A, lu, tol):
+    """P A = L U to `tol` (relative Frobenius) and partial pivoting's bound on
+    the multipliers (|l_ij| <= |u_jj| in the |re| + |im| sense, so |l| <= sqrt 2)."""
+    n = A.shape[0]
+    LU = lu.lu.to(torch.complex128)
+    L = torch.tril(LU, -1) + torch.eye(n, dtype=LU.dtype, device=LU.device)
+    U = torch.triu(LU)
+    perm = torch.tensor(_perm_of(lu.ipiv.cpu().numpy().astype(np.int64)), device=A.device)
+    A64 = A.to(torch.complex128)
+    err = (torch.linalg.matrix_norm(A64[perm] - L @ U) / torch.linalg.matrix_norm(A64)).item()
+    lmax = float(torch.tril(LU, -1).abs().max()) if n > 1 else 0.0
+    assert err < tol, (n, err)
+    assert lmax <= 2 ** 0.5 * (1 + 1e-5), (n, lmax)
+    return err
+
+
+def test_lu_batched_persistent_random_pivoting(dev):
+    """One persistent launch for random complex64 matrices that need row
+    interchanges, orders around the 32-column panel and the 256-column super
+    panel (1 ... 2049): P A = L U to FP32 accuracy with partial pivoting's
+    multiplier bound, and solutions as accurate as the lock-step factors'."""
+    from mesh_bem_amd.dense import lu_factor_batched
+
+    rng = np.random.default_rng(17)
+    sizes = [1, 31, 32, 33, 255, 256, 257, 700, 1301, 2049]
+    print(f"persistent batched LU, random matrices {sizes}", flush=True)
+    mats = [torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), dtype=torch.complex64,
+                         device=dev) for n in sizes]
+    lp = lu_factor_batched([A.clone() for A in mats], persistent=True)
+    ll = lu_factor_batched([A.clone() for A in mats], persistent=False)
+    torch.cuda.synchronize()
+    for A, fp, fl in zip(mats, lp, ll):
+        fp.check()
+        fl.check()
+        n = A.shape[0]
+        e = _check_factor(A, fp, 1e-4)
+        b = torch.tensor(rng.standard_normal(n) + 1j * rng.standard_normal(n), dtype=torch.complex64, device=dev)
+        A64, b64 = A.to(torch.complex128), b.to(torch.complex128)
+        rp = ((A64 @ fp.solve(b).to(torch.complex128) - b64).norm() / b64.norm()).item()
+        rl = ((A64 @ fl.solve(b).to(torch.complex128) - b64).norm() / b64.norm()).item()
+        print(f"  n={n} backward {e:.2e} residual persistent {rp:.2e} lock-step {rl:.2e}")
+        assert rp < max(10 * rl, 1e-5)
+
+
+def test_lu_batched_persistent_split_panel_phases(dev):
+    """Forty 4000 x 4000 matrices: the first panels' rows exceed the LDS of
+    the resident workgroups together, so each of those panel steps runs as two
+    panel phases; every factor still solves its system to FP32 accuracy."""
+    from mesh_bem_amd.dense import lu_factor_batched
+
+    rng = np.random.default_rng(23)
+    n, nm = 4000, 40
+    print(f"persistent batched LU, {nm} x {n}", flush=True)
+    base = torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), dtype=torch.complex64,
+                        device=dev)
+    # distinct matrices: column scalings and a per-matrix diagonal shift of the base
+    mats = []
+    for i in range(nm):
+        s = torch.tensor(rng.uniform(0.5, 2.0, n), dtype=torch.float32, device=dev)
+        mats.append((base * s[None, :] + (0.1 * i) * torch.eye(n, dtype=torch.complex64, device=dev)).contiguous())
+    lus = lu_factor_batched([A.clone() for A in mats], persistent=True)
+    torch.cuda.synchronize()
+    for i, (A, f) in enumerate(zip(mats, lus)):
+        f.check()
+        b = torch.ones(n, dtype=torch.complex64, device=dev)
+        A64, x = A.to(torch.complex128), f.solve(b).to(torch.complex128)
+        # normwise backward error of the solution
+        r = ((A64 @ x - b.to(torch.complex128)).norm() /
+             (torch.linalg.matrix_norm(A64) * x.norm() + b.norm())).item()
+        assert r < 1e-5, (i, r)
+        if i in (0, nm - 1):
+            _check_factor(A, f, 1e-4)
+
+
+def test_lu_batched_persistent_complex128_takes_lock_step(dev):
+    """complex128 through the persistent entry point factors in lock step:
+    pivots equal the single-matrix complex128 LU's."""
+    from mesh_bem_amd.dense import LUFactor, lu_factor_batched
+
+    rng = np.random.default_rng(5)
+    mats = [torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), dtype=torch.complex128,
+                         device=dev) for n in (90, 300)]
+    lus = lu_factor_batched([A.clone() for A in mats], persistent=True)
+    torch.cuda.synchronize()
+    for A, f in zip(mats, lus):
+        f.check()
+        assert torch.equal(f.ipiv, LUFactor(A.clone()).ipiv)

@@ -80,3 +80,20 @@ def test_generated_triples_round_trip(tmp_path):
         assert a.mesh == b.mesh and a.k == b.k
         assert np.array_equal(a.dirs, b.dirs) and np.array_equal(a.pols, b.pols)
         assert torch.equal(a.x, b.x) and torch.equal(a.J, b.J)
+
+
+def test_to_c64_checks_dtype_before_launch():
+    """The rounding kernel reads 16-byte elements: a complex64 matrix is
+    copied as is and anything but complex128 / complex64 is rejected, both
+    before any library call (so this runs without a GPU)."""
+    import torch
+
+    from mesh_bem_amd.dense import to_c64
+
+    z = torch.arange(6, dtype=torch.float32).reshape(2, 3).to(torch.complex64)
+    c = to_c64(z)
+    assert c.dtype == torch.complex64 and torch.equal(c, z) and c.data_ptr() != z.data_ptr()
+    with pytest.raises(ValueError):
+        to_c64(torch.zeros((2, 2), dtype=torch.float64))
+    with pytest.raises(ValueError):
+        to_c64(torch.zeros(4, dtype=torch.complex128))
