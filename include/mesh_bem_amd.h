@@ -313,9 +313,12 @@ int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix *mats, int32_
 /* mba_lu_factor_batched_persistent (complex64) with phase timing from the
  * launch's own wall clock (the call synchronises): per phase kind (0 init,
  * 1 products, 2 panels, 3 row interchanges, 4 triangular solves, 5 final)
- * kind_ms[kind] = summed work time (phase start to the last workgroup's end
- * of work), kind_ms[6 + kind] = summed grid-barrier time, kind_phases[kind]
- * = phase count.  No reference counterpart. */
+ * kind_ms[kind] = summed work time (from the last workgroup's release by the
+ * previous barrier to the last workgroup's end of work), kind_ms[6 + kind] =
+ * summed grid-barrier time, kind_ms[12 + i] = the products' work time split
+ * into panel-column updates (i = 0), U-row updates (1) and trailing products
+ * (2), kind_ms[15] = 0; kind_phases[kind] = phase count.  No reference
+ * counterpart. */
 int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix *mats, int32_t n_mat, void *scratch,
                                               size_t scratch_bytes, int32_t *err_out, double *kind_ms,
                                               int64_t *kind_phases, mba_stream_t stream);

@@ -3925,7 +3925,7 @@ struct PbPhase {
     int kind;
     int njob;      // jobs in this phase
     int nvb;       // workgroup tasks in this phase
-    int pad;
+    int sub;       // products: 0 panel columns, 1 U rows, 2 trailing (timing only)
     int64_t job0;  // first job in the kind's job table
     int64_t off0;  // first entry in the kind's offset table (njob + 1 entries)
 };
@@ -4360,6 +4360,7 @@ inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSche
             // [A] panel columns <- the super panel's earlier panels
             if (p > 0) {
                 PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
+                ph.sub = 0;
                 for (int i = 0; i < nmat; ++i) {
                     const int64_t n = M[i].n;
                     if (kp >= n) continue;
@@ -4470,6 +4471,7 @@ inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSche
             // [D] U rows right of the panel: the earlier panels' part, then L11^{-1}
             if (p > 0) {
                 PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
+                ph.sub = 1;
                 for (int i = 0; i < nmat; ++i) {
                     const int64_t n = M[i].n;
                     if (kp >= n) continue;
@@ -4505,6 +4507,7 @@ inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSche
         // [E] trailing product of the super panel, k = its width
         {
             PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
+            ph.sub = 2;
             for (int i = 0; i < nmat; ++i) {
                 const int64_t n = M[i].n;
                 const int64_t c3 = (k + SP) < n ? (k + SP) : n;
@@ -4659,7 +4662,7 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
                 hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return MBA_ERR_LAUNCH;
-        for (int q = 0; q < 12; ++q) kind_ms[q] = 0.0;
+        for (int q = 0; q < 16; ++q) kind_ms[q] = 0.0;
         for (int q = 0; q < 6; ++q)
             if (kind_phases) kind_phases[q] = 0;
         for (size_t q = 0; q < np; ++q) {
@@ -4672,6 +4675,8 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
             const uint64_t t2 = q + 1 < np ? (ts[2 * np + 1 + q] > t1 ? ts[2 * np + 1 + q] : t1) : (ts[np] > t1 ? ts[np] : t1);
             kind_ms[kd] += (double)(t1 - t0) * 1e-5;  // 100 MHz ticks -> ms
             kind_ms[6 + kd] += (double)(t2 - t1) * 1e-5;
+            if (kd == PB_GEMM && S.phases[q].sub >= 0 && S.phases[q].sub < 3)
+                kind_ms[12 + S.phases[q].sub] += (double)(t1 - t0) * 1e-5;
             if (kind_phases) kind_phases[kd] += 1;
         }
     }

@@ -209,22 +209,32 @@ def load():
 
 
 def _quiesce():
-    """At interpreter exit: wait for all queued device work (no kernel of the
-    library is still running when the HIP runtime and any profiler tear down)
-    and hand the memory cached by the stream-ordered allocator (the library's
-    hipMallocAsync records) back to the device."""
+    """At interpreter exit: wait for all queued device work, hand the memory
+    cached by the stream-ordered allocator (the library's hipMallocAsync
+    records) back to the device, then unload the library.  Every HIP shared
+    library registers, at load, an exit handler that unregisters its code
+    objects from the HIP runtime; left to the C exit handlers it runs after a
+    profiler's own exit-time finalisation (rocprofv3's "tool finalization"),
+    into a runtime that profiler has already taken apart (the SIGSEGV inside
+    exit() of profiled runs, absent from runs that never load this library).
+    Unloading here runs that handler while the runtime is whole and takes it
+    off the exit list."""
+    global _LIB
     try:
-        if not torch.cuda.is_available() or not torch.cuda.is_initialized():
-            return
-        torch.cuda.synchronize()
-        hip = ctypes.CDLL("libamdhip64.so")
-        dev = ctypes.c_int(0)
-        if hip.hipGetDevice(ctypes.byref(dev)) != 0:
-            return
-        pool = ctypes.c_void_p()
-        if hip.hipDeviceGetDefaultMemPool(ctypes.byref(pool), dev) == 0:
-            hip.hipMemPoolTrimTo(pool, ctypes.c_size_t(0))
-        hip.hipDeviceSynchronize()
+        if torch.cuda.is_available() and torch.cuda.is_initialized():
+            torch.cuda.synchronize()
+            hip = ctypes.CDLL("libamdhip64.so")
+            dev = ctypes.c_int(0)
+            if hip.hipGetDevice(ctypes.byref(dev)) == 0:
+                pool = ctypes.c_void_p()
+                if hip.hipDeviceGetDefaultMemPool(ctypes.byref(pool), dev) == 0:
+                    hip.hipMemPoolTrimTo(pool, ctypes.c_size_t(0))
+                hip.hipDeviceSynchronize()
+        if _LIB is not None:
+            import _ctypes
+
+            handle, _LIB = _LIB._handle, None
+            _ctypes.dlclose(handle)
     except Exception:
         pass
 

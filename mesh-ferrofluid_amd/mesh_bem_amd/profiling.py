@@ -9,6 +9,11 @@
       a plain device-to-device copy of N MiB (R times) on cuda:0: the
       reference workload whose FETCH_SIZE / WRITE_SIZE readings calibrate the
       counters (bytes moved are known exactly: N MiB read, N MiB written).
+  python -m mesh_bem_amd.profiling mfma-rate [--n N] [--m M]
+      what the matrix cores deliver on this device: torch.matmul (the vendor
+      BLAS) FP16 / BF16 at N^3, and this library's split-FP16 complex product
+      (mba_gemm_split_f16) at M x M x k for k = 512 and 1024, each timed with
+      device events after a warm-up.
 """
 
 import argparse
@@ -118,6 +123,44 @@ def copy_run(mib: int = 1024, reps: int = 5):
     print(f"copied {mib} MiB x {reps}", flush=True)
 
 
+def mfma_rate(n: int = 16384, m: int = 29696, reps: int = 5):
+    import os
+    import torch
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mesh_bem_amd.dense import gemm_split_f16
+
+    dev = torch.device("cuda:0")
+
+    def timed(fn, r):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(r):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / r * 1e-3
+
+    for dt in (torch.float16, torch.bfloat16):
+        a = torch.randn(n, n, device=dev, dtype=dt)
+        b = torch.randn(n, n, device=dev, dtype=dt)
+        t = timed(lambda: torch.matmul(a, b), reps)
+        print(f"torch.matmul {str(dt).split('.')[-1]} {n}^3: {t * 1e3:.2f} ms, {2 * n ** 3 / t / 1e12:.1f} TFLOP/s",
+              flush=True)
+        del a, b
+    for k in (512, 1024):
+        A = torch.randn(m, k, device=dev, dtype=torch.complex64)
+        B = torch.randn(k, m, device=dev, dtype=torch.complex64)
+        C = torch.randn(m, m, device=dev, dtype=torch.complex64)
+        t = timed(lambda: gemm_split_f16(A, B, C, -1.0, 1.0), 3)
+        tf = 8 * m * m * k / t / 1e12
+        print(f"split-FP16 complex product {m}^2 x {k}: {t * 1e3:.2f} ms, {tf:.1f} TFLOP/s algorithmic "
+              f"({3 * tf:.1f} TFLOP/s of FP16 MFMA work)", flush=True)
+        del A, B, C
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -132,11 +175,16 @@ def main(argv=None):
     cp = sub.add_parser("copy")
     cp.add_argument("--mib", type=int, default=1024)
     cp.add_argument("--reps", type=int, default=5)
+    mr = sub.add_parser("mfma-rate")
+    mr.add_argument("--n", type=int, default=16384)
+    mr.add_argument("--m", type=int, default=29696)
     a = ap.parse_args(argv)
     if a.what == "kernels":
         print_kernels(a.db, a.top, a.cmd)
     elif a.what == "pmc":
         print_pmc(a.db, a.kernel, a.cmd)
+    elif a.what == "mfma-rate":
+        mfma_rate(a.n, a.m)
     else:
         copy_run(a.mib, a.reps)
 
