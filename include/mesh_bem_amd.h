@@ -230,7 +230,7 @@ int mba_lu_factor_async(int prec, void *a, int64_t n, int64_t lda, int32_t *ipiv
                         mba_stream_t stream);
 /* Several factorisations in lock step (one matrix per entry, each with the
  * workspace of mba_lu_workspace_bytes for its order): the panels of all
- * matrices are factored together in cooperative launches, so the per-column
+ * matrices are factored together in shared launches, so the per-column
  * exchange latency is shared.  scratch: mba_lu_batched_scratch_bytes; the
  * call copies its panel schedule there and waits for that copy.  err_out:
  * device int32 [n_mat], the cooperative-wait flag of each factorisation.  No
@@ -295,10 +295,24 @@ typedef struct {
 size_t mba_lu_solve_batched_scratch_bytes(int32_t n_jobs);
 int mba_lu_solve_batched(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs, void *scratch,
                          size_t scratch_bytes, mba_stream_t stream);
+/* The same solves (complex64) in ONE persistent launch (all workgroups
+ * resident, checked against the device's occupancy before the launch): the
+ * 64-row blocks of all jobs are eliminated in block steps (one phase per step
+ * of the largest job, forward then backward, every block row and group of 64
+ * right-hand sides of every job a task of that step), phases separated by
+ * grid barriers.  Same jobs and results as mba_lu_solve_batched; scratch:
+ * mba_lu_solve_batched_persistent_scratch_bytes(n_jobs, largest n).
+ * complex128 runs the two-launch path.  Replaces the per-mesh
+ * torch.linalg.solve of src/mesh_BEM/mesh_BEM_solver_3d.py:174 for many
+ * right-hand sides; no single reference call. */
+size_t mba_lu_solve_batched_persistent_scratch_bytes(int32_t n_jobs, int64_t n_max);
+int mba_lu_solve_batched_persistent(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs, void *scratch,
+                                    size_t scratch_bytes, mba_stream_t stream);
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
 int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch, size_t scratch_bytes,
                           int32_t *err_out, mba_stream_t stream);
-/* The same factorisations (complex64) in ONE persistent cooperative launch:
+/* The same factorisations (complex64) in ONE persistent launch (all
+ * workgroups resident, checked against the device's occupancy before the launch):
  * resident workgroups step through a table of phases separated by grid
  * barriers (32-column panels of every matrix together, left-looking super
  * panels of 256 columns, one trailing product per super panel).  Same

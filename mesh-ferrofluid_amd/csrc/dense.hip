@@ -1146,7 +1146,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(PanelArgs<T> P)
     panel_body<T>(P, gridDim.x, blockIdx.x);
 }
 
-// The panels of several matrices in one cooperative launch: matrix i owns the
+// The panels of several matrices in one launch: matrix i owns the
 // workgroups [wg_off[i], wg_off[i+1]) and its own message slots, so the
 // per-column exchange latency is paid once for all of them.
 template <class T>
@@ -1982,6 +1982,25 @@ int device_cus()
     return cus;
 }
 
+// Launch of a kernel whose workgroups wait on each other (panel exchanges,
+// grid barriers): every workgroup must be resident at once.  The grid is
+// checked against what the device holds at once (occupancy per CU x CUs, the
+// same check a cooperative launch makes) and launched as an ordinary kernel.
+// None of these kernels uses the runtime's grid synchronisation, and an
+// ordinary launch keeps the runtime's cooperative queue (and what a profiler
+// hooks onto it at exit) out of the process.  Workgroups of a grid that fits
+// are all dispatched as the CUs free up from earlier work, so the waits end;
+// each wait is bounded in any case.
+int launch_resident(const void* fn, unsigned grid, unsigned block, void** args, size_t shm, hipStream_t st)
+{
+    const int cus = device_cus();
+    int occ = 0;
+    if (cus <= 0 || hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, (int)block, shm) != hipSuccess)
+        return MBA_ERR_LAUNCH;
+    if ((int64_t)grid > (int64_t)occ * cus) return MBA_ERR_SIZE;
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), args, shm, st) == hipSuccess ? MBA_OK : MBA_ERR_LAUNCH;
+}
+
 // Optional per-phase timing: HIP events recorded on the factorisation stream
 // around each phase of every panel, summed after the final synchronisation.
 struct PhaseTimer {
@@ -2103,10 +2122,9 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
             TP.epoch0 = (uint32_t)((k / TP_NB + 1) * 16);
             void* targs[] = {&TP};
             tm.mark(0);
-            if (hipLaunchCooperativeKernel((const void*)k_panel_tp, dim3((unsigned)Gt), dim3(TP_THREADS), targs,
-                                           (unsigned)((TP_RMAX + TP_NB) * TP_PITCH * sizeof(cpx<float>)),
-                                           st) != hipSuccess)
-                return MBA_ERR_LAUNCH;
+            const int lr = launch_resident((const void*)k_panel_tp, (unsigned)Gt, TP_THREADS, targs,
+                                           (TP_RMAX + TP_NB) * TP_PITCH * sizeof(cpx<float>), st);
+            if (lr != MBA_OK) return lr;
             tm.mark(0);
         } else {
             // >= 32 rows per workgroup: fewer candidates to gather and less
@@ -2134,9 +2152,8 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
             void* args[] = {&P};
             size_t shm = (size_t)R * nb * esz;
             tm.mark(0);
-            if (hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(G), dim3(PANEL_THREADS), args,
-                                           (unsigned)shm, st) != hipSuccess)
-                return MBA_ERR_LAUNCH;
+            const int lr = launch_resident((const void*)k_panel<T>, (unsigned)G, PANEL_THREADS, args, shm, st);
+            if (lr != MBA_OK) return lr;
             tm.mark(0);
         }
         const int64_t right = swap_hi - (k + nb);
@@ -3559,7 +3576,7 @@ __global__ void k_c128_to_c64(const cpx<double>* __restrict__ s, int64_t m, int6
 
 // ============================================================== batched LU
 // Several matrices factored in lock step: the same super-panel schedule as
-// lu_factor, with each panel phase's panels of all matrices in cooperative
+// lu_factor, with each panel phase's panels of all matrices in shared
 // launches of k_panel_batched (as many matrices per launch as fit one
 // workgroup per CU); row interchanges, triangular solves and GEMMs are
 // per-matrix launches.  The whole panel schedule (arguments and workgroup
@@ -3850,9 +3867,9 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
             const int* wo = d_woff + L.woff;
             int nm = (int)L.count;
             void* kargs[] = {(void*)&pa, (void*)&wo, (void*)&nm};
-            if (hipLaunchCooperativeKernel((const void*)k_panel_batched<T>, dim3(L.wg), dim3(PANEL_THREADS), kargs,
-                                           (unsigned)L.shm, st) != hipSuccess)
-                return MBA_ERR_LAUNCH;
+            const int lr = launch_resident((const void*)k_panel_batched<T>, (unsigned)L.wg, PANEL_THREADS, kargs,
+                                           L.shm, st);
+            if (lr != MBA_OK) return lr;
         }
         return MBA_OK;
     };
@@ -3899,7 +3916,8 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
 }
 
 // ============================================================== persistent batched LU
-// Every factorisation of a batch (complex64) in ONE cooperative launch: the
+// Every factorisation of a batch (complex64) in ONE launch of resident
+// workgroups (launch_resident): the
 // workgroups stay resident and step through a host-built table of phases,
 // each phase a set of independent workgroup tasks over all matrices, phases
 // separated by a grid barrier (release / acquire at agent scope, so the
@@ -3919,7 +3937,35 @@ constexpr int PB_NB = 32, PB_P = 8, PB_THREADS = 256;
 constexpr int PB_DST = 128;  // offset of the slot destinations in the plan words (2 nb <= 64 pairs before it)
 constexpr int PB_WG_LDS = 80 * 1024;  // per workgroup: two workgroups share a CU
 
-enum PbKind : int { PB_INIT = 0, PB_GEMM = 1, PB_PANEL = 2, PB_LASWP = 3, PB_TRSM = 4, PB_FINAL = 5 };
+enum PbKind : int {
+    PB_INIT = 0,
+    PB_GEMM = 1,
+    PB_PANEL = 2,
+    PB_LASWP = 3,
+    PB_TRSM = 4,
+    PB_FINAL = 5,
+    PB_SPERM = 6,  // solves: W = P b
+    PB_SFWD = 7,   // solves: forward block step (sub = block)
+    PB_SBWD = 8,   // solves: backward block step (sub = block)
+    PB_SCOPY = 9   // solves: b = X
+};
+
+// One triangular-solve job of the persistent solves (complex64): the factor
+// and its prepared solve tables (row permutation, inverted 64 x 64 diagonal
+// blocks of L and U), right-hand sides b [n, ldb] (overwritten with the
+// solution) and a work array [n, nrhs].
+struct PbSolveJob {
+    const cpx<float>* lu;
+    int64_t n, lda;
+    const int32_t* perm;
+    const cpx<float>* linv;
+    const cpx<float>* uinv;
+    cpx<float>* b;
+    int64_t ldb, nrhs;
+    cpx<float>* work;
+    int nblk, ngrp;  // 64-row blocks, groups of PB_SCW right-hand sides
+};
+constexpr int PB_SCW = 64;  // right-hand sides per solve task
 
 struct PbPhase {
     int kind;
@@ -3959,6 +4005,9 @@ struct PbTables {
     const int* trsm_off;
     const GemmJobDev* gemms;
     const int* gemm_off;
+    const PbSolveJob* sjobs;  // solves (null for factorisation-only launches)
+    const int* s_off;
+    cpx<float>* xs;           // per workgroup: one 64 x PB_SCW block of x
     unsigned* barrier;  // barrier words (PB_BAR_WORDS, zero at launch)
     int* gerr;          // barrier timeout flag, zero at launch
     uint64_t* tstamp;   // optional [3 nphase + 1]: wall clock (100 MHz) of workgroup 0 at each phase
@@ -4241,6 +4290,68 @@ __device__ __noinline__ void pb_phase_init(const PbTables& T, int b, int nwg)
     }
 }
 
+// ---- persistent solves: tasks (job, 64-row block i, group g of PB_SCW
+// right-hand sides).  Forward step s (L unit lower, running right-hand sides
+// W = work, output X = b): every task recomputes x_s = inv(L_ss) W_s; the task
+// i = s stores it as X_s, the tasks i > s subtract L_is x_s from W_i.  W_s is
+// only read in step s and each W_i written by one task, so no task of a step
+// conflicts with another.  Backward step s (U, running Y = b, output X =
+// work): the same with inv(U_ss) and the tasks i < s; a last phase copies the
+// solution from work into b.
+__device__ __forceinline__ void pb_solve_task(const PbSolveJob& J, int u, int step, bool fwd, cpx<float>* xs,
+                                              float* lds)
+{
+    const int i = (fwd ? step : 0) + u / J.ngrp, g = u % J.ngrp;
+    const int64_t r_s = (int64_t)step * TS_BS, r_i = (int64_t)i * TS_BS;
+    const int64_t bs_s = (J.n - r_s) < TS_BS ? (J.n - r_s) : TS_BS, bs_i = (J.n - r_i) < TS_BS ? (J.n - r_i) : TS_BS;
+    const int64_t c0 = (int64_t)g * PB_SCW, cw = (J.nrhs - c0) < PB_SCW ? (J.nrhs - c0) : PB_SCW;
+    const cpx<float>* inv = (fwd ? J.linv : J.uinv) + (int64_t)step * TS_BS * TS_BS;
+    // running right-hand sides and the output of this pass
+    cpx<float>* Wr = fwd ? J.work : J.b;
+    const int64_t ldw = fwd ? J.nrhs : J.ldb;
+    cpx<float>* Xo = fwd ? J.b : J.work;
+    const int64_t ldx = fwd ? J.ldb : J.nrhs;
+    if (i == step) {
+        cgemm3m_tile(bs_s, cw, bs_s, 1.f, inv, TS_BS, Wr + r_s * ldw + c0, ldw, 0.f, Xo + r_s * ldx + c0, ldx, 0, 0,
+                     lds);
+        return;
+    }
+    cgemm3m_tile(bs_s, cw, bs_s, 1.f, inv, TS_BS, Wr + r_s * ldw + c0, ldw, 0.f, xs, PB_SCW, 0, 0, lds);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1: x_s was just written through it
+    cgemm3m_tile(bs_i, cw, bs_s, -1.f, J.lu + r_i * J.lda + r_s, J.lda, xs, PB_SCW, 1.f, Wr + r_i * ldw + c0, ldw, 0,
+                 0, lds);
+}
+
+__device__ __noinline__ void pb_phase_solve(const PbTables& T, const PbPhase& ph, int b, int nwg, float* lds)
+{
+    const int* off = T.s_off + ph.off0;
+    cpx<float>* xs = T.xs + (int64_t)b * TS_BS * PB_SCW;
+    for (int vb = b; vb < ph.nvb; vb += nwg) {
+        const int j = job_of(off, ph.njob, vb);
+        const PbSolveJob& J = T.sjobs[ph.job0 + j];
+        const int u = vb - off[j];
+        if (ph.kind == PB_SFWD || ph.kind == PB_SBWD) {
+            pb_solve_task(J, u, ph.sub, ph.kind == PB_SFWD, xs, lds);
+        } else {
+            // PB_SPERM: work rows of block i <- b rows perm[.]; PB_SCOPY: b <- work
+            const int i = u / J.ngrp, g = u % J.ngrp;
+            const int64_t r0 = (int64_t)i * TS_BS, c0 = (int64_t)g * PB_SCW;
+            const int64_t rows = (J.n - r0) < TS_BS ? (J.n - r0) : TS_BS;
+            const int64_t cw = (J.nrhs - c0) < PB_SCW ? (J.nrhs - c0) : PB_SCW;
+            for (int64_t e = threadIdx.x; e < rows * cw; e += PB_THREADS) {
+                const int64_t r = r0 + e / cw, c = c0 + e % cw;
+                if (ph.kind == PB_SPERM)
+                    J.work[r * J.nrhs + c] = J.b[(int64_t)J.perm[r] * J.ldb + c];
+                else
+                    J.b[r * J.ldb + c] = J.work[r * J.nrhs + c];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(PB_THREADS, 2) void k_lu_batched_persistent(PbTables T)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -4266,6 +4377,8 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_lu_batched_persistent(PbTable
             pb_phase_trsm(T, ph, b, nwg, (cpx<float>*)smem_raw);
         } else if (kind == PB_GEMM) {
             pb_phase_gemm(T, ph, b, nwg, (float*)smem_raw);
+        } else if (kind >= PB_SPERM && kind <= PB_SCOPY) {
+            pb_phase_solve(T, ph, b, nwg, (float*)smem_raw);
         } else if (kind == PB_FINAL && b == 0) {
             for (int i = tid; i < T.nmat; i += PB_THREADS) {
                 const int e = __hip_atomic_load(T.mats[i].err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
@@ -4546,9 +4659,9 @@ size_t pb_dyn_lds()
 template <class V>
 size_t pb_bytes(const std::vector<V>& v) { return al256(v.size() * sizeof(V)); }
 
-int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scratch, size_t scratch_bytes,
-                                 int32_t* err_out, hipStream_t st, double* kind_ms = nullptr,
-                                 int64_t* kind_phases = nullptr)
+// Dynamic LDS and resident workgroup count of the persistent launch (the
+// same for every table it runs).
+int pb_launch_shape(size_t* dyn_out, int* nwg_out)
 {
     const int cus = device_cus();
     if (cus <= 0) return MBA_ERR_LAUNCH;
@@ -4562,11 +4675,24 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
         dyn < trsm_lds_bytes<float>(PB_NB))
         return MBA_ERR_LAUNCH;
     int per_cu = 0;
-    int rc = pb_resident(cus, dyn, &per_cu);
+    const int rc = pb_resident(cus, dyn, &per_cu);
     if (rc != MBA_OK) return rc;
     // a multiple of the XCD count, so workgroup b stays on XCD b % 8 in every task loop
     const int nwg = (cus * per_cu) / NUM_XCD * NUM_XCD;
     if (nwg < NUM_XCD) return MBA_ERR_LAUNCH;
+    *dyn_out = dyn;
+    *nwg_out = nwg;
+    return MBA_OK;
+}
+
+int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scratch, size_t scratch_bytes,
+                                 int32_t* err_out, hipStream_t st, double* kind_ms = nullptr,
+                                 int64_t* kind_phases = nullptr)
+{
+    size_t dyn = 0;
+    int nwg = 0;
+    int rc = pb_launch_shape(&dyn, &nwg);
+    if (rc != MBA_OK) return rc;
     const int rmax_lds = (int)(dyn / (PB_NB * sizeof(cpx<float>)));
     std::vector<PbMatIn> in(nmat);
     for (int i = 0; i < nmat; ++i) {
@@ -4648,10 +4774,13 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
     T.barrier = (unsigned*)base;
     T.gerr = (int*)(base + o_gerr);
     T.tstamp = kind_ms ? (uint64_t*)(base + o_ts) : nullptr;
+    T.sjobs = nullptr;
+    T.s_off = nullptr;
+    T.xs = nullptr;
     void* kargs[] = {&T};
-    if (hipLaunchCooperativeKernel((const void*)k_lu_batched_persistent, dim3((unsigned)nwg), dim3(PB_THREADS), kargs,
-                                   (unsigned)dyn, st) != hipSuccess)
-        return MBA_ERR_LAUNCH;
+    if ((rc = launch_resident((const void*)k_lu_batched_persistent, (unsigned)nwg, PB_THREADS, kargs, dyn, st)) !=
+        MBA_OK)
+        return rc;
     if (kind_ms) {
         // per phase kind: summed work time (phase start to the last
         // workgroup's end of work), summed barrier time (from there to the
@@ -4680,6 +4809,103 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
             if (kind_phases) kind_phases[kd] += 1;
         }
     }
+    return MBA_OK;
+}
+
+// Solves of prepared complex64 factorisations, all jobs in one persistent
+// launch: one phase for the permuted right-hand sides, one per 64-row block of
+// the largest job forward and backward, one copy phase.
+size_t pb_solve_scratch(int32_t n_jobs, int64_t n_max, int nwg)
+{
+    const size_t nb = (size_t)((n_max + TS_BS - 1) / TS_BS);
+    const size_t phases = 2 * nb + 2;
+    return al256(PB_BAR_WORDS * sizeof(unsigned)) + 256 + al256(phases * sizeof(PbPhase)) +
+           al256((size_t)n_jobs * sizeof(PbSolveJob)) + al256(phases * ((size_t)n_jobs + 1) * sizeof(int)) +
+           (size_t)nwg * TS_BS * PB_SCW * sizeof(cpx<float>);
+}
+
+int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* scratch, size_t scratch_bytes,
+                                hipStream_t st)
+{
+    size_t dyn = 0;
+    int nwg = 0;
+    int rc = pb_launch_shape(&dyn, &nwg);
+    if (rc != MBA_OK) return rc;
+    std::vector<PbSolveJob> sj;
+    int nblk_max = 0;
+    for (int q = 0; q < njob; ++q) {
+        const mba_lu_solve_job& J = jobs[q];
+        if (J.n <= 0 || J.nrhs <= 0) continue;
+        PbSolveJob d;
+        d.lu = (const cpx<float>*)J.lu;
+        d.n = J.n;
+        d.lda = J.lda;
+        d.nblk = (int)((J.n + TS_BS - 1) / TS_BS);
+        d.perm = (const int32_t*)J.aux;
+        d.linv = (const cpx<float>*)((const char*)J.aux + al256((size_t)J.n * 4));
+        d.uinv = d.linv + (size_t)d.nblk * TS_BS * TS_BS;
+        // the inverse tables are laid out as lu_prepare writes them
+        if ((const char*)d.uinv - (const char*)d.linv != (ptrdiff_t)al256((size_t)d.nblk * TS_BS * TS_BS * 8))
+            return MBA_ERR_ARG;
+        d.b = (cpx<float>*)J.b;
+        d.ldb = J.ldb;
+        d.nrhs = J.nrhs;
+        d.work = (cpx<float>*)J.work;
+        d.ngrp = (int)((J.nrhs + PB_SCW - 1) / PB_SCW);
+        nblk_max = d.nblk > nblk_max ? d.nblk : nblk_max;
+        sj.push_back(d);
+    }
+    if (sj.empty()) return MBA_OK;
+    const int nj = (int)sj.size();
+    std::vector<PbPhase> phases;
+    std::vector<int> off;
+    auto add_phase = [&](int kind, int sub, auto tasks_of) -> int {
+        PbPhase ph{kind, nj, 0, sub, 0, (int64_t)off.size()};
+        off.push_back(0);
+        for (const PbSolveJob& d : sj) {
+            const int64_t t = (int64_t)ph.nvb + tasks_of(d);
+            if (t > 0x7fffffff) return MBA_ERR_SIZE;
+            ph.nvb = (int)t;
+            off.push_back(ph.nvb);
+        }
+        phases.push_back(ph);
+        return MBA_OK;
+    };
+    rc = add_phase(PB_SPERM, 0, [](const PbSolveJob& d) { return (int64_t)d.nblk * d.ngrp; });
+    for (int s2 = 0; s2 < nblk_max && rc == MBA_OK; ++s2)
+        rc = add_phase(PB_SFWD, s2, [s2](const PbSolveJob& d) { return s2 < d.nblk ? (int64_t)(d.nblk - s2) * d.ngrp : 0; });
+    for (int s2 = nblk_max - 1; s2 >= 0 && rc == MBA_OK; --s2)
+        rc = add_phase(PB_SBWD, s2, [s2](const PbSolveJob& d) { return s2 < d.nblk ? (int64_t)(s2 + 1) * d.ngrp : 0; });
+    if (rc == MBA_OK) rc = add_phase(PB_SCOPY, 0, [](const PbSolveJob& d) { return (int64_t)d.nblk * d.ngrp; });
+    if (rc != MBA_OK) return rc;
+    const size_t o_gerr = al256(PB_BAR_WORDS * sizeof(unsigned));
+    const size_t o_ph = o_gerr + 256;
+    const size_t o_sj = o_ph + pb_bytes(phases);
+    const size_t o_off = o_sj + pb_bytes(sj);
+    const size_t o_xs = o_off + pb_bytes(off);
+    const size_t need = o_xs + (size_t)nwg * TS_BS * PB_SCW * sizeof(cpx<float>);
+    if (need > scratch_bytes) return MBA_ERR_ALLOC;
+    std::vector<unsigned char> blob(o_xs, 0);
+    std::memcpy(blob.data() + o_ph, phases.data(), phases.size() * sizeof(PbPhase));
+    std::memcpy(blob.data() + o_sj, sj.data(), sj.size() * sizeof(PbSolveJob));
+    std::memcpy(blob.data() + o_off, off.data(), off.size() * sizeof(int));
+    char* base = (char*)scratch;
+    if (hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, st) != hipSuccess) return MBA_ERR_LAUNCH;
+    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;  // the host table is released on return
+    PbTables T{};
+    T.phases = (const PbPhase*)(base + o_ph);
+    T.nphase = (int)phases.size();
+    T.nmat = 0;
+    T.sjobs = (const PbSolveJob*)(base + o_sj);
+    T.s_off = (const int*)(base + o_off);
+    T.xs = (cpx<float>*)(base + o_xs);
+    T.barrier = (unsigned*)base;
+    T.gerr = (int*)(base + o_gerr);
+    T.tstamp = nullptr;
+    void* kargs[] = {&T};
+    if ((rc = launch_resident((const void*)k_lu_batched_persistent, (unsigned)nwg, PB_THREADS, kargs, dyn, st)) !=
+        MBA_OK)
+        return rc;
     return MBA_OK;
 }
 
@@ -4864,6 +5090,29 @@ int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix* mats, int32_
     return mba_guard([&] {
         if (prec == MBA_F32) return lu_factor_batched_persistent(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
         return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+    });
+}
+
+size_t mba_lu_solve_batched_persistent_scratch_bytes(int32_t n_jobs, int64_t n_max)
+{
+    if (n_jobs <= 0 || n_max <= 0) return 0;
+    const size_t a = pb_solve_scratch(n_jobs, n_max, 2 * MAX_G), b = mba_lu_solve_batched_scratch_bytes(n_jobs);
+    return a > b ? a : b;
+}
+
+int mba_lu_solve_batched_persistent(int prec, const mba_lu_solve_job* jobs, int32_t n_jobs, void* scratch,
+                                    size_t scratch_bytes, mba_stream_t stream)
+{
+    if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    for (int q = 0; q < n_jobs; ++q) {
+        const mba_lu_solve_job& J = jobs[q];
+        if (!J.lu || !J.aux || !J.b || !J.work || J.n < 0 || J.nrhs < 0) return MBA_ERR_ARG;
+        if (J.n > 0 && (J.lda < J.n || J.ldb < J.nrhs)) return MBA_ERR_ARG;
+    }
+    return mba_guard([&] {
+        if (prec == MBA_F32) return lu_solve_batched_persistent(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+        return lu_solve_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
     });
 }
 

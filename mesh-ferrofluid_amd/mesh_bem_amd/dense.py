@@ -308,10 +308,16 @@ def gemm_grouped(products, alpha: float = 1.0, beta: float = 0.0):
     _lib.check(st, "mba_gemm_grouped")
 
 
-def lu_solve_batched(lus, Bs):
+def lu_solve_batched(lus, Bs, persistent: bool = False):
     """X_i = A_i^{-1} B_i for prepared factorisations lus[i] (one dtype) and
-    right-hand sides Bs[i] [n_i, nrhs_i]; two launches for all of them.
-    Returns new tensors (the inputs are not modified)."""
+    right-hand sides Bs[i] [n_i, nrhs_i].  Default: two launches, one
+    workgroup per job and 16 right-hand sides walking its 64-row blocks.
+    ``persistent`` (complex64): all jobs in one persistent launch, eliminated
+    in 64-row block steps with every block row and group of 64 right-hand
+    sides of every job a task (``mba_lu_solve_batched_persistent``); measured
+    slower on the 64-mesh batch (a step costs a grid barrier plus two
+    dependent 64 x 64 products, ~57 steps per triangle).  Returns new tensors
+    (the inputs are not modified)."""
     lib = _lib.load()
     n_job = len(lus)
     if n_job == 0:
@@ -328,10 +334,16 @@ def lu_solve_batched(lus, Bs):
         jobs[i].nrhs, jobs[i].work = X.shape[1], W.data_ptr()
         Xs.append(X)
         keep.append(W)
-    sb = lib.mba_lu_solve_batched_scratch_bytes(n_job)
-    scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)
-    st = lib.mba_lu_solve_batched(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.stream())
-    _lib.check(st, "mba_lu_solve_batched")
+    if persistent:
+        sb = lib.mba_lu_solve_batched_persistent_scratch_bytes(n_job, max(lu.n for lu in lus))
+        scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)
+        st = lib.mba_lu_solve_batched_persistent(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.stream())
+        _lib.check(st, "mba_lu_solve_batched_persistent")
+    else:
+        sb = lib.mba_lu_solve_batched_scratch_bytes(n_job)
+        scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)
+        st = lib.mba_lu_solve_batched(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.stream())
+        _lib.check(st, "mba_lu_solve_batched")
     return Xs
 
 

@@ -73,12 +73,15 @@ def test_lu_factor_batched_matches_single(dev, persistent):
         assert float((x - x1).norm() / x1.norm()) < 1e-5
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_lu_solve_batched_matches_single(dev, prec):
-    """Two-launch triangular solves of several factorisations with ragged
+@pytest.mark.parametrize("prec,persistent", [("f32", True), ("f32", False), ("f64", True)])
+def test_lu_solve_batched_matches_single(dev, prec, persistent):
+    """Batched triangular solves (one persistent launch in 64-row block steps,
+    or two launches walking the blocks) of several factorisations with ragged
     sizes (not multiples of the 64-row block) and right-hand-side counts (not
-    multiples of the 16-column group) agree with the per-matrix solve."""
+    multiples of the 16- or 64-column groups) agree with the per-matrix solve."""
     from mesh_bem_amd.dense import LUFactor, lu_solve_batched
+
+    print(f"batched solves {prec} persistent={persistent}", flush=True)
 
     cdt = torch.complex64 if prec == "f32" else torch.complex128
     bar = 1e-5 if prec == "f32" else 1e-12
@@ -93,7 +96,7 @@ def test_lu_solve_batched_matches_single(dev, prec):
         Bs.append(torch.tensor(rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r)), dtype=cdt,
                                device=dev))
     B0 = [B.clone() for B in Bs]
-    Xs = lu_solve_batched(lus, Bs)
+    Xs = lu_solve_batched(lus, Bs, persistent=persistent)
     torch.cuda.synchronize()
     for A, lu, B, Bk, X in zip(mats, lus, Bs, B0, Xs):
         assert torch.equal(B, Bk)
