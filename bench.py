@@ -1136,6 +1136,32 @@ def main():
         torch.cuda.synchronize()
         t32 = time.perf_counter() - te
         out["assembly_fp32_Mpairs_s"] = round(1e-6 * nF * nF / t32, 1)
+        # FP32 assembly against the vector-ALU issue rates measured on this
+        # device in the same run (FP32 FMA, v_sin / v_cos / v_rsq): the 7 x 7
+        # point-pair body of k_ns_zf<float, 7> issues 1689 VALU instructions
+        # per face pair on gfx950 (hipcc -O3 assembly of the unrolled loop), 147
+        # of them v_rsq / v_sin / v_cos; the issue bound is the time the full-
+        # rate ones take at the FMA rate plus the transcendentals at theirs.
+        import ctypes
+
+        rates = (ctypes.c_double * 4)()
+        if _lib.load().mba_measure_valu_rates(rates, 0, _lib.stream()) == 0:
+            n_valu, n_trans = 1689, 147
+            r_fma, r_trans = rates[0], (rates[1] + rates[2] + rates[3]) / 3.0
+            bound_s = nF * nF * ((n_valu - n_trans) / r_fma + n_trans / r_trans)
+            out["assembly_fp32_roofline"] = {
+                "kernel": "k_ns_zf<float, 7> (+ k_ns_near, k_gather_rwg<float>)",
+                "bound": "valu issue",
+                "valu_instructions_per_face_pair": n_valu,
+                "transcendentals_per_face_pair": n_trans,
+                "measured_fma_per_s": float(f"{rates[0]:.4e}"),
+                "measured_sin_cos_rsq_per_s": [float(f"{rates[q]:.4e}") for q in (1, 2, 3)],
+                "issue_bound_ms": round(bound_s * 1e3, 2),
+                "assembly_ms": round(t32 * 1e3, 2),
+                "frac": round(bound_s / t32, 3),
+                "algorithmic_tflops": round(nF * nF * 49 * 29 / t32 / 1e12, 2),
+                "algorithmic_flops_definition": "49 point pairs x 29 flops per face pair",
+            }
         out["assembly_fp64_Mpairs_s"] = round(1e-6 * nF * nF / (tb - ta), 1)
         Zerr = (Z32a.to(torch.complex128) - Z64).abs().max().item() / Z64.abs().max().item()
         out["assembly_fp32_max_abs_err_rel_max"] = float(f"{Zerr:.3e}")

@@ -26,8 +26,9 @@
  *   - Memory: the caller passes every output and workspace, except that
  *     mba_mom_zmat_faceedge_ref, mba_mom_rhs_faceedge_ref and
  *     mba_mom_rhs_faceedge_ns take their per-face records from the stream-
- *     ordered allocator (hipMallocAsync / hipFreeAsync on `stream`), and
- *     mba_mom_zmat_faceedge_ref_noalloc uses a fixed device array.
+ *     ordered allocator (hipMallocAsync / hipFreeAsync on `stream`),
+ *     mba_mom_zmat_faceedge_ref_noalloc uses a fixed device array, and the
+ *     device measurement mba_measure_valu_rates allocates its own buffer.
  *   - State: calls on different streams are independent except through the
  *     buffers the caller passes and these process-wide defaults, each an
  *     atomic value set by its setter: the FP32 GEMM tile variant
@@ -738,6 +739,13 @@ size_t mba_nearest_transfer_workspace_bytes(int prec, int64_t n_points, int64_t 
 int mba_nearest_transfer(int prec, const void *points, int64_t n_points, const void *verts, int64_t n_verts,
                          const void *vel, int32_t batch, int32_t *nearest, void *new_vel, void *workspace,
                          size_t workspace_bytes, mba_stream_t stream);
+
+/* Vector-ALU issue rates of the current device, measured (the call
+ * synchronises): out[4] = operations per second of FP32 FMA (not packed),
+ * v_sin_f32, v_cos_f32 and v_rsq_f32 -- the rates the FP32 assembly kernels
+ * are judged against.  blocks_per_cu: workgroups of 256 threads per CU (0:
+ * 8).  No reference counterpart (a measurement of the device). */
+int mba_measure_valu_rates(double *out, int blocks_per_cu, mba_stream_t stream);
 
 #ifdef __cplusplus
 }

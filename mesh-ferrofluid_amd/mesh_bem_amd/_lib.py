@@ -110,6 +110,7 @@ SIGNATURES = {
     "mba_lu_factor_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _i32p, _vp]),
     "mba_lu_batched_persistent_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),
     "mba_lu_factor_batched_persistent": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _i32p, _vp]),
+    "mba_measure_valu_rates": (_c.c_int, [_vp, _c.c_int, _vp]),
     "mba_lu_factor_batched_persistent_profiled": (_c.c_int, [_vp, _i32, _vp, _sz, _i32p, _vp, _vp, _vp]),
     "mba_lu_factor_profiled": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp, _vp,
                                           _vp, _vp]),

@@ -14,6 +14,9 @@
       BLAS) FP16 / BF16 at N^3, and this library's split-FP16 complex product
       (mba_gemm_split_f16) at M x M x k for k = 512 and 1024, each timed with
       device events after a warm-up.
+  python -m mesh_bem_amd.profiling valu-rate
+      what the vector ALU issues on this device (mba_measure_valu_rates):
+      FP32 FMA (not packed) and the hardware v_sin / v_cos / v_rsq per second.
 """
 
 import argparse
@@ -161,6 +164,22 @@ def mfma_rate(n: int = 16384, m: int = 29696, reps: int = 5):
         del A, B, C
 
 
+def valu_rate():
+    import ctypes
+    import os
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from mesh_bem_amd import _lib
+
+    lib = _lib.load()
+    out = (ctypes.c_double * 4)()
+    _lib.check(lib.mba_measure_valu_rates(out, 0, _lib.stream()), "mba_measure_valu_rates")
+    names = ("FP32 FMA (not packed)", "v_sin_f32", "v_cos_f32", "v_rsq_f32")
+    for nm, r in zip(names, out):
+        print(f"{nm}: {r:.4e} per second", flush=True)
+    print(f"FP32 FMA = {2 * out[0] / 1e12:.1f} TFLOP/s", flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -178,6 +197,7 @@ def main(argv=None):
     mr = sub.add_parser("mfma-rate")
     mr.add_argument("--n", type=int, default=16384)
     mr.add_argument("--m", type=int, default=29696)
+    sub.add_parser("valu-rate")
     a = ap.parse_args(argv)
     if a.what == "kernels":
         print_kernels(a.db, a.top, a.cmd)
@@ -185,6 +205,8 @@ def main(argv=None):
         print_pmc(a.db, a.kernel, a.cmd)
     elif a.what == "mfma-rate":
         mfma_rate(a.n, a.m)
+    elif a.what == "valu-rate":
+        valu_rate()
     else:
         copy_run(a.mib, a.reps)
 
