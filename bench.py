@@ -735,7 +735,7 @@ def run_c3(args, dev):
     import torch
 
     from mesh_bem_amd import _lib
-    from mesh_bem_amd.batched import BatchedMoM, GemmTuner, star_batch
+    from mesh_bem_amd.batched import BatchedMoM, BatchLuTuner, GemmTuner, star_batch
 
     n_mesh, n_dir = (64, 128) if args.workload == "c3" else (8, 16)
     geoms, ks, dirs, pols = star_batch(n_mesh, seed0=0, n_dirs=n_dir)
@@ -744,6 +744,7 @@ def run_c3(args, dev):
     tuner = GemmTuner()
     for E in sorted(set(Es)):
         tuner.tune(E)
+    lu_tuner = BatchLuTuner() if args.batched_lu == "persistent" else None
     log(f"workload {args.workload}: {n_mesh} meshes, E in [{min(Es)}, {max(Es)}], {n_dir} waves; tuned "
         f"{tuner.best}")
 
@@ -754,7 +755,7 @@ def run_c3(args, dev):
         Vs = bm.rhs(dirs, pols)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        res = bm.solve(Zs, Vs, tuner=tuner, tol=1e-8, persistent=args.batched_lu == "persistent")
+        res = bm.solve(Zs, Vs, tuner=tuner, tol=1e-8, persistent=args.batched_lu == "persistent", lu_tuner=lu_tuner)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         return res, t1 - t0, t2 - t1, Zs, Vs
@@ -803,6 +804,12 @@ def run_c3(args, dev):
         "relerr_vs_fp64_max": float(f"{max(errs):.3e}"),
         "tuned_gemm_variant_per_bucket": {str(k): v for k, v in tuner.best.items()},
     }
+    if lu_tuner is not None and lu_tuner.best:
+        kk = next(iter(lu_tuner.best))
+        out["tuned_batched_lu"] = {"super_panels": lu_tuner.best[kk][0],
+                                   "trailing": ["fp32_3m", "split_fp16"][lu_tuner.best[kk][1]],
+                                   "candidates_ms": {f"{c[0]}x32/{['3m', 'split'][c[1]]}": round(t, 2)
+                                                     for c, t in zip(lu_tuner.CANDIDATES, lu_tuner.timings.get(kk, []))}}
     print(json.dumps(out), flush=True)
 
 

@@ -325,6 +325,19 @@ int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, vo
 size_t mba_lu_batched_persistent_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
 int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch,
                                      size_t scratch_bytes, int32_t *err_out, mba_stream_t stream);
+/* Schedule of the persistent batched LU, per call: super_panels = 32-column
+ * panels per super panel, 1..8 (0: 8; the trailing product's k is 32 x
+ * super_panels); trailing = 0 for the FP32-MFMA 3M trailing product, 1 for the
+ * FP32-accurate split-FP16 product (operand records in each matrix's LU
+ * workspace).  No reference counterpart (what the batch autotuner picks). */
+typedef struct {
+    int32_t super_panels;
+    int32_t trailing;
+} mba_batched_lu_options;
+/* mba_lu_factor_batched_persistent with explicit options (complex64 only). */
+int mba_lu_factor_batched_persistent_opts(int prec, const mba_lu_matrix *mats, int32_t n_mat,
+                                          const mba_batched_lu_options *opts, void *scratch, size_t scratch_bytes,
+                                          int32_t *err_out, mba_stream_t stream);
 /* mba_lu_factor_batched_persistent (complex64) with phase timing from the
  * launch's own wall clock (the call synchronises): per phase kind (0 init,
  * 1 products, 2 panels, 3 row interchanges, 4 triangular solves, 5 final)
@@ -334,7 +347,8 @@ int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix *mats, int32_
  * into panel-column updates (i = 0), U-row updates (1) and trailing products
  * (2), kind_ms[15] = 0; kind_phases[kind] = phase count.  No reference
  * counterpart. */
-int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix *mats, int32_t n_mat, void *scratch,
+int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix *mats, int32_t n_mat,
+                                              const mba_batched_lu_options *opts, void *scratch,
                                               size_t scratch_bytes, int32_t *err_out, double *kind_ms,
                                               int64_t *kind_phases, mba_stream_t stream);
 /* mba_lu_factor with per-phase timing (HIP events on the stream; the call

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gemm_h3.hpp"
+#include "gemm_h3_dev.hpp"
 #include "mba_common.hpp"
 #include "mba_guard.hpp"
 
@@ -3947,8 +3948,28 @@ enum PbKind : int {
     PB_SPERM = 6,  // solves: W = P b
     PB_SFWD = 7,   // solves: forward block step (sub = block)
     PB_SBWD = 8,   // solves: backward block step (sub = block)
-    PB_SCOPY = 9   // solves: b = X
+    PB_SCOPY = 9,  // solves: b = X
+    PB_SPLIT = 10, // trailing product, split FP16: operand splits
+    PB_H3 = 11     // trailing product, split FP16: product tiles
 };
+
+// Split-FP16 trailing product of one matrix: C[c3:, c3:] -= A B with A =
+// L[c3:, k:c3], B = U[k:c3, c3:], operand records in the matrix's split
+// workspace.  Split tasks: Mpad / 4 row tasks, then Npad / SC column tasks.
+struct PbSplitJob {
+    const cpx<float>* A;
+    const cpx<float>* B;
+    cpx<float>* C;
+    int64_t ld, M, N;
+    int K, Kpad, nrec;
+    int64_t Mpad, Npad;
+    _Float16* Ap;
+    _Float16* Bp;
+    int32_t* ea;
+    int32_t* eb;
+};
+constexpr int PB_H3_KM = 256;  // largest k of the split trailing product (32 x 8 panels)
+constexpr int PB_H3_BM = 128;  // product tile rows (4 waves)
 
 // One triangular-solve job of the persistent solves (complex64): the factor
 // and its prepared solve tables (row permutation, inverted 64 x 64 diagonal
@@ -4005,6 +4026,8 @@ struct PbTables {
     const int* trsm_off;
     const GemmJobDev* gemms;
     const int* gemm_off;
+    const PbSplitJob* hjobs;  // split-FP16 trailing products
+    const int* h_off;
     const PbSolveJob* sjobs;  // solves (null for factorisation-only launches)
     const int* s_off;
     cpx<float>* xs;           // per workgroup: one 64 x PB_SCW block of x
@@ -4290,6 +4313,42 @@ __device__ __noinline__ void pb_phase_init(const PbTables& T, int b, int nwg)
     }
 }
 
+__device__ __noinline__ void pb_phase_split(const PbTables& T, const PbPhase& ph, int b, int nwg, void* lds)
+{
+    const int* off = T.h_off + ph.off0;
+    for (int vb = b; vb < ph.nvb; vb += nwg) {
+        const int j = job_of(off, ph.njob, vb);
+        const PbSplitJob& J = T.hjobs[ph.job0 + j];
+        const int64_t u = vb - off[j], nrt = J.Mpad / 4;
+        if (u < nrt)
+            mba_h3::dev::split_rows_task<PB_H3_KM>(J.A, J.ld, J.M, J.K, J.Kpad, J.Mpad, J.Ap, J.ea, u);
+        else
+            mba_h3::dev::split_cols_task<PB_H3_KM>(J.B, J.ld, J.K, J.Kpad, J.N, J.Npad, J.Bp, J.eb, u - nrt,
+                                                   *(mba_h3::dev::SplitColsLds<PB_H3_KM>*)lds);
+        __syncthreads();
+    }
+}
+
+__device__ __noinline__ void pb_phase_h3(const PbTables& T, const PbPhase& ph, int b, int nwg, void* lds)
+{
+    // tiles dealt so that each XCD walks a contiguous run (as pb_phase_gemm)
+    const int* off = T.h_off + ph.off0;
+    const int per = (ph.nvb + NUM_XCD - 1) / NUM_XCD;
+    const int span = per * NUM_XCD;
+    for (int vb = b; vb < span; vb += nwg) {
+        const int t = (vb % NUM_XCD) * per + vb / NUM_XCD;
+        if (t < ph.nvb) {
+            const int j = job_of(off, ph.njob, t);
+            const PbSplitJob& J = T.hjobs[ph.job0 + j];
+            const int64_t nbn = (J.N + mba_h3::dev::BN - 1) / mba_h3::dev::BN;
+            const int64_t u = t - off[j];
+            mba_h3::dev::tile_h3<PB_H3_BM, PB_THREADS>(J.M, J.N, J.nrec, J.Ap, J.ea, J.Bp, J.eb, -1.f, 1.f, J.C, J.ld,
+                                                       u / nbn, u % nbn, *(mba_h3::dev::TileLds<PB_H3_BM>*)lds);
+        }
+        __syncthreads();
+    }
+}
+
 // ---- persistent solves: tasks (job, 64-row block i, group g of PB_SCW
 // right-hand sides).  Forward step s (L unit lower, running right-hand sides
 // W = work, output X = b): every task recomputes x_s = inv(L_ss) W_s; the task
@@ -4379,6 +4438,10 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_lu_batched_persistent(PbTable
             pb_phase_gemm(T, ph, b, nwg, (float*)smem_raw);
         } else if (kind >= PB_SPERM && kind <= PB_SCOPY) {
             pb_phase_solve(T, ph, b, nwg, (float*)smem_raw);
+        } else if (kind == PB_SPLIT) {
+            pb_phase_split(T, ph, b, nwg, smem_raw);
+        } else if (kind == PB_H3) {
+            pb_phase_h3(T, ph, b, nwg, smem_raw);
         } else if (kind == PB_FINAL && b == 0) {
             for (int i = tid; i < T.nmat; i += PB_THREADS) {
                 const int e = __hip_atomic_load(T.mats[i].err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
@@ -4415,6 +4478,8 @@ struct PbSchedule {
     std::vector<int> trsm_off;
     std::vector<GemmJobDev> gemms;
     std::vector<int> gemm_off;
+    std::vector<PbSplitJob> hjobs;
+    std::vector<int> h_off;
     int rmax = 0;           // largest panel rows per workgroup
     std::vector<int> gmax;  // per matrix: largest panel workgroup count
 };
@@ -4427,7 +4492,7 @@ struct PbMatIn {
     LuWs ws;
 };
 
-inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSchedule& S)
+inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, int P, int trailing, PbSchedule& S)
 {
     const int nmat = (int)M.size();
     int64_t nmax = 0;
@@ -4465,9 +4530,9 @@ inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSche
         S.gemm_off.push_back(ph.nvb);
         return MBA_OK;
     };
-    const int64_t SP = (int64_t)PB_NB * PB_P;
+    const int64_t SP = (int64_t)PB_NB * P;
     for (int64_t k = 0; k < nmax; k += SP) {
-        for (int p = 0; p < PB_P; ++p) {
+        for (int p = 0; p < P; ++p) {
             const int64_t kp = k + (int64_t)PB_NB * p;
             if (kp >= nmax) break;
             // [A] panel columns <- the super panel's earlier panels
@@ -4617,8 +4682,61 @@ inline int pb_build(const std::vector<PbMatIn>& M, int nwg, int rmax_lds, PbSche
                 close(ph, S.trsm_off);
             }
         }
-        // [E] trailing product of the super panel, k = its width
-        {
+        // [E] trailing product of the super panel, k = its width: on the
+        // split-FP16 product (operand splits, then product tiles) or the 3M one
+        if (trailing == 1) {
+            PbPhase ps = open(PB_SPLIT, S.hjobs.size(), S.h_off);
+            ps.sub = 2;
+            for (int i = 0; i < nmat; ++i) {
+                const int64_t n = M[i].n;
+                const int64_t c3 = (k + SP) < n ? (k + SP) : n;
+                if (k >= n || c3 >= n) continue;
+                PbSplitJob J;
+                const int64_t lda = M[i].lda, Mr = n - c3, K = c3 - k;
+                J.A = M[i].a + c3 * lda + k;
+                J.B = M[i].a + k * lda + c3;
+                J.C = M[i].a + c3 * lda + c3;
+                J.ld = lda;
+                J.M = Mr;
+                J.N = Mr;
+                J.K = (int)K;
+                if (!M[i].ws.split || M[i].ws.split_bytes < mba_h3::dev::parts_bytes(Mr, Mr, K)) return MBA_ERR_ALLOC;
+                const mba_h3::dev::Parts Pt = mba_h3::dev::parts(Mr, Mr, K, M[i].ws.split);
+                J.Kpad = Pt.Kpad;
+                J.nrec = Pt.nrec;
+                J.Mpad = Pt.Mpad;
+                J.Npad = Pt.Npad;
+                J.Ap = Pt.Ap;
+                J.Bp = Pt.Bp;
+                J.ea = Pt.ea;
+                J.eb = Pt.eb;
+                S.hjobs.push_back(J);
+                const int64_t tasks = (int64_t)ps.nvb + Pt.Mpad / 4 + Pt.Npad / mba_h3::dev::SC;
+                if (tasks > 0x7fffffff) return MBA_ERR_SIZE;
+                ps.nvb = (int)tasks;
+                ps.njob += 1;
+                S.h_off.push_back(ps.nvb);
+            }
+            if (ps.njob > 0) {
+                const size_t first = (size_t)ps.job0;
+                close(ps, S.h_off);
+                // the product tiles of the same jobs
+                PbPhase pm{PB_H3, 0, 0, 2, (int64_t)first, (int64_t)S.h_off.size()};
+                S.h_off.push_back(0);
+                for (size_t q = first; q < S.hjobs.size(); ++q) {
+                    const PbSplitJob& J = S.hjobs[q];
+                    const int64_t tiles = (int64_t)pm.nvb + ((J.M + PB_H3_BM - 1) / PB_H3_BM) *
+                                                                ((J.N + mba_h3::dev::BN - 1) / mba_h3::dev::BN);
+                    if (tiles > 0x7fffffff) return MBA_ERR_SIZE;
+                    pm.nvb = (int)tiles;
+                    pm.njob += 1;
+                    S.h_off.push_back(pm.nvb);
+                }
+                S.phases.push_back(pm);
+            } else {
+                close(ps, S.h_off);
+            }
+        } else {
             PbPhase ph = open(PB_GEMM, S.gemms.size(), S.gemm_off);
             ph.sub = 2;
             for (int i = 0; i < nmat; ++i) {
@@ -4685,10 +4803,13 @@ int pb_launch_shape(size_t* dyn_out, int* nwg_out)
     return MBA_OK;
 }
 
-int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scratch, size_t scratch_bytes,
-                                 int32_t* err_out, hipStream_t st, double* kind_ms = nullptr,
-                                 int64_t* kind_phases = nullptr)
+int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, const mba_batched_lu_options* opts,
+                                 void* scratch, size_t scratch_bytes, int32_t* err_out, hipStream_t st,
+                                 double* kind_ms = nullptr, int64_t* kind_phases = nullptr)
 {
+    const int P = (opts && opts->super_panels > 0) ? opts->super_panels : PB_P;
+    const int trailing = opts ? opts->trailing : 0;
+    if (P < 1 || P > PB_P || (trailing != 0 && trailing != 1)) return MBA_ERR_ARG;
     size_t dyn = 0;
     int nwg = 0;
     int rc = pb_launch_shape(&dyn, &nwg);
@@ -4705,7 +4826,7 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
         if (mats[i].workspace_bytes < in[i].ws.total) return MBA_ERR_ALLOC;
     }
     PbSchedule S;
-    if ((rc = pb_build(in, nwg, rmax_lds, S)) != MBA_OK) return rc;
+    if ((rc = pb_build(in, nwg, rmax_lds, P, trailing, S)) != MBA_OK) return rc;
     // message words each matrix's panels use: meta and cval (two parities of
     // MAX_G), candidate rows of its largest panel workgroup count, row j
     S.mats.resize(nmat);
@@ -4737,7 +4858,9 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
     const size_t o_to = o_tj + pb_bytes(S.trsms);
     const size_t o_gj = o_to + pb_bytes(S.trsm_off);
     const size_t o_go = o_gj + pb_bytes(S.gemms);
-    const size_t o_ts = o_go + pb_bytes(S.gemm_off);
+    const size_t o_hj = o_go + pb_bytes(S.gemm_off);
+    const size_t o_ho = o_hj + pb_bytes(S.hjobs);
+    const size_t o_ts = o_ho + pb_bytes(S.h_off);
     const size_t need = o_ts + al256((3 * S.phases.size() + 1) * sizeof(uint64_t));
     if (need > scratch_bytes) return MBA_ERR_ALLOC;
     std::vector<unsigned char> blob(need, 0);
@@ -4754,6 +4877,8 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
     put(o_to, S.trsm_off);
     put(o_gj, S.gemms);
     put(o_go, S.gemm_off);
+    put(o_hj, S.hjobs);
+    put(o_ho, S.h_off);
     char* base = (char*)scratch;
     if (hipMemcpyAsync(base, blob.data(), need, hipMemcpyHostToDevice, st) != hipSuccess) return MBA_ERR_LAUNCH;
     // the host table is released on return: wait for the copy
@@ -4771,6 +4896,8 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
     T.trsm_off = (const int*)(base + o_to);
     T.gemms = (const GemmJobDev*)(base + o_gj);
     T.gemm_off = (const int*)(base + o_go);
+    T.hjobs = (const PbSplitJob*)(base + o_hj);
+    T.h_off = (const int*)(base + o_ho);
     T.barrier = (unsigned*)base;
     T.gerr = (int*)(base + o_gerr);
     T.tstamp = kind_ms ? (uint64_t*)(base + o_ts) : nullptr;
@@ -4798,13 +4925,13 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, void* scra
             // a phase runs from the last workgroup's release by the previous
             // barrier (the first phase: workgroup 0's start) to the last
             // workgroup's end of work; its barrier from there to the last release
-            const int kd = S.phases[q].kind;
+            const int kd = (S.phases[q].kind == PB_SPLIT || S.phases[q].kind == PB_H3) ? PB_GEMM : S.phases[q].kind;
             const uint64_t t0 = q > 0 && ts[2 * np + q] > ts[q] ? ts[2 * np + q] : ts[q];
             const uint64_t t1 = ts[np + 1 + q] > t0 ? ts[np + 1 + q] : t0;
             const uint64_t t2 = q + 1 < np ? (ts[2 * np + 1 + q] > t1 ? ts[2 * np + 1 + q] : t1) : (ts[np] > t1 ? ts[np] : t1);
             kind_ms[kd] += (double)(t1 - t0) * 1e-5;  // 100 MHz ticks -> ms
             kind_ms[6 + kd] += (double)(t2 - t1) * 1e-5;
-            if (kd == PB_GEMM && S.phases[q].sub >= 0 && S.phases[q].sub < 3)
+            if (kd == PB_GEMM && S.phases[q].sub >= 0 && S.phases[q].sub < 3)  // the split product counts as trailing
                 kind_ms[12 + S.phases[q].sub] += (double)(t1 - t0) * 1e-5;
             if (kind_phases) kind_phases[kd] += 1;
         }
@@ -4896,6 +5023,8 @@ int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* sc
     T.phases = (const PbPhase*)(base + o_ph);
     T.nphase = (int)phases.size();
     T.nmat = 0;
+    T.hjobs = nullptr;
+    T.h_off = nullptr;
     T.sjobs = (const PbSolveJob*)(base + o_sj);
     T.s_off = (const int*)(base + o_off);
     T.xs = (cpx<float>*)(base + o_xs);
@@ -4913,17 +5042,17 @@ int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* sc
 size_t pb_scratch_bound(int32_t n_mat, int64_t n_max)
 {
     const size_t np = (size_t)((n_max + PB_NB - 1) / PB_NB);  // panels per matrix
-    const size_t sp = (size_t)((n_max + PB_NB * PB_P - 1) / (PB_NB * PB_P));
+    const size_t sp = np;  // super panels: at most one per panel (super panels of one panel)
     const size_t nm = (size_t)n_mat;
-    const size_t phases = 2 + sp + np * (4 + nm);  // panel phases may split, at most once per matrix
-    const size_t pj = nm * np, sj = nm * np, tj = nm * np, gj = nm * (2 * np + sp);
+    const size_t phases = 2 + 2 * sp + np * (4 + nm);  // panel phases may split, at most once per matrix
+    const size_t pj = nm * np, sj = nm * np, tj = nm * np, gj = nm * (2 * np + sp), hj = nm * sp;
     auto a = [](size_t x) { return al256(x); };
     return a(PB_BAR_WORDS * sizeof(unsigned)) + 256 + a((3 * phases + 1) * sizeof(uint64_t)) +
            a(phases * sizeof(PbPhase)) + a(nm * sizeof(PbMat)) +
            a(pj * sizeof(PbPanelJob)) +
            a((pj + phases) * sizeof(int)) + a(sj * sizeof(SwapJob<float>)) + a((sj + phases) * sizeof(int)) +
            a(tj * sizeof(TrsmJob<float>)) + a((tj + phases) * sizeof(int)) + a(gj * sizeof(GemmJobDev)) +
-           a((gj + phases) * sizeof(int));
+           a((gj + phases) * sizeof(int)) + a(hj * sizeof(PbSplitJob)) + a((hj + 2 * phases) * sizeof(int));
 }
 
 }  // namespace
@@ -5088,7 +5217,8 @@ int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix* mats, int32_
         if (m.lda < m.n) return MBA_ERR_ARG;
     }
     return mba_guard([&] {
-        if (prec == MBA_F32) return lu_factor_batched_persistent(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
+        if (prec == MBA_F32)
+            return lu_factor_batched_persistent(mats, n_mat, nullptr, scratch, scratch_bytes, err_out, S(stream));
         return lu_factor_batched<double>(mats, n_mat, scratch, scratch_bytes, err_out, S(stream));
     });
 }
@@ -5116,7 +5246,25 @@ int mba_lu_solve_batched_persistent(int prec, const mba_lu_solve_job* jobs, int3
     });
 }
 
-int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix* mats, int32_t n_mat, void* scratch,
+int mba_lu_factor_batched_persistent_opts(int prec, const mba_lu_matrix* mats, int32_t n_mat,
+                                          const mba_batched_lu_options* opts, void* scratch, size_t scratch_bytes,
+                                          int32_t* err_out, mba_stream_t stream)
+{
+    if (!mats || !scratch || !err_out || n_mat <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32) return MBA_ERR_ARG;
+    for (int i = 0; i < n_mat; ++i) {
+        const mba_lu_matrix& m = mats[i];
+        if (!m.a || !m.ipiv || !m.info || !m.workspace) return MBA_ERR_ARG;
+        if (m.n <= 0 || m.n > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+        if (m.lda < m.n) return MBA_ERR_ARG;
+    }
+    return mba_guard([&] {
+        return lu_factor_batched_persistent(mats, n_mat, opts, scratch, scratch_bytes, err_out, S(stream));
+    });
+}
+
+int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix* mats, int32_t n_mat,
+                                              const mba_batched_lu_options* opts, void* scratch,
                                               size_t scratch_bytes, int32_t* err_out, double* kind_ms,
                                               int64_t* kind_phases, mba_stream_t stream)
 {
@@ -5128,7 +5276,7 @@ int mba_lu_factor_batched_persistent_profiled(const mba_lu_matrix* mats, int32_t
         if (m.lda < m.n) return MBA_ERR_ARG;
     }
     return mba_guard([&] {
-        return lu_factor_batched_persistent(mats, n_mat, scratch, scratch_bytes, err_out, S(stream), kind_ms,
+        return lu_factor_batched_persistent(mats, n_mat, opts, scratch, scratch_bytes, err_out, S(stream), kind_ms,
                                             kind_phases);
     });
 }

@@ -111,7 +111,8 @@ SIGNATURES = {
     "mba_lu_batched_persistent_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),
     "mba_lu_factor_batched_persistent": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _i32p, _vp]),
     "mba_measure_valu_rates": (_c.c_int, [_vp, _c.c_int, _vp]),
-    "mba_lu_factor_batched_persistent_profiled": (_c.c_int, [_vp, _i32, _vp, _sz, _i32p, _vp, _vp, _vp]),
+    "mba_lu_factor_batched_persistent_opts": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _vp, _sz, _i32p, _vp]),
+    "mba_lu_factor_batched_persistent_profiled": (_c.c_int, [_vp, _i32, _vp, _vp, _sz, _i32p, _vp, _vp, _vp]),
     "mba_lu_factor_profiled": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i32p, _i32p, _vp, _sz, _vp, _vp,
                                           _vp, _vp]),
     "mba_lu_pivots_to_perm": (_c.c_int, [_i32p, _i64, _i32p, _vp]),
@@ -184,6 +185,12 @@ SIGNATURES = {
 }
 
 _LIB = None
+
+
+class BatchedLuOptions(ctypes.Structure):
+    """mba_batched_lu_options: 32-column panels per super panel (0: 8) and the
+    trailing product (0: FP32 3M, 1: split FP16)."""
+    _fields_ = [("super_panels", ctypes.c_int32), ("trailing", ctypes.c_int32)]
 
 
 class MBAError(RuntimeError):

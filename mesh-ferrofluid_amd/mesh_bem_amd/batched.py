@@ -86,6 +86,62 @@ class GemmTuner:
         _lib.check(lib.mba_gemm_set_variant(_lib.MBA_F32, self.tune(n)), "mba_gemm_set_variant")
 
 
+class BatchLuTuner:
+    """Fastest schedule of the persistent batched LU for a batch: super-panel
+    width (4 or 8 panels of 32 columns, i.e. the trailing product's k = 128 or
+    256) x trailing product (FP32 3M on MFMA, or the FP32-accurate split-FP16
+    product).  Each candidate factors copies of the batch's own matrices
+    (``reps`` timed runs after a warm-up, device events); the fastest is kept
+    per batch shape (the sorted matrix orders), optionally in a JSON file."""
+
+    CANDIDATES = ((8, 0), (8, 1), (4, 0), (4, 1))
+
+    def __init__(self, path: Optional[str] = None, reps: int = 2):
+        self.path = path
+        self.reps = reps
+        self.best: Dict[str, List[int]] = {}
+        self.timings: Dict[str, List[float]] = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.best = {k: list(v) for k, v in json.load(f).items()}
+
+    @staticmethod
+    def key(mats) -> str:
+        return ",".join(str(n) for n in sorted(int(A.shape[0]) for A in mats))
+
+    def tune(self, mats) -> tuple:
+        """mats: the batch's complex64 matrices (not modified)."""
+        k = self.key(mats)
+        if k in self.best:
+            return tuple(self.best[k])
+        times = []
+        for cand in self.CANDIDATES:
+            def run():
+                lus = lu_factor_batched([A.clone() for A in mats], overwrite=True, persistent=True, options=cand)
+                return lus
+            run()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(self.reps):
+                copies = [A.clone() for A in mats]
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                lu_factor_batched(copies, overwrite=True, persistent=True, options=cand)
+                e1.record()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+                del copies
+            times.append(min(ts))
+        self.timings[k] = times
+        self.best[k] = list(self.CANDIDATES[int(np.argmin(times))])
+        if self.path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.path)), exist_ok=True)
+            with open(self.path, "w") as f:
+                json.dump(self.best, f)
+        return tuple(self.best[k])
+
+
 @dataclass
 class BatchResult:
     J: List[torch.Tensor]  # per mesh [E, D] complex128
@@ -165,7 +221,8 @@ class BatchedMoM:
         return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
-              tol: float = 1e-8, maxit: int = 10, persistent: bool = True) -> BatchResult:
+              tol: float = 1e-8, maxit: int = 10, persistent: bool = True,
+              lu_tuner: Optional[BatchLuTuner] = None) -> BatchResult:
         """Mixed precision for all meshes: the FP32 LUs in lock step
         (``lu_factor_batched``: the panels of every matrix in shared
         cooperative launches, the trailing products of every matrix in one
@@ -182,9 +239,11 @@ class BatchedMoM:
         lib = _lib.load()
         if tuner is not None:
             tuner.apply(max(Z.shape[0] for Z in Zs))
+        Z32 = [to_c64(Z) for Z in Zs]
+        opts = lu_tuner.tune(Z32) if (persistent and lu_tuner is not None) else None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lus = lu_factor_batched([to_c64(Z) for Z in Zs], overwrite=True, persistent=persistent)
+        lus = lu_factor_batched(Z32, overwrite=True, persistent=persistent, options=opts)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for lu in lus:

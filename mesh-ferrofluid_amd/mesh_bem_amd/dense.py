@@ -8,6 +8,7 @@
   complex128 matrix, until the correction falls below ``tol`` relative to J.
 """
 
+import ctypes
 from dataclasses import dataclass
 from typing import Optional
 
@@ -214,14 +215,16 @@ class LUFactor:
         return X.reshape(-1) if vec else X
 
 
-def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True):
+def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True, options=None):
     """LU of several square complex matrices (same dtype); returns one
     LUFactor per matrix.  ``persistent`` (default): the whole batch in one
     persistent cooperative launch (``mba_lu_factor_batched_persistent``:
     32-column panels of every matrix together, phases separated by grid
     barriers; complex128 takes the lock-step path); otherwise the lock-step
-    multi-launch form (``mba_lu_factor_batched``).  Nothing is synchronised:
-    call ``check()`` on each after synchronising."""
+    multi-launch form (``mba_lu_factor_batched``).  ``options`` (persistent,
+    complex64): (super_panels, trailing) -- 32-column panels per super panel
+    and the trailing product (0: FP32 3M, 1: split FP16); None: (8, 0).
+    Nothing is synchronised: call ``check()`` on each after synchronising."""
     lib = _lib.load()
     if not mats:
         return []
@@ -252,9 +255,15 @@ def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True):
     if persistent:
         sb = lib.mba_lu_batched_persistent_scratch_bytes(prec, n_mat, nmax)
         scratch = torch.empty(sb, dtype=torch.uint8, device=dev)
-        st = lib.mba_lu_factor_batched_persistent(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err),
-                                                  _lib.stream())
-        _lib.check(st, "mba_lu_factor_batched_persistent")
+        if options is not None and prec == _lib.MBA_F32:
+            opts = _lib.BatchedLuOptions(int(options[0]), int(options[1]))
+            st = lib.mba_lu_factor_batched_persistent_opts(prec, desc, n_mat, ctypes.byref(opts), _lib.ptr(scratch),
+                                                           sb, _lib.ptr(err), _lib.stream())
+            _lib.check(st, "mba_lu_factor_batched_persistent_opts")
+        else:
+            st = lib.mba_lu_factor_batched_persistent(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err),
+                                                      _lib.stream())
+            _lib.check(st, "mba_lu_factor_batched_persistent")
     else:
         sb = lib.mba_lu_batched_scratch_bytes(prec, n_mat, nmax)
         scratch = torch.empty(sb, dtype=torch.uint8, device=dev)

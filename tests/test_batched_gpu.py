@@ -266,3 +266,39 @@ def test_lu_batched_persistent_complex128_takes_lock_step(dev):
     for A, f in zip(mats, lus):
         f.check()
         assert torch.equal(f.ipiv, LUFactor(A.clone()).ipiv)
+
+
+def test_lu_batched_persistent_schedules(dev):
+    """Every schedule the batch tuner can pick -- super panels of 4 or 8
+    32-column panels, FP32 3M or split-FP16 trailing product -- gives factors
+    with P A = L U to FP32 accuracy (ragged orders around 32, 128, 256)."""
+    from mesh_bem_amd.batched import BatchLuTuner
+    from mesh_bem_amd.dense import lu_factor_batched
+
+    rng = np.random.default_rng(29)
+    sizes = [31, 129, 257, 700, 1301]
+    mats = [torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), dtype=torch.complex64,
+                         device=dev) for n in sizes]
+    for cand in BatchLuTuner.CANDIDATES:
+        print(f"persistent LU schedule {cand}", flush=True)
+        lus = lu_factor_batched([A.clone() for A in mats], persistent=True, options=cand)
+        torch.cuda.synchronize()
+        for A, f in zip(mats, lus):
+            f.check()
+            _check_factor(A, f, 1e-4)
+
+
+def test_batch_lu_tuner_picks_and_caches(dev, tmp_path):
+    """The tuner times its candidates on the batch, keeps the fastest per batch
+    shape (also in its JSON file) and answers from the cache afterwards."""
+    from mesh_bem_amd.batched import BatchLuTuner
+
+    rng = np.random.default_rng(31)
+    mats = [torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), dtype=torch.complex64,
+                         device=dev) for n in (300, 900, 1500)]
+    path = str(tmp_path / "lu_tuner.json")
+    t = BatchLuTuner(path=path, reps=1)
+    c = t.tune(mats)
+    assert c in BatchLuTuner.CANDIDATES and len(t.timings[t.key(mats)]) == len(BatchLuTuner.CANDIDATES)
+    t2 = BatchLuTuner(path=path)
+    assert t2.tune(mats) == c and not t2.timings
