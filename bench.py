@@ -755,7 +755,8 @@ def run_c3(args, dev):
         Vs = bm.rhs(dirs, pols)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        res = bm.solve(Zs, Vs, tuner=tuner, tol=1e-8, persistent=args.batched_lu == "persistent", lu_tuner=lu_tuner)
+        res = bm.solve(Zs, Vs, tuner=tuner, tol=1e-8, persistent=args.batched_lu == "persistent", lu_tuner=lu_tuner,
+                       solve_persistent=args.batched_solve == "persistent")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         return res, t1 - t0, t2 - t1, Zs, Vs
@@ -788,8 +789,10 @@ def run_c3(args, dev):
         "vs_baseline": None,
         "dtype": ("c64 factors (" + ("one persistent batched-LU launch" if args.batched_lu == "persistent"
                                      else "lock-step batched LU") +
-                  ", FP32 MFMA 3M trailing updates) + c128 assembly/residual"),
+                  (", tuned trailing update" if lu_tuner is not None else ", FP32 MFMA 3M trailing updates") +
+                  ") + c128 assembly/residual"),
         "batched_lu": args.batched_lu,
+        "batched_solve": args.batched_solve,
         "value_definition": "right-hand sides per second: meshes x waves / (FP64 assembly + RHS + FP32 LUs "
                             "+ solves + FP64 refinement) per step",
         "data": "synthetic",
@@ -971,6 +974,9 @@ def main():
                     help="trailing-update product of the complex64 LU (c2/c1/small)")
     ap.add_argument("--batched-lu", default="persistent", choices=["persistent", "lockstep"],
                     help="batched LU of c3: one persistent launch or the lock-step launches")
+    ap.add_argument("--batched-solve", default="persistent", choices=["walk", "persistent"],
+                    help="triangular solves of c3: per-job block walks (two launches) or one persistent "
+                         "dataflow launch")
     ap.add_argument("--lu-panel", default="partial", choices=["partial", "tournament"],
                     help="panel pivoting of the complex64 LU (c2/c1/small)")
     args = ap.parse_args()

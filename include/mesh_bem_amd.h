@@ -297,18 +297,22 @@ size_t mba_lu_solve_batched_scratch_bytes(int32_t n_jobs);
 int mba_lu_solve_batched(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs, void *scratch,
                          size_t scratch_bytes, mba_stream_t stream);
 /* The same solves (complex64) in ONE persistent launch (all workgroups
- * resident, checked against the device's occupancy before the launch): the
- * 64-row blocks of all jobs are eliminated in block steps (one phase per step
- * of the largest job, forward then backward, every block row and group of 64
- * right-hand sides of every job a task of that step), phases separated by
- * grid barriers.  Same jobs and results as mba_lu_solve_batched; scratch:
- * mba_lu_solve_batched_persistent_scratch_bytes(n_jobs, largest n).
- * complex128 runs the two-launch path.  Replaces the per-mesh
- * torch.linalg.solve of src/mesh_BEM/mesh_BEM_solver_3d.py:174 for many
- * right-hand sides; no single reference call. */
-size_t mba_lu_solve_batched_persistent_scratch_bytes(int32_t n_jobs, int64_t n_max);
+ * resident, checked against the device's occupancy before the launch), as a
+ * dataflow inside each XCD: every job goes to one XCD's queue, every 64-row
+ * block and group of 64 right-hand sides of a job is a forward and a backward
+ * task, taken in ticket order by that XCD's workgroups; a task subtracts the
+ * products of the blocks it depends on as their flags come up, then applies
+ * the diagonal block's inverse and raises its own flag.  Same jobs and
+ * results as mba_lu_solve_batched; scratch:
+ * mba_lu_solve_batched_persistent_scratch_bytes(jobs, n_jobs) (reads n and
+ * nrhs of each job).  err_out (device int32) is written by the launch: zero,
+ * or nonzero after a wait that timed out (the results are then invalid).
+ * complex128 runs the two-launch path and writes zero.  Replaces the
+ * per-mesh torch.linalg.solve of src/mesh_BEM/mesh_BEM_solver_3d.py:174 for
+ * many right-hand sides; no single reference call. */
+size_t mba_lu_solve_batched_persistent_scratch_bytes(const mba_lu_solve_job *jobs, int32_t n_jobs);
 int mba_lu_solve_batched_persistent(int prec, const mba_lu_solve_job *jobs, int32_t n_jobs, void *scratch,
-                                    size_t scratch_bytes, mba_stream_t stream);
+                                    size_t scratch_bytes, int32_t *err_out, mba_stream_t stream);
 size_t mba_lu_batched_scratch_bytes(int prec, int32_t n_mat, int64_t n_max);
 int mba_lu_factor_batched(int prec, const mba_lu_matrix *mats, int32_t n_mat, void *scratch, size_t scratch_bytes,
                           int32_t *err_out, mba_stream_t stream);

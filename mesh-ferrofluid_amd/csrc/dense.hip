@@ -3945,10 +3945,6 @@ enum PbKind : int {
     PB_LASWP = 3,
     PB_TRSM = 4,
     PB_FINAL = 5,
-    PB_SPERM = 6,  // solves: W = P b
-    PB_SFWD = 7,   // solves: forward block step (sub = block)
-    PB_SBWD = 8,   // solves: backward block step (sub = block)
-    PB_SCOPY = 9,  // solves: b = X
     PB_SPLIT = 10, // trailing product, split FP16: operand splits
     PB_H3 = 11     // trailing product, split FP16: product tiles
 };
@@ -3984,9 +3980,17 @@ struct PbSolveJob {
     cpx<float>* b;
     int64_t ldb, nrhs;
     cpx<float>* work;
-    int nblk, ngrp;  // 64-row blocks, groups of PB_SCW right-hand sides
+    uint32_t* flags;  // [2 passes][ngrp][nblk] block done (zero at launch)
+    int nblk, ngrp;   // 64-row blocks, groups of PB_SCW right-hand sides
 };
 constexpr int PB_SCW = 64;  // right-hand sides per solve task
+
+// One task of the dataflow solves: block i of job `job`, group g of
+// right-hand sides, forward (pass 0) or backward (pass 1).
+struct PbDfTask {
+    int32_t job, i;
+    int16_t pass, g;
+};
 
 struct PbPhase {
     int kind;
@@ -4028,9 +4032,6 @@ struct PbTables {
     const int* gemm_off;
     const PbSplitJob* hjobs;  // split-FP16 trailing products
     const int* h_off;
-    const PbSolveJob* sjobs;  // solves (null for factorisation-only launches)
-    const int* s_off;
-    cpx<float>* xs;           // per workgroup: one 64 x PB_SCW block of x
     unsigned* barrier;  // barrier words (PB_BAR_WORDS, zero at launch)
     int* gerr;          // barrier timeout flag, zero at launch
     uint64_t* tstamp;   // optional [3 nphase + 1]: wall clock (100 MHz) of workgroup 0 at each phase
@@ -4141,7 +4142,9 @@ constexpr int PB_BAR_START = 1 * PB_LINE;             // workgroups registered (
 constexpr int PB_BAR_MEMBERS = 2 * PB_LINE;           // [8] workgroups per XCD
 constexpr int PB_BAR_ARRIVE = 10 * PB_LINE;           // [8] arrivals per XCD
 constexpr int PB_BAR_DONE = 18 * PB_LINE;             // [8] barrier number whose L2 invalidate is done
-constexpr int PB_BAR_WORDS = 26 * PB_LINE;
+constexpr int PB_BAR_TICKET = 26 * PB_LINE;           // [8] next task ticket of each XCD queue (solves)
+constexpr int PB_BAR_FIN = 34 * PB_LINE;              // workgroups done with the solve tasks
+constexpr int PB_BAR_WORDS = 35 * PB_LINE;
 constexpr int PB_MAX_XCD = 8;
 
 __device__ __forceinline__ unsigned pb_ld(unsigned* p)
@@ -4223,7 +4226,9 @@ __device__ __forceinline__ void pb_grid_sync(PbBarrier& B, unsigned* bar, int* g
             __hip_atomic_store(done, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             pb_wait_ge(done, e, gerr);
-            asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1
+            // this CU's L1 (the agent-scope invalidate: the workgroup-scope
+            // one leaves the L1 alone when a workgroup's waves share a CU)
+            asm volatile("buffer_inv sc1" ::: "memory");
         }
     }
     __syncthreads();
@@ -4349,66 +4354,191 @@ __device__ __noinline__ void pb_phase_h3(const PbTables& T, const PbPhase& ph, i
     }
 }
 
-// ---- persistent solves: tasks (job, 64-row block i, group g of PB_SCW
-// right-hand sides).  Forward step s (L unit lower, running right-hand sides
-// W = work, output X = b): every task recomputes x_s = inv(L_ss) W_s; the task
-// i = s stores it as X_s, the tasks i > s subtract L_is x_s from W_i.  W_s is
-// only read in step s and each W_i written by one task, so no task of a step
-// conflicts with another.  Backward step s (U, running Y = b, output X =
-// work): the same with inv(U_ss) and the tasks i < s; a last phase copies the
-// solution from work into b.
-__device__ __forceinline__ void pb_solve_task(const PbSolveJob& J, int u, int step, bool fwd, cpx<float>* xs,
-                                              float* lds)
+// ---- persistent solves, dataflow inside each XCD.  Every job (a prepared
+// factorisation with its right-hand sides) is assigned to one XCD, whose
+// workgroups take its tasks from that XCD's ticket counter.  A task (job, i,
+// g, pass) streams its block row: forward, t_i = (P b)_i - sum_{j<i} L_ij x_j
+// over the blocks x_j (in `work`) as their flags come up, then x_i =
+// inv(L_ii) t_i in place and its flag; backward (after the job's last forward
+// block), z_i = x_i - sum_{j>i} U_ij z_j over the finished blocks (in b), then
+// z_i = inv(U_ii) (.).  Tickets are dealt in an order where every task waits
+// only on tasks with lower tickets (forward by ascending block, then backward
+// by descending block), taken by running workgroups, so every wait ends.  All
+// data a job's tasks exchange stays in one XCD's L2: a producer drains its
+// stores before raising the flag, a consumer invalidates its CU's L1 after
+// seeing it.  The invalidate is the agent-scope one (`buffer_inv sc1`): the
+// workgroup-scope form (`sc0`) leaves the L1 alone unless a workgroup's waves
+// are split over CUs, and blocks read through a stale L1 line came out wrong.
+
+// Wait until flag j is up, then return the end of the run of raised flags
+// j, j + step, ... (at most 64, stopping before `end`); j when the wait
+// failed.  Wave 0 polls 64 flags per load; every thread's earlier stores are
+// drained first and the CU's L1 is invalidated after, so the blocks of the
+// run can be read and the caller's own stores are visible to all its threads.
+__device__ __forceinline__ int pb_flag_run(const uint32_t* f, int j, int end, int step, int* gerr)
 {
-    const int i = (fwd ? step : 0) + u / J.ngrp, g = u % J.ngrp;
-    const int64_t r_s = (int64_t)step * TS_BS, r_i = (int64_t)i * TS_BS;
-    const int64_t bs_s = (J.n - r_s) < TS_BS ? (J.n - r_s) : TS_BS, bs_i = (J.n - r_i) < TS_BS ? (J.n - r_i) : TS_BS;
-    const int64_t c0 = (int64_t)g * PB_SCW, cw = (J.nrhs - c0) < PB_SCW ? (J.nrhs - c0) : PB_SCW;
-    const cpx<float>* inv = (fwd ? J.linv : J.uinv) + (int64_t)step * TS_BS * TS_BS;
-    // running right-hand sides and the output of this pass
-    cpx<float>* Wr = fwd ? J.work : J.b;
-    const int64_t ldw = fwd ? J.nrhs : J.ldb;
-    cpx<float>* Xo = fwd ? J.b : J.work;
-    const int64_t ldx = fwd ? J.ldb : J.nrhs;
-    if (i == step) {
-        cgemm3m_tile(bs_s, cw, bs_s, 1.f, inv, TS_BS, Wr + r_s * ldw + c0, ldw, 0.f, Xo + r_s * ldx + c0, ldx, 0, 0,
-                     lds);
-        return;
+    __shared__ int s_k;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (threadIdx.x < 64) {
+        const int q = j + step * (int)threadIdx.x;
+        const bool inr = step > 0 ? q < end : q > end;
+        long spins = 0;
+        int k = j;
+        while (true) {
+            const uint32_t v = inr ? __hip_atomic_load(f + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+            const uint64_t m = __ballot(v != 0u);
+            if (m & 1ull) {
+                k = j + step * (~m == 0ull ? 64 : __builtin_ctzll(~m));
+                break;
+            }
+            if ((spins & 63) == 63 && __hip_atomic_load(gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                break;
+            if (++spins > SPIN_LIMIT) {
+                if (threadIdx.x == 0) __hip_atomic_fetch_or(gerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (threadIdx.x == 0) s_k = k;
     }
-    cgemm3m_tile(bs_s, cw, bs_s, 1.f, inv, TS_BS, Wr + r_s * ldw + c0, ldw, 0.f, xs, PB_SCW, 0, 0, lds);
-    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1: x_s was just written through it
-    cgemm3m_tile(bs_i, cw, bs_s, -1.f, J.lu + r_i * J.lda + r_s, J.lda, xs, PB_SCW, 1.f, Wr + r_i * ldw + c0, ldw, 0,
-                 0, lds);
+    asm volatile("buffer_inv sc1" ::: "memory");  // this CU's L1: the blocks were written on other CUs
+    return s_k;
 }
 
-__device__ __noinline__ void pb_phase_solve(const PbTables& T, const PbPhase& ph, int b, int nwg, float* lds)
+// this workgroup's global writes are complete and visible to all its threads
+__device__ __forceinline__ void pb_local_fence()
 {
-    const int* off = T.s_off + ph.off0;
-    cpx<float>* xs = T.xs + (int64_t)b * TS_BS * PB_SCW;
-    for (int vb = b; vb < ph.nvb; vb += nwg) {
-        const int j = job_of(off, ph.njob, vb);
-        const PbSolveJob& J = T.sjobs[ph.job0 + j];
-        const int u = vb - off[j];
-        if (ph.kind == PB_SFWD || ph.kind == PB_SBWD) {
-            pb_solve_task(J, u, ph.sub, ph.kind == PB_SFWD, xs, lds);
-        } else {
-            // PB_SPERM: work rows of block i <- b rows perm[.]; PB_SCOPY: b <- work
-            const int i = u / J.ngrp, g = u % J.ngrp;
-            const int64_t r0 = (int64_t)i * TS_BS, c0 = (int64_t)g * PB_SCW;
-            const int64_t rows = (J.n - r0) < TS_BS ? (J.n - r0) : TS_BS;
-            const int64_t cw = (J.nrhs - c0) < PB_SCW ? (J.nrhs - c0) : PB_SCW;
-            for (int64_t e = threadIdx.x; e < rows * cw; e += PB_THREADS) {
-                const int64_t r = r0 + e / cw, c = c0 + e % cw;
-                if (ph.kind == PB_SPERM)
-                    J.work[r * J.nrhs + c] = J.b[(int64_t)J.perm[r] * J.ldb + c];
-                else
-                    J.b[r * J.ldb + c] = J.work[r * J.nrhs + c];
-            }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    asm volatile("buffer_inv sc1" ::: "memory");
+}
+
+__device__ __forceinline__ void pb_flag_raise(uint32_t* f)
+{
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the block's stores are in the XCD's L2
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(f, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One tile product of the solves, not inlined: its registers are its own
+// (inlined into the task loop, the product and the loop's 64-bit indices
+// spilled).
+__device__ __noinline__ void pb_df_product(int64_t M, int64_t N, int64_t K, float alpha, const cpx<float>* A,
+                                           int64_t lda, const cpx<float>* B, int64_t ldb, float beta, cpx<float>* C,
+                                           int64_t ldc, float* lds)
+{
+    cgemm3m_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, lds);
+}
+
+__device__ __forceinline__ void pb_df_task(const PbSolveJob& J, const PbDfTask& t, float* lds, int* gerr)
+{
+    const int i = t.i, g = t.g;
+    const bool fwd = t.pass == 0;
+    const int64_t r_i = (int64_t)i * TS_BS, bs_i = (J.n - r_i) < TS_BS ? (J.n - r_i) : TS_BS;
+    const int64_t c0 = (int64_t)g * PB_SCW, cw = (J.nrhs - c0) < PB_SCW ? (J.nrhs - c0) : PB_SCW;
+    const uint32_t* ffwd = J.flags + (int64_t)g * J.nblk;
+    const uint32_t* fbwd = J.flags + ((int64_t)J.ngrp + g) * J.nblk;
+    cpx<float>* Wi = J.work + r_i * J.nrhs + c0;  // forward output x_i (ld nrhs)
+    cpx<float>* Bi = J.b + r_i * J.ldb + c0;      // backward output z_i (ld ldb)
+    // this pass: output block, finished blocks (base, ld), their flags, diagonal inverse
+    cpx<float>* X = fwd ? Wi : Bi;
+    const int64_t ldx = fwd ? J.nrhs : J.ldb;
+    const cpx<float>* src = fwd ? J.work : J.b;
+    const uint32_t* fl = fwd ? ffwd : fbwd;
+    const cpx<float>* inv = (fwd ? J.linv : J.uinv) + (int64_t)i * TS_BS * TS_BS;
+    if (fwd) {
+        for (int64_t e = threadIdx.x; e < bs_i * cw; e += PB_THREADS) {
+            const int64_t r = e / cw, c = e % cw;
+            Wi[r * J.nrhs + c] = J.b[(int64_t)J.perm[r_i + r] * J.ldb + c0 + c];
         }
-        __syncthreads();
+    } else {
+        // the job's forward pass is done once its last block is
+        if (pb_flag_run(ffwd, J.nblk - 1, J.nblk, 1, gerr) == J.nblk - 1) return;
+        for (int64_t e = threadIdx.x; e < bs_i * cw; e += PB_THREADS) {
+            const int64_t r = e / cw, c = e % cw;
+            Bi[r * J.ldb + c] = Wi[r * J.nrhs + c];
+        }
     }
+    // X -= A_ij (block j) over the finished blocks j, run by run of raised
+    // flags.  Each product's epilogue reads and writes the same elements of X
+    // in the same threads, so consecutive products only need the barrier that
+    // frees the LDS.
+    const int step = fwd ? 1 : -1;
+    for (int j = fwd ? 0 : J.nblk - 1; j != i;) {
+        const int k = pb_flag_run(fl, j, i, step, gerr);
+        if (k == j) return;
+        for (; j != k; j += step) {
+            const int64_t r_j = (int64_t)j * TS_BS, bs_j = (J.n - r_j) < TS_BS ? (J.n - r_j) : TS_BS;
+            pb_df_product(bs_i, cw, bs_j, -1.f, J.lu + r_i * J.lda + r_j, J.lda, src + r_j * ldx + c0, ldx, 1.f, X,
+                          ldx, lds);
+            __syncthreads();
+        }
+    }
+    pb_local_fence();
+    // in place: the single tile reads all of X before its epilogue writes it
+    pb_df_product(bs_i, cw, bs_i, 1.f, inv, TS_BS, X, ldx, 0.f, X, ldx, lds);
+    pb_flag_raise((uint32_t*)fl + i);
+}
+
+// Tables of the dataflow solves (one launch).
+struct PbDfTables {
+    const PbSolveJob* sjobs;
+    const PbDfTask* dtasks;  // solve tasks, queue by queue in ticket order
+    const int* d_off;        // [PB_MAX_XCD + 1] each queue's range of dtasks
+    int32_t* err_out;        // written by the last workgroup: nonzero after a timed-out wait
+    unsigned* barrier;       // barrier words (PB_BAR_WORDS, zero at launch): registration, tickets
+    int* gerr;               // zero at launch
+};
+
+__device__ __forceinline__ void pb_df_serve(const PbDfTables& T, const PbBarrier& B, unsigned* bar, float* lds)
+{
+    __shared__ int s_t;
+    // queue x is served by XCD x; a queue of an XCD without workgroups by one
+    // that has some (all of a queue on one XCD: its flags and blocks stay in
+    // that XCD's L2)
+    int present[PB_MAX_XCD], np = 0;
+    for (int x = 0; x < PB_MAX_XCD; ++x)
+        if (pb_ld(bar + PB_BAR_MEMBERS + x * PB_LINE) > 0) present[np++] = x;
+    __shared__ int s_fail;  // one reading for the whole workgroup (uniform loop bounds)
+    if (threadIdx.x == 0) s_fail = __hip_atomic_load(T.gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const bool failed = s_fail != 0;
+    for (int x = 0; x < PB_MAX_XCD && !failed && np > 0; ++x) {
+        const int owner = pb_ld(bar + PB_BAR_MEMBERS + x * PB_LINE) > 0 ? x : present[x % np];
+        if (owner != B.xcc) continue;
+        const int lo = T.d_off[x], cnt = T.d_off[x + 1] - lo;
+        unsigned* ticket = bar + PB_BAR_TICKET + x * PB_LINE;
+        while (true) {
+            if (threadIdx.x == 0)
+                s_t = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const int t = s_t;
+            __syncthreads();
+            if (t >= cnt) break;
+            const PbDfTask task = T.dtasks[lo + t];
+            pb_df_task(T.sjobs[task.job], task, lds, T.gerr);
+            __syncthreads();
+        }
+    }
+    // the last workgroup to finish reports the error word
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned f =
+            __hip_atomic_fetch_add(bar + PB_BAR_FIN, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (f == gridDim.x)
+            T.err_out[0] = __hip_atomic_load(T.gerr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// All workgroups resident (they register their XCDs at the start barrier).
+__global__ __launch_bounds__(PB_THREADS, 2) void k_lu_solve_dataflow(PbDfTables T)
+{
+    __shared__ alignas(16) float lds[G3M_LDS_FLOATS];
+    PbBarrier B;
+    pb_barrier_init(B, T.barrier, gridDim.x, T.gerr);
+    pb_df_serve(T, B, T.barrier, lds);
 }
 
 __global__ __launch_bounds__(PB_THREADS, 2) void k_lu_batched_persistent(PbTables T)
@@ -4436,8 +4566,6 @@ __global__ __launch_bounds__(PB_THREADS, 2) void k_lu_batched_persistent(PbTable
             pb_phase_trsm(T, ph, b, nwg, (cpx<float>*)smem_raw);
         } else if (kind == PB_GEMM) {
             pb_phase_gemm(T, ph, b, nwg, (float*)smem_raw);
-        } else if (kind >= PB_SPERM && kind <= PB_SCOPY) {
-            pb_phase_solve(T, ph, b, nwg, (float*)smem_raw);
         } else if (kind == PB_SPLIT) {
             pb_phase_split(T, ph, b, nwg, smem_raw);
         } else if (kind == PB_H3) {
@@ -4901,9 +5029,6 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, const mba_
     T.barrier = (unsigned*)base;
     T.gerr = (int*)(base + o_gerr);
     T.tstamp = kind_ms ? (uint64_t*)(base + o_ts) : nullptr;
-    T.sjobs = nullptr;
-    T.s_off = nullptr;
-    T.xs = nullptr;
     void* kargs[] = {&T};
     if ((rc = launch_resident((const void*)k_lu_batched_persistent, (unsigned)nwg, PB_THREADS, kargs, dyn, st)) !=
         MBA_OK)
@@ -4939,103 +5064,134 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, const mba_
     return MBA_OK;
 }
 
-// Solves of prepared complex64 factorisations, all jobs in one persistent
-// launch: one phase for the permuted right-hand sides, one per 64-row block of
-// the largest job forward and backward, one copy phase.
-size_t pb_solve_scratch(int32_t n_jobs, int64_t n_max, int nwg)
-{
-    const size_t nb = (size_t)((n_max + TS_BS - 1) / TS_BS);
-    const size_t phases = 2 * nb + 2;
-    return al256(PB_BAR_WORDS * sizeof(unsigned)) + 256 + al256(phases * sizeof(PbPhase)) +
-           al256((size_t)n_jobs * sizeof(PbSolveJob)) + al256(phases * ((size_t)n_jobs + 1) * sizeof(int)) +
-           (size_t)nwg * TS_BS * PB_SCW * sizeof(cpx<float>);
-}
-
-int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* scratch, size_t scratch_bytes,
-                                hipStream_t st)
-{
-    size_t dyn = 0;
-    int nwg = 0;
-    int rc = pb_launch_shape(&dyn, &nwg);
-    if (rc != MBA_OK) return rc;
+// Solves of prepared complex64 factorisations, all jobs in one launch of
+// resident workgroups (k_lu_solve_dataflow).  Host side: the jobs are dealt to the eight
+// XCD queues (largest first, each to the least loaded queue; the load of a job
+// is its nblk^2 x groups block products), and each queue's tickets are
+// ordered forward by ascending block, then backward by descending block, the
+// jobs of a queue interleaved step by step.
+struct PbSolvePlan {
     std::vector<PbSolveJob> sj;
-    int nblk_max = 0;
+    std::vector<PbDfTask> tasks;
+    std::vector<int> d_off;
+    std::vector<size_t> flag0;  // per job: first flag word
+    size_t nflag = 0;
+    size_t o_gerr = 0, o_sj = 0, o_dt = 0, o_doff = 0, o_flag = 0, total = 0;
+};
+
+int pb_solve_plan(const mba_lu_solve_job* jobs, int njob, bool dry, PbSolvePlan& S)
+{
     for (int q = 0; q < njob; ++q) {
         const mba_lu_solve_job& J = jobs[q];
         if (J.n <= 0 || J.nrhs <= 0) continue;
-        PbSolveJob d;
+        PbSolveJob d{};
         d.lu = (const cpx<float>*)J.lu;
         d.n = J.n;
         d.lda = J.lda;
         d.nblk = (int)((J.n + TS_BS - 1) / TS_BS);
         d.perm = (const int32_t*)J.aux;
-        d.linv = (const cpx<float>*)((const char*)J.aux + al256((size_t)J.n * 4));
-        d.uinv = d.linv + (size_t)d.nblk * TS_BS * TS_BS;
-        // the inverse tables are laid out as lu_prepare writes them
-        if ((const char*)d.uinv - (const char*)d.linv != (ptrdiff_t)al256((size_t)d.nblk * TS_BS * TS_BS * 8))
-            return MBA_ERR_ARG;
+        if (!dry) {
+            d.linv = (const cpx<float>*)((const char*)J.aux + al256((size_t)J.n * 4));
+            // the inverse tables are laid out as lu_prepare writes them
+            d.uinv = (const cpx<float>*)((const char*)d.linv + al256((size_t)d.nblk * TS_BS * TS_BS * 8));
+        }
         d.b = (cpx<float>*)J.b;
         d.ldb = J.ldb;
         d.nrhs = J.nrhs;
         d.work = (cpx<float>*)J.work;
-        d.ngrp = (int)((J.nrhs + PB_SCW - 1) / PB_SCW);
-        nblk_max = d.nblk > nblk_max ? d.nblk : nblk_max;
-        sj.push_back(d);
+        const int64_t ngrp = (J.nrhs + PB_SCW - 1) / PB_SCW;
+        if (ngrp > 0x7fff || (int64_t)d.nblk * ngrp > (1 << 24)) return MBA_ERR_SIZE;
+        d.ngrp = (int)ngrp;
+        S.nflag += (size_t)2 * d.ngrp * d.nblk;
+        S.sj.push_back(d);
     }
-    if (sj.empty()) return MBA_OK;
-    const int nj = (int)sj.size();
-    std::vector<PbPhase> phases;
-    std::vector<int> off;
-    auto add_phase = [&](int kind, int sub, auto tasks_of) -> int {
-        PbPhase ph{kind, nj, 0, sub, 0, (int64_t)off.size()};
-        off.push_back(0);
-        for (const PbSolveJob& d : sj) {
-            const int64_t t = (int64_t)ph.nvb + tasks_of(d);
-            if (t > 0x7fffffff) return MBA_ERR_SIZE;
-            ph.nvb = (int)t;
-            off.push_back(ph.nvb);
+    const int nj = (int)S.sj.size();
+    // queues: jobs by descending load, each to the least loaded queue
+    std::vector<int> order(nj), queue(nj);
+    for (int q = 0; q < nj; ++q) order[q] = q;
+    auto load = [&](int q) { return (double)S.sj[q].nblk * S.sj[q].nblk * S.sj[q].ngrp; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return load(a) > load(b); });
+    double qload[PB_MAX_XCD] = {};
+    for (int q : order) {
+        int best = 0;
+        for (int x = 1; x < PB_MAX_XCD; ++x) best = qload[x] < qload[best] ? x : best;
+        queue[q] = best;
+        qload[best] += load(q);
+    }
+    size_t ntask = S.nflag;
+    if (ntask > 0x7fffffff) return MBA_ERR_SIZE;
+    size_t flag_at = 0;
+    S.flag0.assign(nj, 0);
+    for (int q = 0; q < nj; ++q) {
+        S.flag0[q] = flag_at;
+        flag_at += (size_t)2 * S.sj[q].ngrp * S.sj[q].nblk;
+    }
+    S.d_off.assign(PB_MAX_XCD + 1, 0);
+    if (!dry) {
+        S.tasks.reserve(ntask);
+        for (int x = 0; x < PB_MAX_XCD; ++x) {
+            S.d_off[x] = (int)S.tasks.size();
+            int nb_max = 0;
+            for (int q = 0; q < nj; ++q)
+                if (queue[q] == x) nb_max = S.sj[q].nblk > nb_max ? S.sj[q].nblk : nb_max;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int st = 0; st < nb_max; ++st)
+                    for (int q = 0; q < nj; ++q) {
+                        if (queue[q] != x || st >= S.sj[q].nblk) continue;
+                        const int i = pass == 0 ? st : S.sj[q].nblk - 1 - st;
+                        for (int g = 0; g < S.sj[q].ngrp; ++g) S.tasks.push_back(PbDfTask{q, i, (int16_t)pass, (int16_t)g});
+                    }
         }
-        phases.push_back(ph);
-        return MBA_OK;
-    };
-    rc = add_phase(PB_SPERM, 0, [](const PbSolveJob& d) { return (int64_t)d.nblk * d.ngrp; });
-    for (int s2 = 0; s2 < nblk_max && rc == MBA_OK; ++s2)
-        rc = add_phase(PB_SFWD, s2, [s2](const PbSolveJob& d) { return s2 < d.nblk ? (int64_t)(d.nblk - s2) * d.ngrp : 0; });
-    for (int s2 = nblk_max - 1; s2 >= 0 && rc == MBA_OK; --s2)
-        rc = add_phase(PB_SBWD, s2, [s2](const PbSolveJob& d) { return s2 < d.nblk ? (int64_t)(s2 + 1) * d.ngrp : 0; });
-    if (rc == MBA_OK) rc = add_phase(PB_SCOPY, 0, [](const PbSolveJob& d) { return (int64_t)d.nblk * d.ngrp; });
+        S.d_off[PB_MAX_XCD] = (int)S.tasks.size();
+    }
+    S.o_gerr = al256(PB_BAR_WORDS * sizeof(unsigned));
+    S.o_sj = S.o_gerr + 256;
+    S.o_dt = S.o_sj + al256((size_t)nj * sizeof(PbSolveJob));
+    S.o_doff = S.o_dt + al256(ntask * sizeof(PbDfTask));
+    S.o_flag = S.o_doff + al256((PB_MAX_XCD + 1) * sizeof(int));
+    S.total = S.o_flag + al256(S.nflag * sizeof(uint32_t));
+    return MBA_OK;
+}
+
+size_t pb_solve_scratch(const mba_lu_solve_job* jobs, int32_t n_jobs)
+{
+    PbSolvePlan S;
+    if (pb_solve_plan(jobs, n_jobs, true, S) != MBA_OK) return 0;
+    return S.total;
+}
+
+int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* scratch, size_t scratch_bytes,
+                                int32_t* err_out, hipStream_t st)
+{
+    int dev = 0, cus = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_lu_solve_dataflow, PB_THREADS, 0) !=
+            hipSuccess)
+        return MBA_ERR_LAUNCH;
+    if (occ < 1) return MBA_ERR_LAUNCH;
+    const int nwg = cus * (occ < 2 ? occ : 2);
+    PbSolvePlan S;
+    int rc = pb_solve_plan(jobs, njob, false, S);
     if (rc != MBA_OK) return rc;
-    const size_t o_gerr = al256(PB_BAR_WORDS * sizeof(unsigned));
-    const size_t o_ph = o_gerr + 256;
-    const size_t o_sj = o_ph + pb_bytes(phases);
-    const size_t o_off = o_sj + pb_bytes(sj);
-    const size_t o_xs = o_off + pb_bytes(off);
-    const size_t need = o_xs + (size_t)nwg * TS_BS * PB_SCW * sizeof(cpx<float>);
-    if (need > scratch_bytes) return MBA_ERR_ALLOC;
-    std::vector<unsigned char> blob(o_xs, 0);
-    std::memcpy(blob.data() + o_ph, phases.data(), phases.size() * sizeof(PbPhase));
-    std::memcpy(blob.data() + o_sj, sj.data(), sj.size() * sizeof(PbSolveJob));
-    std::memcpy(blob.data() + o_off, off.data(), off.size() * sizeof(int));
+    if (S.total > scratch_bytes) return MBA_ERR_ALLOC;
     char* base = (char*)scratch;
+    for (size_t q = 0; q < S.sj.size(); ++q) S.sj[q].flags = (uint32_t*)(base + S.o_flag) + S.flag0[q];
+    std::vector<unsigned char> blob(S.total, 0);  // barrier words, error word and flags start at zero
+    std::memcpy(blob.data() + S.o_sj, S.sj.data(), S.sj.size() * sizeof(PbSolveJob));
+    std::memcpy(blob.data() + S.o_dt, S.tasks.data(), S.tasks.size() * sizeof(PbDfTask));
+    std::memcpy(blob.data() + S.o_doff, S.d_off.data(), S.d_off.size() * sizeof(int));
     if (hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, st) != hipSuccess) return MBA_ERR_LAUNCH;
     if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;  // the host table is released on return
-    PbTables T{};
-    T.phases = (const PbPhase*)(base + o_ph);
-    T.nphase = (int)phases.size();
-    T.nmat = 0;
-    T.hjobs = nullptr;
-    T.h_off = nullptr;
-    T.sjobs = (const PbSolveJob*)(base + o_sj);
-    T.s_off = (const int*)(base + o_off);
-    T.xs = (cpx<float>*)(base + o_xs);
+    PbDfTables T{};
+    T.sjobs = (const PbSolveJob*)(base + S.o_sj);
+    T.dtasks = (const PbDfTask*)(base + S.o_dt);
+    T.d_off = (const int*)(base + S.o_doff);
+    T.err_out = err_out;
     T.barrier = (unsigned*)base;
-    T.gerr = (int*)(base + o_gerr);
-    T.tstamp = nullptr;
+    T.gerr = (int*)(base + S.o_gerr);
     void* kargs[] = {&T};
-    if ((rc = launch_resident((const void*)k_lu_batched_persistent, (unsigned)nwg, PB_THREADS, kargs, dyn, st)) !=
-        MBA_OK)
-        return rc;
-    return MBA_OK;
+    return launch_resident((const void*)k_lu_solve_dataflow, (unsigned)nwg, PB_THREADS, kargs, 0, st);
 }
 
 // Upper bound of the persistent schedule's scratch for n_mat matrices of order <= n_max.
@@ -5223,17 +5379,17 @@ int mba_lu_factor_batched_persistent(int prec, const mba_lu_matrix* mats, int32_
     });
 }
 
-size_t mba_lu_solve_batched_persistent_scratch_bytes(int32_t n_jobs, int64_t n_max)
+size_t mba_lu_solve_batched_persistent_scratch_bytes(const mba_lu_solve_job* jobs, int32_t n_jobs)
 {
-    if (n_jobs <= 0 || n_max <= 0) return 0;
-    const size_t a = pb_solve_scratch(n_jobs, n_max, 2 * MAX_G), b = mba_lu_solve_batched_scratch_bytes(n_jobs);
-    return a > b ? a : b;
+    if (!jobs || n_jobs <= 0) return 0;
+    const size_t a = pb_solve_scratch(jobs, n_jobs), b = mba_lu_solve_batched_scratch_bytes(n_jobs);
+    return a == 0 ? 0 : (a > b ? a : b);
 }
 
 int mba_lu_solve_batched_persistent(int prec, const mba_lu_solve_job* jobs, int32_t n_jobs, void* scratch,
-                                    size_t scratch_bytes, mba_stream_t stream)
+                                    size_t scratch_bytes, int32_t* err_out, mba_stream_t stream)
 {
-    if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
+    if (!jobs || !scratch || !err_out || n_jobs <= 0) return MBA_ERR_ARG;
     if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
     for (int q = 0; q < n_jobs; ++q) {
         const mba_lu_solve_job& J = jobs[q];
@@ -5241,7 +5397,10 @@ int mba_lu_solve_batched_persistent(int prec, const mba_lu_solve_job* jobs, int3
         if (J.n > 0 && (J.lda < J.n || J.ldb < J.nrhs)) return MBA_ERR_ARG;
     }
     return mba_guard([&] {
-        if (prec == MBA_F32) return lu_solve_batched_persistent(jobs, n_jobs, scratch, scratch_bytes, S(stream));
+        if (prec == MBA_F32)
+            return lu_solve_batched_persistent(jobs, n_jobs, scratch, scratch_bytes, err_out, S(stream));
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, S(stream), err_out, 1);
+        if (hipGetLastError() != hipSuccess) return (int)MBA_ERR_LAUNCH;
         return lu_solve_batched<double>(jobs, n_jobs, scratch, scratch_bytes, S(stream));
     });
 }

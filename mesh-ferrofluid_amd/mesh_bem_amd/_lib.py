@@ -104,8 +104,8 @@ SIGNATURES = {
     "mba_lu_solve_prepare_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
     "mba_lu_solve_batched_scratch_bytes": (_sz, [_i32]),
     "mba_lu_solve_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
-    "mba_lu_solve_batched_persistent_scratch_bytes": (_sz, [_i32, _i64]),
-    "mba_lu_solve_batched_persistent": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
+    "mba_lu_solve_batched_persistent_scratch_bytes": (_sz, [_vp, _i32]),
+    "mba_lu_solve_batched_persistent": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp, _vp]),
     "mba_lu_batched_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),
     "mba_lu_factor_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _i32p, _vp]),
     "mba_lu_batched_persistent_scratch_bytes": (_sz, [_c.c_int, _i32, _i64]),

@@ -222,15 +222,18 @@ class BatchedMoM:
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
               tol: float = 1e-8, maxit: int = 10, persistent: bool = True,
-              lu_tuner: Optional[BatchLuTuner] = None) -> BatchResult:
-        """Mixed precision for all meshes: the FP32 LUs in lock step
-        (``lu_factor_batched``: the panels of every matrix in shared
-        cooperative launches, the trailing products of every matrix in one
-        grouped launch per step and kind; GEMM tile variant of the largest size
-        bucket from the tuner), then refinement sweeps J += LU32 \\ (V - Z J).
-        A sweep is one grouped FP64 MFMA launch for the residuals of every mesh
-        still refining and two launches for their triangular solves with all
-        right-hand sides (``lu_solve_batched``); one host synchronisation then
+              lu_tuner: Optional[BatchLuTuner] = None, solve_persistent: bool = True) -> BatchResult:
+        """Mixed precision for all meshes: the FP32 LUs of the whole batch
+        (``lu_factor_batched``; ``persistent``: one persistent launch, its
+        schedule from ``lu_tuner`` when given; otherwise in lock step, the
+        trailing products of every matrix in one grouped launch per step and
+        kind, GEMM tile variant of the largest size bucket from the tuner),
+        then refinement sweeps J += LU32 \\ (V - Z J).  A sweep is one grouped
+        FP64 MFMA launch for the residuals of every mesh still refining and the
+        triangular solves of all their right-hand sides
+        (``lu_solve_batched``: by default one launch, a
+        dataflow of 64-row block tasks inside each XCD; ``solve_persistent``
+        False: the two-launch block walks); one host synchronisation then
         reads the largest relative correction of every mesh.  The correction of
         sweep s estimates the error before it, and its ratio to the previous
         correction the contraction per sweep; a mesh stops once the error left
@@ -250,7 +253,8 @@ class BatchedMoM:
             lu.check()
         n_mesh = len(Zs)
         V128s = [V.to(torch.complex128).contiguous() for V in Vs]
-        Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs])]
+        Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs],
+                                                                persistent=solve_persistent)]
         Rs = [torch.empty_like(V) for V in V128s]
         its, conv, prev = [0] * n_mesh, [False] * n_mesh, [None] * n_mesh
         active = list(range(n_mesh))
@@ -258,7 +262,8 @@ class BatchedMoM:
             for i in active:
                 Rs[i].copy_(V128s[i])
             gemm_grouped([(Zs[i], Js[i], Rs[i]) for i in active], alpha=-1.0, beta=1.0)
-            Ds = lu_solve_batched([lus[i] for i in active], [Rs[i].to(torch.complex64) for i in active])
+            Ds = lu_solve_batched([lus[i] for i in active], [Rs[i].to(torch.complex64) for i in active],
+                                  persistent=solve_persistent)
             rels = []
             for i, D in zip(active, Ds):
                 Js[i] += D

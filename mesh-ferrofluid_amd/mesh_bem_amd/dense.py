@@ -321,12 +321,11 @@ def lu_solve_batched(lus, Bs, persistent: bool = False):
     """X_i = A_i^{-1} B_i for prepared factorisations lus[i] (one dtype) and
     right-hand sides Bs[i] [n_i, nrhs_i].  Default: two launches, one
     workgroup per job and 16 right-hand sides walking its 64-row blocks.
-    ``persistent`` (complex64): all jobs in one persistent launch, eliminated
-    in 64-row block steps with every block row and group of 64 right-hand
-    sides of every job a task (``mba_lu_solve_batched_persistent``); measured
-    slower on the 64-mesh batch (a step costs a grid barrier plus two
-    dependent 64 x 64 products, ~57 steps per triangle).  Returns new tensors
-    (the inputs are not modified)."""
+    ``persistent`` (complex64): all jobs in one persistent launch, a
+    dataflow of 64-row block tasks inside each XCD
+    (``mba_lu_solve_batched_persistent``); its error word is checked here
+    (one synchronisation).  Returns new tensors (the inputs are not
+    modified)."""
     lib = _lib.load()
     n_job = len(lus)
     if n_job == 0:
@@ -344,10 +343,16 @@ def lu_solve_batched(lus, Bs, persistent: bool = False):
         Xs.append(X)
         keep.append(W)
     if persistent:
-        sb = lib.mba_lu_solve_batched_persistent_scratch_bytes(n_job, max(lu.n for lu in lus))
+        sb = lib.mba_lu_solve_batched_persistent_scratch_bytes(jobs, n_job)
+        if sb == 0:
+            raise ValueError("lu_solve_batched: too many blocks or right-hand sides for one launch")
         scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)
-        st = lib.mba_lu_solve_batched_persistent(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.stream())
+        err = torch.zeros(1, dtype=torch.int32, device=Xs[0].device)
+        st = lib.mba_lu_solve_batched_persistent(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.ptr(err),
+                                                 _lib.stream())
         _lib.check(st, "mba_lu_solve_batched_persistent")
+        if int(err.item()) != 0:
+            raise RuntimeError("mba_lu_solve_batched_persistent: a block wait timed out")
     else:
         sb = lib.mba_lu_solve_batched_scratch_bytes(n_job)
         scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)

@@ -75,8 +75,9 @@ def test_lu_factor_batched_matches_single(dev, persistent):
 
 @pytest.mark.parametrize("prec,persistent", [("f32", True), ("f32", False), ("f64", True)])
 def test_lu_solve_batched_matches_single(dev, prec, persistent):
-    """Batched triangular solves (one persistent launch in 64-row block steps,
-    or two launches walking the blocks) of several factorisations with ragged
+    """Batched triangular solves (one persistent launch, a dataflow of 64-row
+    block tasks per XCD, or two launches walking the blocks) of several
+    factorisations with ragged
     sizes (not multiples of the 64-row block) and right-hand-side counts (not
     multiples of the 16- or 64-column groups) agree with the per-matrix solve."""
     from mesh_bem_amd.dense import LUFactor, lu_solve_batched
@@ -106,6 +107,41 @@ def test_lu_solve_batched_matches_single(dev, prec, persistent):
         r = float((A.to(torch.complex128) @ X.to(torch.complex128) - B.to(torch.complex128)).norm() / B.norm())
         print(f"  n={A.shape[0]} nrhs={B.shape[1]}: vs single {e:.2e}, residual {r:.2e}")
         assert e < bar and r < bar
+
+
+def test_lu_solve_batched_persistent_many_jobs(dev):
+    """The persistent dataflow solve on a batch larger than its eight XCD
+    queues (48 jobs, 40-2100 rows, 1-200 right-hand sides, so up to four
+    groups of 64 per job and chains of 33 blocks) agrees with the two-launch
+    block walks and leaves a small residual; a second call on fresh scratch
+    gives identical results (the task order is fixed, each block's sum is
+    taken in the same order)."""
+    from mesh_bem_amd.dense import LUFactor, lu_solve_batched
+
+    print("persistent dataflow solves, 48 jobs", flush=True)
+    rng = np.random.default_rng(11)
+    mats, lus, Bs = [], [], []
+    for q in range(48):
+        n = int(rng.integers(40, 2101))
+        r = int(rng.integers(1, 201))
+        A = torch.tensor(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * n ** 0.5 * np.eye(n),
+                         dtype=torch.complex64, device=dev)
+        mats.append(A)
+        lus.append(LUFactor(A.clone()))
+        Bs.append(torch.tensor(rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r)),
+                               dtype=torch.complex64, device=dev))
+    Xw = lu_solve_batched(lus, Bs, persistent=False)
+    Xp = lu_solve_batched(lus, Bs, persistent=True)
+    Xp2 = lu_solve_batched(lus, Bs, persistent=True)
+    torch.cuda.synchronize()
+    worst_e, worst_r = 0.0, 0.0
+    for A, B, X0, X1, X2 in zip(mats, Bs, Xw, Xp, Xp2):
+        assert torch.equal(X1, X2)
+        e = float((X1 - X0).norm() / X0.norm())
+        r = float((A.to(torch.complex128) @ X1.to(torch.complex128) - B.to(torch.complex128)).norm() / B.norm())
+        worst_e, worst_r = max(worst_e, e), max(worst_r, r)
+    print(f"  worst vs walks {worst_e:.2e}, worst residual {worst_r:.2e}")
+    assert worst_e < 1e-5 and worst_r < 1e-5
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
