@@ -4365,16 +4365,17 @@ __device__ __noinline__ void pb_phase_h3(const PbTables& T, const PbPhase& ph, i
 // only on tasks with lower tickets (forward by ascending block, then backward
 // by descending block), taken by running workgroups, so every wait ends.  All
 // data a job's tasks exchange stays in one XCD's L2: a producer drains its
-// stores before raising the flag, a consumer invalidates its CU's L1 after
-// seeing it.  The invalidate is the agent-scope one (`buffer_inv sc1`): the
-// workgroup-scope form (`sc0`) leaves the L1 alone unless a workgroup's waves
-// are split over CUs, and blocks read through a stale L1 line came out wrong.
+// stores before raising the flag, a consumer reads the blocks of other tasks
+// with agent-scope loads, which miss its CU's L1 (a line of a block can sit
+// in that L1 from before the block was final: the rows around a block
+// boundary share lines).  Invalidating the L1 instead needs the agent-scope
+// `buffer_inv sc1` (the workgroup-scope `sc0` leaves the L1 alone unless a
+// workgroup's waves are split over CUs), which also drops L2 lines.
 
 // Wait until flag j is up, then return the end of the run of raised flags
 // j, j + step, ... (at most 64, stopping before `end`); j when the wait
 // failed.  Wave 0 polls 64 flags per load; every thread's earlier stores are
-// drained first and the CU's L1 is invalidated after, so the blocks of the
-// run can be read and the caller's own stores are visible to all its threads.
+// drained first, so the caller's own stores are visible to all its threads.
 __device__ __forceinline__ int pb_flag_run(const uint32_t* f, int j, int end, int step, int* gerr)
 {
     __shared__ int s_k;
@@ -4402,16 +4403,15 @@ __device__ __forceinline__ int pb_flag_run(const uint32_t* f, int j, int end, in
         if (threadIdx.x == 0) s_k = k;
     }
     __syncthreads();
-    asm volatile("buffer_inv sc1" ::: "memory");  // this CU's L1: the blocks were written on other CUs
     return s_k;
 }
 
 // this workgroup's global writes are complete and visible to all its threads
+// (one CU: its L1 sees its own stores)
 __device__ __forceinline__ void pb_local_fence()
 {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    asm volatile("buffer_inv sc1" ::: "memory");
 }
 
 __device__ __forceinline__ void pb_flag_raise(uint32_t* f)
@@ -4421,14 +4421,220 @@ __device__ __forceinline__ void pb_flag_raise(uint32_t* f)
     if (threadIdx.x == 0) __hip_atomic_store(f, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One tile product of the solves, not inlined: its registers are its own
-// (inlined into the task loop, the product and the loop's 64-bit indices
-// spilled).
+// C = alpha A B + beta C for one 64 x 64 tile (M, N <= 64, any K; B written
+// by this workgroup or before the launch), 3M product on FP32 MFMA: four
+// waves, each a 32 x 32 quarter.  Not inlined: its registers are its own.
+constexpr int DF_LDS_FLOATS = 8 * G32_BK * 64;
 __device__ __noinline__ void pb_df_product(int64_t M, int64_t N, int64_t K, float alpha, const cpx<float>* A,
                                            int64_t lda, const cpx<float>* B, int64_t ldb, float beta, cpx<float>* C,
                                            int64_t ldc, float* lds)
 {
-    cgemm3m_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0, 0, lds);
+    float(*Ar)[G32_BK][64] = (float(*)[G32_BK][64])lds;
+    float(*Ai)[G32_BK][64] = (float(*)[G32_BK][64])(lds + 2 * G32_BK * 64);
+    float(*Br)[G32_BK][64] = (float(*)[G32_BK][64])(lds + 4 * G32_BK * 64);
+    float(*Bi)[G32_BK][64] = (float(*)[G32_BK][64])(lds + 6 * G32_BK * 64);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // A: 64 rows x 16 k: thread -> row tid/4, k (tid&3)*4 .. +4
+    const int ar = tid >> 2, ak = (tid & 3) * 4;
+    // B: 16 k x 64 columns: thread -> k tid/16, columns (tid&15)*4 .. +4
+    const int bkr = tid >> 4, bc = (tid & 15) * 4;
+    float2 ra[4], rb[4];
+    auto load_chunk = [&](int64_t k0) {
+        const bool rowok = ar < M;
+        const float2* pa = (const float2*)(A + (rowok ? ar : 0) * lda);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t gk = k0 + ak + q;
+            ra[q] = (rowok && gk < K) ? pa[gk] : make_float2(0.f, 0.f);
+        }
+        const int64_t gk = k0 + bkr;
+        const bool kok = gk < K;
+        const cpx<float>* pb = B + (kok ? gk : 0) * ldb;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int gj = bc + q;
+            rb[q] = (kok && gj < N) ? *(const float2*)(pb + gj) : make_float2(0.f, 0.f);
+        }
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Ar[buf][ak + q][ar] = ra[q].x;
+            Ai[buf][ak + q][ar] = ra[q].y;
+        }
+        *(float4*)&Br[buf][bkr][bc] = make_float4(rb[0].x, rb[1].x, rb[2].x, rb[3].x);
+        *(float4*)&Bi[buf][bkr][bc] = make_float4(rb[0].y, rb[1].y, rb[2].y, rb[3].y);
+    };
+    floatx16 t1, t2, t3;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t1[r] = t2[r] = t3[r] = 0.f;
+    const int hi = lane >> 5, jj = lane & 31;
+    const int nchunks = (int)((K + G32_BK - 1) / G32_BK);
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunks) load_chunk((int64_t)(c + 1) * G32_BK);
+#pragma unroll
+        for (int kk = 0; kk < G32_BK; kk += 2) {
+            const int k = kk + hi;
+            const float br = Br[buf][k][wn * 32 + jj], bi = Bi[buf][k][wn * 32 + jj];
+            const float a_r = Ar[buf][k][wm * 32 + jj], a_i = Ai[buf][k][wm * 32 + jj];
+            t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, t1, 0, 0, 0);
+            t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, bi, t2, 0, 0, 0);
+            t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r + a_i, br + bi, t3, 0, 0, 0);
+        }
+        if (c + 1 < nchunks) {
+            __syncthreads();
+            store_chunk(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    // epilogue: register r of lane -> row 8*(r/4) + 4*hi + r%4, column jj
+    const int gj = wn * 32 + jj;
+    if (gj < N) {
+        float2 cv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gi = wm * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            cv[r] = (beta != 0.f && gi < M) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gi = wm * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            if (gi < M) {
+                float re = alpha * (t1[r] - t2[r]);
+                float im = alpha * (t3[r] - t1[r] - t2[r]);
+                if (beta != 0.f) {
+                    re += beta * cv[r].x;
+                    im += beta * cv[r].y;
+                }
+                *(float2*)(C + gi * ldc + gj) = make_float2(re, im);
+            }
+        }
+    }
+}
+
+// X -= A_i,j X_j over a task's finished blocks, streamed: one product whose
+// k range runs over the blocks j = j0, j0 + step, ... (nb of them; A_i,j at
+// A + r_j, block j of the source at src + r_j ld_src), each block's loads
+// issued once its flag is up.  The k order is the same whatever the timing,
+// so the result is reproducible bit for bit.  The source blocks were written
+// by other CUs during the launch: agent-scope loads, past this CU's L1.
+// Tile 64 x 64 (M, N <= 64), 3M on FP32 MFMA, four waves of 32 x 32.
+// Returns false when a wait failed (nothing written).
+__device__ __noinline__ bool pb_df_stream(const cpx<float>* A, int64_t lda, const cpx<float>* src, int64_t lds_,
+                                          cpx<float>* X, int64_t ldx, int64_t M, int64_t N, int nb, int j0,
+                                          int step, int64_t n, const uint32_t* fl, int* gerr, float* lds)
+{
+    float(*Ar)[G32_BK][64] = (float(*)[G32_BK][64])lds;
+    float(*Ai)[G32_BK][64] = (float(*)[G32_BK][64])(lds + 2 * G32_BK * 64);
+    float(*Br)[G32_BK][64] = (float(*)[G32_BK][64])(lds + 4 * G32_BK * 64);
+    float(*Bi)[G32_BK][64] = (float(*)[G32_BK][64])(lds + 6 * G32_BK * 64);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int ar = tid >> 2, ak = (tid & 3) * 4;
+    const int bkr = tid >> 4, bc = (tid & 15) * 4;
+    constexpr int CPB = TS_BS / G32_BK;  // chunks per block
+    float2 ra[4], rb[4];
+    int ready = 0;  // blocks of the sequence whose flags were seen
+    // chunk q: block s = q / CPB of the sequence, rows [k0, k0 + 16) of it
+    auto load_chunk = [&](int q) {
+        const int j = j0 + step * (q / CPB);
+        const int64_t r_j = (int64_t)j * TS_BS, kend = (r_j + TS_BS) < n ? (r_j + TS_BS) : n;
+        const int64_t k0 = r_j + (int64_t)(q % CPB) * G32_BK;
+        const bool rowok = ar < M;
+        const float2* pa = (const float2*)(A + (rowok ? ar : 0) * lda);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t gk = k0 + ak + u;
+            ra[u] = (rowok && gk < kend) ? pa[gk] : make_float2(0.f, 0.f);
+        }
+        const int64_t gk = k0 + bkr;
+        const bool kok = gk < kend;
+        const cpx<float>* pb = src + (kok ? gk : 0) * lds_;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int gj = bc + u;
+            if (kok && gj < N) {
+                const uint64_t w =
+                    __hip_atomic_load((const uint64_t*)(pb + gj), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                rb[u] = make_float2(__uint_as_float((uint32_t)w), __uint_as_float((uint32_t)(w >> 32)));
+            } else {
+                rb[u] = make_float2(0.f, 0.f);
+            }
+        }
+    };
+    // before the first chunk of block s: wait for its flag (and learn how
+    // many after it are up already); uniform over the workgroup
+    auto ensure = [&](int q) -> bool {
+        if (q % CPB != 0 || q / CPB < ready) return true;
+        const int s = q / CPB, j = j0 + step * s;
+        const int end = j0 + step * nb;
+        const int k = pb_flag_run(fl, j, end, step, gerr);
+        if (k == j) return false;
+        ready = s + (k - j) * step;
+        return true;
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            Ar[buf][ak + u][ar] = ra[u].x;
+            Ai[buf][ak + u][ar] = ra[u].y;
+        }
+        *(float4*)&Br[buf][bkr][bc] = make_float4(rb[0].x, rb[1].x, rb[2].x, rb[3].x);
+        *(float4*)&Bi[buf][bkr][bc] = make_float4(rb[0].y, rb[1].y, rb[2].y, rb[3].y);
+    };
+    floatx16 t1, t2, t3;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) t1[r] = t2[r] = t3[r] = 0.f;
+    const int hi = lane >> 5, jj = lane & 31;
+    const int nchunks = nb * CPB;
+    if (!ensure(0)) return false;
+    load_chunk(0);
+    store_chunk(0);
+    __syncthreads();
+    for (int c = 0; c < nchunks; ++c) {
+        const int buf = c & 1;
+        if (c + 1 < nchunks) {
+            if (!ensure(c + 1)) return false;
+            load_chunk(c + 1);
+        }
+#pragma unroll
+        for (int kk = 0; kk < G32_BK; kk += 2) {
+            const int k = kk + hi;
+            const float br = Br[buf][k][wn * 32 + jj], bi = Bi[buf][k][wn * 32 + jj];
+            const float a_r = Ar[buf][k][wm * 32 + jj], a_i = Ai[buf][k][wm * 32 + jj];
+            t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, t1, 0, 0, 0);
+            t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, bi, t2, 0, 0, 0);
+            t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r + a_i, br + bi, t3, 0, 0, 0);
+        }
+        if (c + 1 < nchunks) {
+            __syncthreads();
+            store_chunk(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    // X -= product; register r of lane -> row 8*(r/4) + 4*hi + r%4, column jj
+    const int gj = wn * 32 + jj;
+    if (gj < N) {
+        float2 cv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gi = wm * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            cv[r] = gi < M ? *(const float2*)(X + gi * ldx + gj) : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int gi = wm * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+            if (gi < M)
+                *(float2*)(X + gi * ldx + gj) =
+                    make_float2(cv[r].x - (t1[r] - t2[r]), cv[r].y - (t3[r] - t1[r] - t2[r]));
+        }
+    }
+    return true;
 }
 
 __device__ __forceinline__ void pb_df_task(const PbSolveJob& J, const PbDfTask& t, float* lds, int* gerr)
@@ -4455,26 +4661,18 @@ __device__ __forceinline__ void pb_df_task(const PbSolveJob& J, const PbDfTask& 
     } else {
         // the job's forward pass is done once its last block is
         if (pb_flag_run(ffwd, J.nblk - 1, J.nblk, 1, gerr) == J.nblk - 1) return;
+        // x_i was written on another CU: agent-scope loads (past this CU's L1)
         for (int64_t e = threadIdx.x; e < bs_i * cw; e += PB_THREADS) {
             const int64_t r = e / cw, c = e % cw;
-            Bi[r * J.ldb + c] = Wi[r * J.nrhs + c];
+            const uint64_t w = __hip_atomic_load((const uint64_t*)(Wi + r * J.nrhs + c), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            Bi[r * J.ldb + c] = cpx<float>{__uint_as_float((uint32_t)w), __uint_as_float((uint32_t)(w >> 32))};
         }
     }
-    // X -= A_ij (block j) over the finished blocks j, run by run of raised
-    // flags.  Each product's epilogue reads and writes the same elements of X
-    // in the same threads, so consecutive products only need the barrier that
-    // frees the LDS.
-    const int step = fwd ? 1 : -1;
-    for (int j = fwd ? 0 : J.nblk - 1; j != i;) {
-        const int k = pb_flag_run(fl, j, i, step, gerr);
-        if (k == j) return;
-        for (; j != k; j += step) {
-            const int64_t r_j = (int64_t)j * TS_BS, bs_j = (J.n - r_j) < TS_BS ? (J.n - r_j) : TS_BS;
-            pb_df_product(bs_i, cw, bs_j, -1.f, J.lu + r_i * J.lda + r_j, J.lda, src + r_j * ldx + c0, ldx, 1.f, X,
-                          ldx, lds);
-            __syncthreads();
-        }
-    }
+    const int nb = fwd ? i : J.nblk - 1 - i;
+    if (nb > 0 && !pb_df_stream(J.lu + r_i * J.lda, J.lda, src + c0, ldx, X, ldx, bs_i, cw, nb, fwd ? 0 : J.nblk - 1,
+                                fwd ? 1 : -1, J.n, fl, gerr, lds))
+        return;
     pb_local_fence();
     // in place: the single tile reads all of X before its epilogue writes it
     pb_df_product(bs_i, cw, bs_i, 1.f, inv, TS_BS, X, ldx, 0.f, X, ldx, lds);
@@ -4535,7 +4733,7 @@ __device__ __forceinline__ void pb_df_serve(const PbDfTables& T, const PbBarrier
 // All workgroups resident (they register their XCDs at the start barrier).
 __global__ __launch_bounds__(PB_THREADS, 2) void k_lu_solve_dataflow(PbDfTables T)
 {
-    __shared__ alignas(16) float lds[G3M_LDS_FLOATS];
+    __shared__ alignas(16) float lds[DF_LDS_FLOATS];
     PbBarrier B;
     pb_barrier_init(B, T.barrier, gridDim.x, T.gerr);
     pb_df_serve(T, B, T.barrier, lds);
@@ -5170,7 +5368,7 @@ int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* sc
             hipSuccess)
         return MBA_ERR_LAUNCH;
     if (occ < 1) return MBA_ERR_LAUNCH;
-    const int nwg = cus * (occ < 2 ? occ : 2);
+    const int nwg = cus * (occ < 4 ? occ : 4);
     PbSolvePlan S;
     int rc = pb_solve_plan(jobs, njob, false, S);
     if (rc != MBA_OK) return rc;
