@@ -236,9 +236,10 @@ class BatchedMoM:
         False: the two-launch block walks); one host synchronisation then
         reads the largest relative correction of every mesh.  The correction of
         sweep s estimates the error before it, and its ratio to the previous
-        correction the contraction per sweep; a mesh stops once the error left
-        after the sweep (correction x contraction; the correction alone after
-        the first sweep) is at most ``tol``."""
+        correction the contraction per sweep (the first solve is the same
+        iteration's step from J = 0, a correction of relative size 1); a mesh
+        stops once the error left after the sweep (correction x contraction)
+        is at most ``tol``."""
         lib = _lib.load()
         if tuner is not None:
             tuner.apply(max(Z.shape[0] for Z in Zs))
@@ -256,7 +257,9 @@ class BatchedMoM:
         Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs],
                                                                 persistent=solve_persistent)]
         Rs = [torch.empty_like(V) for V in V128s]
-        its, conv, prev = [0] * n_mesh, [False] * n_mesh, [None] * n_mesh
+        # the first solve is the iteration's first step from J = 0: a
+        # correction of relative size 1
+        its, conv, prev = [0] * n_mesh, [False] * n_mesh, [1.0] * n_mesh
         active = list(range(n_mesh))
         while active:
             for i in active:
@@ -273,7 +276,7 @@ class BatchedMoM:
                 its[i] += 1
                 # error left after this sweep: its correction r times the
                 # observed contraction r / (previous correction)
-                est = r if prev[i] is None else min(r, r * r / max(prev[i], 1e-300))
+                est = min(r, r * r / max(prev[i], 1e-300))
                 prev[i] = r
                 conv[i] = est <= tol
             active = [i for i in active if not conv[i] and its[i] < maxit]

@@ -412,9 +412,14 @@ def refine_solve(Z64: torch.Tensor, V: torch.Tensor, lu32: Optional[LUFactor] = 
     """Mixed-precision solve of Z64 J = V to FP64 accuracy.
 
     J0 = LU32 \\ V; repeat r = V - Z64 J (FP64), d = LU32 \\ r, J += d until
-    ||d|| <= tol ||J||.  If the correction stops shrinking (|d_k| > 0.5 |d_{k-1}|
-    twice) or maxit is reached, falls back to an FP64 LU of Z64 when
-    ``fallback_fp64`` is set.  Returns (J complex128, RefineInfo)."""
+    the error left is at most ``tol``: the relative correction c_k =
+    ||d_k|| / ||J|| estimates the error before sweep k, and c_k / c_{k-1} the
+    contraction of one sweep, so the error after it is about c_k^2 / c_{k-1}
+    (J0 is the same iteration's first step from J = 0, a correction of
+    relative size c_0 = 1).  If the correction stops shrinking
+    (|d_k| > 0.5 |d_{k-1}| twice) or maxit is reached, falls back to an FP64
+    LU of Z64 when ``fallback_fp64`` is set.  Returns (J complex128,
+    RefineInfo)."""
     if lu32 is None:
         lu32 = LUFactor(to_c64(Z64), overwrite=True)
     V128 = V.to(torch.complex128)
@@ -426,8 +431,9 @@ def refine_solve(Z64: torch.Tensor, V: torch.Tensor, lu32: Optional[LUFactor] = 
         d = lu32.solve(r.to(torch.complex64)).to(torch.complex128)
         J = J + d
         rel = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(J))
+        prev = corr[-1] if corr else 1.0
         corr.append(rel)
-        if rel <= tol:
+        if min(rel, rel * rel / max(prev, 1e-300)) <= tol:
             return J, RefineInfo(it, True, corr)
         if len(corr) >= 2 and corr[-1] > 0.5 * corr[-2]:
             stalls += 1
