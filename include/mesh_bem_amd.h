@@ -462,6 +462,40 @@ int mba_gemv(int prec, const void *a, int64_t n, int64_t lda, const void *x, voi
 int mba_convert_c128_c64(const void *src, int64_t m, int64_t n, int64_t lds, void *dst,
                          int64_t ldd, mba_stream_t stream);
 
+/* Many complex128 -> complex64 conversions (contiguous arrays of `count`
+ * complex elements) in one launch; scratch:
+ * mba_convert_c128_c64_batched_scratch_bytes(n_jobs).  Part of the batched
+ * mixed-precision refinement; no counterpart in the reference. */
+typedef struct {
+    const void *src; /* complex128 */
+    void *dst;       /* complex64 */
+    int64_t count;
+} mba_convert_job;
+size_t mba_convert_c128_c64_batched_scratch_bytes(int32_t n_jobs);
+int mba_convert_c128_c64_batched(const mba_convert_job *jobs, int32_t n_jobs, void *scratch,
+                                 size_t scratch_bytes, mba_stream_t stream);
+
+/* Refinement update of many systems in one pass: J += D (complex128 J
+ * [n, ldj], complex64 D [n, ldd], nrhs columns each), and ratio_out[q]
+ * (device double) = max over the columns c of job q of |D_c| / |J_c| after
+ * the update (2-norms; +inf for a zero J column with a nonzero D column, 0
+ * when both are zero), summed in a fixed order (reproducible).  Scratch:
+ * mba_refine_update_batched_scratch_bytes(jobs, n_jobs) (reads nrhs of each
+ * job).  The step J += LU32 \ (V - Z J) of the reference-free mixed-precision
+ * refinement that replaces torch.linalg.solve
+ * (src/mesh_BEM/mesh_BEM_solver_3d.py:174) for many meshes; no counterpart
+ * in the reference. */
+typedef struct {
+    void *j; /* complex128 */
+    int64_t ldj;
+    const void *d; /* complex64 */
+    int64_t ldd;
+    int64_t n, nrhs;
+} mba_refine_job;
+size_t mba_refine_update_batched_scratch_bytes(const mba_refine_job *jobs, int32_t n_jobs);
+int mba_refine_update_batched(const mba_refine_job *jobs, int32_t n_jobs, double *ratio_out, void *scratch,
+                              size_t scratch_bytes, mba_stream_t stream);
+
 /* ================================================================= Laplace */
 
 /* Unpacked Laplace single (layer = 1) or double (layer = 2) layer with the

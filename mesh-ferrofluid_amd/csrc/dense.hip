@@ -21,6 +21,7 @@
 #include "gemm_h3_dev.hpp"
 #include "mba_common.hpp"
 #include "mba_guard.hpp"
+#include "upload.hpp"
 
 namespace {
 
@@ -2912,11 +2913,9 @@ int lu_prepare_batched(const mba_lu_prepare_job* jobs, int njob, void* scratch, 
     if (scratch_bytes < o_w + (size_t)(njob + 1) * sizeof(int)) return MBA_ERR_ALLOC;
     PrepJobDev* dj = (PrepJobDev*)scratch;
     int* dw = (int*)((char*)scratch + o_w);
-    if (hipMemcpyAsync(dj, jobs, (size_t)njob * sizeof(PrepJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dw, wg.data(), (size_t)(njob + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+    if (mba_host::upload_tables(dj, jobs, (size_t)njob * sizeof(PrepJobDev), st) != 0 ||
+        mba_host::upload_tables(dw, wg.data(), (size_t)(njob + 1) * sizeof(int), st) != 0)
         return MBA_ERR_LAUNCH;
-    // the host tables are released on return: wait for the copies
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
     const size_t shm = 2 * (size_t)TS_BS * (TS_BS + 1) * sizeof(cpx<T>);
     static std::once_flag attr_once;  // per element type (template instance)
     std::call_once(attr_once, [&] {
@@ -3316,11 +3315,9 @@ int lu_solve_batched(const mba_lu_solve_job* jobs, int njob, void* scratch, size
     char* base = (char*)scratch;
     TsJobDev* dj = (TsJobDev*)base;
     int* dw = (int*)(base + al256((size_t)njob * sizeof(TsJobDev)));
-    if (hipMemcpyAsync(dj, jobs, (size_t)njob * sizeof(TsJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dw, wg.data(), (size_t)(njob + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+    if (mba_host::upload_tables(dj, jobs, (size_t)njob * sizeof(TsJobDev), st) != 0 ||
+        mba_host::upload_tables(dw, wg.data(), (size_t)(njob + 1) * sizeof(int), st) != 0)
         return MBA_ERR_LAUNCH;
-    // the host tables are released on return: wait for the copies
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
     if (sizeof(T) == 4) {
         const unsigned blocks = (unsigned)(((wg[njob] + NUM_XCD - 1) / NUM_XCD) * NUM_XCD);
         hipLaunchKernelGGL(k_trisolve_mfma<true>, dim3(blocks), dim3(256), 0, st, dj, dw, njob);
@@ -3575,6 +3572,101 @@ __global__ void k_c128_to_c64(const cpx<double>* __restrict__ s, int64_t m, int6
     store_cpx(d + i * ldd + j, (float)v.re, (float)v.im);
 }
 
+// ---- batched refinement helpers ---------------------------------------------
+struct ConvJobDev {
+    const cpx<double>* src;
+    cpx<float>* dst;
+    int64_t count;
+};
+
+// grid (x: blocks per job, y: job); each block strides over its job's elements
+__global__ void k_c128_to_c64_batched(const ConvJobDev* __restrict__ jobs)
+{
+    const ConvJobDev J = jobs[blockIdx.y];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < J.count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const cpx<double> v = J.src[i];
+        store_cpx(J.dst + i, (float)v.re, (float)v.im);
+    }
+}
+
+struct RefJobDev {
+    cpx<double>* j;
+    int64_t ldj;
+    const cpx<float>* d;
+    int64_t ldd;
+    int64_t n, nrhs;
+};
+constexpr int RF_SLABS = 8;    // row slabs per job (partial sums)
+constexpr int RF_COLS = 64;    // columns per workgroup
+constexpr int RF_THREADS = 256;
+
+// grid (x: 64-column group, y: row slab, z: job).  J += D over the slab's rows
+// and the partial sums |D_c|^2, |J_c|^2 (FP64) of its columns: thread (p, c)
+// takes rows p, p + 4, ... of column c; the four row phases are added in a
+// fixed order.  part: [job][slab][ncol_max] pairs.
+__global__ __launch_bounds__(RF_THREADS) void k_refine_partial(const RefJobDev* __restrict__ jobs, int ncol_max,
+                                                                double2* __restrict__ part)
+{
+    __shared__ double2 s_p[RF_THREADS / RF_COLS][RF_COLS];
+    const RefJobDev J = jobs[blockIdx.z];
+    const int tc = threadIdx.x % RF_COLS, tp = threadIdx.x / RF_COLS;
+    const int64_t c = (int64_t)blockIdx.x * RF_COLS + tc;
+    const int64_t rows = (J.n + RF_SLABS - 1) / RF_SLABS;
+    const int64_t r0 = (int64_t)blockIdx.y * rows, r1 = (r0 + rows) < J.n ? (r0 + rows) : J.n;
+    double dd = 0.0, jj = 0.0;
+    if (c < J.nrhs) {
+        for (int64_t r = r0 + tp; r < r1; r += RF_THREADS / RF_COLS) {
+            const cpx<float> d = J.d[r * J.ldd + c];
+            cpx<double> v = J.j[r * J.ldj + c];
+            v.re += (double)d.re;
+            v.im += (double)d.im;
+            J.j[r * J.ldj + c] = v;
+            dd += (double)d.re * d.re + (double)d.im * d.im;
+            jj += v.re * v.re + v.im * v.im;
+        }
+    }
+    s_p[tp][tc] = make_double2(dd, jj);
+    __syncthreads();
+    if (tp == 0 && c < J.nrhs) {
+        double2 a = s_p[0][tc];
+#pragma unroll
+        for (int q = 1; q < RF_THREADS / RF_COLS; ++q) {
+            a.x += s_p[q][tc].x;
+            a.y += s_p[q][tc].y;
+        }
+        part[((int64_t)blockIdx.z * RF_SLABS + blockIdx.y) * ncol_max + c] = a;
+    }
+}
+
+// one workgroup per job: slabs added in order per column, the ratio, the max
+__global__ __launch_bounds__(RF_THREADS) void k_refine_ratio(const RefJobDev* __restrict__ jobs, int ncol_max,
+                                                              const double2* __restrict__ part,
+                                                              double* __restrict__ ratio_out)
+{
+    __shared__ double s_m[RF_THREADS];
+    const RefJobDev J = jobs[blockIdx.x];
+    double m = 0.0;
+    for (int64_t c = threadIdx.x; c < J.nrhs; c += RF_THREADS) {
+        double dd = 0.0, jj = 0.0;
+        for (int s = 0; s < RF_SLABS; ++s) {
+            const double2 a = part[((int64_t)blockIdx.x * RF_SLABS + s) * ncol_max + c];
+            dd += a.x;
+            jj += a.y;
+        }
+        const double r = jj > 0.0 ? sqrt(dd / jj) : (dd > 0.0 ? __builtin_inf() : 0.0);
+        m = r > m ? r : m;
+    }
+    s_m[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = RF_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_m[threadIdx.x] = s_m[threadIdx.x + w] > s_m[threadIdx.x] ? s_m[threadIdx.x + w]
+                                                                                        : s_m[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ratio_out[blockIdx.x] = s_m[0];
+}
+
 // ============================================================== batched LU
 // Several matrices factored in lock step: the same super-panel schedule as
 // lu_factor, with each panel phase's panels of all matrices in shared
@@ -3817,7 +3909,7 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
     GemmJobDev* d_gj = (GemmJobDev*)(base + o_gj);
     int* d_gtoff = (int*)(base + o_gtoff);
     auto up = [&](void* dst, const void* src, size_t bytes) {
-        return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+        return mba_host::upload_tables(dst, src, bytes, st) == 0;
     };
     if (!up(d_args, args.data(), args.size() * sizeof(PanelArgs<T>)) ||
         !up(d_woff, woffs.data(), woffs.size() * sizeof(int)) ||
@@ -3828,8 +3920,6 @@ int lu_factor_batched(const mba_lu_matrix* mats, int nmat, void* scratch, size_t
         !up(d_gj, gjobs.data(), gjobs.size() * sizeof(GemmJobDev)) ||
         !up(d_gtoff, gtoff.data(), gtoff.size() * sizeof(int)))
         return MBA_ERR_LAUNCH;
-    // the host tables are released on return: wait for the copies
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
     for (int i = 0; i < nmat; ++i) {
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ws[i].err, 32);
         hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((ws[i].msg_words + 255) / 256)), dim3(256), 0, st,
@@ -5206,9 +5296,7 @@ int lu_factor_batched_persistent(const mba_lu_matrix* mats, int nmat, const mba_
     put(o_hj, S.hjobs);
     put(o_ho, S.h_off);
     char* base = (char*)scratch;
-    if (hipMemcpyAsync(base, blob.data(), need, hipMemcpyHostToDevice, st) != hipSuccess) return MBA_ERR_LAUNCH;
-    // the host table is released on return: wait for the copy
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
+    if (mba_host::upload_tables(base, blob.data(), need, st) != 0) return MBA_ERR_LAUNCH;
     PbTables T;
     T.phases = (const PbPhase*)(base + o_ph);
     T.nphase = (int)S.phases.size();
@@ -5379,8 +5467,7 @@ int lu_solve_batched_persistent(const mba_lu_solve_job* jobs, int njob, void* sc
     std::memcpy(blob.data() + S.o_sj, S.sj.data(), S.sj.size() * sizeof(PbSolveJob));
     std::memcpy(blob.data() + S.o_dt, S.tasks.data(), S.tasks.size() * sizeof(PbDfTask));
     std::memcpy(blob.data() + S.o_doff, S.d_off.data(), S.d_off.size() * sizeof(int));
-    if (hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, st) != hipSuccess) return MBA_ERR_LAUNCH;
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;  // the host table is released on return
+    if (mba_host::upload_tables(base, blob.data(), blob.size(), st) != 0) return MBA_ERR_LAUNCH;
     PbDfTables T{};
     T.sjobs = (const PbSolveJob*)(base + S.o_sj);
     T.dtasks = (const PbDfTask*)(base + S.o_dt);
@@ -5506,11 +5593,9 @@ int mba_gemm_grouped(int prec, const mba_gemm_job* jobs, int32_t n_jobs, void* s
         char* base = (char*)scratch;
         GemmJobDev* dj = (GemmJobDev*)base;
         int* dt = (int*)(base + al256((size_t)n_jobs * sizeof(GemmJobDev)));
-        if (hipMemcpyAsync(dj, jobs, (size_t)n_jobs * sizeof(GemmJobDev), hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(dt, toff.data(), (size_t)(n_jobs + 1) * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+        if (mba_host::upload_tables(dj, jobs, (size_t)n_jobs * sizeof(GemmJobDev), st) != 0 ||
+            mba_host::upload_tables(dt, toff.data(), (size_t)(n_jobs + 1) * sizeof(int), st) != 0)
             return MBA_ERR_LAUNCH;
-        // the host tables are released on return: wait for the copies
-        if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
         return launch_grouped(prec, prec == MBA_F32 && g_gemm_f32_variant.load(std::memory_order_relaxed) == 5, dj,
                               dt, n_jobs, tiles, st);
     });
@@ -5839,6 +5924,76 @@ int mba_gemv(int prec, const void* a, int64_t n, int64_t lda, const void* x, voi
                            (const cpx<double>*)x, (cpx<double>*)y);
     MBA_LAUNCH_CHECK();
     return MBA_OK;
+}
+
+size_t mba_convert_c128_c64_batched_scratch_bytes(int32_t n_jobs)
+{
+    if (n_jobs <= 0) return 0;
+    return (size_t)n_jobs * sizeof(ConvJobDev);
+}
+
+int mba_convert_c128_c64_batched(const mba_convert_job* jobs, int32_t n_jobs, void* scratch, size_t scratch_bytes,
+                                 mba_stream_t stream)
+{
+    if (!jobs || !scratch || n_jobs <= 0 || n_jobs > 65535) return MBA_ERR_ARG;
+    for (int q = 0; q < n_jobs; ++q)
+        if (jobs[q].count < 0 || (jobs[q].count > 0 && (!jobs[q].src || !jobs[q].dst))) return MBA_ERR_ARG;
+    if (scratch_bytes < mba_convert_c128_c64_batched_scratch_bytes(n_jobs)) return MBA_ERR_ALLOC;
+    return mba_guard([&]() -> int {
+        std::vector<ConvJobDev> dj(n_jobs);
+        for (int q = 0; q < n_jobs; ++q)
+            dj[q] = ConvJobDev{(const cpx<double>*)jobs[q].src, (cpx<float>*)jobs[q].dst, jobs[q].count};
+        hipStream_t st = S(stream);
+        if (mba_host::upload_tables(scratch, dj.data(), dj.size() * sizeof(ConvJobDev), st) != 0)
+            return MBA_ERR_LAUNCH;
+        hipLaunchKernelGGL(k_c128_to_c64_batched, dim3(64, (unsigned)n_jobs), dim3(256), 0, st,
+                           (const ConvJobDev*)scratch);
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
+    });
+}
+
+size_t mba_refine_update_batched_scratch_bytes(const mba_refine_job* jobs, int32_t n_jobs)
+{
+    if (!jobs || n_jobs <= 0) return 0;
+    int64_t ncol = 0;
+    for (int q = 0; q < n_jobs; ++q) ncol = jobs[q].nrhs > ncol ? jobs[q].nrhs : ncol;
+    ncol = (ncol + RF_COLS - 1) / RF_COLS * RF_COLS;
+    return al256((size_t)n_jobs * sizeof(RefJobDev)) + (size_t)n_jobs * RF_SLABS * ncol * sizeof(double2);
+}
+
+int mba_refine_update_batched(const mba_refine_job* jobs, int32_t n_jobs, double* ratio_out, void* scratch,
+                              size_t scratch_bytes, mba_stream_t stream)
+{
+    if (!jobs || !ratio_out || !scratch || n_jobs <= 0 || n_jobs > 65535) return MBA_ERR_ARG;
+    int64_t ncol = 0;
+    for (int q = 0; q < n_jobs; ++q) {
+        const mba_refine_job& J = jobs[q];
+        if (J.n < 0 || J.nrhs < 0 || (J.n > 0 && J.nrhs > 0 && (!J.j || !J.d))) return MBA_ERR_ARG;
+        if (J.n > 0 && (J.ldj < J.nrhs || J.ldd < J.nrhs)) return MBA_ERR_ARG;
+        ncol = J.nrhs > ncol ? J.nrhs : ncol;
+    }
+    ncol = (ncol + RF_COLS - 1) / RF_COLS * RF_COLS;
+    if (ncol == 0 || ncol / RF_COLS > 65535) return MBA_ERR_SIZE;
+    if (scratch_bytes < mba_refine_update_batched_scratch_bytes(jobs, n_jobs)) return MBA_ERR_ALLOC;
+    return mba_guard([&]() -> int {
+        std::vector<RefJobDev> dj(n_jobs);
+        for (int q = 0; q < n_jobs; ++q)
+            dj[q] = RefJobDev{(cpx<double>*)jobs[q].j, jobs[q].ldj, (const cpx<float>*)jobs[q].d, jobs[q].ldd,
+                              jobs[q].n, jobs[q].nrhs};
+        hipStream_t st = S(stream);
+        RefJobDev* d_jobs = (RefJobDev*)scratch;
+        double2* part = (double2*)((char*)scratch + al256((size_t)n_jobs * sizeof(RefJobDev)));
+        if (mba_host::upload_tables(d_jobs, dj.data(), dj.size() * sizeof(RefJobDev), st) != 0)
+            return MBA_ERR_LAUNCH;
+        hipLaunchKernelGGL(k_refine_partial, dim3((unsigned)(ncol / RF_COLS), RF_SLABS, (unsigned)n_jobs),
+                           dim3(RF_THREADS), 0, st, (const RefJobDev*)d_jobs, (int)ncol, part);
+        MBA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_refine_ratio, dim3((unsigned)n_jobs), dim3(RF_THREADS), 0, st, (const RefJobDev*)d_jobs,
+                           (int)ncol, (const double2*)part, ratio_out);
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
+    });
 }
 
 int mba_convert_c128_c64(const void* src, int64_t m, int64_t n, int64_t lds, void* dst, int64_t ldd,

@@ -14,6 +14,7 @@
 
 #include "mba_common.hpp"
 #include "mba_guard.hpp"
+#include "upload.hpp"
 
 namespace {
 
@@ -945,14 +946,10 @@ int zmat_ns_batched(const void* verts, const int32_t* faces, int64_t total_faces
         near_off[m + 1] = near_off[m] + meshes[m].n_near;
     }
     char* base = (char*)ws;
-    if (hipMemcpyAsync(base + od, meshes, (size_t)n * sizeof(NsMeshDev), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(base + ot, tile_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        hipMemcpyAsync(base + on, near_off.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st) !=
-            hipSuccess)
+    if (mba_host::upload_tables(base + od, meshes, (size_t)n * sizeof(NsMeshDev), st) != 0 ||
+        mba_host::upload_tables(base + ot, tile_off.data(), (n + 1) * sizeof(int64_t), st) != 0 ||
+        mba_host::upload_tables(base + on, near_off.data(), (n + 1) * sizeof(int64_t), st) != 0)
         return MBA_ERR_LAUNCH;
-    // the host tables above are released on return: wait for the copies
-    if (hipStreamSynchronize(st) != hipSuccess) return MBA_ERR_LAUNCH;
     const NsMeshDev* dm = (const NsMeshDev*)(base + od);
     const int64_t* dt = (const int64_t*)(base + ot);
     const int64_t* dn = (const int64_t*)(base + on);

@@ -13,7 +13,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(ROOT, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libmesh_bem_amd.so")
-SOURCES = ["mom_assemble.hip", "dense.hip", "laplace.hip", "gmres.hip", "mesh.hip", "mgn.hip", "radiate.hip", "helmholtz.hip", "gemm_h3.hip", "bem2d.hip", "topology.hip", "pressure.hip", "mesh_io.cpp", "remesh.hip", "loop_subdivide.cpp", "decimate.cpp", "rates.hip"]
+SOURCES = ["mom_assemble.hip", "dense.hip", "laplace.hip", "gmres.hip", "mesh.hip", "mgn.hip", "radiate.hip", "helmholtz.hip", "gemm_h3.hip", "bem2d.hip", "topology.hip", "pressure.hip", "mesh_io.cpp", "remesh.hip", "loop_subdivide.cpp", "decimate.cpp", "rates.hip", "upload.hip"]
 ARCH = os.environ.get("MBA_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

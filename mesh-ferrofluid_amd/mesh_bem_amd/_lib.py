@@ -55,6 +55,19 @@ class LuSolveJob(ctypes.Structure):
                 ("b", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("nrhs", ctypes.c_int64), ("work", ctypes.c_void_p)]
 
 
+class ConvertJob(ctypes.Structure):
+    """mba_convert_job of include/mesh_bem_amd.h."""
+
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("count", ctypes.c_int64)]
+
+
+class RefineJob(ctypes.Structure):
+    """mba_refine_job of include/mesh_bem_amd.h."""
+
+    _fields_ = [("j", ctypes.c_void_p), ("ldj", ctypes.c_int64), ("d", ctypes.c_void_p), ("ldd", ctypes.c_int64),
+                ("n", ctypes.c_int64), ("nrhs", ctypes.c_int64)]
+
+
 class MgnVMat(ctypes.Structure):
     """mba_mgn_vmat of include/mesh_bem_amd.h."""
 
@@ -135,6 +148,10 @@ SIGNATURES = {
     "mba_residual_c128": (_c.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mba_gemv": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _vp]),
     "mba_convert_c128_c64": (_c.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp]),
+    "mba_convert_c128_c64_batched_scratch_bytes": (_sz, [_i32]),
+    "mba_convert_c128_c64_batched": (_c.c_int, [_vp, _i32, _vp, _sz, _vp]),
+    "mba_refine_update_batched_scratch_bytes": (_sz, [_vp, _i32]),
+    "mba_refine_update_batched": (_c.c_int, [_vp, _i32, _vp, _vp, _sz, _vp]),
     "mba_mv32": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _i64, _c.c_int, _vp, _vp, _vp, _vp]),
     "mba_gmres_workspace_bytes": (_sz, [_i64, _i32]),
     "mba_gmres_dots": (_c.c_int, [_c.c_int, _vp, _i64, _i32, _c.c_int, _vp, _i64, _vp, _vp, _sz, _vp]),

@@ -23,9 +23,10 @@ import numpy as np
 import torch
 
 from . import _lib
-from .dense import gemm_grouped, lu_factor_batched, lu_solve_batched, to_c64
+from .dense import (check_batched, convert_c128_c64_batched, gemm_grouped, lu_factor_batched, lu_solve_batched,
+                    refine_update_batched, to_c64)
 from .geometry import get_edge_unique
-from .mom import MomMesh, rhs_ns, rule_tensor
+from .mom import MomMesh, rhs_ns, rhs_ns_many, rule_tensor
 
 BUCKETS = (1024, 2048, 3072, 4096, 8192, 16384, 32768)
 
@@ -218,7 +219,7 @@ class BatchedMoM:
         return Zs
 
     def rhs(self, dirs: List[np.ndarray], pols: List[np.ndarray], prec=_lib.MBA_F64) -> List[torch.Tensor]:
-        return [rhs_ns(m, k, prec, d, p) for m, k, d, p in zip(self.meshes, self.ks, dirs, pols)]
+        return rhs_ns_many(self.meshes, self.ks, prec, dirs, pols)
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
               tol: float = 1e-8, maxit: int = 10, persistent: bool = True,
@@ -250,28 +251,37 @@ class BatchedMoM:
         lus = lu_factor_batched(Z32, overwrite=True, persistent=persistent, options=opts)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        for lu in lus:
-            lu.check()
+        check_batched(lus)
         n_mesh = len(Zs)
+        dev = Zs[0].device
         V128s = [V.to(torch.complex128).contiguous() for V in Vs]
-        Js = [X.to(torch.complex128) for X in lu_solve_batched(lus, [V.to(torch.complex64) for V in Vs],
-                                                                persistent=solve_persistent)]
-        Rs = [torch.empty_like(V) for V in V128s]
+        shapes = [tuple(V.shape) for V in V128s]
+        sizes = [V.numel() for V in V128s]
+
+        def views(buf):
+            return [t.view(sh) for t, sh in zip(torch.split(buf, sizes), shapes)]
+
+        # one buffer per kind for the whole batch, a view per mesh
+        Js = views(torch.zeros(sum(sizes), dtype=torch.complex128, device=dev))
+        Rs = views(torch.empty(sum(sizes), dtype=torch.complex128, device=dev))
+        B32s = views(torch.empty(sum(sizes), dtype=torch.complex64, device=dev))
+        errs = torch.zeros(maxit + 1, dtype=torch.int32, device=dev)  # solve launches' error words
+        convert_c128_c64_batched(V128s, B32s)
+        Xs = lu_solve_batched(lus, B32s, persistent=solve_persistent, overwrite=True, err_out=errs[0:1])
+        refine_update_batched(Js, Xs)  # J = LU32 \ V
         # the first solve is the iteration's first step from J = 0: a
         # correction of relative size 1
         its, conv, prev = [0] * n_mesh, [False] * n_mesh, [1.0] * n_mesh
         active = list(range(n_mesh))
+        sweep = 0
         while active:
-            for i in active:
-                Rs[i].copy_(V128s[i])
+            sweep += 1
+            torch._foreach_copy_([Rs[i] for i in active], [V128s[i] for i in active])
             gemm_grouped([(Zs[i], Js[i], Rs[i]) for i in active], alpha=-1.0, beta=1.0)
-            Ds = lu_solve_batched([lus[i] for i in active], [Rs[i].to(torch.complex64) for i in active],
-                                  persistent=solve_persistent)
-            rels = []
-            for i, D in zip(active, Ds):
-                Js[i] += D
-                rels.append((torch.linalg.vector_norm(D, dim=0) / torch.linalg.vector_norm(Js[i], dim=0)).max())
-            rels = torch.stack(rels).cpu().tolist()
+            convert_c128_c64_batched([Rs[i] for i in active], [B32s[i] for i in active])
+            Ds = lu_solve_batched([lus[i] for i in active], [B32s[i] for i in active],
+                                  persistent=solve_persistent, overwrite=True, err_out=errs[sweep:sweep + 1])
+            rels = refine_update_batched([Js[i] for i in active], Ds).cpu().tolist()
             for i, r in zip(active, rels):
                 its[i] += 1
                 # error left after this sweep: its correction r times the
@@ -280,6 +290,8 @@ class BatchedMoM:
                 prev[i] = r
                 conv[i] = est <= tol
             active = [i for i in active if not conv[i] and its[i] < maxit]
+        if int(errs.max().item()) != 0:
+            raise RuntimeError("mba_lu_solve_batched_persistent: a block wait timed out")
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         if tuner is not None:

@@ -234,7 +234,9 @@ def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True, opt
     n_mat = len(mats)
     lus, keep = [], []
     desc = (_lib.LuMatrix * n_mat)()
-    err = torch.zeros(n_mat, dtype=torch.int32, device=dev)
+    # status words of the batch: [0, n_mat) wait errors, [n_mat, 2 n_mat) info
+    status = torch.zeros(2 * n_mat, dtype=torch.int32, device=dev)
+    err = status[:n_mat]
     for i, A in enumerate(mats):
         if A.dim() != 2 or A.shape[0] != A.shape[1] or A.dtype != dt:
             raise ValueError("lu_factor_batched needs square matrices of one dtype")
@@ -243,7 +245,7 @@ def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True, opt
             lu = lu.contiguous()
         n = lu.shape[0]
         ipiv = torch.empty(n, dtype=torch.int32, device=dev)
-        info = torch.zeros(1, dtype=torch.int32, device=dev)
+        info = status[n_mat + i:n_mat + i + 1]
         wsb = lib.mba_lu_workspace_bytes(prec, n)
         ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
         desc[i].a, desc[i].n, desc[i].lda = lu.data_ptr(), n, lu.stride(0)
@@ -270,6 +272,8 @@ def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True, opt
         st = lib.mba_lu_factor_batched(prec, desc, n_mat, _lib.ptr(scratch), sb, _lib.ptr(err), _lib.stream())
         _lib.check(st, "mba_lu_factor_batched")
     out = [LUFactor._from_factored(lu, ipiv, info, err[i:i + 1]) for i, (lu, ipiv, info) in enumerate(lus)]
+    for i, f in enumerate(out):
+        f._batch = (status, i, n_mat)
     jobs = (_lib.LuPrepareJob * n_mat)()
     for i, f in enumerate(out):
         jobs[i].lu, jobs[i].n, jobs[i].lda = f.lu.data_ptr(), f.n, f.lu.stride(0)
@@ -279,6 +283,74 @@ def lu_factor_batched(mats, overwrite: bool = True, persistent: bool = True, opt
     st = lib.mba_lu_solve_prepare_batched(prec, jobs, n_mat, _lib.ptr(pscratch), pb, _lib.stream())
     _lib.check(st, "mba_lu_solve_prepare_batched")
     out[0]._keep = (keep, scratch, pscratch)
+    return out
+
+
+def check_batched(lus) -> None:
+    """``check()`` of many factorisations with one device-to-host copy per
+    batch of ``lu_factor_batched`` (after the stream has been synchronised)."""
+    groups = {}
+    for f in lus:
+        b = getattr(f, "_batch", None)
+        if b is None:
+            f.check()
+        else:
+            groups.setdefault(id(b[0]), (b[0], []))[1].append(f)
+    for status, fs in groups.values():
+        host = status.cpu().tolist()
+        for f in fs:
+            _, i, n_mat = f._batch
+            if host[i] != 0:
+                raise _lib.MBAError("mba_lu_factor_batched: a wait timed out")
+            f.info = int(host[n_mat + i])
+            if f.info != 0:
+                raise _lib.MBAError(f"mba_lu_factor: exactly zero pivot in column {f.info - 1}")
+
+
+def convert_c128_c64_batched(srcs, dsts) -> None:
+    """dst_i = complex64(src_i) for contiguous tensors of equal sizes, one launch."""
+    lib = _lib.load()
+    n = len(srcs)
+    if n == 0:
+        return
+    if len(dsts) != n:
+        raise ValueError("convert_c128_c64_batched: one destination per source")
+    jobs = (_lib.ConvertJob * n)()
+    for q, (a, b) in enumerate(zip(srcs, dsts)):
+        if a.dtype != torch.complex128 or b.dtype != torch.complex64 or a.numel() != b.numel():
+            raise ValueError("convert_c128_c64_batched: complex128 sources, complex64 destinations of equal size")
+        if not (a.is_contiguous() and b.is_contiguous()):
+            raise ValueError("convert_c128_c64_batched: contiguous tensors")
+        jobs[q].src, jobs[q].dst, jobs[q].count = a.data_ptr(), b.data_ptr(), a.numel()
+    sb = lib.mba_convert_c128_c64_batched_scratch_bytes(n)
+    scratch = torch.empty(sb, dtype=torch.uint8, device=srcs[0].device)
+    _lib.check(lib.mba_convert_c128_c64_batched(jobs, n, _lib.ptr(scratch), sb, _lib.stream()),
+               "mba_convert_c128_c64_batched")
+
+
+def refine_update_batched(Js, Ds) -> torch.Tensor:
+    """J_i += D_i (complex128 J_i, complex64 D_i, [n_i, nrhs_i] contiguous), in
+    one pass; returns the device tensor of max_c |D_i[:, c]| / |J_i[:, c]|
+    (after the update) per system, float64."""
+    lib = _lib.load()
+    n = len(Js)
+    if len(Ds) != n:
+        raise ValueError("refine_update_batched: one correction per iterate")
+    out = torch.empty(max(n, 1), dtype=torch.float64, device=Js[0].device if n else "cpu")
+    if n == 0:
+        return out[:0]
+    jobs = (_lib.RefineJob * n)()
+    for q, (J, D) in enumerate(zip(Js, Ds)):
+        if J.dtype != torch.complex128 or D.dtype != torch.complex64 or J.dim() != 2 or J.shape != D.shape:
+            raise ValueError("refine_update_batched: complex128 J and complex64 D of one [n, nrhs] shape")
+        if J.stride(1) != 1 or D.stride(1) != 1:
+            raise ValueError("refine_update_batched: rows must be contiguous")
+        jobs[q].j, jobs[q].ldj, jobs[q].d, jobs[q].ldd = J.data_ptr(), J.stride(0), D.data_ptr(), D.stride(0)
+        jobs[q].n, jobs[q].nrhs = J.shape[0], J.shape[1]
+    sb = lib.mba_refine_update_batched_scratch_bytes(jobs, n)
+    scratch = torch.empty(sb, dtype=torch.uint8, device=Js[0].device)
+    _lib.check(lib.mba_refine_update_batched(jobs, n, _lib.ptr(out), _lib.ptr(scratch), sb, _lib.stream()),
+               "mba_refine_update_batched")
     return out
 
 
@@ -317,7 +389,7 @@ def gemm_grouped(products, alpha: float = 1.0, beta: float = 0.0):
     _lib.check(st, "mba_gemm_grouped")
 
 
-def lu_solve_batched(lus, Bs, persistent: bool = False):
+def lu_solve_batched(lus, Bs, persistent: bool = False, overwrite: bool = False, err_out=None):
     """X_i = A_i^{-1} B_i for prepared factorisations lus[i] (one dtype) and
     right-hand sides Bs[i] [n_i, nrhs_i].  Default: two launches, one
     workgroup per job and 16 right-hand sides walking its 64-row blocks.
@@ -325,7 +397,11 @@ def lu_solve_batched(lus, Bs, persistent: bool = False):
     dataflow of 64-row block tasks inside each XCD
     (``mba_lu_solve_batched_persistent``); its error word is checked here
     (one synchronisation).  Returns new tensors (the inputs are not
-    modified)."""
+    modified; ``overwrite``: a right-hand side that already has the factor's
+    dtype and contiguous rows is solved in place and returned).  ``err_out``
+    (persistent path, a one-element int32 device tensor): receives the
+    launch's error word instead of the check here, so the call does not
+    synchronise; the caller checks it later."""
     lib = _lib.load()
     n_job = len(lus)
     if n_job == 0:
@@ -335,7 +411,9 @@ def lu_solve_batched(lus, Bs, persistent: bool = False):
     jobs = (_lib.LuSolveJob * n_job)()
     Xs, keep = [], []
     for i, (lu, B) in enumerate(zip(lus, Bs)):
-        X = B.reshape(lu.n, -1).to(lu.lu.dtype).clone().contiguous()
+        X = B.reshape(lu.n, -1)
+        if not (overwrite and X.dtype == lu.lu.dtype and X.is_contiguous()):
+            X = X.to(lu.lu.dtype).clone().contiguous()
         W = torch.empty_like(X)
         jobs[i].lu, jobs[i].n, jobs[i].lda = lu.lu.data_ptr(), lu.n, lu.lu.stride(0)
         jobs[i].aux, jobs[i].b, jobs[i].ldb = lu.aux.data_ptr(), X.data_ptr(), X.stride(0)
@@ -347,11 +425,13 @@ def lu_solve_batched(lus, Bs, persistent: bool = False):
         if sb == 0:
             raise ValueError("lu_solve_batched: too many blocks or right-hand sides for one launch")
         scratch = torch.empty(sb, dtype=torch.uint8, device=Xs[0].device)
-        err = torch.zeros(1, dtype=torch.int32, device=Xs[0].device)
+        err = err_out if err_out is not None else torch.zeros(1, dtype=torch.int32, device=Xs[0].device)
+        if err.dtype != torch.int32 or err.numel() < 1 or not err.is_cuda:
+            raise ValueError("lu_solve_batched: err_out must be an int32 device tensor")
         st = lib.mba_lu_solve_batched_persistent(lus[0].prec, jobs, n_job, _lib.ptr(scratch), sb, _lib.ptr(err),
                                                  _lib.stream())
         _lib.check(st, "mba_lu_solve_batched_persistent")
-        if int(err.item()) != 0:
+        if err_out is None and int(err.item()) != 0:
             raise RuntimeError("mba_lu_solve_batched_persistent: a block wait timed out")
     else:
         sb = lib.mba_lu_solve_batched_scratch_bytes(n_job)

@@ -221,13 +221,40 @@ def assemble_ns(mesh: MomMesh, k: float, prec: int, window_faces: Optional[int] 
 
 def rhs_ns(mesh: MomMesh, k: float, prec: int, dirs, pols) -> torch.Tensor:
     """Plane-wave RHS V [E, ndir] (complex)."""
-    lib = _lib.load()
     dev = mesh.verts.device
     dirs = torch.as_tensor(np.atleast_2d(np.asarray(dirs, dtype=np.float64))).to(dev).contiguous()
     pols = torch.as_tensor(np.atleast_2d(np.asarray(pols, dtype=np.float64))).to(dev).contiguous()
+    return _rhs_ns_dev(mesh, k, prec, dirs, pols, rule_tensor(dev))
+
+
+def rhs_ns_many(meshes, ks, prec: int, dirs_list, pols_list) -> list:
+    """rhs_ns for many meshes: the directions and polarisations of all of
+    them and the quadrature rule go to the device in one copy each."""
+    if not meshes:
+        return []
+    dev = meshes[0].verts.device
+    ds = [np.atleast_2d(np.asarray(d, dtype=np.float64)) for d in dirs_list]
+    ps = [np.atleast_2d(np.asarray(p, dtype=np.float64)) for p in pols_list]
+    if any(d.shape != p.shape or d.shape[1] != 3 for d, p in zip(ds, ps)):
+        raise ValueError("rhs_ns_many: dirs and pols must be [ndir, 3] per mesh")
+    D = torch.from_numpy(np.ascontiguousarray(np.concatenate(ds))).to(dev)
+    P = torch.from_numpy(np.ascontiguousarray(np.concatenate(ps))).to(dev)
+    rule = rule_tensor(dev)
+    out, off = [], 0
+    for m, k, d in zip(meshes, ks, ds):
+        nd = d.shape[0]
+        out.append(_rhs_ns_dev(m, k, prec, D[off:off + nd], P[off:off + nd], rule))
+        off += nd
+    return out
+
+
+def _rhs_ns_dev(mesh: MomMesh, k: float, prec: int, dirs: torch.Tensor, pols: torch.Tensor,
+                rule: torch.Tensor) -> torch.Tensor:
+    """rhs_ns with device-resident contiguous dirs / pols [ndir, 3] (FP64) and rule."""
+    lib = _lib.load()
+    dev = mesh.verts.device
     nd = dirs.shape[0]
     F, E = mesh.n_faces, mesh.n_edges
-    rule = rule_tensor(dev)
     verts = mesh.verts.to(_rdt(prec)).contiguous()
     vf = torch.empty((3 * F, nd), dtype=_cdt(prec), device=dev)
     st = lib.mba_mom_rhs_faceedge_ns(prec, _lib.ptr(verts), _lib.ptr(mesh.faces), mesh.n_verts, F,

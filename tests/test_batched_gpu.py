@@ -144,6 +144,55 @@ def test_lu_solve_batched_persistent_many_jobs(dev):
     assert worst_e < 1e-5 and worst_r < 1e-5
 
 
+def test_batched_refinement_helpers(dev):
+    """One-launch helpers of the batched refinement: complex128 -> complex64
+    conversions equal torch's elementwise conversion bit for bit; J += D
+    equals torch's J + D.to(complex128) bit for bit, and the per-system max
+    column ratio |D_c| / |J_c| matches an FP64 torch computation (1e-12),
+    including an empty system, a zero column (ratio 0) and a zero J column
+    with a nonzero correction (ratio inf)."""
+    from mesh_bem_amd.dense import convert_c128_c64_batched, refine_update_batched
+
+    print("batched conversion and refinement update", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    shapes = [(300, 7), (1, 1), (2049, 128), (64, 65), (0, 5), (513, 200)]
+
+    def rnd(sh, dt):
+        return torch.complex(torch.randn(sh, generator=g, dtype=torch.float64),
+                             torch.randn(sh, generator=g, dtype=torch.float64)).to(dt).to(dev)
+
+    srcs = [rnd(sh, torch.complex128) for sh in shapes]
+    dsts = [torch.empty(sh, dtype=torch.complex64, device=dev) for sh in shapes]
+    convert_c128_c64_batched(srcs, dsts)
+    for a, b in zip(srcs, dsts):
+        assert torch.equal(b, a.to(torch.complex64))
+
+    Js = [rnd(sh, torch.complex128) for sh in shapes]
+    Ds = [rnd(sh, torch.complex64) * 1e-3 for sh in shapes]
+    Js[0][:, 3] = 0
+    Ds[0][:, 3] = 0  # zero column: ratio 0
+    Js[3][:, 5] = -Ds[3][:, 5].to(torch.complex128)  # J + D = 0 there, D != 0: ratio inf
+    ref = [J + D.to(torch.complex128) for J, D in zip(Js, Ds)]
+    want = []
+    for R, D in zip(ref, Ds):
+        if R.shape[0] == 0:
+            want.append(0.0)
+            continue
+        dn = torch.linalg.vector_norm(D.to(torch.complex128), dim=0)
+        jn = torch.linalg.vector_norm(R, dim=0)
+        r = torch.where(jn > 0, dn / jn, torch.where(dn > 0, torch.full_like(dn, float("inf")), torch.zeros_like(dn)))
+        want.append(float(r.max()))
+    got = refine_update_batched(Js, Ds).cpu().tolist()
+    for J, R in zip(Js, ref):
+        assert torch.equal(J, R)
+    print(f"  ratios {got}")
+    for a, b in zip(got, want):
+        if b == float("inf"):
+            assert a == float("inf")
+        else:
+            assert abs(a - b) <= 1e-12 * max(b, 1e-300)
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("variant", [4, 5])
 def test_gemm_grouped_matches_torch(dev, prec, variant):
