@@ -14,12 +14,14 @@
  *   - Matrices are row-major with an explicit leading dimension (elements).
  *   - Index arrays are int32 (as the reference's faces / face_edge_indices).
  *   - Work is enqueued on `stream` (a hipStream_t).  Calls return without
- *     waiting for the device, except these, which synchronise `stream`:
- *     mba_lu_factor and mba_lu_factor_profiled (to read the panel wait flag;
- *     mba_lu_factor_async does not wait), and the calls that copy a host
- *     job table to the device (mba_lu_factor_batched,
- *     mba_lu_solve_prepare_batched, mba_lu_solve_batched, mba_gemm_grouped,
- *     mba_mom_zmat_faceedge_ns_batched: the host table is released on return).
+ *     waiting for the device, except mba_lu_factor and mba_lu_factor_profiled
+ *     (to read the panel wait flag; mba_lu_factor_async does not wait) and the
+ *     _profiled batched factorisation (to read its clock).  Calls that take a
+ *     host job table copy it into pinned staging memory of the library (a
+ *     ring of slots per device, each reused after its copy has completed), so
+ *     the caller's table may be released on return; the device scratch and
+ *     every buffer passed must stay allocated until the stream's work is
+ *     done.
  *   - Return value: 0 (MBA_OK) or an MBA_ERR_* code.  Argument and size errors
  *     are detected on the host before any launch.  No C++ exception leaves an
  *     entry point (allocation failures return MBA_ERR_ALLOC).

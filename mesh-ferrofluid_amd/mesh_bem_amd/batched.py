@@ -244,7 +244,14 @@ class BatchedMoM:
         lib = _lib.load()
         if tuner is not None:
             tuner.apply(max(Z.shape[0] for Z in Zs))
-        Z32 = [to_c64(Z) for Z in Zs]
+        if all(Z.dtype == torch.complex128 and Z.is_contiguous() for Z in Zs):
+            # one buffer and one conversion launch for the whole batch
+            zsz = [Z.numel() for Z in Zs]
+            zbuf = torch.empty(sum(zsz), dtype=torch.complex64, device=Zs[0].device)
+            Z32 = [t.view(Z.shape) for t, Z in zip(torch.split(zbuf, zsz), Zs)]
+            convert_c128_c64_batched(list(Zs), Z32)
+        else:
+            Z32 = [to_c64(Z) for Z in Zs]
         opts = lu_tuner.tune(Z32) if (persistent and lu_tuner is not None) else None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
