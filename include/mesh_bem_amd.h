@@ -189,9 +189,9 @@ int mba_mom_gather_rhs(int prec, const void *vf, const int32_t *rwg_fe, int64_t 
  * equal to the running sum); every mesh writes its own face-edge block
  * zf [3 n_faces, ldzf] for all its source faces.  n_near = near_ptr[n_faces];
  * each mesh's near relation must be symmetric (see mba_mom_zmat_faceedge_ns).
- * The descriptor array is host memory; the call copies it to the workspace and
- * waits for that copy.  No reference counterpart (the reference solves one
- * mesh per call). */
+ * The descriptor array is host memory (staged by the call; it may be released
+ * on return).  No reference counterpart (the reference solves one mesh per
+ * call). */
 typedef struct {
     int64_t n_faces;
     int64_t face_off;
@@ -207,6 +207,50 @@ int mba_mom_zmat_faceedge_ns_batched(int prec, const void *verts, const int32_t 
                                      const double *rule, int32_t nq, const mba_ns_mesh *meshes,
                                      int32_t n_meshes, void *workspace, size_t workspace_bytes,
                                      mba_stream_t stream);
+
+/* Plane-wave right-hand sides of several meshes in one pass (north-star rule,
+ * as mba_mom_rhs_faceedge_ns followed by mba_mom_gather_rhs per mesh): verts /
+ * faces concatenated as for mba_mom_zmat_faceedge_ns_batched (face_off the
+ * running sum of n_faces); mesh m's directions and polarisations dirs / pols
+ * [ndir, 3] (double), its RWG table rwg_fe [n_edges, 2] (mesh-local face-edge
+ * indices 3 f + e) and its output v [n_edges, ndir] complex (prec).  Replaces
+ * the per-mesh create_MOM_rhs_3d_forward calls of the reference's batch loop
+ * (demo/demo_Helmholtz_MOM_3d.py:206-216 -> mesh_MOM_3d.cpp:3-36) with the
+ * north-star incidence; the descriptor array is host memory (staged). */
+typedef struct {
+    int64_t n_faces;
+    int64_t face_off;
+    double wavenumber;
+    const double *dirs;
+    const double *pols;
+    int64_t ndir;
+    const int32_t *rwg_fe;
+    int64_t n_edges;
+    void *v;
+} mba_ns_rhs_mesh;
+size_t mba_mom_rhs_ns_batched_workspace_bytes(int prec, int64_t total_faces, const mba_ns_rhs_mesh *meshes,
+                                              int32_t n_meshes);
+int mba_mom_rhs_ns_batched(int prec, const void *verts, const int32_t *faces, int64_t total_faces,
+                           const double *rule, int32_t nq, const mba_ns_rhs_mesh *meshes, int32_t n_meshes,
+                           void *workspace, size_t workspace_bytes, mba_stream_t stream);
+
+/* RWG gathers of several meshes in one launch: job q takes the full
+ * face-edge block zf [3F, ldzf] of its mesh (every source face, as
+ * mba_mom_zmat_faceedge_ns_batched writes it) to z [n_edges, ldz], the same
+ * sums in the same order as mba_mom_gather_rwg (bit-identical).  Replaces the
+ * RWG scatter of mesh_BEM_solver_3d.py:128-164 for a batch; the job array is
+ * host memory (staged); scratch: mba_mom_gather_rwg_batched_scratch_bytes. */
+typedef struct {
+    const void *zf;
+    int64_t ldzf;
+    const int32_t *rwg_fe;
+    int64_t n_edges;
+    void *z;
+    int64_t ldz;
+} mba_gather_job;
+size_t mba_mom_gather_rwg_batched_scratch_bytes(int32_t n_jobs);
+int mba_mom_gather_rwg_batched(int prec, const mba_gather_job *jobs, int32_t n_jobs, void *scratch,
+                               size_t scratch_bytes, mba_stream_t stream);
 
 /* =================================================================== dense
  * Row-major complex LU with partial (row) pivoting, in place, replacing the

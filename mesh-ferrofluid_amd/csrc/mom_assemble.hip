@@ -767,6 +767,111 @@ __global__ void k_gather_rhs(const cpx<T>* __restrict__ vf, const int32_t* __res
     store_cpx(v + id, a.re + b.re, a.im + b.im);
 }
 
+// ---- batched right-hand sides and gathers (many meshes, one launch each) --
+struct RhsMeshDev {
+    int64_t F, face_off;
+    double k;
+    const double* dirs;
+    const double* pols;
+    int64_t ndir;
+    const int32_t* rwg_fe;
+    int64_t E;
+    void* v;
+    int64_t vf_off;  // first element of this mesh's face-edge values in the workspace
+};
+
+// thread (mesh m, face f, direction d), m found in the prefix of F ndir
+template <class T>
+__global__ void k_ns_rhs_batched(const FaceRec<T>* __restrict__ rec_all, const double* __restrict__ rule, int nq,
+                                 const RhsMeshDev* __restrict__ meshes, int n, const int64_t* __restrict__ off,
+                                 cpx<T>* __restrict__ vf_all)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= off[n]) return;
+    const int m = find_segment(off, n, id);
+    const RhsMeshDev M = meshes[m];
+    const int64_t local = id - off[m];
+    const int64_t f = local / M.ndir, d = local % M.ndir;
+    const FaceRec<T> r = rec_all[M.face_off + f];
+    const T k2pi = (T)(M.k / (2.0 * MBA_PI));
+    const T kh[3] = {(T)M.dirs[3 * d], (T)M.dirs[3 * d + 1], (T)M.dirs[3 * d + 2]};
+    const T pl[3] = {(T)M.pols[3 * d], (T)M.pols[3 * d + 1], (T)M.pols[3 * d + 2]};
+    const T ph0 = (kh[0] * r.ohi[0] + kh[1] * r.ohi[1] + kh[2] * r.ohi[2]) * k2pi +
+                  (kh[0] * r.olo[0] + kh[1] * r.olo[1] + kh[2] * r.olo[2]) * k2pi;
+    T acc[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int i = 0; i < nq; ++i) {
+        T x[3];
+        for (int q = 0; q < 3; ++q)
+            x[q] = (T)rule[4 * i] * r.p[q] + (T)rule[4 * i + 1] * r.p[3 + q] + (T)rule[4 * i + 2] * r.p[6 + q];
+        T w = (T)rule[4 * i + 3] * (T)2 * r.area;
+        T cs, sn;
+        cis_rev(ph0 + (kh[0] * x[0] + kh[1] * x[1] + kh[2] * x[2]) * k2pi, cs, sn);
+        for (int e = 0; e < 3; ++e) {
+            T fp = r.c[e] * ((x[0] - r.p[3 * e]) * pl[0] + (x[1] - r.p[3 * e + 1]) * pl[1] +
+                             (x[2] - r.p[3 * e + 2]) * pl[2]);
+            acc[e][0] += w * fp * cs;
+            acc[e][1] += -w * fp * sn;
+        }
+    }
+    cpx<T>* vf = vf_all + M.vf_off;
+    for (int e = 0; e < 3; ++e) store_cpx(vf + (3 * f + e) * M.ndir + d, acc[e][0], acc[e][1]);
+}
+
+// thread (mesh m, RWG row, direction d), m found in the prefix of E ndir
+template <class T>
+__global__ void k_gather_rhs_batched(const RhsMeshDev* __restrict__ meshes, int n, const int64_t* __restrict__ off,
+                                     const cpx<T>* __restrict__ vf_all)
+{
+    const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= off[n]) return;
+    const int m = find_segment(off, n, id);
+    const RhsMeshDev M = meshes[m];
+    const int64_t local = id - off[m];
+    const int64_t row = local / M.ndir, d = local % M.ndir;
+    const cpx<T>* vf = vf_all + M.vf_off;
+    const cpx<T> a = vf[(int64_t)M.rwg_fe[2 * row] * M.ndir + d], b = vf[(int64_t)M.rwg_fe[2 * row + 1] * M.ndir + d];
+    store_cpx((cpx<T>*)M.v + local, a.re + b.re, a.im + b.im);
+}
+
+struct GatherJobDev {
+    const void* zf;
+    int64_t ldzf;
+    const int32_t* rwg_fe;
+    int64_t E;
+    void* z;
+    int64_t ldz;
+    int rows;  // RWG rows per workgroup
+};
+
+// k_gather_rwg over the full face-edge block of many meshes: logical tile L
+// (XCD-grouped as in k_gather_rwg over the whole grid) -> job by the prefix of
+// tile counts, then the same sums in the same order
+template <class T>
+__global__ __launch_bounds__(256) void k_gather_rwg_batched(const GatherJobDev* __restrict__ jobs, int n,
+                                                            const int64_t* __restrict__ tile_off)
+{
+    const int64_t per = (int64_t)gridDim.x / GATHER_XCD;
+    const int64_t L = (int64_t)(blockIdx.x % GATHER_XCD) * per + blockIdx.x / GATHER_XCD;
+    if (L >= tile_off[n]) return;
+    const int q = find_segment(tile_off, n, L);
+    const GatherJobDev J = jobs[q];
+    const int64_t t = L - tile_off[q];
+    const int64_t nbx = (J.E + 255) / 256;
+    const int64_t m0 = (t / nbx) * J.rows;
+    const int64_t c = (t % nbx) * 256 + threadIdx.x;
+    if (c >= J.E) return;
+    const cpx<T>* zf = (const cpx<T>*)J.zf;
+    const int64_t c0 = J.rwg_fe[2 * c], c1 = J.rwg_fe[2 * c + 1];
+    const int64_t m1 = (m0 + J.rows) < J.E ? (m0 + J.rows) : J.E;
+    for (int64_t m = m0; m < m1; ++m) {
+        const cpx<T>* r0 = zf + (int64_t)J.rwg_fe[2 * m] * J.ldzf;
+        const cpx<T>* r1 = zf + (int64_t)J.rwg_fe[2 * m + 1] * J.ldzf;
+        const cpx<T> a = r0[c0], b = r1[c0], e = r0[c1], f = r1[c1];
+        const T re = (a.re + b.re) + (e.re + f.re), im = (a.im + b.im) + (e.im + f.im);
+        store_cpx((cpx<T>*)J.z + m * J.ldz + c, re, im);
+    }
+}
+
 // ----------------------------------------------------------------- interpolation
 template <class T>
 __global__ void k_interp(const T* __restrict__ verts, const int32_t* __restrict__ faces,
@@ -1007,6 +1112,71 @@ void* scratch_address()
 }  // namespace
 
 // ============================================================== C ABI
+// ---- batched right-hand sides / gathers: host side
+struct RhsLayout {
+    size_t rec, vf, mesh, offf, offe, total;
+    int64_t n_vf;  // complex face-edge values of all meshes
+};
+
+inline RhsLayout rhs_batched_layout(int prec, int64_t total_faces, const mba_ns_rhs_mesh* meshes, int n)
+{
+    const size_t csz = prec == MBA_F32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    RhsLayout L{};
+    L.n_vf = 0;
+    for (int m = 0; m < n; ++m) L.n_vf += 3 * meshes[m].n_faces * meshes[m].ndir;
+    L.rec = 0;
+    L.vf = al(records_bytes(prec, total_faces));
+    L.mesh = L.vf + al((size_t)L.n_vf * csz);
+    L.offf = L.mesh + al((size_t)n * sizeof(RhsMeshDev));
+    L.offe = L.offf + al((size_t)(n + 1) * sizeof(int64_t));
+    L.total = L.offe + (size_t)(n + 1) * sizeof(int64_t);
+    return L;
+}
+
+template <class T>
+int rhs_ns_batched(const void* verts, const int32_t* faces, int64_t total_faces, const double* rule, int nq,
+                   const mba_ns_rhs_mesh* meshes, int n, void* ws, hipStream_t st)
+{
+    const RhsLayout L = rhs_batched_layout(sizeof(T) == 4 ? MBA_F32 : MBA_F64, total_faces, meshes, n);
+    char* base = (char*)ws;
+    FaceRec<T>* rec = (FaceRec<T>*)(base + L.rec);
+    int rc = face_records<T>(verts, faces, nullptr, nullptr, total_faces, rec, st);
+    if (rc != MBA_OK) return rc;
+    std::vector<RhsMeshDev> dm(n);
+    std::vector<int64_t> offf(n + 1, 0), offe(n + 1, 0);
+    int64_t vf_off = 0;
+    for (int m = 0; m < n; ++m) {
+        const mba_ns_rhs_mesh& d = meshes[m];
+        dm[m] = RhsMeshDev{d.n_faces, d.face_off, d.wavenumber, d.dirs, d.pols, d.ndir, d.rwg_fe, d.n_edges, d.v,
+                           vf_off};
+        vf_off += 3 * d.n_faces * d.ndir;
+        offf[m + 1] = offf[m] + d.n_faces * d.ndir;
+        offe[m + 1] = offe[m] + d.n_edges * d.ndir;
+    }
+    RhsMeshDev* d_mesh = (RhsMeshDev*)(base + L.mesh);
+    int64_t* d_offf = (int64_t*)(base + L.offf);
+    int64_t* d_offe = (int64_t*)(base + L.offe);
+    if (mba_host::upload_tables(d_mesh, dm.data(), dm.size() * sizeof(RhsMeshDev), st) != 0 ||
+        mba_host::upload_tables(d_offf, offf.data(), offf.size() * sizeof(int64_t), st) != 0 ||
+        mba_host::upload_tables(d_offe, offe.data(), offe.size() * sizeof(int64_t), st) != 0)
+        return MBA_ERR_LAUNCH;
+    cpx<T>* vf = (cpx<T>*)(base + L.vf);
+    hipLaunchKernelGGL(k_ns_rhs_batched<T>, dim3((unsigned)((offf[n] + 255) / 256)), dim3(256), 0, st, rec, rule, nq,
+                       (const RhsMeshDev*)d_mesh, n, (const int64_t*)d_offf, vf);
+    MBA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_gather_rhs_batched<T>, dim3((unsigned)((offe[n] + 255) / 256)), dim3(256), 0, st,
+                       (const RhsMeshDev*)d_mesh, n, (const int64_t*)d_offe, (const cpx<T>*)vf);
+    MBA_LAUNCH_CHECK();
+    return MBA_OK;
+}
+
+inline size_t gather_batched_layout(int n, size_t* off_tiles)
+{
+    *off_tiles = ((size_t)n * sizeof(GatherJobDev) + 255) & ~(size_t)255;
+    return *off_tiles + (size_t)(n + 1) * sizeof(int64_t);
+}
+
 extern "C" {
 
 const char* mba_status_string(int status)
@@ -1284,6 +1454,91 @@ int mba_mom_zmat_faceedge_ns_batched(int prec, const void* verts, const int32_t*
             return zmat_ns_batched<float>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace,
                                           S(stream));
         return zmat_ns_batched<double>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
+    });
+}
+
+size_t mba_mom_rhs_ns_batched_workspace_bytes(int prec, int64_t total_faces, const mba_ns_rhs_mesh* meshes,
+                                              int32_t n_meshes)
+{
+    if (!meshes || total_faces <= 0 || n_meshes <= 0 || (prec != MBA_F32 && prec != MBA_F64)) return 0;
+    for (int m = 0; m < n_meshes; ++m)
+        if (meshes[m].n_faces <= 0 || meshes[m].ndir <= 0) return 0;
+    return rhs_batched_layout(prec, total_faces, meshes, n_meshes).total;
+}
+
+int mba_mom_rhs_ns_batched(int prec, const void* verts, const int32_t* faces, int64_t total_faces,
+                           const double* rule, int32_t nq, const mba_ns_rhs_mesh* meshes, int32_t n_meshes,
+                           void* workspace, size_t workspace_bytes, mba_stream_t stream)
+{
+    if (!verts || !faces || !rule || !meshes || !workspace) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    if (n_meshes <= 0 || total_faces <= 0 || total_faces > kMaxFaces) return MBA_ERR_SIZE;
+    if (nq < 1 || nq > 64) return MBA_ERR_SIZE;
+    int64_t sum = 0, items_f = 0, items_e = 0;
+    for (int m = 0; m < n_meshes; ++m) {
+        const mba_ns_rhs_mesh& d = meshes[m];
+        if (d.n_faces <= 0 || d.face_off != sum || d.ndir <= 0 || d.n_edges <= 0) return MBA_ERR_SIZE;
+        if (!d.dirs || !d.pols || !d.rwg_fe || !d.v) return MBA_ERR_ARG;
+        sum += d.n_faces;
+        items_f += d.n_faces * d.ndir;
+        items_e += d.n_edges * d.ndir;
+    }
+    if (sum != total_faces) return MBA_ERR_ARG;
+    if ((items_f + 255) / 256 > 0x7fffffff || (items_e + 255) / 256 > 0x7fffffff) return MBA_ERR_SIZE;
+    if (workspace_bytes < mba_mom_rhs_ns_batched_workspace_bytes(prec, total_faces, meshes, n_meshes))
+        return MBA_ERR_ALLOC;
+    return mba_guard([&] {
+        if (prec == MBA_F32)
+            return rhs_ns_batched<float>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
+        return rhs_ns_batched<double>(verts, faces, total_faces, rule, nq, meshes, n_meshes, workspace, S(stream));
+    });
+}
+
+size_t mba_mom_gather_rwg_batched_scratch_bytes(int32_t n_jobs)
+{
+    if (n_jobs <= 0) return 0;
+    size_t o = 0;
+    return gather_batched_layout(n_jobs, &o);
+}
+
+int mba_mom_gather_rwg_batched(int prec, const mba_gather_job* jobs, int32_t n_jobs, void* scratch,
+                               size_t scratch_bytes, mba_stream_t stream)
+{
+    if (!jobs || !scratch || n_jobs <= 0) return MBA_ERR_ARG;
+    if (prec != MBA_F32 && prec != MBA_F64) return MBA_ERR_ARG;
+    for (int q = 0; q < n_jobs; ++q) {
+        const mba_gather_job& J = jobs[q];
+        if (!J.zf || !J.rwg_fe || !J.z) return MBA_ERR_ARG;
+        if (J.n_edges <= 0 || J.n_edges > ((int64_t)1 << 31) - 1) return MBA_ERR_SIZE;
+        if (J.ldz < J.n_edges || J.ldzf < 1) return MBA_ERR_ARG;
+    }
+    if (scratch_bytes < mba_mom_gather_rwg_batched_scratch_bytes(n_jobs)) return MBA_ERR_ALLOC;
+    return mba_guard([&]() -> int {
+        std::vector<GatherJobDev> dj(n_jobs);
+        std::vector<int64_t> toff(n_jobs + 1, 0);
+        for (int q = 0; q < n_jobs; ++q) {
+            const mba_gather_job& J = jobs[q];
+            const int rows = gather_rows(J.n_edges);
+            dj[q] = GatherJobDev{J.zf, J.ldzf, J.rwg_fe, J.n_edges, J.z, J.ldz, rows};
+            toff[q + 1] = toff[q] + ((J.n_edges + 255) / 256) * ((J.n_edges + rows - 1) / rows);
+        }
+        const int64_t grid = (toff[n_jobs] + GATHER_XCD - 1) / GATHER_XCD * GATHER_XCD;
+        if (grid > 0x7fffffff) return MBA_ERR_SIZE;
+        size_t o_t = 0;
+        gather_batched_layout(n_jobs, &o_t);
+        char* base = (char*)scratch;
+        hipStream_t st = S(stream);
+        if (mba_host::upload_tables(base, dj.data(), dj.size() * sizeof(GatherJobDev), st) != 0 ||
+            mba_host::upload_tables(base + o_t, toff.data(), toff.size() * sizeof(int64_t), st) != 0)
+            return MBA_ERR_LAUNCH;
+        if (prec == MBA_F32)
+            hipLaunchKernelGGL(k_gather_rwg_batched<float>, dim3((unsigned)grid), dim3(256), 0, st,
+                               (const GatherJobDev*)base, n_jobs, (const int64_t*)(base + o_t));
+        else
+            hipLaunchKernelGGL(k_gather_rwg_batched<double>, dim3((unsigned)grid), dim3(256), 0, st,
+                               (const GatherJobDev*)base, n_jobs, (const int64_t*)(base + o_t));
+        MBA_LAUNCH_CHECK();
+        return MBA_OK;
     });
 }
 

@@ -55,6 +55,21 @@ class LuSolveJob(ctypes.Structure):
                 ("b", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("nrhs", ctypes.c_int64), ("work", ctypes.c_void_p)]
 
 
+class NsRhsMesh(ctypes.Structure):
+    """mba_ns_rhs_mesh of include/mesh_bem_amd.h."""
+
+    _fields_ = [("n_faces", ctypes.c_int64), ("face_off", ctypes.c_int64), ("wavenumber", ctypes.c_double),
+                ("dirs", ctypes.c_void_p), ("pols", ctypes.c_void_p), ("ndir", ctypes.c_int64),
+                ("rwg_fe", ctypes.c_void_p), ("n_edges", ctypes.c_int64), ("v", ctypes.c_void_p)]
+
+
+class GatherJob(ctypes.Structure):
+    """mba_gather_job of include/mesh_bem_amd.h."""
+
+    _fields_ = [("zf", ctypes.c_void_p), ("ldzf", ctypes.c_int64), ("rwg_fe", ctypes.c_void_p),
+                ("n_edges", ctypes.c_int64), ("z", ctypes.c_void_p), ("ldz", ctypes.c_int64)]
+
+
 class ConvertJob(ctypes.Structure):
     """mba_convert_job of include/mesh_bem_amd.h."""
 
@@ -148,6 +163,10 @@ SIGNATURES = {
     "mba_residual_c128": (_c.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "mba_gemv": (_c.c_int, [_c.c_int, _vp, _i64, _i64, _vp, _vp, _vp]),
     "mba_convert_c128_c64": (_c.c_int, [_vp, _i64, _i64, _i64, _vp, _i64, _vp]),
+    "mba_mom_rhs_ns_batched_workspace_bytes": (_sz, [_c.c_int, _i64, _vp, _i32]),
+    "mba_mom_rhs_ns_batched": (_c.c_int, [_c.c_int, _vp, _vp, _i64, _vp, _i32, _vp, _i32, _vp, _sz, _vp]),
+    "mba_mom_gather_rwg_batched_scratch_bytes": (_sz, [_i32]),
+    "mba_mom_gather_rwg_batched": (_c.c_int, [_c.c_int, _vp, _i32, _vp, _sz, _vp]),
     "mba_convert_c128_c64_batched_scratch_bytes": (_sz, [_i32]),
     "mba_convert_c128_c64_batched": (_c.c_int, [_vp, _i32, _vp, _sz, _vp]),
     "mba_refine_update_batched_scratch_bytes": (_sz, [_vp, _i32]),

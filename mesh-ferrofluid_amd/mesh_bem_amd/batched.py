@@ -207,19 +207,51 @@ class BatchedMoM:
                                                   _lib.ptr(rule), rule.shape[0], desc, n, _lib.ptr(ws), wsb,
                                                   _lib.stream())
         _lib.check(st, "mba_mom_zmat_faceedge_ns_batched")
-        Zs = []
-        for m, zf in zip(self.meshes, zfs):
-            E = m.n_edges
-            Z = torch.empty((E, E), dtype=cdt, device=self.device)
-            st = lib.mba_mom_gather_rwg(prec, _lib.ptr(zf), zf.stride(0), 0, zf.shape[1], _lib.ptr(m.rwg_fe), E,
-                                        _lib.ptr(Z), Z.stride(0), _lib.stream())
-            _lib.check(st, "mba_mom_gather_rwg")
-            Zs.append(Z)
+        # every mesh's RWG gather in one launch
+        Zs = [torch.empty((m.n_edges, m.n_edges), dtype=cdt, device=self.device) for m in self.meshes]
+        jobs = (_lib.GatherJob * n)()
+        for i, (m, zf, Z) in enumerate(zip(self.meshes, zfs, Zs)):
+            jobs[i].zf, jobs[i].ldzf = zf.data_ptr(), zf.stride(0)
+            jobs[i].rwg_fe, jobs[i].n_edges = m.rwg_fe.data_ptr(), m.n_edges
+            jobs[i].z, jobs[i].ldz = Z.data_ptr(), Z.stride(0)
+        sb = lib.mba_mom_gather_rwg_batched_scratch_bytes(n)
+        gs = torch.empty(sb, dtype=torch.uint8, device=self.device)
+        _lib.check(lib.mba_mom_gather_rwg_batched(prec, jobs, n, _lib.ptr(gs), sb, _lib.stream()),
+                   "mba_mom_gather_rwg_batched")
         del zfs
         return Zs
 
     def rhs(self, dirs: List[np.ndarray], pols: List[np.ndarray], prec=_lib.MBA_F64) -> List[torch.Tensor]:
-        return rhs_ns_many(self.meshes, self.ks, prec, dirs, pols)
+        """Plane-wave right-hand sides V_m [E_m, ndir_m] of every mesh: the
+        directions and polarisations go to the device in one copy each, the
+        face-edge integrals and RWG sums of all meshes are one launch each."""
+        lib = _lib.load()
+        cdt = torch.complex64 if prec == _lib.MBA_F32 else torch.complex128
+        rdt = torch.float32 if prec == _lib.MBA_F32 else torch.float64
+        n = len(self.meshes)
+        ds = [np.atleast_2d(np.asarray(d, dtype=np.float64)) for d in dirs]
+        ps = [np.atleast_2d(np.asarray(p, dtype=np.float64)) for p in pols]
+        if len(ds) != n or len(ps) != n or any(d.shape != p.shape or d.shape[1] != 3 for d, p in zip(ds, ps)):
+            raise ValueError("rhs: dirs and pols [ndir, 3] per mesh")
+        D = torch.from_numpy(np.ascontiguousarray(np.concatenate(ds))).to(self.device)
+        P = torch.from_numpy(np.ascontiguousarray(np.concatenate(ps))).to(self.device)
+        rule = rule_tensor(self.device)
+        Vs = [torch.empty((m.n_edges, d.shape[0]), dtype=cdt, device=self.device) for m, d in zip(self.meshes, ds)]
+        desc = (_lib.NsRhsMesh * n)()
+        off = 0
+        for i, (m, d, V) in enumerate(zip(self.meshes, ds, Vs)):
+            desc[i].n_faces, desc[i].face_off, desc[i].wavenumber = m.n_faces, self.face_off[i], self.ks[i]
+            desc[i].dirs, desc[i].pols = D[off].data_ptr(), P[off].data_ptr()
+            desc[i].ndir = d.shape[0]
+            desc[i].rwg_fe, desc[i].n_edges, desc[i].v = m.rwg_fe.data_ptr(), m.n_edges, V.data_ptr()
+            off += d.shape[0]
+        verts = self.verts.to(rdt).contiguous()
+        wsb = lib.mba_mom_rhs_ns_batched_workspace_bytes(prec, self.total_faces, desc, n)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=self.device)
+        _lib.check(lib.mba_mom_rhs_ns_batched(prec, _lib.ptr(verts), _lib.ptr(self.faces), self.total_faces,
+                                              _lib.ptr(rule), rule.shape[0], desc, n, _lib.ptr(ws), wsb,
+                                              _lib.stream()), "mba_mom_rhs_ns_batched")
+        return Vs
 
     def solve(self, Zs: List[torch.Tensor], Vs: List[torch.Tensor], tuner: Optional[GemmTuner] = None,
               tol: float = 1e-8, maxit: int = 10, persistent: bool = True,

@@ -13,7 +13,7 @@ import torch
 import oracle as O
 from mesh_bem_amd import _lib
 from mesh_bem_amd.batched import BatchedMoM, GemmTuner, star_batch
-from mesh_bem_amd.mom import assemble_ns
+from mesh_bem_amd.mom import assemble_ns, rhs_ns
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +27,11 @@ def test_batched_assembly_equals_single(dev):
             mm = m if prec == _lib.MBA_F64 else type(m)(**{**m.__dict__, "verts": m.verts.to(torch.float32)})
             Z1 = assemble_ns(mm, k, prec)
             assert torch.equal(Z, Z1)
+        # right-hand sides of all meshes in one pass: the per-mesh values exactly
+        Vs = bm.rhs(dirs, pols, prec)
+        for m, k, d, p, V in zip(bm.meshes, ks, dirs, pols, Vs):
+            mm = m if prec == _lib.MBA_F64 else type(m)(**{**m.__dict__, "verts": m.verts.to(torch.float32)})
+            assert torch.equal(V, rhs_ns(mm, k, prec, d, p))
 
 
 def test_batched_matches_oracle_and_solves(dev):
