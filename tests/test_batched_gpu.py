@@ -149,6 +149,33 @@ def test_lu_solve_batched_persistent_many_jobs(dev):
     assert worst_e < 1e-5 and worst_r < 1e-5
 
 
+def test_staged_tables_many_calls_in_flight(dev):
+    """Host job tables go through the library's pinned staging ring (64 slots
+    per device, a slot reused once its copy has completed): 200 grouped
+    products queued without a synchronisation, each with its own table
+    (two uploads per call, so the ring wraps six times), all give the
+    product of their own operands."""
+    from mesh_bem_amd.dense import gemm_grouped
+
+    print("200 grouped products in flight", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    ops = []
+    for q in range(200):
+        m, n, k = 17 + q % 13, 9 + q % 7, 5 + q % 11
+        A = torch.randn((m, k), generator=g, dtype=torch.complex64).to(dev)
+        B = torch.randn((k, n), generator=g, dtype=torch.complex64).to(dev)
+        ops.append((A, B, torch.zeros((m, n), dtype=torch.complex64, device=dev)))
+    refs = [A.cpu().to(torch.complex128) @ B.cpu().to(torch.complex128) for A, B, _ in ops]
+    torch.cuda.synchronize()
+    for A, B, C in ops:  # no host copy or synchronisation between the calls
+        gemm_grouped([(A, B, C)], alpha=1.0, beta=0.0)
+    outs = [C for _, _, C in ops]
+    torch.cuda.synchronize()
+    worst = max(float((C.cpu().to(torch.complex128) - R).norm() / R.norm()) for C, R in zip(outs, refs))
+    print(f"  worst relative error {worst:.2e}")
+    assert worst < 1e-5
+
+
 def test_batched_refinement_helpers(dev):
     """One-launch helpers of the batched refinement: complex128 -> complex64
     conversions equal torch's elementwise conversion bit for bit; J += D
