@@ -821,6 +821,68 @@ __device__ __forceinline__ void block_argmax(T& v, int32_t& r, int32_t& q, T* re
     for (int w = 1; w < NW; ++w) better(v, r, q, pv[w], pr[w], pq[w]);
 }
 
+// The same arg-max for FP32 values in one 64-bit key per lane: the value's
+// bits (a non-negative float ranks as its bit pattern) over the complement
+// of the row (the smaller row wins a tie); key 0 is "no candidate" (the empty
+// value -1, and NaN, which never wins).  One DPP chain on two words instead
+// of three values, one LDS word per wave.
+__device__ __forceinline__ uint64_t am_key(float v, int32_t r)
+{
+    if (!(v >= 0.f)) return 0ull;
+    return ((uint64_t)__float_as_uint(v) << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)r);
+}
+__device__ __forceinline__ void am_unkey(uint64_t k, float& v, int32_t& r, int32_t none)
+{
+    if (k == 0ull) {
+        v = -1.f;
+        r = none;
+    } else {
+        v = __uint_as_float((uint32_t)(k >> 32));
+        r = (int32_t)(0xFFFFFFFFu - (uint32_t)k);
+    }
+}
+template <int CTRL>
+__device__ __forceinline__ void key_step(uint64_t& k)
+{
+    const uint32_t lo = (uint32_t)dpp_i<CTRL>((int)(uint32_t)k), hi = (uint32_t)dpp_i<CTRL>((int)(uint32_t)(k >> 32));
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    k = o > k ? o : k;
+}
+__device__ __forceinline__ uint64_t wave_max_key(uint64_t k)
+{
+    key_step<0xB1>(k);   // quad_perm [1, 0, 3, 2]
+    key_step<0x4E>(k);   // quad_perm [2, 3, 0, 1]
+    key_step<0x141>(k);  // row_half_mirror
+    key_step<0x140>(k);  // row_mirror
+    const uint32_t lo = (uint32_t)k, hi = (uint32_t)(k >> 32);
+    uint64_t best = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, 0) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)lo, 0);
+#pragma unroll
+    for (int l = 16; l < 64; l += 16) {
+        const uint64_t o = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)hi, l) << 32) |
+                           (uint32_t)__builtin_amdgcn_readlane((int)lo, l);
+        best = o > best ? o : best;
+    }
+    return best;
+}
+// block-wide key maximum: one barrier; red holds 2 x NW words, two calls not
+// separated by another barrier must use different slots
+template <int NW>
+__device__ __forceinline__ uint64_t block_max_key(uint64_t k, uint64_t* red, int slot)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    k = wave_max_key(k);
+    if (lane == 0) red[slot * NW + wave] = k;
+    __syncthreads();
+    uint64_t b = red[slot * NW];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) {
+        const uint64_t o = red[slot * NW + w];
+        b = o > b ? o : b;
+    }
+    return b;
+}
+
 // Waits until all N tagged words carry `epoch`; the N loads are issued
 // together, so a wait costs one memory round trip, not N.  Only words still
 // stale are re-read.  Bounded like tag_get.
@@ -933,6 +995,8 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
     __shared__ T red_v[2 * NT / 64];
     __shared__ int32_t red_r[2 * NT / 64];
     __shared__ int32_t red_q[2 * NT / 64];
+    __shared__ uint64_t red_k[2 * NT / 64];  // FP32: packed arg-max keys
+    constexpr bool KEYED = sizeof(T) == 4;
     __shared__ cpx<T> s_prow[128];
     __shared__ cpx<T> s_drow[128];
 
@@ -960,8 +1024,12 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 br = (int32_t)(lr0 + r);
             }
         }
-        int32_t bq = 0;
-        block_argmax<NT / 64>(bv, br, bq, red_v, red_r, red_q, 0);
+        if constexpr (KEYED) {
+            am_unkey(block_max_key<NT / 64>(am_key((float)bv, br), red_k, 0), *(float*)&bv, br, NONE);
+        } else {
+            int32_t bq = 0;
+            block_argmax<NT / 64>(bv, br, bq, red_v, red_r, red_q, 0);
+        }
         const int32_t lb = br;
         const uint32_t e0 = (uint32_t)(P.k + 1);
         if (tid == 0) {
@@ -987,7 +1055,13 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             tag_get_n<1 + VW>(pp, epoch, P.err, wd);
             better(v, r, q, val_of<T>(wd + 1), (int32_t)wd[0], (int32_t)c);
         }
-        block_argmax<NT / 64>(v, r, q, red_v, red_r, red_q, 1);
+        if constexpr (KEYED) {
+            // the winner's workgroup is the owner of its row: rows [g R, g R + R)
+            am_unkey(block_max_key<NT / 64>(am_key((float)v, r), red_k, 1), *(float*)&v, r, NONE);
+            q = r == NONE ? 0 : r / R;
+        } else {
+            block_argmax<NT / 64>(v, r, q, red_v, red_r, red_q, 1);
+        }
         const int wg = q;
         const int32_t p = r;
         // --- (2) the winner's row and row j
@@ -1064,8 +1138,12 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 for (int c = tid; c < nb; c += NT) rows[(j - lr0) * nb + c] = s_prow[c];
         }
         if (j1 >= nb) break;
-        int32_t bq = 0;
-        block_argmax<NT / 64>(bv, br, bq, red_v, red_r, red_q, 0);  // its barrier orders (3) before (4)
+        if constexpr (KEYED) {  // its barrier orders (3) before (4)
+            am_unkey(block_max_key<NT / 64>(am_key((float)bv, br), red_k, 0), *(float*)&bv, br, NONE);
+        } else {
+            int32_t bq = 0;
+            block_argmax<NT / 64>(bv, br, bq, red_v, red_r, red_q, 0);
+        }
         const int32_t lb = br;
         const uint32_t e1 = epoch + 1;
         const int par1 = j1 & 1;
