@@ -1179,33 +1179,45 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 put_val(dw + (2 * c + 1) * VW, e1, xd.im);
             }
         }
-        // --- (5) rank-1 update of columns > j + 1 for the other rows, UB rows
-        // per step: the loads use clamped row indices (no branches), so all
-        // 2 UB LDS reads of a step issue before the first is waited for
+        // --- (5) rank-1 update of columns > j + 1 for the other rows: the
+        // (row, column) pairs are dealt to the threads in row-major order, so
+        // every thread has work whatever the number of columns left; UB pairs
+        // per step, all 3 UB LDS reads of a step issued before the first is
+        // waited for (clamped indices, no branches)
         {
             constexpr int UB = 8;
-            const int c = tid % nb;
-            const int rstep = NT / nb;
-            if (c > j1 && tid < rstep * nb) {
-                const cpx<T> u = s_prow[c];
-                for (int rb = tid / nb; rb < nr; rb += rstep * UB) {
-                    cpx<T> l[UB], a[UB];
+            const int c0 = j1 + 1, ncol = nb - c0;
+            const int total = nr * ncol;
+            const float rinv = ncol > 0 ? 1.f / (float)ncol : 0.f;
+            for (int e0 = tid; e0 < total; e0 += NT * UB) {
+                int rq[UB], cq[UB];
+                cpx<T> l[UB], a[UB], u[UB];
 #pragma unroll
-                    for (int q = 0; q < UB; ++q) {
-                        const int rr = rb + q * rstep;
-                        const int rc = rr < nr ? rr : 0;
-                        l[q] = rows[rc * nb + j];
-                        a[q] = rows[rc * nb + c];
+                for (int q = 0; q < UB; ++q) {
+                    const int e = e0 + q * NT;
+                    const int ec = e < total ? e : 0;
+                    int rr = (int)((float)ec * rinv);  // row = ec / ncol, corrected below
+                    int cc = ec - rr * ncol;
+                    if (cc < 0) {
+                        --rr;
+                        cc += ncol;
+                    } else if (cc >= ncol) {
+                        ++rr;
+                        cc -= ncol;
                     }
+                    rq[q] = rr;
+                    cq[q] = c0 + cc;
+                    l[q] = rows[rr * nb + j];
+                    a[q] = rows[rr * nb + c0 + cc];
+                    u[q] = s_prow[c0 + cc];
+                }
 #pragma unroll
-                    for (int q = 0; q < UB; ++q) {
-                        const int rr = rb + q * rstep;
-                        const int64_t pr = lr0 + rr;
-                        if (rr < nr && pr > j && pr != lb && pr != j1) {
-                            a[q].re -= l[q].re * u.re - l[q].im * u.im;
-                            a[q].im -= l[q].re * u.im + l[q].im * u.re;
-                            rows[rr * nb + c] = a[q];
-                        }
+                for (int q = 0; q < UB; ++q) {
+                    const int64_t pr = lr0 + rq[q];
+                    if (e0 + q * NT < total && pr > j && pr != lb && pr != j1) {
+                        a[q].re -= l[q].re * u[q].re - l[q].im * u[q].im;
+                        a[q].im -= l[q].re * u[q].im + l[q].im * u[q].re;
+                        rows[rq[q] * nb + cq[q]] = a[q];
                     }
                 }
             }
