@@ -311,3 +311,37 @@ def test_pivots_to_perm_sequential(dev, n):
     _lib.check(_lib.load().mba_lu_pivots_to_perm(_lib.ptr(ip), n, _lib.ptr(perm), _lib.stream()),
                "mba_lu_pivots_to_perm")
     assert np.array_equal(perm.cpu().numpy().astype(np.int64), want)
+
+
+def test_lu_async_workgroup_cap(dev):
+    """The asynchronous LU with its panels capped at a few workgroups (the
+    panel width then shrinks so the capped workgroups' rows fit the LDS):
+    n = 4096 with 16 workgroups per panel (64-column panels) factors to FP32
+    backward error and solves like the uncapped factorisation (the pivots
+    part ways at the first near-tie that the other blocking rounds
+    differently, so they are not compared); a cap that leaves even 16-column
+    panels too tall for the LDS is refused before any launch."""
+    from mesh_bem_amd.dense import LUFactor
+
+    print("LU with max_wg = 16 at n = 4096", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(8)
+    n = 4096
+    A = torch.complex(torch.randn((n, n), generator=g), torch.randn((n, n), generator=g)).to(torch.complex64).to(dev)
+    lu = LUFactor(A, asynchronous=True, max_wg=16)
+    torch.cuda.synchronize()
+    lu.check()
+    ref = LUFactor(A)
+    same = float((lu.ipiv == ref.ipiv).float().mean())
+    b = torch.complex(torch.randn(n, generator=g), torch.randn(n, generator=g)).to(torch.complex64).to(dev)
+    x, x1 = lu.solve(b), ref.solve(b)
+    A64, b64 = A.to(torch.complex128), b.to(torch.complex128)
+    r = float((A64 @ x.to(torch.complex128) - b64).norm() / (A64.norm() * x.to(torch.complex128).norm()))
+    x64 = torch.linalg.solve(A64, b64)
+    e = float((x.to(torch.complex128) - x64).norm() / x64.norm())
+    e1 = float((x1.to(torch.complex128) - x64).norm() / x64.norm())
+    print(f"  backward error {r:.2e}, pivots equal {same:.4f}, forward error {e:.2e} (uncapped {e1:.2e})")
+    assert r < 1e-6 and e < 10 * e1 + 1e-6
+    # 2 workgroups of 8192 rows: 8192 x 16 columns x 8 bytes exceed the LDS
+    big = torch.empty((16384, 16384), dtype=torch.complex64, device=dev)
+    with pytest.raises(_lib.MBAError):
+        LUFactor(big, overwrite=True, asynchronous=True, max_wg=2)
