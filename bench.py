@@ -983,7 +983,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE={world}); "
-              "the distributed layout is described in DESIGN.md", file=sys.stderr)
+              "a multi-GPU run would shard the batch of meshes, one solve per GPU", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

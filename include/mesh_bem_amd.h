@@ -140,7 +140,8 @@ int mba_mom_vertex_sum(int prec, const void *fvj, const int64_t *vinc_ptr,
 /* Bytes of device workspace the ns-rule calls need for n_faces faces. */
 size_t mba_mom_workspace_bytes(int prec, int64_t n_faces);
 
-/* Face-edge EFIE blocks, north-star rule (no reference counterpart; the RWG
+/* Face-edge EFIE blocks, NS rule: the 7-point degree-5 triangle rule with
+ * analytic FP64 1/R terms on near pairs (no reference counterpart; the RWG
  * space result replaces the Zmn of mesh_BEM_solver_3d.py:128-147):
  *   Z_ij = c_i c_j [ jk <(x-v_i).(y-v_j) G> - (4j/k) <G> ],  G = e^{-jkR}/(4 pi R)
  * rule: [nq, 4] double (l0, l1, l2, w), physical weight w * 2A.
@@ -158,7 +159,7 @@ int mba_mom_zmat_faceedge_ns(int prec, const void *verts, const int32_t *faces, 
                              int64_t nb, void *zf, int64_t ldzf, void *workspace,
                              size_t workspace_bytes, mba_stream_t stream);
 
-/* Face-edge plane-wave RHS, north-star rule:
+/* Face-edge plane-wave RHS, NS rule:
  *   vf[3f+e, d] = sum_i w_i c_e (x_i - v_e).p_d e^{-jk khat_d.x_i}
  * dirs, pols: [ndir, 3] double (unit propagation and real polarisation).
  * vf: [3F, ndir] complex. */
@@ -183,7 +184,7 @@ int mba_mom_gather_rwg(int prec, const void *zf, int64_t ldzf, int64_t col0, int
 int mba_mom_gather_rhs(int prec, const void *vf, const int32_t *rwg_fe, int64_t n_edges,
                        int64_t ndir, void *v, mba_stream_t stream);
 
-/* Several meshes in one assembly launch (north-star EFIE rule).  verts / faces
+/* Several meshes in one assembly launch (NS EFIE rule).  verts / faces
  * are the concatenation of the meshes (faces hold indices into the
  * concatenated verts; mesh m owns faces [face_off, face_off + n_faces), face_off
  * equal to the running sum); every mesh writes its own face-edge block
@@ -208,7 +209,7 @@ int mba_mom_zmat_faceedge_ns_batched(int prec, const void *verts, const int32_t 
                                      int32_t n_meshes, void *workspace, size_t workspace_bytes,
                                      mba_stream_t stream);
 
-/* Plane-wave right-hand sides of several meshes in one pass (north-star rule,
+/* Plane-wave right-hand sides of several meshes in one pass (NS rule,
  * as mba_mom_rhs_faceedge_ns followed by mba_mom_gather_rhs per mesh): verts /
  * faces concatenated as for mba_mom_zmat_faceedge_ns_batched (face_off the
  * running sum of n_faces); mesh m's directions and polarisations dirs / pols
@@ -216,7 +217,7 @@ int mba_mom_zmat_faceedge_ns_batched(int prec, const void *verts, const int32_t 
  * indices 3 f + e) and its output v [n_edges, ndir] complex (prec).  Replaces
  * the per-mesh create_MOM_rhs_3d_forward calls of the reference's batch loop
  * (demo/demo_Helmholtz_MOM_3d.py:206-216 -> mesh_MOM_3d.cpp:3-36) with the
- * north-star incidence; the descriptor array is host memory (staged). */
+ * NS incidence; the descriptor array is host memory (staged). */
 typedef struct {
     int64_t n_faces;
     int64_t face_off;
@@ -569,7 +570,7 @@ int mba_laplace_pack(int prec, const void *unpacked, int64_t n_faces, int32_t np
 int mba_laplace_dirichlet_rhs(int prec, const void *k_tight, int64_t n_verts, const void *g,
                               const void *vert_areas, void *rhs, mba_stream_t stream);
 
-/* North-star P1 Galerkin single and double layer (no reference counterpart:
+/* NS P1 Galerkin single and double layer (no reference counterpart:
  * the reference's layers are double_integration_on_panels_3d,
  * src/mesh_BEM/utils/integrations_3d.py:252-452, whose collapsed rule is
  * mba_laplace_layer_ref).  Unpacked layout of mba_laplace_layer_ref with
@@ -593,7 +594,7 @@ int mba_laplace_mass_apply(int prec, const void *verts, const int32_t *faces, co
                            const int32_t *vinc_idx, int64_t n_verts, const void *g, double alpha, double beta,
                            void *out, mba_stream_t stream);
 
-/* P0 Galerkin Laplace operators, north-star rule (no reference counterpart):
+/* P0 Galerkin Laplace operators, NS rule (no reference counterpart):
  *   V[i, j]  = int_Ti int_Tj 1/(4 pi R)
  *   Kt[i, j] = int_Ti int_Tj n_i . grad_x 1/(4 pi R)   (adjoint double layer)
  * 7-point (or 3-point) rule on far pairs, analytic FP64 inner integrals on
@@ -841,7 +842,7 @@ int mba_nearest_transfer(int prec, const void *points, int64_t n_points, const v
 /* Vector-ALU issue rates of the current device, measured (the call
  * synchronises): out[4] = operations per second of FP32 FMA (not packed),
  * v_sin_f32, v_cos_f32 and v_rsq_f32 -- the rates the FP32 assembly kernels
- * are judged against.  blocks_per_cu: workgroups of 256 threads per CU (0:
+ * are measured against.  blocks_per_cu: workgroups of 256 threads per CU (0:
  * 8).  No reference counterpart (a measurement of the device). */
 int mba_measure_valu_rates(double *out, int blocks_per_cu, mba_stream_t stream);
 

@@ -8,7 +8,7 @@
 //   single layer f = 1/(4 pi R), double layer f = (x - y).n_fy / (4 pi R^3).
 // One lane per column face fx, a uniform loop over a block of row faces fy.
 //
-// North-star rule (P0 Galerkin, 7-point rule, analytic FP64 inner integrals on
+// NS rule (P0 Galerkin, 7-point rule, analytic FP64 inner integrals on
 // near pairs):
 //   V[i, j]  = int_Ti int_Tj 1/(4 pi R)
 //   Kt[i, j] = int_Ti int_Tj d/dn_x 1/(4 pi R) = -int int (x - y).n_i / (4 pi R^3)
@@ -149,7 +149,7 @@ __global__ void k_dirichlet_rhs(const T* __restrict__ Kt, int64_t V, const T* __
     rhs[v] = (T)0.5 * g[v] * va[v] + acc;
 }
 
-// ------------------------------------------------------------ north-star P0
+// ------------------------------------------------------------ NS P0
 // Analytic inner integrals over triangle Q at point r (FP64):
 //   S0 = int 1/R,  grad S0 = -sum_i u_i f_i - n sgn(d) sum_i beta_i.
 __device__ void tri_S0_grad(const double Q[3][3], const double r[3], double& S0, double G[3])
@@ -358,7 +358,7 @@ __global__ void k_p0_near(const P0Rec<T>* __restrict__ rec, int64_t F, const dou
     }
 }
 
-// ------------------------------------------------------------ north-star P1
+// ------------------------------------------------------------ NS P1
 // Galerkin P1 layers in the unpacked layout of the reference rule,
 //   Vu[fy, fx, iy, ix] = int_Tx int_Ty phi_ix(x) phi_iy(y) / (4 pi R)
 //   Ku[fy, fx, iy, ix] = int_Tx int_Ty phi_ix(x) phi_iy(y) (x - y).n_y / (4 pi R^3)

@@ -1,4 +1,4 @@
-// MoM assembly kernels for gfx950: face records, the north-star EFIE face-pair
+// MoM assembly kernels for gfx950: face records, the NS EFIE face-pair
 // kernel, the FP64 near-singular correction, the reference-rule CFIE kernel,
 // plane-wave and reference RHS, RWG gather, vertex interpolation.
 //

@@ -106,7 +106,7 @@ class MeshPressure:
                  gaussQR: int, order_type: int, logger=None, device=torch.device("cpu"),
                  dtype: torch.dtype = torch.float32, *, bem_mode: int = None) -> None:
         """bem_mode (keyword-only, new): AssemblyMode of the Laplace solve;
-        default the reference's rule (REF), NS for the north-star P1 layers."""
+        default the reference's rule (REF), NS for the NS P1 layers."""
         from .logger import Logger
         from .solver import MeshBEMSolver3d
         from .types import BEMType

@@ -15,7 +15,7 @@ src/mesh_BEM/mesh_BEM_solver_3d.py:197-373):
   layer on the GPU; ``pack`` forms the [V, V] matrix, ``dirichlet_rhs`` the
   right-hand side 1/2 M g + K^T g.
 
-North-star P1 rule (``MeshBEMSolver3d(..., mode=AssemblyMode.NS).solve``):
+NS P1 rule (``MeshBEMSolver3d(..., mode=AssemblyMode.NS).solve``):
 ``p1_operators`` returns the unpacked P1 Galerkin single and double layer
 (7-point rule, FP64 closed-form inner integrals on near pairs) in the layout
 of ``layer_ref``; the Dirichlet system uses the consistent P1 mass matrix
@@ -23,7 +23,7 @@ of ``layer_ref``; the Dirichlet system uses the consistent P1 mass matrix
 the lumped mass the exact layers miss the demo's known answer by tens of
 per cent, with the consistent one they converge to it.
 
-North-star rule for the magnetic scalar potential: ``p0_operators`` returns
+NS rule for the magnetic scalar potential: ``p0_operators`` returns
 the P0 Galerkin single layer V and adjoint double layer K' (7-point rule, FP64
 analytic inner integrals on near pairs); ``magnetic_charge`` solves
 [(1/chi + 1/2) M + K'] sigma = M (H0 . n) for the surface charge of a linear
@@ -202,7 +202,7 @@ def laplace_solve(solver, verts: torch.Tensor, faces: torch.Tensor, vert_bc_Diri
     ns = getattr(solver, "mode", int(AssemblyMode.REF)) == int(AssemblyMode.NS)
     if ns:
         if solver.order_type != int(OrderType.LINEAR):
-            raise NotImplementedError("the north-star Laplace rule is P1 (order_type LINEAR)")
+            raise NotImplementedError("the NS Laplace rule is P1 (order_type LINEAR)")
         Vu, Ku = p1_operators(verts, faces, _lib.prec_of(dt))
     else:
         gp, gw = get_gaussion_integration_points_and_weights(solver.gaussQR, dtype=torch.float64)
@@ -237,7 +237,7 @@ def laplace_solve(solver, verts: torch.Tensor, faces: torch.Tensor, vert_bc_Diri
     return torch.stack(neu, 0), torch.stack(dir_, 0)
 
 
-# ---------------------------------------------------------------- P0 (north-star rule)
+# ---------------------------------------------------------------- P0 (NS rule)
 def p0_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, want_v: bool = True,
                  want_k: bool = True, near=None, out_v=None, out_k=None):
     """P0 Galerkin single layer V [F, F] and adjoint double layer K' [F, F]."""
@@ -267,7 +267,7 @@ def p0_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, want_v: bo
     return Vm, Km
 
 
-# ---------------------------------------------------------------- P1 (north-star rule)
+# ---------------------------------------------------------------- P1 (NS rule)
 def p1_operators(verts: torch.Tensor, faces: torch.Tensor, prec: int, near=None, want_v: bool = True,
                  want_k: bool = True):
     """Unpacked P1 Galerkin single and double layer [F, F, 3, 3] (layout of

@@ -692,7 +692,7 @@ void orc_laplace_p0(const double *verts, const int32_t *faces, int64_t F, const 
 }
 
 /* ------------------------------------------------------------ P1 Laplace (ns)
- * North-star P1 Galerkin layers, unpacked like orc_laplace_layer:
+ * NS P1 Galerkin layers, unpacked like orc_laplace_layer:
  *   Vu[fy, fx, iy, ix] = int_Tx int_Ty phi_ix(x) phi_iy(y) / (4 pi R)
  *   Ku[fy, fx, iy, ix] = int_Tx int_Ty phi_ix(x) phi_iy(y) (x - y).n_y / (4 pi R^3)
  * Far pairs: the (lam, w) rule on both triangles.  Near pairs (fx in the

@@ -523,7 +523,7 @@ def laplace_p0(verts, faces, near_ptr, near_idx, rule=None):
 
 
 def laplace_p1_ns(verts, faces, near_ptr, near_idx, rule=None):
-    """North-star P1 layers (Vu, Ku) [F, F, 3, 3] in the layout of laplace_layer:
+    """NS P1 layers (Vu, Ku) [F, F, 3, 3] in the layout of laplace_layer:
     7-point rule on far pairs, outer 7-point rule x analytic inner integral on
     near pairs."""
     v, f = _f64(verts), _fi32(faces)

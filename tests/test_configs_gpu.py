@@ -1,7 +1,7 @@
 """Every workload configuration at its real size on the GPU.
 
 C1  icosphere subdivision 3 (1280 faces, 1920 RWG), ka = 1, x-polarised wave
-    along +z: FP64 north-star assembly elementwise against the oracle, the
+    along +z: FP64 NS assembly elementwise against the oracle, the
     complex128 LU against LAPACK, the mixed-precision solve; the reference
     rule (Q = 4, the demo default) against the oracle's restatement of
     solve_MOM (src/mesh_BEM/mesh_BEM_solver_3d.py:96-195).

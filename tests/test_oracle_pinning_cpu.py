@@ -7,12 +7,12 @@ HIP kernels with that restatement.  These tests tie the restatement to the
 mathematics, each with a check that a dropped or sign-flipped term fails:
 
 * EFIE part: on well-separated face pairs the reference's collapsed rule and
-  the north-star 7-point rule both converge to the same smooth integral, and
-  the north-star EFIE is itself pinned (Mie series, analytic potentials), so
+  the NS 7-point rule both converge to the same smooth integral, and
+  the NS EFIE is itself pinned (Mie series, analytic potentials), so
   the reference's vector- and scalar-potential terms must agree with it to
-  quadrature accuracy.  The RHS E term likewise against the north-star RHS.
+  quadrature accuracy.  The RHS E term likewise against the NS RHS.
 * identity term: the closed-form Gram integral of two RWG functions.
-* MFIE part: its current must equal the (independent) north-star EFIE current
+* MFIE part: its current must equal the (independent) NS EFIE current
   to discretisation accuracy, and its bistatic RCS the Mie series.
 * Laplace layers: fed a proper (non-collapsed) 4-D rule and the two fixes of
   the reference's maps (shared vertex first, left-out same-face sub-domains),
@@ -63,7 +63,7 @@ def sphere2():
 
 
 # ------------------------------------------------------------------ EFIE part
-def test_ref_efie_terms_match_north_star_on_far_pairs(sphere2):
+def test_ref_efie_terms_match_ns_rule_on_far_pairs(sphere2):
     V, f, fe, rfe = sphere2
     k = 1.0
     gpx, gwx = O.gl_ref(4)
@@ -87,7 +87,7 @@ def test_ref_efie_terms_match_north_star_on_far_pairs(sphere2):
     assert rel(Zvec - Zsca) > 0.5 and rel(Zsca - Zvec) > 1e-3
 
 
-def test_ref_rhs_e_term_matches_north_star_rhs(sphere2):
+def test_ref_rhs_e_term_matches_ns_rule_rhs(sphere2):
     V, f, fe, rfe = sphere2
     k = 1.5
     gp, gw = O.gl_ref(4)
@@ -149,7 +149,7 @@ def mfie_parts(sphere2):
     return k, Zid, Zk, Vh, np.linalg.solve(Zn, Vn)
 
 
-def test_ref_mfie_current_matches_north_star_efie_current(mfie_parts):
+def test_ref_mfie_current_matches_ns_rule_efie_current(mfie_parts):
     """Two independent formulations (MFIE with the reference's rule; EFIE with
     the 7-point rule and analytic singular terms) give one current."""
     k, Zid, Zk, Vh, Ins = mfie_parts
@@ -352,9 +352,9 @@ def test_dirichlet_solve_with_exact_layers_reaches_known_answer():
     assert err(t_ref) > 1.0
 
 
-# ------------------------------------------------------------------ north-star P1 layers
+# ------------------------------------------------------------------ NS P1 layers
 def test_ns_p1_layers_against_exact_layers_and_known_answer():
-    """The north-star P1 rule (7-point rule on far pairs; rule on T_x times
+    """The NS P1 rule (7-point rule on far pairs; rule on T_x times
     closed-form inner integral on near pairs) against the exact Galerkin
     layers: far blocks to the rule's accuracy, near blocks within the outer
     rule's error; its Dirichlet solve reaches the demo's known answer as the
