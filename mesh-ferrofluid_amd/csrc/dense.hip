@@ -1931,6 +1931,264 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void k_trsm_l(cpx<T>* 
     trsm_body<T>(A, lda, k, nb, col_begin, col_end, (int64_t)blockIdx.x, X, X + (size_t)nb * TrsmCols<T>::value);
 }
 
+// ---- complex64 panel triangular solve on MFMA (panels of 64 or 128 rows).
+// Block triangular solve with inverted diagonal blocks (the scheme analysed by
+// Du Croz and Higham, IMA J. Numer. Anal. 12 (1992), and used by blocked
+// GPU trsm routines): for L11 = [D0 0; L21 D1] (64-row blocks),
+//   Y0 = D0^{-1} X0,   Y1 = D1^{-1} (X1 - L21 Y0).
+// k_trsm_prep inverts D0 and D1 (one workgroup each) and copies L21 (a third
+// workgroup), all as MFMA A operands: planes [kk][i] (real, then imaginary)
+// of 64 x 64 floats, kk the reduction index.  k_trsm_mfma then gives each
+// workgroup 64 columns of X in LDS and runs the three products on
+// v_mfma_f32_32x32x2f32 (four real products per complex one).
+constexpr int TM_PLANE = 64 * 64;          // floats per plane
+constexpr int TM_OPER = 2 * TM_PLANE;      // floats per operand (re, im)
+constexpr int TM_SET = 3 * TM_OPER;        // D0^{-1}, D1^{-1}, L21 of one panel
+constexpr int TM_SLOTS = 16;               // one set per panel of a super panel
+constexpr int TM_WS_FLOATS = TM_SLOTS * TM_SET;
+constexpr int TM_LD = 65;                  // padded LDS row of the inversion
+constexpr size_t TM_PREP_LDS = (4 * 64 * TM_LD + 2 * 3 * 16 * 17) * sizeof(float);
+
+// Inverse of the 64 x 64 unit lower triangle at (k0, k0): 16 x 16 diagonal
+// blocks by forward substitution (lane c of wave w solves column c of block
+// w), then the blocks below the diagonal one block diagonal at a time,
+//   X_ij = -X_ii sum_{t=j}^{i-1} L_it X_tj      (i > j, 16-row blocks).
+__device__ void tm_tri_inv64(const cpx<float>* __restrict__ A, int64_t lda, int64_t k0, float* __restrict__ out,
+                             float* smem)
+{
+    float* Lr = smem;
+    float* Li = Lr + 64 * TM_LD;
+    float* Xr = Li + 64 * TM_LD;
+    float* Xi = Xr + 64 * TM_LD;
+    float* Tr = Xi + 64 * TM_LD;  // [3][16][17]
+    float* Ti = Tr + 3 * 16 * 17;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    {
+        cpx<float> v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = tid + 256 * q, i = idx >> 6, t = idx & 63;
+            v[q] = A[(k0 + i) * lda + k0 + t];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = tid + 256 * q, i = idx >> 6, t = idx & 63;
+            Lr[i * TM_LD + t] = v[q].re;
+            Li[i * TM_LD + t] = v[q].im;
+            Xr[i * TM_LD + t] = 0.f;
+            Xi[i * TM_LD + t] = 0.f;
+        }
+    }
+    __syncthreads();
+    if (lane < 16) {
+        const int c = lane, b = 16 * w;
+        float xr[16], xi[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            xr[r] = r == c ? 1.f : 0.f;
+            xi[r] = 0.f;
+        }
+#pragma unroll
+        for (int r = 1; r < 16; ++r) {
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int t = 0; t < r; ++t) {
+                const float lr = Lr[(b + r) * TM_LD + b + t], li = Li[(b + r) * TM_LD + b + t];
+                sr += lr * xr[t] - li * xi[t];
+                si += lr * xi[t] + li * xr[t];
+            }
+            if (r > c) {
+                xr[r] = -sr;
+                xi[r] = -si;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            Xr[(b + r) * TM_LD + b + c] = xr[r];
+            Xi[(b + r) * TM_LD + b + c] = xi[r];
+        }
+    }
+    __syncthreads();
+    const int r = tid >> 4, c = tid & 15;
+    for (int s = 1; s < 4; ++s) {
+        for (int q = 0; q + s < 4; ++q) {  // block (q + s, q): S = sum_t L_(q+s),t X_t,q
+            const int ri = 16 * (q + s) + r, cj = 16 * q + c;
+            float sr = 0.f, si = 0.f;
+            for (int tb = q; tb < q + s; ++tb) {
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int t = 16 * tb + u;
+                    const float lr = Lr[ri * TM_LD + t], li = Li[ri * TM_LD + t];
+                    const float xr = Xr[t * TM_LD + cj], xi = Xi[t * TM_LD + cj];
+                    sr += lr * xr - li * xi;
+                    si += lr * xi + li * xr;
+                }
+            }
+            Tr[(q * 16 + r) * 17 + c] = sr;
+            Ti[(q * 16 + r) * 17 + c] = si;
+        }
+        __syncthreads();
+        for (int q = 0; q + s < 4; ++q) {  // X_(q+s),q = -X_(q+s),(q+s) S
+            const int ri = 16 * (q + s) + r, d = 16 * (q + s);
+            float yr = 0.f, yi = 0.f;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const float dr = Xr[ri * TM_LD + d + u], di = Xi[ri * TM_LD + d + u];
+                const float tr = Tr[(q * 16 + u) * 17 + c], ti = Ti[(q * 16 + u) * 17 + c];
+                yr += dr * tr - di * ti;
+                yi += dr * ti + di * tr;
+            }
+            Xr[ri * TM_LD + 16 * q + c] = -yr;
+            Xi[ri * TM_LD + 16 * q + c] = -yi;
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < 64 * 64; idx += 256) {
+        const int kk = idx >> 6, i = idx & 63;
+        out[idx] = Xr[i * TM_LD + kk];
+        out[TM_PLANE + idx] = Xi[i * TM_LD + kk];
+    }
+}
+
+// grid: 1 workgroup (nb = 64) or 3 (nb = 128: D0, D1, L21)
+__global__ __launch_bounds__(256) void k_trsm_prep(const cpx<float>* __restrict__ A, int64_t lda, int64_t k,
+                                                   float* __restrict__ ws)
+{
+    extern __shared__ unsigned char smem_raw[];
+    float* smem = (float*)smem_raw;
+    const int b = blockIdx.x;
+    if (b < 2) {
+        tm_tri_inv64(A, lda, k + 64 * b, ws + (size_t)b * TM_OPER, smem);
+        return;
+    }
+    // L21 = rows [k + 64, k + 128) x columns [k, k + 64), transposed through LDS
+    float* Lr = smem;
+    float* Li = Lr + 64 * TM_LD;
+    const int tid = threadIdx.x;
+    {
+        cpx<float> v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = tid + 256 * q, i = idx >> 6, t = idx & 63;
+            v[q] = A[(k + 64 + i) * lda + k + t];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = tid + 256 * q, i = idx >> 6, t = idx & 63;
+            Lr[i * TM_LD + t] = v[q].re;
+            Li[i * TM_LD + t] = v[q].im;
+        }
+    }
+    __syncthreads();
+    float* out = ws + 2 * TM_OPER;
+    for (int idx = tid; idx < 64 * 64; idx += 256) {
+        const int kk = idx >> 6, i = idx & 63;
+        out[idx] = Lr[i * TM_LD + kk];
+        out[TM_PLANE + idx] = Li[i * TM_LD + kk];
+    }
+}
+
+// acc (32 x 32 block of wave (bi, bj)) += sign * P[bi rows] X[k rows] over
+// kk in [0, 2 * steps): P an operand of k_trsm_prep, X planes in LDS with rows
+// of 64 floats starting at row x0.
+template <bool NEG>
+__device__ __forceinline__ void tm_product(floatx16& cr, floatx16& ci, const float* __restrict__ P, const float* Xr,
+                                           const float* Xi, int x0, int bi, int bj, int steps)
+{
+    const int lane = threadIdx.x & 63, hi = lane >> 5, jj = lane & 31;
+    const float* pr = P + hi * 64 + bi * 32 + jj;
+    const float* pi = pr + TM_PLANE;
+    float ar[32], ai[32];
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+        ar[s] = s < steps ? pr[s * 128] : 0.f;
+        ai[s] = s < steps ? pi[s * 128] : 0.f;
+    }
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+        if (s < steps) {
+            const int kr = x0 + 2 * s + hi;
+            const float br = Xr[kr * 64 + bj * 32 + jj], bim = Xi[kr * 64 + bj * 32 + jj];
+            const float a_r = NEG ? -ar[s] : ar[s], a_i = NEG ? -ai[s] : ai[s];
+            // the two accumulators alternate, so no MFMA waits on the one before it
+            cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, bim, ci, 0, 0, 0);
+            cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a_i, bim, cr, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, br, ci, 0, 0, 0);
+        }
+    }
+}
+
+// X = L11^{-1} X for rows [k, k + NB) and columns [c_lo, c_hi), 64 columns per
+// workgroup; ws holds the operands k_trsm_prep wrote for this panel.
+template <int NB>
+__global__ __launch_bounds__(256, 2) void k_trsm_mfma(cpx<float>* __restrict__ A, int64_t lda, int64_t k,
+                                                      int64_t c_lo, int64_t c_hi, const float* __restrict__ ws)
+{
+    extern __shared__ unsigned char smem_raw[];
+    float* Xr = (float*)smem_raw;  // [NB][64]
+    float* Xi = Xr + NB * 64;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, hi = lane >> 5, jj = lane & 31;
+    const int bi = w >> 1, bj = w & 1;
+    const int64_t c0 = c_lo + (int64_t)blockIdx.x * 64;
+    const int wc = (int)((c_hi - c0) < 64 ? (c_hi - c0) : 64);
+    cpx<float>* X = A + k * lda + c0;
+    {
+        constexpr int PER = NB * 64 / 256;
+        float2 v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int idx = tid + 256 * q, r = idx >> 6, c = idx & 63;
+            v[q] = c < wc ? *(const float2*)(X + r * lda + c) : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int idx = tid + 256 * q;
+            Xr[idx] = v[q].x;
+            Xi[idx] = v[q].y;
+        }
+    }
+    __syncthreads();
+    const int col = bj * 32 + jj;
+    auto row_of = [&](int r) { return bi * 32 + 8 * (r >> 2) + 4 * hi + (r & 3); };
+    floatx16 cr, ci;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cr[r] = ci[r] = 0.f;
+    // Y0 = D0^{-1} X0 (D0^{-1} is lower triangular: row block bi needs kk < 32 (bi + 1))
+    tm_product<false>(cr, ci, ws, Xr, Xi, 0, bi, bj, 16 * (bi + 1));
+    if (col < wc) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) *(float2*)(X + row_of(r) * lda + col) = make_float2(cr[r], ci[r]);
+    }
+    if (NB == 128) {
+        __syncthreads();  // every wave is done reading X0
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            Xr[row_of(r) * 64 + col] = cr[r];
+            Xi[row_of(r) * 64 + col] = ci[r];
+            cr[r] = Xr[(64 + row_of(r)) * 64 + col];
+            ci[r] = Xi[(64 + row_of(r)) * 64 + col];
+        }
+        __syncthreads();  // Y0 in LDS
+        // X1 -= L21 Y0
+        tm_product<true>(cr, ci, ws + 2 * TM_OPER, Xr, Xi, 0, bi, bj, 32);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {  // each wave writes back only its own block of X1
+            Xr[(64 + row_of(r)) * 64 + col] = cr[r];
+            Xi[(64 + row_of(r)) * 64 + col] = ci[r];
+            cr[r] = ci[r] = 0.f;
+        }
+        __syncthreads();
+        // Y1 = D1^{-1} X1
+        tm_product<false>(cr, ci, ws + TM_OPER, Xr, Xi, 64, bi, bj, 16 * (bi + 1));
+        if (col < wc) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                *(float2*)(X + (64 + row_of(r)) * lda + col) = make_float2(cr[r], ci[r]);
+        }
+    }
+}
+
 // ---- the same three steps for many matrices in one launch each: job j owns
 // workgroups [wg_off[j], wg_off[j + 1]).
 template <class T>
@@ -2028,6 +2286,7 @@ struct LuWs {
     int32_t* plan_n;
     void* split;        // operands of the split-FP16 trailing update (complex64 only)
     size_t split_bytes;
+    float* tm;          // operands of the MFMA panel triangular solve (complex64 only)
     size_t total;
     size_t msg_words;  // words from meta to the end of diag (zeroed per factorisation)
 };
@@ -2061,6 +2320,7 @@ LuWs lu_ws_layout(int prec, int64_t n, char* base)
     w.plan_n = (int32_t*)take(256);
     w.split_bytes = prec == MBA_F32 ? mba_h3::workspace_bytes(n, n, mba_h3::KMAX) : 0;
     w.split = w.split_bytes ? (void*)take(w.split_bytes) : nullptr;
+    w.tm = prec == MBA_F32 ? (float*)take((size_t)TM_WS_FLOATS * sizeof(float)) : nullptr;
     w.total = off;
     (void)n;
     return w;
@@ -2167,9 +2427,16 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                                   (int)lds_budget);
         (void)hipFuncSetAttribute((const void*)k_trsm_l<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)trsm_lds_bytes<T>(128));
-        if (sizeof(T) == 4)
+        if (sizeof(T) == 4) {
             (void)hipFuncSetAttribute((const void*)k_panel_tp, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)((TP_RMAX + TP_NB) * TP_PITCH * sizeof(cpx<float>)));
+            (void)hipFuncSetAttribute((const void*)k_trsm_prep, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)TM_PREP_LDS);
+            (void)hipFuncSetAttribute((const void*)k_trsm_mfma<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(64 * 64 * 2 * sizeof(float)));
+            (void)hipFuncSetAttribute((const void*)k_trsm_mfma<128>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(128 * 64 * 2 * sizeof(float)));
+        }
     });
     // err := 0, every message word := epoch 0 (epochs of this factorisation are k + j + 1 >= 1)
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)ws.err, 32);
@@ -2264,15 +2531,38 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
         }
         return MBA_OK;
     };
-    // rows [r0, r0 + nb) x columns [c_lo, n): X = L(r0 block)^{-1} X
-    auto do_trsm = [&](int64_t r0, int nb, int64_t c_lo, int64_t c_hi = -1) -> int {
-        if (c_hi < 0) c_hi = n;
+    // rows [r0, r0 + nb) x columns [c_lo, c_hi): X = L(r0 block)^{-1} X.  slot:
+    // the panel's place in its super panel; its inverted diagonal blocks are
+    // formed by the first call that has columns to solve and kept for the
+    // second-half panels' later call past the super panel.
+    bool tm_ready[TM_SLOTS] = {};
+    auto do_trsm = [&](int64_t r0, int nb, int64_t c_lo, int64_t c_hi, int slot) -> int {
         if (c_lo >= c_hi) return MBA_OK;
         tm.mark(2);
-        constexpr int TC = TrsmCols<T>::value;
-        hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((c_hi - c_lo + TC - 1) / TC)), dim3(256),
-                           trsm_lds_bytes<T>(nb), st, A, lda, r0, nb, c_lo, c_hi);
-        MBA_LAUNCH_CHECK();
+        if (sizeof(T) == 4 && (nb == 64 || nb == 128) && slot >= 0 && slot < TM_SLOTS) {
+            // complex64: inverted diagonal blocks, then three MFMA products per
+            // 64 columns
+            float* set = ws.tm + (size_t)slot * TM_SET;
+            if (!tm_ready[slot]) {
+                hipLaunchKernelGGL(k_trsm_prep, dim3(nb == 128 ? 3u : 1u), dim3(256), TM_PREP_LDS, st,
+                                   (const cpx<float>*)A, lda, r0, set);
+                MBA_LAUNCH_CHECK();
+                tm_ready[slot] = true;
+            }
+            const dim3 g((unsigned)((c_hi - c_lo + 63) / 64));
+            if (nb == 128)
+                hipLaunchKernelGGL(k_trsm_mfma<128>, g, dim3(256), (size_t)128 * 64 * 2 * sizeof(float), st,
+                                   (cpx<float>*)A, lda, r0, c_lo, c_hi, (const float*)set);
+            else
+                hipLaunchKernelGGL(k_trsm_mfma<64>, g, dim3(256), (size_t)64 * 64 * 2 * sizeof(float), st,
+                                   (cpx<float>*)A, lda, r0, c_lo, c_hi, (const float*)set);
+            MBA_LAUNCH_CHECK();
+        } else {
+            constexpr int TC = TrsmCols<T>::value;
+            hipLaunchKernelGGL(k_trsm_l<T>, dim3((unsigned)((c_hi - c_lo + TC - 1) / TC)), dim3(256),
+                               trsm_lds_bytes<T>(nb), st, A, lda, r0, nb, c_lo, c_hi);
+            MBA_LAUNCH_CHECK();
+        }
         tm.mark(2);
         return MBA_OK;
     };
@@ -2317,6 +2607,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
     for (int64_t k = 0; k < n; k += (int64_t)P * NB) {
         int64_t kp = k;
         int rc;
+        for (bool& r : tm_ready) r = false;
         const int64_t kh = k + (int64_t)HALF * NB;  // first column of the second half
         const int64_t ce = (k + (int64_t)P * NB) < n ? (k + (int64_t)P * NB) : n;  // end of the super panel
         bool two = false;  // the super panel is split into halves
@@ -2343,7 +2634,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
             // super panel only
             const int64_t ue = two ? ce : n;
             if ((rc = do_gemm_rect(r0, nbp, right, ue - right, kb, (int)(r0 - kb))) != MBA_OK) return rc;
-            if ((rc = do_trsm(r0, nbp, right, ue)) != MBA_OK) return rc;
+            if ((rc = do_trsm(r0, nbp, right, ue, np - 1)) != MBA_OK) return rc;
         }
         if (two && ce < n) {
             // the second half's U rows past the super panel, once all its
@@ -2354,7 +2645,7 @@ int lu_factor(void* a, int64_t n, int64_t lda, int32_t* ipiv, int32_t* info, voi
                 const int64_t r0 = pr0[q];
                 if (r0 < kh) continue;
                 if ((rc = do_gemm_rect(r0, pnb[q], ce, n - ce, kh, (int)(r0 - kh))) != MBA_OK) return rc;
-                if ((rc = do_trsm(r0, pnb[q], ce)) != MBA_OK) return rc;
+                if ((rc = do_trsm(r0, pnb[q], ce, n, q)) != MBA_OK) return rc;
             }
         }
         const int64_t c3 = kp;  // first column after the super panel
