@@ -2088,22 +2088,28 @@ __global__ __launch_bounds__(256) void k_trsm_prep(const cpx<float>* __restrict_
     }
 }
 
-// acc (32 x 32 block of wave (bi, bj)) += sign * P[bi rows] X[k rows] over
-// kk in [0, 2 * steps): P an operand of k_trsm_prep, X planes in LDS with rows
-// of 64 floats starting at row x0.
-template <bool NEG>
-__device__ __forceinline__ void tm_product(floatx16& cr, floatx16& ci, const float* __restrict__ P, const float* Xr,
-                                           const float* Xi, int x0, int bi, int bj, int steps)
+// A operands of wave row block bi for kk in [0, 2 * steps): lane (jj, hi) of
+// step s holds P[kk = 2 s + hi][bi * 32 + jj] (P an operand of k_trsm_prep).
+__device__ __forceinline__ void tm_load(const float* __restrict__ P, int bi, int steps, float (&ar)[32],
+                                        float (&ai)[32])
 {
     const int lane = threadIdx.x & 63, hi = lane >> 5, jj = lane & 31;
     const float* pr = P + hi * 64 + bi * 32 + jj;
     const float* pi = pr + TM_PLANE;
-    float ar[32], ai[32];
 #pragma unroll
     for (int s = 0; s < 32; ++s) {
         ar[s] = s < steps ? pr[s * 128] : 0.f;
         ai[s] = s < steps ? pi[s * 128] : 0.f;
     }
+}
+
+// acc (32 x 32 block of wave (bi, bj)) += sign * P[bi rows] X[k rows] over
+// kk in [0, 2 * steps): X planes in LDS with rows of 64 floats from row x0.
+template <bool NEG>
+__device__ __forceinline__ void tm_product(floatx16& cr, floatx16& ci, const float (&ar)[32], const float (&ai)[32],
+                                           const float* Xr, const float* Xi, int x0, int bj, int steps)
+{
+    const int lane = threadIdx.x & 63, hi = lane >> 5, jj = lane & 31;
 #pragma unroll
     for (int s = 0; s < 32; ++s) {
         if (s < steps) {
@@ -2133,6 +2139,12 @@ __global__ __launch_bounds__(256, 2) void k_trsm_mfma(cpx<float>* __restrict__ A
     const int64_t c0 = c_lo + (int64_t)blockIdx.x * 64;
     const int wc = (int)((c_hi - c0) < 64 ? (c_hi - c0) : 64);
     cpx<float>* X = A + k * lda + c0;
+    // operands of the first two products in flight with the X loads, the
+    // third's during the first product
+    const int tri = 16 * (bi + 1);  // steps of the triangular D^{-1} blocks
+    float a0r[32], a0i[32], a2r[32], a2i[32], a1r[32], a1i[32];
+    tm_load(ws, bi, tri, a0r, a0i);
+    if (NB == 128) tm_load(ws + 2 * TM_OPER, bi, 32, a2r, a2i);
     {
         constexpr int PER = NB * 64 / 256;
         float2 v[PER];
@@ -2154,8 +2166,9 @@ __global__ __launch_bounds__(256, 2) void k_trsm_mfma(cpx<float>* __restrict__ A
     floatx16 cr, ci;
 #pragma unroll
     for (int r = 0; r < 16; ++r) cr[r] = ci[r] = 0.f;
+    if (NB == 128) tm_load(ws + TM_OPER, bi, tri, a1r, a1i);
     // Y0 = D0^{-1} X0 (D0^{-1} is lower triangular: row block bi needs kk < 32 (bi + 1))
-    tm_product<false>(cr, ci, ws, Xr, Xi, 0, bi, bj, 16 * (bi + 1));
+    tm_product<false>(cr, ci, a0r, a0i, Xr, Xi, 0, bj, tri);
     if (col < wc) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) *(float2*)(X + row_of(r) * lda + col) = make_float2(cr[r], ci[r]);
@@ -2171,7 +2184,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_mfma(cpx<float>* __restrict__ A
         }
         __syncthreads();  // Y0 in LDS
         // X1 -= L21 Y0
-        tm_product<true>(cr, ci, ws + 2 * TM_OPER, Xr, Xi, 0, bi, bj, 32);
+        tm_product<true>(cr, ci, a2r, a2i, Xr, Xi, 0, bj, 32);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {  // each wave writes back only its own block of X1
             Xr[(64 + row_of(r)) * 64 + col] = cr[r];
@@ -2180,7 +2193,7 @@ __global__ __launch_bounds__(256, 2) void k_trsm_mfma(cpx<float>* __restrict__ A
         }
         __syncthreads();
         // Y1 = D1^{-1} X1
-        tm_product<false>(cr, ci, ws + TM_OPER, Xr, Xi, 64, bi, bj, 16 * (bi + 1));
+        tm_product<false>(cr, ci, a1r, a1i, Xr, Xi, 64, bj, tri);
         if (col < wc) {
 #pragma unroll
             for (int r = 0; r < 16; ++r)
