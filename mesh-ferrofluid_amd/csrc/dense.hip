@@ -1063,7 +1063,9 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
             block_argmax<NT / 64>(v, r, q, red_v, red_r, red_q, 1);
         }
         const int wg = q;
-        const int32_t p = r;
+        // no candidate at all (every value NaN): no interchange, the column
+        // is reported singular below, and no row index leaves the panel
+        const int32_t p = r == NONE ? (int32_t)j : r;
         // --- (2) the winner's row and row j
         {
             uint64_t* pw = P.cand + ((int64_t)par * G + wg) * row_words;
@@ -1095,7 +1097,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs<T>& P, const int G, c
                 inv.im = -pv.im / d;
             }
             if (tid == 0 && g == 0) {
-                if (d == (T)0) atomicCAS(P.info, 0, (int32_t)(P.k + j + 1));
+                if (d == (T)0 || r == NONE) atomicCAS(P.info, 0, (int32_t)(P.k + j + 1));
                 P.ipiv[P.k + j] = (int32_t)(P.k + p);
             }
         }
