@@ -57,11 +57,17 @@ __global__ __launch_bounds__(NT, 2) void k_cgemm_h3(int64_t M, int64_t N, int nr
                                                       const int32_t* __restrict__ eb, float alpha, float beta,
                                                       cpx<float>* __restrict__ C, int64_t ldc)
 {
-    __shared__ TileLds<BMV> L;
     const int64_t nbm = (M + BMV - 1) / BMV, nbn = (N + BN - 1) / BN;
     int64_t tm, tn;
     if (!tile_of(blockIdx.x, nbm, nbn, tm, tn)) return;
-    tile_h3<BMV, NT>(M, N, nrec, Ap, ea, Bp, eb, alpha, beta, C, ldc, tm, tn, L);
+    if constexpr (NT == 512) {
+        // operand records staged by direct global -> LDS loads
+        __shared__ TileLdsDma<BMV> L;
+        tile_h3_dma<BMV, NT>(M, N, nrec, Ap, ea, Bp, eb, alpha, beta, C, ldc, tm, tn, L);
+    } else {
+        __shared__ TileLds<BMV> L;
+        tile_h3<BMV, NT>(M, N, nrec, Ap, ea, Bp, eb, alpha, beta, C, ldc, tm, tn, L);
+    }
 }
 
 }  // namespace

@@ -319,6 +319,154 @@ __device__ __forceinline__ void tile_h3(int64_t M, int64_t N, int nrec, const _F
 }
 
 
+// ---- the same tile with the operand records staged by direct global -> LDS
+// loads (global_load_lds_dwordx4: each lane's 16 bytes land at LDS base +
+// 16 lane), no VGPR round trip and no LDS stores.  Records sit unpadded (128
+// B per row) with the 16-byte parts of row R permuted by part ^ (R & 7), so
+// the operand reads of eight consecutive rows at one part hit eight distinct
+// 16-byte bank groups; a lane loads the global part that belongs at its
+// fixed LDS slot.  The loads of chunk rc + 1 go to the buffer that chunk rc - 1
+// read (every wave is past the barrier that ended it); a chunk's records are
+// complete for all waves after each wave's vector memory counter reaches 0
+// and the barrier.  Same MFMA order and epilogue as tile_h3: bit-identical
+// results.
+typedef __attribute__((address_space(3))) void lds_void_t;
+constexpr int DREC_H = REC_H;  // halves per LDS row (no padding)
+
+template <int BMV>
+struct TileLdsDma {
+    alignas(16) _Float16 As[2][BMV * DREC_H];
+    alignas(16) _Float16 Bs[2][BN * DREC_H];
+};
+
+template <int BMV, int NT>
+__device__ __forceinline__ void tile_h3_dma(int64_t M, int64_t N, int nrec, const _Float16* __restrict__ Ap,
+                                            const int32_t* __restrict__ ea, const _Float16* __restrict__ Bp,
+                                            const int32_t* __restrict__ eb, float alpha, float beta,
+                                            cpx<float>* __restrict__ C, int64_t ldc, int64_t tm, int64_t tn,
+                                            TileLdsDma<BMV>& L)
+{
+    static_assert(NT == 512 && BMV % 64 == 0, "eight waves, 64-row steps");
+    constexpr int QA = BMV / 64, QB = BN / 64;
+    const int64_t m0 = tm * BMV, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int l32 = lane & 31, oct = lane >> 5;
+    // instruction q of this wave stages rows 64 q + 8 wave + lane / 8; lane
+    // writes LDS slot lane % 8 of its row, i.e. global part (lane % 8) ^ (lane / 8)
+    const int rsub = 8 * wave + (lane >> 3), gpart = (lane & 7) ^ (lane >> 3);
+    // record offsets (halves) of this lane's first staged row; instruction q
+    // adds 64 q records rows, chunk rc adds rc records
+    // a wave-uniform base (scalar registers) plus one 32-bit lane offset per
+    // operand (the records of one product stay far below 4 GiB)
+    const uint32_t la = (uint32_t)((rsub * (int64_t)nrec * REC_H + gpart * 8) * 2);
+    const int64_t qstride = (int64_t)64 * nrec * REC_H;
+    const _Float16* Ab = Ap + m0 * nrec * REC_H;
+    const _Float16* Bb = Bp + n0 * nrec * REC_H;
+    auto dma = [&](int rc, _Float16* As, _Float16* Bs) {
+#pragma unroll
+        for (int q = 0; q < QA; ++q)
+            __builtin_amdgcn_global_load_lds((const void*)((const char*)(Ab + q * qstride + rc * REC_H) + la),
+                                             (lds_void_t*)(As + (64 * q + 8 * wave) * DREC_H), 16, 0, 0);
+#pragma unroll
+        for (int q = 0; q < QB; ++q)
+            __builtin_amdgcn_global_load_lds((const void*)((const char*)(Bb + q * qstride + rc * REC_H) + la),
+                                             (lds_void_t*)(Bs + (64 * q + 8 * wave) * DREC_H), 16, 0, 0);
+    };
+    // operand reads: rows R = w* 64 + i 32 + l32 (R & 7 = l32 & 7), part
+    // 2 p + oct at slot (2 p + oct) ^ (l32 & 7): one lane offset per p, the
+    // row block i and the buffer are immediate offsets
+    int offA[4], offB[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int slot = ((2 * p + oct) ^ (l32 & 7)) * 8;
+        offA[p] = (wm * 64 + l32) * DREC_H + slot;
+        offB[p] = (wn * 64 + l32) * DREC_H + slot;
+    }
+
+    floatx16 acc[2][2][2];  // [i][j][re/im]
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][0][r] = acc[i][j][1][r] = 0.f;
+
+    auto compute = [&](const _Float16* As, const _Float16* Bs) {
+        half8 a[2][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int p = 0; p < 4; ++p) a[i][p] = *(const half8*)&As[offA[p] + i * 32 * DREC_H];
+        half8 b[2][4], nbh[2], nbl[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) b[j][p] = *(const half8*)&Bs[offB[p] + j * 32 * DREC_H];
+            nbh[j] = -b[j][1];
+            nbl[j] = -b[j][3];
+        }
+#define MBA_H3_STEP(AR, BR, AI, BI)                                                                           \
+        _Pragma("unroll") for (int j = 0; j < 2; ++j) _Pragma("unroll") for (int i = 0; i < 2; ++i) {        \
+            acc[i][j][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][AR], BR, acc[i][j][0], 0, 0, 0);       \
+            acc[i][j][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i][AI], BI, acc[i][j][1], 0, 0, 0);       \
+        }
+        MBA_H3_STEP(0, b[j][0], 0, b[j][1])  // hi x hi: Ar Br, Ar Bi
+        MBA_H3_STEP(1, nbh[j], 1, b[j][0])   //          -Ai Bi, Ai Br
+        MBA_H3_STEP(0, b[j][2], 0, b[j][3])  // hi x lo
+        MBA_H3_STEP(1, nbl[j], 1, b[j][2])
+        MBA_H3_STEP(2, b[j][0], 2, b[j][1])  // lo x hi
+        MBA_H3_STEP(3, nbh[j], 3, b[j][0])
+#undef MBA_H3_STEP
+    };
+    // a chunk's records are visible to every wave once each wave's loads of
+    // it have landed (vector memory counter 0) and the barrier has passed
+    auto landed = [&]() {
+        __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
+        __syncthreads();
+    };
+
+    dma(0, L.As[0], L.Bs[0]);
+    landed();
+    for (int rc = 0; rc < nrec; ++rc) {
+        const int buf = rc & 1;
+        if (rc + 1 < nrec) dma(rc + 1, L.As[buf ^ 1], L.Bs[buf ^ 1]);
+        compute(L.As[buf], L.Bs[buf]);
+        landed();
+    }
+    // epilogue: as tile_h3
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int64_t gj = n0 + wn * 64 + j * 32 + l32;
+        const bool jok = gj < N;
+        const int ej = jok ? eb[gj] : 0;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t gi0 = m0 + wm * 64 + i * 32 + 4 * oct;
+            float2 cv[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = gi0 + 8 * (r >> 2) + (r & 3);
+                cv[r] = (beta != 0.f && jok && gi < M) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = gi0 + 8 * (r >> 2) + (r & 3);
+                if (jok && gi < M) {
+                    const int e = ea[gi] + ej;
+                    float re = ldexpf(alpha * acc[i][j][0][r], e), im = ldexpf(alpha * acc[i][j][1][r], e);
+                    if (beta != 0.f) {
+                        re += beta * cv[r].x;
+                        im += beta * cv[r].y;
+                    }
+                    *(float2*)(C + gi * ldc + gj) = make_float2(re, im);
+                }
+            }
+        }
+    }
+}
+
+
 // Workspace layout of the split operands for M x N x K: A records [Mpad][Kpad
 // / 16], B records [Npad][Kpad / 16] (128 bytes each), exponents ea [Mpad],
 // eb [Npad]; pads of BM rows and BN columns.
