@@ -214,33 +214,37 @@ __global__ __launch_bounds__(256) void k_cgemm_f32(int64_t M, int64_t N, int64_t
 // and im planes stored k-major (A) and row-major (B).
 constexpr int G3M_BN = 64;
 
-// LDS of one tile: Ar, Ai [2][G32_BK][G32_BM], Br, Bi [2][G32_BK][G3M_BN] floats
+// LDS of one tile: Ar, Ai [2][G32_BK][rows], Br, Bi [2][G32_BK][G3M_BN] floats
 constexpr int G3M_LDS_FLOATS = 4 * G32_BK * G32_BM + 4 * G32_BK * G3M_BN;
 
+template <int BMT = G32_BM>
 __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, float alpha,
                                              const cpx<float>* __restrict__ A, int64_t lda,
                                              const cpx<float>* __restrict__ B, int64_t ldb, float beta,
                                              cpx<float>* __restrict__ C, int64_t ldc, int64_t m0, int64_t n0,
                                              float* lds)
 {
+    static_assert(BMT == 128 || BMT == 64, "128 or 64 rows");
     constexpr int BN = G3M_BN;
-    float(*Ar)[G32_BK][G32_BM] = (float(*)[G32_BK][G32_BM])lds;
-    float(*Ai)[G32_BK][G32_BM] = (float(*)[G32_BK][G32_BM])(lds + 2 * G32_BK * G32_BM);
-    float(*Br)[G32_BK][BN] = (float(*)[G32_BK][BN])(lds + 4 * G32_BK * G32_BM);
-    float(*Bi)[G32_BK][BN] = (float(*)[G32_BK][BN])(lds + 4 * G32_BK * G32_BM + 2 * G32_BK * BN);
+    constexpr int MI = BMT / 64;               // 32-row blocks per wave
+    constexpr int TPR = 256 / BMT, KPT = G32_BK / TPR;  // A staging: threads per row, k per thread
+    float(*Ar)[G32_BK][BMT] = (float(*)[G32_BK][BMT])lds;
+    float(*Ai)[G32_BK][BMT] = (float(*)[G32_BK][BMT])(lds + 2 * G32_BK * BMT);
+    float(*Br)[G32_BK][BN] = (float(*)[G32_BK][BN])(lds + 4 * G32_BK * BMT);
+    float(*Bi)[G32_BK][BN] = (float(*)[G32_BK][BN])(lds + 4 * G32_BK * BMT + 2 * G32_BK * BN);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    // A: 128 rows x 16 complex k: thread -> row tid/2, k (tid&1)*8 .. +8
-    const int ar = tid >> 1, ak = (tid & 1) * 8;
+    // A: BMT rows x 16 complex k: thread -> row tid / TPR, k (tid % TPR) * KPT .. + KPT
+    const int ar = tid / TPR, ak = (tid % TPR) * KPT;
     // B: 16 k x 64 complex: thread -> k tid/16, cols (tid&15)*4 .. +4
     const int bkr = tid >> 4, bc = (tid & 15) * 4;
-    float2 ra[8], rb[4];
+    float2 ra[KPT], rb[4];
     auto load_chunk = [&](int64_t k0) {
         const int64_t gi = m0 + ar;
         const bool rowok = gi < M;
         const float2* pa = (const float2*)(A + (rowok ? gi : 0) * lda);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < KPT; ++q) {
             const int64_t gk = k0 + ak + q;
             ra[q] = (rowok && gk < K) ? pa[gk] : make_float2(0.f, 0.f);
         }
@@ -255,16 +259,16 @@ __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, fl
     };
     auto store_chunk = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < KPT; ++q) {
             Ar[buf][ak + q][ar] = ra[q].x;
             Ai[buf][ak + q][ar] = ra[q].y;
         }
         *(float4*)&Br[buf][bkr][bc] = make_float4(rb[0].x, rb[1].x, rb[2].x, rb[3].x);
         *(float4*)&Bi[buf][bkr][bc] = make_float4(rb[0].y, rb[1].y, rb[2].y, rb[3].y);
     };
-    floatx16 t1[2], t2[2], t3[2];
+    floatx16 t1[MI], t2[MI], t3[MI];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int r = 0; r < 16; ++r) t1[i][r] = t2[i][r] = t3[i][r] = 0.f;
     const int hi = lane >> 5, jj = lane & 31;
@@ -281,8 +285,8 @@ __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, fl
             const float br = Br[buf][k][wn * 32 + jj], bi = Bi[buf][k][wn * 32 + jj];
             const float bs = br + bi;
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int row = wm * 64 + i * 32 + jj;
+            for (int i = 0; i < MI; ++i) {
+                const int row = wm * (BMT / 2) + i * 32 + jj;
                 const float a_r = Ar[buf][k][row], a_i = Ai[buf][k][row];
                 t1[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, t1[i], 0, 0, 0);
                 t2[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, bi, t2[i], 0, 0, 0);
@@ -301,16 +305,16 @@ __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, fl
     const int64_t gj = n0 + wn * 32 + jj;
     if (gj < N) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < MI; ++i) {
             float2 cv[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                const int64_t gi = m0 + wm * (BMT / 2) + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
                 cv[r] = (beta != 0.f && gi < M) ? *(const float2*)(C + gi * ldc + gj) : make_float2(0.f, 0.f);
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+                const int64_t gi = m0 + wm * (BMT / 2) + i * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
                 if (gi < M) {
                     float re = alpha * (t1[i][r] - t2[i][r]);
                     float im = alpha * (t3[i][r] - t1[i][r] - t2[i][r]);
@@ -325,13 +329,13 @@ __device__ __forceinline__ void cgemm3m_tile(int64_t M, int64_t N, int64_t K, fl
     }
 }
 
-template <bool XCD>
+template <bool XCD, int BMT = G32_BM>
 __global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha,
                                                      const cpx<float>* __restrict__ A, int64_t lda,
                                                      const cpx<float>* __restrict__ B, int64_t ldb,
                                                      float beta, cpx<float>* __restrict__ C, int64_t ldc)
 {
-    const int64_t nbm = (M + G32_BM - 1) / G32_BM, nbn = (N + G3M_BN - 1) / G3M_BN;
+    const int64_t nbm = (M + BMT - 1) / BMT, nbn = (N + G3M_BN - 1) / G3M_BN;
     int64_t tmi, tni;
     if (XCD) {
         tile_of(blockIdx.x, nbm, nbn, tmi, tni);
@@ -340,8 +344,8 @@ __global__ __launch_bounds__(256, 3) void k_cgemm3m_f32(int64_t M, int64_t N, in
         tmi = blockIdx.x / nbn;
         tni = blockIdx.x % nbn;
     }
-    __shared__ alignas(16) float lds[G3M_LDS_FLOATS];
-    cgemm3m_tile(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tmi * G32_BM, tni * G3M_BN, lds);
+    __shared__ alignas(16) float lds[4 * G32_BK * BMT + 4 * G32_BK * G3M_BN];
+    cgemm3m_tile<BMT>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tmi * BMT, tni * G3M_BN, lds);
 }
 
 // ---- grouped products: job j owns tiles [tile_off[j], tile_off[j + 1]),
@@ -385,14 +389,27 @@ __global__ __launch_bounds__(256, 3) void k_cgemm3m_grouped(const GemmJobDev* __
                  (float)J.beta, (cpx<float>*)J.c, J.ldc, (u / nbn) * G32_BM, (u % nbn) * G3M_BN, lds);
 }
 
+int device_cus();
+
+// 128 x 64 tiles, or 64 x 64 when 128-row tiles would give fewer than two
+// workgroups per CU (narrow LU products late in a factorisation): twice the
+// workgroups, each element's MFMA sequence unchanged (bit-identical results)
 template <bool XCD>
 int launch_cgemm3m_f32(int64_t M, int64_t N, int64_t K, float alpha, const void* A, int64_t lda, const void* B,
                        int64_t ldb, float beta, void* C, int64_t ldc, hipStream_t st)
 {
-    const int64_t tiles = ((N + 63) / 64) * ((M + G32_BM - 1) / G32_BM);
+    const int64_t nbn = (N + 63) / 64;
+    const int64_t tiles128 = nbn * ((M + G32_BM - 1) / G32_BM);
+    const int cus = device_cus();
+    const bool small = cus > 0 && tiles128 < 2 * (int64_t)cus;
+    const int64_t tiles = small ? nbn * ((M + 63) / 64) : tiles128;
     const int64_t blocks = XCD ? ((tiles + NUM_XCD - 1) / NUM_XCD) * NUM_XCD : tiles;
-    hipLaunchKernelGGL((k_cgemm3m_f32<XCD>), dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, alpha,
-                       (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, beta, (cpx<float>*)C, ldc);
+    if (small)
+        hipLaunchKernelGGL((k_cgemm3m_f32<XCD, 64>), dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, alpha,
+                           (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, beta, (cpx<float>*)C, ldc);
+    else
+        hipLaunchKernelGGL((k_cgemm3m_f32<XCD, 128>), dim3((unsigned)blocks), dim3(256), 0, st, M, N, K, alpha,
+                           (const cpx<float>*)A, lda, (const cpx<float>*)B, ldb, beta, (cpx<float>*)C, ldc);
     MBA_LAUNCH_CHECK();
     return MBA_OK;
 }
