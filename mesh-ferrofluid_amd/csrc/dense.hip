@@ -4850,8 +4850,11 @@ __device__ __noinline__ void pb_phase_h3(const PbTables& T, const PbPhase& ph, i
             const PbSplitJob& J = T.hjobs[ph.job0 + j];
             const int64_t nbn = (J.N + mba_h3::dev::BN - 1) / mba_h3::dev::BN;
             const int64_t u = t - off[j];
-            mba_h3::dev::tile_h3<PB_H3_BM, PB_THREADS>(J.M, J.N, J.nrec, J.Ap, J.ea, J.Bp, J.eb, -1.f, 1.f, J.C, J.ld,
-                                                       u / nbn, u % nbn, *(mba_h3::dev::TileLds<PB_H3_BM>*)lds);
+            static_assert(sizeof(mba_h3::dev::TileLdsDma<PB_H3_BM>) <= sizeof(mba_h3::dev::TileLds<PB_H3_BM>),
+                          "the product's LDS fits the region sized for it");
+            mba_h3::dev::tile_h3_dma<PB_H3_BM, PB_THREADS>(J.M, J.N, J.nrec, J.Ap, J.ea, J.Bp, J.eb, -1.f, 1.f, J.C,
+                                                           J.ld, u / nbn, u % nbn,
+                                                           *(mba_h3::dev::TileLdsDma<PB_H3_BM>*)lds);
         }
         __syncthreads();
     }

@@ -346,13 +346,14 @@ __device__ __forceinline__ void tile_h3_dma(int64_t M, int64_t N, int nrec, cons
                                             cpx<float>* __restrict__ C, int64_t ldc, int64_t tm, int64_t tn,
                                             TileLdsDma<BMV>& L)
 {
-    static_assert(NT == 512 && BMV % 64 == 0, "eight waves, 64-row steps");
-    constexpr int QA = BMV / 64, QB = BN / 64;
+    static_assert((NT == 512 || NT == 256) && BMV == NT / 2, "2 x 2 or 4 x 2 waves of 64 x 64");
+    constexpr int RSTEP = NT / 8;  // record rows staged per load instruction of all waves
+    constexpr int QA = BMV / RSTEP, QB = BN / RSTEP;
     const int64_t m0 = tm * BMV, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int l32 = lane & 31, oct = lane >> 5;
-    // instruction q of this wave stages rows 64 q + 8 wave + lane / 8; lane
+    // instruction q of this wave stages rows RSTEP q + 8 wave + lane / 8; lane
     // writes LDS slot lane % 8 of its row, i.e. global part (lane % 8) ^ (lane / 8)
     const int rsub = 8 * wave + (lane >> 3), gpart = (lane & 7) ^ (lane >> 3);
     // record offsets (halves) of this lane's first staged row; instruction q
@@ -360,18 +361,18 @@ __device__ __forceinline__ void tile_h3_dma(int64_t M, int64_t N, int nrec, cons
     // a wave-uniform base (scalar registers) plus one 32-bit lane offset per
     // operand (the records of one product stay far below 4 GiB)
     const uint32_t la = (uint32_t)((rsub * (int64_t)nrec * REC_H + gpart * 8) * 2);
-    const int64_t qstride = (int64_t)64 * nrec * REC_H;
+    const int64_t qstride = (int64_t)RSTEP * nrec * REC_H;
     const _Float16* Ab = Ap + m0 * nrec * REC_H;
     const _Float16* Bb = Bp + n0 * nrec * REC_H;
     auto dma = [&](int rc, _Float16* As, _Float16* Bs) {
 #pragma unroll
         for (int q = 0; q < QA; ++q)
             __builtin_amdgcn_global_load_lds((const void*)((const char*)(Ab + q * qstride + rc * REC_H) + la),
-                                             (lds_void_t*)(As + (64 * q + 8 * wave) * DREC_H), 16, 0, 0);
+                                             (lds_void_t*)(As + (RSTEP * q + 8 * wave) * DREC_H), 16, 0, 0);
 #pragma unroll
         for (int q = 0; q < QB; ++q)
             __builtin_amdgcn_global_load_lds((const void*)((const char*)(Bb + q * qstride + rc * REC_H) + la),
-                                             (lds_void_t*)(Bs + (64 * q + 8 * wave) * DREC_H), 16, 0, 0);
+                                             (lds_void_t*)(Bs + (RSTEP * q + 8 * wave) * DREC_H), 16, 0, 0);
     };
     // operand reads: rows R = w* 64 + i 32 + l32 (R & 7 = l32 & 7), part
     // 2 p + oct at slot (2 p + oct) ^ (l32 & 7): one lane offset per p, the
