@@ -439,7 +439,9 @@ int mba_gemm(int prec, int64_t m, int64_t n, int64_t k, double alpha, const void
 
 /* Tile variants of the FP32 GEMM (0: 128x64 complex tile, 1: 128x64 ordered
  * for the 8 XCDs, 2: 128x128, 3: 128x128 XCD-ordered, 4: 128x64 with three
- * real products per complex product (3M), 5: 3M XCD-ordered).  mba_gemm_variant runs
+ * real products per complex product (3M), 5: 3M XCD-ordered; 4 and 5 switch
+ * to 64x64 tiles, bit-identical, when 128-row tiles would give fewer than two
+ * workgroups per CU).  mba_gemm_variant runs
  * one variant; mba_gemm_set_variant selects the one mba_gemm and the LU use
  * (process-wide, default 4; set once by a tuner, not per call).  No
  * counterpart in the reference. */
@@ -463,8 +465,10 @@ int mba_gemm_split_f16(int64_t m, int64_t n, int64_t k, double alpha, const void
                        size_t workspace_bytes, mba_stream_t stream);
 
 /* Workgroup tile of mba_gemm_split_f16 (and of the LU's split update): 0 =
- * 256 x 128 complex with 8 waves (one workgroup per CU, default), 1 = 128 x
- * 128 with 4 waves (two workgroups per CU).  Process-wide tuning knob, like
+ * 256 x 128 complex with 8 waves (one workgroup per CU, operand records
+ * staged by direct global-to-LDS loads; default), 1 = 128 x 128 with 4 waves
+ * (two workgroups per CU, register-staged); bit-identical results.
+ * Process-wide tuning knob, like
  * mba_gemm_set_variant.  No counterpart in the reference. */
 int mba_gemm_split_f16_set_tile(int tile);
 
