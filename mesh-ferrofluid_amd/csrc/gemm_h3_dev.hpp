@@ -356,10 +356,9 @@ __device__ __forceinline__ void tile_h3_dma(int64_t M, int64_t N, int nrec, cons
     // instruction q of this wave stages rows RSTEP q + 8 wave + lane / 8; lane
     // writes LDS slot lane % 8 of its row, i.e. global part (lane % 8) ^ (lane / 8)
     const int rsub = 8 * wave + (lane >> 3), gpart = (lane & 7) ^ (lane >> 3);
-    // record offsets (halves) of this lane's first staged row; instruction q
-    // adds 64 q records rows, chunk rc adds rc records
-    // a wave-uniform base (scalar registers) plus one 32-bit lane offset per
-    // operand (the records of one product stay far below 4 GiB)
+    // addresses: a wave-uniform base (the tile's first record row; +RSTEP q
+    // rows for instruction q, +rc records for chunk rc) plus one 32-bit lane
+    // offset in bytes (this lane's row and part within the first RSTEP rows)
     const uint32_t la = (uint32_t)((rsub * (int64_t)nrec * REC_H + gpart * 8) * 2);
     const int64_t qstride = (int64_t)RSTEP * nrec * REC_H;
     const _Float16* Ab = Ap + m0 * nrec * REC_H;
