@@ -153,6 +153,35 @@ def test_gemm_variants(dev, variant, shape):
     assert err < 3e-6, (variant, float(err))
 
 
+def test_gemm_3m_row_tiles_bit_identical(dev):
+    """The 3M FP32 product takes 128-row tiles on grids of at least two
+    workgroups per CU and 64-row tiles below that; each element's MFMA
+    sequence is the same, so a product computed on 128-row tiles and a
+    sub-block of it computed alone (64-row tiles) agree bit for bit."""
+    from mesh_bem_amd import _lib
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    M = 128 * 16
+    N = 64 * ((2 * cus + 15) // 16 + 1)  # (M / 128) x (N / 64) >= 2 x CUs tiles
+    K = 96
+    assert (M // 128) * (N // 64) >= 2 * cus
+    g = torch.Generator().manual_seed(21)
+    A = torch.randn((M, K), dtype=torch.complex64, generator=g).to(dev)
+    B = torch.randn((K, N), dtype=torch.complex64, generator=g).to(dev)
+    lib = _lib.load()
+    C = torch.zeros((M, N), dtype=torch.complex64, device=dev)
+    _lib.check(lib.mba_gemm_variant(_lib.MBA_F32, 4, M, N, K, 1.0, _lib.ptr(A), K, _lib.ptr(B), N, 0.0, _lib.ptr(C),
+                                    N, _lib.stream()))
+    m, n = 200, 300  # (2 x 5) 128-row tiles: the 64-row form
+    Cs = torch.zeros((m, n), dtype=torch.complex64, device=dev)
+    _lib.check(lib.mba_gemm_variant(_lib.MBA_F32, 4, m, n, K, 1.0, _lib.ptr(A), K, _lib.ptr(B), N, 0.0,
+                                    _lib.ptr(Cs), n, _lib.stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(C[:m, :n], Cs)
+    ref = A.cpu().to(torch.complex128) @ B.cpu().to(torch.complex128)
+    assert float((C.cpu().to(torch.complex128) - ref).abs().max() / ref.abs().max()) < 1e-5
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 130, 700])
 @pytest.mark.parametrize("nrhs", [1, 2, 3, 4])
 def test_lu_solve_persistent_matches_per_block(dev, n, nrhs):
