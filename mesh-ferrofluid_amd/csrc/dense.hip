@@ -1959,7 +1959,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void k_trsm_l(cpx<T>* 
 // workgroup), all as MFMA A operands: planes [kk][i] (real, then imaginary)
 // of 64 x 64 floats, kk the reduction index.  k_trsm_mfma then gives each
 // workgroup 64 columns of X in LDS and runs the three products on
-// v_mfma_f32_32x32x2f32 (four real products per complex one).
+// v_mfma_f32_32x32x2f32 (three real products per complex one, 3M).
 constexpr int TM_PLANE = 64 * 64;          // floats per plane
 constexpr int TM_OPER = 2 * TM_PLANE;      // floats per operand (re, im)
 constexpr int TM_SET = 3 * TM_OPER;        // D0^{-1}, D1^{-1}, L21 of one panel
@@ -2129,18 +2129,30 @@ __device__ __forceinline__ void tm_product(floatx16& cr, floatx16& ci, const flo
                                            const float* Xr, const float* Xi, int x0, int bj, int steps)
 {
     const int lane = threadIdx.x & 63, hi = lane >> 5, jj = lane & 31;
+    // 3M: t1 = Cr + sum ar br, t2 = sum ai bi, t3 = Cr + Ci + sum (ar + ai)(br + bi);
+    // Re = t1 - t2, Im = t3 - t1 - t2 (three chains that alternate, so no MFMA
+    // waits on the one before it)
+    floatx16 t1 = cr, t2, t3;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        t2[r] = 0.f;
+        t3[r] = cr[r] + ci[r];
+    }
 #pragma unroll
     for (int s = 0; s < 32; ++s) {
         if (s < steps) {
             const int kr = x0 + 2 * s + hi;
             const float br = Xr[kr * 64 + bj * 32 + jj], bim = Xi[kr * 64 + bj * 32 + jj];
             const float a_r = NEG ? -ar[s] : ar[s], a_i = NEG ? -ai[s] : ai[s];
-            // the two accumulators alternate, so no MFMA waits on the one before it
-            cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, cr, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, bim, ci, 0, 0, 0);
-            cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a_i, bim, cr, 0, 0, 0);
-            ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, br, ci, 0, 0, 0);
+            t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r, br, t1, 0, 0, 0);
+            t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_i, bim, t2, 0, 0, 0);
+            t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a_r + a_i, br + bim, t3, 0, 0, 0);
         }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        cr[r] = t1[r] - t2[r];
+        ci[r] = t3[r] - t1[r] - t2[r];
     }
 }
 

@@ -262,7 +262,7 @@ def test_lu_factors_reproduce_pa(dev, n, update):
     pa = _interchange(a64, f.ipiv.cpu().numpy().astype(np.int64))
     err = np.linalg.norm(pa - L @ U) / np.linalg.norm(a64)
     print(f"||PA - LU|| / ||A|| = {err:.3e}", flush=True)
-    # measured 1.5e-5 / 3.1e-5 / 7.0e-5 (the per-column substitution it
+    # measured 1.5e-5 / 3.1e-5 / 6.8e-5 (the per-column substitution it
     # replaced: 1.4e-5 / 3.1e-5 / 6.9e-5; LAPACK cgetrf at n = 2100: 1.5e-5)
     assert err < 40 * 6e-8 * np.sqrt(n)
 
